@@ -1,0 +1,175 @@
+"""ctypes mirrors of include/blackframe_hip_abi.h (record layouts shared by every binding)."""
+
+import ctypes as C
+
+LANES = 4
+MAX_CLOSURES = 8
+INVALID = 0xFFFFFFFF
+
+# closure kinds / lobes (TransportLobesDevice.cuh:25-44)
+LOBE_DIFFUSE, LOBE_GLOSSY, LOBE_SPECULAR, LOBE_REFLECTION, LOBE_TRANSMISSION = 0x1, 0x2, 0x4, 0x8, 0x10
+CLOSURE_LAMBERTIAN, CLOSURE_ROUGH_DIFFUSE, CLOSURE_ROUGH_CONDUCTOR, CLOSURE_ROUGH_DIELECTRIC = 1, 2, 3, 4
+CLOSURE_SPECULAR_REFLECTION, CLOSURE_SPECULAR_TRANSMISSION = 5, 6
+
+SAMPLER_INDEXED_HASH, SAMPLER_SOBOL_OWEN = 0, 1
+WAVELENGTHS_FRAME_PACKET, WAVELENGTHS_PER_PATH = 0, 1
+MIS_BALANCE, MIS_POWER = 0, 1
+JITTER_CENTER, JITTER_UNIFORM = 0, 1
+
+TERM_NONE, TERM_ESCAPED, TERM_DEPTH_LIMIT, TERM_ZERO_THROUGHPUT, TERM_OUTSIDE_SUPPORT, TERM_RUSSIAN_ROULETTE = range(6)
+TERM_FAILED = 15
+
+
+class SampleStreamIndex(C.Structure):
+    _fields_ = [("pixel_x", C.c_uint32), ("pixel_y", C.c_uint32), ("sample_index", C.c_uint64), ("seed", C.c_uint64)]
+
+
+class TransportRay(C.Structure):
+    _fields_ = [
+        ("origin_x", C.c_float), ("origin_y", C.c_float), ("origin_z", C.c_float), ("t_min", C.c_float),
+        ("direction_x", C.c_float), ("direction_y", C.c_float), ("direction_z", C.c_float), ("t_max", C.c_float),
+        ("time", C.c_float), ("visibility_mask", C.c_uint32), ("current_medium", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class ClosestHit(C.Structure):
+    _fields_ = [
+        ("parameter", C.c_float), ("barycentric0", C.c_float), ("barycentric1", C.c_float), ("barycentric2", C.c_float),
+        ("normal_x", C.c_float), ("normal_y", C.c_float), ("normal_z", C.c_float), ("reserved", C.c_uint32),
+        ("object", C.c_uint32), ("instance", C.c_uint32), ("geometry", C.c_uint32), ("primitive", C.c_uint32),
+        ("material", C.c_uint32), ("ids_reserved", C.c_uint32 * 3),
+    ]
+
+
+class ClosestHitResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("reserved", C.c_uint32 * 3), ("hit", ClosestHit)]
+
+
+class OcclusionResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
+
+class BvhNode(C.Structure):
+    _fields_ = [
+        ("min_x", C.c_float), ("min_y", C.c_float), ("min_z", C.c_float), ("kind", C.c_uint32),
+        ("max_x", C.c_float), ("max_y", C.c_float), ("max_z", C.c_float), ("split_axis", C.c_uint32),
+        ("first_child", C.c_uint32), ("second_child", C.c_uint32),
+        ("reference_offset", C.c_uint32), ("reference_count", C.c_uint32),
+    ]
+
+
+class BvhBlas(C.Structure):
+    _fields_ = [
+        ("geometry_id", C.c_uint32), ("root_node", C.c_uint32), ("node_offset", C.c_uint32), ("node_count", C.c_uint32),
+        ("primitive_reference_offset", C.c_uint32), ("primitive_reference_count", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
+    ]
+
+
+class BvhInstanceRef(C.Structure):
+    _fields_ = [
+        ("scene_instance_index", C.c_uint32), ("instance_id", C.c_uint32), ("scene_geometry_index", C.c_uint32),
+        ("geometry_id", C.c_uint32), ("blas_index", C.c_uint32), ("visibility_mask", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
+    ]
+
+
+class SpectrumDesc(C.Structure):
+    _fields_ = [("offset", C.c_uint32), ("count", C.c_uint32)]
+
+
+class ClosureDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32), ("lobes", C.c_uint32), ("weight_spectrum", C.c_uint32), ("eta_spectrum", C.c_uint32),
+        ("k_spectrum", C.c_uint32), ("reserved", C.c_uint32 * 3), ("parameters", C.c_float * 10),
+        ("reserved_tail", C.c_uint32 * 2),
+    ]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32), ("closure_offset", C.c_uint32), ("closure_count", C.c_uint32),
+        ("emission_spectrum", C.c_uint32), ("frame_mode", C.c_uint32), ("tangent_rotation", C.c_float),
+        ("reserved", C.c_uint32 * 2), ("probabilities", C.c_float * 8), ("cdf", C.c_float * 9),
+        ("reserved_tail", C.c_uint32 * 3),
+    ]
+
+
+class GeometryDesc(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32), ("reserved", C.c_uint32), ("vertex_offset", C.c_uint64), ("vertex_count", C.c_uint64),
+        ("triangle_offset", C.c_uint64), ("triangle_count", C.c_uint64),
+    ]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint32), ("object_id", C.c_uint32), ("geometry_index", C.c_uint32), ("material_index", C.c_uint32),
+        ("visibility_mask", C.c_uint32), ("reserved", C.c_uint32 * 3),
+        ("local_to_world", C.c_float * 12), ("world_to_local", C.c_float * 12),
+    ]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("geometry_count", C.c_uint32), ("instance_count", C.c_uint32), ("material_count", C.c_uint32),
+        ("closure_count", C.c_uint32), ("spectrum_count", C.c_uint32), ("has_environment", C.c_uint32),
+        ("vertex_count", C.c_uint64), ("triangle_count", C.c_uint64), ("spectrum_value_count", C.c_uint64),
+        ("positions", C.c_void_p), ("normals", C.c_void_p), ("uvs", C.c_void_p), ("indices", C.c_void_p),
+        ("geometries", C.c_void_p), ("instances", C.c_void_p), ("materials", C.c_void_p), ("closures", C.c_void_p),
+        ("spectra", C.c_void_p), ("spectrum_values", C.c_void_p),
+        ("environment_spectrum", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [
+        ("origin", C.c_float * 3), ("tangent", C.c_float * 3), ("bitangent", C.c_float * 3), ("normal", C.c_float * 3),
+        ("half_width", C.c_float), ("half_height", C.c_float), ("t_min", C.c_float), ("t_max", C.c_float),
+        ("time", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32), ("visibility_mask", C.c_uint32),
+        ("jitter_mode", C.c_uint32), ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class TransportOptions(C.Structure):
+    _fields_ = [
+        ("sampler", C.c_uint32), ("wavelength_mode", C.c_uint32), ("mis_heuristic", C.c_uint32),
+        ("diffuse_depth_limit", C.c_uint32), ("glossy_depth_limit", C.c_uint32), ("specular_depth_limit", C.c_uint32),
+        ("transmission_depth_limit", C.c_uint32), ("volume_depth_limit", C.c_uint32),
+        ("russian_roulette_enabled", C.c_uint32), ("russian_roulette_first_depth", C.c_uint32),
+        ("russian_roulette_minimum", C.c_float), ("russian_roulette_maximum", C.c_float),
+        ("packet_wavelengths", C.c_float * 4), ("record_queues", C.c_uint32), ("reserved", C.c_uint32 * 3),
+    ]
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [("camera", CameraDesc), ("seed", C.c_uint64), ("samples_per_pixel", C.c_uint32),
+                ("first_sample_index", C.c_uint32)]
+
+
+MAX_BOUNCES = 64
+
+
+class TransportReport(C.Structure):
+    _fields_ = [
+        ("path_count", C.c_uint64), ("closest_hit_queries", C.c_uint64), ("shadow_queries", C.c_uint64),
+        ("shade_lanes", C.c_uint64), ("miss_lanes", C.c_uint64), ("closure_samples", C.c_uint64),
+        ("light_samples", C.c_uint64), ("failed_paths", C.c_uint64), ("bounces", C.c_uint32),
+        ("first_failure_slot", C.c_uint32), ("first_failure_code", C.c_uint32), ("reserved", C.c_uint32),
+        ("ray_queue_hash", C.c_uint64 * MAX_BOUNCES), ("ray_queue_size", C.c_uint32 * MAX_BOUNCES),
+        ("shadow_queue_hash", C.c_uint64 * MAX_BOUNCES), ("shadow_queue_size", C.c_uint32 * MAX_BOUNCES),
+    ]
+
+
+class PathResult(C.Structure):
+    _fields_ = [
+        ("radiance", C.c_float * 4), ("beta", C.c_float * 4), ("wavelengths", C.c_float * 4),
+        ("depth", C.c_uint32), ("termination", C.c_uint32), ("depth_counters", C.c_uint32), ("eta_scale", C.c_float),
+    ]
+
+
+for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
+                       (BvhNode, 48), (BvhBlas, 32), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
+                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 136), (CameraDesc, 96),
+                       (TransportOptions, 80), (FrameDesc, 112), (TransportReport, 1616)):
+    assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))

@@ -1,0 +1,430 @@
+/*
+ * Closure library on the device: records resolved at the path's wavelengths, depth filtering,
+ * mixture sampling / evaluation / density. Arithmetic follows
+ * Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/TransportLobesDevice.cuh and TransportDevice.cuh.
+ */
+#ifndef BF_DEVICE_LOBES_H
+#define BF_DEVICE_LOBES_H
+
+#include "bf_device_math.h"
+
+struct closure_t {
+    uint32_t kind, lobes;
+    float weight[4];
+    float parameters[10];
+};
+
+struct mixture_t {
+    uint32_t active_count;
+    closure_t closures[BF_MAX_CLOSURES];
+    float probabilities[BF_MAX_CLOSURES];
+    float cdf[BF_MAX_CLOSURES + 1];
+};
+
+struct filter_t {
+    uint32_t source_indices[BF_MAX_CLOSURES];
+    uint32_t allowed[BF_MAX_CLOSURES];
+    uint32_t source_count;
+    uint32_t blocked;
+    int status;
+};
+
+struct lobe_sample_t {
+    float value[4];
+    v3f incoming;
+    float pdf;
+    uint32_t discrete;
+    uint32_t lobes;
+    float eta_scale;
+    int status;
+};
+
+/* ---- Lambert (TransportDevice.cuh:335-478) ------------------------------------------------- */
+BF_D bool reflectance_ok(const float* w) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (!bf_isfinite(w[k]) || w[k] < 0.0f || w[k] > 1.0f) return false;
+    return true;
+}
+
+BF_D int lambert_eval(const float* r, v3f wo, v3f wi, float* out) {
+    if (!reflectance_ok(r) || !unit_vector(wo) || !unit_vector(wi)) return BF_L_INVALID;
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return BF_L_OK;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = r[k] * BF_INV_PI;
+    return BF_L_OK;
+}
+
+BF_D int lambert_pdf(v3f wo, v3f wi, float& pdf) {
+    if (!unit_vector(wo) || !unit_vector(wi)) return BF_L_INVALID;
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+        pdf = 0.0f;
+        return BF_L_OK;
+    }
+    float d;
+    if (!checked_product(wi.z, BF_INV_PI, d) || !(d > 0.0f)) return BF_L_NOT_REPRESENTABLE;
+    pdf = d;
+    return BF_L_OK;
+}
+
+/* TransportDevice.cuh:388-427 */
+BF_D int cosine_hemisphere(float u, float v, v3f& out) {
+    if (!bf_isfinite(u) || u < 0.0f || !(u < 1.0f) || !bf_isfinite(v) || v < 0.0f || !(v < 1.0f)) return BF_L_INVALID;
+    const float ox = 2.0f * u - 1.0f;
+    const float oy = 2.0f * v - 1.0f;
+    if (ox == 0.0f && oy == 0.0f) {
+        out = mk(0.0f, 0.0f, 1.0f);
+        return BF_L_OK;
+    }
+    float r, phi;
+    if (fabsf(ox) > fabsf(oy)) {
+        r = ox;
+        phi = BF_QUARTER_PI * (oy / ox);
+    } else {
+        r = oy;
+        phi = 2.0f * BF_QUARTER_PI - BF_QUARTER_PI * (ox / oy);
+    }
+    const float radius = fabsf(r);
+    float s, c;
+    bf_sincosf(phi, &s, &c);
+    out = mk(r * c, r * s, sqrtf((1.0f - radius) * (1.0f + radius)));
+    return unit_vector(out) ? BF_L_OK : BF_L_NOT_REPRESENTABLE;
+}
+
+/* ---- record dispatch (TransportLobesDevice.cuh:2102-2260) ---------------------------------- */
+BF_D int record_eval(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float* out) {
+    (void)allowed;
+    if (c.kind == BF_CLOSURE_LAMBERTIAN) return lambert_eval(c.weight, wo, wi, out);
+    return BF_L_INVALID;
+}
+
+BF_D int record_pdf(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float& pdf) {
+    (void)allowed;
+    if (c.kind == BF_CLOSURE_LAMBERTIAN) return lambert_pdf(wo, wi, pdf);
+    return BF_L_INVALID;
+}
+
+BF_D void empty_sample(lobe_sample_t& s, int status) {
+    s.value[0] = s.value[1] = s.value[2] = s.value[3] = 0.0f;
+    s.incoming = mk(0.0f, 0.0f, 0.0f);
+    s.pdf = 0.0f;
+    s.discrete = 0u;
+    s.lobes = 0u;
+    s.eta_scale = 1.0f;
+    s.status = status;
+}
+
+BF_D void record_sample(const closure_t& c, uint32_t allowed, v3f wo, float event, float u, float v, lobe_sample_t& out) {
+    (void)allowed;
+    (void)event;
+    empty_sample(out, BF_L_INVALID);
+    if (c.kind == BF_CLOSURE_LAMBERTIAN) {
+        if (!reflectance_ok(c.weight) || !unit_vector(wo)) return;
+        v3f wi;
+        const int st = cosine_hemisphere(u, v, wi);
+        if (st != BF_L_OK) {
+            out.status = st;
+            return;
+        }
+        if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+            out.incoming = wi;
+            out.status = BF_L_OUTSIDE;
+            return;
+        }
+        float value[4], pdf;
+        const int s1 = lambert_eval(c.weight, wo, wi, value);
+        const int s2 = lambert_pdf(wo, wi, pdf);
+        if (s1 != BF_L_OK || s2 != BF_L_OK) {
+            out.status = s1 != BF_L_OK ? s1 : s2;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out.value[k] = value[k];
+        out.incoming = wi;
+        out.pdf = pdf;
+        out.lobes = c.lobes;
+        out.eta_scale = 1.0f;
+        out.status = BF_L_OK;
+    }
+}
+
+/* ---- mixtures ------------------------------------------------------------------------------- */
+BF_D bool compensated_add(float value, float& sum, float& correction) {
+    if (!bf_isfinite(value) || value < 0.0f) return false;
+    const float next = sum + value;
+    if (!bf_isfinite(next)) return false;
+    correction += sum >= value ? (sum - next) + value : (value - next) + sum;
+    if (!bf_isfinite(correction)) return false;
+    sum = next;
+    return true;
+}
+
+BF_D bool mixture_nonblack(const mixture_t& m) {
+    for (uint32_t c = 0; c < m.active_count; ++c)
+        for (int k = 0; k < 4; ++k)
+            if (m.closures[c].weight[k] != 0.0f) return true;
+    return false;
+}
+
+/* TransportLobesDevice.cuh:2863-2887 */
+BF_D int mixture_eval(const mixture_t& m, const filter_t& f, v3f wo, v3f wi, float* out) {
+    if (!unit_vector(wo) || !unit_vector(wi)) return BF_L_INVALID;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t i = 0; i < m.active_count; ++i) {
+        float comp[4];
+        const int st = record_eval(m.closures[i], f.allowed[i], wo, wi, comp);
+        if (st != BF_L_OK) return st;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!bf_isfinite(comp[k]) || comp[k] < 0.0f) return BF_L_INVALID;
+            acc[k] += comp[k];
+            if (!bf_isfinite(acc[k]) || acc[k] < 0.0f) return BF_L_NOT_REPRESENTABLE;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = acc[k];
+    return BF_L_OK;
+}
+
+/* TransportLobesDevice.cuh:2889-2958 */
+BF_D int mixture_pdf(const mixture_t& m, const filter_t& f, v3f wo, v3f wi, float& out) {
+    if (!unit_vector(wo) || !unit_vector(wi)) return BF_L_INVALID;
+    if (m.active_count == 0) {
+        out = 0.0f;
+        return BF_L_OK;
+    }
+    float conditional[BF_MAX_CLOSURES];
+    float maximum = 0.0f;
+    for (uint32_t i = 0; i < m.active_count; ++i) {
+        float p;
+        const int st = record_pdf(m.closures[i], f.allowed[i], wo, wi, p);
+        if (st != BF_L_OK) return st;
+        if (!bf_isfinite(p) || p < 0.0f) return BF_L_NOT_REPRESENTABLE;
+        conditional[i] = p;
+        maximum = fmaxf(maximum, p);
+    }
+    if (m.active_count == 1) {
+        out = conditional[0];
+        return BF_L_OK;
+    }
+    if (maximum == 0.0f) {
+        out = 0.0f;
+        return BF_L_OK;
+    }
+    float sum = 0.0f, correction = 0.0f;
+    for (uint32_t i = 0; i < m.active_count; ++i) {
+        const float ratio = conditional[i] / maximum;
+        float contribution;
+        if (!checked_product(m.probabilities[i], ratio, contribution) || !compensated_add(contribution, sum, correction))
+            return BF_L_NOT_REPRESENTABLE;
+    }
+    const float normalized = sum + correction;
+    if (!bf_isfinite(normalized) || !(normalized > 0.0f) || normalized > 1.0f) return BF_L_NOT_REPRESENTABLE;
+    float mixed;
+    if (!checked_product(normalized, maximum, mixed)) return BF_L_NOT_REPRESENTABLE;
+    out = mixed;
+    return BF_L_OK;
+}
+
+/* TransportLobesDevice.cuh:2618-2641 */
+BF_D uint32_t depth_blocked(const uint32_t* limits, const uint32_t* counters, uint32_t event) {
+    uint32_t b = 0;
+    if ((event & BF_LOBE_DIFFUSE) && counters[0] >= limits[0]) b |= BF_LOBE_DIFFUSE;
+    if ((event & BF_LOBE_GLOSSY) && counters[1] >= limits[1]) b |= BF_LOBE_GLOSSY;
+    if ((event & BF_LOBE_SPECULAR) && counters[2] >= limits[2]) b |= BF_LOBE_SPECULAR;
+    if ((event & BF_LOBE_TRANSMISSION) && counters[3] >= limits[3]) b |= BF_LOBE_TRANSMISSION;
+    return b;
+}
+
+/* TransportLobesDevice.cuh:2643-2718 */
+BF_D bool build_filtered_distribution(mixture_t& m) {
+    if (m.active_count == 0) return true;
+    if (m.active_count == 1) {
+        m.probabilities[0] = 1.0f;
+        m.cdf[0] = 0.0f;
+        m.cdf[1] = 1.0f;
+        return true;
+    }
+    float ts = 0.0f, tc = 0.0f;
+    for (uint32_t i = 0; i < m.active_count; ++i)
+        if (!compensated_add(m.probabilities[i], ts, tc)) return false;
+    const float total = ts + tc;
+    if (!bf_isfinite(total) || !(total > 0.0f)) return false;
+    const uint64_t scale = 1ull << 24;
+    uint64_t masses[BF_MAX_CLOSURES];
+    float rem[BF_MAX_CLOSURES];
+    uint64_t base_mass = 0;
+    for (uint32_t i = 0; i < m.active_count; ++i) {
+        const float normalized = m.probabilities[i] / total;
+        if (!bf_isfinite(normalized) || !(normalized > 0.0f) || !(normalized < 1.0f)) return false;
+        const float scaled = bf_ldexpf(normalized, 24);
+        const float base = floorf(scaled);
+        if (!bf_isfinite(scaled) || !(scaled > 0.0f) || base < 0.0f || base > (float)scale) return false;
+        masses[i] = (uint64_t)base;
+        rem[i] = scaled - base;
+        if (!bf_isfinite(rem[i]) || rem[i] < 0.0f || !(rem[i] < 1.0f) || base_mass > scale - masses[i]) return false;
+        base_mass += masses[i];
+    }
+    if (base_mass > scale) return false;
+    float rs = 0.0f, rc = 0.0f;
+    for (uint32_t i = 0; i < m.active_count; ++i)
+        if (!compensated_add(rem[i], rs, rc)) return false;
+    const uint64_t missing = scale - base_mass;
+    const float rtotal = rs + rc;
+    if (missing > m.active_count || fabsf(rtotal - (float)missing) > 0.5f + 0x1p-23f * 16.0f) return false;
+    bool received[BF_MAX_CLOSURES];
+    for (uint32_t i = 0; i < BF_MAX_CLOSURES; ++i) received[i] = false;
+    for (uint64_t unit = 0; unit < missing; ++unit) {
+        uint32_t sel = m.active_count;
+        for (uint32_t i = 0; i < m.active_count; ++i) {
+            if (received[i] || !(rem[i] > 0.0f)) continue;
+            if (sel == m.active_count || rem[i] > rem[sel]) sel = i;
+        }
+        if (sel == m.active_count) return false;
+        ++masses[sel];
+        received[sel] = true;
+    }
+    uint64_t cumulative = 0;
+    m.cdf[0] = 0.0f;
+    for (uint32_t i = 0; i < m.active_count; ++i) {
+        if (masses[i] == 0 || cumulative > scale - masses[i]) return false;
+        cumulative += masses[i];
+        m.probabilities[i] = bf_ldexpf((float)masses[i], -24);
+        m.cdf[i + 1] = bf_ldexpf((float)cumulative, -24);
+    }
+    return cumulative == scale && m.cdf[m.active_count] == 1.0f;
+}
+
+/* TransportLobesDevice.cuh:2720-2795 */
+BF_D void filter_by_depth(mixture_t& m, const uint32_t* limits, const uint32_t* counters, uint32_t expected_total,
+                          filter_t& r) {
+    r.source_count = m.active_count;
+    r.blocked = 0;
+    r.status = BF_L_OK;
+    const uint64_t total = (uint64_t)counters[0] + counters[1] + counters[2] + counters[4];
+    const uint64_t surface = (uint64_t)counters[0] + counters[1] + counters[2];
+    if (total != expected_total || counters[3] > surface || counters[0] > limits[0] || counters[1] > limits[1] ||
+        counters[2] > limits[2] || counters[3] > limits[3] || counters[4] > limits[4]) {
+        r.status = BF_L_INVALID;
+        return;
+    }
+    uint32_t filtered = 0;
+    for (uint32_t si = 0; si < r.source_count; ++si) {
+        const closure_t src = m.closures[si];
+        const float prob = m.probabilities[si];
+        uint32_t allowed = 0;
+        if (src.kind == BF_CLOSURE_ROUGH_DIELECTRIC) {
+            const uint32_t refl = depth_blocked(limits, counters, BF_LOBE_GLOSSY | BF_LOBE_REFLECTION);
+            const uint32_t trans = depth_blocked(limits, counters, BF_LOBE_GLOSSY | BF_LOBE_TRANSMISSION);
+            if (refl == 0) allowed |= BF_LOBE_REFLECTION;
+            else r.blocked |= refl;
+            if (trans == 0) allowed |= BF_LOBE_TRANSMISSION;
+            else r.blocked |= trans;
+        } else {
+            if (!valid_surface_event(src.lobes)) {
+                r.status = BF_L_INVALID;
+                return;
+            }
+            const uint32_t blocked = depth_blocked(limits, counters, src.lobes);
+            if (blocked == 0) allowed = src.lobes & (BF_LOBE_REFLECTION | BF_LOBE_TRANSMISSION);
+            else r.blocked |= blocked;
+        }
+        if (allowed == 0) continue;
+        m.closures[filtered] = src;
+        m.probabilities[filtered] = prob;
+        r.source_indices[filtered] = si;
+        r.allowed[filtered] = allowed;
+        ++filtered;
+    }
+    m.active_count = filtered;
+    if (filtered == r.source_count) return;
+    for (uint32_t i = filtered; i < BF_MAX_CLOSURES; ++i) m.probabilities[i] = 0.0f;
+    for (uint32_t i = 0; i <= BF_MAX_CLOSURES; ++i) m.cdf[i] = 0.0f;
+    if (!build_filtered_distribution(m)) r.status = BF_L_NOT_REPRESENTABLE;
+}
+
+struct mixture_sample_t {
+    float value[4];
+    v3f incoming;
+    float pdf;
+    uint32_t discrete;
+    uint32_t selected;
+    uint32_t lobes;
+    float eta_scale;
+    int status;
+};
+
+/* TransportLobesDevice.cuh:2960-3062 */
+BF_D void sample_mixture(const mixture_t& m, const filter_t& f, v3f wo, float component, float u, float v,
+                         mixture_sample_t& out) {
+    out.eta_scale = 1.0f;
+    out.discrete = 0u;
+    out.selected = 0u;
+    out.lobes = 0u;
+    out.pdf = 0.0f;
+    if (!unit_vector(wo) || !bf_isfinite(component) || component < 0.0f || !(component < 1.0f) || !bf_isfinite(u) ||
+        u < 0.0f || !(u < 1.0f) || !bf_isfinite(v) || v < 0.0f || !(v < 1.0f)) {
+        out.status = BF_L_INVALID;
+        return;
+    }
+    if (m.active_count == 0) {
+        out.status = BF_L_OUTSIDE;
+        return;
+    }
+    uint32_t sel = m.active_count;
+    for (uint32_t i = 0; i < m.active_count; ++i) {
+        if (component < m.cdf[i + 1]) {
+            sel = i;
+            break;
+        }
+    }
+    if (sel == m.active_count) {
+        out.status = BF_L_NOT_REPRESENTABLE;
+        return;
+    }
+    const float event = (component - m.cdf[sel]) / m.probabilities[sel];
+    if (!bf_isfinite(event) || event < 0.0f || !(event < 1.0f)) {
+        out.status = BF_L_NOT_REPRESENTABLE;
+        return;
+    }
+    lobe_sample_t s;
+    record_sample(m.closures[sel], f.allowed[sel], wo, event, u, v, s);
+    if (s.status != BF_L_OK) {
+        out.status = s.status;
+        return;
+    }
+    out.incoming = s.incoming;
+    out.lobes = s.lobes;
+    out.eta_scale = s.eta_scale;
+    if (m.active_count == 1) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out.value[k] = s.value[k];
+        out.pdf = s.pdf;
+        out.discrete = s.discrete;
+        out.selected = f.source_indices[0];
+        out.status = BF_L_OK;
+        return;
+    }
+    if (!s.discrete) {
+        float value[4], pdf;
+        const int st1 = mixture_eval(m, f, wo, s.incoming, value);
+        const int st2 = mixture_pdf(m, f, wo, s.incoming, pdf);
+        if (st1 != BF_L_OK || st2 != BF_L_OK) {
+            out.status = st1 != BF_L_OK ? st1 : st2;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out.value[k] = value[k];
+        out.pdf = pdf;
+    } else {
+        out.status = BF_L_NOT_REPRESENTABLE; /* delta atoms: not produced by the supported closures */
+        return;
+    }
+    out.selected = f.source_indices[sel];
+    out.status = BF_L_OK;
+}
+
+#endif

@@ -1,0 +1,118 @@
+/*
+ * Device-resident scene and path-state layout of the HIP wavefront backend.
+ *
+ * Everything the bounce kernels touch is a flat array in HBM, laid out for 64-lane coalescing:
+ *  - wide BVH nodes: one 64-byte line per internal node holding both child boxes (bf_wide_node);
+ *  - leaf triangles: 48-byte records in BLAS leaf order (3 x float4: vertex xyz + payload word),
+ *    so a leaf costs one 48-byte fetch and never a node fetch;
+ *  - TLAS leaves: one 64-byte record per instance reference (world->local 3x4 + ids + mask);
+ *  - path state: structure-of-arrays columns indexed by path slot, float4-wide where possible.
+ */
+#ifndef BF_DEVICE_SCENE_H
+#define BF_DEVICE_SCENE_H
+
+#include <stdint.h>
+
+#include "../../include/blackframe_hip_abi.h"
+
+#define BF_STACK_DEPTH 32u
+
+typedef struct bf_dev_blas {
+    uint32_t root_ref;         /* wide-node index, or BF_WIDE_LEAF | leaf-triangle index */
+    uint32_t geometry_index;
+    uint32_t canonical_node_offset;
+    uint32_t depth;
+    float root_box[6];         /* min xyz, max xyz of the canonical root node */
+    uint32_t reserved[2];
+} bf_dev_blas;
+
+typedef struct bf_dev_tlas_leaf {
+    float world_to_local[12];
+    uint32_t scene_instance;
+    uint32_t instance_id;
+    uint32_t blas_index;
+    uint32_t visibility_mask;
+} bf_dev_tlas_leaf;
+
+typedef struct bf_dev_instance {
+    float local_to_world[12];
+    float world_to_local[12];
+    uint32_t geometry_index;
+    uint32_t material_index;
+    uint32_t id;
+    uint32_t object_id;
+    uint32_t visibility_mask;
+    int32_t light_index;       /* mesh-area light registry slot or -1 */
+    uint32_t reserved[2];
+} bf_dev_instance;
+
+typedef struct bf_dev_geometry {
+    uint32_t id;
+    uint32_t vertex_offset;
+    uint32_t vertex_count;
+    uint32_t triangle_offset;
+    uint32_t triangle_count;
+    uint32_t reserved[3];
+} bf_dev_geometry;
+
+typedef struct bf_dev_light {
+    uint32_t instance;
+    uint32_t geometry;
+    uint32_t triangle_count;
+    uint32_t table_offset;     /* into area / probability / cumulative / next tables */
+    float inverse_area;
+    uint32_t reserved[3];
+} bf_dev_light;
+
+typedef struct bf_dev_scene {
+    /* traversal */
+    const bf_wide_node* tlas_nodes;
+    const bf_wide_node* blas_nodes;
+    const bf_dev_blas* blases;
+    const float4* leaf_triangles;      /* 3 float4 per BLAS leaf; [0].w = primitive (bits) */
+    const bf_dev_tlas_leaf* tlas_leaves;
+    uint32_t tlas_root_ref;            /* BF_INVALID_INDEX when the scene has no instances */
+    uint32_t instance_count;
+    float tlas_root_box[6];
+    /* shading */
+    const bf_dev_instance* instances;
+    const bf_dev_geometry* geometries;
+    const float* positions;
+    const float* normals;
+    const float* uvs;
+    const uint32_t* indices;
+    const bf_material_desc* materials;
+    const bf_closure_desc* closures;
+    const bf_spectrum_desc* spectra;
+    const float* spectrum_values;
+    const bf_dev_light* lights;
+    const float* light_areas;
+    const float* light_probabilities;
+    const float* light_cumulative;
+    const float* light_next;
+    const float4* light_normals;       /* exact world normal of every emitter triangle */
+    uint32_t light_count;
+    uint32_t has_environment;
+    uint32_t environment_spectrum;
+    uint32_t material_count;
+} bf_dev_scene;
+
+/* Path-state columns (one entry per path slot). */
+typedef struct bf_dev_paths {
+    float4* ray_origin;      /* xyz origin, w = ray time */
+    float4* ray_direction;   /* xyz direction, w = t_min */
+    float4* beta;
+    float4* radiance;
+    float4* wavelengths;
+    uint4* depth;            /* x: packed counters, y: depth, z: delta flags, w: eta scale bits */
+    float4* previous;        /* xyz context, w = pdf (0 = no previous continuous sample) */
+    float4* hit;             /* t, b0, b1, b2 */
+    uint2* hit_ids;          /* scene instance, primitive */
+    float4* shadow_origin;   /* xyz, w = t_max */
+    float4* shadow_direction;/* xyz, w = bits(continuation | termination << 8) */
+    float4* shadow_contribution;
+    uint32_t* status;        /* termination | failure code << 8 */
+    uint32_t* visibility;    /* ray visibility mask */
+} bf_dev_paths;
+
+#endif

@@ -1,0 +1,1054 @@
+/*
+ * Host side of the MI355X wavefront backend: scene ingestion, the deterministic BVH builder and
+ * its 64-byte relayout, mesh-area-light tables, the transport workspace, the bounce loop and the
+ * C ABI of include/blackframe_hip.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/blackframe_hip.h"
+#include "bf_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int status, const char* fmt, ...) {
+    char buffer[512];
+    va_list args;
+    va_start(args, fmt);
+    std::vsnprintf(buffer, sizeof buffer, fmt, args);
+    va_end(args);
+    g_last_error = buffer;
+    return status;
+}
+
+int hip_error(hipError_t e, const char* what) {
+    return set_error(BF_DEVICE_ERROR, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(call)                                        \
+    do {                                                     \
+        const hipError_t _e = (call);                        \
+        if (_e != hipSuccess) return hip_error(_e, #call);   \
+    } while (0)
+
+float canonical_zero(float v) { return v == 0.0f ? 0.0f : v; }
+float next_up(float v) { return std::nextafter(v, INFINITY); }
+float next_down(float v) { return std::nextafter(v, -INFINITY); }
+
+/* ---- Sobol direction numbers (Blackframe/Renderer/Source/SobolSampling.cpp:53-88) ----------- */
+struct JoeKuoRow {
+    uint32_t degree;
+    uint32_t a;
+    uint32_t m[18];
+};
+const JoeKuoRow kJoeKuo[] = {
+#include "data/sobol_joe_kuo.inc"
+};
+constexpr uint32_t kSobolDimensions = sizeof(kJoeKuo) / sizeof(kJoeKuo[0]) + 1u;
+
+/* CIE 1931 table: the film normalisation (1/4) / integral(ybar) of Cie1931Sensor.cpp:26-40,131 */
+const double kCieTable[471][3] = {
+#include "data/cie1931_cmf.inc"
+};
+float film_normalization() {
+    double integral = 0.0;
+    for (int i = 0; i + 1 < 471; ++i) integral += (kCieTable[i][1] + kCieTable[i + 1][1]) / 2.0;
+    return (1.0f / 4.0f) / (float)integral;
+}
+
+std::vector<uint64_t> sobol_directions() {
+    std::vector<uint64_t> d((size_t)kSobolDimensions * 64u);
+    for (uint32_t c = 0; c < 64u; ++c) d[c] = 1ull << (63u - c);
+    for (uint32_t dim = 1; dim < kSobolDimensions; ++dim) {
+        const JoeKuoRow& row = kJoeKuo[dim - 1];
+        uint64_t* v = d.data() + (size_t)dim * 64u;
+        const uint32_t s = row.degree;
+        for (uint32_t c = 0; c < s; ++c) v[c] = (uint64_t)row.m[c] << (63u - c);
+        for (uint32_t c = s; c < 64u; ++c) {
+            uint64_t x = v[c - s] ^ (v[c - s] >> s);
+            for (uint32_t k = 1; k < s; ++k)
+                if ((row.a >> (s - 1u - k)) & 1u) x ^= v[c - k];
+            v[c] = x;
+        }
+    }
+    return d;
+}
+
+struct SobolTable {
+    std::mutex lock;
+    int device = -1;
+    uint64_t* directions = nullptr;
+};
+SobolTable g_sobol;
+
+int sobol_table_for_current_device(const uint64_t** out) {
+    std::lock_guard<std::mutex> guard(g_sobol.lock);
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (g_sobol.directions == nullptr || g_sobol.device != device) {
+        const std::vector<uint64_t> table = sobol_directions();
+        uint64_t* ptr = nullptr;
+        HIP_TRY(hipMalloc(&ptr, table.size() * sizeof(uint64_t)));
+        HIP_TRY(hipMemcpy(ptr, table.data(), table.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+        g_sobol.directions = ptr; /* one table per process and device, kept until exit */
+        g_sobol.device = device;
+    }
+    *out = g_sobol.directions;
+    return BF_OK;
+}
+
+/* ---- exact triangle area (CudaWavefrontStages.cu:1724-2042, restated on the host) ----------- */
+struct Expansion {
+    float c[32];
+    uint32_t size = 0;
+    uint32_t capacity = 32;
+};
+bool normal_float(float v) { return std::isfinite(v) && std::fabs(v) >= 0x1p-126f; }
+bool expansion_add(Expansion& e, float component) {
+    if (!std::isfinite(component)) return false;
+    float acc = component;
+    uint32_t out = 0;
+    const uint32_t in = e.size;
+    for (uint32_t i = 0; i < in; ++i) {
+        const float existing = e.c[i];
+        const float rs = acc + existing;
+        const float efs = rs - acc;
+        const float afs = rs - efs;
+        const float error = (existing - efs) + (acc - afs);
+        if (!std::isfinite(rs) || !std::isfinite(error)) return false;
+        if (error != 0.0f) {
+            if (out == e.capacity) return false;
+            e.c[out++] = error;
+        }
+        acc = rs;
+    }
+    if (acc != 0.0f || out == 0) {
+        if (out == e.capacity) return false;
+        e.c[out++] = acc;
+    }
+    e.size = out;
+    return true;
+}
+bool expansion_add_product(Expansion& e, float l, float r) {
+    const float rounded = l * r;
+    const float err = std::fma(l, r, -rounded);
+    if (!std::isfinite(rounded) || !std::isfinite(err) || (l != 0.0f && r != 0.0f && !normal_float(rounded)) ||
+        (err != 0.0f && !normal_float(err)))
+        return false;
+    return expansion_add(e, err) && expansion_add(e, rounded);
+}
+bool resolved_difference(const Expansion& a, const Expansion& b, const Expansion& c, const Expansion& d, int& sign,
+                         float& value) {
+    Expansion p1, p2, diff;
+    p1.capacity = p2.capacity = 16;
+    for (uint32_t i = 0; i < a.size; ++i)
+        for (uint32_t j = 0; j < b.size; ++j)
+            if (!expansion_add_product(p1, a.c[i], b.c[j])) return false;
+    for (uint32_t i = 0; i < c.size; ++i)
+        for (uint32_t j = 0; j < d.size; ++j)
+            if (!expansion_add_product(p2, c.c[i], d.c[j])) return false;
+    for (uint32_t i = 0; i < p1.size; ++i)
+        if (!expansion_add(diff, p1.c[i])) return false;
+    for (uint32_t i = 0; i < p2.size; ++i)
+        if (!expansion_add(diff, -1.0f * p2.c[i])) return false;
+    int s = 0;
+    for (uint32_t i = diff.size; i > 0; --i)
+        if (diff.c[i - 1] != 0.0f) {
+            s = diff.c[i - 1] > 0.0f ? 1 : -1;
+            break;
+        }
+    if (s == 0) {
+        sign = 0;
+        value = 0.0f;
+        return true;
+    }
+    float resolved = 0.0f;
+    for (uint32_t i = 0; i < diff.size; ++i) resolved += diff.c[i];
+    if (!std::isfinite(resolved) || resolved == 0.0f || (resolved > 0.0f ? 1 : -1) != s) return false;
+    sign = s;
+    value = resolved;
+    return true;
+}
+
+bool triangle_area(const float v[3][3], float& area, float normal[3]) {
+    int ex[3];
+    for (int a = 0; a < 3; ++a) {
+        float m = 0.0f;
+        for (int i = 0; i < 3; ++i) m = std::fmax(m, std::fabs(v[i][a]));
+        std::frexp(m, &ex[a]);
+    }
+    float s[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int a = 0; a < 3; ++a) {
+            s[i][a] = std::ldexp(v[i][a], -ex[a]);
+            if (v[i][a] != 0.0f && !normal_float(s[i][a])) return false;
+        }
+    Expansion f[3], g[3];
+    for (int a = 0; a < 3; ++a) {
+        f[a].capacity = g[a].capacity = 2;
+        if (!expansion_add(f[a], s[1][a]) || !expansion_add(f[a], -s[0][a]) || !expansion_add(g[a], s[2][a]) ||
+            !expansion_add(g[a], -s[0][a]))
+            return false;
+    }
+    int xs, ys, zs;
+    float xv, yv, zv;
+    if (!resolved_difference(f[1], g[2], f[2], g[1], xs, xv) || !resolved_difference(f[2], g[0], f[0], g[2], ys, yv) ||
+        !resolved_difference(f[0], g[1], f[1], g[0], zs, zv))
+        return false;
+    const int cx = ex[1] + ex[2], cy = ex[2] + ex[0], cz = ex[0] + ex[1];
+    int mag = INT32_MIN, e;
+    if (xs != 0) {
+        std::frexp(std::fabs(xv), &e);
+        mag = std::max(mag, e + cx);
+    }
+    if (ys != 0) {
+        std::frexp(std::fabs(yv), &e);
+        mag = std::max(mag, e + cy);
+    }
+    if (zs != 0) {
+        std::frexp(std::fabs(zv), &e);
+        mag = std::max(mag, e + cz);
+    }
+    if (mag == INT32_MIN) return false;
+    const float ax = xs == 0 ? 0.0f : std::ldexp(xv, cx - mag);
+    const float ay = ys == 0 ? 0.0f : std::ldexp(yv, cy - mag);
+    const float az = zs == 0 ? 0.0f : std::ldexp(zv, cz - mag);
+    if ((xs != 0 && ax == 0.0f) || (ys != 0 && ay == 0.0f) || (zs != 0 && az == 0.0f)) return false;
+    const float maxc = std::fmax(std::fabs(ax), std::fmax(std::fabs(ay), std::fabs(az)));
+    if (!(maxc > 0.0f)) return false;
+    const float sx = ax / maxc, sy = ay / maxc, sz = az / maxc;
+    const float sq = std::fma(sx, sx, std::fma(sy, sy, sz * sz));
+    const float sl = std::sqrt(sq);
+    const float al = maxc * sl;
+    area = std::ldexp(al * 0.5f, mag);
+    if (!(sl > 0.0f) || !(al > 0.0f) || !std::isfinite(area) || !(area > 0.0f)) return false;
+    if ((xv != 0.0f && !normal_float(ax)) || (yv != 0.0f && !normal_float(ay)) || (zv != 0.0f && !normal_float(az)))
+        return false;
+    normal[0] = sx / sl;
+    normal[1] = sy / sl;
+    normal[2] = sz / sl;
+    return std::isfinite(normal[0]) && std::isfinite(normal[1]) && std::isfinite(normal[2]);
+}
+
+/* ---- deterministic BVH builder (SceneBvh.cpp:355-427, 477-736) ------------------------------ */
+struct Item {
+    float b[6];
+    double c[3];
+    uint32_t key;
+    bf_bvh_instance_ref ref;
+};
+
+struct Builder {
+    std::vector<bf_bvh_node>* nodes;
+    std::vector<uint32_t>* prim_refs;
+    std::vector<bf_bvh_instance_ref>* inst_refs;
+};
+
+uint32_t build_node(std::vector<Item>& items, size_t begin, size_t end, Builder& b) {
+    const uint32_t index = (uint32_t)b.nodes->size();
+    b.nodes->emplace_back();
+    auto init = [](const float* box) {
+        bf_bvh_node n{};
+        n.min_x = box[0];
+        n.min_y = box[1];
+        n.min_z = box[2];
+        n.max_x = box[3];
+        n.max_y = box[4];
+        n.max_z = box[5];
+        n.split_axis = n.first_child = n.second_child = BF_INVALID_INDEX;
+        return n;
+    };
+    if (end - begin == 1) {
+        bf_bvh_node n = init(items[begin].b);
+        n.kind = 1;
+        if (b.prim_refs) {
+            n.reference_offset = (uint32_t)b.prim_refs->size();
+            b.prim_refs->push_back(items[begin].key);
+        } else {
+            n.reference_offset = (uint32_t)b.inst_refs->size();
+            b.inst_refs->push_back(items[begin].ref);
+        }
+        n.reference_count = 1;
+        (*b.nodes)[index] = n;
+        return index;
+    }
+    double mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) mn[a] = mx[a] = items[begin].c[a];
+    for (size_t i = begin + 1; i < end; ++i)
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], items[i].c[a]);
+            mx[a] = std::max(mx[a], items[i].c[a]);
+        }
+    uint32_t axis = 0;
+    double largest = mx[0] - mn[0];
+    for (uint32_t a = 1; a < 3; ++a)
+        if (mx[a] - mn[a] > largest) {
+            largest = mx[a] - mn[a];
+            axis = a;
+        }
+    std::sort(items.begin() + (ptrdiff_t)begin, items.begin() + (ptrdiff_t)end, [axis](const Item& l, const Item& r) {
+        if (l.c[axis] < r.c[axis]) return true;
+        if (r.c[axis] < l.c[axis]) return false;
+        return l.key < r.key;
+    });
+    const size_t middle = begin + (end - begin) / 2;
+    const uint32_t first = build_node(items, begin, middle, b);
+    const uint32_t second = build_node(items, middle, end, b);
+    const bf_bvh_node f = (*b.nodes)[first], s = (*b.nodes)[second];
+    const float box[6] = {canonical_zero(std::min(f.min_x, s.min_x)), canonical_zero(std::min(f.min_y, s.min_y)),
+                          canonical_zero(std::min(f.min_z, s.min_z)), canonical_zero(std::max(f.max_x, s.max_x)),
+                          canonical_zero(std::max(f.max_y, s.max_y)), canonical_zero(std::max(f.max_z, s.max_z))};
+    bf_bvh_node n = init(box);
+    n.split_axis = axis;
+    n.first_child = first;
+    n.second_child = second;
+    (*b.nodes)[index] = n;
+    return index;
+}
+
+void item_centroid(Item& it) {
+    for (int a = 0; a < 3; ++a) it.c[a] = ((double)it.b[a] + (double)it.b[3 + a]) * 0.5;
+}
+
+/* SceneBvh.cpp:555-601 */
+bool point_bounds(const float* m, const float* p, float* out) {
+    const double eps = (double)0x1p-23f;
+    const double gamma7 = (7.0 * eps) / (1.0 - 7.0 * eps);
+    const double underflow = 7.0 * (double)0x1p-149f;
+    const double maximum = (double)3.40282346638528859812e+38f;
+    for (int row = 0; row < 3; ++row) {
+        double exact = (double)m[row * 4 + 3];
+        double magnitude = std::fabs(exact);
+        for (int col = 0; col < 3; ++col) {
+            const double product = (double)m[row * 4 + col] * (double)p[col];
+            exact += product;
+            magnitude += std::fabs(product);
+        }
+        const double error = std::fma(gamma7, magnitude, underflow);
+        const double lower = exact - error, upper = exact + error;
+        if (!std::isfinite(exact) || !std::isfinite(magnitude) || lower < -maximum || upper > maximum) return false;
+        float rl = (float)lower, ru = (float)upper;
+        if ((double)rl > lower) rl = next_down(rl);
+        if ((double)ru < upper) ru = next_up(ru);
+        out[row] = canonical_zero(rl);
+        out[3 + row] = canonical_zero(ru);
+    }
+    return true;
+}
+
+uint32_t tree_depth(const std::vector<bf_bvh_node>& nodes, uint32_t root) {
+    uint32_t best = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> stack{{root, 1u}};
+    while (!stack.empty()) {
+        auto [n, d] = stack.back();
+        stack.pop_back();
+        best = std::max(best, d);
+        if (nodes[n].kind == 0) {
+            stack.push_back({nodes[n].first_child, d + 1});
+            stack.push_back({nodes[n].second_child, d + 1});
+        }
+    }
+    return best;
+}
+
+} /* namespace */
+
+/* ---- scene ----------------------------------------------------------------------------------- */
+struct bf_hip_scene {
+    int device = -1;
+    std::vector<bf_bvh_blas> blases;
+    std::vector<bf_bvh_node> blas_nodes, tlas_nodes;
+    std::vector<uint32_t> prim_refs;
+    std::vector<bf_bvh_instance_ref> inst_refs;
+    uint32_t tlas_root = BF_INVALID_INDEX;
+    std::vector<bf_wide_node> blas_wide, tlas_wide;
+    void* device_block = nullptr;
+    bf_dev_scene dev{};
+    uint32_t max_blas_depth = 0, tlas_depth = 0;
+    uint32_t light_count = 0;
+};
+
+namespace {
+
+/* Canonical pre-order nodes -> wide nodes (one per internal node, same relative order). */
+void relayout(const std::vector<bf_bvh_node>& nodes, uint32_t node_offset, uint32_t node_count, uint32_t leaf_base_is_ref,
+              std::vector<bf_wide_node>& wide, std::vector<uint32_t>& canonical_to_ref) {
+    (void)leaf_base_is_ref;
+    for (uint32_t i = node_offset; i < node_offset + node_count; ++i) {
+        if (nodes[i].kind == 0) {
+            canonical_to_ref[i] = (uint32_t)wide.size();
+            wide.emplace_back();
+        } else {
+            canonical_to_ref[i] = BF_WIDE_LEAF | nodes[i].reference_offset;
+        }
+    }
+    for (uint32_t i = node_offset; i < node_offset + node_count; ++i) {
+        if (nodes[i].kind != 0) continue;
+        bf_wide_node& w = wide[canonical_to_ref[i]];
+        const bf_bvh_node* c[2] = {&nodes[nodes[i].first_child], &nodes[nodes[i].second_child]};
+        for (int k = 0; k < 2; ++k) {
+            w.lo_x[k] = c[k]->min_x;
+            w.lo_y[k] = c[k]->min_y;
+            w.lo_z[k] = c[k]->min_z;
+            w.hi_x[k] = c[k]->max_x;
+            w.hi_y[k] = c[k]->max_y;
+            w.hi_z[k] = c[k]->max_z;
+        }
+        w.child[0] = canonical_to_ref[nodes[i].first_child];
+        w.child[1] = canonical_to_ref[nodes[i].second_child];
+        w.canonical_index = i;
+        w.split_axis = nodes[i].split_axis;
+    }
+}
+
+bool valid_box(const bf_bvh_node& n) {
+    return std::isfinite(n.min_x) && std::isfinite(n.min_y) && std::isfinite(n.min_z) && std::isfinite(n.max_x) &&
+           std::isfinite(n.max_y) && std::isfinite(n.max_z) && n.min_x <= n.max_x && n.min_y <= n.max_y &&
+           n.min_z <= n.max_z;
+}
+
+struct Arena {
+    std::vector<std::pair<size_t, const void*>> parts;
+    std::vector<size_t> sizes;
+    size_t total = 0;
+    size_t add(const void* data, size_t bytes) {
+        const size_t offset = total;
+        parts.push_back({offset, data});
+        sizes.push_back(bytes);
+        total += (bytes + 255u) & ~(size_t)255u;
+        return offset;
+    }
+};
+
+int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
+    if (d->geometry_count == 0 || d->instance_count == 0 || d->material_count == 0 || !d->positions || !d->normals ||
+        !d->uvs || !d->indices || !d->geometries || !d->instances || !d->materials || !d->spectra ||
+        !d->spectrum_values)
+        return set_error(BF_INVALID_ARGUMENT, "scene description is missing arrays or counts");
+    if (d->vertex_count > 0xFFFFFFFFull || d->triangle_count > 0x7FFFFFFFull || d->instance_count > 0x7FFFFFFFu)
+        return set_error(BF_RESOURCE_EXHAUSTED, "scene exceeds the 31-bit reference domain of the wide BVH");
+    /* -- BLAS per geometry -- */
+    uint32_t prev_id = 0;
+    for (uint32_t g = 0; g < d->geometry_count; ++g) {
+        const bf_geometry_desc& geo = d->geometries[g];
+        if (g > 0 && geo.id <= prev_id)
+            return set_error(BF_INVALID_ARGUMENT, "geometry identifiers must be strictly increasing");
+        prev_id = geo.id;
+        if (geo.triangle_count == 0 || geo.vertex_count == 0 || geo.vertex_offset + geo.vertex_count > d->vertex_count ||
+            geo.triangle_offset + geo.triangle_count > d->triangle_count)
+            return set_error(BF_INVALID_ARGUMENT, "geometry %u has an invalid range", g);
+        std::vector<Item> items(geo.triangle_count);
+        for (uint32_t p = 0; p < geo.triangle_count; ++p) {
+            const uint64_t t = geo.triangle_offset + p;
+            Item& it = items[p];
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t vi = d->indices[3 * t + c];
+                if (vi >= geo.vertex_count) return set_error(BF_INVALID_ARGUMENT, "triangle index out of range");
+                const float* pos = d->positions + 3 * (geo.vertex_offset + vi);
+                for (int a = 0; a < 3; ++a) {
+                    if (!std::isfinite(pos[a])) return set_error(BF_INVALID_ARGUMENT, "non-finite vertex position");
+                    if (c == 0) it.b[a] = it.b[3 + a] = canonical_zero(pos[a]);
+                    else {
+                        it.b[a] = canonical_zero(std::min(it.b[a], pos[a]));
+                        it.b[3 + a] = canonical_zero(std::max(it.b[3 + a], pos[a]));
+                    }
+                }
+            }
+            item_centroid(it);
+            it.key = p;
+        }
+        const uint32_t node_offset = (uint32_t)s->blas_nodes.size();
+        const uint32_t ref_offset = (uint32_t)s->prim_refs.size();
+        Builder b{&s->blas_nodes, &s->prim_refs, nullptr};
+        const uint32_t root = build_node(items, 0, items.size(), b);
+        bf_bvh_blas blas{};
+        blas.geometry_id = geo.id;
+        blas.root_node = root;
+        blas.node_offset = node_offset;
+        blas.node_count = (uint32_t)(s->blas_nodes.size() - node_offset);
+        blas.primitive_reference_offset = ref_offset;
+        blas.primitive_reference_count = (uint32_t)geo.triangle_count;
+        s->blases.push_back(blas);
+        s->max_blas_depth = std::max(s->max_blas_depth, tree_depth(s->blas_nodes, root));
+    }
+    /* -- TLAS over instances (SceneBvh.cpp:603-736) -- */
+    std::vector<Item> items(d->instance_count);
+    prev_id = 0;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const bf_instance_desc& inst = d->instances[i];
+        if (i > 0 && inst.id <= prev_id) return set_error(BF_INVALID_ARGUMENT, "instance identifiers must be strictly increasing");
+        prev_id = inst.id;
+        if (inst.geometry_index >= d->geometry_count || inst.material_index >= d->material_count)
+            return set_error(BF_INVALID_ARGUMENT, "instance %u references a missing geometry or material", i);
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(inst.local_to_world[k]) || !std::isfinite(inst.world_to_local[k]))
+                return set_error(BF_INVALID_ARGUMENT, "instance %u has a non-finite transform", i);
+        const bf_geometry_desc& geo = d->geometries[inst.geometry_index];
+        bool have = false;
+        float box[6];
+        for (uint64_t p = 0; p < geo.triangle_count; ++p)
+            for (int c = 0; c < 3; ++c) {
+                const float* pos = d->positions + 3 * (geo.vertex_offset + d->indices[3 * (geo.triangle_offset + p) + c]);
+                float pb[6];
+                if (!point_bounds(inst.local_to_world, pos, pb))
+                    return set_error(BF_INVALID_ARGUMENT, "instance %u bounds are not representable", i);
+                if (!have) {
+                    std::memcpy(box, pb, sizeof box);
+                    have = true;
+                } else
+                    for (int a = 0; a < 3; ++a) {
+                        box[a] = canonical_zero(std::min(box[a], pb[a]));
+                        box[3 + a] = canonical_zero(std::max(box[3 + a], pb[3 + a]));
+                    }
+            }
+        Item& it = items[i];
+        std::memcpy(it.b, box, sizeof box);
+        item_centroid(it);
+        it.key = inst.id;
+        it.ref = bf_bvh_instance_ref{};
+        it.ref.scene_instance_index = i;
+        it.ref.instance_id = inst.id;
+        it.ref.scene_geometry_index = inst.geometry_index;
+        it.ref.geometry_id = geo.id;
+        it.ref.blas_index = inst.geometry_index;
+        it.ref.visibility_mask = inst.visibility_mask;
+    }
+    Builder tb{&s->tlas_nodes, nullptr, &s->inst_refs};
+    s->tlas_root = build_node(items, 0, items.size(), tb);
+    s->tlas_depth = tree_depth(s->tlas_nodes, s->tlas_root);
+    if (s->tlas_depth + s->max_blas_depth + 2u > BF_STACK_DEPTH)
+        return set_error(BF_RESOURCE_EXHAUSTED, "BVH depth %u + %u exceeds the %u-entry traversal stack", s->tlas_depth,
+                         s->max_blas_depth, BF_STACK_DEPTH);
+    for (const auto& n : s->blas_nodes)
+        if (!valid_box(n)) return set_error(BF_INVALID_ARGUMENT, "BLAS node box is not finite and ordered");
+    for (const auto& n : s->tlas_nodes)
+        if (!valid_box(n)) return set_error(BF_INVALID_ARGUMENT, "TLAS node box is not finite and ordered");
+
+    /* -- wide relayout -- */
+    std::vector<uint32_t> blas_ref(s->blas_nodes.size()), tlas_ref(s->tlas_nodes.size());
+    std::vector<bf_dev_blas> dev_blases(d->geometry_count);
+    for (uint32_t g = 0; g < d->geometry_count; ++g) {
+        const bf_bvh_blas& b = s->blases[g];
+        relayout(s->blas_nodes, b.node_offset, b.node_count, 0, s->blas_wide, blas_ref);
+        bf_dev_blas& db = dev_blases[g];
+        db = bf_dev_blas{};
+        db.root_ref = blas_ref[b.root_node];
+        db.geometry_index = g;
+        db.canonical_node_offset = b.node_offset;
+        const bf_bvh_node& r = s->blas_nodes[b.root_node];
+        const float rb[6] = {r.min_x, r.min_y, r.min_z, r.max_x, r.max_y, r.max_z};
+        std::memcpy(db.root_box, rb, sizeof rb);
+    }
+    relayout(s->tlas_nodes, 0, (uint32_t)s->tlas_nodes.size(), 1, s->tlas_wide, tlas_ref);
+    /* leaf triangles in BLAS leaf order: object-space vertices + primitive index */
+    std::vector<float> leaf_tris(s->prim_refs.size() * 12);
+    for (uint32_t g = 0; g < d->geometry_count; ++g) {
+        const bf_bvh_blas& b = s->blases[g];
+        const bf_geometry_desc& geo = d->geometries[g];
+        for (uint32_t r = 0; r < b.primitive_reference_count; ++r) {
+            const uint32_t ref = b.primitive_reference_offset + r;
+            const uint32_t prim = s->prim_refs[ref];
+            for (int c = 0; c < 3; ++c) {
+                const float* pos = d->positions + 3 * (geo.vertex_offset + d->indices[3 * (geo.triangle_offset + prim) + c]);
+                leaf_tris[12 * (size_t)ref + 4 * c + 0] = pos[0];
+                leaf_tris[12 * (size_t)ref + 4 * c + 1] = pos[1];
+                leaf_tris[12 * (size_t)ref + 4 * c + 2] = pos[2];
+                float payload = 0.0f;
+                if (c == 0) std::memcpy(&payload, &prim, sizeof payload);
+                leaf_tris[12 * (size_t)ref + 4 * c + 3] = payload;
+            }
+        }
+    }
+    std::vector<bf_dev_tlas_leaf> tlas_leaves(s->inst_refs.size());
+    for (size_t r = 0; r < s->inst_refs.size(); ++r) {
+        const bf_bvh_instance_ref& ir = s->inst_refs[r];
+        bf_dev_tlas_leaf& L = tlas_leaves[r];
+        std::memcpy(L.world_to_local, d->instances[ir.scene_instance_index].world_to_local, sizeof L.world_to_local);
+        L.scene_instance = ir.scene_instance_index;
+        L.instance_id = ir.instance_id;
+        L.blas_index = ir.blas_index;
+        L.visibility_mask = ir.visibility_mask;
+    }
+    /* -- shading tables -- */
+    std::vector<bf_dev_instance> instances(d->instance_count);
+    std::vector<bf_dev_geometry> geometries(d->geometry_count);
+    for (uint32_t g = 0; g < d->geometry_count; ++g) {
+        const bf_geometry_desc& geo = d->geometries[g];
+        geometries[g] = bf_dev_geometry{geo.id, (uint32_t)geo.vertex_offset, (uint32_t)geo.vertex_count,
+                                        (uint32_t)geo.triangle_offset, (uint32_t)geo.triangle_count, {0, 0, 0}};
+    }
+    /* mesh-area lights: instances with non-black emission, in instance order (FrameScene.cpp:200-264) */
+    std::vector<bf_dev_light> lights;
+    std::vector<float> areas, probs, cumul, nexts;
+    std::vector<float> normals4;
+    for (uint32_t i = 0; i < d->instance_count; ++i) {
+        const bf_instance_desc& inst = d->instances[i];
+        bf_dev_instance& di = instances[i];
+        di = bf_dev_instance{};
+        std::memcpy(di.local_to_world, inst.local_to_world, sizeof di.local_to_world);
+        std::memcpy(di.world_to_local, inst.world_to_local, sizeof di.world_to_local);
+        di.geometry_index = inst.geometry_index;
+        di.material_index = inst.material_index;
+        di.id = inst.id;
+        di.object_id = inst.object_id;
+        di.visibility_mask = inst.visibility_mask;
+        di.light_index = -1;
+        const bf_material_desc& m = d->materials[inst.material_index];
+        if (m.emission_spectrum == BF_INVALID_INDEX) continue;
+        if (m.emission_spectrum >= d->spectrum_count) return set_error(BF_INVALID_ARGUMENT, "emission spectrum out of range");
+        const bf_spectrum_desc& sp = d->spectra[m.emission_spectrum];
+        bool black = true;
+        for (uint32_t k = 0; k < sp.count; ++k)
+            if (d->spectrum_values[sp.offset + k] != 0.0f) black = false;
+        if (black) continue;
+        const bf_geometry_desc& geo = d->geometries[inst.geometry_index];
+        const uint32_t n = (uint32_t)geo.triangle_count;
+        bf_dev_light L{};
+        L.instance = i;
+        L.geometry = inst.geometry_index;
+        L.triangle_count = n;
+        L.table_offset = (uint32_t)areas.size();
+        std::vector<float> tri_area(n);
+        float sum = 0.0f, comp = 0.0f;
+        for (uint32_t t = 0; t < n; ++t) {
+            float w[3][3];
+            for (int c = 0; c < 3; ++c) {
+                const float* p = d->positions + 3 * (geo.vertex_offset + d->indices[3 * (geo.triangle_offset + t) + c]);
+                const float* M = inst.local_to_world;
+                w[c][0] = std::fma(M[0], p[0], std::fma(M[1], p[1], std::fma(M[2], p[2], M[3])));
+                w[c][1] = std::fma(M[4], p[0], std::fma(M[5], p[1], std::fma(M[6], p[2], M[7])));
+                w[c][2] = std::fma(M[8], p[0], std::fma(M[9], p[1], std::fma(M[10], p[2], M[11])));
+            }
+            float area, nrm[3];
+            if (!triangle_area(w, area, nrm)) return set_error(BF_INVALID_ARGUMENT, "emitter triangle area is not representable");
+            tri_area[t] = area;
+            normals4.insert(normals4.end(), {nrm[0], nrm[1], nrm[2], 0.0f});
+            /* compensated running sum (CudaWavefrontStages.cu:2085-2107) */
+            const float next = sum + area;
+            comp += std::fabs(sum) >= std::fabs(area) ? (sum - next) + area : (area - next) + sum;
+            sum = next;
+        }
+        const float total = sum + comp;
+        L.inverse_area = 1.0f / total;
+        float cumulative = 0.0f;
+        for (uint32_t t = 0; t < n; ++t) {
+            float probability = 1.0f - cumulative, next = 1.0f;
+            if (t + 1 != n) {
+                probability = tri_area[t] / total;
+                next = cumulative + probability;
+                if (!(probability > 0.0f) || !(next > cumulative) || !(next < 1.0f))
+                    return set_error(BF_INVALID_ARGUMENT, "emitter area distribution is not representable");
+            } else if (!(probability > 0.0f) || probability > 1.0f) {
+                return set_error(BF_INVALID_ARGUMENT, "emitter area distribution is not representable");
+            }
+            areas.push_back(tri_area[t]);
+            probs.push_back(probability);
+            cumul.push_back(cumulative);
+            nexts.push_back(next);
+            cumulative = next;
+        }
+        di.light_index = (int32_t)lights.size();
+        lights.push_back(L);
+    }
+    if (lights.size() > (1u << 24)) return set_error(BF_RESOURCE_EXHAUSTED, "more than 2^24 lights");
+
+    if (d->has_environment && d->environment_spectrum >= d->spectrum_count)
+        return set_error(BF_INVALID_ARGUMENT, "environment spectrum out of range");
+    s->light_count = (uint32_t)lights.size();
+    if (!upload) return BF_OK;
+    /* -- device upload: one allocation, 256-byte aligned parts -- */
+    Arena A;
+    const size_t o_tlas = A.add(s->tlas_wide.data(), s->tlas_wide.size() * sizeof(bf_wide_node));
+    const size_t o_blasw = A.add(s->blas_wide.data(), s->blas_wide.size() * sizeof(bf_wide_node));
+    const size_t o_blas = A.add(dev_blases.data(), dev_blases.size() * sizeof(bf_dev_blas));
+    const size_t o_tris = A.add(leaf_tris.data(), leaf_tris.size() * sizeof(float));
+    const size_t o_leaves = A.add(tlas_leaves.data(), tlas_leaves.size() * sizeof(bf_dev_tlas_leaf));
+    const size_t o_inst = A.add(instances.data(), instances.size() * sizeof(bf_dev_instance));
+    const size_t o_geo = A.add(geometries.data(), geometries.size() * sizeof(bf_dev_geometry));
+    const size_t o_pos = A.add(d->positions, d->vertex_count * 3 * sizeof(float));
+    const size_t o_nrm = A.add(d->normals, d->vertex_count * 3 * sizeof(float));
+    const size_t o_uv = A.add(d->uvs, d->vertex_count * 2 * sizeof(float));
+    const size_t o_idx = A.add(d->indices, d->triangle_count * 3 * sizeof(uint32_t));
+    const size_t o_mat = A.add(d->materials, d->material_count * sizeof(bf_material_desc));
+    const size_t o_clo = A.add(d->closures, d->closure_count * sizeof(bf_closure_desc));
+    const size_t o_spe = A.add(d->spectra, d->spectrum_count * sizeof(bf_spectrum_desc));
+    const size_t o_spv = A.add(d->spectrum_values, d->spectrum_value_count * sizeof(float));
+    const size_t o_lig = A.add(lights.data(), lights.size() * sizeof(bf_dev_light));
+    const size_t o_are = A.add(areas.data(), areas.size() * sizeof(float));
+    const size_t o_pro = A.add(probs.data(), probs.size() * sizeof(float));
+    const size_t o_cum = A.add(cumul.data(), cumul.size() * sizeof(float));
+    const size_t o_nex = A.add(nexts.data(), nexts.size() * sizeof(float));
+    const size_t o_lno = A.add(normals4.data(), normals4.size() * sizeof(float));
+    HIP_TRY(hipGetDevice(&s->device));
+    HIP_TRY(hipMalloc(&s->device_block, A.total ? A.total : 256));
+    std::vector<uint8_t> staging(A.total, 0);
+    for (size_t k = 0; k < A.parts.size(); ++k)
+        if (A.sizes[k] && A.parts[k].second) std::memcpy(staging.data() + A.parts[k].first, A.parts[k].second, A.sizes[k]);
+    HIP_TRY(hipMemcpy(s->device_block, staging.data(), A.total, hipMemcpyHostToDevice));
+    uint8_t* base = (uint8_t*)s->device_block;
+    bf_dev_scene& S = s->dev;
+    S.tlas_nodes = (const bf_wide_node*)(base + o_tlas);
+    S.blas_nodes = (const bf_wide_node*)(base + o_blasw);
+    S.blases = (const bf_dev_blas*)(base + o_blas);
+    S.leaf_triangles = (const float4*)(base + o_tris);
+    S.tlas_leaves = (const bf_dev_tlas_leaf*)(base + o_leaves);
+    S.tlas_root_ref = tlas_ref[s->tlas_root];
+    S.instance_count = d->instance_count;
+    const bf_bvh_node& tr = s->tlas_nodes[s->tlas_root];
+    const float trb[6] = {tr.min_x, tr.min_y, tr.min_z, tr.max_x, tr.max_y, tr.max_z};
+    std::memcpy(S.tlas_root_box, trb, sizeof trb);
+    S.instances = (const bf_dev_instance*)(base + o_inst);
+    S.geometries = (const bf_dev_geometry*)(base + o_geo);
+    S.positions = (const float*)(base + o_pos);
+    S.normals = (const float*)(base + o_nrm);
+    S.uvs = (const float*)(base + o_uv);
+    S.indices = (const uint32_t*)(base + o_idx);
+    S.materials = (const bf_material_desc*)(base + o_mat);
+    S.closures = (const bf_closure_desc*)(base + o_clo);
+    S.spectra = (const bf_spectrum_desc*)(base + o_spe);
+    S.spectrum_values = (const float*)(base + o_spv);
+    S.lights = (const bf_dev_light*)(base + o_lig);
+    S.light_areas = (const float*)(base + o_are);
+    S.light_probabilities = (const float*)(base + o_pro);
+    S.light_cumulative = (const float*)(base + o_cum);
+    S.light_next = (const float*)(base + o_nex);
+    S.light_normals = (const float4*)(base + o_lno);
+    S.light_count = (uint32_t)lights.size();
+    S.has_environment = d->has_environment ? 1u : 0u;
+    S.environment_spectrum = d->environment_spectrum;
+    S.material_count = d->material_count;
+    return BF_OK;
+}
+
+} /* namespace */
+
+/* ---- workspace --------------------------------------------------------------------------------- */
+struct bf_hip_workspace {
+    uint64_t capacity = 0;
+    void* block = nullptr;
+    bf_dev_paths paths{};
+    uint32_t* queues[4] = {nullptr, nullptr, nullptr, nullptr}; /* ray, hit, shadow, continuation */
+    uint8_t* route = nullptr;
+    uint32_t* block_counts = nullptr;
+    uint32_t* block_offsets = nullptr;
+    uint32_t* sizes = nullptr;     /* [0] ray [1] hit [2] shadow [3] cont [4..7] scratch */
+    uint32_t* failure = nullptr;   /* [0] count [1] first slot */
+    uint32_t* sample_counts = nullptr;
+    unsigned long long* traversal = nullptr; /* 6 counters */
+    unsigned long long* hashes = nullptr;    /* 2 words */
+    uint32_t* host_words = nullptr;          /* pinned readback */
+};
+
+extern "C" {
+
+const char* blackframe_hip_last_error(void) { return g_last_error.c_str(); }
+
+int blackframe_hip_device_count(uint32_t* count) {
+    if (!count) return set_error(BF_INVALID_ARGUMENT, "count is NULL");
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? (uint32_t)n : 0u;
+    return BF_OK;
+}
+
+static int create_scene(const bf_scene_desc* desc, bf_hip_scene** out, bool upload) {
+    if (!desc || !out) return set_error(BF_INVALID_ARGUMENT, "scene description or output is NULL");
+    *out = nullptr;
+    bf_hip_scene* s = new (std::nothrow) bf_hip_scene();
+    if (!s) return set_error(BF_RESOURCE_EXHAUSTED, "host allocation failed");
+    int status;
+    try {
+        status = build_scene(desc, s, upload);
+    } catch (const std::bad_alloc&) {
+        status = set_error(BF_RESOURCE_EXHAUSTED, "host allocation failed during scene build");
+    }
+    if (status != BF_OK) {
+        blackframe_hip_scene_destroy(s);
+        return status;
+    }
+    *out = s;
+    return BF_OK;
+}
+
+int blackframe_hip_scene_create(const bf_scene_desc* desc, bf_hip_scene** out) { return create_scene(desc, out, true); }
+
+int blackframe_hip_scene_prepare(const bf_scene_desc* desc, bf_hip_scene** out) { return create_scene(desc, out, false); }
+
+int blackframe_hip_scene_destroy(bf_hip_scene* s) {
+    if (!s) return BF_OK;
+    hipError_t e = hipSuccess;
+    if (s->device_block) e = hipFree(s->device_block);
+    delete s;
+    return e == hipSuccess ? BF_OK : hip_error(e, "hipFree(scene)");
+}
+
+int blackframe_hip_scene_bvh_export(const bf_hip_scene* s, uint64_t counts[5], bf_bvh_blas* blases, bf_bvh_node* blas_nodes,
+                                    uint32_t* primitive_refs, bf_bvh_node* tlas_nodes, bf_bvh_instance_ref* instance_refs,
+                                    uint32_t* tlas_root) {
+    if (!s || !counts) return set_error(BF_INVALID_ARGUMENT, "scene or counts is NULL");
+    counts[0] = s->blases.size();
+    counts[1] = s->blas_nodes.size();
+    counts[2] = s->prim_refs.size();
+    counts[3] = s->tlas_nodes.size();
+    counts[4] = s->inst_refs.size();
+    if (blases) std::memcpy(blases, s->blases.data(), counts[0] * sizeof(bf_bvh_blas));
+    if (blas_nodes) std::memcpy(blas_nodes, s->blas_nodes.data(), counts[1] * sizeof(bf_bvh_node));
+    if (primitive_refs) std::memcpy(primitive_refs, s->prim_refs.data(), counts[2] * sizeof(uint32_t));
+    if (tlas_nodes) std::memcpy(tlas_nodes, s->tlas_nodes.data(), counts[3] * sizeof(bf_bvh_node));
+    if (instance_refs) std::memcpy(instance_refs, s->inst_refs.data(), counts[4] * sizeof(bf_bvh_instance_ref));
+    if (tlas_root) *tlas_root = s->tlas_root;
+    return BF_OK;
+}
+
+int blackframe_hip_scene_wide_export(const bf_hip_scene* s, uint64_t counts[2], bf_wide_node* blas_wide,
+                                     bf_wide_node* tlas_wide) {
+    if (!s || !counts) return set_error(BF_INVALID_ARGUMENT, "scene or counts is NULL");
+    counts[0] = s->blas_wide.size();
+    counts[1] = s->tlas_wide.size();
+    if (blas_wide) std::memcpy(blas_wide, s->blas_wide.data(), counts[0] * sizeof(bf_wide_node));
+    if (tlas_wide) std::memcpy(tlas_wide, s->tlas_wide.data(), counts[1] * sizeof(bf_wide_node));
+    return BF_OK;
+}
+
+int blackframe_hip_launch_scene_closest_hit(const bf_hip_scene* s, const bf_transport_ray* rays, uint32_t count,
+                                            bf_closest_hit_result* results, uint32_t* counters, void* stream) {
+    if (!s || (count != 0 && (!rays || !results))) return set_error(BF_INVALID_ARGUMENT, "NULL scene, rays or results");
+    if (!s->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
+    HIP_TRY(bfk_scene_closest_hit(&s->dev, rays, count, results, counters, (hipStream_t)stream));
+    return BF_OK;
+}
+
+int blackframe_hip_launch_scene_occlusion(const bf_hip_scene* s, const bf_transport_ray* rays, uint32_t count,
+                                          bf_occlusion_result* results, void* stream) {
+    if (!s || (count != 0 && (!rays || !results))) return set_error(BF_INVALID_ARGUMENT, "NULL scene, rays or results");
+    if (!s->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
+    HIP_TRY(bfk_scene_occlusion(&s->dev, rays, count, results, (hipStream_t)stream));
+    return BF_OK;
+}
+
+int blackframe_hip_launch_sample_stream_dump(uint32_t sampler, const bf_sample_stream_index* indices,
+                                             const uint64_t* dimensions, uint32_t count, float* samples, void* stream) {
+    if (sampler > BF_SAMPLER_SOBOL_OWEN) return set_error(BF_INVALID_ARGUMENT, "unknown sampler %u", sampler);
+    if (count != 0 && (!indices || !dimensions || !samples)) return set_error(BF_INVALID_ARGUMENT, "NULL argument");
+    const uint64_t* directions = nullptr;
+    const int st = sobol_table_for_current_device(&directions);
+    if (st != BF_OK) return st;
+    HIP_TRY(bfk_sample_dump(sampler, directions, kSobolDimensions, indices, dimensions, count, samples, (hipStream_t)stream));
+    return BF_OK;
+}
+
+int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
+    if (!out || capacity == 0 || capacity > 0x7FFFFFFFull)
+        return set_error(BF_INVALID_ARGUMENT, "workspace capacity must be in [1, 2^31)");
+    *out = nullptr;
+    bf_hip_workspace* w = new (std::nothrow) bf_hip_workspace();
+    if (!w) return set_error(BF_RESOURCE_EXHAUSTED, "host allocation failed");
+    w->capacity = capacity;
+    const uint64_t n = capacity;
+    const uint32_t blocks = bfk_compact_blocks((uint32_t)n);
+    Arena A;
+    const size_t o_ro = A.add(nullptr, n * 16), o_rd = A.add(nullptr, n * 16), o_beta = A.add(nullptr, n * 16),
+                 o_rad = A.add(nullptr, n * 16), o_wl = A.add(nullptr, n * 16), o_dep = A.add(nullptr, n * 16),
+                 o_prev = A.add(nullptr, n * 16), o_hit = A.add(nullptr, n * 16), o_ids = A.add(nullptr, n * 8),
+                 o_so = A.add(nullptr, n * 16), o_sd = A.add(nullptr, n * 16), o_sc = A.add(nullptr, n * 16),
+                 o_st = A.add(nullptr, n * 4), o_vis = A.add(nullptr, n * 4);
+    size_t o_q[4];
+    for (int k = 0; k < 4; ++k) o_q[k] = A.add(nullptr, n * 4);
+    const size_t o_route = A.add(nullptr, n), o_bc = A.add(nullptr, (size_t)blocks * 8), o_bo = A.add(nullptr, (size_t)blocks * 8),
+                 o_sizes = A.add(nullptr, 64), o_fail = A.add(nullptr, 16), o_samp = A.add(nullptr, 16),
+                 o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, 16);
+    hipError_t e = hipMalloc(&w->block, A.total);
+    if (e != hipSuccess) {
+        delete w;
+        return set_error(BF_RESOURCE_EXHAUSTED, "workspace allocation of %zu bytes failed: %s", A.total, hipGetErrorString(e));
+    }
+    e = hipHostMalloc((void**)&w->host_words, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipFree(w->block);
+        delete w;
+        return hip_error(e, "hipHostMalloc");
+    }
+    uint8_t* b = (uint8_t*)w->block;
+    w->paths.ray_origin = (float4*)(b + o_ro);
+    w->paths.ray_direction = (float4*)(b + o_rd);
+    w->paths.beta = (float4*)(b + o_beta);
+    w->paths.radiance = (float4*)(b + o_rad);
+    w->paths.wavelengths = (float4*)(b + o_wl);
+    w->paths.depth = (uint4*)(b + o_dep);
+    w->paths.previous = (float4*)(b + o_prev);
+    w->paths.hit = (float4*)(b + o_hit);
+    w->paths.hit_ids = (uint2*)(b + o_ids);
+    w->paths.shadow_origin = (float4*)(b + o_so);
+    w->paths.shadow_direction = (float4*)(b + o_sd);
+    w->paths.shadow_contribution = (float4*)(b + o_sc);
+    w->paths.status = (uint32_t*)(b + o_st);
+    w->paths.visibility = (uint32_t*)(b + o_vis);
+    for (int k = 0; k < 4; ++k) w->queues[k] = (uint32_t*)(b + o_q[k]);
+    w->route = b + o_route;
+    w->block_counts = (uint32_t*)(b + o_bc);
+    w->block_offsets = (uint32_t*)(b + o_bo);
+    w->sizes = (uint32_t*)(b + o_sizes);
+    w->failure = (uint32_t*)(b + o_fail);
+    w->sample_counts = (uint32_t*)(b + o_samp);
+    w->traversal = (unsigned long long*)(b + o_trav);
+    w->hashes = (unsigned long long*)(b + o_hash);
+    *out = w;
+    return BF_OK;
+}
+
+int blackframe_hip_workspace_destroy(bf_hip_workspace* w) {
+    if (!w) return BF_OK;
+    hipError_t e1 = w->block ? hipFree(w->block) : hipSuccess;
+    hipError_t e2 = w->host_words ? hipHostFree(w->host_words) : hipSuccess;
+    delete w;
+    if (e1 != hipSuccess) return hip_error(e1, "hipFree(workspace)");
+    if (e2 != hipSuccess) return hip_error(e2, "hipHostFree(workspace)");
+    return BF_OK;
+}
+
+int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_scene* s, const bf_frame_desc* frame,
+                                             const bf_transport_options* o, float* film, double* film_compensated,
+                                             uint32_t* film_counts, bf_path_result* paths, bf_transport_report* report,
+                                             uint64_t* traversal_counters, void* stream_handle) {
+    if (!w || !s || !frame || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, scene, frame or options");
+    if (!s->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
+    const hipStream_t stream = (hipStream_t)stream_handle;
+    const bf_camera_desc& cam = frame->camera;
+    const uint64_t pixels = (uint64_t)cam.width * cam.height;
+    if (pixels == 0 || frame->samples_per_pixel == 0) return set_error(BF_INVALID_ARGUMENT, "empty frame");
+    if (pixels > w->capacity)
+        return set_error(BF_RESOURCE_EXHAUSTED, "workspace capacity %llu is below one sample pass (%llu paths)",
+                         (unsigned long long)w->capacity, (unsigned long long)pixels);
+    if (o->sampler > BF_SAMPLER_SOBOL_OWEN || o->wavelength_mode > BF_WAVELENGTHS_PER_PATH || o->mis_heuristic > BF_MIS_POWER)
+        return set_error(BF_INVALID_ARGUMENT, "unknown sampler, wavelength mode or MIS heuristic");
+    if (o->volume_depth_limit != 0) return set_error(BF_UNAVAILABLE, "participating media are not supported");
+    if (o->russian_roulette_enabled &&
+        (o->russian_roulette_first_depth == 0 || !(o->russian_roulette_minimum > 0.0f) || !(o->russian_roulette_minimum < 1.0f) ||
+         o->russian_roulette_maximum < o->russian_roulette_minimum || o->russian_roulette_maximum > 1.0f))
+        return set_error(BF_INVALID_ARGUMENT, "invalid Russian-roulette policy");
+    if (o->diffuse_depth_limit > 255 || o->glossy_depth_limit > 255 || o->specular_depth_limit > 255 ||
+        o->transmission_depth_limit > 255)
+        return set_error(BF_INVALID_ARGUMENT, "depth limits above 255 are not supported");
+    if (!film || !film_compensated || !film_counts) return set_error(BF_INVALID_ARGUMENT, "film buffers are required");
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (device != s->device) return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", s->device, device);
+
+    bf_transport_params P{};
+    P.scene = s->dev;
+    P.paths = w->paths;
+    const uint64_t* directions = nullptr;
+    const int sst = sobol_table_for_current_device(&directions);
+    if (sst != BF_OK) return sst;
+    P.sampler.kind = o->sampler;
+    P.sampler.sobol_dims = kSobolDimensions;
+    P.sampler.directions = directions;
+    P.camera = cam;
+    P.seed = frame->seed;
+    P.first_sample_index = frame->first_sample_index;
+    P.pixels = (uint32_t)pixels;
+    P.mis = o->mis_heuristic;
+    P.wavelength_mode = o->wavelength_mode;
+    P.limits[0] = o->diffuse_depth_limit;
+    P.limits[1] = o->glossy_depth_limit;
+    P.limits[2] = o->specular_depth_limit;
+    P.limits[3] = o->transmission_depth_limit;
+    P.limits[4] = 0;
+    P.rr_enabled = o->russian_roulette_enabled;
+    P.rr_first = o->russian_roulette_first_depth;
+    P.rr_min = o->russian_roulette_minimum;
+    P.rr_max = o->russian_roulette_maximum;
+    for (int k = 0; k < 4; ++k) P.packet[k] = o->packet_wavelengths[k];
+    P.failure = w->failure;
+
+    bf_transport_report rep{};
+    rep.first_failure_slot = BF_INVALID_INDEX;
+    HIP_TRY(bfk_set_words((uint32_t*)w->traversal, 12, 0u, stream));
+    HIP_TRY(bfk_set_words(w->failure, 1, 0u, stream));
+    HIP_TRY(bfk_set_words(w->failure + 1, 1, BF_INVALID_INDEX, stream));
+    HIP_TRY(bfk_set_words(w->sample_counts, 2, 0u, stream));
+    unsigned long long* counters = traversal_counters ? w->traversal : nullptr;
+    const uint64_t passes_per_launch = w->capacity / pixels;
+    const uint64_t bound = (uint64_t)o->diffuse_depth_limit + o->glossy_depth_limit + o->specular_depth_limit + 2u;
+    const float normalization = film_normalization();
+    for (uint64_t pass0 = 0; pass0 < frame->samples_per_pixel; pass0 += passes_per_launch) {
+        const uint64_t passes = std::min<uint64_t>(passes_per_launch, frame->samples_per_pixel - pass0);
+        const uint32_t n = (uint32_t)(passes * pixels);
+        P.slot_base = pass0 * pixels;
+        uint32_t* ray_q = w->queues[0];
+        uint32_t* cont_q = w->queues[3];
+        HIP_TRY(bfk_camera(&P, n, ray_q, stream));
+        HIP_TRY(bfk_set_words(w->sizes, 1, n, stream));
+        uint32_t n_ray = n;
+        uint32_t bounce = 0;
+        while (n_ray != 0 && bounce <= bound) {
+            const bool record = o->record_queues && pass0 == 0 && bounce < BF_REPORT_MAX_BOUNCES;
+            if (record) {
+                HIP_TRY(bfk_queue_hash(ray_q, &w->sizes[0], w->hashes, stream));
+                HIP_TRY(hipMemcpyAsync(&rep.ray_queue_hash[bounce], w->hashes, 8, hipMemcpyDeviceToHost, stream));
+                rep.ray_queue_size[bounce] = n_ray;
+            }
+            HIP_TRY(bfk_trace_closest(&P, ray_q, &w->sizes[0], n_ray, w->route, counters, stream));
+            HIP_TRY(bfk_compact(w->route, &w->sizes[0], n_ray, ray_q, w->block_counts, w->block_offsets, w->queues[1],
+                                &w->sizes[1], 0u, w->queues[2], &w->sizes[4], 0u, stream));
+            HIP_TRY(bfk_shade(&P, w->queues[1], &w->sizes[1], n_ray, w->route, w->sample_counts, stream));
+            HIP_TRY(bfk_compact(w->route, &w->sizes[1], n_ray, w->queues[1], w->block_counts, w->block_offsets, w->queues[2],
+                                &w->sizes[2], 0u, cont_q, &w->sizes[3], 0u, stream));
+            if (record) {
+                HIP_TRY(bfk_queue_hash(w->queues[2], &w->sizes[2], w->hashes + 1, stream));
+                HIP_TRY(hipMemcpyAsync(&rep.shadow_queue_hash[bounce], w->hashes + 1, 8, hipMemcpyDeviceToHost, stream));
+            }
+            HIP_TRY(bfk_trace_shadow(&P, w->queues[2], &w->sizes[2], n_ray, w->route, counters, stream));
+            HIP_TRY(bfk_compact(w->route, &w->sizes[2], n_ray, w->queues[2], w->block_counts, w->block_offsets, w->queues[1],
+                                &w->sizes[5], 0u, cont_q, &w->sizes[3], 1u, stream));
+            HIP_TRY(hipMemcpyAsync(w->host_words, w->sizes, 16, hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+            rep.closest_hit_queries += n_ray;
+            rep.shade_lanes += w->host_words[1];
+            rep.miss_lanes += n_ray - w->host_words[1];
+            rep.shadow_queries += w->host_words[2];
+            if (record) rep.shadow_queue_size[bounce] = w->host_words[2];
+            const uint32_t next = w->host_words[3];
+            std::swap(w->queues[0], w->queues[3]);
+            ray_q = w->queues[0];
+            cont_q = w->queues[3];
+            HIP_TRY(bfk_copy_word(&w->sizes[3], &w->sizes[0], stream));
+            n_ray = next;
+            ++bounce;
+        }
+        rep.bounces = std::max<uint32_t>(rep.bounces, bounce);
+        HIP_TRY(bfk_film(&w->paths, (uint32_t)pixels, (uint32_t)passes, 1.0f / 470.0f, normalization, film,
+                         film_compensated, film_counts, stream));
+        if (paths) HIP_TRY(bfk_export_paths(&w->paths, n, paths + P.slot_base, stream));
+    }
+    rep.path_count = pixels * frame->samples_per_pixel;
+    HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 8, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(w->host_words + 2, w->sample_counts, 8, hipMemcpyDeviceToHost, stream));
+    unsigned long long trav[6] = {0, 0, 0, 0, 0, 0};
+    if (traversal_counters) HIP_TRY(hipMemcpyAsync(trav, w->traversal, sizeof trav, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    rep.failed_paths = w->host_words[0];
+    rep.first_failure_slot = w->host_words[1];
+    rep.closure_samples = w->host_words[2];
+    rep.light_samples = w->host_words[3];
+    if (traversal_counters)
+        for (int k = 0; k < 6; ++k) traversal_counters[k] = trav[k];
+    if (report) *report = rep;
+    if (rep.failed_paths != 0)
+        return set_error(BF_TRANSPORT_FAILURE, "%llu paths failed; first failing slot %u",
+                         (unsigned long long)rep.failed_paths, rep.first_failure_slot);
+    return BF_OK;
+}
+
+} /* extern "C" */

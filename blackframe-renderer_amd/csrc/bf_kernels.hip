@@ -1,0 +1,1148 @@
+/*
+ * HIP kernels of the MI355X wavefront path-tracing backend (gfx950, wave64).
+ *
+ * One bounce of the path loop is: trace_closest (BVH2 closest hit; misses resolved in place) ->
+ * stable compaction into the hit queue -> shade (emission + MIS, NEE light sample, BSDF sample,
+ * Russian roulette) -> stable compaction into shadow / continuation queues -> trace_shadow
+ * (any-hit; visible contributions accumulated) -> stable append of shadow continuations.
+ * Stage semantics follow Blackframe/XPU/CUDA/Source/CudaWavefrontStages.cu; queue routing follows
+ * Blackframe/Backends/GPU/CUDA/Source/WavefrontTransport.cpp:2222-2649.
+ */
+#include <hip/hip_runtime.h>
+
+#include "bf_kernels.h"
+#include "bf_shading.h"
+#include "bf_traversal.h"
+
+#define BF_TRACE_BLOCK 128u
+#define BF_BLOCK 256u
+
+/* ---- path helpers ----------------------------------------------------------------------------- */
+BF_D sample_key key_for_slot(const bf_transport_params& P, uint32_t slot) {
+    const uint64_t global = P.slot_base + slot;
+    const uint64_t s = global / P.pixels;
+    const uint32_t rem = (uint32_t)(global - s * P.pixels);
+    sample_key k;
+    k.px = rem % P.camera.width;
+    k.py = rem / P.camera.width;
+    k.s = (uint64_t)P.first_sample_index + s;
+    k.seed = P.seed;
+    return k;
+}
+
+BF_D void fail_path(const bf_transport_params& P, uint32_t slot, uint32_t code) {
+    P.paths.status[slot] = BF_TERM_FAILED | (code << 8);
+    atomicAdd(&P.failure[0], 1u);
+    atomicMin(&P.failure[1], slot);
+}
+
+BF_D float spectrum_at(const bf_dev_scene& S, uint32_t id, float wavelength) {
+    if (id == BF_INVALID_INDEX) return 0.0f;
+    const bf_spectrum_desc sp = S.spectra[id];
+    const float* v = S.spectrum_values + sp.offset;
+    if (sp.count <= 1u) return v[0];
+    const float step = 470.0f / (float)(sp.count - 1u);
+    const float t = (wavelength - 360.0f) / step;
+    uint32_t i = t <= 0.0f ? 0u : (uint32_t)t;
+    if (i > sp.count - 2u) i = sp.count - 2u;
+    const float f = t - (float)i;
+    return fmaf(v[i + 1] - v[i], f, v[i]);
+}
+
+/* PinholeCamera.hpp:140-301 with PixelJitter.hpp:37-63 */
+BF_D bool robust_unit(v3f d, v3f& out) {
+    if (!fin3(d)) return false;
+    const float m = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+    if (m == 0.0f) return false;
+    const v3f s = mk(d.x / m, d.y / m, d.z / m);
+    const float mag = sqrtf(s.x * s.x + s.y * s.y + s.z * s.z);
+    out = mk(s.x / mag, s.y / mag, s.z / mag);
+    return fin3(out);
+}
+
+BF_D bool camera_ray(const bf_camera_desc& cam, const sampler_ctx& sc, const sample_key& k, v3f& origin, v3f& dir) {
+    float ox = 0.5f, oy = 0.5f;
+    if (cam.jitter_mode == BF_JITTER_UNIFORM) {
+        ox = sample_1d(sc, k, BF_DIM_RASTER_X);
+        oy = sample_1d(sc, k, BF_DIM_RASTER_Y);
+    }
+    if (k.px >= cam.width || k.py >= cam.height || !(ox >= 0.0f) || !(ox < 1.0f) || !(oy >= 0.0f) || !(oy < 1.0f))
+        return false;
+    const float w = (float)cam.width, h = (float)cam.height;
+    const float tx = 2.0f * (float)k.px;
+    const float ty = 2.0f * (float)k.py;
+    const float nx = (tx + 2.0f * ox) / w - 1.0f;
+    const float ny = 1.0f - (ty + 2.0f * oy) / h;
+    v3f unit;
+    if (!robust_unit(mk(nx * cam.half_width, ny * cam.half_height, -1.0f), unit)) return false;
+    const v3f wd = mk(cam.tangent[0] * unit.x + cam.bitangent[0] * unit.y + cam.normal[0] * unit.z,
+                      cam.tangent[1] * unit.x + cam.bitangent[1] * unit.y + cam.normal[1] * unit.z,
+                      cam.tangent[2] * unit.x + cam.bitangent[2] * unit.y + cam.normal[2] * unit.z);
+    if (!robust_unit(wd, dir)) return false;
+    origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+    return true;
+}
+
+/* ---- camera stage: device-side primary rays, wavelengths and initial state ------------------- */
+__global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint32_t count, uint32_t* ray_queue) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const sample_key k = key_for_slot(P, slot);
+    v3f o, d;
+    ray_queue[slot] = slot;
+    P.paths.status[slot] = BF_TERM_NONE;
+    P.paths.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+    P.paths.radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.paths.depth[slot] = make_uint4(0u, 0u, 0u, __float_as_uint(1.0f));
+    P.paths.previous[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    P.paths.visibility[slot] = P.camera.visibility_mask;
+    float wl[4];
+    if (P.wavelength_mode == BF_WAVELENGTHS_PER_PATH) {
+        const float u = sample_1d(P.sampler, k, 3ull);
+        if (!(u >= 0.0f) || !(u < 1.0f)) {
+            fail_path(P, slot, 2);
+            return;
+        }
+        /* WavelengthSampling.hpp:62-84 */
+#pragma unroll
+        for (int lane = 0; lane < 4; ++lane) {
+            const float offset = (float)lane / 4.0f;
+            const float wrap = 1.0f - offset;
+            const float shifted = lane != 0 && u >= wrap ? u - wrap : u + offset;
+            wl[lane] = fmaf(470.0f, shifted, 360.0f);
+        }
+    } else {
+#pragma unroll
+        for (int lane = 0; lane < 4; ++lane) wl[lane] = P.packet[lane];
+    }
+    P.paths.wavelengths[slot] = make_float4(wl[0], wl[1], wl[2], wl[3]);
+    if (!camera_ray(P.camera, P.sampler, k, o, d)) {
+        fail_path(P, slot, 1);
+        return;
+    }
+    P.paths.ray_origin[slot] = make_float4(o.x, o.y, o.z, P.camera.time);
+    P.paths.ray_direction[slot] = make_float4(d.x, d.y, d.z, P.camera.t_min);
+}
+
+/* ---- closest-hit stage: gather + traversal + classify + miss ---------------------------------- */
+template <bool COUNT>
+__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_trace_closest(bf_transport_params P, const uint32_t* queue,
+                                                                  const uint32_t* count_ptr, uint8_t* route,
+                                                                  unsigned long long* counters) {
+    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *count_ptr;
+    trav_counters ctr{0u, 0u, 0u};
+    if (i < n) {
+        const uint32_t slot = queue[i];
+        const float4 ro = P.paths.ray_origin[slot];
+        const float4 rd = P.paths.ray_direction[slot];
+        trav_ray ray;
+        ray.o = mk(ro.x, ro.y, ro.z);
+        ray.d = mk(rd.x, rd.y, rd.z);
+        ray.t_min = rd.w;
+        ray.t_max = (P.paths.depth[slot].y == 0u) ? P.camera.t_max : BF_INFINITY_F;
+        ray.mask = P.paths.visibility[slot];
+        trav_hit best;
+        bool occluded;
+        const uint32_t st = traverse<false, COUNT>(P.scene, ray, s_ref + threadIdx.x, s_near + threadIdx.x,
+                                                   BF_TRACE_BLOCK, best, occluded, ctr);
+        uint8_t r = 0;
+        if (st != BF_HIT_MISS) {
+            fail_path(P, slot, 60u + st);
+        } else if (best.present) {
+            P.paths.hit[slot] = make_float4(best.t, best.b0, best.b1, best.b2);
+            P.paths.hit_ids[slot] = make_uint2(best.scene_instance, best.primitive);
+            r = 1;
+        } else {
+            /* miss stage: CudaWavefrontStages.cu:3962-4040 */
+            if (P.scene.has_environment) {
+                const float4 wl = P.paths.wavelengths[slot];
+                const float4 beta = P.paths.beta[slot];
+                float4 L = P.paths.radiance[slot];
+                const float w4[4] = {wl.x, wl.y, wl.z, wl.w};
+                const float b4[4] = {beta.x, beta.y, beta.z, beta.w};
+                float l4[4] = {L.x, L.y, L.z, L.w};
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    float product;
+                    const float radiance = spectrum_at(P.scene, P.scene.environment_spectrum, w4[k]);
+                    if (!checked_product(b4[k], radiance, product)) ok = false;
+                    else {
+                        l4[k] += product;
+                        if (!bf_isfinite(l4[k]) || l4[k] < 0.0f) ok = false;
+                    }
+                }
+                if (!ok) {
+                    fail_path(P, slot, 50);
+                } else {
+                    P.paths.radiance[slot] = make_float4(l4[0], l4[1], l4[2], l4[3]);
+                    P.paths.status[slot] = BF_TERM_ESCAPED;
+                }
+            } else {
+                P.paths.status[slot] = BF_TERM_ESCAPED;
+            }
+        }
+        route[i] = r;
+    }
+    if (COUNT) {
+        /* wave-reduced traversal counters (nodes, triangles, instances) */
+        for (int off = 32; off > 0; off >>= 1) {
+            ctr.nodes += __shfl_down(ctr.nodes, off, 64);
+            ctr.triangles += __shfl_down(ctr.triangles, off, 64);
+            ctr.instances += __shfl_down(ctr.instances, off, 64);
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            atomicAdd(&counters[0], (unsigned long long)ctr.nodes);
+            atomicAdd(&counters[1], (unsigned long long)ctr.triangles);
+            atomicAdd(&counters[2], (unsigned long long)ctr.instances);
+        }
+    }
+}
+
+/* ---- shade stage: CudaWavefrontStages.cu:4107-4727 -------------------------------------------- */
+struct surface_t {
+    v3f position, error, ng, ns, dpdu;
+    uint32_t material;
+};
+
+/* surface_data (:1523-1665) plus the traversal's world normal (CudaSceneClosestHit.cu:826-871) */
+BF_D bool surface_data(const bf_dev_scene& S, uint32_t inst, uint32_t prim, float4 hit, v3f ro, v3f rd, surface_t& out) {
+    const bf_dev_instance& I = S.instances[inst];
+    const bf_dev_geometry G = S.geometries[I.geometry_index];
+    if (prim >= G.triangle_count) return false;
+    const size_t t = (size_t)G.triangle_offset + prim;
+    uint32_t idx[3] = {S.indices[3 * t], S.indices[3 * t + 1], S.indices[3 * t + 2]};
+    const v3f bary = mk(hit.y, hit.z, hit.w);
+    v3f lv[3], ln = mk(0.0f, 0.0f, 0.0f);
+    float tc[3][2];
+    for (int c = 0; c < 3; ++c) {
+        if (idx[c] >= G.vertex_count) return false;
+        const size_t gv = (size_t)G.vertex_offset + idx[c];
+        lv[c] = mk(S.positions[3 * gv], S.positions[3 * gv + 1], S.positions[3 * gv + 2]);
+        tc[c][0] = S.uvs[2 * gv];
+        tc[c][1] = S.uvs[2 * gv + 1];
+        const float w = c == 0 ? bary.x : (c == 1 ? bary.y : bary.z);
+        const v3f vn = mk(S.normals[3 * gv], S.normals[3 * gv + 1], S.normals[3 * gv + 2]);
+        if (!fin3(lv[c]) || !fin3(vn) || !bf_isfinite(tc[c][0]) || !bf_isfinite(tc[c][1])) return false;
+        ln = add3(ln, mul3(vn, w));
+    }
+    v3f ng;
+    if (!world_normal(I.local_to_world, lv[0], lv[1], lv[2], ng)) return false;
+    const float* inv = I.world_to_local;
+    const float* m = I.local_to_world;
+    const v3f tn = mk(fmaf(inv[0], ln.x, fmaf(inv[4], ln.y, inv[8] * ln.z)), fmaf(inv[1], ln.x, fmaf(inv[5], ln.y, inv[9] * ln.z)),
+                      fmaf(inv[2], ln.x, fmaf(inv[6], ln.y, inv[10] * ln.z)));
+    v3f ns;
+    if (!normalize3(tn, ns) || !normalize3(ng, ng)) return false;
+    if (dot3(ns, ng) < 0.0f) ns = mul3(ns, -1.0f);
+    if (!(dot3(ns, ng) > 0.0f)) return false;
+    v3f wv[3];
+    for (int c = 0; c < 3; ++c) {
+        wv[c] = xform_point(m, lv[c]);
+        if (!fin3(wv[c])) return false;
+    }
+    /* surface_derivatives (:1036-1060) */
+    const v3f e1 = sub3(wv[1], wv[0]), e2 = sub3(wv[2], wv[0]);
+    const float du1 = tc[1][0] - tc[0][0], dv1 = tc[1][1] - tc[0][1];
+    const float du2 = tc[2][0] - tc[0][0], dv2 = tc[2][1] - tc[0][1];
+    const float det = fmaf(du1, dv2, -dv1 * du2);
+    v3f dpdu = mk(0.0f, 0.0f, 0.0f);
+    if (!fin3(e1) || !fin3(e2) || !bf_isfinite(det)) return false;
+    if (det != 0.0f) {
+        const float rcp = 1.0f / det;
+        dpdu = mul3(sub3(mul3(e1, dv2), mul3(e2, dv1)), rcp);
+        const v3f dpdv = mul3(sub3(mul3(e2, du1), mul3(e1, du2)), rcp);
+        if (!bf_isfinite(rcp) || !fin3(dpdu) || !fin3(dpdv)) return false;
+    }
+    positioned_t obj, world;
+    const v3f traversal = mk(fmaf(hit.x, rd.x, ro.x), fmaf(hit.x, rd.y, ro.y), fmaf(hit.x, rd.z, ro.z));
+    if (!interpolate_position_with_error(lv, bary, obj) || !transform_position_with_error(m, obj, traversal, world))
+        return false;
+    out.position = world.position;
+    out.error = world.error;
+    out.ng = ng;
+    out.ns = ns;
+    out.dpdu = dpdu;
+    out.material = I.material_index;
+    return fin3(out.position) && fin3(out.error);
+}
+
+BF_D void load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const float* wl, mixture_t& mix) {
+    mix.active_count = M.closure_count;
+    for (uint32_t c = 0; c < M.closure_count; ++c) {
+        const bf_closure_desc D = S.closures[M.closure_offset + c];
+        closure_t& r = mix.closures[c];
+        r.kind = D.kind;
+        r.lobes = D.lobes;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) r.parameters[k] = D.parameters[k];
+#pragma unroll
+        for (int lane = 0; lane < 4; ++lane) {
+            r.weight[lane] = spectrum_at(S, D.weight_spectrum, wl[lane]);
+            if (D.kind == BF_CLOSURE_ROUGH_CONDUCTOR) {
+                r.parameters[lane] = spectrum_at(S, D.eta_spectrum, wl[lane]);
+                r.parameters[4 + lane] = spectrum_at(S, D.k_spectrum, wl[lane]);
+            }
+        }
+        mix.probabilities[c] = M.probabilities[c];
+    }
+#pragma unroll
+    for (uint32_t c = 0; c <= BF_MAX_CLOSURES; ++c) mix.cdf[c] = M.cdf[c];
+}
+
+struct incident_t {
+    float radiance[4];
+    v3f direction, position, error, normal;
+    float conditional, selection;
+    bool present;
+};
+
+/* sample_registered_light (:2378-2460) + sample_mesh_area_light (:2044-2201); the per-light area
+ * CDF of :2085-2147 is a scene constant evaluated by the host, searched here in O(log n). */
+BF_D bool sample_light(const bf_transport_params& P, const sample_key& k, uint64_t first_dim, const float* wl,
+                       uint32_t visibility, const surface_t& surf, incident_t& inc) {
+    const bf_dev_scene& S = P.scene;
+    inc.present = false;
+    const float u_sel = sample_1d(P.sampler, k, first_dim + 0);
+    uint32_t li;
+    float sel;
+    if (!uniform_light_selection(S.light_count, u_sel, li, sel)) return false;
+    const bf_dev_light L = S.lights[li];
+    const bf_dev_instance& I = S.instances[L.instance];
+    if ((I.visibility_mask & visibility) == 0u) return true;
+    const float area_sample = sample_1d(P.sampler, k, first_dim + 1);
+    const float triangle_sample = sample_1d(P.sampler, k, first_dim + 2);
+    if (!(area_sample >= 0.0f) || !(area_sample < 1.0f) || !(triangle_sample >= 0.0f) || !(triangle_sample < 1.0f))
+        return false;
+    /* first triangle whose cumulative bound exceeds the sample (bounds strictly increase) */
+    const float* next = S.light_next + L.table_offset;
+    uint32_t lo = 0, hi = L.triangle_count;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (area_sample < next[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    const uint32_t t = lo;
+    if (t >= L.triangle_count) return false;
+    const float prob = S.light_probabilities[L.table_offset + t];
+    const float area = S.light_areas[L.table_offset + t];
+    const float remapped = (area_sample - S.light_cumulative[L.table_offset + t]) / prob;
+    if (!bf_isfinite(remapped) || remapped < 0.0f || !(remapped < 1.0f)) return false;
+    const float root = sqrtf(remapped);
+    const float b0 = 1.0f - root, b1 = root * (1.0f - triangle_sample), b2 = root * triangle_sample;
+    const bf_dev_geometry G = S.geometries[L.geometry];
+    const size_t tri = (size_t)G.triangle_offset + t;
+    v3f lv[3], wv[3];
+    for (int c = 0; c < 3; ++c) {
+        const size_t gv = (size_t)G.vertex_offset + S.indices[3 * tri + c];
+        lv[c] = mk(S.positions[3 * gv], S.positions[3 * gv + 1], S.positions[3 * gv + 2]);
+        wv[c] = xform_point(I.local_to_world, lv[c]);
+    }
+    const float4 nrm = S.light_normals[L.table_offset + t];
+    const v3f n = mk(nrm.x, nrm.y, nrm.z);
+    const float density = prob / area;
+    if (!bf_isfinite(density) || !(density > 0.0f)) return false;
+    const bf_material_desc& M = S.materials[I.material_index];
+    float radiance[4];
+    bool black = true;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        radiance[c] = spectrum_at(S, M.emission_spectrum, wl[c]);
+        if (!bf_isfinite(radiance[c]) || radiance[c] < 0.0f) return false;
+        if (radiance[c] != 0.0f) black = false;
+    }
+    const v3f traversal = add3(add3(mul3(wv[0], b0), mul3(wv[1], b1)), mul3(wv[2], b2));
+    positioned_t obj, world;
+    if (!interpolate_position_with_error(lv, mk(b0, b1, b2), obj) ||
+        !transform_position_with_error(I.local_to_world, obj, traversal, world))
+        return false;
+    if (black) return true;
+    segment_t seg;
+    if (!scaled_segment(sub3(world.position, surf.position), seg)) return false;
+    float absolute;
+    bool supported;
+    if (!light_surface_alignment(n, seg.scaled, absolute, supported)) return false;
+    if (!supported) return true;
+    const float num[4] = {density, seg.scale, seg.scale, seg.cubed};
+    const float den[1] = {absolute};
+    float cond;
+    if (checked_product_quotient(num, den, cond) != BF_L_OK || !(cond > 0.0f)) return false;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) inc.radiance[c] = radiance[c];
+    inc.direction = seg.direction;
+    inc.position = world.position;
+    inc.error = world.error;
+    inc.normal = n;
+    inc.conditional = cond;
+    inc.selection = sel;
+    inc.present = true;
+    return true;
+}
+
+/* emissive_hit_mis_weight (:2513-2594) */
+BF_D bool emission_weight(const bf_transport_params& P, uint32_t inst, float4 prev, const surface_t& surf, float& w) {
+    w = 1.0f;
+    if (!(prev.w > 0.0f)) return true;
+    const bf_dev_scene& S = P.scene;
+    const int32_t li = S.instances[inst].light_index;
+    if (li < 0) return false;
+    float sel;
+    if (!uniform_light_probability(S.light_count, (uint32_t)li, sel)) return false;
+    const float inv_area = S.lights[li].inverse_area;
+    segment_t seg;
+    if (!scaled_segment(sub3(surf.position, mk(prev.x, prev.y, prev.z)), seg)) return false;
+    float absolute;
+    bool supported;
+    if (!light_surface_alignment(surf.ng, seg.scaled, absolute, supported) || !supported) return false;
+    const float num[4] = {inv_area, seg.scale, seg.scale, seg.cubed};
+    const float den[1] = {absolute};
+    float cond, joint;
+    if (checked_product_quotient(num, den, cond) != BF_L_OK) return false;
+    if (!joint_light_pdf(sel, cond, joint)) return false;
+    return mis_weight(P.mis, prev.w, joint, w);
+}
+
+BF_D bool zero4(const float* v) { return v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f && v[3] == 0.0f; }
+
+/* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation */
+__global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const uint32_t* queue, const uint32_t* count_ptr,
+                                                    uint8_t* route, uint32_t* sample_counts) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count_ptr) return;
+    const uint32_t slot = queue[i];
+    route[i] = 0;
+    const bf_dev_scene& S = P.scene;
+    const sample_key key = key_for_slot(P, slot);
+    const float4 ro4 = P.paths.ray_origin[slot];
+    const float4 rd4 = P.paths.ray_direction[slot];
+    const v3f ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
+    const float4 hit = P.paths.hit[slot];
+    const uint2 ids = P.paths.hit_ids[slot];
+    const float4 wl4 = P.paths.wavelengths[slot];
+    const float wl[4] = {wl4.x, wl4.y, wl4.z, wl4.w};
+    float4 beta4 = P.paths.beta[slot];
+    float beta[4] = {beta4.x, beta4.y, beta4.z, beta4.w};
+    const float4 L4 = P.paths.radiance[slot];
+    float L[4] = {L4.x, L4.y, L4.z, L4.w};
+    uint4 dstate = P.paths.depth[slot];
+    uint32_t diffuse = dstate.x & 0xFFu, glossy = (dstate.x >> 8) & 0xFFu, specular = (dstate.x >> 16) & 0xFFu,
+             transmission = (dstate.x >> 24) & 0xFFu;
+    uint32_t depth = dstate.y, delta_flags = dstate.z;
+    float eta_scale = __uint_as_float(dstate.w);
+    const float4 prev = P.paths.previous[slot];
+    const uint32_t visibility = P.paths.visibility[slot];
+
+    surface_t surf;
+    if (!surface_data(S, ids.x, ids.y, hit, ro, rd, surf)) return fail_path(P, slot, 10);
+    v3f wo;
+    if (!normalize3(mk(-rd.x, -rd.y, -rd.z), wo)) return fail_path(P, slot, 11);
+    bool front = false;
+    if (!one_sided_emission_support(surf.ng, wo, front)) return fail_path(P, slot, 12);
+    const bf_material_desc& M = S.materials[surf.material];
+    float emitted[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        emitted[c] = spectrum_at(S, M.emission_spectrum, wl[c]);
+        if (!bf_isfinite(emitted[c]) || emitted[c] < 0.0f) return fail_path(P, slot, 13);
+    }
+    mixture_t mix;
+    load_mixture(S, M, wl, mix);
+    frame3 frame, unrotated;
+    const bool created = M.frame_mode == 0u ? make_local_frame(surf.ns, unrotated) : frame_from_tangent(surf.ns, surf.dpdu, unrotated);
+    if (!created || !rotate_frame(unrotated, M.tangent_rotation, frame)) return fail_path(P, slot, 14);
+    if (front && !zero4(emitted)) {
+        float weight;
+        if (!emission_weight(P, ids.x, prev, surf, weight)) return fail_path(P, slot, 15);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (beta[c] == 0.0f || emitted[c] == 0.0f || weight == 0.0f) continue;
+            const float num[3] = {beta[c], emitted[c], weight};
+            const float den[1] = {1.0f};
+            float contribution;
+            if (checked_product_quotient(num, den, contribution) != BF_L_OK) return fail_path(P, slot, 16);
+            L[c] += contribution;
+            if (!bf_isfinite(L[c])) return fail_path(P, slot, 16);
+        }
+        P.paths.radiance[slot] = make_float4(L[0], L[1], L[2], L[3]);
+    }
+    if (zero4(beta)) {
+        P.paths.status[slot] = BF_TERM_ZERO_THROUGHPUT;
+        return;
+    }
+    const uint32_t limits[5] = {P.limits[0], P.limits[1], P.limits[2], P.limits[3], P.limits[4]};
+    const uint32_t counters[5] = {diffuse, glossy, specular, transmission, 0u};
+    if (diffuse > limits[0] || glossy > limits[1] || specular > limits[2] || transmission > limits[3])
+        return fail_path(P, slot, 17);
+    filter_t filtered;
+    filter_by_depth(mix, limits, counters, depth, filtered);
+    if (filtered.status != BF_L_OK) return fail_path(P, slot, 18);
+    if (mix.active_count == 0 && filtered.source_count != 0) {
+        P.paths.status[slot] = BF_TERM_DEPTH_LIMIT;
+        return;
+    }
+    const v3f wo_local = to_local(frame, wo);
+    const uint64_t first_dim = 4ull + (uint64_t)depth * 10ull;
+    const float bsdf_component = sample_1d(P.sampler, key, first_dim + 3);
+    const float bsdf_u = sample_1d(P.sampler, key, first_dim + 4);
+    const float bsdf_v = sample_1d(P.sampler, key, first_dim + 5);
+    atomicAdd(&sample_counts[0], 1u);
+    mixture_sample_t bs;
+    sample_mixture(mix, filtered, wo_local, bsdf_component, bsdf_u, bsdf_v, bs);
+    const bool has_continuation = bs.status == BF_L_OK;
+    if (!has_continuation && bs.status != BF_L_OUTSIDE) return fail_path(P, slot, 19);
+    if (has_continuation) {
+        const bool delta = valid_surface_event(bs.lobes) && (bs.lobes & BF_LOBE_SPECULAR);
+        if (!valid_surface_event(bs.lobes) || (bs.discrete != 0u) != delta || !bf_isfinite(bs.pdf) || !(bs.pdf > 0.0f) ||
+            (delta && bs.pdf > 1.0f))
+            return fail_path(P, slot, 20);
+    }
+    incident_t inc;
+    inc.present = false;
+    if (S.light_count != 0u && mixture_nonblack(mix)) {
+        atomicAdd(&sample_counts[1], 1u);
+        if (!sample_light(P, key, first_dim, wl, visibility, surf, inc)) return fail_path(P, slot, 21);
+    }
+    bool has_shadow = false;
+    float4 shadow_o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), shadow_d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float contribution[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (inc.present) {
+        const v3f wi_local = to_local(frame, inc.direction);
+        if (wo_local.z != 0.0f && wi_local.z != 0.0f) {
+            const bool singleton_lambert = mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN;
+            float bsdf[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (singleton_lambert) {
+                if (wo_local.z > 0.0f && wi_local.z > 0.0f)
+                    for (int c = 0; c < 4; ++c) bsdf[c] = mix.closures[0].weight[c];
+            } else if (mixture_eval(mix, filtered, wo_local, wi_local, bsdf) != BF_L_OK) {
+                return fail_path(P, slot, 22);
+            }
+            if (!zero4(bsdf)) {
+                float corr;
+                if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, corr)) return fail_path(P, slot, 23);
+                if (corr > 0.0f) {
+                    float joint, bsdf_pdf, w;
+                    if (!joint_light_pdf(inc.selection, inc.conditional, joint)) return fail_path(P, slot, 24);
+                    if (mixture_pdf(mix, filtered, wo_local, wi_local, bsdf_pdf) != BF_L_OK) return fail_path(P, slot, 24);
+                    if (!mis_weight(P.mis, joint, bsdf_pdf, w)) return fail_path(P, slot, 24);
+                    /* the visible-shadow contribution of process_shadow_kernel (:4772-4908),
+                     * evaluated now from the same frozen factors */
+                    const float coefficient = singleton_lambert ? BF_INV_PI : 1.0f;
+                    const float cosine = fabsf(wi_local.z);
+                    for (int c = 0; c < 4; ++c) {
+                        const float den[2] = {inc.selection, inc.conditional};
+                        int st;
+                        if (w == 1.0f) {
+                            const float num[6] = {beta[c], bsdf[c], coefficient, inc.radiance[c], cosine, corr};
+                            st = checked_product_quotient(num, den, contribution[c]);
+                        } else {
+                            const float num[7] = {beta[c], bsdf[c], coefficient, inc.radiance[c], cosine, corr, w};
+                            st = checked_product_quotient(num, den, contribution[c]);
+                        }
+                        if (st != BF_L_OK) return fail_path(P, slot, 41);
+                    }
+                    /* prepare_shadow_ray (:4042-4105) */
+                    v3f src, endpoint;
+                    segment_t seg;
+                    if (!offset_point(surf.position, surf.error, surf.ng, inc.direction, src) ||
+                        !offset_point(inc.position, inc.error, inc.normal, mul3(inc.direction, -1.0f), endpoint) ||
+                        !scaled_segment(sub3(endpoint, src), seg) || !(dot3(seg.direction, inc.direction) > 0.0f))
+                        return fail_path(P, slot, 25);
+                    const float tmax = bf_next_toward_zero_f(seg.length);
+                    if (!(tmax > 0.0f)) return fail_path(P, slot, 25);
+                    shadow_o = make_float4(src.x, src.y, src.z, tmax);
+                    shadow_d = make_float4(seg.direction.x, seg.direction.y, seg.direction.z, 0.0f);
+                    has_shadow = true;
+                }
+            }
+        }
+    }
+    v3f wi = mk(0.0f, 0.0f, 0.0f);
+    bool continuation = has_continuation;
+    float corr = 0.0f;
+    if (continuation) {
+        if (!to_world(frame, bs.incoming, wi)) return fail_path(P, slot, 26);
+        bool geo = false;
+        if (!shading_correction(surf.ng, surf.ns, wo, wi, corr) || !geometric_event_support(surf.ng, wo, wi, bs.lobes, geo))
+            return fail_path(P, slot, 27);
+        continuation = corr > 0.0f && geo;
+    }
+    uint32_t pending_term = BF_TERM_OUTSIDE_SUPPORT;
+    bool cont_pending = false;
+    if (continuation) {
+        /* update_closure_throughput (:3894-3938) */
+        float nb[4];
+        const float cosine = fabsf(bs.incoming.z);
+        if (!bf_isfinite(cosine) || !(cosine > 0.0f)) return fail_path(P, slot, 28);
+        if (mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN && corr == 1.0f) {
+            for (int c = 0; c < 4; ++c)
+                if (!checked_product(beta[c], mix.closures[0].weight[c], nb[c])) return fail_path(P, slot, 28);
+        } else {
+            for (int c = 0; c < 4; ++c) {
+                const float num[4] = {beta[c], bs.value[c], cosine, corr};
+                const float den[1] = {bs.pdf};
+                if (checked_product_quotient(num, den, nb[c]) != BF_L_OK) return fail_path(P, slot, 28);
+            }
+        }
+        const float next_eta = eta_scale * bs.eta_scale;
+        if (!bf_isfinite(bs.eta_scale) || !(bs.eta_scale > 0.0f) || !bf_isfinite(next_eta) || !(next_eta > 0.0f))
+            return fail_path(P, slot, 29);
+        v3f origin;
+        if (!offset_point(surf.position, surf.error, surf.ng, wi, origin)) return fail_path(P, slot, 30);
+        const bool delta_event = (bs.lobes & BF_LOBE_SPECULAR) != 0u;
+        delta_flags = ((delta_flags & 2u) ? 2u : 0u) | (delta_event ? 1u : 2u);
+        /* advance_depth_event (:3851-3880) */
+        if (bs.lobes & BF_LOBE_DIFFUSE) ++diffuse;
+        else if (bs.lobes & BF_LOBE_GLOSSY) ++glossy;
+        else ++specular;
+        if (bs.lobes & BF_LOBE_TRANSMISSION) ++transmission;
+        ++depth;
+        eta_scale = next_eta;
+        beta4 = make_float4(nb[0], nb[1], nb[2], nb[3]);
+        P.paths.previous[slot] = delta_event ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
+                                             : make_float4(surf.position.x, surf.position.y, surf.position.z, bs.pdf);
+        P.paths.ray_origin[slot] = make_float4(origin.x, origin.y, origin.z, ro4.w);
+        P.paths.ray_direction[slot] = make_float4(wi.x, wi.y, wi.z, 0.0f);
+        cont_pending = true;
+        pending_term = BF_TERM_NONE;
+        if (nb[0] == 0.0f && nb[1] == 0.0f && nb[2] == 0.0f && nb[3] == 0.0f) {
+            cont_pending = false;
+            pending_term = BF_TERM_ZERO_THROUGHPUT;
+        } else if (P.rr_enabled && depth >= P.rr_first) {
+            const float u = sample_1d(P.sampler, key, first_dim + 9);
+            bool survived;
+            if (!russian_roulette(beta4, eta_scale, u, P.rr_min, P.rr_max, survived)) return fail_path(P, slot, 32);
+            if (!survived) {
+                cont_pending = false;
+                pending_term = BF_TERM_RUSSIAN_ROULETTE;
+            }
+        }
+        P.paths.beta[slot] = beta4;
+        P.paths.depth[slot] = make_uint4(diffuse | (glossy << 8) | (specular << 16) | (transmission << 24), depth,
+                                         delta_flags, __float_as_uint(eta_scale));
+    }
+    if (has_shadow) {
+        shadow_d.w = __uint_as_float((cont_pending ? 1u : 0u) | (pending_term << 8));
+        P.paths.shadow_origin[slot] = shadow_o;
+        P.paths.shadow_direction[slot] = shadow_d;
+        P.paths.shadow_contribution[slot] = make_float4(contribution[0], contribution[1], contribution[2], contribution[3]);
+        route[i] = 1;
+        return;
+    }
+    if (!cont_pending) {
+        P.paths.status[slot] = pending_term;
+        return;
+    }
+    route[i] = 2;
+}
+
+/* ---- shadow stage: gather + any-hit + process_shadow (:4729-4908) ---------------------------- */
+template <bool COUNT>
+__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
+                                                                 const uint32_t* count_ptr, uint8_t* route,
+                                                                 unsigned long long* counters) {
+    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *count_ptr;
+    trav_counters ctr{0u, 0u, 0u};
+    if (i < n) {
+        const uint32_t slot = queue[i];
+        const float4 so = P.paths.shadow_origin[slot];
+        const float4 sd = P.paths.shadow_direction[slot];
+        trav_ray ray;
+        ray.o = mk(so.x, so.y, so.z);
+        ray.d = mk(sd.x, sd.y, sd.z);
+        ray.t_min = 0.0f;
+        ray.t_max = so.w;
+        ray.mask = P.paths.visibility[slot];
+        trav_hit best;
+        bool occluded;
+        const uint32_t st = traverse<true, COUNT>(P.scene, ray, s_ref + threadIdx.x, s_near + threadIdx.x, BF_TRACE_BLOCK,
+                                                  best, occluded, ctr);
+        const uint32_t packed = __float_as_uint(sd.w);
+        uint8_t r = 0;
+        if (st != BF_HIT_MISS) {
+            fail_path(P, slot, 40);
+        } else {
+            bool ok = true;
+            if (!occluded) {
+                const float4 c = P.paths.shadow_contribution[slot];
+                float4 L = P.paths.radiance[slot];
+                L.x += c.x;
+                L.y += c.y;
+                L.z += c.z;
+                L.w += c.w;
+                if (!bf_isfinite(L.x) || !bf_isfinite(L.y) || !bf_isfinite(L.z) || !bf_isfinite(L.w) || L.x < 0.0f ||
+                    L.y < 0.0f || L.z < 0.0f || L.w < 0.0f) {
+                    ok = false;
+                    fail_path(P, slot, 41);
+                } else {
+                    P.paths.radiance[slot] = L;
+                }
+            }
+            if (ok) {
+                if (packed & 1u) r = 2;
+                else P.paths.status[slot] = packed >> 8;
+            }
+        }
+        route[i] = r;
+    }
+    if (COUNT) {
+        for (int off = 32; off > 0; off >>= 1) {
+            ctr.nodes += __shfl_down(ctr.nodes, off, 64);
+            ctr.triangles += __shfl_down(ctr.triangles, off, 64);
+            ctr.instances += __shfl_down(ctr.instances, off, 64);
+        }
+        if ((threadIdx.x & 63u) == 0u) {
+            atomicAdd(&counters[3], (unsigned long long)ctr.nodes);
+            atomicAdd(&counters[4], (unsigned long long)ctr.triangles);
+            atomicAdd(&counters[5], (unsigned long long)ctr.instances);
+        }
+    }
+}
+
+/* ---- stable compaction: wave64 ballot/popcount ranks + block-offset scan ---------------------
+ * For route values r = 1, 2: dst_r[base_r + exclusive_prefix(route == r)] = src[i], which is the
+ * stable exclusive-scan append of CudaWavefrontQueueCompaction.cu:55-315. */
+__global__ __launch_bounds__(BF_BLOCK) void k_compact_count(const uint8_t* route, const uint32_t* count_ptr,
+                                                            uint32_t* block_counts) {
+    __shared__ uint32_t wave_counts[2][BF_BLOCK / 64];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *count_ptr;
+    const uint8_t r = i < n ? route[i] : 0;
+    const uint64_t m1 = __ballot(r == 1);
+    const uint64_t m2 = __ballot(r == 2);
+    const uint32_t wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63u) == 0u) {
+        wave_counts[0][wave] = (uint32_t)__popcll(m1);
+        wave_counts[1][wave] = (uint32_t)__popcll(m2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2u) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < BF_BLOCK / 64; ++w) total += wave_counts[threadIdx.x][w];
+        block_counts[2 * blockIdx.x + threadIdx.x] = total;
+    }
+}
+
+/* single workgroup: exclusive scan of per-block counts for both routes, plus the queue bases */
+__global__ __launch_bounds__(1024) void k_compact_scan(const uint32_t* block_counts, uint32_t block_count,
+                                                       uint32_t* block_offsets, uint32_t* size1, uint32_t* size2,
+                                                       uint32_t append1, uint32_t append2) {
+    __shared__ uint32_t partial[2][1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (block_count + 1023u) / 1024u;
+    const uint32_t begin = t * per;
+    const uint32_t end = begin + per < block_count ? begin + per : block_count;
+    uint32_t s1 = 0, s2 = 0;
+    for (uint32_t b = begin; b < end; ++b) {
+        s1 += block_counts[2 * b];
+        s2 += block_counts[2 * b + 1];
+    }
+    partial[0][t] = s1;
+    partial[1][t] = s2;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t a = t >= off ? partial[0][t - off] : 0u;
+        const uint32_t c = t >= off ? partial[1][t - off] : 0u;
+        __syncthreads();
+        partial[0][t] += a;
+        partial[1][t] += c;
+        __syncthreads();
+    }
+    const uint32_t base1 = append1 ? *size1 : 0u;
+    const uint32_t base2 = append2 ? *size2 : 0u;
+    uint32_t run1 = base1 + partial[0][t] - s1;
+    uint32_t run2 = base2 + partial[1][t] - s2;
+    for (uint32_t b = begin; b < end; ++b) {
+        block_offsets[2 * b] = run1;
+        block_offsets[2 * b + 1] = run2;
+        run1 += block_counts[2 * b];
+        run2 += block_counts[2 * b + 1];
+    }
+    __syncthreads();
+    if (t == 1023u) {
+        *size1 = base1 + partial[0][1023];
+        *size2 = base2 + partial[1][1023];
+    }
+}
+
+__global__ __launch_bounds__(BF_BLOCK) void k_compact_scatter(const uint8_t* route, const uint32_t* count_ptr,
+                                                              const uint32_t* src, const uint32_t* block_offsets,
+                                                              uint32_t* dst1, uint32_t* dst2) {
+    __shared__ uint32_t wave_counts[2][BF_BLOCK / 64];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = *count_ptr;
+    const uint8_t r = i < n ? route[i] : 0;
+    const uint64_t m1 = __ballot(r == 1);
+    const uint64_t m2 = __ballot(r == 2);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    if (lane == 0u) {
+        wave_counts[0][wave] = (uint32_t)__popcll(m1);
+        wave_counts[1][wave] = (uint32_t)__popcll(m2);
+    }
+    __syncthreads();
+    if (r == 0) return;
+    const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+    const uint32_t k = r - 1u;
+    uint32_t rank = (uint32_t)__popcll((k == 0 ? m1 : m2) & below);
+    for (uint32_t w = 0; w < wave; ++w) rank += wave_counts[k][w];
+    const uint32_t position = block_offsets[2 * blockIdx.x + k] + rank;
+    (k == 0 ? dst1 : dst2)[position] = src[i];
+}
+
+/* ---- film: per-pixel accumulation in sample order (Film.hpp:150-248, Cie1931Sensor.cpp) ----- */
+__constant__ double c_cie_table_rows[471][3] = {
+#include "data/cie1931_cmf.inc"
+};
+
+BF_D void cie_xyz(const float* radiance, const float* wavelengths, float pdf, float normalization, float* xyz) {
+    float sum[3] = {0.0f, 0.0f, 0.0f};
+    for (int lane = 0; lane < 4; ++lane) {
+        /* color_matching_functions (Cie1931Sensor.cpp:69-93) */
+        const float coordinate = wavelengths[lane] - 360.0f;
+        const uint32_t lower_index = (uint32_t)coordinate;
+        float response[3];
+        if (lower_index + 1u >= 471u) {
+            for (int c = 0; c < 3; ++c) response[c] = (float)c_cie_table_rows[470][c];
+        } else {
+            const float fraction = coordinate - (float)lower_index;
+            for (int c = 0; c < 3; ++c) {
+                const float lower = (float)c_cie_table_rows[lower_index][c];
+                const float upper = (float)c_cie_table_rows[lower_index + 1][c];
+                response[c] = fmaf(upper - lower, fraction, lower);
+            }
+        }
+        for (int c = 0; c < 3; ++c) {
+            /* scaled_sensor_term (Cie1931Sensor.cpp:104-124) */
+            float term = 0.0f;
+            if (radiance[lane] != 0.0f && response[c] != 0.0f) {
+                int ce, re, ne, pe;
+                const float cm = bf_frexpf(radiance[lane], &ce);
+                const float rm = bf_frexpf(response[c], &re);
+                const float nm = bf_frexpf(normalization, &ne);
+                const float pm = bf_frexpf(pdf, &pe);
+                const float mantissa = cm * rm * nm / pm;
+                term = bf_ldexpf(mantissa, ce + re + ne - pe);
+            }
+            sum[c] += term;
+        }
+    }
+    xyz[0] = sum[0];
+    xyz[1] = sum[1];
+    xyz[2] = sum[2];
+}
+
+__global__ __launch_bounds__(BF_BLOCK) void k_film(bf_dev_paths paths, uint32_t pixels, uint32_t passes, float pdf,
+                                                   float normalization, float* film, double* film_compensated,
+                                                   uint32_t* film_counts) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float f[3] = {film[3 * (size_t)p], film[3 * (size_t)p + 1], film[3 * (size_t)p + 2]};
+    double s[3], cmp[3];
+    for (int c = 0; c < 3; ++c) {
+        s[c] = film_compensated[6 * (size_t)p + 2 * c];
+        cmp[c] = film_compensated[6 * (size_t)p + 2 * c + 1];
+    }
+    uint32_t count = film_counts[p];
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const size_t slot = (size_t)pass * pixels + p;
+        const float4 L4 = paths.radiance[slot];
+        const float4 W4 = paths.wavelengths[slot];
+        const float L[4] = {L4.x, L4.y, L4.z, L4.w};
+        const float W[4] = {W4.x, W4.y, W4.z, W4.w};
+        float xyz[3];
+        cie_xyz(L, W, pdf, normalization, xyz);
+        /* Color.hpp:77-87 */
+        const float rgb[3] = {3.2404542f * xyz[0] - 1.5371385f * xyz[1] - 0.4985314f * xyz[2],
+                              -0.9692660f * xyz[0] + 1.8760108f * xyz[1] + 0.0415560f * xyz[2],
+                              0.0556434f * xyz[0] - 0.2040259f * xyz[1] + 1.0572252f * xyz[2]};
+        for (int c = 0; c < 3; ++c) {
+            f[c] += rgb[c] * 1.0f;
+            const double value = (double)rgb[c] * 1.0;
+            const double updated = s[c] + value;
+            if (fabs(s[c]) >= fabs(value)) cmp[c] += (s[c] - updated) + value;
+            else cmp[c] += (value - updated) + s[c];
+            s[c] = updated;
+        }
+        ++count;
+    }
+    for (int c = 0; c < 3; ++c) {
+        film[3 * (size_t)p + c] = f[c];
+        film_compensated[6 * (size_t)p + 2 * c] = s[c];
+        film_compensated[6 * (size_t)p + 2 * c + 1] = cmp[c];
+    }
+    film_counts[p] = count;
+}
+
+/* FNV-1a-64 of a queue's slot bytes (diagnostic; one thread) */
+__global__ void k_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, unsigned long long* out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t h = 14695981039346656037ull;
+    const uint32_t n = *count_ptr;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t v = queue[i];
+        for (int b = 0; b < 4; ++b) {
+            h ^= (v >> (8 * b)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    }
+    out[0] = h;
+    out[1] = n;
+}
+
+/* ---- stand-alone traversal (blackframe_hip_launch_scene_closest_hit / _occlusion) ----------- */
+__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_scene S, const bf_transport_ray* rays,
+                                                                      uint32_t count, bf_closest_hit_result* results,
+                                                                      uint32_t* counters) {
+    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const bf_transport_ray r = rays[i];
+    bf_closest_hit_result out;
+    memset(&out, 0, sizeof out);
+    trav_counters ctr{0u, 0u, 0u};
+    if (!valid_traversal_ray(r)) {
+        out.status = BF_HIT_INVALID_RAY;
+    } else {
+        trav_ray ray;
+        ray.o = mk(r.origin_x, r.origin_y, r.origin_z);
+        ray.d = mk(r.direction_x, r.direction_y, r.direction_z);
+        ray.t_min = r.t_min;
+        ray.t_max = r.t_max;
+        ray.mask = r.visibility_mask;
+        trav_hit best;
+        bool occ;
+        const uint32_t st = traverse<false, true>(S, ray, s_ref + threadIdx.x, s_near + threadIdx.x, BF_TRACE_BLOCK, best,
+                                                  occ, ctr);
+        if (st != BF_HIT_MISS) {
+            out.status = st;
+        } else if (best.present) {
+            const bf_dev_instance& I = S.instances[best.scene_instance];
+            const float4 q0 = S.leaf_triangles[3 * (size_t)best.leaf + 0];
+            const float4 q1 = S.leaf_triangles[3 * (size_t)best.leaf + 1];
+            const float4 q2 = S.leaf_triangles[3 * (size_t)best.leaf + 2];
+            v3f n;
+            if (!world_normal(I.local_to_world, mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), n)) {
+                out.status = BF_HIT_NUMERICAL_FAILURE;
+            } else {
+                out.status = BF_HIT_HIT;
+                out.hit.parameter = best.t;
+                out.hit.barycentric0 = best.b0;
+                out.hit.barycentric1 = best.b1;
+                out.hit.barycentric2 = best.b2;
+                out.hit.normal_x = n.x;
+                out.hit.normal_y = n.y;
+                out.hit.normal_z = n.z;
+                out.hit.ids.object = I.object_id;
+                out.hit.ids.instance = I.id;
+                out.hit.ids.geometry = S.geometries[I.geometry_index].id;
+                out.hit.ids.primitive = best.primitive;
+                out.hit.ids.material = S.materials[I.material_index].id;
+            }
+        }
+    }
+    results[i] = out;
+    if (counters) {
+        counters[3 * (size_t)i] = ctr.nodes;
+        counters[3 * (size_t)i + 1] = ctr.triangles;
+        counters[3 * (size_t)i + 2] = ctr.instances;
+    }
+}
+
+__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_occlusion(bf_dev_scene S, const bf_transport_ray* rays,
+                                                                    uint32_t count, bf_occlusion_result* results) {
+    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const bf_transport_ray r = rays[i];
+    bf_occlusion_result out;
+    memset(&out, 0, sizeof out);
+    if (!valid_traversal_ray(r)) {
+        out.status = BF_HIT_INVALID_RAY;
+    } else {
+        trav_ray ray;
+        ray.o = mk(r.origin_x, r.origin_y, r.origin_z);
+        ray.d = mk(r.direction_x, r.direction_y, r.direction_z);
+        ray.t_min = r.t_min;
+        ray.t_max = r.t_max;
+        ray.mask = r.visibility_mask;
+        trav_hit best;
+        bool occ;
+        trav_counters ctr{0u, 0u, 0u};
+        const uint32_t st = traverse<true, false>(S, ray, s_ref + threadIdx.x, s_near + threadIdx.x, BF_TRACE_BLOCK, best,
+                                                  occ, ctr);
+        out.status = st != BF_HIT_MISS ? st : (occ ? BF_OCCLUDED : BF_VISIBLE);
+    }
+    results[i] = out;
+}
+
+/* ---- sampler probe (SampleStreamKernel.hpp:170-176 counterpart) ------------------------------ */
+__global__ void k_sample_dump(sampler_ctx sc, const bf_sample_stream_index* indices, const uint64_t* dims, uint32_t count,
+                              float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    sample_key k;
+    k.px = indices[i].pixel_x;
+    k.py = indices[i].pixel_y;
+    k.s = indices[i].sample_index;
+    k.seed = indices[i].seed;
+    out[i] = sample_1d(sc, k, dims[i]);
+}
+
+/* ---- small utility kernels (used instead of hipMemsetAsync so the sequence stays capturable) - */
+__global__ void k_set_words(uint32_t* dst, uint32_t count, uint32_t value) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = value;
+}
+
+__global__ void k_copy_word(const uint32_t* src, uint32_t* dst) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = src[0];
+}
+
+__global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, uint32_t count, bf_path_result* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    bf_path_result r;
+    const float4 L = paths.radiance[i], B = paths.beta[i], W = paths.wavelengths[i];
+    const uint4 D = paths.depth[i];
+    r.radiance[0] = L.x;
+    r.radiance[1] = L.y;
+    r.radiance[2] = L.z;
+    r.radiance[3] = L.w;
+    r.beta[0] = B.x;
+    r.beta[1] = B.y;
+    r.beta[2] = B.z;
+    r.beta[3] = B.w;
+    r.wavelengths[0] = W.x;
+    r.wavelengths[1] = W.y;
+    r.wavelengths[2] = W.z;
+    r.wavelengths[3] = W.w;
+    r.depth = D.y;
+    r.termination = paths.status[i];
+    r.depth_counters = D.x;
+    r.eta_scale = __uint_as_float(D.w);
+    out[i] = r;
+}
+
+/* ---- host-side launch helpers (stream-ordered, never synchronising) -------------------------- */
+static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
+
+extern "C" {
+
+hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_camera, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *P, count, ray_queue);
+    return hipGetLastError();
+}
+
+hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
+                             uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s) {
+    if (max_count == 0) return hipSuccess;
+    if (counters)
+        hipLaunchKernelGGL(k_trace_closest<true>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
+                           *P, queue, count_ptr, route, counters);
+    else
+        hipLaunchKernelGGL(k_trace_closest<false>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
+                           *P, queue, count_ptr, route, counters);
+    return hipGetLastError();
+}
+
+hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
+                     uint8_t* route, uint32_t* sample_counts, hipStream_t s) {
+    if (max_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade, dim3(blocks_for(max_count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *P, queue, count_ptr, route,
+                       sample_counts);
+    return hipGetLastError();
+}
+
+hipError_t bfk_trace_shadow(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
+                            uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s) {
+    if (max_count == 0) return hipSuccess;
+    if (counters)
+        hipLaunchKernelGGL(k_trace_shadow<true>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
+                           *P, queue, count_ptr, route, counters);
+    else
+        hipLaunchKernelGGL(k_trace_shadow<false>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
+                           *P, queue, count_ptr, route, counters);
+    return hipGetLastError();
+}
+
+uint32_t bfk_compact_blocks(uint32_t max_count) { return blocks_for(max_count ? max_count : 1u, BF_BLOCK); }
+
+hipError_t bfk_compact(const uint8_t* route, const uint32_t* count_ptr, uint32_t max_count, const uint32_t* src,
+                       uint32_t* block_counts, uint32_t* block_offsets, uint32_t* dst1, uint32_t* size1, uint32_t append1,
+                       uint32_t* dst2, uint32_t* size2, uint32_t append2, hipStream_t s) {
+    const uint32_t blocks = bfk_compact_blocks(max_count);
+    hipLaunchKernelGGL(k_compact_count, dim3(blocks), dim3(BF_BLOCK), 0, s, route, count_ptr, block_counts);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, s, block_counts, blocks, block_offsets, size1, size2,
+                       append1, append2);
+    hipLaunchKernelGGL(k_compact_scatter, dim3(blocks), dim3(BF_BLOCK), 0, s, route, count_ptr, src, block_offsets, dst1,
+                       dst2);
+    return hipGetLastError();
+}
+
+hipError_t bfk_film(const bf_dev_paths* paths, uint32_t pixels, uint32_t passes, float pdf, float normalization,
+                    float* film, double* film_compensated, uint32_t* film_counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_film, dim3(blocks_for(pixels, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, pixels, passes, pdf,
+                       normalization, film, film_compensated, film_counts);
+    return hipGetLastError();
+}
+
+hipError_t bfk_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, unsigned long long* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_queue_hash, dim3(1), dim3(64), 0, s, queue, count_ptr, out);
+    return hipGetLastError();
+}
+
+hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
+                                 bf_closest_hit_result* results, uint32_t* counters, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scene_closest_hit, dim3(blocks_for(count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s, *S, rays,
+                       count, results, counters);
+    return hipGetLastError();
+}
+
+hipError_t bfk_scene_occlusion(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
+                               bf_occlusion_result* results, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scene_occlusion, dim3(blocks_for(count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s, *S, rays,
+                       count, results);
+    return hipGetLastError();
+}
+
+hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t dims, const bf_sample_stream_index* indices,
+                           const uint64_t* dimensions, uint32_t count, float* out, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    sampler_ctx sc;
+    sc.kind = kind;
+    sc.sobol_dims = dims;
+    sc.directions = directions;
+    hipLaunchKernelGGL(k_sample_dump, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, sc, indices, dimensions, count,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_words, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, dst, count, value);
+    return hipGetLastError();
+}
+
+hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_word, dim3(1), dim3(64), 0, s, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t bfk_export_paths(const bf_dev_paths* paths, uint32_t count, bf_path_result* out, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_export_paths, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, count, out);
+    return hipGetLastError();
+}
+
+} /* extern "C" */

@@ -1,0 +1,377 @@
+/*
+ * BVH2 closest-hit and any-hit traversal over 64-byte wide nodes with a per-lane LDS stack.
+ *
+ * Decisions are bit-identical to the reference traversal (CudaSceneClosestHit.cu:407-1186):
+ *  - the slab test is the binary64 quotient widened by one ulp (:407-455);
+ *  - children are pushed far-then-near with the first (lower canonical index) child winning ties
+ *    (:466-512); the first child of an internal node always has the lower pre-order index;
+ *  - triangles use the watertight binary32 test with binary64 re-decision of ambiguous edges
+ *    (:583-824); candidates are ordered by (t, instance id, primitive id) (:873-881).
+ * The reference re-tests a node's own box with the current t_max when it is popped. A node's box
+ * passed its push-time test, so that pop test reduces to near <= t_max; the stack stores near
+ * rounded UP to binary32 (exactly equivalent for a binary32 t_max), which lets a popped leaf go
+ * straight to its triangle without fetching a node. The reference keeps one 64-entry stack per
+ * structure; here TLAS and BLAS share one stack of BF_STACK_DEPTH entries and the host rejects
+ * scenes whose TLAS depth + BLAS depth + 2 exceeds it.
+ */
+#ifndef BF_TRAVERSAL_H
+#define BF_TRAVERSAL_H
+
+#include "bf_device_math.h"
+#include "bf_device_scene.h"
+
+struct trav_ray {
+    v3f o, d;
+    float t_min, t_max;
+    uint32_t mask;
+};
+
+struct trav_hit {
+    bool present;
+    float t, b0, b1, b2;
+    uint32_t instance_id, primitive, scene_instance, leaf;
+};
+
+struct trav_counters {
+    uint32_t nodes, triangles, instances;
+};
+
+/* CudaSceneClosestHit.cu:407-455 (boxes are validated finite and ordered by the host builder) */
+BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, v3f o, v3f d, float t_min, float t_max,
+                   bool& hit, double& near_value) {
+    double nv = (double)t_min;
+    double fv = (double)t_max;
+    const float lo[3] = {lx, ly, lz};
+    const float hi[3] = {hx, hy, hz};
+    const float oo[3] = {o.x, o.y, o.z};
+    const float dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (dd[a] == 0.0f) {
+            if (oo[a] < lo[a] || oo[a] > hi[a]) {
+                hit = false;
+                return BF_HIT_MISS;
+            }
+            continue;
+        }
+        const double first = ((double)lo[a] - (double)oo[a]) / (double)dd[a];
+        const double second = ((double)hi[a] - (double)oo[a]) / (double)dd[a];
+        if (bf_isnand(first) || bf_isnand(second)) return BF_HIT_NUMERICAL_FAILURE;
+        nv = fmax(nv, bf_next_down(fmin(first, second)));
+        fv = fmin(fv, bf_next_up(fmax(first, second)));
+        if (nv > fv) {
+            hit = false;
+            return BF_HIT_MISS;
+        }
+    }
+    hit = true;
+    near_value = nv;
+    return BF_HIT_MISS;
+}
+
+BF_D float round_up_float(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = bf_next_up_f(f);
+    return f;
+}
+
+struct tri_result {
+    bool hit;
+    float t, b0, b1, b2;
+};
+
+/* CudaSceneClosestHit.cu:583-692 */
+template <bool BARY>
+BF_D uint32_t refine_edges(v3f v0, v3f v1, v3f v2, v3f o, v3f d, uint32_t kx, uint32_t ky, uint32_t kz, float t_min,
+                           float t_max, tri_result& r) {
+    const double dx = (double)comp3(d, kx), dy = (double)comp3(d, ky), dz = (double)comp3(d, kz);
+    if (dz == 0.0) return BF_HIT_NUMERICAL_FAILURE;
+    const double p0x = (double)comp3(v0, kx) - (double)comp3(o, kx);
+    const double p0y = (double)comp3(v0, ky) - (double)comp3(o, ky);
+    const double p0z = (double)comp3(v0, kz) - (double)comp3(o, kz);
+    const double p1x = (double)comp3(v1, kx) - (double)comp3(o, kx);
+    const double p1y = (double)comp3(v1, ky) - (double)comp3(o, ky);
+    const double p1z = (double)comp3(v1, kz) - (double)comp3(o, kz);
+    const double p2x = (double)comp3(v2, kx) - (double)comp3(o, kx);
+    const double p2y = (double)comp3(v2, ky) - (double)comp3(o, ky);
+    const double p2z = (double)comp3(v2, kz) - (double)comp3(o, kz);
+    const double ax = p0x * dz - dx * p0z;
+    const double ay = p0y * dz - dy * p0z;
+    const double bx = p1x * dz - dx * p1z;
+    const double by = p1y * dz - dy * p1z;
+    const double cx = p2x * dz - dx * p2z;
+    const double cy = p2y * dz - dy * p2z;
+    const double e0 = bx * cy - by * cx;
+    const double e1 = cx * ay - cy * ax;
+    const double e2 = ax * by - ay * bx;
+    if (!bf_isfinited(e0) || !bf_isfinited(e1) || !bf_isfinited(e2)) return BF_HIT_NUMERICAL_FAILURE;
+    const bool neg = e0 < 0.0 || e1 < 0.0 || e2 < 0.0;
+    const bool pos = e0 > 0.0 || e1 > 0.0 || e2 > 0.0;
+    if (neg && pos) return BF_HIT_MISS;
+    const double det = e0 + e1 + e2;
+    if (!bf_isfinited(det)) return BF_HIT_NUMERICAL_FAILURE;
+    if (det == 0.0) return BF_HIT_MISS;
+    const double num = fma(e0, p0z, fma(e1, p1z, e2 * p2z));
+    const double den = det * dz;
+    if (!bf_isfinited(num) || !bf_isfinited(den) || den == 0.0) return BF_HIT_NUMERICAL_FAILURE;
+    const double t = num / den;
+    if (!BARY) {
+        if (!bf_isfinited(t)) return BF_HIT_NUMERICAL_FAILURE;
+        if (t < (double)t_min || t > (double)t_max) return BF_HIT_MISS;
+        const float ft = (float)t;
+        if (!bf_isfinite(ft)) return BF_HIT_NUMERICAL_FAILURE;
+        r.hit = true;
+        r.t = ft == 0.0f ? 0.0f : ft;
+        return BF_HIT_MISS;
+    }
+    const double b0 = e0 / det, b1 = e1 / det, b2 = e2 / det;
+    if (!bf_isfinited(t) || !bf_isfinited(b0) || !bf_isfinited(b1) || !bf_isfinited(b2)) return BF_HIT_NUMERICAL_FAILURE;
+    if (t < (double)t_min || t > (double)t_max) return BF_HIT_MISS;
+    const float ft = (float)t, f0 = (float)b0, f1 = (float)b1, f2 = (float)b2;
+    if (!bf_isfinite(ft) || !bf_isfinite(f0) || !bf_isfinite(f1) || !bf_isfinite(f2)) return BF_HIT_NUMERICAL_FAILURE;
+    r.hit = true;
+    r.t = ft == 0.0f ? 0.0f : ft;
+    r.b0 = f0 == 0.0f ? 0.0f : f0;
+    r.b1 = f1 == 0.0f ? 0.0f : f1;
+    r.b2 = f2 == 0.0f ? 0.0f : f2;
+    return BF_HIT_MISS;
+}
+
+/* CudaSceneClosestHit.cu:694-824 */
+template <bool BARY>
+BF_D uint32_t watertight(v3f v0, v3f v1, v3f v2, v3f o, v3f d, float t_min, float t_max, tri_result& r) {
+    uint32_t kz = 0;
+    float maxd = fabsf(d.x);
+    if (fabsf(d.y) > maxd) {
+        kz = 1;
+        maxd = fabsf(d.y);
+    }
+    if (fabsf(d.z) > maxd) kz = 2;
+    uint32_t kx = kz + 1;
+    if (kx == 3) kx = 0;
+    uint32_t ky = kx + 1;
+    if (ky == 3) ky = 0;
+    if (comp3(d, kz) < 0.0f) {
+        const uint32_t t = kx;
+        kx = ky;
+        ky = t;
+    }
+    const v3f t0 = sub3(v0, o), t1 = sub3(v1, o), t2 = sub3(v2, o);
+    if (!fin3(t0) || !fin3(t1) || !fin3(t2)) return BF_HIT_NUMERICAL_FAILURE;
+    const float rz = 1.0f / comp3(d, kz);
+    const float sx = comp3(d, kx) * rz;
+    const float sy = comp3(d, ky) * rz;
+    if (!bf_isfinite(rz) || !bf_isfinite(sx) || !bf_isfinite(sy)) return BF_HIT_NUMERICAL_FAILURE;
+    const float ax = fmaf(-sx, comp3(t0, kz), comp3(t0, kx));
+    const float ay = fmaf(-sy, comp3(t0, kz), comp3(t0, ky));
+    const float bx = fmaf(-sx, comp3(t1, kz), comp3(t1, kx));
+    const float by = fmaf(-sy, comp3(t1, kz), comp3(t1, ky));
+    const float cx = fmaf(-sx, comp3(t2, kz), comp3(t2, kx));
+    const float cy = fmaf(-sy, comp3(t2, kz), comp3(t2, ky));
+    const float az = comp3(t0, kz) * rz;
+    const float bz = comp3(t1, kz) * rz;
+    const float cz = comp3(t2, kz) * rz;
+    if (!bf_isfinite(ax) || !bf_isfinite(ay) || !bf_isfinite(bx) || !bf_isfinite(by) || !bf_isfinite(cx) ||
+        !bf_isfinite(cy) || !bf_isfinite(az) || !bf_isfinite(bz) || !bf_isfinite(cz))
+        return BF_HIT_NUMERICAL_FAILURE;
+    const float e0 = bx * cy - by * cx;
+    const float e1 = cx * ay - cy * ax;
+    const float e2 = ax * by - ay * bx;
+    if (!bf_isfinite(e0) || !bf_isfinite(e1) || !bf_isfinite(e2)) return BF_HIT_NUMERICAL_FAILURE;
+    const bool neg = e0 < 0.0f || e1 < 0.0f || e2 < 0.0f;
+    const bool pos = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f;
+    if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f || (neg && pos))
+        return refine_edges<BARY>(v0, v1, v2, o, d, kx, ky, kz, t_min, t_max, r);
+    const float det = e0 + e1 + e2;
+    if (!bf_isfinite(det)) return BF_HIT_NUMERICAL_FAILURE;
+    if (det == 0.0f) return BF_HIT_MISS;
+    const float sp = fmaf(e0, az, fmaf(e1, bz, e2 * cz));
+    if (!bf_isfinite(sp)) return BF_HIT_NUMERICAL_FAILURE;
+    const float smin = t_min * det, smax = t_max * det;
+    if (bf_isnan(smin) || bf_isnan(smax)) return BF_HIT_NUMERICAL_FAILURE;
+    if ((det > 0.0f && (sp < smin || sp > smax)) || (det < 0.0f && (sp > smin || sp < smax))) return BF_HIT_MISS;
+    const float inv = 1.0f / det;
+    const float t = sp * inv;
+    if (!BARY) {
+        if (!bf_isfinite(t)) return BF_HIT_NUMERICAL_FAILURE;
+        if (t < t_min || t > t_max) return BF_HIT_MISS;
+        r.hit = true;
+        r.t = t == 0.0f ? 0.0f : t;
+        return BF_HIT_MISS;
+    }
+    const float b0 = e0 * inv, b1 = e1 * inv, b2 = e2 * inv;
+    if (!bf_isfinite(t) || !bf_isfinite(b0) || !bf_isfinite(b1) || !bf_isfinite(b2)) return BF_HIT_NUMERICAL_FAILURE;
+    if (t < t_min || t > t_max) return BF_HIT_MISS;
+    r.hit = true;
+    r.t = t == 0.0f ? 0.0f : t;
+    r.b0 = b0 == 0.0f ? 0.0f : b0;
+    r.b1 = b1 == 0.0f ? 0.0f : b1;
+    r.b2 = b2 == 0.0f ? 0.0f : b2;
+    return BF_HIT_MISS;
+}
+
+/* CudaSceneClosestHit.cu:873-881 */
+BF_D bool precedes(float t, uint32_t inst, uint32_t prim, const trav_hit& c) {
+    return !c.present || t < c.t || (t == c.t && (inst < c.instance_id || (inst == c.instance_id && prim < c.primitive)));
+}
+
+/*
+ * Traverses TLAS then BLASes depth-first. `sref`/`snear` point at this lane's column of the LDS
+ * stack (entry i at index i*stride). Returns a traversal status (BF_HIT_MISS = no error).
+ */
+template <bool ANY, bool COUNT>
+BF_D uint32_t traverse(const bf_dev_scene& S, const trav_ray& ray, uint32_t* sref, float* snear, uint32_t stride,
+                       trav_hit& best, bool& occluded, trav_counters& ctr) {
+    best.present = false;
+    best.t = ray.t_max;
+    occluded = false;
+    if (S.tlas_root_ref == BF_INVALID_INDEX) return BF_HIT_MISS;
+    uint32_t sp = 0;
+    {
+        bool h;
+        double nv;
+        const uint32_t st = slab(S.tlas_root_box[0], S.tlas_root_box[1], S.tlas_root_box[2], S.tlas_root_box[3],
+                                 S.tlas_root_box[4], S.tlas_root_box[5], ray.o, ray.d, ray.t_min, ray.t_max, h, nv);
+        if (st != BF_HIT_MISS) return st;
+        if (!h) {
+            if (COUNT) ctr.nodes++;
+            return BF_HIT_MISS;
+        }
+        sref[0] = S.tlas_root_ref;
+        snear[0] = round_up_float(nv);
+        sp = 1;
+    }
+    bool in_blas = false;
+    uint32_t blas_base = 0;
+    v3f lo = ray.o, ld = ray.d;
+    uint32_t inst_id = 0, scene_inst = 0;
+    while (sp != 0) {
+        if (in_blas && sp <= blas_base) {
+            in_blas = false;
+            lo = ray.o;
+            ld = ray.d;
+        }
+        --sp;
+        const uint32_t ref = sref[sp * stride];
+        const float nearv = snear[sp * stride];
+        if (COUNT) ctr.nodes++;
+        const float tcur = ANY ? ray.t_max : best.t;
+        if (!(nearv <= tcur)) continue;
+        if (ref & BF_WIDE_LEAF) {
+            const uint32_t leaf = ref & ~BF_WIDE_LEAF;
+            if (!in_blas) {
+                const bf_dev_tlas_leaf L = S.tlas_leaves[leaf];
+                if ((ray.mask & L.visibility_mask) == 0u) continue;
+                lo = xform_point(L.world_to_local, ray.o);
+                ld = xform_vector(L.world_to_local, ray.d);
+                if (!fin3(lo) || !fin3(ld) || (ld.x == 0.0f && ld.y == 0.0f && ld.z == 0.0f))
+                    return BF_HIT_NUMERICAL_FAILURE;
+                if (COUNT) ctr.instances++;
+                inst_id = L.instance_id;
+                scene_inst = L.scene_instance;
+                const bf_dev_blas B = S.blases[L.blas_index];
+                bool h;
+                double nv;
+                const uint32_t st = slab(B.root_box[0], B.root_box[1], B.root_box[2], B.root_box[3], B.root_box[4],
+                                         B.root_box[5], lo, ld, ray.t_min, tcur, h, nv);
+                if (st != BF_HIT_MISS) return st;
+                if (!h) {
+                    if (COUNT) ctr.nodes++;
+                    lo = ray.o;
+                    ld = ray.d;
+                    continue;
+                }
+                if (sp >= BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+                in_blas = true;
+                blas_base = sp;
+                sref[sp * stride] = B.root_ref;
+                snear[sp * stride] = round_up_float(nv);
+                ++sp;
+                continue;
+            }
+            const float4 q0 = S.leaf_triangles[3 * (size_t)leaf + 0];
+            const float4 q1 = S.leaf_triangles[3 * (size_t)leaf + 1];
+            const float4 q2 = S.leaf_triangles[3 * (size_t)leaf + 2];
+            const uint32_t prim = __float_as_uint(q0.w);
+            if (COUNT) ctr.triangles++;
+            tri_result tr;
+            tr.hit = false;
+            const uint32_t st = watertight<!ANY>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), lo, ld,
+                                                 ray.t_min, tcur, tr);
+            if (st != BF_HIT_MISS) return st;
+            if (!tr.hit) continue;
+            if (ANY) {
+                occluded = true;
+                return BF_HIT_MISS;
+            }
+            if (!precedes(tr.t, inst_id, prim, best)) continue;
+            best.present = true;
+            best.t = tr.t;
+            best.b0 = tr.b0;
+            best.b1 = tr.b1;
+            best.b2 = tr.b2;
+            best.instance_id = inst_id;
+            best.primitive = prim;
+            best.scene_instance = scene_inst;
+            best.leaf = leaf;
+            continue;
+        }
+        const bf_wide_node* nodes = in_blas ? S.blas_nodes : S.tlas_nodes;
+        const bf_wide_node N = nodes[ref];
+        bool h0, h1;
+        double n0 = 0.0, n1 = 0.0;
+        uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], lo, ld, ray.t_min, tcur, h0, n0);
+        if (st != BF_HIT_MISS) return st;
+        st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], lo, ld, ray.t_min, tcur, h1, n1);
+        if (st != BF_HIT_MISS) return st;
+        if (!h0 && !h1) continue;
+        if (h0 && h1) {
+            if (sp + 2 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+            const float f0 = (float)n0, f1 = (float)n1;
+            const bool first_near = f0 <= f1; /* ties: the first child has the lower canonical index */
+            const uint32_t far_ref = first_near ? N.child[1] : N.child[0];
+            const uint32_t near_ref = first_near ? N.child[0] : N.child[1];
+            sref[sp * stride] = far_ref;
+            snear[sp * stride] = round_up_float(first_near ? n1 : n0);
+            ++sp;
+            sref[sp * stride] = near_ref;
+            snear[sp * stride] = round_up_float(first_near ? n0 : n1);
+            ++sp;
+        } else {
+            if (sp + 1 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+            sref[sp * stride] = h0 ? N.child[0] : N.child[1];
+            snear[sp * stride] = round_up_float(h0 ? n0 : n1);
+            ++sp;
+        }
+    }
+    return BF_HIT_MISS;
+}
+
+/* CudaSceneClosestHit.cu:826-871 */
+BF_D bool world_normal(const float* m, v3f l0, v3f l1, v3f l2, v3f& n) {
+    const v3f w0 = xform_point(m, l0), w1 = xform_point(m, l1), w2 = xform_point(m, l2);
+    if (!fin3(w0) || !fin3(w1) || !fin3(w2)) return false;
+    const v3f f = sub3(w1, w0), s = sub3(w2, w0);
+    if (!fin3(f) || !fin3(s)) return false;
+    const double cx = (double)f.y * (double)s.z - (double)f.z * (double)s.y;
+    const double cy = (double)f.z * (double)s.x - (double)f.x * (double)s.z;
+    const double cz = (double)f.x * (double)s.y - (double)f.y * (double)s.x;
+    const double mx = fmax(fabs(cx), fmax(fabs(cy), fabs(cz)));
+    if (!bf_isfinited(cx) || !bf_isfinited(cy) || !bf_isfinited(cz) || !bf_isfinited(mx) || mx == 0.0) return false;
+    const double sx = cx / mx, sy = cy / mx, sz = cz / mx;
+    const double len = sqrt(sx * sx + sy * sy + sz * sz);
+    if (!bf_isfinited(len) || len == 0.0) return false;
+    n = mk((float)(sx / len), (float)(sy / len), (float)(sz / len));
+    return fin3(n);
+}
+
+/* CudaSceneClosestHit.cu:397-405 */
+BF_D bool valid_traversal_ray(const bf_transport_ray& r) {
+    const v3f o = mk(r.origin_x, r.origin_y, r.origin_z);
+    const v3f d = mk(r.direction_x, r.direction_y, r.direction_z);
+    return fin3(o) && fin3(d) && (d.x != 0.0f || d.y != 0.0f || d.z != 0.0f) && bf_isfinite(r.t_min) && r.t_min >= 0.0f &&
+           !bf_isnan(r.t_max) && r.t_max >= r.t_min && bf_isfinite(r.time) && r.time >= 0.0f && r.time <= 1.0f &&
+           r.reserved == 0u;
+}
+
+#endif

@@ -1,0 +1,239 @@
+"""Python host mirror of the HIP backend (ctypes over libblackframe_hip.so; torch for device memory).
+
+The names follow the reference's C++ API for the CUDA wavefront path
+(Blackframe/Backends/GPU/CUDA/Include/Blackframe/Backends/GPU/CUDA/WavefrontTransport.hpp:147-208):
+XpuSceneSoA/XpuSceneBvh collapse into XpuScene (scene upload + BVH build), XpuWavefrontTransportWorkspace
+owns the fixed-capacity device scratch, and trace_xpu_wavefront_transport runs one frame. Errors raise
+XpuError carrying the library status and message; there is no fallback path.
+"""
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libblackframe_hip.so"
+
+STATUS_NAMES = {0: "ok", 1: "invalid_argument", 2: "incompatible", 3: "resource_exhausted", 4: "unavailable",
+                5: "device_error", 6: "internal_error", 7: "transport_failure"}
+
+
+class XpuError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+_lib = None
+
+
+def library():
+    """Loads the in-tree HIP library; raises if it has not been built.
+
+    torch is imported first so that the library binds to the HIP runtime torch already loaded
+    (both carry the SONAME libamdhip64.so.7): one runtime per process, shared device memory and
+    streams."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401
+    if not LIB_PATH.exists():
+        raise XpuError(4, f"{LIB_PATH} is missing; run __graft_entry__.build()")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+    lib.blackframe_hip_last_error.restype = C.c_char_p
+    lib.blackframe_hip_device_count.argtypes = [C.POINTER(u32)]
+    lib.blackframe_hip_scene_create.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(vp)]
+    lib.blackframe_hip_scene_prepare.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(vp)]
+    lib.blackframe_hip_scene_destroy.argtypes = [vp]
+    lib.blackframe_hip_scene_bvh_export.argtypes = [vp, C.POINTER(u64), vp, vp, vp, vp, vp, C.POINTER(u32)]
+    lib.blackframe_hip_scene_wide_export.argtypes = [vp, C.POINTER(u64), vp, vp]
+    lib.blackframe_hip_launch_scene_closest_hit.argtypes = [vp, vp, u32, vp, vp, vp]
+    lib.blackframe_hip_launch_scene_occlusion.argtypes = [vp, vp, u32, vp, vp]
+    lib.blackframe_hip_launch_sample_stream_dump.argtypes = [u32, vp, vp, u32, vp, vp]
+    lib.blackframe_hip_workspace_create.argtypes = [u64, C.POINTER(vp)]
+    lib.blackframe_hip_workspace_destroy.argtypes = [vp]
+    lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
+                                                             C.POINTER(abi.TransportOptions), vp, vp, vp, vp,
+                                                             C.POINTER(abi.TransportReport), vp, vp]
+    _lib = lib
+    return lib
+
+
+def _check(status):
+    if status != 0:
+        raise XpuError(status, library().blackframe_hip_last_error().decode())
+
+
+def _stream_handle(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+class XpuScene:
+    """Device scene + deterministic BVH (CudaSceneSoA::upload + CudaSceneBvh::build)."""
+
+    def __init__(self, scene, upload=True):
+        self._lib = library()
+        self.handle = C.c_void_p()
+        self.scene = scene  # keeps host arrays alive for the duration of the build
+        create = self._lib.blackframe_hip_scene_create if upload else self._lib.blackframe_hip_scene_prepare
+        _check(create(C.byref(scene.desc), C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            _check(self._lib.blackframe_hip_scene_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bvh(self):
+        counts = (C.c_uint64 * 5)()
+        root = C.c_uint32()
+        _check(self._lib.blackframe_hip_scene_bvh_export(self.handle, counts, None, None, None, None, None, C.byref(root)))
+        blases = (abi.BvhBlas * max(1, counts[0]))()
+        nodes = (abi.BvhNode * max(1, counts[1]))()
+        prims = np.zeros(max(1, counts[2]), dtype=np.uint32)
+        tlas = (abi.BvhNode * max(1, counts[3]))()
+        irefs = (abi.BvhInstanceRef * max(1, counts[4]))()
+        _check(self._lib.blackframe_hip_scene_bvh_export(self.handle, counts, blases, nodes, prims.ctypes.data, tlas, irefs,
+                                                         C.byref(root)))
+        as_bytes = lambda arr, n, size: bytes(memoryview(arr).cast("B")[: n * size])
+        return {
+            "counts": list(counts),
+            "blas": as_bytes(blases, counts[0], 32),
+            "blas_nodes": as_bytes(nodes, counts[1], 48),
+            "primitive_refs": prims[: counts[2]].tobytes(),
+            "tlas_nodes": as_bytes(tlas, counts[3], 48),
+            "instance_refs": as_bytes(irefs, counts[4], 32),
+            "tlas_root": root.value,
+        }
+
+    def wide_nodes(self):
+        counts = (C.c_uint64 * 2)()
+        _check(self._lib.blackframe_hip_scene_wide_export(self.handle, counts, None, None))
+        blas = np.zeros((max(1, counts[0]), 16), dtype=np.uint32)
+        tlas = np.zeros((max(1, counts[1]), 16), dtype=np.uint32)
+        _check(self._lib.blackframe_hip_scene_wide_export(self.handle, counts, blas.ctypes.data, tlas.ctypes.data))
+        return blas[: counts[0]], tlas[: counts[1]]
+
+    def closest_hit(self, rays, counters=False, stream=None):
+        """rays: ctypes array of abi.TransportRay (host). Returns host results (and counters)."""
+        import torch
+
+        n = len(rays)
+        d_rays = torch.frombuffer(bytearray(memoryview(rays).cast("B")), dtype=torch.uint8).cuda()
+        d_out = torch.zeros(n * C.sizeof(abi.ClosestHitResult), dtype=torch.uint8, device="cuda")
+        d_ctr = torch.zeros(3 * n, dtype=torch.int32, device="cuda") if counters else None
+        _check(self._lib.blackframe_hip_launch_scene_closest_hit(
+            self.handle, C.c_void_p(d_rays.data_ptr()), n, C.c_void_p(d_out.data_ptr()),
+            C.c_void_p(d_ctr.data_ptr()) if counters else None, _stream_handle(stream)))
+        torch.cuda.synchronize()
+        out = (abi.ClosestHitResult * n).from_buffer_copy(d_out.cpu().numpy().tobytes())
+        if counters:
+            return out, d_ctr.cpu().numpy().view(np.uint32).reshape(n, 3)
+        return out
+
+    def occlusion(self, rays, stream=None):
+        import torch
+
+        n = len(rays)
+        d_rays = torch.frombuffer(bytearray(memoryview(rays).cast("B")), dtype=torch.uint8).cuda()
+        d_out = torch.zeros(n * C.sizeof(abi.OcclusionResult), dtype=torch.uint8, device="cuda")
+        _check(self._lib.blackframe_hip_launch_scene_occlusion(self.handle, C.c_void_p(d_rays.data_ptr()), n,
+                                                               C.c_void_p(d_out.data_ptr()), _stream_handle(stream)))
+        torch.cuda.synchronize()
+        return (abi.OcclusionResult * n).from_buffer_copy(d_out.cpu().numpy().tobytes())
+
+
+class XpuWavefrontTransportWorkspace:
+    """Fixed-capacity device scratch (CudaWavefrontTransportWorkspace::create / close)."""
+
+    def __init__(self, path_capacity):
+        self._lib = library()
+        self.handle = C.c_void_p()
+        self.capacity = int(path_capacity)
+        _check(self._lib.blackframe_hip_workspace_create(self.capacity, C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            _check(self._lib.blackframe_hip_workspace_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Film:
+    """Device film buffers: float sums, compensated double sums and sample counts."""
+
+    def __init__(self, width, height):
+        import torch
+
+        self.width, self.height = width, height
+        self.float_sum = torch.zeros(width * height * 3, dtype=torch.float32, device="cuda")
+        self.compensated = torch.zeros(width * height * 6, dtype=torch.float64, device="cuda")
+        self.counts = torch.zeros(width * height, dtype=torch.int32, device="cuda")
+
+    def resolved(self):
+        counts = self.counts.cpu().numpy().reshape(self.height, self.width, 1).astype(np.float64)
+        return self.float_sum.cpu().numpy().reshape(self.height, self.width, 3) / np.maximum(counts, 1)
+
+    def host(self):
+        comp = self.compensated.cpu().numpy().reshape(self.height, self.width, 3, 2)
+        return {"film": self.float_sum.cpu().numpy().reshape(self.height, self.width, 3),
+                "film_compensated": comp[..., 0] + comp[..., 1],
+                "film_counts": self.counts.cpu().numpy().view(np.uint32).reshape(self.height, self.width)}
+
+
+def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, want_paths=False, want_counters=False,
+                                  stream=None, synchronize=True):
+    """Renders one frame; mirrors engine::trace_cuda_wavefront_transport (WavefrontTransport.hpp:197-208)."""
+    import torch
+
+    cam = frame.camera
+    film = film if film is not None else Film(cam.width, cam.height)
+    n_paths = cam.width * cam.height * frame.samples_per_pixel
+    d_paths = torch.zeros(n_paths * C.sizeof(abi.PathResult), dtype=torch.uint8, device="cuda") if want_paths else None
+    report = abi.TransportReport()
+    counters = np.zeros(6, dtype=np.uint64)
+    status = library().blackframe_hip_trace_wavefront_transport(
+        workspace.handle, scene.handle, C.byref(frame), C.byref(options), C.c_void_p(film.float_sum.data_ptr()),
+        C.c_void_p(film.compensated.data_ptr()), C.c_void_p(film.counts.data_ptr()),
+        C.c_void_p(d_paths.data_ptr()) if want_paths else None, C.byref(report),
+        C.c_void_p(counters.ctypes.data) if want_counters else None, _stream_handle(stream))
+    _check(status)
+    if synchronize:
+        torch.cuda.synchronize()
+    result = {"film": film, "report": report}
+    if want_counters:
+        result["traversal_counters"] = counters
+    if want_paths:
+        result["paths"] = (abi.PathResult * n_paths).from_buffer_copy(d_paths.cpu().numpy().tobytes())
+    return result
+
+
+def sample_stream_dump(sampler, indices, dimensions, stream=None):
+    """indices: ctypes array of SampleStreamIndex; dimensions: uint64 numpy array. Returns float32 numpy."""
+    import torch
+
+    n = len(indices)
+    d_idx = torch.frombuffer(bytearray(memoryview(indices).cast("B")), dtype=torch.uint8).cuda()
+    d_dim = torch.from_numpy(np.ascontiguousarray(dimensions, dtype=np.uint64).view(np.int64)).cuda()
+    d_out = torch.zeros(n, dtype=torch.float32, device="cuda")
+    _check(library().blackframe_hip_launch_sample_stream_dump(sampler, C.c_void_p(d_idx.data_ptr()),
+                                                              C.c_void_p(d_dim.data_ptr()), n,
+                                                              C.c_void_p(d_out.data_ptr()), _stream_handle(stream)))
+    torch.cuda.synchronize()
+    return d_out.cpu().numpy()

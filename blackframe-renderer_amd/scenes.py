@@ -1,0 +1,415 @@
+"""Seeded scene authoring for the wavefront workloads (host side, numpy).
+
+A scene is authored in float64 and rounded once to the float32 arrays of bf_scene_desc, so every
+consumer (the HIP library and the test oracle) sees identical bytes. Matrices are 3x4 row-major
+affine transforms; world_to_local is the float64 inverse rounded once.
+
+Workloads:
+  cornell(...)          Cornell box at 64x64/16 spp: 5 axis-aligned walls, 2 rotated boxes
+                        (36 triangles), one emissive quad with constant radiance 17, Lambert walls
+                        0.73 / left 0.12 / right 0.45. Wall, emitter and camera coordinates follow
+                        Tests/Backends/CPU/Embree/CornellWavefrontScene.hpp:192-413.
+  icosphere_grid(...)   32x32 grid of seeded displaced icospheres (subdivision 3, 1280 triangles
+                        each, 1.31 M triangles), per-object Lambert / GGX conductor / GGX dielectric,
+                        64 emissive triangles and a constant environment.
+"""
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+
+SPLITMIX_GAMMA = 0x9E3779B97F4A7C15
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the seeded generator behind every synthetic scene."""
+
+    def __init__(self, seed: int):
+        self.state = seed & MASK64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + SPLITMIX_GAMMA) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+
+def _affine_inverse(m):
+    full = np.eye(4)
+    full[:3, :] = m
+    return np.linalg.inv(full)[:3, :]
+
+
+def rotation_y(angle_radians):
+    c, s = math.cos(angle_radians), math.sin(angle_radians)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def affine(linear=None, translation=(0.0, 0.0, 0.0)):
+    m = np.zeros((3, 4))
+    m[:, :3] = np.eye(3) if linear is None else linear
+    m[:, 3] = translation
+    return m
+
+
+class Scene:
+    """Owns the float32 / uint32 arrays behind one bf_scene_desc."""
+
+    def __init__(self):
+        self._positions, self._normals, self._uvs, self._indices = [], [], [], []
+        self.geometries, self.instances, self.materials, self.closures = [], [], [], []
+        self.spectra, self.spectrum_values = [], []
+        self.vertex_count = 0
+        self.triangle_count = 0
+        self.environment_spectrum = abi.INVALID
+        self._constant_cache = {}
+
+    # -- spectra ------------------------------------------------------------------------------
+    def spectrum(self, values):
+        values = [float(v) for v in np.atleast_1d(values)]
+        if len(values) == 1 and values[0] in self._constant_cache:
+            return self._constant_cache[values[0]]
+        sid = len(self.spectra)
+        self.spectra.append((len(self.spectrum_values), len(values)))
+        self.spectrum_values.extend(values)
+        if len(values) == 1:
+            self._constant_cache[values[0]] = sid
+        return sid
+
+    # -- geometry -----------------------------------------------------------------------------
+    def add_geometry(self, gid, positions, normals, uvs, indices):
+        positions = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+        normals = np.asarray(normals, dtype=np.float64).reshape(-1, 3)
+        uvs = np.asarray(uvs, dtype=np.float64).reshape(-1, 2)
+        indices = np.asarray(indices, dtype=np.uint32).reshape(-1, 3)
+        index = len(self.geometries)
+        self.geometries.append((gid, self.vertex_count, len(positions), self.triangle_count, len(indices)))
+        self._positions.append(positions.astype(np.float32))
+        self._normals.append(normals.astype(np.float32))
+        self._uvs.append(uvs.astype(np.float32))
+        self._indices.append(indices)
+        self.vertex_count += len(positions)
+        self.triangle_count += len(indices)
+        return index
+
+    # -- materials ----------------------------------------------------------------------------
+    def add_material(self, mid, closures, emission=None, frame_mode=0, tangent_rotation=0.0):
+        """closures: list of dicts {kind, lobes, weight, eta, k, params, probability}."""
+        offset = len(self.closures)
+        probabilities = []
+        for c in closures:
+            rec = abi.ClosureDesc()
+            rec.kind = c["kind"]
+            rec.lobes = c["lobes"]
+            rec.weight_spectrum = self.spectrum(c["weight"]) if c.get("weight") is not None else abi.INVALID
+            rec.eta_spectrum = self.spectrum(c["eta"]) if c.get("eta") is not None else abi.INVALID
+            rec.k_spectrum = self.spectrum(c["k"]) if c.get("k") is not None else abi.INVALID
+            for i, p in enumerate(c.get("params", [])):
+                rec.parameters[i] = p
+            self.closures.append(rec)
+            probabilities.append(float(c.get("probability", 1.0)))
+        m = abi.MaterialDesc()
+        m.id = mid
+        m.closure_offset = offset
+        m.closure_count = len(closures)
+        m.emission_spectrum = self.spectrum(emission) if emission is not None else abi.INVALID
+        m.frame_mode = frame_mode
+        m.tangent_rotation = tangent_rotation
+        # host-projected sampler grid (TransportLobesDevice.cuh:76-82): normalised probabilities and CDF
+        total = sum(probabilities) if probabilities else 1.0
+        cumulative = 0.0
+        m.cdf[0] = 0.0
+        for i, p in enumerate(probabilities):
+            m.probabilities[i] = np.float32(p / total)
+            cumulative += p / total
+            m.cdf[i + 1] = 1.0 if i + 1 == len(probabilities) else np.float32(cumulative)
+        index = len(self.materials)
+        self.materials.append(m)
+        return index
+
+    # -- instances ----------------------------------------------------------------------------
+    def add_instance(self, iid, object_id, geometry, material, local_to_world=None, mask=0xFFFFFFFF):
+        m = affine() if local_to_world is None else np.asarray(local_to_world, dtype=np.float64)
+        inv = _affine_inverse(m)
+        rec = abi.InstanceDesc()
+        rec.id = iid
+        rec.object_id = object_id
+        rec.geometry_index = geometry
+        rec.material_index = material
+        rec.visibility_mask = mask
+        for i, v in enumerate(m.astype(np.float32).reshape(-1)):
+            rec.local_to_world[i] = v
+        for i, v in enumerate(inv.astype(np.float32).reshape(-1)):
+            rec.world_to_local[i] = v
+        self.instances.append(rec)
+        return len(self.instances) - 1
+
+    def set_environment(self, radiance):
+        self.environment_spectrum = self.spectrum(radiance)
+
+    # -- packing ------------------------------------------------------------------------------
+    def finalize(self):
+        self.positions = np.ascontiguousarray(np.concatenate(self._positions)).astype(np.float32)
+        self.normals = np.ascontiguousarray(np.concatenate(self._normals)).astype(np.float32)
+        self.uvs = np.ascontiguousarray(np.concatenate(self._uvs)).astype(np.float32)
+        self.indices = np.ascontiguousarray(np.concatenate(self._indices)).astype(np.uint32)
+        self._geometry_array = (abi.GeometryDesc * len(self.geometries))()
+        for i, (gid, vo, vc, to, tc) in enumerate(self.geometries):
+            g = self._geometry_array[i]
+            g.id, g.vertex_offset, g.vertex_count, g.triangle_offset, g.triangle_count = gid, vo, vc, to, tc
+        self._instance_array = (abi.InstanceDesc * len(self.instances))(*self.instances)
+        self._material_array = (abi.MaterialDesc * len(self.materials))(*self.materials)
+        self._closure_array = (abi.ClosureDesc * max(1, len(self.closures)))(*self.closures)
+        self._spectrum_array = (abi.SpectrumDesc * len(self.spectra))(*[abi.SpectrumDesc(o, c) for o, c in self.spectra])
+        self._spectrum_values = np.asarray(self.spectrum_values, dtype=np.float32)
+        d = abi.SceneDesc()
+        d.geometry_count = len(self.geometries)
+        d.instance_count = len(self.instances)
+        d.material_count = len(self.materials)
+        d.closure_count = len(self.closures)
+        d.spectrum_count = len(self.spectra)
+        d.has_environment = 0 if self.environment_spectrum == abi.INVALID else 1
+        d.vertex_count = self.vertex_count
+        d.triangle_count = self.triangle_count
+        d.spectrum_value_count = len(self._spectrum_values)
+        d.positions = self.positions.ctypes.data
+        d.normals = self.normals.ctypes.data
+        d.uvs = self.uvs.ctypes.data
+        d.indices = self.indices.ctypes.data
+        d.geometries = C.addressof(self._geometry_array)
+        d.instances = C.addressof(self._instance_array)
+        d.materials = C.addressof(self._material_array)
+        d.closures = C.addressof(self._closure_array)
+        d.spectra = C.addressof(self._spectrum_array)
+        d.spectrum_values = self._spectrum_values.ctypes.data
+        d.environment_spectrum = self.environment_spectrum
+        self.desc = d
+        return self
+
+
+# ---- camera ------------------------------------------------------------------------------------
+def look_at_camera(origin, target, width, height, vertical_fov, time=0.5, jitter=abi.JITTER_UNIFORM):
+    """Pinhole camera frame as OrthonormalFrame::from_normal_and_tangent(-forward, +x) builds it
+    (Tests/Backends/CPU/Embree/CornellWavefrontScene.hpp:391-401, PinholeCamera.hpp:57-90),
+    evaluated in float64 and rounded once."""
+    origin = np.asarray(origin, dtype=np.float64)
+    forward = np.asarray(target, dtype=np.float64) - origin
+    forward /= np.linalg.norm(forward)
+    normal = -forward
+    tangent = np.array([1.0, 0.0, 0.0])
+    tangent = tangent - normal * np.dot(tangent, normal)
+    tangent /= np.linalg.norm(tangent)
+    bitangent = np.cross(normal, tangent)
+    bitangent /= np.linalg.norm(bitangent)
+    tangent = np.cross(bitangent, normal)
+    half_height = math.tan(np.float32(vertical_fov) / 2.0)
+    half_width = half_height * (width / height)
+    cam = abi.CameraDesc()
+    for i in range(3):
+        cam.origin[i] = origin[i]
+        cam.tangent[i] = tangent[i]
+        cam.bitangent[i] = bitangent[i]
+        cam.normal[i] = normal[i]
+    cam.half_width = half_width
+    cam.half_height = half_height
+    cam.t_min = 0.0
+    cam.t_max = float("inf")
+    cam.time = time
+    cam.width = width
+    cam.height = height
+    cam.visibility_mask = 0xFFFFFFFF
+    cam.jitter_mode = jitter
+    return cam
+
+
+def frame_desc(camera, spp, seed=0, first_sample_index=0):
+    f = abi.FrameDesc()
+    f.camera = camera
+    f.seed = seed
+    f.samples_per_pixel = spp
+    f.first_sample_index = first_sample_index
+    return f
+
+
+def transport_options(max_depth=8, mis=abi.MIS_BALANCE, sampler=abi.SAMPLER_SOBOL_OWEN,
+                      wavelength_mode=abi.WAVELENGTHS_PER_PATH, rr_first_depth=3, rr_min=0.1, rr_max=0.9,
+                      packet=(360.0 + 47.0, 360.0 + 164.5, 360.0 + 282.0, 360.0 + 399.5)):
+    o = abi.TransportOptions()
+    o.sampler = sampler
+    o.wavelength_mode = wavelength_mode
+    o.mis_heuristic = mis
+    o.diffuse_depth_limit = max_depth
+    o.glossy_depth_limit = max_depth
+    o.specular_depth_limit = max_depth
+    o.transmission_depth_limit = max_depth
+    o.volume_depth_limit = 0
+    o.russian_roulette_enabled = 1 if rr_first_depth else 0
+    o.russian_roulette_first_depth = rr_first_depth
+    o.russian_roulette_minimum = rr_min if rr_first_depth else 0.0
+    o.russian_roulette_maximum = rr_max if rr_first_depth else 0.0
+    for i in range(4):
+        o.packet_wavelengths[i] = packet[i]
+    return o
+
+
+# ---- shapes --------------------------------------------------------------------------------------
+def quad(corners, normal):
+    return (np.asarray(corners, dtype=np.float64), np.tile(np.asarray(normal, dtype=np.float64), (4, 1)),
+            np.array([[0, 0], [1, 0], [1, 1], [0, 1]], dtype=np.float64), np.array([[0, 1, 2], [0, 2, 3]]))
+
+
+def unit_box():
+    """Axis-aligned cube [-0.5, 0.5]^3 with outward flat normals, 24 vertices, 12 triangles."""
+    faces = []
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            n = np.zeros(3)
+            n[axis] = sign
+            u = np.zeros(3)
+            u[(axis + 1) % 3] = 1.0
+            v = np.cross(n, u)
+            c = n * 0.5
+            corners = [c - 0.5 * u - 0.5 * v, c + 0.5 * u - 0.5 * v, c + 0.5 * u + 0.5 * v, c - 0.5 * u + 0.5 * v]
+            faces.append((corners, n))
+    positions, normals, uvs, indices = [], [], [], []
+    for f, (corners, n) in enumerate(faces):
+        positions.extend(corners)
+        normals.extend([n] * 4)
+        uvs.extend([[0, 0], [1, 0], [1, 1], [0, 1]])
+        b = 4 * f
+        indices.extend([[b, b + 1, b + 2], [b, b + 2, b + 3]])
+    return np.array(positions), np.array(normals), np.array(uvs, dtype=np.float64), np.array(indices)
+
+
+def lambert(reflectance):
+    return {"kind": abi.CLOSURE_LAMBERTIAN, "lobes": abi.LOBE_DIFFUSE | abi.LOBE_REFLECTION, "weight": reflectance}
+
+
+def cornell(width=64, height=64):
+    """Cornell box (see module docstring)."""
+    s = Scene()
+    white = s.add_material(21, [lambert(0.73)])
+    left = s.add_material(22, [lambert(0.12)])
+    right = s.add_material(23, [lambert(0.45)])
+    emitter = s.add_material(24, [lambert(0.0)], emission=17.0)
+    walls = [
+        (11, quad([[-1, -1, 0], [-1, -1, 2], [1, -1, 2], [1, -1, 0]], [0, 1, 0]), white),
+        (12, quad([[-1, 1, 0], [1, 1, 0], [1, 1, 2], [-1, 1, 2]], [0, -1, 0]), white),
+        (13, quad([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], [0, 0, 1]), white),
+        (14, quad([[-1, -1, 0], [-1, 1, 0], [-1, 1, 2], [-1, -1, 2]], [1, 0, 0]), left),
+        (15, quad([[1, -1, 0], [1, -1, 2], [1, 1, 2], [1, 1, 0]], [-1, 0, 0]), right),
+        (16, quad([[-0.32, 0.985, 0.62], [0.32, 0.985, 0.62], [0.32, 0.985, 1.18], [-0.32, 0.985, 1.18]],
+                  [0, -1, 0]), emitter),
+    ]
+    for k, (gid, shape, material) in enumerate(walls):
+        g = s.add_geometry(gid, *shape)
+        s.add_instance(31 + k, 1 + k, g, material)
+    box = s.add_geometry(17, *unit_box())
+    short = affine(rotation_y(math.radians(15.0)) @ np.diag([0.55, 0.55, 0.55]), (0.36, -0.725, 1.30))
+    tall = affine(rotation_y(math.radians(-18.0)) @ np.diag([0.55, 1.15, 0.55]), (-0.38, -0.425, 0.72))
+    s.add_instance(37, 7, box, white, short)
+    s.add_instance(38, 8, box, white, tall)
+    s.finalize()
+    camera = look_at_camera((0.0, 0.0, 4.0), (0.0, -0.05, 0.92), width, height, 0.70)
+    return s, camera
+
+
+def _icosphere(subdivisions):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+             [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+             [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+             [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        new_faces = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            new_faces += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        faces = new_faces
+    return np.array(verts), np.array(faces, dtype=np.uint32)
+
+
+# Tabulated complex IOR of gold and copper on 8 samples over [360, 830] nm (smooth fits of
+# published optical constants; spectral shape, not a colorimetric reference).
+GOLD_ETA = [1.66, 1.48, 0.62, 0.27, 0.18, 0.16, 0.17, 0.20]
+GOLD_K = [1.95, 1.89, 2.11, 2.78, 3.35, 3.86, 4.33, 4.78]
+COPPER_ETA = [1.34, 1.18, 1.05, 0.64, 0.25, 0.22, 0.24, 0.27]
+COPPER_K = [1.94, 2.33, 2.58, 2.58, 3.28, 3.90, 4.42, 4.95]
+
+
+def icosphere_grid(width=1920, height=1080, grid=32, subdivisions=3, seed=0x5EED_1C05, emitter_count=64,
+                   materials="mixed"):
+    """32x32 displaced icospheres (see module docstring). materials='lambert' keeps every object
+    Lambertian (used where a closure kind is not yet available)."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    base_vertices, faces = _icosphere(subdivisions)
+    spacing = 2.6
+    emitter_material = s.add_material(900, [lambert(0.0)], emission=40.0)
+    for gy in range(grid):
+        for gx in range(grid):
+            obj = gy * grid + gx
+            # seeded value noise: 8 random lobes per sphere
+            lobes = [(np.array([rng.uniform() * 2 - 1, rng.uniform() * 2 - 1, rng.uniform() * 2 - 1]),
+                      0.08 * rng.uniform()) for _ in range(8)]
+            verts = base_vertices.copy()
+            scale = np.ones(len(verts))
+            for direction, amplitude in lobes:
+                direction /= max(np.linalg.norm(direction), 1e-9)
+                scale += amplitude * np.maximum(verts @ direction, 0.0) ** 2
+            verts = verts * scale[:, None]
+            normals = verts / np.linalg.norm(verts, axis=1, keepdims=True)
+            uvs = np.stack([0.5 + np.arctan2(normals[:, 2], normals[:, 0]) / (2 * math.pi),
+                            0.5 - np.arcsin(np.clip(normals[:, 1], -1, 1)) / math.pi], axis=1)
+            g = s.add_geometry(1000 + obj, verts, normals, uvs, faces)
+            kind = 0 if materials == "lambert" else int(rng.uniform() * 3)
+            roughness = 0.05 + 0.55 * rng.uniform()
+            albedo = 0.1 + 0.8 * rng.uniform()
+            if kind == 0:
+                closures = [lambert(albedo)]
+            elif kind == 1:
+                metal = rng.uniform() < 0.5
+                closures = [{"kind": abi.CLOSURE_ROUGH_CONDUCTOR, "lobes": abi.LOBE_GLOSSY | abi.LOBE_REFLECTION,
+                             "weight": 1.0, "eta": GOLD_ETA if metal else COPPER_ETA,
+                             "k": GOLD_K if metal else COPPER_K,
+                             "params": [0, 0, 0, 0, 0, 0, 0, 0, roughness * roughness, roughness * roughness]}]
+            else:
+                closures = [{"kind": abi.CLOSURE_ROUGH_DIELECTRIC, "lobes": abi.LOBE_GLOSSY | abi.LOBE_REFLECTION,
+                             "weight": 1.0, "params": [1.5, 1.0, roughness * roughness, roughness * roughness]}]
+            m = s.add_material(2000 + obj, closures)
+            x = (gx - (grid - 1) / 2.0) * spacing
+            z = (gy - (grid - 1) / 2.0) * spacing
+            s.add_instance(3000 + obj, 3000 + obj, g, m, affine(translation=(x, 0.0, z)))
+    # 64 emissive triangles: one shared triangle geometry, one light instance each
+    tri = s.add_geometry(5000, [[-1.5, 0, -1.5], [1.5, 0, -1.5], [0, 0, 1.8]], [[0, -1, 0]] * 3,
+                         [[0, 0], [1, 0], [0, 1]], [[0, 1, 2]])
+    side = int(math.sqrt(emitter_count))
+    for k in range(emitter_count):
+        ex = ((k % side) - (side - 1) / 2.0) * spacing * grid / side
+        ez = ((k // side) - (side - 1) / 2.0) * spacing * grid / side
+        s.add_instance(10000 + k, 10000 + k, tri, emitter_material, affine(translation=(ex, 14.0, ez)))
+    s.set_environment(0.05)
+    s.finalize()
+    extent = spacing * grid / 2.0
+    camera = look_at_camera((0.0, extent * 0.9, extent * 1.6), (0.0, -2.0, 0.0), width, height, 0.9)
+    return s, camera

@@ -1,0 +1,114 @@
+/*
+ * Blackframe XPU/HIP — C ABI of the MI355X wavefront path-tracing backend (libblackframe_hip.so).
+ *
+ * Conventions (the reference launcher ABI's, Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/
+ * WavefrontStageKernel.hpp:324-401 and SceneClosestHit.hpp:1-17):
+ *  - every entry point returns an int status (enum bf_status, 0 = success); invalid arguments are
+ *    rejected synchronously before any launch;
+ *  - `stream` is an opaque hipStream_t (NULL = the legacy default stream);
+ *  - launchers never allocate and never synchronise; the caller owns completion;
+ *  - pointers documented "device" name storage on the scene's device, "host" name host memory;
+ *  - there is no fallback: an unsupported request is an error.
+ * No C++ types cross this boundary.
+ */
+#ifndef BLACKFRAME_HIP_H
+#define BLACKFRAME_HIP_H
+
+#include "blackframe_hip_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bf_hip_scene bf_hip_scene;
+typedef struct bf_hip_workspace bf_hip_workspace;
+
+/* Human-readable text of the last error raised on this thread (never NULL). */
+const char* blackframe_hip_last_error(void);
+
+/* Number of visible HIP devices (0 when the runtime reports none).
+ * Replaces: blackframe_xpu_backend_api_v1::get_device_count (XpuBackendAbi.h:45-55). */
+int blackframe_hip_device_count(uint32_t* count);
+
+/*
+ * Builds the device scene on the current device: the SoA scene upload plus the deterministic
+ * BVH (BLAS per geometry + TLAS over instances, median split, leaf size 1) re-laid out as 64-byte
+ * wide nodes, and the mesh-area-light tables.
+ * Replaces: engine::CudaSceneSoA::upload (Backends/GPU/CUDA/Source/SceneSoA.cpp:1180) followed by
+ *           engine::CudaSceneBvh::build (Backends/GPU/CUDA/Source/SceneBvh.cpp:1109-1146).
+ */
+int blackframe_hip_scene_create(const bf_scene_desc* desc, bf_hip_scene** out_scene);
+
+/* Host-only variant: builds the canonical and wide BVH and the light tables without touching a
+ * device (for inspection and export; such a scene cannot be traced). */
+int blackframe_hip_scene_prepare(const bf_scene_desc* desc, bf_hip_scene** out_scene);
+int blackframe_hip_scene_destroy(bf_hip_scene* scene);
+
+/*
+ * Copies the canonical (reference-layout, 48-byte node) BVH arrays to host memory. counts
+ * receives {blas, blas_node, primitive_reference, tlas_node, instance_reference}; pass NULL
+ * arrays to query the counts only. The serialized form of the reference is SceneBvhAbi.hpp:37-98.
+ */
+int blackframe_hip_scene_bvh_export(const bf_hip_scene* scene, uint64_t counts[5], bf_bvh_blas* blases,
+                                    bf_bvh_node* blas_nodes, uint32_t* primitive_refs, bf_bvh_node* tlas_nodes,
+                                    bf_bvh_instance_ref* instance_refs, uint32_t* tlas_root);
+
+/* Copies the device-resident 64-byte wide nodes (host memory): counts = {blas wide, tlas wide}. */
+int blackframe_hip_scene_wide_export(const bf_hip_scene* scene, uint64_t counts[2], bf_wide_node* blas_wide,
+                                     bf_wide_node* tlas_wide);
+
+/*
+ * Closest-hit traversal of `count` device rays into device results; optional device `counters`
+ * receives per ray {canonical nodes visited, triangle tests, instances entered}.
+ * Replaces: blackframe_cuda_launch_scene_closest_hit (XPU/CUDA/Include/Blackframe/XPU/CUDA/SceneClosestHit.hpp).
+ */
+int blackframe_hip_launch_scene_closest_hit(const bf_hip_scene* scene, const bf_transport_ray* rays, uint32_t count,
+                                            bf_closest_hit_result* results, uint32_t* counters, void* stream);
+
+/* Any-hit traversal. Replaces: blackframe_cuda_launch_scene_occlusion (SceneOcclusion.hpp). */
+int blackframe_hip_launch_scene_occlusion(const bf_hip_scene* scene, const bf_transport_ray* rays, uint32_t count,
+                                          bf_occlusion_result* results, void* stream);
+
+/*
+ * Evaluates sample_1d for `count` (index, dimension) pairs with the chosen sampler
+ * (bf_sampler_kind) into device floats.
+ * Replaces: blackframe_cuda_launch_sample_stream_dump (SampleStreamKernel.hpp:170-176).
+ */
+int blackframe_hip_launch_sample_stream_dump(uint32_t sampler, const bf_sample_stream_index* indices,
+                                             const uint64_t* dimensions, uint32_t count, float* samples, void* stream);
+
+/*
+ * Fixed-capacity transport workspace: path-state columns, the four queues, compaction scratch.
+ * `path_capacity` paths are in flight per launch (a multiple of the frame's pixel count).
+ * Replaces: engine::CudaWavefrontTransportWorkspace::create / close (WavefrontTransport.hpp:152-185).
+ */
+int blackframe_hip_workspace_create(uint64_t path_capacity, bf_hip_workspace** out_workspace);
+int blackframe_hip_workspace_destroy(bf_hip_workspace* workspace);
+
+/*
+ * Renders `frame` (spp passes over width*height pixels, path slot s*W*H + y*W + x) through the
+ * wavefront: device-side camera rays and wavelengths, closest hit, shade (emission + MIS, NEE,
+ * BSDF sample, Russian roulette), shadow rays, stable queue compaction, and device film
+ * accumulation in sample order. Film buffers (device) accumulate on top of their contents:
+ *   film            float   [W*H*3]   running float sums (Film<float32>)
+ *   film_compensated double [W*H*6]  (sum, compensation) pairs per channel (Film<float64>)
+ *   film_counts     uint32 [W*H]     samples per pixel
+ * `paths` (device, optional) receives one bf_path_result per path slot of the frame.
+ * `report` (host, optional) receives lane counts, failures and queue fingerprints; this call
+ * synchronises `stream` once per bounce to read the queue sizes.
+ * `traversal_counters` (host, optional, 6 x uint64): {closest nodes, closest triangles,
+ * closest instances, shadow nodes, shadow triangles, shadow instances} over the whole frame.
+ * Replaces: engine::trace_cuda_wavefront_transport (Backends/GPU/CUDA/Include/Blackframe/Backends/
+ *           GPU/CUDA/WavefrontTransport.hpp:197-208, Source/WavefrontTransport.cpp:1983-2833).
+ */
+int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* workspace, const bf_hip_scene* scene,
+                                             const bf_frame_desc* frame, const bf_transport_options* options,
+                                             float* film, double* film_compensated, uint32_t* film_counts,
+                                             bf_path_result* paths, bf_transport_report* report,
+                                             uint64_t* traversal_counters, void* stream);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* BLACKFRAME_HIP_H */

@@ -1,0 +1,362 @@
+/*
+ * Blackframe XPU/HIP — host/device record layouts shared by the C ABI (blackframe_hip.h),
+ * the HIP kernels and any host that binds the library.
+ *
+ * Records are plain C: fixed-width scalars, explicit padding, no native pointers. Where a record
+ * restates one of the reference's frozen host/device records its size and field offsets are kept
+ * identical (static asserts below) so that a host which already fills the reference record can pass
+ * the same bytes:
+ *   bf_sample_stream_index   <- Blackframe/XPU/Shared/Include/Blackframe/XPU/Shared/TransportAbi.hpp:38-43
+ *   bf_transport_ray         <- TransportAbi.hpp:22-35
+ *   bf_closest_hit           <- TransportAbi.hpp:100-110 (+ SurfaceIdentifiers :92-99)
+ *   bf_closest_hit_result    <- Blackframe/XPU/Shared/Include/Blackframe/XPU/Shared/SceneTraversalAbi.hpp:36-40
+ *   bf_occlusion_result      <- SceneTraversalAbi.hpp:44-47
+ *   bf_bvh_node              <- Blackframe/XPU/Shared/Include/Blackframe/XPU/Shared/SceneBvhAbi.hpp:37-50
+ *   bf_bvh_blas              <- SceneBvhAbi.hpp:52-60
+ *   bf_bvh_instance_ref      <- SceneBvhAbi.hpp:62-70
+ *   bf_closure_record        <- Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/TransportLobesDevice.cuh:66-71
+ *   bf_queue_header          <- TransportAbi.hpp:48-58
+ * Everything else (scene description, device-resident wide BVH node, transport options and
+ * report) is this backend's own design.
+ */
+#ifndef BLACKFRAME_HIP_ABI_H
+#define BLACKFRAME_HIP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BF_SPECTRUM_LANES 4u
+#define BF_MAX_CLOSURES 8u
+#define BF_CLOSURE_PARAMETERS 10u
+#define BF_INVALID_INDEX 0xFFFFFFFFu
+#define BF_TRAVERSAL_STACK_CAPACITY 64u
+
+/* ---- status codes returned by every entry point (0 = success) ---------------------------- */
+enum bf_status {
+    BF_OK = 0,
+    BF_INVALID_ARGUMENT = 1,
+    BF_INCOMPATIBLE = 2,
+    BF_RESOURCE_EXHAUSTED = 3,
+    BF_UNAVAILABLE = 4,
+    BF_DEVICE_ERROR = 5,
+    BF_INTERNAL_ERROR = 6,
+    BF_TRANSPORT_FAILURE = 7
+};
+
+/* ---- traversal status (SceneTraversalAbi.hpp:11-32) ---------------------------------------- */
+enum bf_closest_hit_status {
+    BF_HIT_MISS = 0,
+    BF_HIT_HIT = 1,
+    BF_HIT_INVALID_RAY = 2,
+    BF_HIT_INVALID_SCENE = 3,
+    BF_HIT_INVALID_BVH = 4,
+    BF_HIT_STACK_OVERFLOW = 5,
+    BF_HIT_INVALID_TOPOLOGY = 6,
+    BF_HIT_NUMERICAL_FAILURE = 7
+};
+enum bf_occlusion_status { BF_VISIBLE = 0, BF_OCCLUDED = 1 };
+
+/* ---- sampler / wavelength / MIS selectors -------------------------------------------------- */
+enum bf_sampler_kind { BF_SAMPLER_INDEXED_HASH = 0, BF_SAMPLER_SOBOL_OWEN = 1 };
+enum bf_wavelength_mode { BF_WAVELENGTHS_FRAME_PACKET = 0, BF_WAVELENGTHS_PER_PATH = 1 };
+enum bf_mis_heuristic { BF_MIS_BALANCE = 0, BF_MIS_POWER = 1 };
+enum bf_jitter_mode { BF_JITTER_CENTER = 0, BF_JITTER_UNIFORM = 1 };
+
+/* ---- scattering lobes and closure kinds (TransportLobesDevice.cuh:25-44) ------------------ */
+enum bf_lobe {
+    BF_LOBE_DIFFUSE = 0x01u,
+    BF_LOBE_GLOSSY = 0x02u,
+    BF_LOBE_SPECULAR = 0x04u,
+    BF_LOBE_REFLECTION = 0x08u,
+    BF_LOBE_TRANSMISSION = 0x10u,
+    BF_LOBE_VOLUME = 0x20u
+};
+enum bf_closure_kind {
+    BF_CLOSURE_NONE = 0,
+    BF_CLOSURE_LAMBERTIAN = 1,
+    BF_CLOSURE_ROUGH_DIFFUSE = 2,
+    BF_CLOSURE_ROUGH_CONDUCTOR = 3,
+    BF_CLOSURE_ROUGH_DIELECTRIC = 4,
+    BF_CLOSURE_SPECULAR_REFLECTION = 5,
+    BF_CLOSURE_SPECULAR_TRANSMISSION = 6
+};
+
+/* ---- path termination (WavefrontStageKernel.hpp:51-58) ------------------------------------- */
+enum bf_termination {
+    BF_TERM_NONE = 0,
+    BF_TERM_ESCAPED = 1,
+    BF_TERM_DEPTH_LIMIT = 2,
+    BF_TERM_ZERO_THROUGHPUT = 3,
+    BF_TERM_OUTSIDE_SUPPORT = 4,
+    BF_TERM_RUSSIAN_ROULETTE = 5,
+    BF_TERM_FAILED = 15
+};
+
+/* ---- frozen records ------------------------------------------------------------------------ */
+typedef struct bf_sample_stream_index {
+    uint32_t pixel_x;
+    uint32_t pixel_y;
+    uint64_t sample_index;
+    uint64_t seed;
+} bf_sample_stream_index;
+
+typedef struct bf_transport_ray {
+    float origin_x, origin_y, origin_z, t_min;
+    float direction_x, direction_y, direction_z, t_max;
+    float time;
+    uint32_t visibility_mask;
+    uint32_t current_medium;
+    uint32_t reserved;
+} bf_transport_ray;
+
+typedef struct bf_surface_identifiers {
+    uint32_t object, instance, geometry, primitive, material;
+    uint32_t reserved[3];
+} bf_surface_identifiers;
+
+typedef struct bf_closest_hit {
+    float parameter;
+    float barycentric0, barycentric1, barycentric2;
+    float normal_x, normal_y, normal_z;
+    uint32_t reserved;
+    bf_surface_identifiers ids;
+} bf_closest_hit;
+
+typedef struct bf_closest_hit_result {
+    uint32_t status;
+    uint32_t reserved[3];
+    bf_closest_hit hit;
+} bf_closest_hit_result;
+
+typedef struct bf_occlusion_result {
+    uint32_t status;
+    uint32_t reserved[3];
+} bf_occlusion_result;
+
+typedef struct bf_queue_header {
+    uint16_t abi_major, abi_minor;
+    uint32_t struct_size;
+    uint32_t queue_kind;
+    uint32_t capacity;
+    uint32_t size;
+    uint32_t overflow_count;
+    uint32_t rejected_count;
+    uint32_t reserved;
+} bf_queue_header;
+
+/* Canonical BVH node, bit-identical to the reference's 48-byte SceneBvhNode. */
+typedef struct bf_bvh_node {
+    float min_x, min_y, min_z;
+    uint32_t kind; /* 0 internal, 1 leaf */
+    float max_x, max_y, max_z;
+    uint32_t split_axis;
+    uint32_t first_child, second_child;
+    uint32_t reference_offset, reference_count;
+} bf_bvh_node;
+
+typedef struct bf_bvh_blas {
+    uint32_t geometry_id, root_node, node_offset, node_count;
+    uint32_t primitive_reference_offset, primitive_reference_count;
+    uint32_t reserved[2];
+} bf_bvh_blas;
+
+typedef struct bf_bvh_instance_ref {
+    uint32_t scene_instance_index, instance_id, scene_geometry_index, geometry_id;
+    uint32_t blas_index, visibility_mask;
+    uint32_t reserved[2];
+} bf_bvh_instance_ref;
+
+/*
+ * Device-resident wide node (this backend's layout). One 64-byte record per INTERNAL canonical
+ * node, in canonical pre-order. It stores both children's boxes so one cache line decides both
+ * child tests. child[i] is either (internal 64-byte record index) or (BF_WIDE_LEAF | reference
+ * offset) for a leaf child; leaves are never fetched as nodes.
+ */
+#define BF_WIDE_LEAF 0x80000000u
+typedef struct bf_wide_node {
+    float lo_x[2], lo_y[2], lo_z[2];
+    float hi_x[2], hi_y[2], hi_z[2];
+    uint32_t child[2];
+    uint32_t canonical_index; /* canonical pre-order index of this internal node */
+    uint32_t split_axis;
+} bf_wide_node;
+
+typedef struct bf_closure_record {
+    uint32_t kind;
+    uint32_t lobes;
+    float weight[BF_SPECTRUM_LANES];
+    float parameters[BF_CLOSURE_PARAMETERS];
+} bf_closure_record;
+
+/* ---- scene description (input to the host scene builder) ----------------------------------- */
+
+/* A spectrum is `count` samples on a uniform grid spanning [360, 830] nm; count == 1 is constant. */
+typedef struct bf_spectrum_desc {
+    uint32_t offset; /* into bf_scene_desc.spectrum_values */
+    uint32_t count;
+} bf_spectrum_desc;
+
+/*
+ * A closure's spectral inputs are spectrum ids (index into bf_scene_desc.spectra or
+ * BF_INVALID_INDEX). They are resolved at the path's wavelengths into the reference's
+ * bf_closure_record layout: weight <- weight_spectrum; for rough conductors parameters[0..3] <-
+ * eta_spectrum and parameters[4..7] <- k_spectrum; every other parameter is copied verbatim.
+ */
+typedef struct bf_closure_desc {
+    uint32_t kind;
+    uint32_t lobes;
+    uint32_t weight_spectrum;
+    uint32_t eta_spectrum;
+    uint32_t k_spectrum;
+    uint32_t reserved[3];
+    float parameters[BF_CLOSURE_PARAMETERS];
+    uint32_t reserved_tail[2];
+} bf_closure_desc;
+
+typedef struct bf_material_desc {
+    uint32_t id;
+    uint32_t closure_offset;
+    uint32_t closure_count;
+    uint32_t emission_spectrum; /* BF_INVALID_INDEX when not emissive */
+    uint32_t frame_mode;        /* 0 shading-normal frame, 1 dpdu-tangent frame */
+    float tangent_rotation;
+    uint32_t reserved[2];
+    float probabilities[BF_MAX_CLOSURES];
+    float cdf[BF_MAX_CLOSURES + 1];
+    uint32_t reserved_tail[3];
+} bf_material_desc;
+
+typedef struct bf_geometry_desc {
+    uint32_t id;
+    uint32_t reserved;
+    uint64_t vertex_offset, vertex_count;
+    uint64_t triangle_offset, triangle_count;
+} bf_geometry_desc;
+
+typedef struct bf_instance_desc {
+    uint32_t id;
+    uint32_t object_id;
+    uint32_t geometry_index;
+    uint32_t material_index;
+    uint32_t visibility_mask;
+    uint32_t reserved[3];
+    float local_to_world[12]; /* row-major 3x4 affine */
+    float world_to_local[12];
+} bf_instance_desc;
+
+typedef struct bf_scene_desc {
+    uint32_t geometry_count, instance_count, material_count, closure_count, spectrum_count;
+    uint32_t has_environment;
+    uint64_t vertex_count, triangle_count, spectrum_value_count;
+    const float* positions; /* [vertex_count*3] */
+    const float* normals;   /* [vertex_count*3] */
+    const float* uvs;       /* [vertex_count*2] */
+    const uint32_t* indices; /* [triangle_count*3], geometry-local vertex indices */
+    const bf_geometry_desc* geometries;
+    const bf_instance_desc* instances;
+    const bf_material_desc* materials;
+    const bf_closure_desc* closures;
+    const bf_spectrum_desc* spectra;
+    const float* spectrum_values;
+    uint32_t environment_spectrum;
+    uint32_t reserved;
+} bf_scene_desc;
+
+/* ---- camera / render description ------------------------------------------------------------ */
+typedef struct bf_camera_desc {
+    float origin[3];
+    float tangent[3];
+    float bitangent[3];
+    float normal[3];
+    float half_width, half_height; /* tan(fov/2)*aspect and tan(fov/2), precomputed by the host */
+    float t_min, t_max, time;
+    uint32_t width, height;
+    uint32_t visibility_mask;
+    uint32_t jitter_mode;
+    uint32_t reserved[3];
+} bf_camera_desc;
+
+typedef struct bf_transport_options {
+    uint32_t sampler;         /* bf_sampler_kind */
+    uint32_t wavelength_mode; /* bf_wavelength_mode */
+    uint32_t mis_heuristic;   /* bf_mis_heuristic */
+    uint32_t diffuse_depth_limit, glossy_depth_limit, specular_depth_limit;
+    uint32_t transmission_depth_limit, volume_depth_limit;
+    uint32_t russian_roulette_enabled;
+    uint32_t russian_roulette_first_depth;
+    float russian_roulette_minimum;
+    float russian_roulette_maximum;
+    float packet_wavelengths[BF_SPECTRUM_LANES]; /* used in frame-packet mode */
+    uint32_t record_queues; /* nonzero: fill bf_transport_report queue fingerprints */
+    uint32_t reserved[3];
+} bf_transport_options;
+
+/* One frame of work: spp passes over width*height pixels; path slot p = s*W*H + y*W + x. */
+typedef struct bf_frame_desc {
+    bf_camera_desc camera;
+    uint64_t seed;
+    uint32_t samples_per_pixel;
+    uint32_t first_sample_index;
+} bf_frame_desc;
+
+#define BF_REPORT_MAX_BOUNCES 64u
+typedef struct bf_transport_report {
+    uint64_t path_count;
+    uint64_t closest_hit_queries;
+    uint64_t shadow_queries;
+    uint64_t shade_lanes;
+    uint64_t miss_lanes;
+    uint64_t closure_samples;
+    uint64_t light_samples;
+    uint64_t failed_paths;
+    uint32_t bounces;
+    uint32_t first_failure_slot;
+    uint32_t first_failure_code;
+    uint32_t reserved;
+    /* per bounce: FNV-1a-64 of the ray queue contents (slot order) and its size */
+    uint64_t ray_queue_hash[BF_REPORT_MAX_BOUNCES];
+    uint32_t ray_queue_size[BF_REPORT_MAX_BOUNCES];
+    uint64_t shadow_queue_hash[BF_REPORT_MAX_BOUNCES];
+    uint32_t shadow_queue_size[BF_REPORT_MAX_BOUNCES];
+} bf_transport_report;
+
+/* Per-path terminal record written by the transport (optional output). */
+typedef struct bf_path_result {
+    float radiance[BF_SPECTRUM_LANES];
+    float beta[BF_SPECTRUM_LANES];
+    float wavelengths[BF_SPECTRUM_LANES];
+    uint32_t depth;
+    uint32_t termination;
+    uint32_t depth_counters; /* diffuse | glossy<<8 | specular<<16 | transmission<<24 */
+    float eta_scale;
+} bf_path_result;
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+#define BF_STATIC_ASSERT(c, m) static_assert(c, m)
+#else
+#define BF_STATIC_ASSERT(c, m) _Static_assert(c, m)
+#endif
+BF_STATIC_ASSERT(sizeof(bf_sample_stream_index) == 24, "SampleStreamIndex is 24 bytes");
+BF_STATIC_ASSERT(sizeof(bf_transport_ray) == 48, "TransportRay is 48 bytes");
+BF_STATIC_ASSERT(sizeof(bf_closest_hit) == 64, "ClosestHit is 64 bytes");
+BF_STATIC_ASSERT(sizeof(bf_closest_hit_result) == 80, "SceneClosestHitResult is 80 bytes");
+BF_STATIC_ASSERT(sizeof(bf_occlusion_result) == 16, "SceneOcclusionResult is 16 bytes");
+BF_STATIC_ASSERT(sizeof(bf_queue_header) == 32, "QueueHeader is 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_bvh_node) == 48, "SceneBvhNode is 48 bytes");
+BF_STATIC_ASSERT(sizeof(bf_bvh_blas) == 32, "SceneBvhBlas is 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_bvh_instance_ref) == 32, "SceneBvhInstanceReference is 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_wide_node) == 64, "wide node is one 64-byte line");
+BF_STATIC_ASSERT(sizeof(bf_closure_record) == 64, "ClosureRecord is 64 bytes");
+BF_STATIC_ASSERT(offsetof(bf_closest_hit, ids) == 32, "ClosestHit identifiers at 32");
+BF_STATIC_ASSERT(offsetof(bf_bvh_node, first_child) == 32, "SceneBvhNode first_child at 32");
+BF_STATIC_ASSERT(offsetof(bf_closure_record, parameters) == 24, "ClosureRecord parameters at 24");
+
+#endif /* BLACKFRAME_HIP_ABI_H */

@@ -1,0 +1,79 @@
+/*
+ * ORACLE — test infrastructure only. Nothing in the product links, includes or calls this.
+ *
+ * A plain-C CPU restatement of the reference's wavefront path-tracing hot path, used by tests/
+ * as the parity checker, by __graft_entry__.smoke() as the smoke checker and by bench.py as the
+ * CPU baseline (a std::thread-style worker pool over contiguous lane partitions, as the
+ * reference's CpuWavefrontScheduler does). Every function cites the reference file:line it
+ * restates; paths are relative to the reference repository root.
+ *
+ * Parity status per part is recorded in oracle/README.md.
+ */
+#ifndef BF_ORACLE_H
+#define BF_ORACLE_H
+
+#include "../include/blackframe_hip_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bfo_scene bfo_scene;
+
+/* ---- sampling ----------------------------------------------------------------------------- */
+uint64_t bfo_mix_bits(uint64_t value);
+uint64_t bfo_indexed_bits(const bf_sample_stream_index* index, uint64_t dimension);
+uint64_t bfo_sobol_bits(uint64_t sample_index, uint64_t sobol_dimension);
+uint64_t bfo_owen_bits(const bf_sample_stream_index* index, uint64_t sobol_dimension);
+float bfo_sample_1d(uint32_t sampler, const bf_sample_stream_index* index, uint64_t dimension);
+uint64_t bfo_sobol_dimension_count(void);
+
+/* ---- film sensor -------------------------------------------------------------------------- */
+void bfo_cie_xyz(const float radiance[4], const float wavelengths[4], float pdf, float xyz[3]);
+void bfo_xyz_to_rgb(const float xyz[3], float rgb[3]);
+double bfo_cie_y_integral(void);
+uint64_t bfo_cie_fingerprint(void);
+
+/* ---- scene ---------------------------------------------------------------------------------- */
+int bfo_scene_build(const bf_scene_desc* desc, bfo_scene** out);
+void bfo_scene_free(bfo_scene* scene);
+/* canonical BVH arrays: pass NULL to query counts */
+int bfo_scene_bvh(const bfo_scene* scene, uint64_t counts[5], bf_bvh_blas* blases,
+                  bf_bvh_node* blas_nodes, uint32_t* primitive_refs, bf_bvh_node* tlas_nodes,
+                  bf_bvh_instance_ref* instance_refs);
+uint32_t bfo_scene_tlas_root(const bfo_scene* scene);
+/* mesh-area-light tables: per light (instance index, triangle count, inverse area) */
+uint32_t bfo_scene_light_count(const bfo_scene* scene);
+int bfo_scene_light(const bfo_scene* scene, uint32_t light, uint32_t* instance, uint32_t* triangle_count,
+                    float* inverse_area, float* areas, float* probabilities, float* next_probabilities);
+int bfo_triangle_area(const float vertices[9], float* area, float normal[3]);
+
+/* ---- traversal -------------------------------------------------------------------------------
+ * counters (optional, may be NULL): [0] canonical nodes visited, [1] triangle tests,
+ * [2] instances entered, per ray. */
+void bfo_closest_hit(const bfo_scene* scene, const bf_transport_ray* rays, uint32_t count,
+                     bf_closest_hit_result* results, uint32_t* counters);
+void bfo_occlusion(const bfo_scene* scene, const bf_transport_ray* rays, uint32_t count,
+                   bf_occlusion_result* results, uint32_t* counters);
+void bfo_closest_hit_brute_force(const bfo_scene* scene, const bf_transport_ray* rays,
+                                 uint32_t count, bf_closest_hit_result* results);
+
+/* ---- camera --------------------------------------------------------------------------------- */
+int bfo_camera_ray(const bf_camera_desc* camera, uint32_t sampler, const bf_sample_stream_index* index,
+                   bf_transport_ray* ray);
+
+/* ---- transport ------------------------------------------------------------------------------
+ * Renders paths [path_begin, path_begin + path_count) of the frame with the 7-queue CPU
+ * wavefront. Outputs are optional (NULL). film_* are indexed [pixel*3 + channel] and accumulate
+ * in path-slot order (sample-major), matching the reference's host film loop.
+ * queue_log (optional, 2*64 entries): per bounce FNV-1a of the ray / shadow queue contents.
+ * threads: worker count for each stage (contiguous partitions); 0 = 1. */
+int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame,
+               const bf_transport_options* options, uint64_t path_begin, uint64_t path_count,
+               uint32_t threads, bf_path_result* paths, float* film_float, double* film_compensated,
+               uint32_t* film_counts, bf_transport_report* report, uint64_t* traversal_counters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,583 @@
+/*
+ * ORACLE — test infrastructure only (see bf_oracle.h).
+ * Scalar transport helpers and the closure library restated from
+ *   Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/TransportDevice.cuh
+ *   Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/TransportLobesDevice.cuh
+ */
+#ifndef BF_ORACLE_LOBES_H
+#define BF_ORACLE_LOBES_H
+
+#include "bf_oracle_internal.h"
+
+enum { LOBE_OK = 0, LOBE_INVALID = 1, LOBE_NOT_REPRESENTABLE = 2, LOBE_OUTSIDE_SUPPORT = 3 };
+enum { LOBE_MEASURE_DISCRETE = 0, LOBE_MEASURE_SOLID_ANGLE = 1 };
+
+typedef struct {
+    uint32_t kind, lobes;
+    float weight[4];
+    float parameters[10];
+} closure_rec;
+
+typedef struct {
+    uint32_t active_count;
+    closure_rec closures[BF_MAX_CLOSURES];
+    float probabilities[BF_MAX_CLOSURES];
+    float cdf[BF_MAX_CLOSURES + 1];
+} lobe_mixture;
+
+typedef struct {
+    uint32_t source_indices[BF_MAX_CLOSURES];
+    uint32_t allowed_directions[BF_MAX_CLOSURES];
+    uint32_t source_count;
+    uint32_t blocked_lobes;
+    uint32_t status;
+} lobe_filter_state;
+
+typedef struct {
+    float value[4];
+    v3 incoming;
+    float pdf;
+    uint32_t measure;
+    uint32_t lobes;
+    float eta_scale;
+    uint32_t status;
+} lobe_sample;
+
+typedef struct {
+    float value[4];
+    v3 incoming;
+    float pdf;
+    uint32_t measure;
+    uint32_t selected;
+    uint32_t lobes;
+    float eta_scale;
+    uint32_t status;
+} lobe_mixture_sample;
+
+/* TransportDevice.cuh:327-333 */
+static inline int lobe_unit_vector(v3 d) {
+    if (!v3_finite(d)) return 0;
+    const float sq = fmaf(d.x, d.x, fmaf(d.y, d.y, d.z * d.z));
+    return bf_isfinite(sq) && fabsf(sq - 1.0f) <= 128.0f * BFO_FLOAT_EPSILON;
+}
+
+/* TransportDevice.cuh:196-207 */
+static inline int lobe_checked_product(float l, float r, float* out) {
+    if (!bf_isfinite(l) || l < 0.0f || !bf_isfinite(r) || r < 0.0f) return 0;
+    const float p = l * r;
+    if (!bf_isfinite(p) || p < 0.0f || (l != 0.0f && r != 0.0f && p == 0.0f)) return 0;
+    *out = p;
+    return 1;
+}
+
+/* TransportDevice.cuh:209-259 */
+static inline int lobe_checked_product_quotient(const float* num, int nn, const float* den, int nd, float* out) {
+    int has_zero = 0;
+    for (int i = 0; i < nd; ++i)
+        if (!bf_isfinite(den[i]) || !(den[i] > 0.0f)) return LOBE_INVALID;
+    for (int i = 0; i < nn; ++i) {
+        if (!bf_isfinite(num[i]) || num[i] < 0.0f) return LOBE_INVALID;
+        has_zero = has_zero || num[i] == 0.0f;
+    }
+    if (has_zero) {
+        *out = 0.0f;
+        return LOBE_OK;
+    }
+    float significand = 1.0f;
+    int exponent = 0;
+    for (int i = 0; i < nn; ++i) {
+        int e;
+        significand *= bf_frexpf(num[i], &e);
+        exponent += e;
+    }
+    for (int i = 0; i < nd; ++i) {
+        int e;
+        significand /= bf_frexpf(den[i], &e);
+        exponent -= e;
+    }
+    if (!bf_isfinite(significand) || !(significand > 0.0f)) return LOBE_NOT_REPRESENTABLE;
+    int ne;
+    significand = bf_frexpf(significand, &ne);
+    exponent += ne;
+    const float r = bf_ldexpf(significand, exponent);
+    if (!bf_isfinite(r) || !(r > 0.0f)) return LOBE_NOT_REPRESENTABLE;
+    *out = r;
+    return LOBE_OK;
+}
+
+/* TransportDevice.cuh:552-590 */
+static inline int lobe_joint_light_pdf(float selection, float conditional, float* out) {
+    if (!bf_isfinite(selection) || !(selection > 0.0f) || selection > 1.0f || !bf_isfinite(conditional) ||
+        conditional < 0.0f)
+        return 0;
+    if (conditional == 0.0f) {
+        *out = 0.0f;
+        return 1;
+    }
+    const float num[2] = {selection, conditional};
+    const float den[1] = {1.0f};
+    return lobe_checked_product_quotient(num, 2, den, 1, out) == LOBE_OK;
+}
+
+/* TransportDevice.cuh:592-628 */
+static inline int lobe_ratio_at_most_one(float n, float d, int squared, float* out) {
+    if (!bf_isfinite(n) || !(n > 0.0f) || !bf_isfinite(d) || !(d > 0.0f) || n > d) return 0;
+    int ne, de;
+    const float ns = bf_frexpf(n, &ne);
+    const float ds = bf_frexpf(d, &de);
+    const float ratio = ns / ds;
+    const float r = squared ? bf_ldexpf(ratio * ratio, 2 * (ne - de)) : bf_ldexpf(ratio, ne - de);
+    if (!bf_isfinite(r) || r < 0.0f || r > 1.0f) return 0;
+    *out = r;
+    return 1;
+}
+
+/* TransportDevice.cuh:630-689 (same measure on both densities) */
+static inline int lobe_mis_weight(uint32_t heuristic, float sampled, float competing, float* out) {
+    if (!bf_isfinite(sampled) || !(sampled > 0.0f) || !bf_isfinite(competing) || competing < 0.0f) return 0;
+    if (heuristic > 1u) return 0;
+    if (competing == 0.0f) {
+        *out = 1.0f;
+        return 1;
+    }
+    float ratio;
+    if (sampled < competing) {
+        if (!lobe_ratio_at_most_one(sampled, competing, heuristic == 1u, &ratio) || !(ratio > 0.0f)) return 0;
+        const float w = ratio / (1.0f + ratio);
+        if (!bf_isfinite(w) || !(w > 0.0f) || w > 1.0f) return 0;
+        *out = w;
+        return 1;
+    }
+    if (!lobe_ratio_at_most_one(competing, sampled, heuristic == 1u, &ratio)) return 0;
+    const float w = 1.0f / (1.0f + ratio);
+    if (!bf_isfinite(w) || !(w > 0.0f) || w > 1.0f) return 0;
+    *out = w;
+    return 1;
+}
+
+/* TransportDevice.cuh:711-782 (policy validated by the host) */
+static inline int lobe_russian_roulette(float* throughput, float eta_scale, float u, float minimum, float maximum,
+                                        int* survived) {
+    if (!bf_isfinite(eta_scale) || !(eta_scale > 0.0f) || !bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return 0;
+    float m = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        if (!bf_isfinite(throughput[k]) || throughput[k] < 0.0f) return 0;
+        m = fmaxf(m, throughput[k]);
+    }
+    if (m == 0.0f) return 0;
+    float p;
+    if (m >= maximum / eta_scale) p = maximum;
+    else if (m <= minimum / eta_scale) p = minimum;
+    else {
+        const float candidate = m * eta_scale;
+        p = candidate < minimum ? minimum : (maximum < candidate ? maximum : candidate);
+    }
+    if (!bf_isfinite(p) || !(p > 0.0f) || p > 1.0f) return 0;
+    if (!(u < p)) {
+        *survived = 0;
+        return 1;
+    }
+    float out[4];
+    for (int k = 0; k < 4; ++k) {
+        out[k] = throughput[k] / p;
+        if (!bf_isfinite(out[k]) || (throughput[k] != 0.0f && out[k] == 0.0f)) return 0;
+    }
+    memcpy(throughput, out, sizeof out);
+    *survived = 1;
+    return 1;
+}
+
+/* ---- Lambert (TransportDevice.cuh:335-478) ------------------------------------------------- */
+static inline int lobe_reflectance(const float* w) {
+    for (int k = 0; k < 4; ++k)
+        if (!bf_isfinite(w[k]) || w[k] < 0.0f || w[k] > 1.0f) return 0;
+    return 1;
+}
+
+static inline int lobe_lambert_eval(const float* reflectance, v3 wo, v3 wi, float* out) {
+    if (!lobe_reflectance(reflectance) || !lobe_unit_vector(wo) || !lobe_unit_vector(wi)) return LOBE_INVALID;
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return LOBE_OK;
+    }
+    for (int k = 0; k < 4; ++k) out[k] = reflectance[k] * BFO_INV_PI;
+    return LOBE_OK;
+}
+
+static inline int lobe_lambert_pdf(v3 wo, v3 wi, float* pdf) {
+    if (!lobe_unit_vector(wo) || !lobe_unit_vector(wi)) return LOBE_INVALID;
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+        *pdf = 0.0f;
+        return LOBE_OK;
+    }
+    float d;
+    if (!lobe_checked_product(wi.z, BFO_INV_PI, &d) || !(d > 0.0f)) return LOBE_NOT_REPRESENTABLE;
+    *pdf = d;
+    return LOBE_OK;
+}
+
+/* TransportDevice.cuh:388-427 */
+static inline int lobe_cosine_hemisphere(float u, float v, v3* out) {
+    if (!bf_isfinite(u) || u < 0.0f || !(u < 1.0f) || !bf_isfinite(v) || v < 0.0f || !(v < 1.0f)) return LOBE_INVALID;
+    const float ox = 2.0f * u - 1.0f;
+    const float oy = 2.0f * v - 1.0f;
+    if (ox == 0.0f && oy == 0.0f) {
+        *out = v3_make(0.0f, 0.0f, 1.0f);
+        return LOBE_OK;
+    }
+    float r, phi;
+    if (fabsf(ox) > fabsf(oy)) {
+        r = ox;
+        phi = BFO_QUARTER_PI * (oy / ox);
+    } else {
+        r = oy;
+        phi = 2.0f * BFO_QUARTER_PI - BFO_QUARTER_PI * (ox / oy);
+    }
+    const float radius = fabsf(r);
+    float s, c;
+    bf_sincosf(phi, &s, &c);
+    *out = v3_make(r * c, r * s, sqrtf((1.0f - radius) * (1.0f + radius)));
+    if (!lobe_unit_vector(*out)) return LOBE_NOT_REPRESENTABLE;
+    return LOBE_OK;
+}
+
+static inline int lobe_valid_surface_event(uint32_t bits) {
+    const uint32_t family = bits & 0x7u;
+    const uint32_t direction = bits & 0x18u;
+    return (bits & ~0x3Fu) == 0u && (bits & BF_LOBE_VOLUME) == 0u && family != 0u && (family & (family - 1u)) == 0u &&
+           direction != 0u && (direction & (direction - 1u)) == 0u;
+}
+
+/* ---- closure record dispatch (TransportLobesDevice.cuh:2102-2260) --------------------------- */
+static inline int lobe_record_eval(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* out) {
+    (void)allowed;
+    switch (c->kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+        return lobe_lambert_eval(c->weight, wo, wi, out);
+    default:
+        return LOBE_INVALID;
+    }
+}
+
+static inline int lobe_record_pdf(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* pdf) {
+    (void)allowed;
+    switch (c->kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+        return lobe_lambert_pdf(wo, wi, pdf);
+    default:
+        return LOBE_INVALID;
+    }
+}
+
+static inline void lobe_empty_sample(lobe_sample* s, uint32_t status) {
+    memset(s, 0, sizeof *s);
+    s->measure = LOBE_MEASURE_SOLID_ANGLE;
+    s->eta_scale = 1.0f;
+    s->status = status;
+}
+
+static inline void lobe_record_sample(const closure_rec* c, uint32_t allowed, v3 wo, float event, float u, float v,
+                                      lobe_sample* out) {
+    (void)allowed;
+    (void)event;
+    lobe_empty_sample(out, LOBE_INVALID);
+    switch (c->kind) {
+    case BF_CLOSURE_LAMBERTIAN: {
+        if (!lobe_reflectance(c->weight) || !lobe_unit_vector(wo)) return;
+        v3 wi;
+        const int st = lobe_cosine_hemisphere(u, v, &wi);
+        if (st != LOBE_OK) {
+            out->status = st;
+            return;
+        }
+        if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+            out->incoming = wi;
+            out->status = LOBE_OUTSIDE_SUPPORT;
+            return;
+        }
+        float value[4], pdf;
+        int s1 = lobe_lambert_eval(c->weight, wo, wi, value);
+        int s2 = lobe_lambert_pdf(wo, wi, &pdf);
+        if (s1 != LOBE_OK || s2 != LOBE_OK) {
+            out->status = s1 != LOBE_OK ? s1 : s2;
+            return;
+        }
+        memcpy(out->value, value, sizeof value);
+        out->incoming = wi;
+        out->pdf = pdf;
+        out->measure = LOBE_MEASURE_SOLID_ANGLE;
+        out->lobes = c->lobes;
+        out->eta_scale = 1.0f;
+        out->status = LOBE_OK;
+        return;
+    }
+    default:
+        return;
+    }
+}
+
+/* ---- mixtures (TransportLobesDevice.cuh:2262-2590, 2843-3080) ------------------------------ */
+static inline int lobe_compensated_add(float value, float* sum, float* correction) {
+    if (!bf_isfinite(value) || value < 0.0f) return 0;
+    const float next = *sum + value;
+    if (!bf_isfinite(next)) return 0;
+    *correction += *sum >= value ? (*sum - next) + value : (value - next) + *sum;
+    if (!bf_isfinite(*correction)) return 0;
+    *sum = next;
+    return 1;
+}
+
+static inline int lobe_mixture_has_nonblack_weight(const lobe_mixture* m) {
+    for (uint32_t c = 0; c < m->active_count; ++c)
+        for (int k = 0; k < 4; ++k)
+            if (m->closures[c].weight[k] != 0.0f) return 1;
+    return 0;
+}
+
+static inline int lobe_filtered_mixture_eval(const lobe_mixture* m, const lobe_filter_state* f, v3 wo, v3 wi, float* out) {
+    if (!lobe_unit_vector(wo) || !lobe_unit_vector(wi)) return LOBE_INVALID;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t i = 0; i < m->active_count; ++i) {
+        float comp[4];
+        const int st = lobe_record_eval(&m->closures[i], f->allowed_directions[i], wo, wi, comp);
+        if (st != LOBE_OK) return st;
+        for (int k = 0; k < 4; ++k) {
+            if (!bf_isfinite(comp[k]) || comp[k] < 0.0f) return LOBE_INVALID;
+            acc[k] += comp[k];
+            if (!bf_isfinite(acc[k]) || acc[k] < 0.0f) return LOBE_NOT_REPRESENTABLE;
+        }
+    }
+    memcpy(out, acc, sizeof acc);
+    return LOBE_OK;
+}
+
+static inline int lobe_filtered_mixture_pdf(const lobe_mixture* m, const lobe_filter_state* f, v3 wo, v3 wi, float* out) {
+    if (!lobe_unit_vector(wo) || !lobe_unit_vector(wi)) return LOBE_INVALID;
+    if (m->active_count == 0) {
+        *out = 0.0f;
+        return LOBE_OK;
+    }
+    float conditional[BF_MAX_CLOSURES];
+    float maximum = 0.0f;
+    for (uint32_t i = 0; i < m->active_count; ++i) {
+        float p;
+        const int st = lobe_record_pdf(&m->closures[i], f->allowed_directions[i], wo, wi, &p);
+        if (st != LOBE_OK) return st;
+        if (!bf_isfinite(p) || p < 0.0f) return LOBE_NOT_REPRESENTABLE;
+        conditional[i] = p;
+        maximum = fmaxf(maximum, p);
+    }
+    if (m->active_count == 1) {
+        *out = conditional[0];
+        return LOBE_OK;
+    }
+    if (maximum == 0.0f) {
+        *out = 0.0f;
+        return LOBE_OK;
+    }
+    float sum = 0.0f, correction = 0.0f;
+    for (uint32_t i = 0; i < m->active_count; ++i) {
+        const float ratio = conditional[i] / maximum;
+        float contribution;
+        if (!lobe_checked_product(m->probabilities[i], ratio, &contribution) ||
+            !lobe_compensated_add(contribution, &sum, &correction))
+            return LOBE_NOT_REPRESENTABLE;
+    }
+    const float normalized = sum + correction;
+    if (!bf_isfinite(normalized) || !(normalized > 0.0f) || normalized > 1.0f) return LOBE_NOT_REPRESENTABLE;
+    float mixed;
+    if (!lobe_checked_product(normalized, maximum, &mixed)) return LOBE_NOT_REPRESENTABLE;
+    *out = mixed;
+    return LOBE_OK;
+}
+
+/* TransportLobesDevice.cuh:2618-2641 */
+static inline uint32_t lobe_depth_blocked(const uint32_t* limits, const uint32_t* counters, uint32_t event) {
+    uint32_t blocked = 0;
+    if ((event & BF_LOBE_DIFFUSE) && counters[0] >= limits[0]) blocked |= BF_LOBE_DIFFUSE;
+    if ((event & BF_LOBE_GLOSSY) && counters[1] >= limits[1]) blocked |= BF_LOBE_GLOSSY;
+    if ((event & BF_LOBE_SPECULAR) && counters[2] >= limits[2]) blocked |= BF_LOBE_SPECULAR;
+    if ((event & BF_LOBE_TRANSMISSION) && counters[3] >= limits[3]) blocked |= BF_LOBE_TRANSMISSION;
+    return blocked;
+}
+
+/* TransportLobesDevice.cuh:2643-2718 */
+static inline int lobe_build_filtered_distribution(lobe_mixture* m) {
+    if (m->active_count == 0) return 1;
+    if (m->active_count == 1) {
+        m->probabilities[0] = 1.0f;
+        m->cdf[0] = 0.0f;
+        m->cdf[1] = 1.0f;
+        return 1;
+    }
+    float total_sum = 0.0f, total_correction = 0.0f;
+    for (uint32_t i = 0; i < m->active_count; ++i)
+        if (!lobe_compensated_add(m->probabilities[i], &total_sum, &total_correction)) return 0;
+    const float total = total_sum + total_correction;
+    if (!bf_isfinite(total) || !(total > 0.0f)) return 0;
+    const uint64_t scale = 1ull << 24;
+    uint64_t masses[BF_MAX_CLOSURES] = {0};
+    float remainders[BF_MAX_CLOSURES] = {0};
+    uint64_t base_mass = 0;
+    for (uint32_t i = 0; i < m->active_count; ++i) {
+        const float normalized = m->probabilities[i] / total;
+        if (!bf_isfinite(normalized) || !(normalized > 0.0f) || !(normalized < 1.0f)) return 0;
+        const float scaled = bf_ldexpf(normalized, 24);
+        const float base = floorf(scaled);
+        if (!bf_isfinite(scaled) || !(scaled > 0.0f) || base < 0.0f || base > (float)scale) return 0;
+        masses[i] = (uint64_t)base;
+        remainders[i] = scaled - base;
+        if (!bf_isfinite(remainders[i]) || remainders[i] < 0.0f || !(remainders[i] < 1.0f) || base_mass > scale - masses[i])
+            return 0;
+        base_mass += masses[i];
+    }
+    if (base_mass > scale) return 0;
+    float rsum = 0.0f, rcorr = 0.0f;
+    for (uint32_t i = 0; i < m->active_count; ++i)
+        if (!lobe_compensated_add(remainders[i], &rsum, &rcorr)) return 0;
+    const uint64_t missing = scale - base_mass;
+    const float rtotal = rsum + rcorr;
+    if (missing > m->active_count || fabsf(rtotal - (float)missing) > 0.5f + 0x1p-23f * 16.0f) return 0;
+    int received[BF_MAX_CLOSURES] = {0};
+    for (uint64_t unit = 0; unit < missing; ++unit) {
+        uint32_t selected = m->active_count;
+        for (uint32_t i = 0; i < m->active_count; ++i) {
+            if (received[i] || !(remainders[i] > 0.0f)) continue;
+            if (selected == m->active_count || remainders[i] > remainders[selected]) selected = i;
+        }
+        if (selected == m->active_count) return 0;
+        ++masses[selected];
+        received[selected] = 1;
+    }
+    uint64_t cumulative = 0;
+    m->cdf[0] = 0.0f;
+    for (uint32_t i = 0; i < m->active_count; ++i) {
+        if (masses[i] == 0 || cumulative > scale - masses[i]) return 0;
+        cumulative += masses[i];
+        m->probabilities[i] = bf_ldexpf((float)masses[i], -24);
+        m->cdf[i + 1] = bf_ldexpf((float)cumulative, -24);
+    }
+    return cumulative == scale && m->cdf[m->active_count] == 1.0f;
+}
+
+/* TransportLobesDevice.cuh:2720-2795 */
+static inline void lobe_filter_mixture_by_depth(lobe_mixture* m, const uint32_t* limits, const uint32_t* counters,
+                                                uint32_t expected_total, lobe_filter_state* r) {
+    memset(r, 0, sizeof *r);
+    r->source_count = m->active_count;
+    const uint64_t total = (uint64_t)counters[0] + counters[1] + counters[2] + counters[4];
+    const uint64_t surface = (uint64_t)counters[0] + counters[1] + counters[2];
+    if (total != expected_total || counters[3] > surface || counters[0] > limits[0] || counters[1] > limits[1] ||
+        counters[2] > limits[2] || counters[3] > limits[3] || counters[4] > limits[4]) {
+        r->status = LOBE_INVALID;
+        return;
+    }
+    uint32_t filtered = 0;
+    for (uint32_t si = 0; si < r->source_count; ++si) {
+        const closure_rec src = m->closures[si];
+        const float prob = m->probabilities[si];
+        uint32_t allowed = 0;
+        if (src.kind == BF_CLOSURE_ROUGH_DIELECTRIC) {
+            const uint32_t refl = lobe_depth_blocked(limits, counters, BF_LOBE_GLOSSY | BF_LOBE_REFLECTION);
+            const uint32_t trans = lobe_depth_blocked(limits, counters, BF_LOBE_GLOSSY | BF_LOBE_TRANSMISSION);
+            if (refl == 0) allowed |= BF_LOBE_REFLECTION;
+            else r->blocked_lobes |= refl;
+            if (trans == 0) allowed |= BF_LOBE_TRANSMISSION;
+            else r->blocked_lobes |= trans;
+        } else {
+            if (!lobe_valid_surface_event(src.lobes)) {
+                r->status = LOBE_INVALID;
+                return;
+            }
+            const uint32_t blocked = lobe_depth_blocked(limits, counters, src.lobes);
+            if (blocked == 0) allowed = src.lobes & (BF_LOBE_REFLECTION | BF_LOBE_TRANSMISSION);
+            else r->blocked_lobes |= blocked;
+        }
+        if (allowed == 0) continue;
+        m->closures[filtered] = src;
+        m->probabilities[filtered] = prob;
+        r->source_indices[filtered] = si;
+        r->allowed_directions[filtered] = allowed;
+        ++filtered;
+    }
+    m->active_count = filtered;
+    if (filtered == r->source_count) return;
+    for (uint32_t i = filtered; i < BF_MAX_CLOSURES; ++i) {
+        memset(&m->closures[i], 0, sizeof m->closures[i]);
+        m->probabilities[i] = 0.0f;
+    }
+    for (uint32_t i = 0; i <= BF_MAX_CLOSURES; ++i) m->cdf[i] = 0.0f;
+    if (!lobe_build_filtered_distribution(m)) r->status = LOBE_NOT_REPRESENTABLE;
+}
+
+/* TransportLobesDevice.cuh:2979-3062 */
+static inline void lobe_sample_filtered_mixture(const lobe_mixture* m, const lobe_filter_state* f, v3 wo, float component,
+                                                float u, float v, lobe_mixture_sample* out) {
+    memset(out, 0, sizeof *out);
+    out->eta_scale = 1.0f;
+    out->measure = LOBE_MEASURE_SOLID_ANGLE;
+    if (!lobe_unit_vector(wo) || !bf_isfinite(component) || component < 0.0f || !(component < 1.0f) || !bf_isfinite(u) ||
+        u < 0.0f || !(u < 1.0f) || !bf_isfinite(v) || v < 0.0f || !(v < 1.0f)) {
+        out->status = LOBE_INVALID;
+        return;
+    }
+    if (m->active_count == 0) {
+        out->status = LOBE_OUTSIDE_SUPPORT;
+        return;
+    }
+    uint32_t selected = m->active_count;
+    for (uint32_t i = 0; i < m->active_count; ++i) {
+        if (component < m->cdf[i + 1]) {
+            selected = i;
+            break;
+        }
+    }
+    if (selected == m->active_count) {
+        out->status = LOBE_NOT_REPRESENTABLE;
+        return;
+    }
+    const float event = (component - m->cdf[selected]) / m->probabilities[selected];
+    if (!bf_isfinite(event) || event < 0.0f || !(event < 1.0f)) {
+        out->status = LOBE_NOT_REPRESENTABLE;
+        return;
+    }
+    lobe_sample s;
+    lobe_record_sample(&m->closures[selected], f->allowed_directions[selected], wo, event, u, v, &s);
+    if (s.status != LOBE_OK) {
+        out->status = s.status;
+        return;
+    }
+    if (m->active_count == 1) {
+        memcpy(out->value, s.value, sizeof s.value);
+        out->incoming = s.incoming;
+        out->pdf = s.pdf;
+        out->measure = s.measure;
+        out->selected = f->source_indices[0];
+        out->lobes = s.lobes;
+        out->eta_scale = s.eta_scale;
+        out->status = LOBE_OK;
+        return;
+    }
+    if (s.measure == LOBE_MEASURE_SOLID_ANGLE) {
+        float value[4], pdf;
+        const int st1 = lobe_filtered_mixture_eval(m, f, wo, s.incoming, value);
+        const int st2 = lobe_filtered_mixture_pdf(m, f, wo, s.incoming, &pdf);
+        if (st1 != LOBE_OK || st2 != LOBE_OK) {
+            out->status = st1 != LOBE_OK ? st1 : st2;
+            return;
+        }
+        memcpy(out->value, value, sizeof value);
+        out->pdf = pdf;
+        out->measure = LOBE_MEASURE_SOLID_ANGLE;
+    } else {
+        out->status = LOBE_NOT_REPRESENTABLE; /* delta atoms: not exercised by the supported closures */
+        return;
+    }
+    out->incoming = s.incoming;
+    out->selected = f->source_indices[selected];
+    out->lobes = s.lobes;
+    out->eta_scale = s.eta_scale;
+    out->status = LOBE_OK;
+}
+
+#endif

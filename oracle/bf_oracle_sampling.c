@@ -1,0 +1,196 @@
+/*
+ * ORACLE — test infrastructure only (see bf_oracle.h).
+ * Sampling, Sobol–Owen and the CIE 1931 sensor restated from the reference renderer library.
+ */
+#include "bf_oracle_internal.h"
+
+#include <pthread.h>
+
+/* Blackframe/Renderer/Include/Blackframe/Renderer/SampleStream.hpp:23-29 */
+uint64_t bfo_mix_bits(uint64_t value) {
+    value ^= value >> 30;
+    value *= 0xBF58476D1CE4E5B9ull;
+    value ^= value >> 27;
+    value *= 0x94D049BB133111EBull;
+    return value ^ (value >> 31);
+}
+
+/* SampleStream.hpp:31-39 */
+uint64_t bfo_indexed_bits(const bf_sample_stream_index* index, uint64_t dimension) {
+    const uint64_t packed_pixel = ((uint64_t)index->pixel_x << 32) | (uint64_t)index->pixel_y;
+    uint64_t state = bfo_mix_bits(index->seed ^ 0x9E3779B97F4A7C15ull);
+    state = bfo_mix_bits(state ^ packed_pixel ^ 0xD1B54A32D192ED03ull);
+    state = bfo_mix_bits(state ^ index->sample_index ^ 0x8CB92BA72F3D8DD7ull);
+    return bfo_mix_bits(state ^ dimension);
+}
+
+/* ---- Sobol (Blackframe/Renderer/Source/SobolSampling.cpp:53-117) ---------------------------- */
+typedef struct {
+    uint32_t degree;
+    uint32_t a;
+    uint32_t m[18];
+} joe_kuo_row;
+
+static const joe_kuo_row joe_kuo_rows[] = {
+#include "../blackframe-renderer_amd/csrc/data/sobol_joe_kuo.inc"
+};
+#define BFO_SOBOL_DIMENSIONS ((uint64_t)(sizeof(joe_kuo_rows) / sizeof(joe_kuo_rows[0]) + 1u))
+
+static uint64_t* sobol_directions; /* [dimension][64] */
+static pthread_once_t sobol_once = PTHREAD_ONCE_INIT;
+
+static void sobol_init(void) {
+    sobol_directions = (uint64_t*)calloc(BFO_SOBOL_DIMENSIONS * 64u, sizeof(uint64_t));
+    for (uint32_t column = 0; column < 64u; ++column) sobol_directions[column] = 1ull << (63u - column);
+    for (uint64_t dimension = 1; dimension < BFO_SOBOL_DIMENSIONS; ++dimension) {
+        const joe_kuo_row* row = &joe_kuo_rows[dimension - 1];
+        uint64_t* d = sobol_directions + dimension * 64u;
+        const uint32_t s = row->degree;
+        for (uint32_t column = 0; column < s; ++column) d[column] = (uint64_t)row->m[column] << (63u - column);
+        for (uint32_t column = s; column < 64u; ++column) {
+            uint64_t direction = d[column - s] ^ (d[column - s] >> s);
+            for (uint32_t k = 1; k < s; ++k) {
+                if (((row->a >> (s - 1u - k)) & 1u) != 0u) direction ^= d[column - k];
+            }
+            d[column] = direction;
+        }
+    }
+}
+
+uint64_t bfo_sobol_dimension_count(void) { return BFO_SOBOL_DIMENSIONS; }
+
+uint64_t bfo_sobol_bits(uint64_t sample_index, uint64_t sobol_dimension) {
+    pthread_once(&sobol_once, sobol_init);
+    if (sobol_dimension >= BFO_SOBOL_DIMENSIONS) return 0;
+    uint64_t gray = sample_index ^ (sample_index >> 1);
+    const uint64_t* d = sobol_directions + sobol_dimension * 64u;
+    uint64_t sample = 0;
+    while (gray != 0) {
+        sample ^= d[__builtin_ctzll(gray)];
+        gray &= gray - 1u;
+    }
+    return sample;
+}
+
+/* Blackframe/Renderer/Source/OwenScrambling.cpp:8-43 */
+static uint64_t owen_tree_key(const bf_sample_stream_index* index, uint64_t sobol_dimension) {
+    const uint64_t packed_pixel = ((uint64_t)index->pixel_x << 32) | (uint64_t)index->pixel_y;
+    const uint64_t dimension_key = bfo_mix_bits(index->seed ^ bfo_mix_bits(sobol_dimension + 1u));
+    return dimension_key ^ bfo_mix_bits(packed_pixel);
+}
+
+static uint64_t nested_binary_scramble(uint64_t input, uint64_t tree_key, uint32_t depth_count) {
+    uint64_t prefix = 0, output = 0;
+    for (uint32_t depth = 0; depth < depth_count; ++depth) {
+        const uint32_t shift = 63u - depth;
+        const uint64_t bit = (input >> shift) & 1u;
+        const uint64_t node_id = (1ull << depth) | prefix;
+        const uint64_t flip = bfo_mix_bits(tree_key ^ node_id) >> 63;
+        output |= (bit ^ flip) << shift;
+        prefix = (prefix << 1) | bit;
+    }
+    return output;
+}
+
+uint64_t bfo_owen_bits(const bf_sample_stream_index* index, uint64_t sobol_dimension) {
+    return nested_binary_scramble(bfo_sobol_bits(index->sample_index, sobol_dimension),
+                                  owen_tree_key(index, sobol_dimension), 64u);
+}
+
+/*
+ * Transport-dimension -> sampler mapping. The indexed hash follows
+ * Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/SampleStreamDevice.cuh:29-32. The Sobol–Owen
+ * backend (no reference transport counterpart) keeps the same addressing: the two raster tags
+ * (Blackframe/Renderer/Include/Blackframe/Renderer/SampleDimensionMap.hpp:34-35) map to Sobol
+ * dimensions 0 and 1, every other transport dimension d to Sobol dimension d + 2.
+ */
+float bfo_sample_1d(uint32_t sampler, const bf_sample_stream_index* index, uint64_t dimension) {
+    uint64_t bits;
+    if (sampler == BF_SAMPLER_SOBOL_OWEN) {
+        uint64_t sobol_dimension;
+        if (dimension == BFO_DIM_RASTER_X) sobol_dimension = 0;
+        else if (dimension == BFO_DIM_RASTER_Y) sobol_dimension = 1;
+        else sobol_dimension = dimension + 2u;
+        if (sobol_dimension >= BFO_SOBOL_DIMENSIONS) return -1.0f;
+        bits = bfo_owen_bits(index, sobol_dimension);
+    } else {
+        bits = bfo_indexed_bits(index, dimension);
+    }
+    return (float)(bits >> 40) * 0x1p-24f;
+}
+
+/* ---- CIE 1931 sensor (Blackframe/Renderer/Source/Cie1931Sensor.cpp:11-165) ----------------- */
+static const double cie_table[471][3] = {
+#include "../blackframe-renderer_amd/csrc/data/cie1931_cmf.inc"
+};
+
+double bfo_cie_y_integral(void) {
+    double integral = 0.0;
+    for (int i = 0; i + 1 < 471; ++i) integral += (cie_table[i][1] + cie_table[i + 1][1]) / 2.0;
+    return integral;
+}
+
+static uint64_t fingerprint_word(uint64_t hash, uint64_t word) {
+    for (uint32_t shift = 0; shift < 64; shift += 8) {
+        hash ^= (word >> shift) & 0xFFu;
+        hash *= 1099511628211ull;
+    }
+    return hash;
+}
+
+uint64_t bfo_cie_fingerprint(void) {
+    uint64_t hash = 14695981039346656037ull;
+    for (uint64_t i = 0; i < 471; ++i) {
+        hash = fingerprint_word(hash, i);
+        for (int c = 0; c < 3; ++c) hash = fingerprint_word(hash, bf_double_bits(cie_table[i][c]));
+    }
+    return hash;
+}
+
+/* Cie1931Sensor.cpp:69-93 (TransportScalar instantiation) */
+static void cmf(float wavelength, float out[3]) {
+    const float coordinate = wavelength - 360.0f;
+    const size_t lower_index = (size_t)coordinate;
+    if (lower_index + 1 >= 471) {
+        for (int c = 0; c < 3; ++c) out[c] = (float)cie_table[470][c];
+        return;
+    }
+    const float fraction = coordinate - (float)lower_index;
+    for (int c = 0; c < 3; ++c) {
+        const float lower = (float)cie_table[lower_index][c];
+        const float upper = (float)cie_table[lower_index + 1][c];
+        out[c] = fmaf(upper - lower, fraction, lower);
+    }
+}
+
+/* Cie1931Sensor.cpp:104-124 */
+static float scaled_sensor_term(float contribution, float response, float normalization, float probability) {
+    if (contribution == 0.0f || response == 0.0f) return 0.0f;
+    int ce, re, ne, pe;
+    const float cm = bf_frexpf(contribution, &ce);
+    const float rm = bf_frexpf(response, &re);
+    const float nm = bf_frexpf(normalization, &ne);
+    const float pm = bf_frexpf(probability, &pe);
+    const float mantissa = cm * rm * nm / pm;
+    return bf_ldexpf(mantissa, ce + re + ne - pe);
+}
+
+/* Cie1931Sensor.cpp:126-165: equal pdf on every lane (uniform visible-wavelength sampling) */
+void bfo_cie_xyz(const float radiance[4], const float wavelengths[4], float pdf, float xyz[3]) {
+    static float normalization = 0.0f;
+    if (normalization == 0.0f) normalization = (1.0f / 4.0f) / (float)bfo_cie_y_integral();
+    float sum[3] = {0.0f, 0.0f, 0.0f};
+    for (int lane = 0; lane < 4; ++lane) {
+        float response[3];
+        cmf(wavelengths[lane], response);
+        for (int c = 0; c < 3; ++c) sum[c] += scaled_sensor_term(radiance[lane], response[c], normalization, pdf);
+    }
+    for (int c = 0; c < 3; ++c) xyz[c] = sum[c];
+}
+
+/* Blackframe/Renderer/Include/Blackframe/Renderer/Color.hpp:77-87 */
+void bfo_xyz_to_rgb(const float xyz[3], float rgb[3]) {
+    rgb[0] = 3.2404542f * xyz[0] - 1.5371385f * xyz[1] - 0.4985314f * xyz[2];
+    rgb[1] = -0.9692660f * xyz[0] + 1.8760108f * xyz[1] + 0.0415560f * xyz[2];
+    rgb[2] = 0.0556434f * xyz[0] - 0.2040259f * xyz[1] + 1.0572252f * xyz[2];
+}

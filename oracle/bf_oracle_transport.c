@@ -1,0 +1,1129 @@
+/*
+ * ORACLE — test infrastructure only (see bf_oracle.h).
+ * Camera, shading, NEE/MIS, Russian roulette and the 7-queue CPU wavefront, restated from
+ *   Blackframe/XPU/CUDA/Source/CudaWavefrontStages.cu        (stage kernels)
+ *   Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/TransportDevice.cuh      (scalar helpers)
+ *   Blackframe/XPU/CUDA/Include/Blackframe/XPU/CUDA/TransportLobesDevice.cuh (closures)
+ *   Blackframe/Backends/GPU/CUDA/Source/WavefrontTransport.cpp:1983-2833     (queue routing)
+ *   Blackframe/Renderer/Include/Blackframe/Renderer/{PinholeCamera,PixelJitter,WavelengthSampling}.hpp
+ */
+#include <pthread.h>
+#include <stdio.h>
+
+#include "bf_oracle_internal.h"
+#include "bf_oracle_lobes.h"
+
+/* ---- path lane ------------------------------------------------------------------------------ */
+typedef struct {
+    bf_sample_stream_index sample;
+    bf_transport_ray ray;
+    float beta[4], L[4], wl[4], wl_pdf;
+    uint32_t diffuse, glossy, specular, transmission, volume, depth;
+    float eta_scale;
+    uint32_t delta_flags;
+    /* closest hit */
+    bf_closest_hit hit;
+    uint32_t hit_instance, hit_geometry;
+    /* previous continuous BSDF sample (WavefrontStageKernel.hpp:170-182) */
+    uint32_t prev_valid;
+    float prev_ctx[3];
+    float prev_pdf;
+    /* pending shadow (WavefrontStageKernel.hpp:153-166) */
+    bf_transport_ray shadow_ray;
+    float p_beta[4], p_bsdf[4], p_L[4];
+    float p_cos, p_weight, p_sel, p_cond, p_corr;
+    uint32_t p_lambert, cont_pending, pending_term;
+    uint32_t termination, failed, fail_code;
+    uint32_t route;
+    uint32_t light_sampled, closure_sampled;
+} lane;
+
+enum { ROUTE_NONE = 0, ROUTE_RAY = 1, ROUTE_HIT = 2, ROUTE_MISS = 3, ROUTE_SHADE = 4, ROUTE_SHADOW = 5, ROUTE_CONT = 6, ROUTE_TERM = 7 };
+
+typedef struct {
+    const bfo_scene* scene;
+    const bf_frame_desc* frame;
+    const bf_transport_options* options;
+    lane* lanes;
+} ctx;
+
+static void fail(lane* l, uint32_t code) {
+    l->failed = 1;
+    l->fail_code = code;
+    l->termination = BF_TERM_FAILED;
+    l->route = ROUTE_TERM;
+}
+
+/* ---- spectra at the path's wavelengths (backend design: no reference counterpart) ----------- */
+static float spectrum_at(const bfo_scene* s, uint32_t id, float wavelength) {
+    if (id == BF_INVALID_INDEX) return 0.0f;
+    const bf_spectrum_desc* sp = &s->spectra[id];
+    const float* v = s->spectrum_values + sp->offset;
+    if (sp->count <= 1) return v[0];
+    const float step = 470.0f / (float)(sp->count - 1);
+    const float t = (wavelength - 360.0f) / step;
+    uint32_t i = t <= 0.0f ? 0u : (uint32_t)t;
+    if (i > sp->count - 2) i = sp->count - 2;
+    const float f = t - (float)i;
+    return fmaf(v[i + 1] - v[i], f, v[i]);
+}
+
+static void load_mixture(const bfo_scene* s, const bf_material_desc* m, const float* wl, lobe_mixture* mix) {
+    memset(mix, 0, sizeof *mix);
+    mix->active_count = m->closure_count;
+    for (uint32_t c = 0; c < m->closure_count; ++c) {
+        const bf_closure_desc* d = &s->closures[m->closure_offset + c];
+        closure_rec* r = &mix->closures[c];
+        r->kind = d->kind;
+        r->lobes = d->lobes;
+        for (int k = 0; k < 10; ++k) r->parameters[k] = d->parameters[k];
+        for (int lane = 0; lane < 4; ++lane) {
+            r->weight[lane] = spectrum_at(s, d->weight_spectrum, wl[lane]);
+            if (d->kind == BF_CLOSURE_ROUGH_CONDUCTOR) {
+                r->parameters[lane] = spectrum_at(s, d->eta_spectrum, wl[lane]);
+                r->parameters[4 + lane] = spectrum_at(s, d->k_spectrum, wl[lane]);
+            }
+        }
+        mix->probabilities[c] = m->probabilities[c];
+    }
+    for (uint32_t c = 0; c <= BF_MAX_CLOSURES; ++c) mix->cdf[c] = m->cdf[c];
+}
+
+/* ---- camera (PinholeCamera.hpp:140-301, PixelJitter.hpp:37-63) ------------------------------ */
+static int robust_unit(v3 d, v3* out) {
+    if (!v3_finite(d)) return 0;
+    const float m = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+    if (m == 0.0f) return 0;
+    const v3 s = v3_make(d.x / m, d.y / m, d.z / m);
+    const float mag = sqrtf(s.x * s.x + s.y * s.y + s.z * s.z);
+    *out = v3_make(s.x / mag, s.y / mag, s.z / mag);
+    return v3_finite(*out);
+}
+
+int bfo_camera_ray(const bf_camera_desc* cam, uint32_t sampler, const bf_sample_stream_index* index,
+                   bf_transport_ray* ray) {
+    float ox = 0.5f, oy = 0.5f;
+    if (cam->jitter_mode == BF_JITTER_UNIFORM) {
+        ox = bfo_sample_1d(sampler, index, BFO_DIM_RASTER_X);
+        oy = bfo_sample_1d(sampler, index, BFO_DIM_RASTER_Y);
+    }
+    if (index->pixel_x >= cam->width || index->pixel_y >= cam->height || !(ox >= 0.0f) || !(ox < 1.0f) ||
+        !(oy >= 0.0f) || !(oy < 1.0f))
+        return 0;
+    const float w = (float)cam->width, h = (float)cam->height;
+    const float tx = 2.0f * (float)index->pixel_x;
+    const float ty = 2.0f * (float)index->pixel_y;
+    const float nx = (tx + 2.0f * ox) / w - 1.0f;
+    const float ny = 1.0f - (ty + 2.0f * oy) / h;
+    v3 unit, world;
+    if (!robust_unit(v3_make(nx * cam->half_width, ny * cam->half_height, -1.0f), &unit)) return 0;
+    const v3 wd = v3_make(cam->tangent[0] * unit.x + cam->bitangent[0] * unit.y + cam->normal[0] * unit.z,
+                          cam->tangent[1] * unit.x + cam->bitangent[1] * unit.y + cam->normal[1] * unit.z,
+                          cam->tangent[2] * unit.x + cam->bitangent[2] * unit.y + cam->normal[2] * unit.z);
+    if (!robust_unit(wd, &world)) return 0;
+    memset(ray, 0, sizeof *ray);
+    ray->origin_x = cam->origin[0];
+    ray->origin_y = cam->origin[1];
+    ray->origin_z = cam->origin[2];
+    ray->t_min = cam->t_min;
+    ray->direction_x = world.x;
+    ray->direction_y = world.y;
+    ray->direction_z = world.z;
+    ray->t_max = cam->t_max;
+    ray->time = cam->time;
+    ray->visibility_mask = cam->visibility_mask;
+    return 1;
+}
+
+/* WavelengthSampling.hpp:62-84 */
+static void sample_wavelengths(float u, float* wl) {
+    for (int lane = 0; lane < 4; ++lane) {
+        const float offset = (float)lane / 4.0f;
+        const float wrap = 1.0f - offset;
+        const float shifted = lane != 0 && u >= wrap ? u - wrap : u + offset;
+        wl[lane] = fmaf(470.0f, shifted, 360.0f);
+    }
+}
+
+static void camera_lane(const ctx* c, lane* l, uint64_t global_path) {
+    const bf_frame_desc* f = c->frame;
+    const uint64_t pixels = (uint64_t)f->camera.width * f->camera.height;
+    const uint64_t s = global_path / pixels;
+    const uint64_t rem = global_path % pixels;
+    memset(l, 0, sizeof *l);
+    l->sample.pixel_x = (uint32_t)(rem % f->camera.width);
+    l->sample.pixel_y = (uint32_t)(rem / f->camera.width);
+    l->sample.sample_index = f->first_sample_index + s;
+    l->sample.seed = f->seed;
+    if (!bfo_camera_ray(&f->camera, c->options->sampler, &l->sample, &l->ray)) {
+        fail(l, 1);
+        return;
+    }
+    if (c->options->wavelength_mode == BF_WAVELENGTHS_PER_PATH) {
+        const float u = bfo_sample_1d(c->options->sampler, &l->sample, BFO_DIM_WAVELENGTH);
+        if (!(u >= 0.0f) || !(u < 1.0f)) {
+            fail(l, 2);
+            return;
+        }
+        sample_wavelengths(u, l->wl);
+    } else {
+        for (int k = 0; k < 4; ++k) l->wl[k] = c->options->packet_wavelengths[k];
+    }
+    l->wl_pdf = 1.0f / 470.0f;
+    for (int k = 0; k < 4; ++k) l->beta[k] = 1.0f;
+    l->eta_scale = 1.0f;
+    l->route = ROUTE_RAY;
+}
+
+/* ---- geometry helpers (CudaWavefrontStages.cu) ---------------------------------------------- */
+/* :326-349 */
+static int normalize3(v3 v, v3* out) {
+    if (!v3_finite(v)) return 0;
+    const float m = fmaxf(fabsf(v.x), fmaxf(fabsf(v.y), fabsf(v.z)));
+    if (!bf_isfinite(m) || m == 0.0f) return 0;
+    const v3 s = v3_make(v.x / m, v.y / m, v.z / m);
+    const float mag = sqrtf(v3_dot(s, s));
+    if (!bf_isfinite(mag) || !(mag > 0.0f)) return 0;
+    *out = v3_make(s.x / mag, s.y / mag, s.z / mag);
+    return v3_finite(*out);
+}
+
+typedef struct {
+    v3 scaled, direction;
+    float scale, scaled_distance_cubed, length;
+} segment_t;
+
+/* :351-388 */
+static int scaled_segment(v3 disp, segment_t* seg) {
+    if (!v3_finite(disp)) return 0;
+    const float scale = fmaxf(fabsf(disp.x), fmaxf(fabsf(disp.y), fabsf(disp.z)));
+    if (!bf_isfinite(scale) || !(scale > 0.0f)) return 0;
+    const v3 scaled = v3_make(disp.x / scale, disp.y / scale, disp.z / scale);
+    const float sq = v3_dot(scaled, scaled);
+    const float sl = sqrtf(sq);
+    const float cubed = sq * sl;
+    const float length = scale * sl;
+    if (!v3_finite(scaled) || !bf_isfinite(sq) || !(sq > 0.0f) || !bf_isfinite(sl) || !(sl > 0.0f) ||
+        !bf_isfinite(cubed) || !(cubed > 0.0f) || !bf_isfinite(length) || !(length > 0.0f))
+        return 0;
+    const v3 dir = v3_mul(scaled, 1.0f / sl);
+    if (!v3_finite(dir) || !lobe_unit_vector(dir)) return 0;
+    seg->scaled = scaled;
+    seg->direction = dir;
+    seg->scale = scale;
+    seg->scaled_distance_cubed = cubed;
+    seg->length = length;
+    return 1;
+}
+
+/* :390-419 */
+static int light_surface_alignment(v3 n, v3 scaled_to_light, float* absolute, int* supported) {
+    const v3 out = v3_mul(scaled_to_light, -1.0f);
+    const float signed_alignment = v3_dot(n, out);
+    const int ortho = (n.x == 0.0f || out.x == 0.0f) && (n.y == 0.0f || out.y == 0.0f) && (n.z == 0.0f || out.z == 0.0f);
+    if (signed_alignment == 0.0f && ortho) {
+        *absolute = 0.0f;
+        *supported = 0;
+        return 1;
+    }
+    const float sum = fmaf(fabsf(n.x), fabsf(out.x), fmaf(fabsf(n.y), fabsf(out.y), fabsf(n.z) * fabsf(out.z)));
+    const float uncertainty = fmaf(4.0f * BFO_FLOAT_EPSILON, sum, 4.0f * BFO_DENORM_MIN);
+    *absolute = fabsf(signed_alignment);
+    if (!bf_isfinite(signed_alignment) || !bf_isfinite(*absolute) || !bf_isfinite(sum) || !bf_isfinite(uncertainty) ||
+        *absolute <= uncertainty)
+        return 0;
+    *supported = signed_alignment > 0.0f;
+    return 1;
+}
+
+/* :421-465 */
+static int one_sided_emission_support(v3 n, v3 wo, int* supported) {
+    const float sq = v3_dot(n, n);
+    if (!v3_finite(n) || !bf_isfinite(sq) || fabsf(sq - 1.0f) > 128.0f * BFO_FLOAT_EPSILON || !v3_finite(wo)) return 0;
+    const float m = fmaxf(fabsf(wo.x), fmaxf(fabsf(wo.y), fabsf(wo.z)));
+    if (!bf_isfinite(m) || m == 0.0f) return 0;
+    const v3 s = v3_make(wo.x / m, wo.y / m, wo.z / m);
+    const float alignment = v3_dot(n, s);
+    if (!bf_isfinite(alignment)) return 0;
+    const int ortho = (n.x == 0.0f || wo.x == 0.0f) && (n.y == 0.0f || wo.y == 0.0f) && (n.z == 0.0f || wo.z == 0.0f);
+    if (alignment == 0.0f && ortho) {
+        *supported = 0;
+        return 1;
+    }
+    const float sum = fmaf(fabsf(n.x), fabsf(s.x), fmaf(fabsf(n.y), fabsf(s.y), fabsf(n.z) * fabsf(s.z)));
+    const float uncertainty = fmaf(4.0f * BFO_FLOAT_EPSILON, sum, 4.0f * BFO_DENORM_MIN);
+    if (!bf_isfinite(sum) || !bf_isfinite(uncertainty) || fabsf(alignment) <= uncertainty) return 0;
+    *supported = alignment > 0.0f;
+    return 1;
+}
+
+typedef struct {
+    v3 position, error;
+} positioned_t;
+
+/* :1078-1110 */
+static int interpolate_position_with_error(const v3* v, v3 b, positioned_t* out) {
+    float scale = 0.0f;
+    for (uint32_t a = 0; a < 3; ++a) {
+        const float first = v3_comp(v[0], a);
+        scale = fmaxf(scale, fmaxf(fabsf(v3_comp(v[1], a) - first), fabsf(v3_comp(v[2], a) - first)));
+    }
+    const float floor_value = BFO_FLOAT_EPSILON * scale;
+    if (!bf_isfinite(scale) || !bf_isfinite(floor_value) || !(floor_value > 0.0f)) return 0;
+    const float gamma7 = (7.0f * BFO_FLOAT_EPSILON) / (1.0f - 7.0f * BFO_FLOAT_EPSILON);
+    for (uint32_t a = 0; a < 3; ++a) {
+        const float p0 = b.x * v3_comp(v[0], a);
+        const float p1 = b.y * v3_comp(v[1], a);
+        const float p2 = b.z * v3_comp(v[2], a);
+        const float position = fmaf(b.x, v3_comp(v[0], a), fmaf(b.y, v3_comp(v[1], a), p2));
+        const float magnitude = fabsf(p0) + fabsf(p1) + fabsf(p2);
+        const float error = fmaxf(gamma7 * magnitude, floor_value);
+        if (!bf_isfinite(position) || !bf_isfinite(magnitude) || !bf_isfinite(error) || error < 0.0f) return 0;
+        v3_set(&out->position, a, position);
+        v3_set(&out->error, a, error);
+    }
+    return 1;
+}
+
+/* :1112-1150 */
+static int transform_position_with_error(const float* m, const positioned_t* obj, v3 traversal, positioned_t* out) {
+    const double eps = (double)BFO_FLOAT_EPSILON;
+    const double gamma7 = (7.0 * eps) / (1.0 - 7.0 * eps);
+    out->position = bfo_transform_point(m, obj->position);
+    if (!v3_finite(out->position) || !v3_finite(traversal)) return 0;
+    for (uint32_t row = 0; row < 3; ++row) {
+        double mag = fabs((double)m[row * 4 + 3]);
+        double prop = 0.0;
+        for (uint32_t col = 0; col < 3; ++col) {
+            const double coefficient = fabs((double)m[row * 4 + col]);
+            mag += coefficient * fabs((double)v3_comp(obj->position, col));
+            prop += coefficient * (double)v3_comp(obj->error, col);
+        }
+        const double discrepancy = fabs((double)v3_comp(out->position, row) - (double)v3_comp(traversal, row));
+        const double bound = gamma7 * mag + (1.0 + gamma7) * prop + discrepancy;
+        if (!bf_isfinited(bound) || bound < 0.0 || bound > (double)3.40282346638528859812e+38f) return 0;
+        float r = (float)bound;
+        if ((double)r < bound) r = bf_next_up_f(r);
+        if (!bf_isfinite(r) || r < 0.0f || (bound > 0.0 && r == 0.0f)) return 0;
+        v3_set(&out->error, row, r);
+    }
+    return 1;
+}
+
+/* :2596-2629 */
+static int offset_point(v3 p, v3 err, v3 n, v3 w, v3* out) {
+    if (!v3_finite(p) || !v3_finite(err) || err.x < 0.0f || err.y < 0.0f || err.z < 0.0f || !v3_finite(n) || !v3_finite(w))
+        return 0;
+    const v3 oriented = v3_dot(n, w) >= 0.0f ? n : v3_mul(n, -1.0f);
+    const float distance = fabsf(n.x) * err.x + fabsf(n.y) * err.y + fabsf(n.z) * err.z;
+    if (!bf_isfinite(distance) || !(distance > 0.0f)) return 0;
+    const v3 disp = v3_mul(oriented, distance);
+    v3 o = v3_add(p, disp);
+    if (disp.x != 0.0f) o.x = disp.x > 0.0f ? bf_next_up_f(o.x) : bf_next_down_f(o.x);
+    if (disp.y != 0.0f) o.y = disp.y > 0.0f ? bf_next_up_f(o.y) : bf_next_down_f(o.y);
+    if (disp.z != 0.0f) o.z = disp.z > 0.0f ? bf_next_up_f(o.z) : bf_next_down_f(o.z);
+    *out = o;
+    return v3_finite(o) && (o.x != p.x || o.y != p.y || o.z != p.z);
+}
+
+typedef struct {
+    v3 t, b, n;
+} frame_t;
+
+/* :2692-2703 */
+static int orthonormal_frame(const frame_t* f) {
+    const float tol = 128.0f * BFO_FLOAT_EPSILON;
+    return v3_finite(f->t) && v3_finite(f->b) && v3_finite(f->n) && fabsf(v3_dot(f->t, f->t) - 1.0f) <= tol &&
+           fabsf(v3_dot(f->b, f->b) - 1.0f) <= tol && fabsf(v3_dot(f->n, f->n) - 1.0f) <= tol &&
+           fabsf(v3_dot(f->t, f->b)) <= tol && fabsf(v3_dot(f->t, f->n)) <= tol && fabsf(v3_dot(f->b, f->n)) <= tol &&
+           fabsf(v3_dot(v3_cross(f->t, f->b), f->n) - 1.0f) <= tol;
+}
+
+/* :2705-2732 */
+static int frame_from_tangent(v3 normal, v3 seed, frame_t* f) {
+    v3 un, t, b;
+    if (!normalize3(normal, &un) || !normalize3(seed, &t)) return 0;
+    seed = v3_sub(t, v3_mul(un, v3_dot(t, un)));
+    if (!normalize3(seed, &t)) return 0;
+    if (!normalize3(v3_cross(un, t), &b) || !normalize3(v3_cross(b, un), &t)) return 0;
+    frame_t c = {t, b, un};
+    if (!orthonormal_frame(&c)) return 0;
+    *f = c;
+    return 1;
+}
+
+/* :2734-2748 */
+static int make_local_frame(v3 normal, frame_t* f) {
+    v3 n;
+    if (!normalize3(normal, &n)) return 0;
+    const float sign = copysignf(1.0f, n.z);
+    const float coefficient = -1.0f / (sign + n.z);
+    const float product = n.x * n.y * coefficient;
+    const v3 seed = v3_make(1.0f + sign * n.x * n.x * coefficient, sign * product, -sign * n.x);
+    return frame_from_tangent(normal, seed, f);
+}
+
+/* :2750-2766 */
+static int rotate_frame(const frame_t* src, float angle, frame_t* out) {
+    if (!orthonormal_frame(src) || !bf_isfinite(angle) || angle < -BFO_PI || !(angle < BFO_PI)) return 0;
+    if (angle == 0.0f) {
+        *out = *src;
+        return 1;
+    }
+    float s, c;
+    bf_sincosf(angle, &s, &c);
+    const v3 t = v3_add(v3_mul(src->t, c), v3_mul(src->b, s));
+    return bf_isfinite(c) && bf_isfinite(s) && frame_from_tangent(src->n, t, out);
+}
+
+static v3 to_local(const frame_t* f, v3 d) { return v3_make(v3_dot(d, f->t), v3_dot(d, f->b), v3_dot(d, f->n)); }
+static int to_world(const frame_t* f, v3 l, v3* out) {
+    return normalize3(v3_add(v3_add(v3_mul(f->t, l.x), v3_mul(f->b, l.y)), v3_mul(f->n, l.z)), out);
+}
+
+/* :3788-3818 */
+static int shading_correction(v3 ng, v3 ns, v3 wo, v3 wi, float* correction) {
+    const float og = v3_dot(ng, wo), os = v3_dot(ns, wo), ig = v3_dot(ng, wi), is = v3_dot(ns, wi);
+    if (!lobe_unit_vector(ng) || !lobe_unit_vector(ns) || !lobe_unit_vector(wo) || !lobe_unit_vector(wi) ||
+        !(v3_dot(ng, ns) > 0.0f) || !bf_isfinite(og) || !bf_isfinite(os) || !bf_isfinite(ig) || !bf_isfinite(is))
+        return 0;
+    if (og == 0.0f || os == 0.0f || ig == 0.0f || is == 0.0f || bf_signbit(og) != bf_signbit(os) ||
+        bf_signbit(ig) != bf_signbit(is)) {
+        *correction = 0.0f;
+        return 1;
+    }
+    *correction = 1.0f;
+    return 1;
+}
+
+/* :3820-3840 */
+static int geometric_event_support(v3 ng, v3 wo, v3 wi, uint32_t lobes, int* supported) {
+    if (!lobe_valid_surface_event(lobes)) return 0;
+    const float o = v3_dot(ng, wo), i = v3_dot(ng, wi);
+    if (!bf_isfinite(o) || !bf_isfinite(i)) return 0;
+    if (o == 0.0f || i == 0.0f) {
+        *supported = 0;
+        return 1;
+    }
+    const int same = bf_signbit(o) == bf_signbit(i);
+    *supported = (lobes & BF_LOBE_REFLECTION) ? same : !same;
+    return 1;
+}
+
+/* ---- stages --------------------------------------------------------------------------------- */
+typedef struct {
+    v3 position, error, ng, ns, dpdu;
+    float uv[2];
+    uint32_t material;
+} surface_t;
+
+/* :1523-1665 */
+static int surface_data(const bfo_scene* s, const lane* l, surface_t* out) {
+    const uint32_t inst = l->hit_instance, g = l->hit_geometry;
+    const bf_instance_desc* id = &s->instances[inst];
+    uint32_t idx[3];
+    if (!bfo_triangle_vertices(s, g, l->hit.ids.primitive, idx)) return 0;
+    const v3 bary = v3_make(l->hit.barycentric0, l->hit.barycentric1, l->hit.barycentric2);
+    v3 lv[3], ln = v3_make(0.0f, 0.0f, 0.0f);
+    float tc[3][2];
+    float uv[2] = {0.0f, 0.0f};
+    for (int c = 0; c < 3; ++c) {
+        const uint64_t gv = s->geometries[g].vertex_offset + idx[c];
+        lv[c] = v3_make(s->positions[3 * gv], s->positions[3 * gv + 1], s->positions[3 * gv + 2]);
+        tc[c][0] = s->uvs[2 * gv];
+        tc[c][1] = s->uvs[2 * gv + 1];
+        const float w = c == 0 ? bary.x : (c == 1 ? bary.y : bary.z);
+        const v3 vn = v3_make(s->normals[3 * gv], s->normals[3 * gv + 1], s->normals[3 * gv + 2]);
+        if (!v3_finite(lv[c]) || !v3_finite(vn) || !bf_isfinite(tc[c][0]) || !bf_isfinite(tc[c][1])) return 0;
+        ln = v3_add(ln, v3_mul(vn, w));
+        uv[0] = fmaf(tc[c][0], w, uv[0]);
+        uv[1] = fmaf(tc[c][1], w, uv[1]);
+    }
+    const float* inv = id->world_to_local;
+    const float* m = id->local_to_world;
+    /* transform_normal_from_inverse (:1152-1159) */
+    const v3 tn = v3_make(fmaf(inv[0], ln.x, fmaf(inv[4], ln.y, inv[8] * ln.z)),
+                          fmaf(inv[1], ln.x, fmaf(inv[5], ln.y, inv[9] * ln.z)),
+                          fmaf(inv[2], ln.x, fmaf(inv[6], ln.y, inv[10] * ln.z)));
+    v3 ns, ng = v3_make(l->hit.normal_x, l->hit.normal_y, l->hit.normal_z);
+    if (!normalize3(tn, &ns) || !normalize3(ng, &ng)) return 0;
+    if (v3_dot(ns, ng) < 0.0f) ns = v3_mul(ns, -1.0f);
+    if (!(v3_dot(ns, ng) > 0.0f)) return 0;
+    v3 wv[3];
+    for (int c = 0; c < 3; ++c) {
+        wv[c] = bfo_transform_point(m, lv[c]);
+        if (!v3_finite(wv[c])) return 0;
+    }
+    /* surface_derivatives (:1036-1060) */
+    const v3 e1 = v3_sub(wv[1], wv[0]), e2 = v3_sub(wv[2], wv[0]);
+    const float du1 = tc[1][0] - tc[0][0], dv1 = tc[1][1] - tc[0][1];
+    const float du2 = tc[2][0] - tc[0][0], dv2 = tc[2][1] - tc[0][1];
+    const float det = fmaf(du1, dv2, -dv1 * du2);
+    v3 dpdu = v3_make(0.0f, 0.0f, 0.0f);
+    if (!v3_finite(e1) || !v3_finite(e2) || !bf_isfinite(det)) return 0;
+    if (det != 0.0f) {
+        const float rcp = 1.0f / det;
+        dpdu = v3_mul(v3_sub(v3_mul(e1, dv2), v3_mul(e2, dv1)), rcp);
+        const v3 dpdv = v3_mul(v3_sub(v3_mul(e2, du1), v3_mul(e1, du2)), rcp);
+        if (!bf_isfinite(rcp) || !v3_finite(dpdu) || !v3_finite(dpdv)) return 0;
+    }
+    positioned_t obj, world;
+    const v3 traversal = v3_make(fmaf(l->hit.parameter, l->ray.direction_x, l->ray.origin_x),
+                                 fmaf(l->hit.parameter, l->ray.direction_y, l->ray.origin_y),
+                                 fmaf(l->hit.parameter, l->ray.direction_z, l->ray.origin_z));
+    if (!interpolate_position_with_error(lv, bary, &obj) || !transform_position_with_error(m, &obj, traversal, &world))
+        return 0;
+    out->position = world.position;
+    out->error = world.error;
+    out->ng = ng;
+    out->ns = ns;
+    out->dpdu = dpdu;
+    out->uv[0] = uv[0];
+    out->uv[1] = uv[1];
+    out->material = id->material_index;
+    return v3_finite(out->position) && v3_finite(out->error);
+}
+
+/* :2203-2214 via TransportDevice.cuh:479-551 */
+static int uniform_light_selection(uint32_t count, float u, uint32_t* index, float* probability) {
+    if (count == 0 || !bf_isfinite(u) || u < 0.0f || !(u < 1.0f) || count > (1u << 24)) return 0;
+    const float sc = (float)count;
+    uint32_t first = 1, last = count + 1;
+    while (first < last) {
+        const uint32_t mid = first + (last - first) / 2;
+        const float boundary = mid == count ? 1.0f : (float)mid / sc;
+        if (!(u < boundary)) first = mid + 1;
+        else last = mid;
+    }
+    if (first == 0 || first > count) return 0;
+    const uint32_t i = first - 1;
+    const float lower = i == 0 ? 0.0f : (float)i / sc;
+    const float upper = first == count ? 1.0f : (float)first / sc;
+    const float interval = upper - lower;
+    if (!(lower <= u) || !(u < upper) || !(interval > 0.0f) || interval > 1.0f) return 0;
+    *index = i;
+    *probability = interval;
+    return 1;
+}
+
+/* :2216-2230 */
+static int uniform_light_probability(uint32_t count, uint32_t i, float* probability) {
+    if (count == 0 || count > (1u << 24) || i >= count) return 0;
+    const float sc = (float)count;
+    const float lower = i == 0 ? 0.0f : (float)i / sc;
+    const float upper = i + 1 == count ? 1.0f : (float)(i + 1) / sc;
+    *probability = upper - lower;
+    return *probability > 0.0f && *probability <= 1.0f;
+}
+
+typedef struct {
+    float radiance[4];
+    v3 direction, position, error, normal;
+    float distance, conditional, selection;
+    uint32_t kind; /* 0 none, 2 finite surface */
+} incident_t;
+
+/* :2044-2201 + :2378-2460 (mesh-area lights; the scene has no punctual lights) */
+static int sample_light(const ctx* c, const lane* l, const surface_t* surf, uint64_t first_dim, incident_t* inc) {
+    const bfo_scene* s = c->scene;
+    memset(inc, 0, sizeof *inc);
+    const uint32_t count = s->light_count;
+    if (count == 0) return 0;
+    const uint32_t sampler = c->options->sampler;
+    const float u_sel = bfo_sample_1d(sampler, &l->sample, first_dim + 0);
+    uint32_t li;
+    float sel;
+    if (!uniform_light_selection(count, u_sel, &li, &sel)) return 0;
+    const bfo_light* light = &s->lights[li];
+    const bf_instance_desc* id = &s->instances[light->instance];
+    if ((id->visibility_mask & l->ray.visibility_mask) == 0u) return 1;
+    const float area_sample = bfo_sample_1d(sampler, &l->sample, first_dim + 1);
+    const float triangle_sample = bfo_sample_1d(sampler, &l->sample, first_dim + 2);
+    if (!(area_sample >= 0.0f) || !(area_sample < 1.0f) || !(triangle_sample >= 0.0f) || !(triangle_sample < 1.0f)) return 0;
+    uint32_t t = light->triangle_count;
+    for (uint32_t k = 0; k < light->triangle_count; ++k) {
+        if (area_sample < light->next_probabilities[k]) {
+            t = k;
+            break;
+        }
+    }
+    if (t >= light->triangle_count) return 0;
+    const float prob = light->probabilities[t];
+    const float area = light->areas[t];
+    const float remapped = (area_sample - light->cumulative_before[t]) / prob;
+    if (!bf_isfinite(remapped) || remapped < 0.0f || !(remapped < 1.0f)) return 0;
+    const float root = sqrtf(remapped);
+    const float b0 = 1.0f - root, b1 = root * (1.0f - triangle_sample), b2 = root * triangle_sample;
+    uint32_t idx[3];
+    v3 lv[3], wv[3], n;
+    if (!bfo_triangle_vertices(s, light->geometry, t, idx)) return 0;
+    for (int k = 0; k < 3; ++k) {
+        lv[k] = bfo_vertex(s, light->geometry, idx[k]);
+        wv[k] = bfo_transform_point(id->local_to_world, lv[k]);
+    }
+    float ignored;
+    if (!bfo_triangle_area_v(wv, &ignored, &n)) return 0;
+    const float density = prob / area;
+    if (!bf_isfinite(density) || !(density > 0.0f)) return 0;
+    float radiance[4];
+    const bf_material_desc* mat = &s->materials[id->material_index];
+    int black = 1;
+    for (int k = 0; k < 4; ++k) {
+        radiance[k] = spectrum_at(s, mat->emission_spectrum, l->wl[k]);
+        if (!bf_isfinite(radiance[k]) || radiance[k] < 0.0f) return 0;
+        if (radiance[k] != 0.0f) black = 0;
+    }
+    const v3 traversal = v3_add(v3_add(v3_mul(wv[0], b0), v3_mul(wv[1], b1)), v3_mul(wv[2], b2));
+    positioned_t obj, world;
+    if (!interpolate_position_with_error(lv, v3_make(b0, b1, b2), &obj) ||
+        !transform_position_with_error(id->local_to_world, &obj, traversal, &world))
+        return 0;
+    if (black) return 1;
+    segment_t seg;
+    if (!scaled_segment(v3_sub(world.position, surf->position), &seg)) return 0;
+    float absolute;
+    int supported;
+    if (!light_surface_alignment(n, seg.scaled, &absolute, &supported)) return 0;
+    if (!supported) return 1;
+    const float num[4] = {density, seg.scale, seg.scale, seg.scaled_distance_cubed};
+    const float den[1] = {absolute};
+    float cond;
+    if (lobe_checked_product_quotient(num, 4, den, 1, &cond) != LOBE_OK || !(cond > 0.0f)) return 0;
+    memcpy(inc->radiance, radiance, sizeof radiance);
+    inc->direction = seg.direction;
+    inc->position = world.position;
+    inc->error = world.error;
+    inc->normal = n;
+    inc->distance = seg.length;
+    inc->conditional = cond;
+    inc->selection = sel;
+    inc->kind = 2;
+    return 1;
+}
+
+/* :2513-2594 */
+static int emissive_hit_mis_weight(const ctx* c, const lane* l, const surface_t* surf, float* weight) {
+    *weight = 1.0f;
+    if (!l->prev_valid) return 1;
+    const bfo_scene* s = c->scene;
+    const int32_t li = s->instance_light[l->hit_instance];
+    if (li < 0) return 0;
+    float sel;
+    if (!uniform_light_probability(s->light_count, (uint32_t)li, &sel)) return 0;
+    const float inv_area = s->lights[li].inverse_area;
+    segment_t seg;
+    if (!scaled_segment(v3_sub(surf->position, v3_make(l->prev_ctx[0], l->prev_ctx[1], l->prev_ctx[2])), &seg)) return 0;
+    float absolute;
+    int supported;
+    if (!light_surface_alignment(surf->ng, seg.scaled, &absolute, &supported)) return 0;
+    if (!supported) return 0;
+    const float num[4] = {inv_area, seg.scale, seg.scale, seg.scaled_distance_cubed};
+    const float den[1] = {absolute};
+    float cond;
+    if (lobe_checked_product_quotient(num, 4, den, 1, &cond) != LOBE_OK) return 0;
+    float joint;
+    if (!lobe_joint_light_pdf(sel, cond, &joint)) return 0;
+    float w;
+    if (!lobe_mis_weight(c->options->mis_heuristic, l->prev_pdf, joint, &w)) return 0;
+    *weight = w;
+    return 1;
+}
+
+/* :4042-4105 */
+static int prepare_shadow_ray(const surface_t* surf, const incident_t* inc, lane* l) {
+    v3 src, endpoint;
+    if (!offset_point(surf->position, surf->error, surf->ng, inc->direction, &src)) return 0;
+    if (!offset_point(inc->position, inc->error, inc->normal, v3_mul(inc->direction, -1.0f), &endpoint)) return 0;
+    segment_t seg;
+    if (!scaled_segment(v3_sub(endpoint, src), &seg) || !(v3_dot(seg.direction, inc->direction) > 0.0f)) return 0;
+    bf_transport_ray* r = &l->shadow_ray;
+    memset(r, 0, sizeof *r);
+    r->origin_x = src.x;
+    r->origin_y = src.y;
+    r->origin_z = src.z;
+    r->t_min = 0.0f;
+    r->direction_x = seg.direction.x;
+    r->direction_y = seg.direction.y;
+    r->direction_z = seg.direction.z;
+    r->t_max = bf_next_toward_zero_f(seg.length);
+    r->time = l->ray.time;
+    r->visibility_mask = l->ray.visibility_mask;
+    return r->t_max > 0.0f;
+}
+
+/* :3851-3880 */
+static int advance_depth(uint32_t lobes, lane* l) {
+    if (!lobe_valid_surface_event(lobes)) return 0;
+    if (lobes & BF_LOBE_DIFFUSE) l->diffuse++;
+    else if (lobes & BF_LOBE_GLOSSY) l->glossy++;
+    else l->specular++;
+    if (lobes & BF_LOBE_TRANSMISSION) l->transmission++;
+    l->depth++;
+    return 1;
+}
+
+static int is_zero4(const float* v) { return v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f && v[3] == 0.0f; }
+
+/* CudaWavefrontStages.cu:4107-4727 */
+static void shade_lane(const ctx* c, lane* l) {
+    const bfo_scene* s = c->scene;
+    const bf_transport_options* o = c->options;
+    surface_t surf;
+    if (!surface_data(s, l, &surf)) return fail(l, 10);
+    v3 wo = v3_make(-l->ray.direction_x, -l->ray.direction_y, -l->ray.direction_z);
+    if (!normalize3(wo, &wo)) return fail(l, 11);
+    int front = 0;
+    if (!one_sided_emission_support(surf.ng, wo, &front)) return fail(l, 12);
+    const bf_material_desc* mat = &s->materials[surf.material];
+    float emitted[4];
+    for (int k = 0; k < 4; ++k) {
+        emitted[k] = spectrum_at(s, mat->emission_spectrum, l->wl[k]);
+        if (!bf_isfinite(emitted[k]) || emitted[k] < 0.0f) return fail(l, 13);
+    }
+    lobe_mixture mix;
+    load_mixture(s, mat, l->wl, &mix);
+    frame_t frame, unrotated;
+    const int created = mat->frame_mode == 0 ? make_local_frame(surf.ns, &unrotated)
+                                             : frame_from_tangent(surf.ns, surf.dpdu, &unrotated);
+    if (!created || !rotate_frame(&unrotated, mat->tangent_rotation, &frame)) return fail(l, 14);
+    if (front && !is_zero4(emitted)) {
+        float weight = 1.0f;
+        if (!emissive_hit_mis_weight(c, l, &surf, &weight)) return fail(l, 15);
+        for (int k = 0; k < 4; ++k) {
+            if (l->beta[k] == 0.0f || emitted[k] == 0.0f || weight == 0.0f) continue;
+            const float num[3] = {l->beta[k], emitted[k], weight};
+            const float den[1] = {1.0f};
+            float contribution;
+            if (lobe_checked_product_quotient(num, 3, den, 1, &contribution) != LOBE_OK) return fail(l, 16);
+            l->L[k] += contribution;
+            if (!bf_isfinite(l->L[k])) return fail(l, 16);
+        }
+    }
+    if (is_zero4(l->beta)) {
+        l->termination = BF_TERM_ZERO_THROUGHPUT;
+        l->route = ROUTE_TERM;
+        return;
+    }
+    if (l->diffuse > o->diffuse_depth_limit || l->glossy > o->glossy_depth_limit || l->specular > o->specular_depth_limit ||
+        l->transmission > o->transmission_depth_limit || l->volume > o->volume_depth_limit)
+        return fail(l, 17);
+    const uint32_t limits[5] = {o->diffuse_depth_limit, o->glossy_depth_limit, o->specular_depth_limit,
+                                o->transmission_depth_limit, o->volume_depth_limit};
+    const uint32_t counters[5] = {l->diffuse, l->glossy, l->specular, l->transmission, l->volume};
+    lobe_filter_state filtered;
+    lobe_filter_mixture_by_depth(&mix, limits, counters, l->depth, &filtered);
+    if (filtered.status != LOBE_OK) return fail(l, 18);
+    if (mix.active_count == 0 && filtered.source_count != 0) {
+        l->termination = BF_TERM_DEPTH_LIMIT;
+        l->route = ROUTE_TERM;
+        return;
+    }
+    const v3 wo_local = to_local(&frame, wo);
+    const uint64_t first_dim = BFO_FIRST_BOUNCE_DIM + (uint64_t)l->depth * BFO_DIMS_PER_BOUNCE;
+    const uint32_t sampler = o->sampler;
+    const float bsdf_component = bfo_sample_1d(sampler, &l->sample, first_dim + 3);
+    const float bsdf_u = bfo_sample_1d(sampler, &l->sample, first_dim + 4);
+    const float bsdf_v = bfo_sample_1d(sampler, &l->sample, first_dim + 5);
+    l->closure_sampled = 1;
+    lobe_mixture_sample bs;
+    lobe_sample_filtered_mixture(&mix, &filtered, wo_local, bsdf_component, bsdf_u, bsdf_v, &bs);
+    const int has_continuation = bs.status == LOBE_OK;
+    if (!has_continuation && bs.status != LOBE_OUTSIDE_SUPPORT) return fail(l, 19);
+    if (has_continuation) {
+        const int delta = lobe_valid_surface_event(bs.lobes) && (bs.lobes & BF_LOBE_SPECULAR);
+        if (!lobe_valid_surface_event(bs.lobes) || bs.measure != (delta ? LOBE_MEASURE_DISCRETE : LOBE_MEASURE_SOLID_ANGLE) ||
+            !bf_isfinite(bs.pdf) || !(bs.pdf > 0.0f) || (delta && bs.pdf > 1.0f))
+            return fail(l, 20);
+    }
+    incident_t inc;
+    memset(&inc, 0, sizeof inc);
+    if (s->light_count != 0 && lobe_mixture_has_nonblack_weight(&mix)) {
+        l->light_sampled = 1;
+        if (!sample_light(c, l, &surf, first_dim, &inc)) return fail(l, 21);
+    }
+    int has_shadow = 0;
+    if (inc.kind != 0) {
+        const v3 wi_local = to_local(&frame, inc.direction);
+        if (wo_local.z != 0.0f && wi_local.z != 0.0f) {
+            const int singleton_lambert = mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN;
+            float bsdf[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (singleton_lambert) {
+                if (wo_local.z > 0.0f && wi_local.z > 0.0f)
+                    for (int k = 0; k < 4; ++k) bsdf[k] = mix.closures[0].weight[k];
+            } else if (lobe_filtered_mixture_eval(&mix, &filtered, wo_local, wi_local, bsdf) != LOBE_OK) {
+                return fail(l, 22);
+            }
+            if (!is_zero4(bsdf)) {
+                float corr;
+                if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, &corr)) return fail(l, 23);
+                if (corr > 0.0f) {
+                    float joint, bsdf_pdf, w;
+                    if (!lobe_joint_light_pdf(inc.selection, inc.conditional, &joint)) return fail(l, 24);
+                    if (lobe_filtered_mixture_pdf(&mix, &filtered, wo_local, wi_local, &bsdf_pdf) != LOBE_OK) return fail(l, 24);
+                    if (!lobe_mis_weight(o->mis_heuristic, joint, bsdf_pdf, &w)) return fail(l, 24);
+                    memcpy(l->p_beta, l->beta, sizeof l->p_beta);
+                    memcpy(l->p_bsdf, bsdf, sizeof bsdf);
+                    memcpy(l->p_L, inc.radiance, sizeof l->p_L);
+                    l->p_cos = fabsf(wi_local.z);
+                    l->p_weight = w;
+                    l->p_sel = inc.selection;
+                    l->p_cond = inc.conditional;
+                    l->p_corr = corr;
+                    l->p_lambert = (uint32_t)singleton_lambert;
+                    if (!prepare_shadow_ray(&surf, &inc, l)) return fail(l, 25);
+                    has_shadow = 1;
+                }
+            }
+        }
+    }
+    v3 wi;
+    int continuation = has_continuation;
+    float corr = 0.0f;
+    if (continuation) {
+        if (!to_world(&frame, bs.incoming, &wi)) return fail(l, 26);
+        int geo = 0;
+        if (!shading_correction(surf.ng, surf.ns, wo, wi, &corr) || !geometric_event_support(surf.ng, wo, wi, bs.lobes, &geo))
+            return fail(l, 27);
+        continuation = corr > 0.0f && geo;
+    }
+    if (!continuation) {
+        l->cont_pending = 0;
+        l->pending_term = BF_TERM_OUTSIDE_SUPPORT;
+        if (has_shadow) {
+            l->route = ROUTE_SHADOW;
+            return;
+        }
+        l->termination = BF_TERM_OUTSIDE_SUPPORT;
+        l->route = ROUTE_TERM;
+        return;
+    }
+    /* update_closure_throughput (:3894-3938) */
+    float nb[4];
+    const float cosine = fabsf(bs.incoming.z);
+    if (!bf_isfinite(cosine) || !(cosine > 0.0f)) return fail(l, 28);
+    if (mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN && corr == 1.0f) {
+        for (int k = 0; k < 4; ++k) {
+            if (!lobe_checked_product(l->beta[k], mix.closures[0].weight[k], &nb[k])) return fail(l, 28);
+        }
+    } else {
+        for (int k = 0; k < 4; ++k) {
+            const float num[4] = {l->beta[k], bs.value[k], cosine, corr};
+            const float den[1] = {bs.pdf};
+            if (lobe_checked_product_quotient(num, 4, den, 1, &nb[k]) != LOBE_OK) return fail(l, 28);
+        }
+    }
+    const float next_eta = l->eta_scale * bs.eta_scale;
+    if (!bf_isfinite(bs.eta_scale) || !(bs.eta_scale > 0.0f) || !bf_isfinite(next_eta) || !(next_eta > 0.0f)) return fail(l, 29);
+    v3 origin;
+    if (!offset_point(surf.position, surf.error, surf.ng, wi, &origin)) return fail(l, 30);
+    const int delta_event = (bs.lobes & BF_LOBE_SPECULAR) != 0;
+    memcpy(l->beta, nb, sizeof nb);
+    l->eta_scale = next_eta;
+    l->delta_flags = ((l->delta_flags & 2u) ? 2u : 0u) | (delta_event ? 1u : 2u);
+    if (!advance_depth(bs.lobes, l)) return fail(l, 31);
+    if (!delta_event) {
+        l->prev_valid = 1;
+        l->prev_ctx[0] = surf.position.x;
+        l->prev_ctx[1] = surf.position.y;
+        l->prev_ctx[2] = surf.position.z;
+        l->prev_pdf = bs.pdf;
+    } else {
+        l->prev_valid = 0;
+        l->prev_ctx[0] = l->prev_ctx[1] = l->prev_ctx[2] = 0.0f;
+        l->prev_pdf = 0.0f;
+    }
+    l->ray.origin_x = origin.x;
+    l->ray.origin_y = origin.y;
+    l->ray.origin_z = origin.z;
+    l->ray.t_min = 0.0f;
+    l->ray.direction_x = wi.x;
+    l->ray.direction_y = wi.y;
+    l->ray.direction_z = wi.z;
+    l->ray.t_max = BF_INFINITY_F;
+    int cont_pending = 1;
+    uint32_t pending_term = BF_TERM_NONE;
+    if (is_zero4(l->beta)) {
+        cont_pending = 0;
+        pending_term = BF_TERM_ZERO_THROUGHPUT;
+    } else if (o->russian_roulette_enabled && l->depth >= o->russian_roulette_first_depth) {
+        const float u = bfo_sample_1d(sampler, &l->sample, first_dim + 9);
+        int survived;
+        if (!lobe_russian_roulette(l->beta, l->eta_scale, u, o->russian_roulette_minimum, o->russian_roulette_maximum,
+                                   &survived))
+            return fail(l, 32);
+        if (!survived) {
+            cont_pending = 0;
+            pending_term = BF_TERM_RUSSIAN_ROULETTE;
+        }
+    }
+    if (has_shadow) {
+        l->cont_pending = (uint32_t)cont_pending;
+        l->pending_term = pending_term;
+        l->route = ROUTE_SHADOW;
+        return;
+    }
+    if (!cont_pending) {
+        l->termination = pending_term;
+        l->route = ROUTE_TERM;
+        return;
+    }
+    l->route = ROUTE_CONT;
+}
+
+/* CudaWavefrontStages.cu:4772-4908 */
+static void shadow_lane(lane* l, uint32_t occlusion_status) {
+    if (occlusion_status != BF_VISIBLE && occlusion_status != BF_OCCLUDED) return fail(l, 40);
+    if (occlusion_status == BF_VISIBLE) {
+        for (int k = 0; k < 4; ++k) {
+            const float den[2] = {l->p_sel, l->p_cond};
+            float contribution;
+            const float coefficient = l->p_lambert ? BFO_INV_PI : 1.0f;
+            int status;
+            if (l->p_weight == 1.0f) {
+                const float num[6] = {l->p_beta[k], l->p_bsdf[k], coefficient, l->p_L[k], l->p_cos, l->p_corr};
+                status = lobe_checked_product_quotient(num, 6, den, 2, &contribution);
+            } else {
+                const float num[7] = {l->p_beta[k], l->p_bsdf[k], coefficient, l->p_L[k], l->p_cos, l->p_corr, l->p_weight};
+                status = lobe_checked_product_quotient(num, 7, den, 2, &contribution);
+            }
+            if (status != LOBE_OK) return fail(l, 41);
+            l->L[k] += contribution;
+            if (!bf_isfinite(l->L[k]) || l->L[k] < 0.0f) return fail(l, 41);
+        }
+    }
+    if (!l->cont_pending) {
+        l->termination = l->pending_term;
+        l->route = ROUTE_TERM;
+        return;
+    }
+    l->route = ROUTE_CONT;
+}
+
+/* CudaWavefrontStages.cu:3962-4040 */
+static void miss_lane(const ctx* c, lane* l) {
+    const bfo_scene* s = c->scene;
+    if (s->desc.has_environment) {
+        for (int k = 0; k < 4; ++k) {
+            const float radiance = spectrum_at(s, s->desc.environment_spectrum, l->wl[k]);
+            float product;
+            if (!lobe_checked_product(l->beta[k], radiance, &product)) return fail(l, 50);
+            l->L[k] += product;
+            if (!bf_isfinite(l->L[k]) || l->L[k] < 0.0f) return fail(l, 50);
+        }
+    }
+    l->termination = BF_TERM_ESCAPED;
+    l->route = ROUTE_TERM;
+}
+
+/* ---- worker pool: contiguous lane partitions (CpuWavefrontScheduler.cpp:53-62) ------------- */
+typedef struct {
+    const ctx* c;
+    const uint32_t* items;
+    uint32_t begin, end;
+    int stage;
+    uint64_t counters[3];
+    uint64_t global_begin;
+} job;
+
+static void run_item(job* j, uint32_t item) {
+    const ctx* c = j->c;
+    lane* l = &c->lanes[item];
+    switch (j->stage) {
+    case 0:
+        camera_lane(c, l, j->global_begin + item);
+        break;
+    case 1: {
+        bf_closest_hit_result r;
+        uint32_t ctr[3];
+        bfo_closest_hit(c->scene, &l->ray, 1, &r, ctr);
+        for (int k = 0; k < 3; ++k) j->counters[k] += ctr[k];
+        if (r.status == BF_HIT_HIT) {
+            l->hit = r.hit;
+            /* the scene index of the hit instance/geometry (find_id of :990-999) */
+            for (uint32_t i = 0; i < c->scene->desc.instance_count; ++i) {
+                if (c->scene->instances[i].id == r.hit.ids.instance) {
+                    l->hit_instance = i;
+                    l->hit_geometry = c->scene->instances[i].geometry_index;
+                    break;
+                }
+            }
+            l->route = ROUTE_HIT;
+        } else if (r.status == BF_HIT_MISS) {
+            l->route = ROUTE_MISS;
+            miss_lane(c, l);
+            if (!l->failed) l->route = ROUTE_MISS;
+        } else {
+            fail(l, 60 + r.status);
+        }
+        break;
+    }
+    case 2:
+        shade_lane(c, l);
+        break;
+    case 3: {
+        bf_occlusion_result r;
+        uint32_t ctr[3];
+        bfo_occlusion(c->scene, &l->shadow_ray, 1, &r, ctr);
+        for (int k = 0; k < 3; ++k) j->counters[k] += ctr[k];
+        shadow_lane(l, r.status);
+        break;
+    }
+    }
+}
+
+static void* run_job(void* arg) {
+    job* j = (job*)arg;
+    for (uint32_t i = j->begin; i < j->end; ++i) run_item(j, j->items ? j->items[i] : i);
+    return NULL;
+}
+
+static void dispatch(const ctx* c, int stage, const uint32_t* items, uint32_t count, uint32_t threads, uint64_t global_begin,
+                     uint64_t* counters) {
+    if (threads == 0) threads = 1;
+    if (threads > count) threads = count ? count : 1;
+    job jobs[256];
+    pthread_t tids[256];
+    if (threads > 256) threads = 256;
+    const uint32_t base = count / threads, extra = count % threads;
+    uint32_t cursor = 0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        const uint32_t n = base + (t < extra ? 1u : 0u);
+        jobs[t] = (job){c, items, cursor, cursor + n, stage, {0, 0, 0}, global_begin};
+        cursor += n;
+    }
+    if (threads == 1) {
+        run_job(&jobs[0]);
+    } else {
+        for (uint32_t t = 0; t < threads; ++t) pthread_create(&tids[t], NULL, run_job, &jobs[t]);
+        for (uint32_t t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
+    }
+    if (counters)
+        for (uint32_t t = 0; t < threads; ++t)
+            for (int k = 0; k < 3; ++k) counters[k] += jobs[t].counters[k];
+}
+
+static uint64_t queue_hash(const uint32_t* q, uint32_t n) {
+    uint64_t h = 14695981039346656037ull;
+    for (uint32_t i = 0; i < n; ++i)
+        for (int b = 0; b < 4; ++b) {
+            h ^= (q[i] >> (8 * b)) & 0xFFu;
+            h *= 1099511628211ull;
+        }
+    return h;
+}
+
+/* Queue routing of WavefrontTransport.cpp:2222-2649: every compaction is a stable filter of its
+ * producer's order; shadow continuations are appended after the shade stage's continuations. */
+int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+               uint64_t path_begin, uint64_t path_count, uint32_t threads, bf_path_result* paths, float* film_float,
+               double* film_compensated, uint32_t* film_counts, bf_transport_report* report, uint64_t* traversal_counters) {
+    if (path_count == 0 || path_count > 0xFFFFFFFFull) return BF_INVALID_ARGUMENT;
+    const uint32_t n = (uint32_t)path_count;
+    lane* lanes = (lane*)calloc(n, sizeof(lane));
+    uint32_t* ray_q = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* hit_q = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* shadow_q = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* cont_q = (uint32_t*)malloc(n * sizeof(uint32_t));
+    if (!lanes || !ray_q || !hit_q || !shadow_q || !cont_q) return BF_RESOURCE_EXHAUSTED;
+    ctx c = {scene, frame, options, lanes};
+    bf_transport_report rep;
+    memset(&rep, 0, sizeof rep);
+    rep.path_count = n;
+    rep.first_failure_slot = BF_INVALID_INDEX;
+    uint64_t tc[6] = {0, 0, 0, 0, 0, 0};
+    dispatch(&c, 0, NULL, n, threads, path_begin, NULL);
+    uint32_t ray_n = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (lanes[i].route == ROUTE_RAY) ray_q[ray_n++] = i;
+    const uint64_t bound = (uint64_t)options->diffuse_depth_limit + options->glossy_depth_limit +
+                           options->specular_depth_limit + options->volume_depth_limit + 2u;
+    uint32_t bounce = 0;
+    while (ray_n != 0 && bounce <= bound) {
+        if (bounce < BF_REPORT_MAX_BOUNCES) {
+            rep.ray_queue_hash[bounce] = queue_hash(ray_q, ray_n);
+            rep.ray_queue_size[bounce] = ray_n;
+        }
+        rep.closest_hit_queries += ray_n;
+        dispatch(&c, 1, ray_q, ray_n, threads, 0, tc);
+        uint32_t hit_n = 0;
+        for (uint32_t i = 0; i < ray_n; ++i) {
+            const uint32_t s = ray_q[i];
+            if (lanes[s].route == ROUTE_HIT) hit_q[hit_n++] = s;
+            else if (lanes[s].route == ROUTE_MISS) rep.miss_lanes++;
+        }
+        rep.shade_lanes += hit_n;
+        dispatch(&c, 2, hit_q, hit_n, threads, 0, NULL);
+        uint32_t shadow_n = 0, cont_n = 0;
+        for (uint32_t i = 0; i < hit_n; ++i) {
+            lane* l = &lanes[hit_q[i]];
+            rep.closure_samples += l->closure_sampled;
+            rep.light_samples += l->light_sampled;
+            l->closure_sampled = l->light_sampled = 0;
+            if (l->route == ROUTE_SHADOW) shadow_q[shadow_n++] = hit_q[i];
+            else if (l->route == ROUTE_CONT) cont_q[cont_n++] = hit_q[i];
+        }
+        if (bounce < BF_REPORT_MAX_BOUNCES) {
+            rep.shadow_queue_hash[bounce] = queue_hash(shadow_q, shadow_n);
+            rep.shadow_queue_size[bounce] = shadow_n;
+        }
+        rep.shadow_queries += shadow_n;
+        dispatch(&c, 3, shadow_q, shadow_n, threads, 0, tc + 3);
+        for (uint32_t i = 0; i < shadow_n; ++i)
+            if (lanes[shadow_q[i]].route == ROUTE_CONT) cont_q[cont_n++] = shadow_q[i];
+        memcpy(ray_q, cont_q, cont_n * sizeof(uint32_t));
+        ray_n = cont_n;
+        bounce++;
+    }
+    rep.bounces = bounce;
+    const uint32_t W = frame->camera.width, H = frame->camera.height;
+    for (uint32_t i = 0; i < n; ++i) {
+        lane* l = &lanes[i];
+        if (l->failed) {
+            if (rep.failed_paths == 0) {
+                rep.first_failure_slot = i;
+                rep.first_failure_code = l->fail_code;
+            }
+            rep.failed_paths++;
+        }
+        if (paths) {
+            bf_path_result* p = &paths[i];
+            memcpy(p->radiance, l->L, sizeof p->radiance);
+            memcpy(p->beta, l->beta, sizeof p->beta);
+            memcpy(p->wavelengths, l->wl, sizeof p->wavelengths);
+            p->depth = l->depth;
+            p->termination = l->termination;
+            p->depth_counters = (l->diffuse & 0xFFu) | ((l->glossy & 0xFFu) << 8) | ((l->specular & 0xFFu) << 16) |
+                                ((l->transmission & 0xFFu) << 24);
+            p->eta_scale = l->eta_scale;
+        }
+        if (film_float || film_compensated || film_counts) {
+            float xyz[3], rgb[3];
+            bfo_cie_xyz(l->L, l->wl, l->wl_pdf, xyz);
+            bfo_xyz_to_rgb(xyz, rgb);
+            const uint64_t pixel = (uint64_t)l->sample.pixel_y * W + l->sample.pixel_x;
+            if (pixel >= (uint64_t)W * H) continue;
+            if (film_float)
+                for (int k = 0; k < 3; ++k) film_float[pixel * 3 + k] += rgb[k] * 1.0f;
+            if (film_compensated) {
+                /* Film.hpp:221-230 Neumaier: [sum, compensation] pairs per channel */
+                for (int k = 0; k < 3; ++k) {
+                    double* sum = &film_compensated[(pixel * 3 + k) * 2];
+                    const double value = (double)rgb[k] * 1.0;
+                    const double updated = sum[0] + value;
+                    if (fabs(sum[0]) >= fabs(value)) sum[1] += (sum[0] - updated) + value;
+                    else sum[1] += (value - updated) + sum[0];
+                    sum[0] = updated;
+                }
+            }
+            if (film_counts) film_counts[pixel]++;
+        }
+    }
+    if (report) *report = rep;
+    if (traversal_counters) memcpy(traversal_counters, tc, sizeof tc);
+    free(lanes);
+    free(ray_q);
+    free(hit_q);
+    free(shadow_q);
+    free(cont_q);
+    return BF_OK;
+}

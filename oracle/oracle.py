@@ -1,0 +1,156 @@
+"""ORACLE — test infrastructure only: ctypes binding of oracle/liboracle.so (the C restatement).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
+"""
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+
+def build(force=False):
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"] + (["-B"] if force else []), check=True)
+
+
+def load(abi):
+    if not LIB_PATH.exists():
+        build()
+    lib = C.CDLL(str(LIB_PATH))
+    lib.bfo_mix_bits.restype = C.c_uint64
+    lib.bfo_mix_bits.argtypes = [C.c_uint64]
+    lib.bfo_indexed_bits.restype = C.c_uint64
+    lib.bfo_indexed_bits.argtypes = [C.POINTER(abi.SampleStreamIndex), C.c_uint64]
+    lib.bfo_sobol_bits.restype = C.c_uint64
+    lib.bfo_sobol_bits.argtypes = [C.c_uint64, C.c_uint64]
+    lib.bfo_owen_bits.restype = C.c_uint64
+    lib.bfo_owen_bits.argtypes = [C.POINTER(abi.SampleStreamIndex), C.c_uint64]
+    lib.bfo_sample_1d.restype = C.c_float
+    lib.bfo_sample_1d.argtypes = [C.c_uint32, C.POINTER(abi.SampleStreamIndex), C.c_uint64]
+    lib.bfo_sobol_dimension_count.restype = C.c_uint64
+    lib.bfo_cie_y_integral.restype = C.c_double
+    lib.bfo_cie_fingerprint.restype = C.c_uint64
+    lib.bfo_cie_xyz.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_float)]
+    lib.bfo_xyz_to_rgb.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.bfo_scene_build.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(C.c_void_p)]
+    lib.bfo_scene_free.argtypes = [C.c_void_p]
+    lib.bfo_scene_bvh.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
+    lib.bfo_scene_tlas_root.restype = C.c_uint32
+    lib.bfo_scene_tlas_root.argtypes = [C.c_void_p]
+    lib.bfo_scene_light_count.restype = C.c_uint32
+    lib.bfo_scene_light_count.argtypes = [C.c_void_p]
+    lib.bfo_scene_light.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_float), C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.bfo_triangle_area.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.bfo_closest_hit.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.bfo_occlusion.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.bfo_closest_hit_brute_force.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+    lib.bfo_camera_ray.argtypes = [C.POINTER(abi.CameraDesc), C.c_uint32, C.POINTER(abi.SampleStreamIndex),
+                                   C.POINTER(abi.TransportRay)]
+    lib.bfo_render.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
+                               C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.POINTER(abi.TransportReport), C.c_void_p]
+    return lib
+
+
+class Oracle:
+    def __init__(self, abi):
+        self.abi = abi
+        self.lib = load(abi)
+
+    def scene(self, scene):
+        handle = C.c_void_p()
+        status = self.lib.bfo_scene_build(C.byref(scene.desc), C.byref(handle))
+        if status != 0:
+            raise RuntimeError(f"oracle scene build failed with status {status}")
+        return OracleScene(self, handle)
+
+
+class OracleScene:
+    def __init__(self, oracle, handle):
+        self.oracle = oracle
+        self.lib = oracle.lib
+        self.abi = oracle.abi
+        self.handle = handle
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self.lib.bfo_scene_free(self.handle)
+            self.handle = None
+
+    def bvh(self):
+        counts = (C.c_uint64 * 5)()
+        self.lib.bfo_scene_bvh(self.handle, counts, None, None, None, None, None)
+        abi = self.abi
+        blases = (abi.BvhBlas * max(1, counts[0]))()
+        blas_nodes = (abi.BvhNode * max(1, counts[1]))()
+        prims = np.zeros(max(1, counts[2]), dtype=np.uint32)
+        tlas = (abi.BvhNode * max(1, counts[3]))()
+        irefs = (abi.BvhInstanceRef * max(1, counts[4]))()
+        self.lib.bfo_scene_bvh(self.handle, counts, blases, blas_nodes, prims.ctypes.data, tlas, irefs)
+        as_bytes = lambda arr, n, size: bytes(memoryview(arr).cast("B")[: n * size])
+        return {
+            "counts": list(counts),
+            "blas": as_bytes(blases, counts[0], 32),
+            "blas_nodes": as_bytes(blas_nodes, counts[1], 48),
+            "primitive_refs": prims[: counts[2]].tobytes(),
+            "tlas_nodes": as_bytes(tlas, counts[3], 48),
+            "instance_refs": as_bytes(irefs, counts[4], 32),
+            "tlas_root": self.lib.bfo_scene_tlas_root(self.handle),
+        }
+
+    def closest_hit(self, rays, counters=False):
+        n = len(rays)
+        out = (self.abi.ClosestHitResult * n)()
+        ctr = np.zeros(3 * n, dtype=np.uint32) if counters else None
+        self.lib.bfo_closest_hit(self.handle, C.addressof(rays), n, C.addressof(out),
+                                 ctr.ctypes.data if counters else None)
+        return (out, ctr.reshape(n, 3)) if counters else out
+
+    def occlusion(self, rays):
+        n = len(rays)
+        out = (self.abi.OcclusionResult * n)()
+        self.lib.bfo_occlusion(self.handle, C.addressof(rays), n, C.addressof(out), None)
+        return out
+
+    def closest_hit_brute_force(self, rays):
+        n = len(rays)
+        out = (self.abi.ClosestHitResult * n)()
+        self.lib.bfo_closest_hit_brute_force(self.handle, C.addressof(rays), n, C.addressof(out))
+        return out
+
+    def render(self, frame, options, path_begin=0, path_count=None, threads=1, want_paths=True, want_film=True):
+        cam = frame.camera
+        total = cam.width * cam.height * frame.samples_per_pixel
+        if path_count is None:
+            path_count = total - path_begin
+        paths = (self.abi.PathResult * path_count)() if want_paths else None
+        pixels = cam.width * cam.height
+        film = np.zeros(pixels * 3, dtype=np.float32) if want_film else None
+        film_c = np.zeros(pixels * 6, dtype=np.float64) if want_film else None
+        counts = np.zeros(pixels, dtype=np.uint32) if want_film else None
+        report = self.abi.TransportReport()
+        tc = np.zeros(6, dtype=np.uint64)
+        status = self.lib.bfo_render(self.handle, C.byref(frame), C.byref(options), path_begin, path_count, threads,
+                                     C.addressof(paths) if want_paths else None,
+                                     film.ctypes.data if want_film else None,
+                                     film_c.ctypes.data if want_film else None,
+                                     counts.ctypes.data if want_film else None, C.byref(report), tc.ctypes.data)
+        if status != 0:
+            raise RuntimeError(f"oracle render failed with status {status}")
+        result = {"report": report, "traversal_counters": tc}
+        if want_paths:
+            result["paths"] = paths
+        if want_film:
+            result["film"] = film.reshape(cam.height, cam.width, 3)
+            comp = film_c.reshape(cam.height, cam.width, 3, 2)
+            result["film_compensated"] = comp[..., 0] + comp[..., 1]
+            result["film_counts"] = counts.reshape(cam.height, cam.width)
+        return result
