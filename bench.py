@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X wavefront path tracer (plain forward).
+
+One step = one full frame through trace_xpu_wavefront_transport: device camera rays, every bounce
+(closest hit, shade with NEE/MIS, shadow rays, stable queue compaction) and device film
+accumulation, with the scene and BVH already resident in HBM. The default workload is the plain
+forward at scale: a 32x32 grid of displaced subdivision-3 icospheres (1 310 720 triangles) with
+per-object Lambert / GGX conductor / GGX dielectric materials, 64 emissive triangles plus a
+constant environment, 1920x1080 at 64 spp, 4 wavelengths per path, max depth 16, power MIS.
+
+Printed: one JSON line. `value` is Mrays/s = (closest-hit + shadow queries) / frame time.
+`roofline` prices the dominant kernel (closest-hit traversal) by its algorithmic bytes: per ray
+48 B ray + 80 B result + 64 B per BVH node visited + 48 B per triangle tested + 48 B per instance
+entered, counted by the traversal counters of an untimed warmup frame, divided by the kernel's
+summed HIP-event duration over the timed frames (events recorded on the launch stream by the
+library's stage instrumentation). `cpu_baseline` times the CPU wavefront restatement (the oracle,
+threaded with contiguous partitions) on a bounded band of the same frame on this host.
+"""
+import argparse
+import importlib
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+METRIC = "Mrays/s plain vs batched; adjoint/forward time at grad rel err ≤1e-2"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = ROOT / "profiles" / "traffic_trace_closest_grid_c1.json"
+
+
+def load_package():
+    name = "blackframe_renderer_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(
+        name, ROOT / "blackframe-renderer_amd" / "__init__.py",
+        submodule_search_locations=[str(ROOT / "blackframe-renderer_amd")])
+    module = importlib.util.module_from_spec(spec)
+    sys.modules[name] = module
+    spec.loader.exec_module(module)
+    return module
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(bf, args):
+    if args.workload == "cornell":
+        scene, cam = bf.scenes.cornell(args.width or 64, args.height or 64)
+        spp = args.spp or 16
+        options = bf.scenes.transport_options(max_depth=args.depth or 8, mis=0)
+        desc = f"Cornell box {cam.width}x{cam.height}x{spp} spp, depth {options.diffuse_depth_limit}, balance MIS"
+    else:
+        scene, cam = bf.scenes.icosphere_grid(args.width or 1920, args.height or 1080, materials=args.materials)
+        spp = args.spp or 64
+        options = bf.scenes.transport_options(max_depth=args.depth or 16, mis=1)
+        desc = (f"32x32 displaced icosphere grid ({scene.triangle_count} tris, {args.materials} materials), "
+                f"64 emissive tris + environment, {cam.width}x{cam.height}x{spp} spp, "
+                f"depth {options.diffuse_depth_limit}, power MIS")
+    frame = bf.scenes.frame_desc(cam, spp, seed=args.seed)
+    return scene, cam, frame, options, desc
+
+
+def algorithmic_bytes(report, tc, path_count):
+    """Per-kernel and whole-frame algorithmic bytes (ray/result records, node/triangle/instance
+    fetches, per-lane state records, queue slots)."""
+    closest = 128 * report.closest_hit_queries + 64 * int(tc[0]) + 48 * int(tc[1]) + 48 * int(tc[2])
+    shadow = 64 * report.shadow_queries + 64 * int(tc[3]) + 48 * int(tc[4]) + 48 * int(tc[5])
+    slots = report.closest_hit_queries + report.shade_lanes + report.shadow_queries + (report.closest_hit_queries - path_count)
+    frame = (closest + shadow + 680 * report.shade_lanes + 192 * report.miss_lanes + 208 * report.shadow_queries
+             + 416 * path_count + 8 * slots)
+    return closest, shadow, frame
+
+
+def cpu_baseline(bf, scene, frame, options, target_seconds, threads):
+    """Times the oracle's threaded CPU wavefront on a contiguous range of path slots starting at the
+    middle pixel row of the first sample pass (no film), growing the range until the run lasts about
+    target_seconds."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    oracle_module = importlib.import_module("oracle")
+    t0 = time.perf_counter()
+    oscene = oracle_module.Oracle(bf.abi).scene(scene)
+    build_s = time.perf_counter() - t0
+    cam = frame.camera
+    pixels = cam.width * cam.height
+    total = pixels * frame.samples_per_pixel
+    # a band of pixel rows centred in the image, continued over the following sample passes
+    begin = (cam.height // 2) * cam.width
+    count = min(4096, total - begin)
+    while True:
+        t0 = time.perf_counter()
+        result = oscene.render(frame, options, path_begin=begin, path_count=count, threads=threads, want_paths=False,
+                               want_film=False)
+        elapsed = time.perf_counter() - t0
+        if elapsed >= target_seconds * 0.5 or count == total - begin:
+            break
+        count = min(total - begin, max(2 * count, int(count * target_seconds / max(elapsed, 1e-3))))
+    rep = result["report"]
+    rays = rep.closest_hit_queries + rep.shadow_queries
+    return {
+        "value": rays / elapsed / 1e6,
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"path slots {begin}..{begin + count - 1} of the frame (from pixel row {cam.height // 2} of pass 0 "
+                   f"on; {count} paths, {rays} rays) in {elapsed:.1f} s; paths/s {count / elapsed:.0f}; "
+                   f"oracle BVH build {build_s:.1f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=("grid", "cornell"), default="grid")
+    ap.add_argument("--materials", choices=("mixed", "lambert"), default="mixed")
+    ap.add_argument("--width", type=int, default=0)
+    ap.add_argument("--height", type=int, default=0)
+    ap.add_argument("--spp", type=int, default=0)
+    ap.add_argument("--depth", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--capacity", type=int, default=0, help="paths in flight per launch (0: the whole frame)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the frame would shard by "
+              "sample pass with one film reduction, which this single-GPU build does not implement)", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 1
+    torch.cuda.set_device(0)
+    bf = load_package()
+    hip = importlib.import_module("blackframe_renderer_amd.hip")
+
+    t0 = time.perf_counter()
+    scene, cam, frame, options, desc = build_workload(bf, args)
+    log(f"scene generated in {time.perf_counter() - t0:.1f} s: {desc}")
+    t0 = time.perf_counter()
+    xscene = hip.XpuScene(scene)
+    torch.cuda.synchronize()
+    upload_s = time.perf_counter() - t0
+    log(f"BVH build + upload {upload_s:.1f} s")
+    pixels = cam.width * cam.height
+    path_count = pixels * frame.samples_per_pixel
+    capacity = args.capacity or min(path_count, (2 ** 31 - 1) // pixels * pixels)
+    ws = hip.XpuWavefrontTransportWorkspace(capacity)
+    film = hip.Film(cam.width, cam.height)
+
+    # warmup: the first frame also gathers the deterministic traversal counters
+    report = counters = None
+    for w in range(max(1, args.warmup)):
+        t0 = time.perf_counter()
+        r = hip.trace_xpu_wavefront_transport(ws, xscene, frame, options, film=film, want_counters=(w == 0))
+        if w == 0:
+            report, counters = r["report"], r["traversal_counters"]
+        log(f"warmup {w}: {time.perf_counter() - t0:.2f} s, {r['report'].bounces} bounces")
+    closest_bytes, shadow_bytes, frame_bytes = algorithmic_bytes(report, counters, path_count)
+    rays = report.closest_hit_queries + report.shadow_queries
+
+    options.instrumentation = bf.abi.INSTRUMENT_STAGE_EVENTS
+    stage_ns = np.zeros(8, dtype=np.float64)
+    stage_dispatches = np.zeros(8, dtype=np.float64)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        r = hip.trace_xpu_wavefront_transport(ws, xscene, frame, options, film=film, synchronize=False, stream=stream)
+        rep = r["report"]
+        for k in range(8):
+            stage_ns[k] += rep.stage_metrics[k].gpu_elapsed_nanoseconds
+            stage_dispatches[k] += rep.stage_metrics[k].kernel_dispatches
+        assert rep.closest_hit_queries + rep.shadow_queries == rays, "non-deterministic ray count"
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    mrays = rays * args.steps / elapsed / 1e6
+    log(f"timed {args.steps} frames: {ms_per_step:.1f} ms/frame, {mrays:.1f} Mrays/s")
+
+    closest_launches = stage_dispatches[1] / args.steps
+    closest_avg_s = stage_ns[1] / max(stage_dispatches[1], 1) * 1e-9
+    achieved = closest_bytes / closest_launches / closest_avg_s / 1e9
+    traffic = None
+    if TRAFFIC_SUMMARY.exists() and args.workload == "grid" and args.materials == "mixed":
+        summary = json.loads(TRAFFIC_SUMMARY.read_text())
+        traffic = summary.get("hbm_bytes_per_launch")
+    stages = {name: {"ms_per_frame": round(stage_ns[k] / args.steps * 1e-6, 3),
+                     "dispatches_per_frame": int(stage_dispatches[k] / args.steps)}
+              for k, name in enumerate(bf.abi.STAGES[:7])}
+    line = {
+        "metric": METRIC,
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": f"synthetic (seeded procedural scene, frame seed {args.seed})",
+        "config": {"workload": desc, "mode": "plain", "paths_per_frame": path_count, "rays_per_frame": rays,
+                   "paths_in_flight": capacity, "bvh_build_and_upload_s": round(upload_s, 2)},
+        "paths_per_s": round(path_count * args.steps / elapsed, 1),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_trace_closest",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(closest_bytes / closest_launches),
+            "avg_launch_ms": round(closest_avg_s * 1e3, 4),
+            "frame_algorithmic_gbs": round(frame_bytes * args.steps / elapsed / 1e9, 2),
+        },
+        "stages": stages,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        log(f"CPU baseline with {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(bf, scene, frame, options, args.cpu_seconds, threads)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

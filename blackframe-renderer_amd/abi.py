@@ -138,7 +138,8 @@ class TransportOptions(C.Structure):
         ("transmission_depth_limit", C.c_uint32), ("volume_depth_limit", C.c_uint32),
         ("russian_roulette_enabled", C.c_uint32), ("russian_roulette_first_depth", C.c_uint32),
         ("russian_roulette_minimum", C.c_float), ("russian_roulette_maximum", C.c_float),
-        ("packet_wavelengths", C.c_float * 4), ("record_queues", C.c_uint32), ("reserved", C.c_uint32 * 3),
+        ("packet_wavelengths", C.c_float * 4), ("record_queues", C.c_uint32), ("instrumentation", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -148,6 +149,13 @@ class FrameDesc(C.Structure):
 
 
 MAX_BOUNCES = 64
+INSTRUMENT_DISABLED, INSTRUMENT_STAGE_EVENTS = 0, 1
+STAGES = ("camera", "closest_hit", "shade", "shadow", "compaction", "film", "auxiliary", "reserved")
+
+
+class StageMetric(C.Structure):
+    _fields_ = [("kernel_dispatches", C.c_uint64), ("kernel_lanes", C.c_uint64),
+                ("gpu_elapsed_nanoseconds", C.c_uint64)]
 
 
 class TransportReport(C.Structure):
@@ -158,6 +166,7 @@ class TransportReport(C.Structure):
         ("first_failure_slot", C.c_uint32), ("first_failure_code", C.c_uint32), ("reserved", C.c_uint32),
         ("ray_queue_hash", C.c_uint64 * MAX_BOUNCES), ("ray_queue_size", C.c_uint32 * MAX_BOUNCES),
         ("shadow_queue_hash", C.c_uint64 * MAX_BOUNCES), ("shadow_queue_size", C.c_uint32 * MAX_BOUNCES),
+        ("stage_metrics", StageMetric * 8),
     ]
 
 
@@ -171,5 +180,5 @@ class PathResult(C.Structure):
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 136), (CameraDesc, 96),
-                       (TransportOptions, 80), (FrameDesc, 112), (TransportReport, 1616)):
+                       (TransportOptions, 80), (FrameDesc, 112), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))

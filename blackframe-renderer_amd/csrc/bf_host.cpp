@@ -13,6 +13,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/blackframe_hip.h"
@@ -748,7 +749,52 @@ struct bf_hip_workspace {
     unsigned long long* traversal = nullptr; /* 6 counters */
     unsigned long long* hashes = nullptr;    /* 2 words */
     uint32_t* host_words = nullptr;          /* pinned readback */
+    /* stage instrumentation: a reusable pool of hipEvent pairs, drained at every host sync */
+    std::vector<hipEvent_t> events;
+    std::vector<std::pair<uint32_t, uint32_t>> pending; /* (stage, first event of the pair) */
+    uint32_t events_used = 0;
 };
+
+namespace {
+
+/* Records the launch(es) of one stage, bracketed by an event pair when instrumentation is on. */
+template <typename Launch>
+hipError_t staged(bf_hip_workspace* w, bf_transport_report& rep, bool instrument, uint32_t stage, uint64_t dispatches,
+                  hipStream_t stream, Launch&& launch) {
+    rep.stage_metrics[stage].kernel_dispatches += dispatches;
+    if (!instrument) return launch();
+    while (w->events.size() < (size_t)w->events_used + 2) {
+        hipEvent_t e = nullptr;
+        const hipError_t st = hipEventCreate(&e);
+        if (st != hipSuccess) return st;
+        w->events.push_back(e);
+    }
+    const uint32_t first = w->events_used;
+    w->events_used += 2;
+    hipError_t st = hipEventRecord(w->events[first], stream);
+    if (st != hipSuccess) return st;
+    st = launch();
+    if (st != hipSuccess) return st;
+    st = hipEventRecord(w->events[first + 1], stream);
+    if (st != hipSuccess) return st;
+    w->pending.emplace_back(stage, first);
+    return hipSuccess;
+}
+
+/* Folds the elapsed times of every recorded pair into the report; the stream must be idle. */
+hipError_t drain_stage_events(bf_hip_workspace* w, bf_transport_report& rep) {
+    for (const auto& p : w->pending) {
+        float ms = 0.0f;
+        const hipError_t st = hipEventElapsedTime(&ms, w->events[p.second], w->events[p.second + 1]);
+        if (st != hipSuccess) return st;
+        rep.stage_metrics[p.first].gpu_elapsed_nanoseconds += (uint64_t)std::llround((double)ms * 1.0e6);
+    }
+    w->pending.clear();
+    w->events_used = 0;
+    return hipSuccess;
+}
+
+} // namespace
 
 extern "C" {
 
@@ -909,6 +955,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
 
 int blackframe_hip_workspace_destroy(bf_hip_workspace* w) {
     if (!w) return BF_OK;
+    for (hipEvent_t e : w->events) (void)hipEventDestroy(e);
     hipError_t e1 = w->block ? hipFree(w->block) : hipSuccess;
     hipError_t e2 = w->host_words ? hipHostFree(w->host_words) : hipSuccess;
     delete w;
@@ -974,10 +1021,18 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
 
     bf_transport_report rep{};
     rep.first_failure_slot = BF_INVALID_INDEX;
-    HIP_TRY(bfk_set_words((uint32_t*)w->traversal, 12, 0u, stream));
-    HIP_TRY(bfk_set_words(w->failure, 1, 0u, stream));
-    HIP_TRY(bfk_set_words(w->failure + 1, 1, BF_INVALID_INDEX, stream));
-    HIP_TRY(bfk_set_words(w->sample_counts, 2, 0u, stream));
+    if (o->instrumentation > BF_INSTRUMENT_STAGE_EVENTS) return set_error(BF_INVALID_ARGUMENT, "unknown instrumentation mode");
+    const bool instrument = o->instrumentation == BF_INSTRUMENT_STAGE_EVENTS;
+    w->pending.clear();
+    w->events_used = 0;
+#define STAGE(stage, dispatches, call) HIP_TRY(staged(w, rep, instrument, (stage), (dispatches), stream, [&] { return (call); }))
+    STAGE(BF_STAGE_AUXILIARY, 4, [&] {
+        hipError_t e = bfk_set_words((uint32_t*)w->traversal, 12, 0u, stream);
+        if (e == hipSuccess) e = bfk_set_words(w->failure, 1, 0u, stream);
+        if (e == hipSuccess) e = bfk_set_words(w->failure + 1, 1, BF_INVALID_INDEX, stream);
+        if (e == hipSuccess) e = bfk_set_words(w->sample_counts, 2, 0u, stream);
+        return e;
+    }());
     unsigned long long* counters = traversal_counters ? w->traversal : nullptr;
     const uint64_t passes_per_launch = w->capacity / pixels;
     const uint64_t bound = (uint64_t)o->diffuse_depth_limit + o->glossy_depth_limit + o->specular_depth_limit + 2u;
@@ -988,56 +1043,69 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
         P.slot_base = pass0 * pixels;
         uint32_t* ray_q = w->queues[0];
         uint32_t* cont_q = w->queues[3];
-        HIP_TRY(bfk_camera(&P, n, ray_q, stream));
-        HIP_TRY(bfk_set_words(w->sizes, 1, n, stream));
+        STAGE(BF_STAGE_CAMERA, 1, bfk_camera(&P, n, ray_q, stream));
+        rep.stage_metrics[BF_STAGE_CAMERA].kernel_lanes += n;
+        STAGE(BF_STAGE_AUXILIARY, 1, bfk_set_words(w->sizes, 1, n, stream));
         uint32_t n_ray = n;
         uint32_t bounce = 0;
         while (n_ray != 0 && bounce <= bound) {
             const bool record = o->record_queues && pass0 == 0 && bounce < BF_REPORT_MAX_BOUNCES;
             if (record) {
-                HIP_TRY(bfk_queue_hash(ray_q, &w->sizes[0], w->hashes, stream));
+                STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(ray_q, &w->sizes[0], w->hashes, stream));
                 HIP_TRY(hipMemcpyAsync(&rep.ray_queue_hash[bounce], w->hashes, 8, hipMemcpyDeviceToHost, stream));
                 rep.ray_queue_size[bounce] = n_ray;
             }
-            HIP_TRY(bfk_trace_closest(&P, ray_q, &w->sizes[0], n_ray, w->route, counters, stream));
-            HIP_TRY(bfk_compact(w->route, &w->sizes[0], n_ray, ray_q, w->block_counts, w->block_offsets, w->queues[1],
-                                &w->sizes[1], 0u, w->queues[2], &w->sizes[4], 0u, stream));
-            HIP_TRY(bfk_shade(&P, w->queues[1], &w->sizes[1], n_ray, w->route, w->sample_counts, stream));
-            HIP_TRY(bfk_compact(w->route, &w->sizes[1], n_ray, w->queues[1], w->block_counts, w->block_offsets, w->queues[2],
-                                &w->sizes[2], 0u, cont_q, &w->sizes[3], 0u, stream));
+            STAGE(BF_STAGE_CLOSEST_HIT, 1, bfk_trace_closest(&P, ray_q, &w->sizes[0], n_ray, w->route, counters, stream));
+            STAGE(BF_STAGE_COMPACTION, 3,
+                  bfk_compact(w->route, &w->sizes[0], n_ray, ray_q, w->block_counts, w->block_offsets, w->queues[1],
+                              &w->sizes[1], 0u, w->queues[2], &w->sizes[4], 0u, stream));
+            STAGE(BF_STAGE_SHADE, 1, bfk_shade(&P, w->queues[1], &w->sizes[1], n_ray, w->route, w->sample_counts, stream));
+            STAGE(BF_STAGE_COMPACTION, 3,
+                  bfk_compact(w->route, &w->sizes[1], n_ray, w->queues[1], w->block_counts, w->block_offsets, w->queues[2],
+                              &w->sizes[2], 0u, cont_q, &w->sizes[3], 0u, stream));
             if (record) {
-                HIP_TRY(bfk_queue_hash(w->queues[2], &w->sizes[2], w->hashes + 1, stream));
+                STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(w->queues[2], &w->sizes[2], w->hashes + 1, stream));
                 HIP_TRY(hipMemcpyAsync(&rep.shadow_queue_hash[bounce], w->hashes + 1, 8, hipMemcpyDeviceToHost, stream));
             }
-            HIP_TRY(bfk_trace_shadow(&P, w->queues[2], &w->sizes[2], n_ray, w->route, counters, stream));
-            HIP_TRY(bfk_compact(w->route, &w->sizes[2], n_ray, w->queues[2], w->block_counts, w->block_offsets, w->queues[1],
-                                &w->sizes[5], 0u, cont_q, &w->sizes[3], 1u, stream));
+            STAGE(BF_STAGE_SHADOW, 1, bfk_trace_shadow(&P, w->queues[2], &w->sizes[2], n_ray, w->route, counters, stream));
+            STAGE(BF_STAGE_COMPACTION, 3,
+                  bfk_compact(w->route, &w->sizes[2], n_ray, w->queues[2], w->block_counts, w->block_offsets, w->queues[1],
+                              &w->sizes[5], 0u, cont_q, &w->sizes[3], 1u, stream));
             HIP_TRY(hipMemcpyAsync(w->host_words, w->sizes, 16, hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
+            HIP_TRY(drain_stage_events(w, rep));
             rep.closest_hit_queries += n_ray;
             rep.shade_lanes += w->host_words[1];
             rep.miss_lanes += n_ray - w->host_words[1];
             rep.shadow_queries += w->host_words[2];
+            rep.stage_metrics[BF_STAGE_CLOSEST_HIT].kernel_lanes += n_ray;
+            rep.stage_metrics[BF_STAGE_SHADE].kernel_lanes += w->host_words[1];
+            rep.stage_metrics[BF_STAGE_SHADOW].kernel_lanes += w->host_words[2];
+            rep.stage_metrics[BF_STAGE_COMPACTION].kernel_lanes += (uint64_t)n_ray + w->host_words[1] + w->host_words[2];
             if (record) rep.shadow_queue_size[bounce] = w->host_words[2];
             const uint32_t next = w->host_words[3];
             std::swap(w->queues[0], w->queues[3]);
             ray_q = w->queues[0];
             cont_q = w->queues[3];
-            HIP_TRY(bfk_copy_word(&w->sizes[3], &w->sizes[0], stream));
+            STAGE(BF_STAGE_AUXILIARY, 1, bfk_copy_word(&w->sizes[3], &w->sizes[0], stream));
             n_ray = next;
             ++bounce;
         }
         rep.bounces = std::max<uint32_t>(rep.bounces, bounce);
-        HIP_TRY(bfk_film(&w->paths, (uint32_t)pixels, (uint32_t)passes, 1.0f / 470.0f, normalization, film,
-                         film_compensated, film_counts, stream));
-        if (paths) HIP_TRY(bfk_export_paths(&w->paths, n, paths + P.slot_base, stream));
+        STAGE(BF_STAGE_FILM, 1,
+              bfk_film(&w->paths, (uint32_t)pixels, (uint32_t)passes, 1.0f / 470.0f, normalization, film,
+                       film_compensated, film_counts, stream));
+        rep.stage_metrics[BF_STAGE_FILM].kernel_lanes += n;
+        if (paths) STAGE(BF_STAGE_AUXILIARY, 1, bfk_export_paths(&w->paths, n, paths + P.slot_base, stream));
     }
+#undef STAGE
     rep.path_count = pixels * frame->samples_per_pixel;
     HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 8, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipMemcpyAsync(w->host_words + 2, w->sample_counts, 8, hipMemcpyDeviceToHost, stream));
     unsigned long long trav[6] = {0, 0, 0, 0, 0, 0};
     if (traversal_counters) HIP_TRY(hipMemcpyAsync(trav, w->traversal, sizeof trav, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(drain_stage_events(w, rep));
     rep.failed_paths = w->host_words[0];
     rep.first_failure_slot = w->host_words[1];
     rep.closure_samples = w->host_words[2];

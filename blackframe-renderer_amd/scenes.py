@@ -411,5 +411,5 @@ def icosphere_grid(width=1920, height=1080, grid=32, subdivisions=3, seed=0x5EED
     s.set_environment(0.05)
     s.finalize()
     extent = spacing * grid / 2.0
-    camera = look_at_camera((0.0, extent * 0.9, extent * 1.6), (0.0, -2.0, 0.0), width, height, 0.9)
+    camera = look_at_camera((0.0, extent * 1.3, extent * 0.75), (0.0, -2.0, -extent * 0.1), width, height, 0.9)
     return s, camera

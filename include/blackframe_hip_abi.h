@@ -292,7 +292,8 @@ typedef struct bf_transport_options {
     float russian_roulette_maximum;
     float packet_wavelengths[BF_SPECTRUM_LANES]; /* used in frame-packet mode */
     uint32_t record_queues; /* nonzero: fill bf_transport_report queue fingerprints */
-    uint32_t reserved[3];
+    uint32_t instrumentation; /* bf_instrumentation_mode: BF_INSTRUMENT_STAGE_EVENTS fills stage_metrics */
+    uint32_t reserved[2];
 } bf_transport_options;
 
 /* One frame of work: spp passes over width*height pixels; path slot p = s*W*H + y*W + x. */
@@ -302,6 +303,26 @@ typedef struct bf_frame_desc {
     uint32_t samples_per_pixel;
     uint32_t first_sample_index;
 } bf_frame_desc;
+
+/* Per-stage GPU metrics, mirroring CudaWavefrontStageMetric (WavefrontTransport.hpp:72-96): the
+ * dispatch and lane counts are deterministic, the elapsed time (hipEvent pairs recorded on the
+ * transport stream around every dispatch of the stage) is observational. */
+enum bf_instrumentation_mode { BF_INSTRUMENT_DISABLED = 0, BF_INSTRUMENT_STAGE_EVENTS = 1 };
+enum bf_stage {
+    BF_STAGE_CAMERA = 0,
+    BF_STAGE_CLOSEST_HIT = 1,
+    BF_STAGE_SHADE = 2,
+    BF_STAGE_SHADOW = 3,
+    BF_STAGE_COMPACTION = 4,
+    BF_STAGE_FILM = 5,
+    BF_STAGE_AUXILIARY = 6, /* queue fingerprints, path export, counter resets */
+    BF_STAGE_COUNT = 8
+};
+typedef struct bf_stage_metric {
+    uint64_t kernel_dispatches;
+    uint64_t kernel_lanes;            /* lanes launched (queue sizes; grid-stride upper bounds excluded) */
+    uint64_t gpu_elapsed_nanoseconds; /* 0 unless instrumentation is BF_INSTRUMENT_STAGE_EVENTS */
+} bf_stage_metric;
 
 #define BF_REPORT_MAX_BOUNCES 64u
 typedef struct bf_transport_report {
@@ -322,6 +343,7 @@ typedef struct bf_transport_report {
     uint32_t ray_queue_size[BF_REPORT_MAX_BOUNCES];
     uint64_t shadow_queue_hash[BF_REPORT_MAX_BOUNCES];
     uint32_t shadow_queue_size[BF_REPORT_MAX_BOUNCES];
+    bf_stage_metric stage_metrics[BF_STAGE_COUNT];
 } bf_transport_report;
 
 /* Per-path terminal record written by the transport (optional output). */
