@@ -62,6 +62,8 @@ def build_workload(bf, args):
                 f"64 emissive tris + environment, {cam.width}x{cam.height}x{spp} spp, "
                 f"depth {options.diffuse_depth_limit}, power MIS")
     frame = bf.scenes.frame_desc(cam, spp, seed=args.seed)
+    # a numerically failed path is counted and reported instead of aborting the frame
+    options.failure_policy = bf.abi.FAILURES_COUNT
     return scene, cam, frame, options, desc
 
 
@@ -127,6 +129,8 @@ def main():
     ap.add_argument("--capacity", type=int, default=0, help="paths in flight per launch (0: the whole frame)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the frame would shard by "
@@ -139,6 +143,14 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": src.numel() * 4}), flush=True)
+        return 0
     bf = load_package()
     hip = importlib.import_module("blackframe_renderer_amd.hip")
 
@@ -189,10 +201,11 @@ def main():
     closest_launches = stage_dispatches[1] / args.steps
     closest_avg_s = stage_ns[1] / max(stage_dispatches[1], 1) * 1e-9
     achieved = closest_bytes / closest_launches / closest_avg_s / 1e9
+    # HBM bytes per launch of the same kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # runs of this workload, scaled by the counters' reading for a plain copy of known size
     traffic = None
-    if TRAFFIC_SUMMARY.exists() and args.workload == "grid" and args.materials == "mixed":
-        summary = json.loads(TRAFFIC_SUMMARY.read_text())
-        traffic = summary.get("hbm_bytes_per_launch")
+    if TRAFFIC_SUMMARY.exists() and args.workload == "grid" and args.materials == "mixed" and not args.capacity:
+        traffic = json.loads(TRAFFIC_SUMMARY.read_text()).get("hbm_bytes_per_launch")
     stages = {name: {"ms_per_frame": round(stage_ns[k] / args.steps * 1e-6, 3),
                      "dispatches_per_frame": int(stage_dispatches[k] / args.steps)}
               for k, name in enumerate(bf.abi.STAGES[:7])}
@@ -210,7 +223,9 @@ def main():
         "dtype": "f32",
         "data": f"synthetic (seeded procedural scene, frame seed {args.seed})",
         "config": {"workload": desc, "mode": "plain", "paths_per_frame": path_count, "rays_per_frame": rays,
-                   "paths_in_flight": capacity, "bvh_build_and_upload_s": round(upload_s, 2)},
+                   "paths_in_flight": capacity, "bvh_build_and_upload_s": round(upload_s, 2),
+                   "failed_paths_per_frame": report.failed_paths,
+                   "first_failure": [report.first_failure_slot, report.first_failure_code] if report.failed_paths else None},
         "paths_per_s": round(path_count * args.steps / elapsed, 1),
         "roofline": {
             "bound": "hbm",
@@ -220,6 +235,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
+            "traffic_source": TRAFFIC_SUMMARY.name if traffic else None,
             "algorithmic_bytes_per_launch": round(closest_bytes / closest_launches),
             "avg_launch_ms": round(closest_avg_s * 1e3, 4),
             "frame_algorithmic_gbs": round(frame_bytes * args.steps / elapsed / 1e9, 2),

@@ -74,6 +74,11 @@ class BvhInstanceRef(C.Structure):
     ]
 
 
+class ClosureRecord(C.Structure):
+    """A closure resolved at the path's four wavelengths (ClosureRecord, TransportLobesDevice.cuh:66-71)."""
+    _fields_ = [("kind", C.c_uint32), ("lobes", C.c_uint32), ("weight", C.c_float * 4), ("parameters", C.c_float * 10)]
+
+
 class SpectrumDesc(C.Structure):
     _fields_ = [("offset", C.c_uint32), ("count", C.c_uint32)]
 
@@ -139,7 +144,7 @@ class TransportOptions(C.Structure):
         ("russian_roulette_enabled", C.c_uint32), ("russian_roulette_first_depth", C.c_uint32),
         ("russian_roulette_minimum", C.c_float), ("russian_roulette_maximum", C.c_float),
         ("packet_wavelengths", C.c_float * 4), ("record_queues", C.c_uint32), ("instrumentation", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("failure_policy", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
 
@@ -150,6 +155,7 @@ class FrameDesc(C.Structure):
 
 MAX_BOUNCES = 64
 INSTRUMENT_DISABLED, INSTRUMENT_STAGE_EVENTS = 0, 1
+FAILURES_ABORT, FAILURES_COUNT = 0, 1
 STAGES = ("camera", "closest_hit", "shade", "shadow", "compaction", "film", "auxiliary", "reserved")
 
 
@@ -178,7 +184,7 @@ class PathResult(C.Structure):
 
 
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
-                       (BvhNode, 48), (BvhBlas, 32), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
+                       (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 136), (CameraDesc, 96),
                        (TransportOptions, 80), (FrameDesc, 112), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))

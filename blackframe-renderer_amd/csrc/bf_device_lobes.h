@@ -94,19 +94,6 @@ BF_D int cosine_hemisphere(float u, float v, v3f& out) {
     return unit_vector(out) ? BF_L_OK : BF_L_NOT_REPRESENTABLE;
 }
 
-/* ---- record dispatch (TransportLobesDevice.cuh:2102-2260) ---------------------------------- */
-BF_D int record_eval(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float* out) {
-    (void)allowed;
-    if (c.kind == BF_CLOSURE_LAMBERTIAN) return lambert_eval(c.weight, wo, wi, out);
-    return BF_L_INVALID;
-}
-
-BF_D int record_pdf(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float& pdf) {
-    (void)allowed;
-    if (c.kind == BF_CLOSURE_LAMBERTIAN) return lambert_pdf(wo, wi, pdf);
-    return BF_L_INVALID;
-}
-
 BF_D void empty_sample(lobe_sample_t& s, int status) {
     s.value[0] = s.value[1] = s.value[2] = s.value[3] = 0.0f;
     s.incoming = mk(0.0f, 0.0f, 0.0f);
@@ -117,37 +104,91 @@ BF_D void empty_sample(lobe_sample_t& s, int status) {
     s.status = status;
 }
 
-BF_D void record_sample(const closure_t& c, uint32_t allowed, v3f wo, float event, float u, float v, lobe_sample_t& out) {
-    (void)allowed;
-    (void)event;
-    empty_sample(out, BF_L_INVALID);
-    if (c.kind == BF_CLOSURE_LAMBERTIAN) {
-        if (!reflectance_ok(c.weight) || !unit_vector(wo)) return;
-        v3f wi;
-        const int st = cosine_hemisphere(u, v, wi);
-        if (st != BF_L_OK) {
-            out.status = st;
-            return;
-        }
-        if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
-            out.incoming = wi;
-            out.status = BF_L_OUTSIDE;
-            return;
-        }
-        float value[4], pdf;
-        const int s1 = lambert_eval(c.weight, wo, wi, value);
-        const int s2 = lambert_pdf(wo, wi, pdf);
-        if (s1 != BF_L_OK || s2 != BF_L_OK) {
-            out.status = s1 != BF_L_OK ? s1 : s2;
-            return;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) out.value[k] = value[k];
+#include "bf_device_microfacet.h"
+
+/* ---- record dispatch (TransportLobesDevice.cuh:2092-2260); transport mode radiance ---------- */
+BF_D int record_eval(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float* out) {
+    if (!mf_valid_closure(&c)) return BF_L_INVALID;
+    const float* p = c.parameters;
+    switch (c.kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+        return lambert_eval(c.weight, wo, wi, out);
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        return mf_rough_diffuse_eval(c.weight, p[0], wo, wi, out);
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        return mf_conductor_eval(c.weight, p, p + 4, p[8], p[9], wo, wi, out);
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        return mf_dielectric_eval_masked(c.weight, p[0], p[1], p[2], p[3], wo, wi, BF_TRANSPORT_RADIANCE, allowed, out);
+    default:
+        return BF_L_INVALID;
+    }
+}
+
+BF_D int record_pdf(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float& pdf) {
+    if (!mf_valid_closure(&c)) return BF_L_INVALID;
+    const float* p = c.parameters;
+    switch (c.kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        return lambert_pdf(wo, wi, pdf);
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        return mf_conductor_pdf(p[8], p[9], wo, wi, &pdf);
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        return mf_dielectric_pdf_masked(c.weight, p[0], p[1], p[2], p[3], wo, wi, BF_TRANSPORT_RADIANCE, allowed, &pdf);
+    default:
+        return BF_L_INVALID;
+    }
+}
+
+/* sample_lambert / sample_rough_diffuse: cosine hemisphere, then eval and pdf at the sample */
+BF_D void diffuse_sample(const closure_t& c, v3f wo, float u, float v, lobe_sample_t& out) {
+    v3f wi;
+    const int st = cosine_hemisphere(u, v, wi);
+    if (st != BF_L_OK) {
+        out.status = st;
+        return;
+    }
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
         out.incoming = wi;
-        out.pdf = pdf;
-        out.lobes = c.lobes;
-        out.eta_scale = 1.0f;
-        out.status = BF_L_OK;
+        out.status = BF_L_OUTSIDE;
+        return;
+    }
+    float value[4], pdf = 0.0f;
+    const int s1 = c.kind == BF_CLOSURE_LAMBERTIAN ? lambert_eval(c.weight, wo, wi, value)
+                                                   : mf_rough_diffuse_eval(c.weight, c.parameters[0], wo, wi, value);
+    const int s2 = lambert_pdf(wo, wi, pdf);
+    if (s1 != BF_L_OK || s2 != BF_L_OK) {
+        out.status = s1 != BF_L_OK ? s1 : s2;
+        return;
+    }
+    copy4(out.value, value);
+    out.incoming = wi;
+    out.pdf = pdf;
+    out.discrete = 0u;
+    out.lobes = BF_LOBE_DIFFUSE | BF_LOBE_REFLECTION;
+    out.eta_scale = 1.0f;
+    out.status = BF_L_OK;
+}
+
+BF_D void record_sample(const closure_t& c, uint32_t allowed, v3f wo, float event, float u, float v, lobe_sample_t& out) {
+    empty_sample(out, BF_L_INVALID);
+    if (!mf_valid_closure(&c)) return;
+    const float* p = c.parameters;
+    switch (c.kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        if (!unit_vector(wo)) return;
+        diffuse_sample(c, wo, u, v, out);
+        return;
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        mf_conductor_sample(c.weight, p, p + 4, p[8], p[9], wo, u, v, &out);
+        return;
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        mf_dielectric_sample_masked(c.weight, p[0], p[1], p[2], p[3], wo, event, u, v, BF_TRANSPORT_RADIANCE, allowed,
+                                    &out);
+        return;
+    default:
+        return;
     }
 }
 

@@ -744,7 +744,7 @@ struct bf_hip_workspace {
     uint32_t* block_counts = nullptr;
     uint32_t* block_offsets = nullptr;
     uint32_t* sizes = nullptr;     /* [0] ray [1] hit [2] shadow [3] cont [4..7] scratch */
-    uint32_t* failure = nullptr;   /* [0] count [1] first slot */
+    uint32_t* failure = nullptr;   /* [0] count, [2..3] lowest (global slot << 8 | failure code) */
     uint32_t* sample_counts = nullptr;
     unsigned long long* traversal = nullptr; /* 6 counters */
     unsigned long long* hashes = nullptr;    /* 2 words */
@@ -1022,14 +1022,15 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     bf_transport_report rep{};
     rep.first_failure_slot = BF_INVALID_INDEX;
     if (o->instrumentation > BF_INSTRUMENT_STAGE_EVENTS) return set_error(BF_INVALID_ARGUMENT, "unknown instrumentation mode");
+    if (o->failure_policy > BF_FAILURES_COUNT) return set_error(BF_INVALID_ARGUMENT, "unknown failure policy");
     const bool instrument = o->instrumentation == BF_INSTRUMENT_STAGE_EVENTS;
     w->pending.clear();
     w->events_used = 0;
 #define STAGE(stage, dispatches, call) HIP_TRY(staged(w, rep, instrument, (stage), (dispatches), stream, [&] { return (call); }))
     STAGE(BF_STAGE_AUXILIARY, 4, [&] {
         hipError_t e = bfk_set_words((uint32_t*)w->traversal, 12, 0u, stream);
-        if (e == hipSuccess) e = bfk_set_words(w->failure, 1, 0u, stream);
-        if (e == hipSuccess) e = bfk_set_words(w->failure + 1, 1, BF_INVALID_INDEX, stream);
+        if (e == hipSuccess) e = bfk_set_words(w->failure, 2, 0u, stream);
+        if (e == hipSuccess) e = bfk_set_words(w->failure + 2, 2, 0xFFFFFFFFu, stream);
         if (e == hipSuccess) e = bfk_set_words(w->sample_counts, 2, 0u, stream);
         return e;
     }());
@@ -1100,20 +1101,24 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     }
 #undef STAGE
     rep.path_count = pixels * frame->samples_per_pixel;
-    HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 8, hipMemcpyDeviceToHost, stream));
-    HIP_TRY(hipMemcpyAsync(w->host_words + 2, w->sample_counts, 8, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 16, hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipMemcpyAsync(w->host_words + 4, w->sample_counts, 8, hipMemcpyDeviceToHost, stream));
     unsigned long long trav[6] = {0, 0, 0, 0, 0, 0};
     if (traversal_counters) HIP_TRY(hipMemcpyAsync(trav, w->traversal, sizeof trav, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     HIP_TRY(drain_stage_events(w, rep));
     rep.failed_paths = w->host_words[0];
-    rep.first_failure_slot = w->host_words[1];
-    rep.closure_samples = w->host_words[2];
-    rep.light_samples = w->host_words[3];
+    if (rep.failed_paths != 0) {
+        const uint64_t key = (uint64_t)w->host_words[2] | ((uint64_t)w->host_words[3] << 32);
+        rep.first_failure_slot = (uint32_t)(key >> 8);
+        rep.first_failure_code = (uint32_t)(key & 0xFFu);
+    }
+    rep.closure_samples = w->host_words[4];
+    rep.light_samples = w->host_words[5];
     if (traversal_counters)
         for (int k = 0; k < 6; ++k) traversal_counters[k] = trav[k];
     if (report) *report = rep;
-    if (rep.failed_paths != 0)
+    if (rep.failed_paths != 0 && o->failure_policy == BF_FAILURES_ABORT)
         return set_error(BF_TRANSPORT_FAILURE, "%llu paths failed; first failing slot %u",
                          (unsigned long long)rep.failed_paths, rep.first_failure_slot);
     return BF_OK;

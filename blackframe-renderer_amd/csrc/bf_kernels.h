@@ -27,7 +27,7 @@ typedef struct bf_transport_params {
     float rr_min;
     float rr_max;
     float packet[4];
-    uint32_t* failure;          /* [0] failed path count, [1] lowest failing slot */
+    uint32_t* failure;          /* [0] failed path count, [2..3] lowest (global slot << 8 | code) */
 } bf_transport_params;
 
 #ifdef __cplusplus

@@ -30,10 +30,11 @@ BF_D sample_key key_for_slot(const bf_transport_params& P, uint32_t slot) {
     return k;
 }
 
+/* Marks a path failed; the workspace keeps the failure count and the lowest (global slot, code). */
 BF_D void fail_path(const bf_transport_params& P, uint32_t slot, uint32_t code) {
     P.paths.status[slot] = BF_TERM_FAILED | (code << 8);
     atomicAdd(&P.failure[0], 1u);
-    atomicMin(&P.failure[1], slot);
+    atomicMin((unsigned long long*)(P.failure + 2), ((unsigned long long)(P.slot_base + slot) << 8) | code);
 }
 
 BF_D float spectrum_at(const bf_dev_scene& S, uint32_t id, float wavelength) {
@@ -1025,7 +1026,7 @@ __global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, u
     r.wavelengths[2] = W.z;
     r.wavelengths[3] = W.w;
     r.depth = D.y;
-    r.termination = paths.status[i];
+    r.termination = paths.status[i] & 0xFFu; /* the failure code stays in the workspace report */
     r.depth_counters = D.x;
     r.eta_scale = __uint_as_float(D.w);
     out[i] = r;

@@ -199,7 +199,10 @@ class Film:
 
 def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, want_paths=False, want_counters=False,
                                   stream=None, synchronize=True):
-    """Renders one frame; mirrors engine::trace_cuda_wavefront_transport (WavefrontTransport.hpp:197-208)."""
+    """Renders one frame; mirrors engine::trace_cuda_wavefront_transport (WavefrontTransport.hpp:197-208).
+
+    want_paths: False, True (host ctypes array of PathResult) or "device" (a torch uint8 tensor of
+    n_paths * 64 bytes left in device memory, for frames whose results are only sampled)."""
     import torch
 
     cam = frame.camera
@@ -219,7 +222,9 @@ def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, w
     result = {"film": film, "report": report}
     if want_counters:
         result["traversal_counters"] = counters
-    if want_paths:
+    if want_paths == "device":
+        result["paths"] = d_paths
+    elif want_paths:
         result["paths"] = (abi.PathResult * n_paths).from_buffer_copy(d_paths.cpu().numpy().tobytes())
     return result
 

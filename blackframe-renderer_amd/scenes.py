@@ -123,14 +123,12 @@ class Scene:
         m.emission_spectrum = self.spectrum(emission) if emission is not None else abi.INVALID
         m.frame_mode = frame_mode
         m.tangent_rotation = tangent_rotation
-        # host-projected sampler grid (TransportLobesDevice.cuh:76-82): normalised probabilities and CDF
-        total = sum(probabilities) if probabilities else 1.0
-        cumulative = 0.0
+        # host-projected sampler grid (TransportLobesDevice.cuh:76-82): probabilities are whole
+        # multiples of 2^-24, so every CDF step equals its probability exactly in binary32
+        for i, (p, c) in enumerate(zip(project_probabilities(probabilities), np.cumsum(project_probabilities(probabilities)))):
+            m.probabilities[i] = p
+            m.cdf[i + 1] = c
         m.cdf[0] = 0.0
-        for i, p in enumerate(probabilities):
-            m.probabilities[i] = np.float32(p / total)
-            cumulative += p / total
-            m.cdf[i + 1] = 1.0 if i + 1 == len(probabilities) else np.float32(cumulative)
         index = len(self.materials)
         self.materials.append(m)
         return index
@@ -193,6 +191,22 @@ class Scene:
         d.environment_spectrum = self.environment_spectrum
         self.desc = d
         return self
+
+
+def project_probabilities(weights):
+    """Closure selection probabilities on the 2^-24 grid: floor of each share, then the missing
+    units to the largest remainders (one each, lowest index first on ties), as the reference's
+    sampler-grid projection does (TransportLobesDevice.cuh:2643-2718)."""
+    weights = np.asarray(weights, dtype=np.float64)
+    if len(weights) == 0:
+        return np.zeros(0)
+    scale = 1 << 24
+    scaled = weights / weights.sum() * scale
+    masses = np.floor(scaled).astype(np.int64)
+    remainders = scaled - masses
+    for i in sorted(range(len(weights)), key=lambda k: (-remainders[k], k))[: scale - int(masses.sum())]:
+        masses[i] += 1
+    return masses.astype(np.float64) / scale
 
 
 # ---- camera ------------------------------------------------------------------------------------
@@ -289,21 +303,52 @@ def unit_box():
     return np.array(positions), np.array(normals), np.array(uvs, dtype=np.float64), np.array(indices)
 
 
-def lambert(reflectance):
-    return {"kind": abi.CLOSURE_LAMBERTIAN, "lobes": abi.LOBE_DIFFUSE | abi.LOBE_REFLECTION, "weight": reflectance}
+def lambert(reflectance, probability=1.0):
+    return {"kind": abi.CLOSURE_LAMBERTIAN, "lobes": abi.LOBE_DIFFUSE | abi.LOBE_REFLECTION, "weight": reflectance,
+            "probability": probability}
 
 
-def cornell(width=64, height=64):
-    """Cornell box (see module docstring)."""
+def rough_diffuse(reflectance, roughness, probability=1.0):
+    """Energy-preserving Oren–Nayar; parameters[0] = roughness in [0, 1]."""
+    return {"kind": abi.CLOSURE_ROUGH_DIFFUSE, "lobes": abi.LOBE_DIFFUSE | abi.LOBE_REFLECTION, "weight": reflectance,
+            "params": [roughness], "probability": probability}
+
+
+def rough_conductor(weight, eta, k, alpha_x, alpha_y=None, probability=1.0):
+    """GGX conductor; eta / k are spectra (scalars or samples over [360, 830] nm) resolved into
+    parameters[0..3] / [4..7] at the path's wavelengths; parameters[8..9] are the GGX widths."""
+    alpha_y = alpha_x if alpha_y is None else alpha_y
+    return {"kind": abi.CLOSURE_ROUGH_CONDUCTOR, "lobes": abi.LOBE_GLOSSY | abi.LOBE_REFLECTION, "weight": weight,
+            "eta": eta, "k": k, "params": [0.0] * 8 + [alpha_x, alpha_y], "probability": probability}
+
+
+def rough_dielectric(weight, exterior_eta, interior_eta, alpha_x, alpha_y=None, probability=1.0):
+    """GGX dielectric; parameters = [exterior eta, interior eta, alpha_x, alpha_y]."""
+    alpha_y = alpha_x if alpha_y is None else alpha_y
+    return {"kind": abi.CLOSURE_ROUGH_DIELECTRIC,
+            "lobes": abi.LOBE_GLOSSY | abi.LOBE_REFLECTION | abi.LOBE_TRANSMISSION, "weight": weight,
+            "params": [exterior_eta, interior_eta, alpha_x, alpha_y], "probability": probability}
+
+
+def cornell(width=64, height=64, materials="lambert"):
+    """Cornell box (see module docstring). materials='mixed' makes the tall box a gold GGX
+    conductor (alpha 0.3), the short box a GGX dielectric (eta 1.5, alpha 0.2), the left wall an
+    energy-preserving Oren–Nayar surface (roughness 0.5) and the back wall a Lambert + GGX
+    conductor mixture."""
     s = Scene()
     white = s.add_material(21, [lambert(0.73)])
     left = s.add_material(22, [lambert(0.12)])
     right = s.add_material(23, [lambert(0.45)])
     emitter = s.add_material(24, [lambert(0.0)], emission=17.0)
+    back = white
+    if materials == "mixed":
+        left = s.add_material(27, [rough_diffuse(0.12, 0.5)])
+        back = s.add_material(28, [lambert(0.5, probability=0.7),
+                                   rough_conductor(0.8, COPPER_ETA, COPPER_K, 0.15, 0.4, probability=0.3)])
     walls = [
         (11, quad([[-1, -1, 0], [-1, -1, 2], [1, -1, 2], [1, -1, 0]], [0, 1, 0]), white),
         (12, quad([[-1, 1, 0], [1, 1, 0], [1, 1, 2], [-1, 1, 2]], [0, -1, 0]), white),
-        (13, quad([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], [0, 0, 1]), white),
+        (13, quad([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], [0, 0, 1]), back),
         (14, quad([[-1, -1, 0], [-1, 1, 0], [-1, 1, 2], [-1, -1, 2]], [1, 0, 0]), left),
         (15, quad([[1, -1, 0], [1, -1, 2], [1, 1, 2], [1, 1, 0]], [-1, 0, 0]), right),
         (16, quad([[-0.32, 0.985, 0.62], [0.32, 0.985, 0.62], [0.32, 0.985, 1.18], [-0.32, 0.985, 1.18]],
@@ -315,8 +360,12 @@ def cornell(width=64, height=64):
     box = s.add_geometry(17, *unit_box())
     short = affine(rotation_y(math.radians(15.0)) @ np.diag([0.55, 0.55, 0.55]), (0.36, -0.725, 1.30))
     tall = affine(rotation_y(math.radians(-18.0)) @ np.diag([0.55, 1.15, 0.55]), (-0.38, -0.425, 0.72))
-    s.add_instance(37, 7, box, white, short)
-    s.add_instance(38, 8, box, white, tall)
+    short_material = tall_material = white
+    if materials == "mixed":
+        short_material = s.add_material(25, [rough_dielectric(1.0, 1.0, 1.5, 0.2)])
+        tall_material = s.add_material(26, [rough_conductor(1.0, GOLD_ETA, GOLD_K, 0.3)])
+    s.add_instance(37, 7, box, short_material, short)
+    s.add_instance(38, 8, box, tall_material, tall)
     s.finalize()
     camera = look_at_camera((0.0, 0.0, 4.0), (0.0, -0.05, 0.92), width, height, 0.70)
     return s, camera
@@ -389,13 +438,10 @@ def icosphere_grid(width=1920, height=1080, grid=32, subdivisions=3, seed=0x5EED
                 closures = [lambert(albedo)]
             elif kind == 1:
                 metal = rng.uniform() < 0.5
-                closures = [{"kind": abi.CLOSURE_ROUGH_CONDUCTOR, "lobes": abi.LOBE_GLOSSY | abi.LOBE_REFLECTION,
-                             "weight": 1.0, "eta": GOLD_ETA if metal else COPPER_ETA,
-                             "k": GOLD_K if metal else COPPER_K,
-                             "params": [0, 0, 0, 0, 0, 0, 0, 0, roughness * roughness, roughness * roughness]}]
+                closures = [rough_conductor(1.0, GOLD_ETA if metal else COPPER_ETA, GOLD_K if metal else COPPER_K,
+                                            roughness * roughness)]
             else:
-                closures = [{"kind": abi.CLOSURE_ROUGH_DIELECTRIC, "lobes": abi.LOBE_GLOSSY | abi.LOBE_REFLECTION,
-                             "weight": 1.0, "params": [1.5, 1.0, roughness * roughness, roughness * roughness]}]
+                closures = [rough_dielectric(1.0, 1.0, 1.5, roughness * roughness)]
             m = s.add_material(2000 + obj, closures)
             x = (gx - (grid - 1) / 2.0) * spacing
             z = (gy - (grid - 1) / 2.0) * spacing

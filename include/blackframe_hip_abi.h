@@ -293,8 +293,17 @@ typedef struct bf_transport_options {
     float packet_wavelengths[BF_SPECTRUM_LANES]; /* used in frame-packet mode */
     uint32_t record_queues; /* nonzero: fill bf_transport_report queue fingerprints */
     uint32_t instrumentation; /* bf_instrumentation_mode: BF_INSTRUMENT_STAGE_EVENTS fills stage_metrics */
-    uint32_t reserved[2];
+    uint32_t failure_policy;  /* bf_failure_policy */
+    uint32_t reserved;
 } bf_transport_options;
+
+/* What a numerically failed path (termination BF_TERM_FAILED) does to the call. The reference
+ * aborts the batch on the first failing work item (WavefrontTransport.cpp:880-925), which is
+ * BF_FAILURES_ABORT: the frame still completes, then the call returns BF_TRANSPORT_FAILURE.
+ * BF_FAILURES_COUNT returns success and leaves the count and first failing slot in the report;
+ * a failed path keeps the radiance it gathered before failing, on both the device and the CPU
+ * restatement. */
+enum bf_failure_policy { BF_FAILURES_ABORT = 0, BF_FAILURES_COUNT = 1 };
 
 /* One frame of work: spp passes over width*height pixels; path slot p = s*W*H + y*W + x. */
 typedef struct bf_frame_desc {
@@ -377,6 +386,9 @@ BF_STATIC_ASSERT(sizeof(bf_bvh_blas) == 32, "SceneBvhBlas is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_instance_ref) == 32, "SceneBvhInstanceReference is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_wide_node) == 64, "wide node is one 64-byte line");
 BF_STATIC_ASSERT(sizeof(bf_closure_record) == 64, "ClosureRecord is 64 bytes");
+BF_STATIC_ASSERT(sizeof(bf_transport_options) == 80, "transport options are 80 bytes");
+BF_STATIC_ASSERT(sizeof(bf_stage_metric) == 24, "stage metric is 24 bytes");
+BF_STATIC_ASSERT(sizeof(bf_transport_report) == 1808, "transport report is 1808 bytes");
 BF_STATIC_ASSERT(offsetof(bf_closest_hit, ids) == 32, "ClosestHit identifiers at 32");
 BF_STATIC_ASSERT(offsetof(bf_bvh_node, first_child) == 32, "SceneBvhNode first_child at 32");
 BF_STATIC_ASSERT(offsetof(bf_closure_record, parameters) == 24, "ClosureRecord parameters at 24");

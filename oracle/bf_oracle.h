@@ -73,6 +73,28 @@ int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame,
                uint32_t threads, bf_path_result* paths, float* film_float, double* film_compensated,
                uint32_t* film_counts, bf_transport_report* report, uint64_t* traversal_counters);
 
+/* ---- closure library (radiance transport) ------------------------------------------------
+ * allowed: direction mask (BF_LOBE_REFLECTION / TRANSMISSION bits) after depth filtering.
+ * Return values are the lobe statuses: 0 ok, 1 invalid, 2 not representable, 3 outside support. */
+int bfo_closure_eval(const bf_closure_record* closure, uint32_t allowed, const float wo[3], const float wi[3],
+                     float value[4]);
+int bfo_closure_pdf(const bf_closure_record* closure, uint32_t allowed, const float wo[3], const float wi[3],
+                    float* pdf);
+int bfo_closure_sample(const bf_closure_record* closure, uint32_t allowed, const float wo[3], float event, float u,
+                       float v, float out[9], uint32_t* lobes);
+int bfo_dielectric_fresnel(float cos_i, float eta_i, float eta_t, float* out);
+int bfo_conductor_fresnel(float cos_i, float eta, float k, float* out);
+int bfo_ggx_d(float ax, float ay, const float m[3], float* out);
+int bfo_smith_g1(float ax, float ay, const float d[3], float* out);
+int bfo_smith_g2(float ax, float ay, const float wo[3], const float wi[3], float* out);
+int bfo_smith_lambda(float ax, float ay, const float d[3], float* out);
+int bfo_vndf_pdf(float ax, float ay, const float wo[3], const float m[3], float* out);
+int bfo_sample_vndf(float ax, float ay, const float wo[3], float u, float v, float m[3], float* pdf);
+int bfo_ggx_widths_representable(float ax, float ay);
+float bfo_hypotf(float x, float y);
+float bfo_atan2f(float y, float x);
+void bfo_sincosf(float angle, float* sine, float* cosine);
+
 #ifdef __cplusplus
 }
 #endif

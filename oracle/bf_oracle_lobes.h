@@ -248,27 +248,6 @@ static inline int lobe_valid_surface_event(uint32_t bits) {
            direction != 0u && (direction & (direction - 1u)) == 0u;
 }
 
-/* ---- closure record dispatch (TransportLobesDevice.cuh:2102-2260) --------------------------- */
-static inline int lobe_record_eval(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* out) {
-    (void)allowed;
-    switch (c->kind) {
-    case BF_CLOSURE_LAMBERTIAN:
-        return lobe_lambert_eval(c->weight, wo, wi, out);
-    default:
-        return LOBE_INVALID;
-    }
-}
-
-static inline int lobe_record_pdf(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* pdf) {
-    (void)allowed;
-    switch (c->kind) {
-    case BF_CLOSURE_LAMBERTIAN:
-        return lobe_lambert_pdf(wo, wi, pdf);
-    default:
-        return LOBE_INVALID;
-    }
-}
-
 static inline void lobe_empty_sample(lobe_sample* s, uint32_t status) {
     memset(s, 0, sizeof *s);
     s->measure = LOBE_MEASURE_SOLID_ANGLE;
@@ -276,41 +255,89 @@ static inline void lobe_empty_sample(lobe_sample* s, uint32_t status) {
     s->status = status;
 }
 
-static inline void lobe_record_sample(const closure_rec* c, uint32_t allowed, v3 wo, float event, float u, float v,
-                                      lobe_sample* out) {
-    (void)allowed;
-    (void)event;
-    lobe_empty_sample(out, LOBE_INVALID);
+#include "bf_oracle_microfacet.h"
+
+/* ---- closure record dispatch (TransportLobesDevice.cuh:2092-2260); transport mode radiance --- */
+static inline int lobe_record_eval(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* out) {
+    if (!mf_valid_closure(c)) return LOBE_INVALID;
+    const float* p = c->parameters;
     switch (c->kind) {
-    case BF_CLOSURE_LAMBERTIAN: {
-        if (!lobe_reflectance(c->weight) || !lobe_unit_vector(wo)) return;
-        v3 wi;
-        const int st = lobe_cosine_hemisphere(u, v, &wi);
-        if (st != LOBE_OK) {
-            out->status = st;
-            return;
-        }
-        if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
-            out->incoming = wi;
-            out->status = LOBE_OUTSIDE_SUPPORT;
-            return;
-        }
-        float value[4], pdf;
-        int s1 = lobe_lambert_eval(c->weight, wo, wi, value);
-        int s2 = lobe_lambert_pdf(wo, wi, &pdf);
-        if (s1 != LOBE_OK || s2 != LOBE_OK) {
-            out->status = s1 != LOBE_OK ? s1 : s2;
-            return;
-        }
-        memcpy(out->value, value, sizeof value);
-        out->incoming = wi;
-        out->pdf = pdf;
-        out->measure = LOBE_MEASURE_SOLID_ANGLE;
-        out->lobes = c->lobes;
-        out->eta_scale = 1.0f;
-        out->status = LOBE_OK;
+    case BF_CLOSURE_LAMBERTIAN:
+        return lobe_lambert_eval(c->weight, wo, wi, out);
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        return mf_rough_diffuse_eval(c->weight, p[0], wo, wi, out);
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        return mf_conductor_eval(c->weight, p, p + 4, p[8], p[9], wo, wi, out);
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        return mf_dielectric_eval_masked(c->weight, p[0], p[1], p[2], p[3], wo, wi, BFO_TRANSPORT_RADIANCE, allowed, out);
+    default:
+        return LOBE_INVALID;
+    }
+}
+
+static inline int lobe_record_pdf(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* pdf) {
+    if (!mf_valid_closure(c)) return LOBE_INVALID;
+    const float* p = c->parameters;
+    switch (c->kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        return lobe_lambert_pdf(wo, wi, pdf);
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        return mf_conductor_pdf(p[8], p[9], wo, wi, pdf);
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        return mf_dielectric_pdf_masked(c->weight, p[0], p[1], p[2], p[3], wo, wi, BFO_TRANSPORT_RADIANCE, allowed, pdf);
+    default:
+        return LOBE_INVALID;
+    }
+}
+
+/* sample_lambert / sample_rough_diffuse: cosine hemisphere, then eval and pdf at the sample */
+static inline void lobe_diffuse_sample(const closure_rec* c, v3 wo, float u, float v, lobe_sample* out) {
+    v3 wi;
+    const int st = lobe_cosine_hemisphere(u, v, &wi);
+    if (st != LOBE_OK) {
+        out->status = st;
         return;
     }
+    if (!(wo.z > 0.0f) || !(wi.z > 0.0f)) {
+        out->incoming = wi;
+        out->status = LOBE_OUTSIDE_SUPPORT;
+        return;
+    }
+    float value[4], pdf;
+    const int s1 = c->kind == BF_CLOSURE_LAMBERTIAN ? lobe_lambert_eval(c->weight, wo, wi, value)
+                                                    : mf_rough_diffuse_eval(c->weight, c->parameters[0], wo, wi, value);
+    const int s2 = lobe_lambert_pdf(wo, wi, &pdf);
+    if (s1 != LOBE_OK || s2 != LOBE_OK) {
+        out->status = s1 != LOBE_OK ? s1 : s2;
+        return;
+    }
+    memcpy(out->value, value, sizeof value);
+    out->incoming = wi;
+    out->pdf = pdf;
+    out->measure = LOBE_MEASURE_SOLID_ANGLE;
+    out->lobes = BF_LOBE_DIFFUSE | BF_LOBE_REFLECTION;
+    out->eta_scale = 1.0f;
+    out->status = LOBE_OK;
+}
+
+static inline void lobe_record_sample(const closure_rec* c, uint32_t allowed, v3 wo, float event, float u, float v,
+                                      lobe_sample* out) {
+    lobe_empty_sample(out, LOBE_INVALID);
+    if (!mf_valid_closure(c)) return;
+    const float* p = c->parameters;
+    switch (c->kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        if (!lobe_unit_vector(wo)) return;
+        lobe_diffuse_sample(c, wo, u, v, out);
+        return;
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        mf_conductor_sample(c->weight, p, p + 4, p[8], p[9], wo, u, v, out);
+        return;
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        mf_dielectric_sample_masked(c->weight, p[0], p[1], p[2], p[3], wo, event, u, v, BFO_TRANSPORT_RADIANCE, allowed, out);
+        return;
     default:
         return;
     }

@@ -57,6 +57,24 @@ def load(abi):
     lib.bfo_render.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
                                C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.POINTER(abi.TransportReport), C.c_void_p]
+    f3 = C.POINTER(C.c_float)
+    rec = C.POINTER(abi.ClosureRecord)
+    lib.bfo_closure_eval.argtypes = [rec, C.c_uint32, f3, f3, f3]
+    lib.bfo_closure_pdf.argtypes = [rec, C.c_uint32, f3, f3, f3]
+    lib.bfo_closure_sample.argtypes = [rec, C.c_uint32, f3, C.c_float, C.c_float, C.c_float, f3, C.POINTER(C.c_uint32)]
+    lib.bfo_dielectric_fresnel.argtypes = [C.c_float, C.c_float, C.c_float, f3]
+    lib.bfo_conductor_fresnel.argtypes = [C.c_float, C.c_float, C.c_float, f3]
+    for name in ("bfo_ggx_d", "bfo_smith_g1", "bfo_smith_lambda"):
+        getattr(lib, name).argtypes = [C.c_float, C.c_float, f3, f3]
+    lib.bfo_smith_g2.argtypes = [C.c_float, C.c_float, f3, f3, f3]
+    lib.bfo_vndf_pdf.argtypes = [C.c_float, C.c_float, f3, f3, f3]
+    lib.bfo_sample_vndf.argtypes = [C.c_float, C.c_float, f3, C.c_float, C.c_float, f3, f3]
+    lib.bfo_ggx_widths_representable.argtypes = [C.c_float, C.c_float]
+    lib.bfo_hypotf.restype = C.c_float
+    lib.bfo_hypotf.argtypes = [C.c_float, C.c_float]
+    lib.bfo_atan2f.restype = C.c_float
+    lib.bfo_atan2f.argtypes = [C.c_float, C.c_float]
+    lib.bfo_sincosf.argtypes = [C.c_float, f3, f3]
     return lib
 
 

@@ -9,8 +9,8 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def run_both(bf, oracle, hip, width, height, spp, **opts):
-    scene, cam = bf.scenes.cornell(width, height)
+def run_both(bf, oracle, hip, width, height, spp, scene_factory=None, **opts):
+    scene, cam = (scene_factory or bf.scenes.cornell)(width, height)
     frame = bf.scenes.frame_desc(cam, spp, seed=opts.pop("seed", 0))
     options = bf.scenes.transport_options(**opts)
     options.record_queues = 1
@@ -30,7 +30,24 @@ def path_arrays(paths):
     dict(mis=1, max_depth=4, rr_first_depth=2),
 ])
 def test_cornell_transport_bit_exact(bf, oracle, hip, opts):
-    expected, got = run_both(bf, oracle, hip, 32, 32, 4, **dict(opts))
+    check_parity(*run_both(bf, oracle, hip, 32, 32, 4, **dict(opts)))
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(mis=1, max_depth=12)])
+def test_cornell_microfacet_transport_bit_exact(bf, oracle, hip, opts):
+    """Gold GGX conductor, GGX dielectric, Oren–Nayar wall and a Lambert + anisotropic GGX mixture."""
+    factory = lambda w, h: bf.scenes.cornell(w, h, materials="mixed")
+    check_parity(*run_both(bf, oracle, hip, 32, 32, 4, scene_factory=factory, **dict(opts)))
+
+
+def test_icosphere_grid_transport_bit_exact(bf, oracle, hip):
+    """A 4x4 grid of the plain-forward-at-scale generator: mixed Lambert / conductor / dielectric,
+    64 emissive triangles, environment, depth 16, power MIS."""
+    factory = lambda w, h: bf.scenes.icosphere_grid(w, h, grid=4, subdivisions=2)
+    check_parity(*run_both(bf, oracle, hip, 48, 27, 2, scene_factory=factory, mis=1, max_depth=16))
+
+
+def check_parity(expected, got):
     er, gr = expected["report"], got["report"]
     assert er.failed_paths == 0 and gr.failed_paths == 0
     assert gr.bounces == er.bounces
