@@ -188,3 +188,56 @@ for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitR
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 136), (CameraDesc, 96),
                        (TransportOptions, 80), (FrameDesc, 112), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
+
+
+# ---- XPU extension (plugin) ABI: ExtensionAbi.h / XpuBackendAbi.h ------------------------------
+XPU_BACKEND_INTERFACE_ID = (0x96C6E70DD06B4B93, 0xA9514AB777F49D37)
+EXT_STATUS = {0: "success", 1: "invalid_argument", 2: "unsupported_abi", 3: "interface_not_found",
+              4: "invalid_state", 5: "out_of_range", 6: "extension_failure"}
+
+
+class InterfaceId(C.Structure):
+    _fields_ = [("high", C.c_uint64), ("low", C.c_uint64)]
+
+
+class Utf8View(C.Structure):
+    _fields_ = [("data", C.c_char_p), ("size", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def text(self):
+        return C.string_at(self.data, self.size).decode() if self.size else ""
+
+
+class ExtensionQuery(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("abi_major", C.c_uint32), ("abi_minor", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+GetInterfaceFn = C.CFUNCTYPE(C.c_uint32, C.c_void_p, InterfaceId, C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32)
+ShutdownFn = C.CFUNCTYPE(None, C.c_void_p)
+
+
+class ExtensionApi(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("abi_major", C.c_uint32), ("abi_minor", C.c_uint32),
+                ("reserved", C.c_uint32), ("extension_context", C.c_void_p), ("get_interface", GetInterfaceFn),
+                ("shutdown", ShutdownFn)]
+
+
+class DeviceDescriptor(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("kind", C.c_uint32), ("identifier_high", C.c_uint64),
+                ("identifier_low", C.c_uint64), ("name", Utf8View), ("vendor", Utf8View),
+                ("reserved", C.c_uint64 * 4)]
+
+
+GetDeviceCountFn = C.CFUNCTYPE(C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32))
+GetDeviceDescriptorFn = C.CFUNCTYPE(C.c_uint32, C.c_void_p, C.c_uint32, C.POINTER(DeviceDescriptor))
+
+
+class XpuBackendApi(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("abi_major", C.c_uint32), ("abi_minor", C.c_uint32),
+                ("reserved", C.c_uint32), ("backend_context", C.c_void_p), ("backend_name", Utf8View),
+                ("get_device_count", GetDeviceCountFn), ("get_device_descriptor", GetDeviceDescriptorFn),
+                ("reserved_slots", C.c_uint64 * 4)]
+
+
+for _record, _size in ((ExtensionQuery, 16), (ExtensionApi, 40), (DeviceDescriptor, 88), (XpuBackendApi, 88)):
+    assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))

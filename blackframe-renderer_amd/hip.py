@@ -55,6 +55,8 @@ def library():
     lib.blackframe_hip_launch_sample_stream_dump.argtypes = [u32, vp, vp, u32, vp, vp]
     lib.blackframe_hip_workspace_create.argtypes = [u64, C.POINTER(vp)]
     lib.blackframe_hip_workspace_destroy.argtypes = [vp]
+    lib.blackframe_query_extension.restype = C.c_uint32
+    lib.blackframe_query_extension.argtypes = [C.POINTER(abi.ExtensionQuery), C.POINTER(abi.ExtensionApi)]
     lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
                                                              C.POINTER(abi.TransportOptions), vp, vp, vp, vp,
                                                              C.POINTER(abi.TransportReport), vp, vp]
@@ -242,3 +244,30 @@ def sample_stream_dump(sampler, indices, dimensions, stream=None):
                                                               C.c_void_p(d_out.data_ptr()), _stream_handle(stream)))
     torch.cuda.synchronize()
     return d_out.cpu().numpy()
+
+
+def xpu_devices():
+    """Enumerates devices through the plugin ABI (blackframe_query_extension -> XPU backend v1),
+    as the reference's extension loader does. Returns (backend name, [(kind, name, vendor, id)])."""
+    lib = library()
+    query = abi.ExtensionQuery(C.sizeof(abi.ExtensionQuery), 1, 0, 0)
+    ext = abi.ExtensionApi()
+    ext.struct_size = C.sizeof(abi.ExtensionApi)
+    status = lib.blackframe_query_extension(C.byref(query), C.byref(ext))
+    if status:
+        raise XpuError(status, abi.EXT_STATUS.get(status, "extension query failed"))
+    backend = abi.XpuBackendApi()
+    status = ext.get_interface(ext.extension_context, abi.InterfaceId(*abi.XPU_BACKEND_INTERFACE_ID), 1, 0,
+                               C.addressof(backend), C.sizeof(backend))
+    if status:
+        raise XpuError(status, abi.EXT_STATUS.get(status, "interface query failed"))
+    count = C.c_uint32()
+    backend.get_device_count(backend.backend_context, C.byref(count))
+    devices = []
+    for d in range(count.value):
+        desc = abi.DeviceDescriptor()
+        desc.struct_size = C.sizeof(desc)
+        if backend.get_device_descriptor(backend.backend_context, d, C.byref(desc)) == 0:
+            devices.append((desc.kind, desc.name.text(), desc.vendor.text(),
+                            (desc.identifier_high, desc.identifier_low)))
+    return backend.backend_name.text(), devices

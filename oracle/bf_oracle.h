@@ -91,6 +91,12 @@ int bfo_smith_lambda(float ax, float ay, const float d[3], float* out);
 int bfo_vndf_pdf(float ax, float ay, const float wo[3], const float m[3], float* out);
 int bfo_sample_vndf(float ax, float ay, const float wo[3], float u, float v, float m[3], float* pdf);
 int bfo_ggx_widths_representable(float ax, float ay);
+/* MIS weight of `sampled` against `competing` (same measure); heuristic 0 balance, 1 power. */
+int bfo_mis_weight(uint32_t heuristic, float sampled, float competing, float* weight);
+/* Russian roulette on a 4-lane throughput (updated in place when the path survives). */
+int bfo_russian_roulette(float throughput[4], float eta_scale, float u, float minimum, float maximum, int* survived);
+/* Four circularly stratified wavelengths in [360, 830) nm from one sample. */
+void bfo_sample_wavelengths(float u, float wavelengths[4]);
 float bfo_hypotf(float x, float y);
 float bfo_atan2f(float y, float x);
 void bfo_sincosf(float angle, float* sine, float* cosine);

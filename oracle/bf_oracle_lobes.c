@@ -62,3 +62,11 @@ int bfo_ggx_widths_representable(float ax, float ay) { return mf_ggx_widths(ax, 
 float bfo_hypotf(float x, float y) { return bf_hypotf(x, y); }
 float bfo_atan2f(float y, float x) { return bf_atan2f(y, x); }
 void bfo_sincosf(float a, float* s, float* c) { bf_sincosf(a, s, c); }
+
+int bfo_mis_weight(uint32_t heuristic, float sampled, float competing, float* weight) {
+    return lobe_mis_weight(heuristic, sampled, competing, weight);
+}
+
+int bfo_russian_roulette(float throughput[4], float eta_scale, float u, float minimum, float maximum, int* survived) {
+    return lobe_russian_roulette(throughput, eta_scale, u, minimum, maximum, survived);
+}

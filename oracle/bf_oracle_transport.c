@@ -145,6 +145,8 @@ static void sample_wavelengths(float u, float* wl) {
     }
 }
 
+void bfo_sample_wavelengths(float u, float wavelengths[4]) { sample_wavelengths(u, wavelengths); }
+
 static void camera_lane(const ctx* c, lane* l, uint64_t global_path) {
     const bf_frame_desc* f = c->frame;
     const uint64_t pixels = (uint64_t)f->camera.width * f->camera.height;

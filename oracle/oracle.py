@@ -70,6 +70,9 @@ def load(abi):
     lib.bfo_vndf_pdf.argtypes = [C.c_float, C.c_float, f3, f3, f3]
     lib.bfo_sample_vndf.argtypes = [C.c_float, C.c_float, f3, C.c_float, C.c_float, f3, f3]
     lib.bfo_ggx_widths_representable.argtypes = [C.c_float, C.c_float]
+    lib.bfo_mis_weight.argtypes = [C.c_uint32, C.c_float, C.c_float, f3]
+    lib.bfo_russian_roulette.argtypes = [f3, C.c_float, C.c_float, C.c_float, C.c_float, C.POINTER(C.c_int)]
+    lib.bfo_sample_wavelengths.argtypes = [C.c_float, f3]
     lib.bfo_hypotf.restype = C.c_float
     lib.bfo_hypotf.argtypes = [C.c_float, C.c_float]
     lib.bfo_atan2f.restype = C.c_float

@@ -16,14 +16,15 @@ WIDE_LEAF = 0x80000000
 def declared_entry_points():
     names = []
     for header in sorted((ROOT / "include").glob("*.h")):
-        names += re.findall(r"^\w[\w\s\*]*?\b(blackframe_hip_\w+)\s*\(", header.read_text(), flags=re.M)
+        names += re.findall(r"^\w[\w\s\*]*?\b(blackframe_hip_\w+|blackframe_query_extension)\s*\(", header.read_text(),
+                            flags=re.M)
     return sorted(set(names))
 
 
 def test_library_exports_every_declared_entry_point(hip):
     lib = hip.library()
     names = declared_entry_points()
-    assert len(names) >= 13
+    assert len(names) >= 14 and "blackframe_query_extension" in names
     raw = C.CDLL(str(hip.LIB_PATH))
     missing = [n for n in names if not hasattr(raw, n)]
     assert not missing, missing
@@ -50,6 +51,9 @@ def test_product_bvh_is_the_oracle_bvh(bf, oracle, hip, index):
     got = hip.XpuScene(scene, upload=False).bvh()
     for key in expected:
         assert got[key] == expected[key], (name, key)
+    # closed-form node counts of a leaf-size-1 binary BVH (SceneBvhAbi.hpp:181-193)
+    blases, blas_nodes, prims, tlas_nodes, instance_refs = got["counts"]
+    assert blas_nodes == 2 * prims - blases and tlas_nodes == 2 * instance_refs - 1
 
 
 def nodes_array(raw, count):
@@ -118,3 +122,27 @@ def test_material_probabilities_on_the_sampler_grid(bf):
     assert probs.sum() == 1.0 and np.all(probs * (1 << 24) == np.round(probs * (1 << 24)))
     probs = bf.scenes.project_probabilities([1.0, 1.0, 1.0])
     assert list(probs * (1 << 24)) == [5592406.0, 5592405.0, 5592405.0]
+
+
+def test_extension_negotiation(abi, hip):
+    """ExtensionAbi.h / XpuBackendAbi.h negotiation rules: struct sizes, ABI major, interface ids."""
+    lib = hip.library()
+    ext = abi.ExtensionApi()
+    ext.struct_size = C.sizeof(abi.ExtensionApi)
+    assert lib.blackframe_query_extension(C.byref(abi.ExtensionQuery(8, 1, 0, 0)), C.byref(ext)) == 1
+    assert lib.blackframe_query_extension(C.byref(abi.ExtensionQuery(16, 2, 0, 0)), C.byref(ext)) == 2
+    assert lib.blackframe_query_extension(C.byref(abi.ExtensionQuery(16, 1, 0, 0)), C.byref(ext)) == 0
+    assert (ext.abi_major, ext.struct_size) == (1, 40)
+    backend = abi.XpuBackendApi()
+    bad = abi.InterfaceId(1, 2)
+    good = abi.InterfaceId(*abi.XPU_BACKEND_INTERFACE_ID)
+    assert ext.get_interface(None, bad, 1, 0, C.addressof(backend), C.sizeof(backend)) == 3
+    assert ext.get_interface(None, good, 2, 0, C.addressof(backend), C.sizeof(backend)) == 2
+    assert ext.get_interface(None, good, 1, 0, C.addressof(backend), 8) == 1
+    assert ext.get_interface(None, good, 1, 0, C.addressof(backend), C.sizeof(backend)) == 0
+    assert backend.backend_name.text().startswith("Blackframe HIP")
+    desc = abi.DeviceDescriptor()
+    desc.struct_size = C.sizeof(desc)
+    assert backend.get_device_descriptor(None, 1 << 20, C.byref(desc)) == 5
+    desc.struct_size = 4
+    assert backend.get_device_descriptor(None, 0, C.byref(desc)) == 1

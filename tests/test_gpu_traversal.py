@@ -36,3 +36,10 @@ def test_occlusion_matches_oracle(bf, abi, oracle, hip):
     assert result_bytes(got) == result_bytes(expected)
     statuses = np.array([r.status for r in expected])
     assert 0 < np.count_nonzero(statuses == 1) < len(statuses)
+
+
+def test_extension_enumerates_the_gpu(hip):
+    name, devices = hip.xpu_devices()
+    assert name.startswith("Blackframe HIP") and len(devices) >= 1
+    kind, device_name, vendor, identifier = devices[0]
+    assert kind == 2 and vendor == "AMD" and "gfx950" in device_name and identifier != (0, 0)
