@@ -15,7 +15,14 @@
 
 #include "../../include/blackframe_hip_abi.h"
 
-#define BF_STACK_DEPTH 32u
+#define BF_STACK_DEPTH 32u   /* traversal stack entries per lane (TLAS + BLAS) */
+#ifndef BF_SHORT_STACK
+#define BF_SHORT_STACK 8u    /* of which the transport kernels keep this many in LDS */
+#endif
+#define BF_TRACE_BLOCK 128u  /* lanes per traversal workgroup */
+#ifndef BF_TRACE_WAVES_PER_SIMD
+#define BF_TRACE_WAVES_PER_SIMD 6 /* register budget of the traversal kernels (waves per SIMD) */
+#endif
 
 typedef struct bf_dev_blas {
     uint32_t root_ref;         /* wide-node index, or BF_WIDE_LEAF | leaf-triangle index */

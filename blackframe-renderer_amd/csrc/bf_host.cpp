@@ -749,6 +749,9 @@ struct bf_hip_workspace {
     unsigned long long* traversal = nullptr; /* 6 counters */
     unsigned long long* hashes = nullptr;    /* 2 words */
     uint32_t* host_words = nullptr;          /* pinned readback */
+    uint2* spill = nullptr;                  /* traversal stack spill, BF_STACK_DEPTH x spill_lanes */
+    uint32_t spill_lanes = 0;
+    int device = 0;
     /* stage instrumentation: a reusable pool of hipEvent pairs, drained at every host sync */
     std::vector<hipEvent_t> events;
     std::vector<std::pair<uint32_t, uint32_t>> pending; /* (stage, first event of the pair) */
@@ -903,6 +906,13 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     w->capacity = capacity;
     const uint64_t n = capacity;
     const uint32_t blocks = bfk_compact_blocks((uint32_t)n);
+    hipError_t de = hipGetDevice(&w->device);
+    const uint32_t resident = de == hipSuccess ? bfk_trace_resident_blocks(w->device) : 0u;
+    if (resident == 0) {
+        delete w;
+        return set_error(BF_DEVICE_ERROR, "cannot query the traversal kernels' occupancy on the current device");
+    }
+    w->spill_lanes = resident * BF_TRACE_BLOCK;
     Arena A;
     const size_t o_ro = A.add(nullptr, n * 16), o_rd = A.add(nullptr, n * 16), o_beta = A.add(nullptr, n * 16),
                  o_rad = A.add(nullptr, n * 16), o_wl = A.add(nullptr, n * 16), o_dep = A.add(nullptr, n * 16),
@@ -913,7 +923,8 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     for (int k = 0; k < 4; ++k) o_q[k] = A.add(nullptr, n * 4);
     const size_t o_route = A.add(nullptr, n), o_bc = A.add(nullptr, (size_t)blocks * 8), o_bo = A.add(nullptr, (size_t)blocks * 8),
                  o_sizes = A.add(nullptr, 64), o_fail = A.add(nullptr, 16), o_samp = A.add(nullptr, 16),
-                 o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, 16);
+                 o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, 16),
+                 o_spill = A.add(nullptr, (size_t)w->spill_lanes * BF_STACK_DEPTH * sizeof(uint2));
     hipError_t e = hipMalloc(&w->block, A.total);
     if (e != hipSuccess) {
         delete w;
@@ -949,6 +960,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     w->sample_counts = (uint32_t*)(b + o_samp);
     w->traversal = (unsigned long long*)(b + o_trav);
     w->hashes = (unsigned long long*)(b + o_hash);
+    w->spill = (uint2*)(b + o_spill);
     *out = w;
     return BF_OK;
 }
@@ -991,6 +1003,8 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     int device = 0;
     HIP_TRY(hipGetDevice(&device));
     if (device != s->device) return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", s->device, device);
+    if (device != w->device)
+        return set_error(BF_INCOMPATIBLE, "workspace lives on device %d, current device is %d", w->device, device);
 
     bf_transport_params P{};
     P.scene = s->dev;
@@ -1018,6 +1032,8 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     P.rr_max = o->russian_roulette_maximum;
     for (int k = 0; k < 4; ++k) P.packet[k] = o->packet_wavelengths[k];
     P.failure = w->failure;
+    P.spill = w->spill;
+    P.spill_stride = w->spill_lanes;
 
     bf_transport_report rep{};
     rep.first_failure_slot = BF_INVALID_INDEX;

@@ -28,6 +28,8 @@ typedef struct bf_transport_params {
     float rr_max;
     float packet[4];
     uint32_t* failure;          /* [0] failed path count, [2..3] lowest (global slot << 8 | code) */
+    uint2* spill;               /* traversal stack spill: BF_STACK_DEPTH entries per persistent lane */
+    uint32_t spill_stride;      /* persistent lanes = resident traversal workgroups * BF_TRACE_BLOCK */
 } bf_transport_params;
 
 #ifdef __cplusplus
@@ -41,6 +43,8 @@ hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const 
 hipError_t bfk_trace_shadow(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                             uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s);
 uint32_t bfk_compact_blocks(uint32_t max_count);
+/* Traversal workgroups the device holds at once (all traversal kernels), 0 on error. */
+uint32_t bfk_trace_resident_blocks(int device);
 hipError_t bfk_compact(const uint8_t* route, const uint32_t* count_ptr, uint32_t max_count, const uint32_t* src,
                        uint32_t* block_counts, uint32_t* block_offsets, uint32_t* dst1, uint32_t* size1, uint32_t append1,
                        uint32_t* dst2, uint32_t* size2, uint32_t append2, hipStream_t s);

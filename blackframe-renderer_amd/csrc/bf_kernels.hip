@@ -14,7 +14,6 @@
 #include "bf_shading.h"
 #include "bf_traversal.h"
 
-#define BF_TRACE_BLOCK 128u
 #define BF_BLOCK 256u
 
 /* ---- path helpers ----------------------------------------------------------------------------- */
@@ -126,81 +125,108 @@ __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint
 }
 
 /* ---- closest-hit stage: gather + traversal + classify + miss ---------------------------------- */
+/* The traversal kernels are persistent: a grid sized to the device's resident capacity, each wave
+ * taking 64 consecutive queue items at a time with a static stride, so that the per-lane stack
+ * spill area is bounded by the grid and empty tail workgroups cost nothing. */
+BF_D uint32_t wave_index() { return (blockIdx.x * BF_TRACE_BLOCK + threadIdx.x) >> 6; }
+BF_D uint32_t wave_count() { return (gridDim.x * BF_TRACE_BLOCK) >> 6; }
+
 template <bool COUNT>
-__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_trace_closest(bf_transport_params P, const uint32_t* queue,
-                                                                  const uint32_t* count_ptr, uint8_t* route,
-                                                                  unsigned long long* counters) {
-    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = *count_ptr;
-    trav_counters ctr{0u, 0u, 0u};
-    if (i < n) {
-        const uint32_t slot = queue[i];
-        const float4 ro = P.paths.ray_origin[slot];
-        const float4 rd = P.paths.ray_direction[slot];
-        trav_ray ray;
-        ray.o = mk(ro.x, ro.y, ro.z);
-        ray.d = mk(rd.x, rd.y, rd.z);
-        ray.t_min = rd.w;
-        ray.t_max = (P.paths.depth[slot].y == 0u) ? P.camera.t_max : BF_INFINITY_F;
-        ray.mask = P.paths.visibility[slot];
-        trav_hit best;
-        bool occluded;
-        const uint32_t st = traverse<false, COUNT>(P.scene, ray, s_ref + threadIdx.x, s_near + threadIdx.x,
-                                                   BF_TRACE_BLOCK, best, occluded, ctr);
-        uint8_t r = 0;
-        if (st != BF_HIT_MISS) {
-            fail_path(P, slot, 60u + st);
-        } else if (best.present) {
-            P.paths.hit[slot] = make_float4(best.t, best.b0, best.b1, best.b2);
-            P.paths.hit_ids[slot] = make_uint2(best.scene_instance, best.primitive);
-            r = 1;
-        } else {
-            /* miss stage: CudaWavefrontStages.cu:3962-4040 */
-            if (P.scene.has_environment) {
-                const float4 wl = P.paths.wavelengths[slot];
-                const float4 beta = P.paths.beta[slot];
-                float4 L = P.paths.radiance[slot];
-                const float w4[4] = {wl.x, wl.y, wl.z, wl.w};
-                const float b4[4] = {beta.x, beta.y, beta.z, beta.w};
-                float l4[4] = {L.x, L.y, L.z, L.w};
-                bool ok = true;
+BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
+                             trav_stack<BF_SHORT_STACK>& stack, trav_counters& ctr) {
+    const uint32_t slot = queue[i];
+    const float4 ro = P.paths.ray_origin[slot];
+    const float4 rd = P.paths.ray_direction[slot];
+    trav_ray ray;
+    ray.o = mk(ro.x, ro.y, ro.z);
+    ray.d = mk(rd.x, rd.y, rd.z);
+    ray.t_min = rd.w;
+    ray.t_max = (P.paths.depth[slot].y == 0u) ? P.camera.t_max : BF_INFINITY_F;
+    ray.mask = P.paths.visibility[slot];
+    trav_hit best;
+    bool occluded;
+    stack.sp = 0u;
+    stack.gsp = 0u;
+    const auto world = [&](v3f& o, v3f& d) {
+        const float4 a = P.paths.ray_origin[slot];
+        const float4 b = P.paths.ray_direction[slot];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+    };
+    uint32_t st = traverse<false, COUNT>(P.scene, ray, world, stack, best, occluded, ctr);
+    if (st == BF_HIT_MISS && best.present) {
+        v3f o, d;
+        world(o, d);
+        st = closest_hit_barycentrics(P.scene, o, d, ray.t_min, ray.t_max, best);
+    }
+    uint8_t r = 0;
+    if (st != BF_HIT_MISS) {
+        fail_path(P, slot, 60u + st);
+    } else if (best.present) {
+        P.paths.hit[slot] = make_float4(best.t, best.b0, best.b1, best.b2);
+        P.paths.hit_ids[slot] = make_uint2(best.scene_instance, best.primitive);
+        r = 1;
+    } else {
+        /* miss stage: CudaWavefrontStages.cu:3962-4040 */
+        if (P.scene.has_environment) {
+            const float4 wl = P.paths.wavelengths[slot];
+            const float4 beta = P.paths.beta[slot];
+            float4 L = P.paths.radiance[slot];
+            const float w4[4] = {wl.x, wl.y, wl.z, wl.w};
+            const float b4[4] = {beta.x, beta.y, beta.z, beta.w};
+            float l4[4] = {L.x, L.y, L.z, L.w};
+            bool ok = true;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    float product;
-                    const float radiance = spectrum_at(P.scene, P.scene.environment_spectrum, w4[k]);
-                    if (!checked_product(b4[k], radiance, product)) ok = false;
-                    else {
-                        l4[k] += product;
-                        if (!bf_isfinite(l4[k]) || l4[k] < 0.0f) ok = false;
-                    }
+            for (int k = 0; k < 4; ++k) {
+                float product;
+                const float radiance = spectrum_at(P.scene, P.scene.environment_spectrum, w4[k]);
+                if (!checked_product(b4[k], radiance, product)) ok = false;
+                else {
+                    l4[k] += product;
+                    if (!bf_isfinite(l4[k]) || l4[k] < 0.0f) ok = false;
                 }
-                if (!ok) {
-                    fail_path(P, slot, 50);
-                } else {
-                    P.paths.radiance[slot] = make_float4(l4[0], l4[1], l4[2], l4[3]);
-                    P.paths.status[slot] = BF_TERM_ESCAPED;
-                }
+            }
+            if (!ok) {
+                fail_path(P, slot, 50);
             } else {
+                P.paths.radiance[slot] = make_float4(l4[0], l4[1], l4[2], l4[3]);
                 P.paths.status[slot] = BF_TERM_ESCAPED;
             }
-        }
-        route[i] = r;
-    }
-    if (COUNT) {
-        /* wave-reduced traversal counters (nodes, triangles, instances) */
-        for (int off = 32; off > 0; off >>= 1) {
-            ctr.nodes += __shfl_down(ctr.nodes, off, 64);
-            ctr.triangles += __shfl_down(ctr.triangles, off, 64);
-            ctr.instances += __shfl_down(ctr.instances, off, 64);
-        }
-        if ((threadIdx.x & 63u) == 0u) {
-            atomicAdd(&counters[0], (unsigned long long)ctr.nodes);
-            atomicAdd(&counters[1], (unsigned long long)ctr.triangles);
-            atomicAdd(&counters[2], (unsigned long long)ctr.instances);
+        } else {
+            P.paths.status[slot] = BF_TERM_ESCAPED;
         }
     }
+    route[i] = r;
+}
+
+BF_D void reduce_counters(const trav_counters& c, unsigned long long* counters) {
+    trav_counters ctr = c;
+    for (int off = 32; off > 0; off >>= 1) {
+        ctr.nodes += __shfl_down(ctr.nodes, off, 64);
+        ctr.triangles += __shfl_down(ctr.triangles, off, 64);
+        ctr.instances += __shfl_down(ctr.instances, off, 64);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        atomicAdd(&counters[0], (unsigned long long)ctr.nodes);
+        atomicAdd(&counters[1], (unsigned long long)ctr.triangles);
+        atomicAdd(&counters[2], (unsigned long long)ctr.instances);
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(BF_TRACE_BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closest(bf_transport_params P, const uint32_t* queue,
+                                                                  const uint32_t* count_ptr, uint8_t* route,
+                                                                  unsigned long long* counters) {
+    __shared__ uint2 s_stack[BF_SHORT_STACK * BF_TRACE_BLOCK];
+    const uint32_t n = *count_ptr;
+    trav_stack<BF_SHORT_STACK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BF_TRACE_BLOCK + threadIdx.x,
+                                     P.spill_stride, 0u, 0u};
+    trav_counters ctr{0u, 0u, 0u};
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = wave_index() * 64u; base < n; base += wave_count() * 64u) {
+        if (base + lane < n) trace_closest_item<COUNT>(P, queue, base + lane, route, stack, ctr);
+    }
+    if (COUNT) reduce_counters(ctr, counters);
 }
 
 /* ---- shade stage: CudaWavefrontStages.cu:4107-4727 -------------------------------------------- */
@@ -639,69 +665,75 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
     route[i] = 2;
 }
 
+template <bool COUNT>
+BF_D void trace_shadow_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
+                            trav_stack<BF_SHORT_STACK>& stack, trav_counters& ctr) {
+    const uint32_t slot = queue[i];
+    const float4 so = P.paths.shadow_origin[slot];
+    const float4 sd = P.paths.shadow_direction[slot];
+    trav_ray ray;
+    ray.o = mk(so.x, so.y, so.z);
+    ray.d = mk(sd.x, sd.y, sd.z);
+    ray.t_min = 0.0f;
+    ray.t_max = so.w;
+    ray.mask = P.paths.visibility[slot];
+    trav_hit best;
+    bool occluded;
+    stack.sp = 0u;
+    stack.gsp = 0u;
+    const auto world = [&](v3f& o, v3f& d) {
+        const float4 a = P.paths.shadow_origin[slot];
+        const float4 b = P.paths.shadow_direction[slot];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+    };
+    const uint32_t st = traverse<true, COUNT>(P.scene, ray, world, stack, best, occluded, ctr);
+    const uint32_t packed = __float_as_uint(sd.w);
+    uint8_t r = 0;
+    if (st != BF_HIT_MISS) {
+        fail_path(P, slot, 40);
+    } else {
+        bool ok = true;
+        if (!occluded) {
+            const float4 c = P.paths.shadow_contribution[slot];
+            float4 L = P.paths.radiance[slot];
+            L.x += c.x;
+            L.y += c.y;
+            L.z += c.z;
+            L.w += c.w;
+            if (!bf_isfinite(L.x) || !bf_isfinite(L.y) || !bf_isfinite(L.z) || !bf_isfinite(L.w) || L.x < 0.0f ||
+                L.y < 0.0f || L.z < 0.0f || L.w < 0.0f) {
+                ok = false;
+                fail_path(P, slot, 41);
+            } else {
+                P.paths.radiance[slot] = L;
+            }
+        }
+        if (ok) {
+            if (packed & 1u) r = 2;
+            else P.paths.status[slot] = packed >> 8;
+        }
+    }
+    route[i] = r;
+}
+
 /* ---- shadow stage: gather + any-hit + process_shadow (:4729-4908) ---------------------------- */
 template <bool COUNT>
-__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
+__global__ __launch_bounds__(BF_TRACE_BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
                                                                  const uint32_t* count_ptr, uint8_t* route,
                                                                  unsigned long long* counters) {
-    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint2 s_stack[BF_SHORT_STACK * BF_TRACE_BLOCK];
     const uint32_t n = *count_ptr;
+    trav_stack<BF_SHORT_STACK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BF_TRACE_BLOCK + threadIdx.x,
+                                     P.spill_stride, 0u, 0u};
     trav_counters ctr{0u, 0u, 0u};
-    if (i < n) {
-        const uint32_t slot = queue[i];
-        const float4 so = P.paths.shadow_origin[slot];
-        const float4 sd = P.paths.shadow_direction[slot];
-        trav_ray ray;
-        ray.o = mk(so.x, so.y, so.z);
-        ray.d = mk(sd.x, sd.y, sd.z);
-        ray.t_min = 0.0f;
-        ray.t_max = so.w;
-        ray.mask = P.paths.visibility[slot];
-        trav_hit best;
-        bool occluded;
-        const uint32_t st = traverse<true, COUNT>(P.scene, ray, s_ref + threadIdx.x, s_near + threadIdx.x, BF_TRACE_BLOCK,
-                                                  best, occluded, ctr);
-        const uint32_t packed = __float_as_uint(sd.w);
-        uint8_t r = 0;
-        if (st != BF_HIT_MISS) {
-            fail_path(P, slot, 40);
-        } else {
-            bool ok = true;
-            if (!occluded) {
-                const float4 c = P.paths.shadow_contribution[slot];
-                float4 L = P.paths.radiance[slot];
-                L.x += c.x;
-                L.y += c.y;
-                L.z += c.z;
-                L.w += c.w;
-                if (!bf_isfinite(L.x) || !bf_isfinite(L.y) || !bf_isfinite(L.z) || !bf_isfinite(L.w) || L.x < 0.0f ||
-                    L.y < 0.0f || L.z < 0.0f || L.w < 0.0f) {
-                    ok = false;
-                    fail_path(P, slot, 41);
-                } else {
-                    P.paths.radiance[slot] = L;
-                }
-            }
-            if (ok) {
-                if (packed & 1u) r = 2;
-                else P.paths.status[slot] = packed >> 8;
-            }
-        }
-        route[i] = r;
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = wave_index() * 64u; base < n; base += wave_count() * 64u) {
+        if (base + lane < n) trace_shadow_item<COUNT>(P, queue, base + lane, route, stack, ctr);
     }
     if (COUNT) {
-        for (int off = 32; off > 0; off >>= 1) {
-            ctr.nodes += __shfl_down(ctr.nodes, off, 64);
-            ctr.triangles += __shfl_down(ctr.triangles, off, 64);
-            ctr.instances += __shfl_down(ctr.instances, off, 64);
-        }
-        if ((threadIdx.x & 63u) == 0u) {
-            atomicAdd(&counters[3], (unsigned long long)ctr.nodes);
-            atomicAdd(&counters[4], (unsigned long long)ctr.triangles);
-            atomicAdd(&counters[5], (unsigned long long)ctr.instances);
-        }
+        trav_counters c = ctr;
+        reduce_counters(c, counters + 3);
     }
 }
 
@@ -900,8 +932,8 @@ __global__ void k_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, u
 __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_scene S, const bf_transport_ray* rays,
                                                                       uint32_t count, bf_closest_hit_result* results,
                                                                       uint32_t* counters) {
-    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    __shared__ uint2 s_stack[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    trav_stack<BF_STACK_DEPTH> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const bf_transport_ray r = rays[i];
@@ -919,8 +951,13 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_sce
         ray.mask = r.visibility_mask;
         trav_hit best;
         bool occ;
-        const uint32_t st = traverse<false, true>(S, ray, s_ref + threadIdx.x, s_near + threadIdx.x, BF_TRACE_BLOCK, best,
-                                                  occ, ctr);
+        const auto world = [&](v3f& o, v3f& d) {
+            o = mk(rays[i].origin_x, rays[i].origin_y, rays[i].origin_z);
+            d = mk(rays[i].direction_x, rays[i].direction_y, rays[i].direction_z);
+        };
+        uint32_t st = traverse<false, true>(S, ray, world, stack, best, occ, ctr);
+        if (st == BF_HIT_MISS && best.present)
+            st = closest_hit_barycentrics(S, ray.o, ray.d, ray.t_min, ray.t_max, best);
         if (st != BF_HIT_MISS) {
             out.status = st;
         } else if (best.present) {
@@ -958,8 +995,8 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_sce
 
 __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_occlusion(bf_dev_scene S, const bf_transport_ray* rays,
                                                                     uint32_t count, bf_occlusion_result* results) {
-    __shared__ uint32_t s_ref[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    __shared__ float s_near[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    __shared__ uint2 s_stack[BF_STACK_DEPTH * BF_TRACE_BLOCK];
+    trav_stack<BF_STACK_DEPTH> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const bf_transport_ray r = rays[i];
@@ -977,8 +1014,11 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_occlusion(bf_dev_scene
         trav_hit best;
         bool occ;
         trav_counters ctr{0u, 0u, 0u};
-        const uint32_t st = traverse<true, false>(S, ray, s_ref + threadIdx.x, s_near + threadIdx.x, BF_TRACE_BLOCK, best,
-                                                  occ, ctr);
+        const auto world = [&](v3f& o, v3f& d) {
+            o = mk(rays[i].origin_x, rays[i].origin_y, rays[i].origin_z);
+            d = mk(rays[i].direction_x, rays[i].direction_y, rays[i].direction_z);
+        };
+        const uint32_t st = traverse<true, false>(S, ray, world, stack, best, occ, ctr);
         out.status = st != BF_HIT_MISS ? st : (occ ? BF_OCCLUDED : BF_VISIBLE);
     }
     results[i] = out;
@@ -1043,16 +1083,35 @@ hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ra
     return hipGetLastError();
 }
 
+/* Workgroups of one persistent traversal launch: enough to fill every CU at the kernels'
+ * occupancy, never more than the workspace's spill area covers, and no more than the items need. */
+static uint32_t trace_blocks(const bf_transport_params* P, uint32_t max_count) {
+    const uint32_t needed = blocks_for((max_count + 63u) / 64u * 64u, BF_TRACE_BLOCK);
+    const uint32_t resident = P->spill_stride / BF_TRACE_BLOCK;
+    return needed < resident ? needed : resident;
+}
+
 hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                              uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s) {
     if (max_count == 0) return hipSuccess;
+    const dim3 grid(trace_blocks(P, max_count));
     if (counters)
-        hipLaunchKernelGGL(k_trace_closest<true>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
-                           *P, queue, count_ptr, route, counters);
+        hipLaunchKernelGGL(k_trace_closest<true>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
     else
-        hipLaunchKernelGGL(k_trace_closest<false>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
-                           *P, queue, count_ptr, route, counters);
+        hipLaunchKernelGGL(k_trace_closest<false>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
     return hipGetLastError();
+}
+
+uint32_t bfk_trace_resident_blocks(int device) {
+    int cus = 0, per_cu = 0, m = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) return 0;
+    const void* kernels[4] = {(const void*)k_trace_closest<false>, (const void*)k_trace_closest<true>,
+                              (const void*)k_trace_shadow<false>, (const void*)k_trace_shadow<true>};
+    for (const void* k : kernels) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, BF_TRACE_BLOCK, 0) != hipSuccess) return 0;
+        if (per_cu > m) m = per_cu;
+    }
+    return (uint32_t)(cus * m);
 }
 
 hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
@@ -1066,12 +1125,11 @@ hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const 
 hipError_t bfk_trace_shadow(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                             uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s) {
     if (max_count == 0) return hipSuccess;
+    const dim3 grid(trace_blocks(P, max_count));
     if (counters)
-        hipLaunchKernelGGL(k_trace_shadow<true>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
-                           *P, queue, count_ptr, route, counters);
+        hipLaunchKernelGGL(k_trace_shadow<true>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
     else
-        hipLaunchKernelGGL(k_trace_shadow<false>, dim3(blocks_for(max_count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s,
-                           *P, queue, count_ptr, route, counters);
+        hipLaunchKernelGGL(k_trace_shadow<false>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
     return hipGetLastError();
 }
 

@@ -12,7 +12,8 @@
  * rounded UP to binary32 (exactly equivalent for a binary32 t_max), which lets a popped leaf go
  * straight to its triangle without fetching a node. The reference keeps one 64-entry stack per
  * structure; here TLAS and BLAS share one stack of BF_STACK_DEPTH entries and the host rejects
- * scenes whose TLAS depth + BLAS depth + 2 exceeds it.
+ * scenes whose TLAS depth + BLAS depth + 2 exceeds it. Slab quotients are formed from a per-ray
+ * binary64 reciprocal with one correction step, which yields the correctly rounded quotient.
  */
 #ifndef BF_TRAVERSAL_H
 #define BF_TRAVERSAL_H
@@ -36,15 +37,40 @@ struct trav_counters {
     uint32_t nodes, triangles, instances;
 };
 
-/* CudaSceneClosestHit.cu:407-455 (boxes are validated finite and ordered by the host builder) */
-BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, v3f o, v3f d, float t_min, float t_max,
-                   bool& hit, double& near_value) {
+/* A ray expressed in one space (world for the TLAS, an instance's object space for its BLAS),
+ * with the correctly rounded binary64 reciprocal of every non-zero direction component. */
+struct trav_space {
+    v3f o, d;
+    double inv[3];
+};
+
+BF_D void make_space(v3f o, v3f d, trav_space& s) {
+    s.o = o;
+    s.d = d;
+    s.inv[0] = d.x != 0.0f ? 1.0 / (double)d.x : 0.0;
+    s.inv[1] = d.y != 0.0f ? 1.0 / (double)d.y : 0.0;
+    s.inv[2] = d.z != 0.0f ? 1.0 / (double)d.z : 0.0;
+}
+
+/* a / b rounded to nearest from y = RN(1/b): q0 = RN(a y), r = a - b q0 (exact through the fma),
+ * q0 + r y rounded once is RN(a / b) (Markstein's quotient correction), i.e. the same binary64
+ * quotient the reference's division returns, for three fused operations instead of a division. */
+BF_D double quotient(double a, double b, double y) {
+    const double q0 = a * y;
+    const double r = fma(-q0, b, a);
+    return fma(r, y, q0);
+}
+
+/* CudaSceneClosestHit.cu:407-455 (boxes are validated finite and ordered by the host builder):
+ * per axis the binary64 plane distances (lo - o) / d and (hi - o) / d, widened by one ulp. */
+BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, const trav_space& s, float t_min,
+                   float t_max, bool& hit, double& near_value) {
     double nv = (double)t_min;
     double fv = (double)t_max;
     const float lo[3] = {lx, ly, lz};
     const float hi[3] = {hx, hy, hz};
-    const float oo[3] = {o.x, o.y, o.z};
-    const float dd[3] = {d.x, d.y, d.z};
+    const float oo[3] = {s.o.x, s.o.y, s.o.z};
+    const float dd[3] = {s.d.x, s.d.y, s.d.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (dd[a] == 0.0f) {
@@ -54,8 +80,9 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, v
             }
             continue;
         }
-        const double first = ((double)lo[a] - (double)oo[a]) / (double)dd[a];
-        const double second = ((double)hi[a] - (double)oo[a]) / (double)dd[a];
+        const double db = (double)dd[a];
+        const double first = quotient((double)lo[a] - (double)oo[a], db, s.inv[a]);
+        const double second = quotient((double)hi[a] - (double)oo[a], db, s.inv[a]);
         if (bf_isnand(first) || bf_isnand(second)) return BF_HIT_NUMERICAL_FAILURE;
         nv = fmax(nv, bf_next_down(fmin(first, second)));
         fv = fmin(fv, bf_next_up(fmax(first, second)));
@@ -216,134 +243,194 @@ BF_D bool precedes(float t, uint32_t inst, uint32_t prim, const trav_hit& c) {
 }
 
 /*
- * Traverses TLAS then BLASes depth-first. `sref`/`snear` point at this lane's column of the LDS
- * stack (entry i at index i*stride). Returns a traversal status (BF_HIT_MISS = no error).
+ * Per-lane traversal stack of (node reference, near bits) pairs. The top S entries live in LDS,
+ * laid out [entry][lane] so that a wave's pushes and pops touch consecutive banks; when a push
+ * finds them full, the lower half moves to the lane's column of a global spill area and comes
+ * back when the LDS part runs empty. With S = BF_STACK_DEPTH nothing ever spills.
  */
-template <bool ANY, bool COUNT>
-BF_D uint32_t traverse(const bf_dev_scene& S, const trav_ray& ray, uint32_t* sref, float* snear, uint32_t stride,
-                       trav_hit& best, bool& occluded, trav_counters& ctr) {
+template <uint32_t S>
+struct trav_stack {
+    uint2* lds;             /* entry k at lds[k * BF_TRACE_BLOCK] */
+    uint2* spill;           /* entry k at spill[k * spill_stride] */
+    uint32_t spill_stride;
+    uint32_t sp, gsp;       /* entries in LDS, entries spilled */
+};
+
+template <uint32_t S>
+BF_D uint32_t stack_depth(const trav_stack<S>& s) { return s.sp + s.gsp; }
+
+template <uint32_t S>
+BF_D void stack_push(trav_stack<S>& s, uint32_t ref, float nearv) {
+    if (S < BF_STACK_DEPTH && s.sp == S) {
+#pragma unroll
+        for (uint32_t k = 0; k < S / 2; ++k) s.spill[(size_t)(s.gsp + k) * s.spill_stride] = s.lds[k * BF_TRACE_BLOCK];
+#pragma unroll
+        for (uint32_t k = 0; k < S / 2; ++k) s.lds[k * BF_TRACE_BLOCK] = s.lds[(k + S / 2) * BF_TRACE_BLOCK];
+        s.gsp += S / 2;
+        s.sp = S / 2;
+    }
+    s.lds[s.sp * BF_TRACE_BLOCK] = make_uint2(ref, __float_as_uint(nearv));
+    ++s.sp;
+}
+
+template <uint32_t S>
+BF_D uint2 stack_pop(trav_stack<S>& s) {
+    if (S < BF_STACK_DEPTH && s.sp == 0u) {
+        s.gsp -= S / 2;
+#pragma unroll
+        for (uint32_t k = 0; k < S / 2; ++k) s.lds[k * BF_TRACE_BLOCK] = s.spill[(size_t)(s.gsp + k) * s.spill_stride];
+        s.sp = S / 2;
+    }
+    --s.sp;
+    return s.lds[s.sp * BF_TRACE_BLOCK];
+}
+
+/*
+ * Traverses TLAS then BLASes depth-first in the reference's order. Returns a traversal status
+ * (BF_HIT_MISS = no error). The stack must be empty on entry and is empty on a normal return.
+ * `world(o, d)` re-reads the world-space ray when a BLAS is left, so that no copy of it stays
+ * live through the loop. The winning candidate keeps (t, ids, leaf); its barycentrics are
+ * re-derived by closest_hit_barycentrics from the same deterministic triangle test.
+ */
+template <bool ANY, bool COUNT, uint32_t S, typename WorldRay>
+BF_D uint32_t traverse(const bf_dev_scene& S_, const trav_ray& ray, WorldRay world, trav_stack<S>& stack, trav_hit& best,
+                       bool& occluded, trav_counters& ctr) {
     best.present = false;
     best.t = ray.t_max;
     occluded = false;
-    if (S.tlas_root_ref == BF_INVALID_INDEX) return BF_HIT_MISS;
-    uint32_t sp = 0;
+    if (S_.tlas_root_ref == BF_INVALID_INDEX) return BF_HIT_MISS;
+    const float t_min = ray.t_min;
+    const float t_max = ray.t_max;
+    const uint32_t mask = ray.mask;
+    trav_space space;
+    make_space(ray.o, ray.d, space);
     {
         bool h;
         double nv;
-        const uint32_t st = slab(S.tlas_root_box[0], S.tlas_root_box[1], S.tlas_root_box[2], S.tlas_root_box[3],
-                                 S.tlas_root_box[4], S.tlas_root_box[5], ray.o, ray.d, ray.t_min, ray.t_max, h, nv);
+        const uint32_t st = slab(S_.tlas_root_box[0], S_.tlas_root_box[1], S_.tlas_root_box[2], S_.tlas_root_box[3],
+                                 S_.tlas_root_box[4], S_.tlas_root_box[5], space, t_min, t_max, h, nv);
         if (st != BF_HIT_MISS) return st;
         if (!h) {
             if (COUNT) ctr.nodes++;
             return BF_HIT_MISS;
         }
-        sref[0] = S.tlas_root_ref;
-        snear[0] = round_up_float(nv);
-        sp = 1;
+        stack_push(stack, S_.tlas_root_ref, round_up_float(nv));
     }
     bool in_blas = false;
     uint32_t blas_base = 0;
-    v3f lo = ray.o, ld = ray.d;
     uint32_t inst_id = 0, scene_inst = 0;
-    while (sp != 0) {
-        if (in_blas && sp <= blas_base) {
+    while (stack_depth(stack) != 0u) {
+        if (in_blas && stack_depth(stack) <= blas_base) {
             in_blas = false;
-            lo = ray.o;
-            ld = ray.d;
+            v3f o, d;
+            world(o, d);
+            make_space(o, d, space); /* back in world space (recomputing is exact) */
         }
-        --sp;
-        const uint32_t ref = sref[sp * stride];
-        const float nearv = snear[sp * stride];
+        const uint2 entry = stack_pop(stack);
+        const uint32_t ref = entry.x;
+        const float nearv = __uint_as_float(entry.y);
         if (COUNT) ctr.nodes++;
-        const float tcur = ANY ? ray.t_max : best.t;
+        const float tcur = ANY ? t_max : best.t;
         if (!(nearv <= tcur)) continue;
         if (ref & BF_WIDE_LEAF) {
             const uint32_t leaf = ref & ~BF_WIDE_LEAF;
             if (!in_blas) {
-                const bf_dev_tlas_leaf L = S.tlas_leaves[leaf];
-                if ((ray.mask & L.visibility_mask) == 0u) continue;
-                lo = xform_point(L.world_to_local, ray.o);
-                ld = xform_vector(L.world_to_local, ray.d);
+                const bf_dev_tlas_leaf L = S_.tlas_leaves[leaf];
+                if ((mask & L.visibility_mask) == 0u) continue;
+                const v3f lo = xform_point(L.world_to_local, space.o);
+                const v3f ld = xform_vector(L.world_to_local, space.d);
                 if (!fin3(lo) || !fin3(ld) || (ld.x == 0.0f && ld.y == 0.0f && ld.z == 0.0f))
                     return BF_HIT_NUMERICAL_FAILURE;
                 if (COUNT) ctr.instances++;
-                inst_id = L.instance_id;
-                scene_inst = L.scene_instance;
-                const bf_dev_blas B = S.blases[L.blas_index];
+                const bf_dev_blas B = S_.blases[L.blas_index];
+                trav_space local;
+                make_space(lo, ld, local);
                 bool h;
                 double nv;
                 const uint32_t st = slab(B.root_box[0], B.root_box[1], B.root_box[2], B.root_box[3], B.root_box[4],
-                                         B.root_box[5], lo, ld, ray.t_min, tcur, h, nv);
+                                         B.root_box[5], local, t_min, tcur, h, nv);
                 if (st != BF_HIT_MISS) return st;
                 if (!h) {
                     if (COUNT) ctr.nodes++;
-                    lo = ray.o;
-                    ld = ray.d;
                     continue;
                 }
-                if (sp >= BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+                if (stack_depth(stack) >= BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+                inst_id = L.instance_id;
+                scene_inst = L.scene_instance;
                 in_blas = true;
-                blas_base = sp;
-                sref[sp * stride] = B.root_ref;
-                snear[sp * stride] = round_up_float(nv);
-                ++sp;
+                blas_base = stack_depth(stack);
+                space = local;
+                stack_push(stack, B.root_ref, round_up_float(nv));
                 continue;
             }
-            const float4 q0 = S.leaf_triangles[3 * (size_t)leaf + 0];
-            const float4 q1 = S.leaf_triangles[3 * (size_t)leaf + 1];
-            const float4 q2 = S.leaf_triangles[3 * (size_t)leaf + 2];
+            const float4 q0 = S_.leaf_triangles[3 * (size_t)leaf + 0];
+            const float4 q1 = S_.leaf_triangles[3 * (size_t)leaf + 1];
+            const float4 q2 = S_.leaf_triangles[3 * (size_t)leaf + 2];
             const uint32_t prim = __float_as_uint(q0.w);
             if (COUNT) ctr.triangles++;
             tri_result tr;
             tr.hit = false;
-            const uint32_t st = watertight<!ANY>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), lo, ld,
-                                                 ray.t_min, tcur, tr);
+            const uint32_t st = watertight<false>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), space.o,
+                                                  space.d, t_min, tcur, tr);
             if (st != BF_HIT_MISS) return st;
             if (!tr.hit) continue;
             if (ANY) {
                 occluded = true;
+                stack.sp = 0;
+                stack.gsp = 0;
                 return BF_HIT_MISS;
             }
             if (!precedes(tr.t, inst_id, prim, best)) continue;
             best.present = true;
             best.t = tr.t;
-            best.b0 = tr.b0;
-            best.b1 = tr.b1;
-            best.b2 = tr.b2;
             best.instance_id = inst_id;
             best.primitive = prim;
             best.scene_instance = scene_inst;
             best.leaf = leaf;
             continue;
         }
-        const bf_wide_node* nodes = in_blas ? S.blas_nodes : S.tlas_nodes;
+        const bf_wide_node* nodes = in_blas ? S_.blas_nodes : S_.tlas_nodes;
         const bf_wide_node N = nodes[ref];
         bool h0, h1;
         double n0 = 0.0, n1 = 0.0;
-        uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], lo, ld, ray.t_min, tcur, h0, n0);
+        uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], space, t_min, tcur, h0, n0);
         if (st != BF_HIT_MISS) return st;
-        st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], lo, ld, ray.t_min, tcur, h1, n1);
+        st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], space, t_min, tcur, h1, n1);
         if (st != BF_HIT_MISS) return st;
         if (!h0 && !h1) continue;
         if (h0 && h1) {
-            if (sp + 2 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+            if (stack_depth(stack) + 2 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
             const float f0 = (float)n0, f1 = (float)n1;
             const bool first_near = f0 <= f1; /* ties: the first child has the lower canonical index */
-            const uint32_t far_ref = first_near ? N.child[1] : N.child[0];
-            const uint32_t near_ref = first_near ? N.child[0] : N.child[1];
-            sref[sp * stride] = far_ref;
-            snear[sp * stride] = round_up_float(first_near ? n1 : n0);
-            ++sp;
-            sref[sp * stride] = near_ref;
-            snear[sp * stride] = round_up_float(first_near ? n0 : n1);
-            ++sp;
+            stack_push(stack, first_near ? N.child[1] : N.child[0], round_up_float(first_near ? n1 : n0));
+            stack_push(stack, first_near ? N.child[0] : N.child[1], round_up_float(first_near ? n0 : n1));
         } else {
-            if (sp + 1 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
-            sref[sp * stride] = h0 ? N.child[0] : N.child[1];
-            snear[sp * stride] = round_up_float(h0 ? n0 : n1);
-            ++sp;
+            if (stack_depth(stack) + 1 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+            stack_push(stack, h0 ? N.child[0] : N.child[1], round_up_float(h0 ? n0 : n1));
         }
     }
+    return BF_HIT_MISS;
+}
+
+/* Barycentrics of the winning candidate: the triangle test is a pure function of the triangle, the
+ * object-space ray and the parameter range, and the winner passed it within [t_min, t_max] of the
+ * ray (re-testing against its own t could fail: the range test compares sp with t_max * det
+ * before the division that produces t). */
+BF_D uint32_t closest_hit_barycentrics(const bf_dev_scene& S, v3f o, v3f d, float t_min, float t_max, trav_hit& best) {
+    const float* m = S.instances[best.scene_instance].world_to_local;
+    const v3f lo = xform_point(m, o), ld = xform_vector(m, d);
+    const float4 q0 = S.leaf_triangles[3 * (size_t)best.leaf + 0];
+    const float4 q1 = S.leaf_triangles[3 * (size_t)best.leaf + 1];
+    const float4 q2 = S.leaf_triangles[3 * (size_t)best.leaf + 2];
+    tri_result tr;
+    tr.hit = false;
+    const uint32_t st = watertight<true>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), lo, ld, t_min,
+                                         t_max, tr);
+    if (st != BF_HIT_MISS) return st;
+    if (!tr.hit || tr.t != best.t) return BF_HIT_NUMERICAL_FAILURE;
+    best.b0 = tr.b0;
+    best.b1 = tr.b1;
+    best.b2 = tr.b2;
     return BF_HIT_MISS;
 }
 
