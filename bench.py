@@ -114,6 +114,167 @@ def cpu_baseline(bf, scene, frame, options, target_seconds, threads):
     }
 
 
+def batched_section(bf, hip, args, log):
+    """Plain vs batched on the uneven-batch workload: 32 seeded scenes (10k-500k triangles,
+    128^2-1024^2, 4-256 spp, the same material and light generator), rendered once frame by frame
+    through trace_xpu_wavefront_transport and once through trace_xpu_wavefront_batch (shared
+    queues, bounce groups replayed as hipGraphs). One step of each mode renders every frame."""
+    import torch
+
+    t0 = time.perf_counter()
+    items = bf.scenes.batch_scenes()
+    log(f"batch: {len(items)} scenes generated in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    xscenes = [hip.XpuScene(scene) for scene, _, _ in items]
+    torch.cuda.synchronize()
+    log(f"batch: BVH builds + uploads {time.perf_counter() - t0:.1f} s")
+    frames = [bf.scenes.frame_desc(cam, spp, seed=args.seed) for _, cam, spp in items]
+    films = [hip.Film(cam.width, cam.height) for _, cam, _ in items]
+    paths = sum(f.camera.width * f.camera.height * f.samples_per_pixel for f in frames)
+    options = bf.scenes.transport_options(max_depth=16, mis=bf.abi.MIS_POWER)
+    options.failure_policy = bf.abi.FAILURES_COUNT
+    capacity = args.batch_capacity
+    ws = hip.XpuWavefrontTransportWorkspace(capacity)
+
+    def plain():
+        rays = 0
+        for x, f, film in zip(xscenes, frames, films):
+            r = hip.trace_xpu_wavefront_transport(ws, x, f, options, film=film, synchronize=False)
+            rays += r["report"].closest_hit_queries + r["report"].shadow_queries
+        return rays
+
+    def batched():
+        options.launch = bf.abi.LAUNCH_GRAPH | 4
+        try:
+            r = hip.trace_xpu_wavefront_batch(ws, [(x, f, film) for x, f, film in zip(xscenes, frames, films)], options,
+                                              synchronize=False)
+        finally:
+            options.launch = 0
+        return r["report"].closest_hit_queries + r["report"].shadow_queries
+
+    result = {"workload": (f"{len(items)} seeded scenes, {sum(s.triangle_count for s, _, _ in items)} tris in total "
+                           f"(10k-500k each), 128^2-1024^2, 4-256 spp, depth 16, power MIS"),
+              "paths": paths, "paths_in_flight": capacity}
+    for name, fn in (("plain", plain), ("batched", batched)):
+        rays = fn()  # warmup
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.batch_steps):
+            assert fn() == rays, "non-deterministic ray count"
+        torch.cuda.synchronize()
+        elapsed = (time.perf_counter() - t0) / args.batch_steps
+        result[f"{name}_s"] = round(elapsed, 4)
+        result[f"{name}_mrays_per_s"] = round(rays / elapsed / 1e6, 2)
+        result["rays"] = rays
+        log(f"batch: {name} {elapsed:.2f} s per batch, {rays / elapsed / 1e6:.1f} Mrays/s")
+    result["speedup"] = round(result["batched_mrays_per_s"] / result["plain_mrays_per_s"], 3)
+    ws.close()
+    return result
+
+
+def adjoint_section(bf, hip, args, log):
+    """Adjoint vs forward on the differentiable Cornell box: 256^2 x 64 spp (4 194 304 paths),
+    depth 8, balance MIS, roulette from depth 3, a 512^2 RGB albedo texture on the back wall and a
+    GGX conductor box (alpha 0.3, roughness texture). Loss = L2 to a target rendered with the
+    texture scaled by 0.8. Times: plain forward (no tape), taped forward, adjoint replay. The
+    gradient is checked against a central finite difference of the forward render along a random
+    direction over all texels (eps 1e-2); the ratio is reported only when that relative error is
+    <= 1e-2. The check runs with roulette off (every sampling decision independent of the
+    texels); the error with roulette on is reported beside it."""
+    import numpy as np
+    import torch
+
+    from blackframe_renderer_amd.autodiff import DifferentiableFrame
+
+    t0 = time.perf_counter()
+    host_albedo = bf.scenes.textured_cornell(8, 8, texture_size=512)[2]["albedo"]
+    albedo = torch.from_numpy(host_albedo).cuda().contiguous().requires_grad_(True)
+    roughness = torch.full((1, 1, 1), 0.3, dtype=torch.float32, device="cuda", requires_grad=True)
+    scene, cam, _ = bf.scenes.textured_cornell(256, 256, albedo=(albedo.data_ptr(), 512, 512, 3),
+                                               roughness=(roughness.data_ptr(), 1, 1, 1))
+    frame = bf.scenes.frame_desc(cam, 64, seed=args.seed)
+    options = bf.scenes.transport_options(max_depth=8, mis=bf.abi.MIS_BALANCE)
+    options.failure_policy = bf.abi.FAILURES_COUNT
+    xs = hip.XpuScene(scene)
+    df = DifferentiableFrame(xs, frame, options, [albedo, roughness])
+    log(f"adjoint: scene + 512^2 texture ready in {time.perf_counter() - t0:.1f} s")
+    with torch.no_grad():
+        base = albedo.detach().clone()
+        albedo.mul_(0.8)
+        target = df.render().clone()
+        albedo.copy_(base)
+
+    def timed(fn, steps):
+        fn()
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - start) / steps
+
+    plain_report = {}
+
+    def plain():
+        r = hip.trace_xpu_wavefront_transport(df.workspace, xs, frame, options, film=df.film, synchronize=False)
+        plain_report["r"] = r["report"]
+
+    image = df.render()
+    grad_image = (2.0 * (image - target)).contiguous()
+    steps = args.adjoint_steps
+    t_plain = timed(plain, steps)
+    t_taped = timed(df.render, steps)
+    df.render()
+    t_adjoint = timed(lambda: df.gradients(grad_image), steps)
+    rays = plain_report["r"].closest_hit_queries + plain_report["r"].shadow_queries
+    adj_report = df.last_adjoint_report
+    log(f"adjoint: forward {t_plain * 1e3:.1f} ms (taped {t_taped * 1e3:.1f} ms), adjoint {t_adjoint * 1e3:.1f} ms")
+
+    def directional_error(rr):
+        opts = bf.scenes.transport_options(max_depth=8, mis=bf.abi.MIS_BALANCE, rr_first_depth=3 if rr else 0)
+        opts.failure_policy = bf.abi.FAILURES_COUNT
+        check = DifferentiableFrame(xs, frame, opts, [albedo, roughness], workspace=df.workspace)
+        albedo.grad = None
+        roughness.grad = None
+        loss = ((check.image(albedo, roughness) - target) ** 2).sum()
+        loss.backward()
+        grad = albedo.grad.detach().double()
+        v = torch.randn(albedo.shape, generator=torch.Generator(device="cuda").manual_seed(7), device="cuda")
+
+        def value():
+            return float(((check.render().double() - target.double()) ** 2).sum())
+
+        eps = 1e-2
+        with torch.no_grad():
+            albedo.copy_(base + eps * v)
+            up = value()
+            albedo.copy_(base - eps * v)
+            dn = value()
+            albedo.copy_(base)
+        fd = (up - dn) / (2 * eps)
+        ad = float((grad * v.double()).sum())
+        return abs(ad - fd) / abs(fd), fd, ad
+
+    err, fd, ad = directional_error(rr=False)
+    err_rr, fd_rr, ad_rr = directional_error(rr=True)
+    log(f"adjoint: directional derivative fd {fd:.6g} adjoint {ad:.6g} (rel err {err:.2e}); "
+        f"with roulette fd {fd_rr:.6g} adjoint {ad_rr:.6g} (rel err {err_rr:.2e})")
+    result = {
+        "workload": ("textured Cornell box 256x256x64 spp (4194304 paths), 512^2 RGB albedo texture, GGX conductor "
+                     "box alpha 0.3, depth 8, balance MIS, roulette from depth 3, L2 loss"),
+        "forward_ms": round(t_plain * 1e3, 3),
+        "forward_taped_ms": round(t_taped * 1e3, 3),
+        "adjoint_ms": round(t_adjoint * 1e3, 3),
+        "forward_mrays_per_s": round(rays / t_plain / 1e6, 2),
+        "adjoint_closest_hit_queries": adj_report.closest_hit_queries,
+        "grad_rel_err": float(f"{err:.3e}"),
+        "grad_rel_err_with_roulette": float(f"{err_rr:.3e}"),
+        "adjoint_over_forward": round(t_adjoint / t_plain, 4) if err <= 1e-2 else None,
+        "taped_forward_plus_adjoint_over_forward": round((t_taped + t_adjoint) / t_plain, 4) if err <= 1e-2 else None,
+    }
+    return result
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +290,11 @@ def main():
     ap.add_argument("--capacity", type=int, default=0, help="paths in flight per launch (0: the whole frame)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the plain-vs-batched measurement")
+    ap.add_argument("--batch-steps", type=int, default=1)
+    ap.add_argument("--batch-capacity", type=int, default=1 << 27, help="paths in flight per batched launch")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -243,6 +409,12 @@ def main():
         "stages": stages,
         "cpu_baseline": None,
     }
+    if not args.no_batched and args.workload == "grid":
+        del ws
+        torch.cuda.empty_cache()
+        line["batched"] = batched_section(bf, hip, args, log)
+    if not args.no_adjoint and args.workload == "grid":
+        line["adjoint"] = adjoint_section(bf, hip, args, log)
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"CPU baseline with {threads} threads")

@@ -86,7 +86,8 @@ class SpectrumDesc(C.Structure):
 class ClosureDesc(C.Structure):
     _fields_ = [
         ("kind", C.c_uint32), ("lobes", C.c_uint32), ("weight_spectrum", C.c_uint32), ("eta_spectrum", C.c_uint32),
-        ("k_spectrum", C.c_uint32), ("reserved", C.c_uint32 * 3), ("parameters", C.c_float * 10),
+        ("k_spectrum", C.c_uint32), ("weight_texture", C.c_uint32), ("roughness_texture", C.c_uint32),
+        ("reserved", C.c_uint32), ("parameters", C.c_float * 10),
         ("reserved_tail", C.c_uint32 * 2),
     ]
 
@@ -123,8 +124,14 @@ class SceneDesc(C.Structure):
         ("positions", C.c_void_p), ("normals", C.c_void_p), ("uvs", C.c_void_p), ("indices", C.c_void_p),
         ("geometries", C.c_void_p), ("instances", C.c_void_p), ("materials", C.c_void_p), ("closures", C.c_void_p),
         ("spectra", C.c_void_p), ("spectrum_values", C.c_void_p),
-        ("environment_spectrum", C.c_uint32), ("reserved", C.c_uint32),
+        ("environment_spectrum", C.c_uint32), ("texture_count", C.c_uint32), ("textures", C.c_void_p),
     ]
+
+
+class TextureDesc(C.Structure):
+    """bf_texture_desc: float32 texels, host (copied) or device (borrowed); see blackframe_texture.h."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("channels", C.c_uint32), ("reserved", C.c_uint32),
+                ("data", C.c_void_p), ("device_data", C.c_void_p)]
 
 
 class CameraDesc(C.Structure):
@@ -144,7 +151,7 @@ class TransportOptions(C.Structure):
         ("russian_roulette_enabled", C.c_uint32), ("russian_roulette_first_depth", C.c_uint32),
         ("russian_roulette_minimum", C.c_float), ("russian_roulette_maximum", C.c_float),
         ("packet_wavelengths", C.c_float * 4), ("record_queues", C.c_uint32), ("instrumentation", C.c_uint32),
-        ("failure_policy", C.c_uint32), ("reserved", C.c_uint32),
+        ("failure_policy", C.c_uint32), ("launch", C.c_uint32),
     ]
 
 
@@ -156,6 +163,13 @@ class FrameDesc(C.Structure):
 MAX_BOUNCES = 64
 INSTRUMENT_DISABLED, INSTRUMENT_STAGE_EVENTS = 0, 1
 FAILURES_ABORT, FAILURES_COUNT = 0, 1
+LAUNCH_GRAPH = 0x80000000
+LAUNCH_GROUP_MASK = 0x0000FFFF
+
+
+def launch_trace_block(lanes):
+    """The launch-word field selecting the traversal workgroup size (64, 128 or 256 lanes)."""
+    return {64: 6, 128: 7, 256: 8}[lanes] << 16
 STAGES = ("camera", "closest_hit", "shade", "shadow", "compaction", "film", "auxiliary", "reserved")
 
 
@@ -169,11 +183,32 @@ class TransportReport(C.Structure):
         ("path_count", C.c_uint64), ("closest_hit_queries", C.c_uint64), ("shadow_queries", C.c_uint64),
         ("shade_lanes", C.c_uint64), ("miss_lanes", C.c_uint64), ("closure_samples", C.c_uint64),
         ("light_samples", C.c_uint64), ("failed_paths", C.c_uint64), ("bounces", C.c_uint32),
-        ("first_failure_slot", C.c_uint32), ("first_failure_code", C.c_uint32), ("reserved", C.c_uint32),
+        ("first_failure_slot", C.c_uint32), ("first_failure_code", C.c_uint32), ("first_failure_job", C.c_uint32),
         ("ray_queue_hash", C.c_uint64 * MAX_BOUNCES), ("ray_queue_size", C.c_uint32 * MAX_BOUNCES),
         ("shadow_queue_hash", C.c_uint64 * MAX_BOUNCES), ("shadow_queue_size", C.c_uint32 * MAX_BOUNCES),
         ("stage_metrics", StageMetric * 8),
     ]
+
+
+class FrameJob(C.Structure):
+    """One frame of a batched call (bf_frame_job): scene handle, frame, device film and path buffers."""
+    _fields_ = [("scene", C.c_void_p), ("frame", FrameDesc), ("film", C.c_void_p), ("film_compensated", C.c_void_p),
+                ("film_counts", C.c_void_p), ("paths", C.c_void_p), ("tape_radiance", C.c_void_p),
+                ("tape_visibility", C.c_void_p)]
+
+
+MAX_TEXTURES = 8
+
+
+class LaunchTuning(C.Structure):
+    """bf_launch_tuning: the autotuner's choice and the timings behind it."""
+    _fields_ = [("launch", C.c_uint32), ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
+                ("tried", C.c_uint32 * 24), ("tried_ms", C.c_float * 24)]
+
+
+class AdjointJob(C.Structure):
+    """bf_adjoint_job: dLoss/d(film RGB sums) in, accumulated texel gradients out (device pointers)."""
+    _fields_ = [("film_gradient", C.c_void_p), ("texture_gradients", C.c_void_p * MAX_TEXTURES)]
 
 
 class PathResult(C.Structure):
@@ -185,8 +220,8 @@ class PathResult(C.Structure):
 
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
-                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 136), (CameraDesc, 96),
-                       (TransportOptions, 80), (FrameDesc, 112), (TransportReport, 1808), (StageMetric, 24)):
+                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 208),
+                       (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 
 

@@ -71,6 +71,11 @@ typedef struct bf_dev_light {
     uint32_t reserved[3];
 } bf_dev_light;
 
+typedef struct bf_dev_texture {
+    const float* data;         /* texels, (y * width + x) * channels + c */
+    uint32_t width, height, channels, reserved;
+} bf_dev_texture;
+
 typedef struct bf_dev_scene {
     /* traversal */
     const bf_wide_node* tlas_nodes;
@@ -102,6 +107,9 @@ typedef struct bf_dev_scene {
     uint32_t has_environment;
     uint32_t environment_spectrum;
     uint32_t material_count;
+    const bf_dev_texture* textures;
+    uint32_t texture_count;
+    uint32_t has_textures;             /* some closure binds a texture */
 } bf_dev_scene;
 
 /* Path-state columns (one entry per path slot). */
@@ -120,6 +128,7 @@ typedef struct bf_dev_paths {
     float4* shadow_contribution;
     uint32_t* status;        /* termination | failure code << 8 */
     uint32_t* visibility;    /* ray visibility mask */
+    uint16_t* job;           /* job (frame) of the slot in a batched launch */
 } bf_dev_paths;
 
 #endif

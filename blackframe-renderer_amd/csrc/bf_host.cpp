@@ -667,6 +667,37 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     if (d->has_environment && d->environment_spectrum >= d->spectrum_count)
         return set_error(BF_INVALID_ARGUMENT, "environment spectrum out of range");
     s->light_count = (uint32_t)lights.size();
+
+    /* -- textures: albedo / roughness images bound to closures (no reference counterpart) -- */
+    if (d->texture_count > BF_MAX_TEXTURES || (d->texture_count && !d->textures))
+        return set_error(BF_INVALID_ARGUMENT, "at most %u textures, and a texture array when any", BF_MAX_TEXTURES);
+    for (uint32_t t = 0; t < d->texture_count; ++t) {
+        const bf_texture_desc& T = d->textures[t];
+        if (T.width == 0 || T.height == 0 || T.channels == 0 || T.channels > 3 ||
+            (uint64_t)T.width * T.height * T.channels > 0xFFFFFFFFull)
+            return set_error(BF_INVALID_ARGUMENT, "texture %u has an invalid size", t);
+        if ((T.data == nullptr) == (T.device_data == nullptr))
+            return set_error(BF_INVALID_ARGUMENT, "texture %u needs exactly one of host data and device data", t);
+    }
+    bool textured = false;
+    for (uint32_t c = 0; c < d->closure_count; ++c) {
+        const bf_closure_desc& C = d->closures[c];
+        if (C.weight_texture) {
+            if (C.weight_texture > d->texture_count) return set_error(BF_INVALID_ARGUMENT, "closure %u: no texture %u", c, C.weight_texture - 1);
+            const uint32_t ch = d->textures[C.weight_texture - 1].channels;
+            if (ch != 1 && ch != 3) return set_error(BF_INVALID_ARGUMENT, "closure %u: weight textures have 1 or 3 channels", c);
+            textured = true;
+        }
+        if (C.roughness_texture) {
+            if (C.roughness_texture > d->texture_count)
+                return set_error(BF_INVALID_ARGUMENT, "closure %u: no texture %u", c, C.roughness_texture - 1);
+            if (C.kind != BF_CLOSURE_ROUGH_CONDUCTOR && C.kind != BF_CLOSURE_ROUGH_DIELECTRIC)
+                return set_error(BF_INVALID_ARGUMENT, "closure %u: roughness textures drive GGX widths (rough conductors and dielectrics)", c);
+            const uint32_t ch = d->textures[C.roughness_texture - 1].channels;
+            if (ch != 1 && ch != 2) return set_error(BF_INVALID_ARGUMENT, "closure %u: roughness textures have 1 or 2 channels", c);
+            textured = true;
+        }
+    }
     if (!upload) return BF_OK;
     /* -- device upload: one allocation, 256-byte aligned parts -- */
     Arena A;
@@ -691,13 +722,24 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     const size_t o_cum = A.add(cumul.data(), cumul.size() * sizeof(float));
     const size_t o_nex = A.add(nexts.data(), nexts.size() * sizeof(float));
     const size_t o_lno = A.add(normals4.data(), normals4.size() * sizeof(float));
+    const size_t o_tex = A.add(nullptr, d->texture_count * sizeof(bf_dev_texture));
+    size_t o_texels[BF_MAX_TEXTURES] = {};
+    for (uint32_t t = 0; t < d->texture_count; ++t) {
+        const bf_texture_desc& T = d->textures[t];
+        if (T.data) o_texels[t] = A.add(T.data, (size_t)T.width * T.height * T.channels * sizeof(float));
+    }
     HIP_TRY(hipGetDevice(&s->device));
     HIP_TRY(hipMalloc(&s->device_block, A.total ? A.total : 256));
+    uint8_t* base = (uint8_t*)s->device_block;
     std::vector<uint8_t> staging(A.total, 0);
     for (size_t k = 0; k < A.parts.size(); ++k)
         if (A.sizes[k] && A.parts[k].second) std::memcpy(staging.data() + A.parts[k].first, A.parts[k].second, A.sizes[k]);
+    for (uint32_t t = 0; t < d->texture_count; ++t) {
+        const bf_texture_desc& T = d->textures[t];
+        const bf_dev_texture D{T.data ? (const float*)(base + o_texels[t]) : T.device_data, T.width, T.height, T.channels, 0u};
+        std::memcpy(staging.data() + o_tex + t * sizeof(bf_dev_texture), &D, sizeof D);
+    }
     HIP_TRY(hipMemcpy(s->device_block, staging.data(), A.total, hipMemcpyHostToDevice));
-    uint8_t* base = (uint8_t*)s->device_block;
     bf_dev_scene& S = s->dev;
     S.tlas_nodes = (const bf_wide_node*)(base + o_tlas);
     S.blas_nodes = (const bf_wide_node*)(base + o_blasw);
@@ -729,29 +771,49 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     S.has_environment = d->has_environment ? 1u : 0u;
     S.environment_spectrum = d->environment_spectrum;
     S.material_count = d->material_count;
+    S.textures = (const bf_dev_texture*)(base + o_tex);
+    S.texture_count = d->texture_count;
+    S.has_textures = textured ? 1u : 0u;
     return BF_OK;
 }
 
 } /* namespace */
 
 /* ---- workspace --------------------------------------------------------------------------------- */
+#define BF_MAX_BATCH_JOBS 4096u /* frames per batched launch (job ids are 16-bit) */
+#define BF_HISTORY_BOUNCES 256u  /* per-bounce queue sizes kept on the device per launch */
+
 struct bf_hip_workspace {
     uint64_t capacity = 0;
     void* block = nullptr;
     bf_dev_paths paths{};
     uint32_t* queues[4] = {nullptr, nullptr, nullptr, nullptr}; /* ray, hit, shadow, continuation */
     uint8_t* route = nullptr;
-    uint32_t* block_counts = nullptr;
-    uint32_t* block_offsets = nullptr;
-    uint32_t* sizes = nullptr;     /* [0] ray [1] hit [2] shadow [3] cont [4..7] scratch */
-    uint32_t* failure = nullptr;   /* [0] count, [2..3] lowest (global slot << 8 | failure code) */
+    uint32_t* tile_counts = nullptr;
+    uint32_t* tile_offsets = nullptr;
+    uint32_t* sizes = nullptr;     /* [0] ray [1] hit [2] shadow [3] cont [4..5] scratch [7] bounce */
+    uint32_t* history = nullptr;   /* [bounce][ray, hit, shadow, cont] */
+    uint32_t* failure = nullptr;   /* [0] count, [2..3] lowest (job << 48 | path << 8 | failure code) */
     uint32_t* sample_counts = nullptr;
     unsigned long long* traversal = nullptr; /* 6 counters */
-    unsigned long long* hashes = nullptr;    /* 2 words */
+    unsigned long long* hashes = nullptr;    /* [bounce][ray, shadow] FNV-1a-64 of the queues */
     uint32_t* host_words = nullptr;          /* pinned readback */
+    uint32_t* host_history = nullptr;        /* pinned readback of the history */
+    unsigned long long* host_hashes = nullptr;
+    bf_dev_job* jobs = nullptr;              /* device job table */
+    bf_dev_job* host_jobs = nullptr;         /* pinned staging of the job table */
     uint2* spill = nullptr;                  /* traversal stack spill, BF_STACK_DEPTH x spill_lanes */
     uint32_t spill_lanes = 0;
+    uint32_t shade_blocks = 0;
     int device = 0;
+    /* a captured group of bounces, replayed while paths remain (keyed by what its arguments hold) */
+    hipGraphExec_t graph = nullptr;
+    bf_transport_params graph_params{};
+    uint32_t graph_n = 0, graph_group = 0;
+    const uint32_t* graph_ray_q = nullptr;
+    /* bounce groups are captured on this stream (the caller's may be the legacy null stream,
+     * which cannot capture) and replayed on the caller's */
+    hipStream_t capture_stream = nullptr;
     /* stage instrumentation: a reusable pool of hipEvent pairs, drained at every host sync */
     std::vector<hipEvent_t> events;
     std::vector<std::pair<uint32_t, uint32_t>> pending; /* (stage, first event of the pair) */
@@ -905,37 +967,45 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     if (!w) return set_error(BF_RESOURCE_EXHAUSTED, "host allocation failed");
     w->capacity = capacity;
     const uint64_t n = capacity;
-    const uint32_t blocks = bfk_compact_blocks((uint32_t)n);
+    const uint32_t tiles = bfk_compact_tiles((uint32_t)n);
     hipError_t de = hipGetDevice(&w->device);
-    const uint32_t resident = de == hipSuccess ? bfk_trace_resident_blocks(w->device) : 0u;
-    if (resident == 0) {
+    const uint32_t resident = de == hipSuccess ? bfk_trace_resident_lanes(w->device) : 0u;
+    w->shade_blocks = de == hipSuccess ? bfk_shade_resident_blocks(w->device) : 0u;
+    if (resident == 0 || w->shade_blocks == 0) {
         delete w;
-        return set_error(BF_DEVICE_ERROR, "cannot query the traversal kernels' occupancy on the current device");
+        return set_error(BF_DEVICE_ERROR, "cannot query the kernels' occupancy on the current device");
     }
-    w->spill_lanes = resident * BF_TRACE_BLOCK;
+    w->spill_lanes = resident;
     Arena A;
     const size_t o_ro = A.add(nullptr, n * 16), o_rd = A.add(nullptr, n * 16), o_beta = A.add(nullptr, n * 16),
                  o_rad = A.add(nullptr, n * 16), o_wl = A.add(nullptr, n * 16), o_dep = A.add(nullptr, n * 16),
                  o_prev = A.add(nullptr, n * 16), o_hit = A.add(nullptr, n * 16), o_ids = A.add(nullptr, n * 8),
                  o_so = A.add(nullptr, n * 16), o_sd = A.add(nullptr, n * 16), o_sc = A.add(nullptr, n * 16),
-                 o_st = A.add(nullptr, n * 4), o_vis = A.add(nullptr, n * 4);
+                 o_st = A.add(nullptr, n * 4), o_vis = A.add(nullptr, n * 4), o_job = A.add(nullptr, n * 2);
     size_t o_q[4];
     for (int k = 0; k < 4; ++k) o_q[k] = A.add(nullptr, n * 4);
-    const size_t o_route = A.add(nullptr, n), o_bc = A.add(nullptr, (size_t)blocks * 8), o_bo = A.add(nullptr, (size_t)blocks * 8),
-                 o_sizes = A.add(nullptr, 64), o_fail = A.add(nullptr, 16), o_samp = A.add(nullptr, 16),
-                 o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, 16),
+    const size_t o_route = A.add(nullptr, n), o_tc = A.add(nullptr, (size_t)tiles * 8), o_to = A.add(nullptr, (size_t)tiles * 8),
+                 o_sizes = A.add(nullptr, 64), o_hist = A.add(nullptr, BF_HISTORY_BOUNCES * 16), o_fail = A.add(nullptr, 16),
+                 o_samp = A.add(nullptr, 16), o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, BF_REPORT_MAX_BOUNCES * 16),
+                 o_jobs = A.add(nullptr, BF_MAX_BATCH_JOBS * sizeof(bf_dev_job)),
                  o_spill = A.add(nullptr, (size_t)w->spill_lanes * BF_STACK_DEPTH * sizeof(uint2));
     hipError_t e = hipMalloc(&w->block, A.total);
     if (e != hipSuccess) {
         delete w;
         return set_error(BF_RESOURCE_EXHAUSTED, "workspace allocation of %zu bytes failed: %s", A.total, hipGetErrorString(e));
     }
-    e = hipHostMalloc((void**)&w->host_words, 64, hipHostMallocDefault);
+    const size_t host_bytes = 64 + BF_HISTORY_BOUNCES * 16 + BF_REPORT_MAX_BOUNCES * 16 + BF_MAX_BATCH_JOBS * sizeof(bf_dev_job);
+    void* host = nullptr;
+    e = hipHostMalloc(&host, host_bytes, hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipFree(w->block);
         delete w;
         return hip_error(e, "hipHostMalloc");
     }
+    w->host_words = (uint32_t*)host;
+    w->host_history = (uint32_t*)((uint8_t*)host + 64);
+    w->host_hashes = (unsigned long long*)((uint8_t*)host + 64 + BF_HISTORY_BOUNCES * 16);
+    w->host_jobs = (bf_dev_job*)((uint8_t*)host + 64 + BF_HISTORY_BOUNCES * 16 + BF_REPORT_MAX_BOUNCES * 16);
     uint8_t* b = (uint8_t*)w->block;
     w->paths.ray_origin = (float4*)(b + o_ro);
     w->paths.ray_direction = (float4*)(b + o_rd);
@@ -951,15 +1021,18 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     w->paths.shadow_contribution = (float4*)(b + o_sc);
     w->paths.status = (uint32_t*)(b + o_st);
     w->paths.visibility = (uint32_t*)(b + o_vis);
+    w->paths.job = (uint16_t*)(b + o_job);
     for (int k = 0; k < 4; ++k) w->queues[k] = (uint32_t*)(b + o_q[k]);
     w->route = b + o_route;
-    w->block_counts = (uint32_t*)(b + o_bc);
-    w->block_offsets = (uint32_t*)(b + o_bo);
+    w->tile_counts = (uint32_t*)(b + o_tc);
+    w->tile_offsets = (uint32_t*)(b + o_to);
     w->sizes = (uint32_t*)(b + o_sizes);
+    w->history = (uint32_t*)(b + o_hist);
     w->failure = (uint32_t*)(b + o_fail);
     w->sample_counts = (uint32_t*)(b + o_samp);
     w->traversal = (unsigned long long*)(b + o_trav);
     w->hashes = (unsigned long long*)(b + o_hash);
+    w->jobs = (bf_dev_job*)(b + o_jobs);
     w->spill = (uint2*)(b + o_spill);
     *out = w;
     return BF_OK;
@@ -968,6 +1041,8 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
 int blackframe_hip_workspace_destroy(bf_hip_workspace* w) {
     if (!w) return BF_OK;
     for (hipEvent_t e : w->events) (void)hipEventDestroy(e);
+    if (w->graph) (void)hipGraphExecDestroy(w->graph);
+    if (w->capture_stream) (void)hipStreamDestroy(w->capture_stream);
     hipError_t e1 = w->block ? hipFree(w->block) : hipSuccess;
     hipError_t e2 = w->host_words ? hipHostFree(w->host_words) : hipSuccess;
     delete w;
@@ -976,19 +1051,17 @@ int blackframe_hip_workspace_destroy(bf_hip_workspace* w) {
     return BF_OK;
 }
 
-int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_scene* s, const bf_frame_desc* frame,
-                                             const bf_transport_options* o, float* film, double* film_compensated,
-                                             uint32_t* film_counts, bf_path_result* paths, bf_transport_report* report,
-                                             uint64_t* traversal_counters, void* stream_handle) {
-    if (!w || !s || !frame || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, scene, frame or options");
-    if (!s->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
-    const hipStream_t stream = (hipStream_t)stream_handle;
-    const bf_camera_desc& cam = frame->camera;
-    const uint64_t pixels = (uint64_t)cam.width * cam.height;
-    if (pixels == 0 || frame->samples_per_pixel == 0) return set_error(BF_INVALID_ARGUMENT, "empty frame");
-    if (pixels > w->capacity)
-        return set_error(BF_RESOURCE_EXHAUSTED, "workspace capacity %llu is below one sample pass (%llu paths)",
-                         (unsigned long long)w->capacity, (unsigned long long)pixels);
+} /* extern "C" */
+
+namespace {
+
+struct chunk_job {
+    uint32_t job;        /* index into the caller's jobs */
+    uint32_t first_pass; /* first sample pass of the frame in this launch */
+    uint32_t passes;
+};
+
+int validate_options(const bf_transport_options* o) {
     if (o->sampler > BF_SAMPLER_SOBOL_OWEN || o->wavelength_mode > BF_WAVELENGTHS_PER_PATH || o->mis_heuristic > BF_MIS_POWER)
         return set_error(BF_INVALID_ARGUMENT, "unknown sampler, wavelength mode or MIS heuristic");
     if (o->volume_depth_limit != 0) return set_error(BF_UNAVAILABLE, "participating media are not supported");
@@ -999,15 +1072,100 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     if (o->diffuse_depth_limit > 255 || o->glossy_depth_limit > 255 || o->specular_depth_limit > 255 ||
         o->transmission_depth_limit > 255)
         return set_error(BF_INVALID_ARGUMENT, "depth limits above 255 are not supported");
-    if (!film || !film_compensated || !film_counts) return set_error(BF_INVALID_ARGUMENT, "film buffers are required");
+    if (o->instrumentation > BF_INSTRUMENT_STAGE_EVENTS) return set_error(BF_INVALID_ARGUMENT, "unknown instrumentation mode");
+    if (o->failure_policy > BF_FAILURES_COUNT) return set_error(BF_INVALID_ARGUMENT, "unknown failure policy");
+    const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
+    if (block_log2 != 0 && (block_log2 < 6 || block_log2 > 8))
+        return set_error(BF_INVALID_ARGUMENT, "traversal workgroups of 2^%u lanes are not available (64, 128 or 256)", block_log2);
+    if ((o->launch & ~(BF_LAUNCH_GRAPH | BF_LAUNCH_TRACE_BLOCK_MASK | BF_LAUNCH_GROUP_MASK)) != 0)
+        return set_error(BF_INVALID_ARGUMENT, "unknown launch bits 0x%08x", o->launch);
+    return BF_OK;
+}
+
+/* Issues one bounce of the loop on `stream` (no host synchronisation). The adjoint replay stops
+ * after the shade stage: its light samples are resolved from the tape, so no shadow ray exists. */
+int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transport_params& P, bool batch, bool instrument,
+                 uint32_t n_max, uint32_t* ray_q, uint32_t* cont_q, unsigned long long* counters,
+                 unsigned long long* hash_slot, hipStream_t stream) {
+    const bool adjoint = P.adjoint != 0u;
+#define STAGE(stage, dispatches, call) HIP_TRY(staged(w, rep, instrument, (stage), (dispatches), stream, [&] { return (call); }))
+    if (hash_slot) STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(ray_q, &w->sizes[0], hash_slot, stream));
+    STAGE(BF_STAGE_CLOSEST_HIT, 1, bfk_trace_closest(&P, ray_q, &w->sizes[0], n_max, w->route, counters, batch, stream));
+    STAGE(BF_STAGE_COMPACTION, 3,
+          bfk_compact(w->route, &w->sizes[0], n_max, ray_q, w->tile_counts, w->tile_offsets, w->queues[1], &w->sizes[1], 0u,
+                      w->queues[2], &w->sizes[4], 0u, stream));
+    STAGE(BF_STAGE_SHADE, 1,
+          bfk_shade(&P, w->queues[1], &w->sizes[1], n_max, w->route, w->sample_counts, batch, adjoint, stream));
+    STAGE(BF_STAGE_COMPACTION, 3,
+          bfk_compact(w->route, &w->sizes[1], n_max, w->queues[1], w->tile_counts, w->tile_offsets, w->queues[2],
+                      &w->sizes[2], 0u, cont_q, &w->sizes[3], 0u, stream));
+    if (adjoint) {
+        STAGE(BF_STAGE_AUXILIARY, 1, bfk_bounce_end(w->sizes, w->history, BF_HISTORY_BOUNCES, stream));
+        return BF_OK;
+    }
+    if (hash_slot) STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(w->queues[2], &w->sizes[2], hash_slot + 1, stream));
+    STAGE(BF_STAGE_SHADOW, 1, bfk_trace_shadow(&P, w->queues[2], &w->sizes[2], n_max, w->route, counters, batch, stream));
+    STAGE(BF_STAGE_COMPACTION, 3,
+          bfk_compact(w->route, &w->sizes[2], n_max, w->queues[2], w->tile_counts, w->tile_offsets, w->queues[1],
+                      &w->sizes[5], 0u, cont_q, &w->sizes[3], 1u, stream));
+    STAGE(BF_STAGE_AUXILIARY, 1, bfk_bounce_end(w->sizes, w->history, BF_HISTORY_BOUNCES, stream));
+#undef STAGE
+    return BF_OK;
+}
+
+/*
+ * The transport driver shared by the single-frame and the batched entry points: frames are packed
+ * into launches of whole sample passes (one contiguous slot range per frame and launch), and each
+ * launch runs camera -> bounce groups (live-path count checked between groups) -> film.
+ */
+int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_job* adjoints, uint32_t job_count,
+               const bf_transport_options* o, bf_transport_report* report, uint64_t* traversal_counters,
+               hipStream_t stream) {
+    if (!w || !jobs || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, jobs or options");
+    if (job_count == 0) return set_error(BF_INVALID_ARGUMENT, "empty batch");
+    const int ost = validate_options(o);
+    if (ost != BF_OK) return ost;
+    const bool adjoint = adjoints != nullptr;
+    /* tape bits are indexed by path vertex: at most diffuse + glossy + specular + 1 vertices */
+    const uint64_t vertices = (uint64_t)o->diffuse_depth_limit + o->glossy_depth_limit + o->specular_depth_limit + 1u;
     int device = 0;
     HIP_TRY(hipGetDevice(&device));
-    if (device != s->device) return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", s->device, device);
     if (device != w->device)
         return set_error(BF_INCOMPATIBLE, "workspace lives on device %d, current device is %d", w->device, device);
+    for (uint32_t j = 0; j < job_count; ++j) {
+        const bf_frame_job& J = jobs[j];
+        if (!J.scene) return set_error(BF_INVALID_ARGUMENT, "job %u has no scene", j);
+        if (!J.scene->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
+        if (J.scene->device != device)
+            return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", J.scene->device, device);
+        const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
+        if (pixels == 0 || J.frame.samples_per_pixel == 0) return set_error(BF_INVALID_ARGUMENT, "job %u: empty frame", j);
+        if (pixels > w->capacity)
+            return set_error(BF_RESOURCE_EXHAUSTED, "workspace capacity %llu is below one sample pass (%llu paths)",
+                             (unsigned long long)w->capacity, (unsigned long long)pixels);
+        if (!adjoint && (!J.film || !J.film_compensated || !J.film_counts))
+            return set_error(BF_INVALID_ARGUMENT, "job %u: film buffers are required", j);
+        if ((J.tape_radiance == nullptr) != (J.tape_visibility == nullptr))
+            return set_error(BF_INVALID_ARGUMENT, "job %u: a tape needs both its radiance and visibility buffers", j);
+        if ((J.tape_radiance || adjoint) && vertices > 64u)
+            return set_error(BF_INVALID_ARGUMENT, "taped frames need diffuse + glossy + specular depth limits <= 63");
+        if (adjoint) {
+            if (!J.tape_radiance) return set_error(BF_INVALID_ARGUMENT, "job %u: the adjoint needs the frame's tape", j);
+            if (!adjoints[j].film_gradient) return set_error(BF_INVALID_ARGUMENT, "job %u: no film gradient", j);
+            for (uint32_t k = J.scene->dev.texture_count; k < BF_MAX_TEXTURES; ++k)
+                if (adjoints[j].texture_gradients[k])
+                    return set_error(BF_INVALID_ARGUMENT, "job %u: gradient for texture %u, scene has %u", j, k,
+                                     J.scene->dev.texture_count);
+        }
+    }
+
+    const bool instrument = o->instrumentation == BF_INSTRUMENT_STAGE_EVENTS;
+    uint32_t group = o->launch & BF_LAUNCH_GROUP_MASK;
+    if (group == 0) group = 4;
+    const bool use_graph = (o->launch & BF_LAUNCH_GRAPH) != 0 && !instrument && !o->record_queues && !traversal_counters;
+    if (use_graph) group += group & 1u; /* even: the ray / continuation queue roles repeat */
 
     bf_transport_params P{};
-    P.scene = s->dev;
     P.paths = w->paths;
     const uint64_t* directions = nullptr;
     const int sst = sobol_table_for_current_device(&directions);
@@ -1015,10 +1173,6 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     P.sampler.kind = o->sampler;
     P.sampler.sobol_dims = kSobolDimensions;
     P.sampler.directions = directions;
-    P.camera = cam;
-    P.seed = frame->seed;
-    P.first_sample_index = frame->first_sample_index;
-    P.pixels = (uint32_t)pixels;
     P.mis = o->mis_heuristic;
     P.wavelength_mode = o->wavelength_mode;
     P.limits[0] = o->diffuse_depth_limit;
@@ -1034,12 +1188,16 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     P.failure = w->failure;
     P.spill = w->spill;
     P.spill_stride = w->spill_lanes;
+    P.shade_blocks = w->shade_blocks;
+    const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
+    P.trace_block = block_log2 == 0 ? 128u : 1u << block_log2;
+    P.jobs = w->jobs;
+    P.film_pdf = 1.0f / 470.0f;
+    P.film_normalization = film_normalization();
+    P.adjoint = adjoint ? 1u : 0u;
 
     bf_transport_report rep{};
     rep.first_failure_slot = BF_INVALID_INDEX;
-    if (o->instrumentation > BF_INSTRUMENT_STAGE_EVENTS) return set_error(BF_INVALID_ARGUMENT, "unknown instrumentation mode");
-    if (o->failure_policy > BF_FAILURES_COUNT) return set_error(BF_INVALID_ARGUMENT, "unknown failure policy");
-    const bool instrument = o->instrumentation == BF_INSTRUMENT_STAGE_EVENTS;
     w->pending.clear();
     w->events_used = 0;
 #define STAGE(stage, dispatches, call) HIP_TRY(staged(w, rep, instrument, (stage), (dispatches), stream, [&] { return (call); }))
@@ -1051,82 +1209,193 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
         return e;
     }());
     unsigned long long* counters = traversal_counters ? w->traversal : nullptr;
-    const uint64_t passes_per_launch = w->capacity / pixels;
     const uint64_t bound = (uint64_t)o->diffuse_depth_limit + o->glossy_depth_limit + o->specular_depth_limit + 2u;
     const float normalization = film_normalization();
-    for (uint64_t pass0 = 0; pass0 < frame->samples_per_pixel; pass0 += passes_per_launch) {
-        const uint64_t passes = std::min<uint64_t>(passes_per_launch, frame->samples_per_pixel - pass0);
-        const uint32_t n = (uint32_t)(passes * pixels);
-        P.slot_base = pass0 * pixels;
+
+    std::vector<uint32_t> next_pass(job_count, 0u);
+    uint32_t cursor = 0;
+    bool first_launch = true;
+    for (;;) {
+        /* pack whole sample passes of the remaining frames, in job order */
+        std::vector<chunk_job> chunk;
+        uint64_t used = 0;
+        for (uint32_t j = cursor; j < job_count && chunk.size() < BF_MAX_BATCH_JOBS; ++j) {
+            const bf_frame_job& J = jobs[j];
+            const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
+            const uint32_t remaining = J.frame.samples_per_pixel - next_pass[j];
+            if (remaining == 0) continue;
+            const uint64_t fit = (w->capacity - used) / pixels;
+            if (fit == 0) break;
+            const uint32_t take = (uint32_t)std::min<uint64_t>(remaining, fit);
+            chunk.push_back({j, next_pass[j], take});
+            used += take * pixels;
+            if (take < remaining) break;
+        }
+        if (chunk.empty()) break;
+        const bool batch = chunk.size() > 1 || use_graph;
+        uint32_t slot = 0;
+        for (size_t c = 0; c < chunk.size(); ++c) {
+            const bf_frame_job& J = jobs[chunk[c].job];
+            bf_dev_job& D = w->host_jobs[c];
+            D = bf_dev_job{};
+            D.scene = J.scene->dev;
+            D.camera = J.frame.camera;
+            D.seed = J.frame.seed;
+            D.pixels = J.frame.camera.width * J.frame.camera.height;
+            D.first_sample_index = J.frame.first_sample_index;
+            D.slot_begin = slot;
+            D.slot_count = chunk[c].passes * D.pixels;
+            D.path_base = (uint64_t)chunk[c].first_pass * D.pixels;
+            D.index = chunk[c].job;
+            D.film = J.film;
+            D.film_compensated = J.film_compensated;
+            D.film_counts = J.film_counts;
+            D.paths = J.paths;
+            D.tape_radiance = (float4*)J.tape_radiance;
+            D.tape_visibility = (unsigned long long*)J.tape_visibility;
+            if (adjoint) {
+                const bf_adjoint_job& A = adjoints[chunk[c].job];
+                D.film = nullptr;
+                D.film_compensated = nullptr;
+                D.film_counts = nullptr;
+                D.paths = nullptr;
+                D.film_gradient = A.film_gradient;
+                for (uint32_t k = 0; k < BF_MAX_TEXTURES; ++k) D.texture_gradients[k] = A.texture_gradients[k];
+            }
+            slot += D.slot_count;
+            next_pass[chunk[c].job] += chunk[c].passes;
+        }
+        while (cursor < job_count && next_pass[cursor] == jobs[cursor].frame.samples_per_pixel) ++cursor;
+        const uint32_t n = slot;
+        P.job_count = (uint32_t)chunk.size();
+        if (batch) {
+            HIP_TRY(hipMemcpyAsync(w->jobs, w->host_jobs, chunk.size() * sizeof(bf_dev_job), hipMemcpyHostToDevice, stream));
+        } else {
+            P.job0 = w->host_jobs[0];
+        }
         uint32_t* ray_q = w->queues[0];
         uint32_t* cont_q = w->queues[3];
-        STAGE(BF_STAGE_CAMERA, 1, bfk_camera(&P, n, ray_q, stream));
+        STAGE(BF_STAGE_CAMERA, 1, bfk_camera(&P, n, ray_q, batch, adjoint, stream));
         rep.stage_metrics[BF_STAGE_CAMERA].kernel_lanes += n;
-        STAGE(BF_STAGE_AUXILIARY, 1, bfk_set_words(w->sizes, 1, n, stream));
-        uint32_t n_ray = n;
+        STAGE(BF_STAGE_AUXILIARY, 2, [&] {
+            hipError_t e = bfk_set_words(w->sizes, 1, n, stream);
+            if (e == hipSuccess) e = bfk_set_words(w->sizes + 1, 7, 0u, stream);
+            return e;
+        }());
+        const bool record = o->record_queues && first_launch;
         uint32_t bounce = 0;
-        while (n_ray != 0 && bounce <= bound) {
-            const bool record = o->record_queues && pass0 == 0 && bounce < BF_REPORT_MAX_BOUNCES;
-            if (record) {
-                STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(ray_q, &w->sizes[0], w->hashes, stream));
-                HIP_TRY(hipMemcpyAsync(&rep.ray_queue_hash[bounce], w->hashes, 8, hipMemcpyDeviceToHost, stream));
-                rep.ray_queue_size[bounce] = n_ray;
+        bool live = true;
+        while (live && bounce <= bound) {
+            const uint32_t steps = (uint32_t)std::min<uint64_t>(group, bound + 1 - bounce);
+            if (use_graph && steps == group) {
+                /* a group is replayed while its launch arguments are unchanged: the parameter block (the
+                 * frames themselves are in the device job table), the item bound and the queue roles */
+                const bool same = w->graph != nullptr && w->graph_n == n && w->graph_group == group &&
+                                  w->graph_ray_q == ray_q && std::memcmp(&w->graph_params, &P, sizeof P) == 0;
+                if (!same) {
+                    if (w->graph) {
+                        (void)hipGraphExecDestroy(w->graph);
+                        w->graph = nullptr;
+                    }
+                    bf_transport_report unused{};
+                    if (!w->capture_stream) HIP_TRY(hipStreamCreateWithFlags(&w->capture_stream, hipStreamNonBlocking));
+                    HIP_TRY(hipStreamBeginCapture(w->capture_stream, hipStreamCaptureModeThreadLocal));
+                    int ist = BF_OK;
+                    uint32_t* rq = ray_q;
+                    uint32_t* cq = cont_q;
+                    for (uint32_t k = 0; k < group && ist == BF_OK; ++k) {
+                        ist = issue_bounce(w, unused, P, batch, false, n, rq, cq, nullptr, nullptr, w->capture_stream);
+                        std::swap(rq, cq);
+                    }
+                    hipGraph_t g = nullptr;
+                    const hipError_t ce = hipStreamEndCapture(w->capture_stream, &g);
+                    if (ist != BF_OK) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return ist;
+                    }
+                    if (ce != hipSuccess) return hip_error(ce, "hipStreamEndCapture");
+                    const hipError_t ie = hipGraphInstantiate(&w->graph, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (ie != hipSuccess) {
+                        w->graph = nullptr;
+                        return hip_error(ie, "hipGraphInstantiate");
+                    }
+                    w->graph_params = P;
+                    w->graph_n = n;
+                    w->graph_group = group;
+                    w->graph_ray_q = ray_q;
+                }
+                HIP_TRY(hipGraphLaunch(w->graph, stream));
+                rep.stage_metrics[BF_STAGE_CLOSEST_HIT].kernel_dispatches += group;
+                rep.stage_metrics[BF_STAGE_SHADE].kernel_dispatches += group;
+                if (!adjoint) rep.stage_metrics[BF_STAGE_SHADOW].kernel_dispatches += group;
+                rep.stage_metrics[BF_STAGE_COMPACTION].kernel_dispatches += (adjoint ? 6u : 9u) * group;
+                rep.stage_metrics[BF_STAGE_AUXILIARY].kernel_dispatches += group;
+            } else {
+                for (uint32_t k = 0; k < steps; ++k) {
+                    const bool rec = record && bounce + k < BF_REPORT_MAX_BOUNCES;
+                    const int ist = issue_bounce(w, rep, P, batch, instrument, n, ray_q, cont_q, counters,
+                                                 rec ? w->hashes + 2 * (bounce + k) : nullptr, stream);
+                    if (ist != BF_OK) return ist;
+                    std::swap(ray_q, cont_q);
+                }
             }
-            STAGE(BF_STAGE_CLOSEST_HIT, 1, bfk_trace_closest(&P, ray_q, &w->sizes[0], n_ray, w->route, counters, stream));
-            STAGE(BF_STAGE_COMPACTION, 3,
-                  bfk_compact(w->route, &w->sizes[0], n_ray, ray_q, w->block_counts, w->block_offsets, w->queues[1],
-                              &w->sizes[1], 0u, w->queues[2], &w->sizes[4], 0u, stream));
-            STAGE(BF_STAGE_SHADE, 1, bfk_shade(&P, w->queues[1], &w->sizes[1], n_ray, w->route, w->sample_counts, stream));
-            STAGE(BF_STAGE_COMPACTION, 3,
-                  bfk_compact(w->route, &w->sizes[1], n_ray, w->queues[1], w->block_counts, w->block_offsets, w->queues[2],
-                              &w->sizes[2], 0u, cont_q, &w->sizes[3], 0u, stream));
-            if (record) {
-                STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(w->queues[2], &w->sizes[2], w->hashes + 1, stream));
-                HIP_TRY(hipMemcpyAsync(&rep.shadow_queue_hash[bounce], w->hashes + 1, 8, hipMemcpyDeviceToHost, stream));
-            }
-            STAGE(BF_STAGE_SHADOW, 1, bfk_trace_shadow(&P, w->queues[2], &w->sizes[2], n_ray, w->route, counters, stream));
-            STAGE(BF_STAGE_COMPACTION, 3,
-                  bfk_compact(w->route, &w->sizes[2], n_ray, w->queues[2], w->block_counts, w->block_offsets, w->queues[1],
-                              &w->sizes[5], 0u, cont_q, &w->sizes[3], 1u, stream));
-            HIP_TRY(hipMemcpyAsync(w->host_words, w->sizes, 16, hipMemcpyDeviceToHost, stream));
+            bounce += steps;
+            HIP_TRY(hipMemcpyAsync(w->host_words, w->sizes, 4, hipMemcpyDeviceToHost, stream));
             HIP_TRY(hipStreamSynchronize(stream));
             HIP_TRY(drain_stage_events(w, rep));
-            rep.closest_hit_queries += n_ray;
-            rep.shade_lanes += w->host_words[1];
-            rep.miss_lanes += n_ray - w->host_words[1];
-            rep.shadow_queries += w->host_words[2];
-            rep.stage_metrics[BF_STAGE_CLOSEST_HIT].kernel_lanes += n_ray;
-            rep.stage_metrics[BF_STAGE_SHADE].kernel_lanes += w->host_words[1];
-            rep.stage_metrics[BF_STAGE_SHADOW].kernel_lanes += w->host_words[2];
-            rep.stage_metrics[BF_STAGE_COMPACTION].kernel_lanes += (uint64_t)n_ray + w->host_words[1] + w->host_words[2];
-            if (record) rep.shadow_queue_size[bounce] = w->host_words[2];
-            const uint32_t next = w->host_words[3];
-            std::swap(w->queues[0], w->queues[3]);
-            ray_q = w->queues[0];
-            cont_q = w->queues[3];
-            STAGE(BF_STAGE_AUXILIARY, 1, bfk_copy_word(&w->sizes[3], &w->sizes[0], stream));
-            n_ray = next;
-            ++bounce;
+            live = w->host_words[0] != 0u;
         }
-        rep.bounces = std::max<uint32_t>(rep.bounces, bounce);
-        STAGE(BF_STAGE_FILM, 1,
-              bfk_film(&w->paths, (uint32_t)pixels, (uint32_t)passes, 1.0f / 470.0f, normalization, film,
-                       film_compensated, film_counts, stream));
-        rep.stage_metrics[BF_STAGE_FILM].kernel_lanes += n;
-        if (paths) STAGE(BF_STAGE_AUXILIARY, 1, bfk_export_paths(&w->paths, n, paths + P.slot_base, stream));
+        /* per-bounce sizes of this launch */
+        const uint32_t recorded = std::min<uint32_t>(bounce, BF_HISTORY_BOUNCES);
+        HIP_TRY(hipMemcpyAsync(w->host_history, w->history, (size_t)recorded * 16, hipMemcpyDeviceToHost, stream));
+        if (record)
+            HIP_TRY(hipMemcpyAsync(w->host_hashes, w->hashes, (size_t)std::min<uint32_t>(recorded, BF_REPORT_MAX_BOUNCES) * 16,
+                                   hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        uint32_t active_bounces = 0;
+        for (uint32_t b = 0; b < recorded; ++b) {
+            const uint32_t* h = w->host_history + 4 * b;
+            if (h[0] == 0u) break;
+            active_bounces = b + 1;
+            rep.closest_hit_queries += h[0];
+            rep.shade_lanes += h[1];
+            rep.miss_lanes += h[0] - h[1];
+            rep.shadow_queries += h[2];
+            rep.stage_metrics[BF_STAGE_CLOSEST_HIT].kernel_lanes += h[0];
+            rep.stage_metrics[BF_STAGE_SHADE].kernel_lanes += h[1];
+            rep.stage_metrics[BF_STAGE_SHADOW].kernel_lanes += h[2];
+            rep.stage_metrics[BF_STAGE_COMPACTION].kernel_lanes += (uint64_t)h[0] + h[1] + h[2];
+            if (record && b < BF_REPORT_MAX_BOUNCES) {
+                rep.ray_queue_size[b] = h[0];
+                rep.shadow_queue_size[b] = h[2];
+                rep.ray_queue_hash[b] = w->host_hashes[2 * b];
+                rep.shadow_queue_hash[b] = w->host_hashes[2 * b + 1];
+            }
+        }
+        rep.bounces = std::max<uint32_t>(rep.bounces, active_bounces);
+        for (size_t c = 0; c < chunk.size() && !adjoint; ++c) {
+            const bf_dev_job& D = w->host_jobs[c];
+            STAGE(BF_STAGE_FILM, 1, bfk_film(&w->paths, &D, P.film_pdf, normalization, stream));
+            rep.stage_metrics[BF_STAGE_FILM].kernel_lanes += D.slot_count;
+            if (D.paths) STAGE(BF_STAGE_AUXILIARY, 1, bfk_export_paths(&w->paths, &D, stream));
+        }
+        first_launch = false;
     }
-#undef STAGE
-    rep.path_count = pixels * frame->samples_per_pixel;
+    for (uint32_t j = 0; j < job_count; ++j)
+        rep.path_count += (uint64_t)jobs[j].frame.camera.width * jobs[j].frame.camera.height * jobs[j].frame.samples_per_pixel;
     HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 16, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipMemcpyAsync(w->host_words + 4, w->sample_counts, 8, hipMemcpyDeviceToHost, stream));
     unsigned long long trav[6] = {0, 0, 0, 0, 0, 0};
     if (traversal_counters) HIP_TRY(hipMemcpyAsync(trav, w->traversal, sizeof trav, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     HIP_TRY(drain_stage_events(w, rep));
+#undef STAGE
     rep.failed_paths = w->host_words[0];
     if (rep.failed_paths != 0) {
         const uint64_t key = (uint64_t)w->host_words[2] | ((uint64_t)w->host_words[3] << 32);
-        rep.first_failure_slot = (uint32_t)(key >> 8);
+        rep.first_failure_job = (uint32_t)(key >> 48);
+        rep.first_failure_slot = (uint32_t)((key >> 8) & 0xFFFFFFFFFFull);
         rep.first_failure_code = (uint32_t)(key & 0xFFu);
     }
     rep.closure_samples = w->host_words[4];
@@ -1135,9 +1404,126 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
         for (int k = 0; k < 6; ++k) traversal_counters[k] = trav[k];
     if (report) *report = rep;
     if (rep.failed_paths != 0 && o->failure_policy == BF_FAILURES_ABORT)
-        return set_error(BF_TRANSPORT_FAILURE, "%llu paths failed; first failing slot %u",
-                         (unsigned long long)rep.failed_paths, rep.first_failure_slot);
+        return set_error(BF_TRANSPORT_FAILURE, "%llu paths failed; first failing path %u of frame %u",
+                         (unsigned long long)rep.failed_paths, rep.first_failure_slot, rep.first_failure_job);
     return BF_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_scene* s, const bf_frame_desc* frame,
+                                             const bf_transport_options* o, float* film, double* film_compensated,
+                                             uint32_t* film_counts, bf_path_result* paths, bf_transport_report* report,
+                                             uint64_t* traversal_counters, void* stream_handle) {
+    if (!w || !s || !frame || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, scene, frame or options");
+    bf_frame_job job{};
+    job.scene = s;
+    job.frame = *frame;
+    job.film = film;
+    job.film_compensated = film_compensated;
+    job.film_counts = film_counts;
+    job.paths = paths;
+    return trace_jobs(w, &job, nullptr, 1, o, report, traversal_counters, (hipStream_t)stream_handle);
+}
+
+int blackframe_hip_trace_wavefront_batch(bf_hip_workspace* w, const bf_frame_job* jobs, uint32_t job_count,
+                                         const bf_transport_options* o, bf_transport_report* report,
+                                         uint64_t* traversal_counters, void* stream_handle) {
+    if (job_count > 0xFFFFu) return set_error(BF_INVALID_ARGUMENT, "at most 65535 frames per batch");
+    return trace_jobs(w, jobs, nullptr, job_count, o, report, traversal_counters, (hipStream_t)stream_handle);
+}
+
+int blackframe_hip_autotune_launch(bf_hip_workspace* w, const bf_frame_job* jobs, uint32_t job_count,
+                                   const bf_transport_options* o, uint32_t trial_passes, bf_launch_tuning* result,
+                                   void* stream_handle) {
+    if (!w || !jobs || !o || !result || job_count == 0 || trial_passes == 0)
+        return set_error(BF_INVALID_ARGUMENT, "NULL or empty autotune arguments");
+    if (job_count > 0xFFFFu) return set_error(BF_INVALID_ARGUMENT, "at most 65535 frames per batch");
+    hipStream_t stream = (hipStream_t)stream_handle;
+    /* trial jobs: the first passes of each frame, rendered into scratch films */
+    std::vector<bf_frame_job> trial(jobs, jobs + job_count);
+    size_t film_floats = 0, film_doubles = 0, film_words = 0;
+    for (bf_frame_job& J : trial) {
+        const size_t pixels = (size_t)J.frame.camera.width * J.frame.camera.height;
+        if (J.frame.samples_per_pixel > trial_passes) J.frame.samples_per_pixel = trial_passes;
+        film_floats = std::max(film_floats, pixels * 3);
+        film_doubles = std::max(film_doubles, pixels * 6);
+        film_words = std::max(film_words, pixels);
+        J.paths = nullptr;
+        J.tape_radiance = nullptr;
+        J.tape_visibility = nullptr;
+    }
+    void* scratch = nullptr;
+    HIP_TRY(hipMalloc(&scratch, film_doubles * 8 + film_floats * 4 + film_words * 4));
+    HIP_TRY(hipMemset(scratch, 0, film_doubles * 8 + film_floats * 4 + film_words * 4));
+    for (bf_frame_job& J : trial) {
+        J.film_compensated = (double*)scratch;
+        J.film = (float*)((char*)scratch + film_doubles * 8);
+        J.film_counts = (uint32_t*)((char*)scratch + film_doubles * 8 + film_floats * 4);
+    }
+    hipEvent_t start = nullptr, stop = nullptr;
+    int status = BF_OK;
+    bf_launch_tuning out{};
+    auto run = [&](uint32_t launch, float& ms) -> int {
+        bf_transport_options opt = *o;
+        opt.launch = launch;
+        opt.instrumentation = BF_INSTRUMENT_DISABLED;
+        opt.record_queues = 0;
+        opt.failure_policy = BF_FAILURES_COUNT;
+        int st = trace_jobs(w, trial.data(), nullptr, job_count, &opt, nullptr, nullptr, stream); /* warm: capture */
+        if (st != BF_OK) return st;
+        HIP_TRY(hipEventRecord(start, stream));
+        st = trace_jobs(w, trial.data(), nullptr, job_count, &opt, nullptr, nullptr, stream);
+        if (st != BF_OK) return st;
+        HIP_TRY(hipEventRecord(stop, stream));
+        HIP_TRY(hipEventSynchronize(stop));
+        HIP_TRY(hipEventElapsedTime(&ms, start, stop));
+        return BF_OK;
+    };
+    const hipError_t e1 = hipEventCreate(&start), e2 = hipEventCreate(&stop);
+    if (e1 != hipSuccess || e2 != hipSuccess) {
+        status = hip_error(e1 != hipSuccess ? e1 : e2, "hipEventCreate");
+    } else {
+        const bool graphs = !o->record_queues && o->instrumentation != BF_INSTRUMENT_STAGE_EVENTS;
+        status = run(0u, out.default_ms);
+        out.best_ms = out.default_ms;
+        out.launch = 0u;
+        const uint32_t blocks[3] = {6u, 7u, 8u}, groups[3] = {2u, 4u, 8u};
+        for (uint32_t b = 0; b < 3 && status == BF_OK; ++b)
+            for (uint32_t g = 0; g < 3 && status == BF_OK; ++g)
+                for (uint32_t graph = 0; graph < (graphs ? 2u : 1u) && status == BF_OK; ++graph) {
+                    const uint32_t launch = groups[g] | BF_LAUNCH_TRACE_BLOCK(blocks[b]) | (graph ? BF_LAUNCH_GRAPH : 0u);
+                    float ms = 0.0f;
+                    status = run(launch, ms);
+                    if (status != BF_OK) break;
+                    if (out.candidates < 24) {
+                        out.tried[out.candidates] = launch;
+                        out.tried_ms[out.candidates] = ms;
+                    }
+                    ++out.candidates;
+                    if (ms < out.best_ms) {
+                        out.best_ms = ms;
+                        out.launch = launch;
+                    }
+                }
+    }
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+    const hipError_t fe = hipFree(scratch);
+    if (status != BF_OK) return status;
+    if (fe != hipSuccess) return hip_error(fe, "hipFree(autotune films)");
+    *result = out;
+    return BF_OK;
+}
+
+int blackframe_hip_trace_wavefront_adjoint(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_job* adjoints,
+                                           uint32_t job_count, const bf_transport_options* o, bf_transport_report* report,
+                                           void* stream_handle) {
+    if (!adjoints) return set_error(BF_INVALID_ARGUMENT, "NULL adjoint jobs");
+    if (job_count > 0xFFFFu) return set_error(BF_INVALID_ARGUMENT, "at most 65535 frames per batch");
+    return trace_jobs(w, jobs, adjoints, job_count, o, report, nullptr, (hipStream_t)stream_handle);
 }
 
 } /* extern "C" */

@@ -10,15 +10,38 @@
 #include "bf_device_math.h"
 #include "bf_device_scene.h"
 
-typedef struct bf_transport_params {
+/*
+ * One frame of a launch (a batch holds one job per scene). Within a launch the job owns the
+ * workspace slots [slot_begin, slot_begin + slot_count): whole sample passes, path slot
+ * slot_begin + k being path path_base + k of its frame (path = pass * pixels + pixel).
+ */
+typedef struct bf_dev_job {
     bf_dev_scene scene;
-    bf_dev_paths paths;
-    sampler_ctx sampler;
     bf_camera_desc camera;
     uint64_t seed;
-    uint64_t slot_base;         /* global path index of slot 0 */
+    uint64_t path_base;
+    uint32_t slot_begin;
+    uint32_t slot_count;
     uint32_t first_sample_index;
-    uint32_t pixels;            /* camera width * height */
+    uint32_t pixels;
+    uint32_t index;             /* position in the batch (failure reports) */
+    uint32_t reserved;
+    float* film;                /* per-frame film buffers (see blackframe_hip.h) */
+    double* film_compensated;
+    uint32_t* film_counts;
+    bf_path_result* paths;      /* optional per-path results of the whole frame */
+    float4* tape_radiance;      /* optional forward tape / adjoint input, frame path order */
+    unsigned long long* tape_visibility;
+    const float* film_gradient; /* adjoint: dLoss/d(film RGB sums) */
+    float* texture_gradients[BF_MAX_TEXTURES];
+} bf_dev_job;
+
+typedef struct bf_transport_params {
+    bf_dev_job job0;            /* the job of a single-frame launch (kernel argument, no gather) */
+    const bf_dev_job* jobs;     /* the job table of a batched launch (device memory) */
+    uint32_t job_count;
+    bf_dev_paths paths;
+    sampler_ctx sampler;
     uint32_t mis;
     uint32_t wavelength_mode;
     uint32_t limits[5];
@@ -27,29 +50,41 @@ typedef struct bf_transport_params {
     float rr_min;
     float rr_max;
     float packet[4];
-    uint32_t* failure;          /* [0] failed path count, [2..3] lowest (global slot << 8 | code) */
+    uint32_t* failure;          /* [0] failed path count, [2..3] lowest (job << 48 | path << 8 | code) */
     uint2* spill;               /* traversal stack spill: BF_STACK_DEPTH entries per persistent lane */
     uint32_t spill_stride;      /* persistent lanes = resident traversal workgroups * BF_TRACE_BLOCK */
+    uint32_t shade_blocks;      /* resident shade workgroups (persistent grid) */
+    uint32_t trace_block;       /* lanes per traversal workgroup: 64, 128 or 256 */
+    float film_pdf;             /* wavelength pdf and sensor normalisation of the film (adjoint) */
+    float film_normalization;
+    uint32_t adjoint;           /* launch mode (host bookkeeping: part of the graph-cache key) */
 } bf_transport_params;
 
 #ifdef __cplusplus
 extern "C" {
 #endif
-hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, hipStream_t s);
+/* adjoint: regenerate the primary rays of a taped frame and load each path's tape and film
+ * gradient instead of clearing the tape */
+hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, bool batch, bool adjoint,
+                      hipStream_t s);
 hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
-                             uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s);
+                             uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s);
+/* adjoint: shadow visibility from the tape (route 1 is never produced) and texel gradients */
 hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
-                     uint8_t* route, uint32_t* sample_counts, hipStream_t s);
+                     uint8_t* route, uint32_t* sample_counts, bool batch, bool adjoint, hipStream_t s);
 hipError_t bfk_trace_shadow(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
-                            uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s);
-uint32_t bfk_compact_blocks(uint32_t max_count);
-/* Traversal workgroups the device holds at once (all traversal kernels), 0 on error. */
-uint32_t bfk_trace_resident_blocks(int device);
+                            uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s);
+/* 256-item tiles the compaction of max_count items uses (size of the tile count / offset arrays / 2). */
+uint32_t bfk_compact_tiles(uint32_t max_count);
 hipError_t bfk_compact(const uint8_t* route, const uint32_t* count_ptr, uint32_t max_count, const uint32_t* src,
-                       uint32_t* block_counts, uint32_t* block_offsets, uint32_t* dst1, uint32_t* size1, uint32_t append1,
+                       uint32_t* tile_counts, uint32_t* tile_offsets, uint32_t* dst1, uint32_t* size1, uint32_t append1,
                        uint32_t* dst2, uint32_t* size2, uint32_t append2, hipStream_t s);
-hipError_t bfk_film(const bf_dev_paths* paths, uint32_t pixels, uint32_t passes, float pdf, float normalization,
-                    float* film, double* film_compensated, uint32_t* film_counts, hipStream_t s);
+/* sizes[0..3] -> history[4 * sizes[7]], sizes[0] = sizes[3], sizes[1..5] = 0, ++sizes[7] */
+hipError_t bfk_bounce_end(uint32_t* sizes, uint32_t* history, uint32_t history_bounces, hipStream_t s);
+hipError_t bfk_film(const bf_dev_paths* paths, const bf_dev_job* job, float pdf, float normalization, hipStream_t s);
+/* Workgroups the device holds at once for the traversal / shade kernels, 0 on error. */
+uint32_t bfk_trace_resident_lanes(int device);
+uint32_t bfk_shade_resident_blocks(int device);
 hipError_t bfk_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, unsigned long long* out, hipStream_t s);
 hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
                                  bf_closest_hit_result* results, uint32_t* counters, hipStream_t s);
@@ -59,7 +94,7 @@ hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t d
                            const uint64_t* dimensions, uint32_t count, float* out, hipStream_t s);
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s);
 hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s);
-hipError_t bfk_export_paths(const bf_dev_paths* paths, uint32_t count, bf_path_result* out, hipStream_t s);
+hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
 #ifdef __cplusplus
 }
 #endif

@@ -10,30 +10,56 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
+#include "../../include/blackframe_texture.h"
+#include "bf_device_adjoint.h"
 #include "bf_kernels.h"
 #include "bf_shading.h"
 #include "bf_traversal.h"
 
 #define BF_BLOCK 256u
 
+/* RGB basis spectra of textured reflectance, [3][BF_RGB_BASIS_SAMPLES] */
+__constant__ float c_rgb_basis[3 * BF_RGB_BASIS_SAMPLES] = {
+#include "data/rgb_basis.inc"
+};
+
 /* ---- path helpers ----------------------------------------------------------------------------- */
-BF_D sample_key key_for_slot(const bf_transport_params& P, uint32_t slot) {
-    const uint64_t global = P.slot_base + slot;
-    const uint64_t s = global / P.pixels;
-    const uint32_t rem = (uint32_t)(global - s * P.pixels);
+/* The frame a slot belongs to: the kernel argument in a single-frame launch, the job table entry
+ * recorded for the slot by the camera stage in a batched one. */
+template <bool BATCH>
+BF_D const bf_dev_job& job_at(const bf_transport_params& P, uint32_t slot) {
+    if constexpr (BATCH) return P.jobs[P.paths.job[slot]];
+    else return P.job0;
+}
+
+BF_D uint64_t path_of(const bf_dev_job& J, uint32_t slot) { return J.path_base + (slot - J.slot_begin); }
+
+BF_D sample_key key_for_slot(const bf_dev_job& J, uint32_t slot) {
+    const uint64_t global = path_of(J, slot);
+    const uint64_t s = global / J.pixels;
+    const uint32_t rem = (uint32_t)(global - s * J.pixels);
     sample_key k;
-    k.px = rem % P.camera.width;
-    k.py = rem / P.camera.width;
-    k.s = (uint64_t)P.first_sample_index + s;
-    k.seed = P.seed;
+    k.px = rem % J.camera.width;
+    k.py = rem / J.camera.width;
+    k.s = (uint64_t)J.first_sample_index + s;
+    k.seed = J.seed;
     return k;
 }
 
-/* Marks a path failed; the workspace keeps the failure count and the lowest (global slot, code). */
-BF_D void fail_path(const bf_transport_params& P, uint32_t slot, uint32_t code) {
+/* Marks a path failed; the workspace keeps the failure count and the lowest (job, path, code). */
+BF_D void fail_path(const bf_transport_params& P, const bf_dev_job& J, uint32_t slot, uint32_t code) {
     P.paths.status[slot] = BF_TERM_FAILED | (code << 8);
     atomicAdd(&P.failure[0], 1u);
-    atomicMin((unsigned long long*)(P.failure + 2), ((unsigned long long)(P.slot_base + slot) << 8) | code);
+    atomicMin((unsigned long long*)(P.failure + 2),
+              ((unsigned long long)J.index << 48) | ((unsigned long long)path_of(J, slot) << 8) | code);
+}
+
+/* Adds the number of active lanes to *counter with one atomic per wave. */
+BF_D void wave_count(uint32_t* counter) {
+    const uint64_t active = __ballot(1);
+    if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((unsigned long long)active) - 1)) atomicAdd(counter, (uint32_t)__popcll(active));
 }
 
 BF_D float spectrum_at(const bf_dev_scene& S, uint32_t id, float wavelength) {
@@ -84,10 +110,63 @@ BF_D bool camera_ray(const bf_camera_desc& cam, const sampler_ctx& sc, const sam
 }
 
 /* ---- camera stage: device-side primary rays, wavelengths and initial state ------------------- */
+__constant__ double c_cie_table_rows[471][3] = {
+#include "data/cie1931_cmf.inc"
+};
+
+/* color_matching_functions (Cie1931Sensor.cpp:69-93) */
+BF_D void cie_cmf(float wavelength, float* response) {
+    const float coordinate = wavelength - 360.0f;
+    const uint32_t lower_index = (uint32_t)coordinate;
+    if (lower_index + 1u >= 471u) {
+        for (int c = 0; c < 3; ++c) response[c] = (float)c_cie_table_rows[470][c];
+    } else {
+        const float fraction = coordinate - (float)lower_index;
+        for (int c = 0; c < 3; ++c) {
+            const float lower = (float)c_cie_table_rows[lower_index][c];
+            const float upper = (float)c_cie_table_rows[lower_index + 1][c];
+            response[c] = fmaf(upper - lower, fraction, lower);
+        }
+    }
+}
+
+/* d(film RGB)/d(radiance lane) for the path's pixel, contracted with dLoss/d(film RGB): the film
+ * adds rgb = M * sum_lanes L cmf(lambda) normalization / pdf per path (cie_xyz below). */
+BF_D float4 film_adjoint(const bf_transport_params& P, const bf_dev_job& J, const sample_key& k, const float* wl) {
+    const size_t pixel = (size_t)k.py * J.camera.width + k.px;
+    const float d[3] = {J.film_gradient[3 * pixel], J.film_gradient[3 * pixel + 1], J.film_gradient[3 * pixel + 2]};
+    const float dx = 3.2404542f * d[0] - 0.9692660f * d[1] + 0.0556434f * d[2];
+    const float dy = -1.5371385f * d[0] + 1.8760108f * d[1] - 0.2040259f * d[2];
+    const float dz = -0.4985314f * d[0] + 0.0415560f * d[1] + 1.0572252f * d[2];
+    const float scale = P.film_normalization / P.film_pdf;
+    float a[4];
+#pragma unroll
+    for (int lane = 0; lane < 4; ++lane) {
+        float r[3];
+        cie_cmf(wl[lane], r);
+        a[lane] = (dx * r[0] + dy * r[1] + dz * r[2]) * scale;
+    }
+    return make_float4(a[0], a[1], a[2], a[3]);
+}
+
+template <bool BATCH, bool ADJ>
 __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint32_t count, uint32_t* ray_queue) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
-    const sample_key k = key_for_slot(P, slot);
+    uint32_t j = 0;
+    if constexpr (BATCH) {
+        /* jobs are ordered by slot_begin: the last one starting at or before this slot */
+        uint32_t lo = 0, hi = P.job_count;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (P.jobs[mid].slot_begin <= slot) lo = mid;
+            else hi = mid;
+        }
+        j = lo;
+        P.paths.job[slot] = (uint16_t)j;
+    }
+    const bf_dev_job& J = BATCH ? P.jobs[j] : P.job0;
+    const sample_key k = key_for_slot(J, slot);
     v3f o, d;
     ray_queue[slot] = slot;
     P.paths.status[slot] = BF_TERM_NONE;
@@ -95,12 +174,12 @@ __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint
     P.paths.radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.paths.depth[slot] = make_uint4(0u, 0u, 0u, __float_as_uint(1.0f));
     P.paths.previous[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.paths.visibility[slot] = P.camera.visibility_mask;
+    P.paths.visibility[slot] = J.camera.visibility_mask;
     float wl[4];
     if (P.wavelength_mode == BF_WAVELENGTHS_PER_PATH) {
         const float u = sample_1d(P.sampler, k, 3ull);
         if (!(u >= 0.0f) || !(u < 1.0f)) {
-            fail_path(P, slot, 2);
+            fail_path(P, J, slot, 2);
             return;
         }
         /* WavelengthSampling.hpp:62-84 */
@@ -116,32 +195,44 @@ __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint
         for (int lane = 0; lane < 4; ++lane) wl[lane] = P.packet[lane];
     }
     P.paths.wavelengths[slot] = make_float4(wl[0], wl[1], wl[2], wl[3]);
-    if (!camera_ray(P.camera, P.sampler, k, o, d)) {
-        fail_path(P, slot, 1);
+    if (J.tape_visibility) {
+        const uint64_t path = path_of(J, slot);
+        if constexpr (ADJ) {
+            /* adjoint lanes keep the tape in the shadow columns, which the replay never traces */
+            const unsigned long long vis = J.tape_visibility[path];
+            P.paths.shadow_origin[slot] = J.tape_radiance[path];
+            P.paths.shadow_direction[slot] =
+                make_float4(__uint_as_float((uint32_t)vis), __uint_as_float((uint32_t)(vis >> 32)), 0.0f, 0.0f);
+            P.paths.shadow_contribution[slot] = film_adjoint(P, J, k, wl);
+        } else {
+            J.tape_visibility[path] = 0ull;
+        }
+    }
+    if (!camera_ray(J.camera, P.sampler, k, o, d)) {
+        fail_path(P, J, slot, 1);
         return;
     }
-    P.paths.ray_origin[slot] = make_float4(o.x, o.y, o.z, P.camera.time);
-    P.paths.ray_direction[slot] = make_float4(d.x, d.y, d.z, P.camera.t_min);
+    P.paths.ray_origin[slot] = make_float4(o.x, o.y, o.z, J.camera.time);
+    P.paths.ray_direction[slot] = make_float4(d.x, d.y, d.z, J.camera.t_min);
 }
 
 /* ---- closest-hit stage: gather + traversal + classify + miss ---------------------------------- */
 /* The traversal kernels are persistent: a grid sized to the device's resident capacity, each wave
  * taking 64 consecutive queue items at a time with a static stride, so that the per-lane stack
  * spill area is bounded by the grid and empty tail workgroups cost nothing. */
-BF_D uint32_t wave_index() { return (blockIdx.x * BF_TRACE_BLOCK + threadIdx.x) >> 6; }
-BF_D uint32_t wave_count() { return (gridDim.x * BF_TRACE_BLOCK) >> 6; }
 
-template <bool COUNT>
+template <bool COUNT, bool BATCH, uint32_t BLOCK>
 BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
-                             trav_stack<BF_SHORT_STACK>& stack, trav_counters& ctr) {
+                             trav_stack<BF_SHORT_STACK, BLOCK>& stack, trav_counters& ctr) {
     const uint32_t slot = queue[i];
+    const bf_dev_job& J = job_at<BATCH>(P, slot);
     const float4 ro = P.paths.ray_origin[slot];
     const float4 rd = P.paths.ray_direction[slot];
     trav_ray ray;
     ray.o = mk(ro.x, ro.y, ro.z);
     ray.d = mk(rd.x, rd.y, rd.z);
     ray.t_min = rd.w;
-    ray.t_max = (P.paths.depth[slot].y == 0u) ? P.camera.t_max : BF_INFINITY_F;
+    ray.t_max = (P.paths.depth[slot].y == 0u) ? J.camera.t_max : BF_INFINITY_F;
     ray.mask = P.paths.visibility[slot];
     trav_hit best;
     bool occluded;
@@ -153,22 +244,22 @@ BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue
         o = mk(a.x, a.y, a.z);
         d = mk(b.x, b.y, b.z);
     };
-    uint32_t st = traverse<false, COUNT>(P.scene, ray, world, stack, best, occluded, ctr);
+    uint32_t st = traverse<false, COUNT>(J.scene, ray, world, stack, best, occluded, ctr);
     if (st == BF_HIT_MISS && best.present) {
         v3f o, d;
         world(o, d);
-        st = closest_hit_barycentrics(P.scene, o, d, ray.t_min, ray.t_max, best);
+        st = closest_hit_barycentrics(J.scene, o, d, ray.t_min, ray.t_max, best);
     }
     uint8_t r = 0;
     if (st != BF_HIT_MISS) {
-        fail_path(P, slot, 60u + st);
+        fail_path(P, J, slot, 60u + st);
     } else if (best.present) {
         P.paths.hit[slot] = make_float4(best.t, best.b0, best.b1, best.b2);
         P.paths.hit_ids[slot] = make_uint2(best.scene_instance, best.primitive);
         r = 1;
     } else {
         /* miss stage: CudaWavefrontStages.cu:3962-4040 */
-        if (P.scene.has_environment) {
+        if (J.scene.has_environment) {
             const float4 wl = P.paths.wavelengths[slot];
             const float4 beta = P.paths.beta[slot];
             float4 L = P.paths.radiance[slot];
@@ -179,7 +270,7 @@ BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 float product;
-                const float radiance = spectrum_at(P.scene, P.scene.environment_spectrum, w4[k]);
+                const float radiance = spectrum_at(J.scene, J.scene.environment_spectrum, w4[k]);
                 if (!checked_product(b4[k], radiance, product)) ok = false;
                 else {
                     l4[k] += product;
@@ -187,7 +278,7 @@ BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue
                 }
             }
             if (!ok) {
-                fail_path(P, slot, 50);
+                fail_path(P, J, slot, 50);
             } else {
                 P.paths.radiance[slot] = make_float4(l4[0], l4[1], l4[2], l4[3]);
                 P.paths.status[slot] = BF_TERM_ESCAPED;
@@ -213,18 +304,19 @@ BF_D void reduce_counters(const trav_counters& c, unsigned long long* counters) 
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(BF_TRACE_BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closest(bf_transport_params P, const uint32_t* queue,
-                                                                  const uint32_t* count_ptr, uint8_t* route,
-                                                                  unsigned long long* counters) {
-    __shared__ uint2 s_stack[BF_SHORT_STACK * BF_TRACE_BLOCK];
+template <bool COUNT, bool BATCH, uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closest(bf_transport_params P, const uint32_t* queue,
+                                                                      const uint32_t* count_ptr, uint8_t* route,
+                                                                      unsigned long long* counters) {
+    __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
-    trav_stack<BF_SHORT_STACK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BF_TRACE_BLOCK + threadIdx.x,
-                                     P.spill_stride, 0u, 0u};
+    trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
+                                            P.spill_stride, 0u, 0u};
     trav_counters ctr{0u, 0u, 0u};
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t base = wave_index() * 64u; base < n; base += wave_count() * 64u) {
-        if (base + lane < n) trace_closest_item<COUNT>(P, queue, base + lane, route, stack, ctr);
+    const uint32_t wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, waves = (gridDim.x * BLOCK) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
+        if (base + lane < n) trace_closest_item<COUNT, BATCH, BLOCK>(P, queue, base + lane, route, stack, ctr);
     }
     if (COUNT) reduce_counters(ctr, counters);
 }
@@ -232,6 +324,7 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_tra
 /* ---- shade stage: CudaWavefrontStages.cu:4107-4727 -------------------------------------------- */
 struct surface_t {
     v3f position, error, ng, ns, dpdu;
+    float uv[2];
     uint32_t material;
 };
 
@@ -284,6 +377,8 @@ BF_D bool surface_data(const bf_dev_scene& S, uint32_t inst, uint32_t prim, floa
         const v3f dpdv = mul3(sub3(mul3(e2, du1), mul3(e1, du2)), rcp);
         if (!bf_isfinite(rcp) || !fin3(dpdu) || !fin3(dpdv)) return false;
     }
+    /* texture coordinates at the hit, for textured closures */
+    for (int k = 0; k < 2; ++k) out.uv[k] = fmaf(tc[2][k], bary.z, fmaf(tc[1][k], bary.y, tc[0][k] * bary.x));
     positioned_t obj, world;
     const v3f traversal = mk(fmaf(hit.x, rd.x, ro.x), fmaf(hit.x, rd.y, ro.y), fmaf(hit.x, rd.z, ro.z));
     if (!interpolate_position_with_error(lv, bary, obj) || !transform_position_with_error(m, obj, traversal, world))
@@ -297,7 +392,17 @@ BF_D bool surface_data(const bf_dev_scene& S, uint32_t inst, uint32_t prim, floa
     return fin3(out.position) && fin3(out.error);
 }
 
-BF_D void load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const float* wl, mixture_t& mix) {
+/* Filtered texel value of a scene texture at (u, v); false when the lookup is not representable. */
+BF_D bool texture_value(const bf_dev_scene& S, uint32_t slot, const float* uv, float* value, bf_texel_footprint& f) {
+    const bf_dev_texture T = S.textures[slot];
+    if (!bf_texture_footprint(T.width, T.height, uv[0], uv[1], &f)) return false;
+    bf_texture_filter(T.data, T.channels, &f, value);
+    return true;
+}
+
+/* Closure records at the path's wavelengths; textured weights and GGX widths are looked up at the
+ * hit's texture coordinates. False when a texture lookup is not representable. */
+BF_D bool load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const float* wl, const float* uv, mixture_t& mix) {
     mix.active_count = M.closure_count;
     for (uint32_t c = 0; c < M.closure_count; ++c) {
         const bf_closure_desc D = S.closures[M.closure_offset + c];
@@ -314,10 +419,28 @@ BF_D void load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const f
                 r.parameters[4 + lane] = spectrum_at(S, D.k_spectrum, wl[lane]);
             }
         }
+        if (D.weight_texture) {
+            float t[3];
+            bf_texel_footprint f;
+            if (!texture_value(S, D.weight_texture - 1u, uv, t, f)) return false;
+            const uint32_t channels = S.textures[D.weight_texture - 1u].channels;
+#pragma unroll
+            for (int lane = 0; lane < 4; ++lane) r.weight[lane] *= bf_texel_reflectance(c_rgb_basis, t, channels, wl[lane]);
+        }
+        if (D.roughness_texture) {
+            float t[3];
+            bf_texel_footprint f;
+            if (!texture_value(S, D.roughness_texture - 1u, uv, t, f)) return false;
+            const float ay = S.textures[D.roughness_texture - 1u].channels == 2u ? t[1] : t[0];
+            const int at = D.kind == BF_CLOSURE_ROUGH_CONDUCTOR ? 8 : 2;
+            r.parameters[at] = t[0];
+            r.parameters[at + 1] = ay;
+        }
         mix.probabilities[c] = M.probabilities[c];
     }
 #pragma unroll
     for (uint32_t c = 0; c <= BF_MAX_CLOSURES; ++c) mix.cdf[c] = M.cdf[c];
+    return true;
 }
 
 struct incident_t {
@@ -329,9 +452,8 @@ struct incident_t {
 
 /* sample_registered_light (:2378-2460) + sample_mesh_area_light (:2044-2201); the per-light area
  * CDF of :2085-2147 is a scene constant evaluated by the host, searched here in O(log n). */
-BF_D bool sample_light(const bf_transport_params& P, const sample_key& k, uint64_t first_dim, const float* wl,
-                       uint32_t visibility, const surface_t& surf, incident_t& inc) {
-    const bf_dev_scene& S = P.scene;
+BF_D bool sample_light(const bf_transport_params& P, const bf_dev_scene& S, const sample_key& k, uint64_t first_dim,
+                       const float* wl, uint32_t visibility, const surface_t& surf, incident_t& inc) {
     inc.present = false;
     const float u_sel = sample_1d(P.sampler, k, first_dim + 0);
     uint32_t li;
@@ -410,10 +532,10 @@ BF_D bool sample_light(const bf_transport_params& P, const sample_key& k, uint64
 }
 
 /* emissive_hit_mis_weight (:2513-2594) */
-BF_D bool emission_weight(const bf_transport_params& P, uint32_t inst, float4 prev, const surface_t& surf, float& w) {
+BF_D bool emission_weight(const bf_transport_params& P, const bf_dev_scene& S, uint32_t inst, float4 prev,
+                          const surface_t& surf, float& w) {
     w = 1.0f;
     if (!(prev.w > 0.0f)) return true;
-    const bf_dev_scene& S = P.scene;
     const int32_t li = S.instances[inst].light_index;
     if (li < 0) return false;
     float sel;
@@ -434,15 +556,105 @@ BF_D bool emission_weight(const bf_transport_params& P, uint32_t inst, float4 pr
 
 BF_D bool zero4(const float* v) { return v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f && v[3] == 0.0f; }
 
-/* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation */
-__global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const uint32_t* queue, const uint32_t* count_ptr,
-                                                    uint8_t* route, uint32_t* sample_counts) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *count_ptr) return;
+/* ---- adjoint: texel gradients of one path vertex ---------------------------------------------
+ * The path's estimate is a sum of products over its vertices; with sampling, MIS weights and
+ * roulette held constant, a parameter p of the closures at vertex j scales every term that passes
+ * through that vertex's BSDF value. For the light sample (visible contribution N, closure value
+ * f_l) and the continuation (radiance R it carried back, value f_s):
+ *   dLoss/dp += sum_lanes a * (N * (df_l/dp) / f_l + R * (df_s/dp) / f_s),
+ * a = dLoss/d(radiance lane) of the path's pixel. Texel gradients follow through the bilinear
+ * weights and, for weights, the RGB basis spectra. */
+BF_D void texel_gradients(const bf_dev_job& J, const bf_dev_scene& S, const bf_material_desc& M, const mixture_t& mix,
+                          const filter_t& filtered, const float* uv, const float* wl, const float* a, bool light, v3f wo,
+                          v3f wi_light, const float* f_light, const float* N, bool cont, v3f wi_cont, const float* f_cont,
+                          const float* R) {
+    for (uint32_t i = 0; i < mix.active_count; ++i) {
+        const bf_closure_desc D = S.closures[M.closure_offset + filtered.source_indices[i]];
+        float* wgrad = D.weight_texture ? J.texture_gradients[D.weight_texture - 1u] : nullptr;
+        float* rgrad = D.roughness_texture ? J.texture_gradients[D.roughness_texture - 1u] : nullptr;
+        if (!wgrad && !rgrad) continue;
+        const closure_t& c = mix.closures[i];
+        float gl[4], gs[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            gl[k] = light && f_light[k] > 0.0f ? a[k] * N[k] / f_light[k] : 0.0f;
+            gs[k] = cont && f_cont[k] > 0.0f ? a[k] * R[k] / f_cont[k] : 0.0f;
+        }
+        if (wgrad) {
+            float dl[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ds[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const bool lw = light && closure_weight_derivative(c, filtered.allowed[i], wo, wi_light, dl) == BF_L_OK;
+            const bool cw = cont && closure_weight_derivative(c, filtered.allowed[i], wo, wi_cont, ds) == BF_L_OK;
+            const uint32_t slot = D.weight_texture - 1u;
+            const uint32_t channels = S.textures[slot].channels;
+            float t[3];
+            bf_texel_footprint f;
+            if (texture_value(S, slot, uv, t, f)) {
+                float sc[3] = {0.0f, 0.0f, 0.0f};
+                for (int k = 0; k < 4; ++k) {
+                    const float g = (lw ? gl[k] * dl[k] : 0.0f) + (cw ? gs[k] * ds[k] : 0.0f);
+                    if (g == 0.0f) continue;
+                    const float w = g * spectrum_at(S, D.weight_spectrum, wl[k]);
+                    if (channels == 1u) {
+                        if (t[0] > 0.0f && t[0] < 1.0f) sc[0] += w;
+                    } else {
+                        float b[3];
+                        bf_rgb_basis_at(c_rgb_basis, wl[k], b);
+                        const float rho = t[0] * b[0] + t[1] * b[1] + t[2] * b[2];
+                        if (!(rho > 0.0f) || !(rho < 1.0f)) continue; /* clamped: no gradient */
+                        for (int ch = 0; ch < 3; ++ch) sc[ch] += w * b[ch];
+                    }
+                }
+                for (int q = 0; q < 4; ++q)
+                    for (uint32_t ch = 0; ch < channels; ++ch)
+                        if (f.weight[q] != 0.0f && sc[ch] != 0.0f)
+                            unsafeAtomicAdd(&wgrad[(size_t)f.index[q] * channels + ch], f.weight[q] * sc[ch]);
+            }
+        }
+        if (rgrad && (c.kind == BF_CLOSURE_ROUGH_CONDUCTOR || c.kind == BF_CLOSURE_ROUGH_DIELECTRIC)) {
+            alpha_gradient ql{0.0f, 0.0f}, qs{0.0f, 0.0f};
+            const bool use_l = light && closure_alpha_log_gradient(c, wo, wi_light, ql);
+            const bool use_s = cont && closure_alpha_log_gradient(c, wo, wi_cont, qs);
+            /* the closure's own value at each direction: the mixture value when it is alone */
+            float fl[4], fs[4];
+            for (int k = 0; k < 4; ++k) {
+                fl[k] = f_light[k];
+                fs[k] = f_cont[k];
+            }
+            if (mix.active_count > 1u) {
+                if (use_l && record_eval(c, filtered.allowed[i], wo, wi_light, fl) != BF_L_OK) continue;
+                if (use_s && record_eval(c, filtered.allowed[i], wo, wi_cont, fs) != BF_L_OK) continue;
+            }
+            float hx = 0.0f, hy = 0.0f;
+            for (int k = 0; k < 4; ++k) {
+                const float el = use_l ? gl[k] * fl[k] : 0.0f, es = use_s ? gs[k] * fs[k] : 0.0f;
+                hx += el * ql.x + es * qs.x;
+                hy += el * ql.y + es * qs.y;
+            }
+            const uint32_t slot = D.roughness_texture - 1u;
+            const uint32_t channels = S.textures[slot].channels;
+            float t[3];
+            bf_texel_footprint f;
+            if (texture_value(S, slot, uv, t, f)) {
+                const float h[2] = {channels == 2u ? hx : hx + hy, hy};
+                for (int q = 0; q < 4; ++q)
+                    for (uint32_t ch = 0; ch < channels; ++ch)
+                        if (f.weight[q] != 0.0f && h[ch] != 0.0f)
+                            unsafeAtomicAdd(&rgrad[(size_t)f.index[q] * channels + ch], f.weight[q] * h[ch]);
+            }
+        }
+    }
+}
+
+/* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation. The adjoint replay
+ * (ADJ) resolves the light sample from the tape and never routes to the shadow queue. */
+template <bool BATCH, bool ADJ>
+BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
+                     uint32_t* sample_counts) {
     const uint32_t slot = queue[i];
     route[i] = 0;
-    const bf_dev_scene& S = P.scene;
-    const sample_key key = key_for_slot(P, slot);
+    const bf_dev_job& J = job_at<BATCH>(P, slot);
+    const bf_dev_scene& S = J.scene;
+    const sample_key key = key_for_slot(J, slot);
     const float4 ro4 = P.paths.ray_origin[slot];
     const float4 rd4 = P.paths.ray_direction[slot];
     const v3f ro = mk(ro4.x, ro4.y, ro4.z), rd = mk(rd4.x, rd4.y, rd4.z);
@@ -458,40 +670,41 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
     uint32_t diffuse = dstate.x & 0xFFu, glossy = (dstate.x >> 8) & 0xFFu, specular = (dstate.x >> 16) & 0xFFu,
              transmission = (dstate.x >> 24) & 0xFFu;
     uint32_t depth = dstate.y, delta_flags = dstate.z;
+    const uint32_t vertex = depth;
     float eta_scale = __uint_as_float(dstate.w);
     const float4 prev = P.paths.previous[slot];
     const uint32_t visibility = P.paths.visibility[slot];
 
     surface_t surf;
-    if (!surface_data(S, ids.x, ids.y, hit, ro, rd, surf)) return fail_path(P, slot, 10);
+    if (!surface_data(S, ids.x, ids.y, hit, ro, rd, surf)) return fail_path(P, J, slot, 10);
     v3f wo;
-    if (!normalize3(mk(-rd.x, -rd.y, -rd.z), wo)) return fail_path(P, slot, 11);
+    if (!normalize3(mk(-rd.x, -rd.y, -rd.z), wo)) return fail_path(P, J, slot, 11);
     bool front = false;
-    if (!one_sided_emission_support(surf.ng, wo, front)) return fail_path(P, slot, 12);
+    if (!one_sided_emission_support(surf.ng, wo, front)) return fail_path(P, J, slot, 12);
     const bf_material_desc& M = S.materials[surf.material];
     float emitted[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         emitted[c] = spectrum_at(S, M.emission_spectrum, wl[c]);
-        if (!bf_isfinite(emitted[c]) || emitted[c] < 0.0f) return fail_path(P, slot, 13);
+        if (!bf_isfinite(emitted[c]) || emitted[c] < 0.0f) return fail_path(P, J, slot, 13);
     }
     mixture_t mix;
-    load_mixture(S, M, wl, mix);
+    if (!load_mixture(S, M, wl, surf.uv, mix)) return fail_path(P, J, slot, 42);
     frame3 frame, unrotated;
     const bool created = M.frame_mode == 0u ? make_local_frame(surf.ns, unrotated) : frame_from_tangent(surf.ns, surf.dpdu, unrotated);
-    if (!created || !rotate_frame(unrotated, M.tangent_rotation, frame)) return fail_path(P, slot, 14);
+    if (!created || !rotate_frame(unrotated, M.tangent_rotation, frame)) return fail_path(P, J, slot, 14);
     if (front && !zero4(emitted)) {
         float weight;
-        if (!emission_weight(P, ids.x, prev, surf, weight)) return fail_path(P, slot, 15);
+        if (!emission_weight(P, S, ids.x, prev, surf, weight)) return fail_path(P, J, slot, 15);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             if (beta[c] == 0.0f || emitted[c] == 0.0f || weight == 0.0f) continue;
             const float num[3] = {beta[c], emitted[c], weight};
             const float den[1] = {1.0f};
             float contribution;
-            if (checked_product_quotient(num, den, contribution) != BF_L_OK) return fail_path(P, slot, 16);
+            if (checked_product_quotient(num, den, contribution) != BF_L_OK) return fail_path(P, J, slot, 16);
             L[c] += contribution;
-            if (!bf_isfinite(L[c])) return fail_path(P, slot, 16);
+            if (!bf_isfinite(L[c])) return fail_path(P, J, slot, 16);
         }
         P.paths.radiance[slot] = make_float4(L[0], L[1], L[2], L[3]);
     }
@@ -502,10 +715,10 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
     const uint32_t limits[5] = {P.limits[0], P.limits[1], P.limits[2], P.limits[3], P.limits[4]};
     const uint32_t counters[5] = {diffuse, glossy, specular, transmission, 0u};
     if (diffuse > limits[0] || glossy > limits[1] || specular > limits[2] || transmission > limits[3])
-        return fail_path(P, slot, 17);
+        return fail_path(P, J, slot, 17);
     filter_t filtered;
     filter_by_depth(mix, limits, counters, depth, filtered);
-    if (filtered.status != BF_L_OK) return fail_path(P, slot, 18);
+    if (filtered.status != BF_L_OK) return fail_path(P, J, slot, 18);
     if (mix.active_count == 0 && filtered.source_count != 0) {
         P.paths.status[slot] = BF_TERM_DEPTH_LIMIT;
         return;
@@ -515,28 +728,31 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
     const float bsdf_component = sample_1d(P.sampler, key, first_dim + 3);
     const float bsdf_u = sample_1d(P.sampler, key, first_dim + 4);
     const float bsdf_v = sample_1d(P.sampler, key, first_dim + 5);
-    atomicAdd(&sample_counts[0], 1u);
+    wave_count(&sample_counts[0]);
     mixture_sample_t bs;
     sample_mixture(mix, filtered, wo_local, bsdf_component, bsdf_u, bsdf_v, bs);
     const bool has_continuation = bs.status == BF_L_OK;
-    if (!has_continuation && bs.status != BF_L_OUTSIDE) return fail_path(P, slot, 19);
+    if (!has_continuation && bs.status != BF_L_OUTSIDE) return fail_path(P, J, slot, 19);
     if (has_continuation) {
         const bool delta = valid_surface_event(bs.lobes) && (bs.lobes & BF_LOBE_SPECULAR);
         if (!valid_surface_event(bs.lobes) || (bs.discrete != 0u) != delta || !bf_isfinite(bs.pdf) || !(bs.pdf > 0.0f) ||
             (delta && bs.pdf > 1.0f))
-            return fail_path(P, slot, 20);
+            return fail_path(P, J, slot, 20);
     }
     incident_t inc;
     inc.present = false;
     if (S.light_count != 0u && mixture_nonblack(mix)) {
-        atomicAdd(&sample_counts[1], 1u);
-        if (!sample_light(P, key, first_dim, wl, visibility, surf, inc)) return fail_path(P, slot, 21);
+        wave_count(&sample_counts[1]);
+        if (!sample_light(P, S, key, first_dim, wl, visibility, surf, inc)) return fail_path(P, J, slot, 21);
     }
     bool has_shadow = false;
     float4 shadow_o = make_float4(0.0f, 0.0f, 0.0f, 0.0f), shadow_d = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float contribution[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float nee_f[4] = {0.0f, 0.0f, 0.0f, 0.0f}; /* closure value toward the light (adjoint) */
+    v3f nee_wi = mk(0.0f, 0.0f, 0.0f);
     if (inc.present) {
         const v3f wi_local = to_local(frame, inc.direction);
+        nee_wi = wi_local;
         if (wo_local.z != 0.0f && wi_local.z != 0.0f) {
             const bool singleton_lambert = mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN;
             float bsdf[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -544,16 +760,17 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
                 if (wo_local.z > 0.0f && wi_local.z > 0.0f)
                     for (int c = 0; c < 4; ++c) bsdf[c] = mix.closures[0].weight[c];
             } else if (mixture_eval(mix, filtered, wo_local, wi_local, bsdf) != BF_L_OK) {
-                return fail_path(P, slot, 22);
+                return fail_path(P, J, slot, 22);
             }
+            for (int c = 0; c < 4; ++c) nee_f[c] = singleton_lambert ? bsdf[c] * BF_INV_PI : bsdf[c];
             if (!zero4(bsdf)) {
                 float corr;
-                if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, corr)) return fail_path(P, slot, 23);
+                if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, corr)) return fail_path(P, J, slot, 23);
                 if (corr > 0.0f) {
                     float joint, bsdf_pdf, w;
-                    if (!joint_light_pdf(inc.selection, inc.conditional, joint)) return fail_path(P, slot, 24);
-                    if (mixture_pdf(mix, filtered, wo_local, wi_local, bsdf_pdf) != BF_L_OK) return fail_path(P, slot, 24);
-                    if (!mis_weight(P.mis, joint, bsdf_pdf, w)) return fail_path(P, slot, 24);
+                    if (!joint_light_pdf(inc.selection, inc.conditional, joint)) return fail_path(P, J, slot, 24);
+                    if (mixture_pdf(mix, filtered, wo_local, wi_local, bsdf_pdf) != BF_L_OK) return fail_path(P, J, slot, 24);
+                    if (!mis_weight(P.mis, joint, bsdf_pdf, w)) return fail_path(P, J, slot, 24);
                     /* the visible-shadow contribution of process_shadow_kernel (:4772-4908),
                      * evaluated now from the same frozen factors */
                     const float coefficient = singleton_lambert ? BF_INV_PI : 1.0f;
@@ -568,7 +785,7 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
                             const float num[7] = {beta[c], bsdf[c], coefficient, inc.radiance[c], cosine, corr, w};
                             st = checked_product_quotient(num, den, contribution[c]);
                         }
-                        if (st != BF_L_OK) return fail_path(P, slot, 41);
+                        if (st != BF_L_OK) return fail_path(P, J, slot, 41);
                     }
                     /* prepare_shadow_ray (:4042-4105) */
                     v3f src, endpoint;
@@ -576,9 +793,9 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
                     if (!offset_point(surf.position, surf.error, surf.ng, inc.direction, src) ||
                         !offset_point(inc.position, inc.error, inc.normal, mul3(inc.direction, -1.0f), endpoint) ||
                         !scaled_segment(sub3(endpoint, src), seg) || !(dot3(seg.direction, inc.direction) > 0.0f))
-                        return fail_path(P, slot, 25);
+                        return fail_path(P, J, slot, 25);
                     const float tmax = bf_next_toward_zero_f(seg.length);
-                    if (!(tmax > 0.0f)) return fail_path(P, slot, 25);
+                    if (!(tmax > 0.0f)) return fail_path(P, J, slot, 25);
                     shadow_o = make_float4(src.x, src.y, src.z, tmax);
                     shadow_d = make_float4(seg.direction.x, seg.direction.y, seg.direction.z, 0.0f);
                     has_shadow = true;
@@ -590,10 +807,10 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
     bool continuation = has_continuation;
     float corr = 0.0f;
     if (continuation) {
-        if (!to_world(frame, bs.incoming, wi)) return fail_path(P, slot, 26);
+        if (!to_world(frame, bs.incoming, wi)) return fail_path(P, J, slot, 26);
         bool geo = false;
         if (!shading_correction(surf.ng, surf.ns, wo, wi, corr) || !geometric_event_support(surf.ng, wo, wi, bs.lobes, geo))
-            return fail_path(P, slot, 27);
+            return fail_path(P, J, slot, 27);
         continuation = corr > 0.0f && geo;
     }
     uint32_t pending_term = BF_TERM_OUTSIDE_SUPPORT;
@@ -602,22 +819,22 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
         /* update_closure_throughput (:3894-3938) */
         float nb[4];
         const float cosine = fabsf(bs.incoming.z);
-        if (!bf_isfinite(cosine) || !(cosine > 0.0f)) return fail_path(P, slot, 28);
+        if (!bf_isfinite(cosine) || !(cosine > 0.0f)) return fail_path(P, J, slot, 28);
         if (mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN && corr == 1.0f) {
             for (int c = 0; c < 4; ++c)
-                if (!checked_product(beta[c], mix.closures[0].weight[c], nb[c])) return fail_path(P, slot, 28);
+                if (!checked_product(beta[c], mix.closures[0].weight[c], nb[c])) return fail_path(P, J, slot, 28);
         } else {
             for (int c = 0; c < 4; ++c) {
                 const float num[4] = {beta[c], bs.value[c], cosine, corr};
                 const float den[1] = {bs.pdf};
-                if (checked_product_quotient(num, den, nb[c]) != BF_L_OK) return fail_path(P, slot, 28);
+                if (checked_product_quotient(num, den, nb[c]) != BF_L_OK) return fail_path(P, J, slot, 28);
             }
         }
         const float next_eta = eta_scale * bs.eta_scale;
         if (!bf_isfinite(bs.eta_scale) || !(bs.eta_scale > 0.0f) || !bf_isfinite(next_eta) || !(next_eta > 0.0f))
-            return fail_path(P, slot, 29);
+            return fail_path(P, J, slot, 29);
         v3f origin;
-        if (!offset_point(surf.position, surf.error, surf.ng, wi, origin)) return fail_path(P, slot, 30);
+        if (!offset_point(surf.position, surf.error, surf.ng, wi, origin)) return fail_path(P, J, slot, 30);
         const bool delta_event = (bs.lobes & BF_LOBE_SPECULAR) != 0u;
         delta_flags = ((delta_flags & 2u) ? 2u : 0u) | (delta_event ? 1u : 2u);
         /* advance_depth_event (:3851-3880) */
@@ -640,7 +857,7 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
         } else if (P.rr_enabled && depth >= P.rr_first) {
             const float u = sample_1d(P.sampler, key, first_dim + 9);
             bool survived;
-            if (!russian_roulette(beta4, eta_scale, u, P.rr_min, P.rr_max, survived)) return fail_path(P, slot, 32);
+            if (!russian_roulette(beta4, eta_scale, u, P.rr_min, P.rr_max, survived)) return fail_path(P, J, slot, 32);
             if (!survived) {
                 cont_pending = false;
                 pending_term = BF_TERM_RUSSIAN_ROULETTE;
@@ -650,8 +867,36 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
         P.paths.depth[slot] = make_uint4(diffuse | (glossy << 8) | (specular << 16) | (transmission << 24), depth,
                                          delta_flags, __float_as_uint(eta_scale));
     }
+    if constexpr (ADJ) {
+        /* the light sample's visibility from the tape, then this vertex's gradients */
+        const float4 vis4 = P.paths.shadow_direction[slot];
+        const unsigned long long vis =
+            (unsigned long long)__float_as_uint(vis4.x) | ((unsigned long long)__float_as_uint(vis4.y) << 32);
+        const bool visible = has_shadow && vertex < 64u && ((vis >> vertex) & 1ull) != 0ull;
+        if (visible) {
+            for (int c = 0; c < 4; ++c) {
+                L[c] += contribution[c];
+                if (!bf_isfinite(L[c]) || L[c] < 0.0f) return fail_path(P, J, slot, 41);
+            }
+            P.paths.radiance[slot] = make_float4(L[0], L[1], L[2], L[3]);
+        }
+        if (S.has_textures && J.film_gradient) {
+            const float4 total = P.paths.shadow_origin[slot];
+            const float4 a4 = P.paths.shadow_contribution[slot];
+            const float a[4] = {a4.x, a4.y, a4.z, a4.w};
+            const float R[4] = {total.x - L[0], total.y - L[1], total.z - L[2], total.w - L[3]};
+            texel_gradients(J, S, M, mix, filtered, surf.uv, wl, a, visible, wo_local, nee_wi, nee_f, contribution,
+                            cont_pending, bs.incoming, bs.value, R);
+        }
+        if (!cont_pending) {
+            P.paths.status[slot] = pending_term;
+            return;
+        }
+        route[i] = 2;
+        return;
+    }
     if (has_shadow) {
-        shadow_d.w = __uint_as_float((cont_pending ? 1u : 0u) | (pending_term << 8));
+        shadow_d.w = __uint_as_float((cont_pending ? 1u : 0u) | (pending_term << 8) | (vertex << 16));
         P.paths.shadow_origin[slot] = shadow_o;
         P.paths.shadow_direction[slot] = shadow_d;
         P.paths.shadow_contribution[slot] = make_float4(contribution[0], contribution[1], contribution[2], contribution[3]);
@@ -665,10 +910,19 @@ __global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const
     route[i] = 2;
 }
 
-template <bool COUNT>
+template <bool BATCH, bool ADJ>
+__global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const uint32_t* queue, const uint32_t* count_ptr,
+                                                    uint8_t* route, uint32_t* sample_counts) {
+    const uint32_t n = *count_ptr;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        shade_item<BATCH, ADJ>(P, queue, i, route, sample_counts);
+}
+
+template <bool COUNT, bool BATCH, uint32_t BLOCK>
 BF_D void trace_shadow_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
-                            trav_stack<BF_SHORT_STACK>& stack, trav_counters& ctr) {
+                            trav_stack<BF_SHORT_STACK, BLOCK>& stack, trav_counters& ctr) {
     const uint32_t slot = queue[i];
+    const bf_dev_job& J = job_at<BATCH>(P, slot);
     const float4 so = P.paths.shadow_origin[slot];
     const float4 sd = P.paths.shadow_direction[slot];
     trav_ray ray;
@@ -687,14 +941,15 @@ BF_D void trace_shadow_item(const bf_transport_params& P, const uint32_t* queue,
         o = mk(a.x, a.y, a.z);
         d = mk(b.x, b.y, b.z);
     };
-    const uint32_t st = traverse<true, COUNT>(P.scene, ray, world, stack, best, occluded, ctr);
+    const uint32_t st = traverse<true, COUNT>(J.scene, ray, world, stack, best, occluded, ctr);
     const uint32_t packed = __float_as_uint(sd.w);
     uint8_t r = 0;
     if (st != BF_HIT_MISS) {
-        fail_path(P, slot, 40);
+        fail_path(P, J, slot, 40);
     } else {
         bool ok = true;
         if (!occluded) {
+            if (J.tape_visibility) J.tape_visibility[path_of(J, slot)] |= 1ull << ((packed >> 16) & 63u);
             const float4 c = P.paths.shadow_contribution[slot];
             float4 L = P.paths.radiance[slot];
             L.x += c.x;
@@ -704,76 +959,81 @@ BF_D void trace_shadow_item(const bf_transport_params& P, const uint32_t* queue,
             if (!bf_isfinite(L.x) || !bf_isfinite(L.y) || !bf_isfinite(L.z) || !bf_isfinite(L.w) || L.x < 0.0f ||
                 L.y < 0.0f || L.z < 0.0f || L.w < 0.0f) {
                 ok = false;
-                fail_path(P, slot, 41);
+                fail_path(P, J, slot, 41);
             } else {
                 P.paths.radiance[slot] = L;
             }
         }
         if (ok) {
             if (packed & 1u) r = 2;
-            else P.paths.status[slot] = packed >> 8;
+            else P.paths.status[slot] = (packed >> 8) & 0xFFu;
         }
     }
     route[i] = r;
 }
 
 /* ---- shadow stage: gather + any-hit + process_shadow (:4729-4908) ---------------------------- */
-template <bool COUNT>
-__global__ __launch_bounds__(BF_TRACE_BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
-                                                                 const uint32_t* count_ptr, uint8_t* route,
-                                                                 unsigned long long* counters) {
-    __shared__ uint2 s_stack[BF_SHORT_STACK * BF_TRACE_BLOCK];
+template <bool COUNT, bool BATCH, uint32_t BLOCK>
+__global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
+                                                                      const uint32_t* count_ptr, uint8_t* route,
+                                                                      unsigned long long* counters) {
+    __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
-    trav_stack<BF_SHORT_STACK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BF_TRACE_BLOCK + threadIdx.x,
-                                     P.spill_stride, 0u, 0u};
+    trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
+                                            P.spill_stride, 0u, 0u};
     trav_counters ctr{0u, 0u, 0u};
     const uint32_t lane = threadIdx.x & 63u;
-    for (uint32_t base = wave_index() * 64u; base < n; base += wave_count() * 64u) {
-        if (base + lane < n) trace_shadow_item<COUNT>(P, queue, base + lane, route, stack, ctr);
+    const uint32_t wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, waves = (gridDim.x * BLOCK) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
+        if (base + lane < n) trace_shadow_item<COUNT, BATCH, BLOCK>(P, queue, base + lane, route, stack, ctr);
     }
-    if (COUNT) {
-        trav_counters c = ctr;
-        reduce_counters(c, counters + 3);
-    }
+    if (COUNT) reduce_counters(ctr, counters + 3);
 }
 
-/* ---- stable compaction: wave64 ballot/popcount ranks + block-offset scan ---------------------
+/* ---- stable compaction: wave64 ballot/popcount ranks + tile-offset scan -----------------------
  * For route values r = 1, 2: dst_r[base_r + exclusive_prefix(route == r)] = src[i], which is the
- * stable exclusive-scan append of CudaWavefrontQueueCompaction.cu:55-315. */
+ * stable exclusive-scan append of CudaWavefrontQueueCompaction.cu:55-315. The item count is read
+ * on the device and the kernels stride over 256-item tiles, so launches can be issued (and
+ * captured) before the count is known. */
 __global__ __launch_bounds__(BF_BLOCK) void k_compact_count(const uint8_t* route, const uint32_t* count_ptr,
-                                                            uint32_t* block_counts) {
+                                                            uint32_t* tile_counts) {
     __shared__ uint32_t wave_counts[2][BF_BLOCK / 64];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = *count_ptr;
-    const uint8_t r = i < n ? route[i] : 0;
-    const uint64_t m1 = __ballot(r == 1);
-    const uint64_t m2 = __ballot(r == 2);
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
     const uint32_t wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63u) == 0u) {
-        wave_counts[0][wave] = (uint32_t)__popcll(m1);
-        wave_counts[1][wave] = (uint32_t)__popcll(m2);
-    }
-    __syncthreads();
-    if (threadIdx.x < 2u) {
-        uint32_t total = 0;
-        for (uint32_t w = 0; w < BF_BLOCK / 64; ++w) total += wave_counts[threadIdx.x][w];
-        block_counts[2 * blockIdx.x + threadIdx.x] = total;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t i = tile * BF_BLOCK + threadIdx.x;
+        const uint8_t r = i < n ? route[i] : 0;
+        const uint64_t m1 = __ballot(r == 1);
+        const uint64_t m2 = __ballot(r == 2);
+        if ((threadIdx.x & 63u) == 0u) {
+            wave_counts[0][wave] = (uint32_t)__popcll(m1);
+            wave_counts[1][wave] = (uint32_t)__popcll(m2);
+        }
+        __syncthreads();
+        if (threadIdx.x < 2u) {
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < BF_BLOCK / 64; ++w) total += wave_counts[threadIdx.x][w];
+            tile_counts[2 * tile + threadIdx.x] = total;
+        }
+        __syncthreads();
     }
 }
 
-/* single workgroup: exclusive scan of per-block counts for both routes, plus the queue bases */
-__global__ __launch_bounds__(1024) void k_compact_scan(const uint32_t* block_counts, uint32_t block_count,
-                                                       uint32_t* block_offsets, uint32_t* size1, uint32_t* size2,
+/* single workgroup: exclusive scan of the per-tile counts for both routes, plus the queue bases */
+__global__ __launch_bounds__(1024) void k_compact_scan(const uint32_t* count_ptr, const uint32_t* tile_counts,
+                                                       uint32_t* tile_offsets, uint32_t* size1, uint32_t* size2,
                                                        uint32_t append1, uint32_t append2) {
     __shared__ uint32_t partial[2][1024];
     const uint32_t t = threadIdx.x;
-    const uint32_t per = (block_count + 1023u) / 1024u;
+    const uint32_t tiles = (*count_ptr + BF_BLOCK - 1u) / BF_BLOCK;
+    const uint32_t per = (tiles + 1023u) / 1024u;
     const uint32_t begin = t * per;
-    const uint32_t end = begin + per < block_count ? begin + per : block_count;
+    const uint32_t end = begin + per < tiles ? begin + per : tiles;
     uint32_t s1 = 0, s2 = 0;
     for (uint32_t b = begin; b < end; ++b) {
-        s1 += block_counts[2 * b];
-        s2 += block_counts[2 * b + 1];
+        s1 += tile_counts[2 * b];
+        s2 += tile_counts[2 * b + 1];
     }
     partial[0][t] = s1;
     partial[1][t] = s2;
@@ -791,10 +1051,10 @@ __global__ __launch_bounds__(1024) void k_compact_scan(const uint32_t* block_cou
     uint32_t run1 = base1 + partial[0][t] - s1;
     uint32_t run2 = base2 + partial[1][t] - s2;
     for (uint32_t b = begin; b < end; ++b) {
-        block_offsets[2 * b] = run1;
-        block_offsets[2 * b + 1] = run2;
-        run1 += block_counts[2 * b];
-        run2 += block_counts[2 * b + 1];
+        tile_offsets[2 * b] = run1;
+        tile_offsets[2 * b + 1] = run2;
+        run1 += tile_counts[2 * b];
+        run2 += tile_counts[2 * b + 1];
     }
     __syncthreads();
     if (t == 1023u) {
@@ -804,52 +1064,61 @@ __global__ __launch_bounds__(1024) void k_compact_scan(const uint32_t* block_cou
 }
 
 __global__ __launch_bounds__(BF_BLOCK) void k_compact_scatter(const uint8_t* route, const uint32_t* count_ptr,
-                                                              const uint32_t* src, const uint32_t* block_offsets,
+                                                              const uint32_t* src, const uint32_t* tile_offsets,
                                                               uint32_t* dst1, uint32_t* dst2) {
     __shared__ uint32_t wave_counts[2][BF_BLOCK / 64];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n = *count_ptr;
-    const uint8_t r = i < n ? route[i] : 0;
-    const uint64_t m1 = __ballot(r == 1);
-    const uint64_t m2 = __ballot(r == 2);
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    if (lane == 0u) {
-        wave_counts[0][wave] = (uint32_t)__popcll(m1);
-        wave_counts[1][wave] = (uint32_t)__popcll(m2);
-    }
-    __syncthreads();
-    if (r == 0) return;
     const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-    const uint32_t k = r - 1u;
-    uint32_t rank = (uint32_t)__popcll((k == 0 ? m1 : m2) & below);
-    for (uint32_t w = 0; w < wave; ++w) rank += wave_counts[k][w];
-    const uint32_t position = block_offsets[2 * blockIdx.x + k] + rank;
-    (k == 0 ? dst1 : dst2)[position] = src[i];
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t i = tile * BF_BLOCK + threadIdx.x;
+        const uint8_t r = i < n ? route[i] : 0;
+        const uint64_t m1 = __ballot(r == 1);
+        const uint64_t m2 = __ballot(r == 2);
+        if (lane == 0u) {
+            wave_counts[0][wave] = (uint32_t)__popcll(m1);
+            wave_counts[1][wave] = (uint32_t)__popcll(m2);
+        }
+        __syncthreads();
+        if (r != 0) {
+            const uint32_t k = r - 1u;
+            uint32_t rank = (uint32_t)__popcll((k == 0 ? m1 : m2) & below);
+            for (uint32_t w = 0; w < wave; ++w) rank += wave_counts[k][w];
+            const uint32_t position = tile_offsets[2 * tile + k] + rank;
+            (k == 0 ? dst1 : dst2)[position] = src[i];
+        }
+        __syncthreads();
+    }
+}
+
+/* End of a bounce: record the queue sizes of the bounce, make the continuation queue the next ray
+ * queue, clear the scratch sizes, and advance the bounce counter (one thread). */
+__global__ void k_bounce_end(uint32_t* sizes, uint32_t* history, uint32_t history_bounces) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t b = sizes[7];
+    if (b < history_bounces) {
+        history[4 * b + 0] = sizes[0];
+        history[4 * b + 1] = sizes[1];
+        history[4 * b + 2] = sizes[2];
+        history[4 * b + 3] = sizes[3];
+    }
+    sizes[0] = sizes[3];
+    sizes[1] = 0u;
+    sizes[2] = 0u;
+    sizes[3] = 0u;
+    sizes[4] = 0u;
+    sizes[5] = 0u;
+    sizes[7] = b + 1u;
 }
 
 /* ---- film: per-pixel accumulation in sample order (Film.hpp:150-248, Cie1931Sensor.cpp) ----- */
-__constant__ double c_cie_table_rows[471][3] = {
-#include "data/cie1931_cmf.inc"
-};
-
 BF_D void cie_xyz(const float* radiance, const float* wavelengths, float pdf, float normalization, float* xyz) {
     float sum[3] = {0.0f, 0.0f, 0.0f};
     for (int lane = 0; lane < 4; ++lane) {
-        /* color_matching_functions (Cie1931Sensor.cpp:69-93) */
-        const float coordinate = wavelengths[lane] - 360.0f;
-        const uint32_t lower_index = (uint32_t)coordinate;
         float response[3];
-        if (lower_index + 1u >= 471u) {
-            for (int c = 0; c < 3; ++c) response[c] = (float)c_cie_table_rows[470][c];
-        } else {
-            const float fraction = coordinate - (float)lower_index;
-            for (int c = 0; c < 3; ++c) {
-                const float lower = (float)c_cie_table_rows[lower_index][c];
-                const float upper = (float)c_cie_table_rows[lower_index + 1][c];
-                response[c] = fmaf(upper - lower, fraction, lower);
-            }
-        }
+        cie_cmf(wavelengths[lane], response);
         for (int c = 0; c < 3; ++c) {
             /* scaled_sensor_term (Cie1931Sensor.cpp:104-124) */
             float term = 0.0f;
@@ -870,9 +1139,12 @@ BF_D void cie_xyz(const float* radiance, const float* wavelengths, float pdf, fl
     xyz[2] = sum[2];
 }
 
-__global__ __launch_bounds__(BF_BLOCK) void k_film(bf_dev_paths paths, uint32_t pixels, uint32_t passes, float pdf,
-                                                   float normalization, float* film, double* film_compensated,
-                                                   uint32_t* film_counts) {
+__global__ __launch_bounds__(BF_BLOCK) void k_film(bf_dev_paths paths, bf_dev_job J, float pdf, float normalization) {
+    const uint32_t pixels = J.pixels;
+    const uint32_t passes = J.slot_count / pixels;
+    float* film = J.film;
+    double* film_compensated = J.film_compensated;
+    uint32_t* film_counts = J.film_counts;
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
     float f[3] = {film[3 * (size_t)p], film[3 * (size_t)p + 1], film[3 * (size_t)p + 2]};
@@ -883,8 +1155,9 @@ __global__ __launch_bounds__(BF_BLOCK) void k_film(bf_dev_paths paths, uint32_t 
     }
     uint32_t count = film_counts[p];
     for (uint32_t pass = 0; pass < passes; ++pass) {
-        const size_t slot = (size_t)pass * pixels + p;
+        const size_t slot = (size_t)J.slot_begin + (size_t)pass * pixels + p;
         const float4 L4 = paths.radiance[slot];
+        if (J.tape_radiance) J.tape_radiance[path_of(J, (uint32_t)slot)] = L4;
         const float4 W4 = paths.wavelengths[slot];
         const float L[4] = {L4.x, L4.y, L4.z, L4.w};
         const float W[4] = {W4.x, W4.y, W4.z, W4.w};
@@ -933,7 +1206,7 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_sce
                                                                       uint32_t count, bf_closest_hit_result* results,
                                                                       uint32_t* counters) {
     __shared__ uint2 s_stack[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    trav_stack<BF_STACK_DEPTH> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
+    trav_stack<BF_STACK_DEPTH, BF_TRACE_BLOCK> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const bf_transport_ray r = rays[i];
@@ -996,7 +1269,7 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_sce
 __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_occlusion(bf_dev_scene S, const bf_transport_ray* rays,
                                                                     uint32_t count, bf_occlusion_result* results) {
     __shared__ uint2 s_stack[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    trav_stack<BF_STACK_DEPTH> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
+    trav_stack<BF_STACK_DEPTH, BF_TRACE_BLOCK> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const bf_transport_ray r = rays[i];
@@ -1047,9 +1320,11 @@ __global__ void k_copy_word(const uint32_t* src, uint32_t* dst) {
     if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = src[0];
 }
 
-__global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, uint32_t count, bf_path_result* out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+__global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, bf_dev_job J) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= J.slot_count) return;
+    const uint32_t i = J.slot_begin + k;
+    bf_path_result* out = J.paths + J.path_base;
     bf_path_result r;
     const float4 L = paths.radiance[i], B = paths.beta[i], W = paths.wavelengths[i];
     const uint4 D = paths.depth[i];
@@ -1069,7 +1344,7 @@ __global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, u
     r.termination = paths.status[i] & 0xFFu; /* the failure code stays in the workspace report */
     r.depth_counters = D.x;
     r.eta_scale = __uint_as_float(D.w);
-    out[i] = r;
+    out[k] = r;
 }
 
 /* ---- host-side launch helpers (stream-ordered, never synchronising) -------------------------- */
@@ -1077,80 +1352,153 @@ static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) 
 
 extern "C" {
 
-hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, hipStream_t s) {
+hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, bool batch, bool adjoint,
+                      hipStream_t s) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_camera, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *P, count, ray_queue);
+    const dim3 grid(blocks_for(count, BF_BLOCK)), block(BF_BLOCK);
+    if (batch && adjoint) hipLaunchKernelGGL((k_camera<true, true>), grid, block, 0, s, *P, count, ray_queue);
+    else if (batch) hipLaunchKernelGGL((k_camera<true, false>), grid, block, 0, s, *P, count, ray_queue);
+    else if (adjoint) hipLaunchKernelGGL((k_camera<false, true>), grid, block, 0, s, *P, count, ray_queue);
+    else hipLaunchKernelGGL((k_camera<false, false>), grid, block, 0, s, *P, count, ray_queue);
     return hipGetLastError();
 }
 
 /* Workgroups of one persistent traversal launch: enough to fill every CU at the kernels'
  * occupancy, never more than the workspace's spill area covers, and no more than the items need. */
-static uint32_t trace_blocks(const bf_transport_params* P, uint32_t max_count) {
-    const uint32_t needed = blocks_for((max_count + 63u) / 64u * 64u, BF_TRACE_BLOCK);
-    const uint32_t resident = P->spill_stride / BF_TRACE_BLOCK;
+static uint32_t trace_blocks(const bf_transport_params* P, uint32_t max_count, uint32_t block) {
+    const uint32_t needed = blocks_for((max_count + 63u) / 64u * 64u, block);
+    const uint32_t resident = P->spill_stride / block;
     return needed < resident ? needed : resident;
 }
 
-hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
-                             uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s) {
-    if (max_count == 0) return hipSuccess;
-    const dim3 grid(trace_blocks(P, max_count));
-    if (counters)
-        hipLaunchKernelGGL(k_trace_closest<true>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
+} /* extern "C" */
+
+template <bool ANY, bool COUNT, bool BATCH, uint32_t BLOCK>
+static void launch_trace(const bf_transport_params* P, uint32_t max_count, const uint32_t* q, const uint32_t* n,
+                         uint8_t* route, unsigned long long* counters, hipStream_t s) {
+    const dim3 grid(trace_blocks(P, max_count, BLOCK));
+    if (ANY)
+        hipLaunchKernelGGL((k_trace_shadow<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, route, counters);
     else
-        hipLaunchKernelGGL(k_trace_closest<false>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
-    return hipGetLastError();
+        hipLaunchKernelGGL((k_trace_closest<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, route, counters);
 }
 
-uint32_t bfk_trace_resident_blocks(int device) {
-    int cus = 0, per_cu = 0, m = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) return 0;
-    const void* kernels[4] = {(const void*)k_trace_closest<false>, (const void*)k_trace_closest<true>,
-                              (const void*)k_trace_shadow<false>, (const void*)k_trace_shadow<true>};
-    for (const void* k : kernels) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, BF_TRACE_BLOCK, 0) != hipSuccess) return 0;
-        if (per_cu > m) m = per_cu;
-    }
-    return (uint32_t)(cus * m);
+template <bool ANY, bool COUNT, bool BATCH>
+static void launch_trace_block(const bf_transport_params* P, uint32_t max_count, const uint32_t* q, const uint32_t* n,
+                               uint8_t* route, unsigned long long* counters, hipStream_t s) {
+    if (P->trace_block == 64u) launch_trace<ANY, COUNT, BATCH, 64u>(P, max_count, q, n, route, counters, s);
+    else if (P->trace_block == 256u) launch_trace<ANY, COUNT, BATCH, 256u>(P, max_count, q, n, route, counters, s);
+    else launch_trace<ANY, COUNT, BATCH, 128u>(P, max_count, q, n, route, counters, s);
 }
 
-hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
-                     uint8_t* route, uint32_t* sample_counts, hipStream_t s) {
+template <bool ANY>
+static hipError_t trace_stage(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
+                              uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s) {
     if (max_count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_shade, dim3(blocks_for(max_count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *P, queue, count_ptr, route,
-                       sample_counts);
+    if (counters) {
+        if (batch) launch_trace_block<ANY, true, true>(P, max_count, queue, count_ptr, route, counters, s);
+        else launch_trace_block<ANY, true, false>(P, max_count, queue, count_ptr, route, counters, s);
+    } else {
+        if (batch) launch_trace_block<ANY, false, true>(P, max_count, queue, count_ptr, route, counters, s);
+        else launch_trace_block<ANY, false, false>(P, max_count, queue, count_ptr, route, counters, s);
+    }
     return hipGetLastError();
+}
+
+template <uint32_t BLOCK>
+static void trace_kernels(const void** out) {
+    out[0] = (const void*)k_trace_closest<false, false, BLOCK>;
+    out[1] = (const void*)k_trace_closest<true, false, BLOCK>;
+    out[2] = (const void*)k_trace_closest<false, true, BLOCK>;
+    out[3] = (const void*)k_trace_closest<true, true, BLOCK>;
+    out[4] = (const void*)k_trace_shadow<false, false, BLOCK>;
+    out[5] = (const void*)k_trace_shadow<true, false, BLOCK>;
+    out[6] = (const void*)k_trace_shadow<false, true, BLOCK>;
+    out[7] = (const void*)k_trace_shadow<true, true, BLOCK>;
+}
+
+extern "C" {
+
+hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
+                             uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s) {
+    return trace_stage<false>(P, queue, count_ptr, max_count, route, counters, batch, s);
 }
 
 hipError_t bfk_trace_shadow(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
-                            uint32_t max_count, uint8_t* route, unsigned long long* counters, hipStream_t s) {
+                            uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s) {
+    return trace_stage<true>(P, queue, count_ptr, max_count, route, counters, batch, s);
+}
+
+static int max_resident(const void* const* kernels, int count, uint32_t block, int device) {
+    int cus = 0, per_cu = 0, m = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) return 0;
+    for (int k = 0; k < count; ++k) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernels[k], (int)block, 0) != hipSuccess) return 0;
+        if (per_cu > m) m = per_cu;
+    }
+    return cus * m;
+}
+
+/* Lanes the device holds at once for the widest-resident traversal kernel variant: the width of
+ * the spill area, one column per lane. */
+uint32_t bfk_trace_resident_lanes(int device) {
+    const void* k64[8];
+    const void* k128[8];
+    const void* k256[8];
+    trace_kernels<64u>(k64);
+    trace_kernels<128u>(k128);
+    trace_kernels<256u>(k256);
+    const uint32_t a = (uint32_t)max_resident(k64, 8, 64u, device) * 64u;
+    const uint32_t b = (uint32_t)max_resident(k128, 8, 128u, device) * 128u;
+    const uint32_t c = (uint32_t)max_resident(k256, 8, 256u, device) * 256u;
+    if (a == 0 || b == 0 || c == 0) return 0;
+    return std::max(a, std::max(b, c));
+}
+
+uint32_t bfk_shade_resident_blocks(int device) {
+    const void* kernels[4] = {(const void*)k_shade<false, false>, (const void*)k_shade<true, false>,
+                              (const void*)k_shade<false, true>, (const void*)k_shade<true, true>};
+    return (uint32_t)max_resident(kernels, 4, BF_BLOCK, device);
+}
+
+hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
+                     uint8_t* route, uint32_t* sample_counts, bool batch, bool adjoint, hipStream_t s) {
     if (max_count == 0) return hipSuccess;
-    const dim3 grid(trace_blocks(P, max_count));
-    if (counters)
-        hipLaunchKernelGGL(k_trace_shadow<true>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
+    const uint32_t needed = blocks_for(max_count, BF_BLOCK);
+    const dim3 grid(needed < P->shade_blocks ? needed : P->shade_blocks), block(BF_BLOCK);
+    if (batch && adjoint)
+        hipLaunchKernelGGL((k_shade<true, true>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+    else if (batch)
+        hipLaunchKernelGGL((k_shade<true, false>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+    else if (adjoint)
+        hipLaunchKernelGGL((k_shade<false, true>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
     else
-        hipLaunchKernelGGL(k_trace_shadow<false>, grid, dim3(BF_TRACE_BLOCK), 0, s, *P, queue, count_ptr, route, counters);
+        hipLaunchKernelGGL((k_shade<false, false>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
     return hipGetLastError();
 }
 
-uint32_t bfk_compact_blocks(uint32_t max_count) { return blocks_for(max_count ? max_count : 1u, BF_BLOCK); }
+uint32_t bfk_compact_tiles(uint32_t max_count) { return blocks_for(max_count ? max_count : 1u, BF_BLOCK); }
 
 hipError_t bfk_compact(const uint8_t* route, const uint32_t* count_ptr, uint32_t max_count, const uint32_t* src,
-                       uint32_t* block_counts, uint32_t* block_offsets, uint32_t* dst1, uint32_t* size1, uint32_t append1,
+                       uint32_t* tile_counts, uint32_t* tile_offsets, uint32_t* dst1, uint32_t* size1, uint32_t append1,
                        uint32_t* dst2, uint32_t* size2, uint32_t append2, hipStream_t s) {
-    const uint32_t blocks = bfk_compact_blocks(max_count);
-    hipLaunchKernelGGL(k_compact_count, dim3(blocks), dim3(BF_BLOCK), 0, s, route, count_ptr, block_counts);
-    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, s, block_counts, blocks, block_offsets, size1, size2,
+    const uint32_t tiles = bfk_compact_tiles(max_count);
+    const dim3 grid(tiles < 4096u ? tiles : 4096u);
+    hipLaunchKernelGGL(k_compact_count, grid, dim3(BF_BLOCK), 0, s, route, count_ptr, tile_counts);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(1024), 0, s, count_ptr, tile_counts, tile_offsets, size1, size2,
                        append1, append2);
-    hipLaunchKernelGGL(k_compact_scatter, dim3(blocks), dim3(BF_BLOCK), 0, s, route, count_ptr, src, block_offsets, dst1,
-                       dst2);
+    hipLaunchKernelGGL(k_compact_scatter, grid, dim3(BF_BLOCK), 0, s, route, count_ptr, src, tile_offsets, dst1, dst2);
     return hipGetLastError();
 }
 
-hipError_t bfk_film(const bf_dev_paths* paths, uint32_t pixels, uint32_t passes, float pdf, float normalization,
-                    float* film, double* film_compensated, uint32_t* film_counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_film, dim3(blocks_for(pixels, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, pixels, passes, pdf,
-                       normalization, film, film_compensated, film_counts);
+hipError_t bfk_bounce_end(uint32_t* sizes, uint32_t* history, uint32_t history_bounces, hipStream_t s) {
+    hipLaunchKernelGGL(k_bounce_end, dim3(1), dim3(64), 0, s, sizes, history, history_bounces);
+    return hipGetLastError();
+}
+
+hipError_t bfk_film(const bf_dev_paths* paths, const bf_dev_job* job, float pdf, float normalization, hipStream_t s) {
+    hipLaunchKernelGGL(k_film, dim3(blocks_for(job->pixels, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, *job, pdf,
+                       normalization);
     return hipGetLastError();
 }
 
@@ -1198,9 +1546,9 @@ hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t bfk_export_paths(const bf_dev_paths* paths, uint32_t count, bf_path_result* out, hipStream_t s) {
-    if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_export_paths, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, count, out);
+hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s) {
+    if (job->slot_count == 0 || !job->paths) return hipSuccess;
+    hipLaunchKernelGGL(k_export_paths, dim3(blocks_for(job->slot_count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, *job);
     return hipGetLastError();
 }
 

@@ -248,41 +248,41 @@ BF_D bool precedes(float t, uint32_t inst, uint32_t prim, const trav_hit& c) {
  * finds them full, the lower half moves to the lane's column of a global spill area and comes
  * back when the LDS part runs empty. With S = BF_STACK_DEPTH nothing ever spills.
  */
-template <uint32_t S>
+template <uint32_t S, uint32_t LANES>
 struct trav_stack {
-    uint2* lds;             /* entry k at lds[k * BF_TRACE_BLOCK] */
+    uint2* lds;             /* entry k at lds[k * LANES] (LANES = lanes of the workgroup) */
     uint2* spill;           /* entry k at spill[k * spill_stride] */
     uint32_t spill_stride;
     uint32_t sp, gsp;       /* entries in LDS, entries spilled */
 };
 
-template <uint32_t S>
-BF_D uint32_t stack_depth(const trav_stack<S>& s) { return s.sp + s.gsp; }
+template <uint32_t S, uint32_t LANES>
+BF_D uint32_t stack_depth(const trav_stack<S, LANES>& s) { return s.sp + s.gsp; }
 
-template <uint32_t S>
-BF_D void stack_push(trav_stack<S>& s, uint32_t ref, float nearv) {
+template <uint32_t S, uint32_t LANES>
+BF_D void stack_push(trav_stack<S, LANES>& s, uint32_t ref, float nearv) {
     if (S < BF_STACK_DEPTH && s.sp == S) {
 #pragma unroll
-        for (uint32_t k = 0; k < S / 2; ++k) s.spill[(size_t)(s.gsp + k) * s.spill_stride] = s.lds[k * BF_TRACE_BLOCK];
+        for (uint32_t k = 0; k < S / 2; ++k) s.spill[(size_t)(s.gsp + k) * s.spill_stride] = s.lds[k * LANES];
 #pragma unroll
-        for (uint32_t k = 0; k < S / 2; ++k) s.lds[k * BF_TRACE_BLOCK] = s.lds[(k + S / 2) * BF_TRACE_BLOCK];
+        for (uint32_t k = 0; k < S / 2; ++k) s.lds[k * LANES] = s.lds[(k + S / 2) * LANES];
         s.gsp += S / 2;
         s.sp = S / 2;
     }
-    s.lds[s.sp * BF_TRACE_BLOCK] = make_uint2(ref, __float_as_uint(nearv));
+    s.lds[s.sp * LANES] = make_uint2(ref, __float_as_uint(nearv));
     ++s.sp;
 }
 
-template <uint32_t S>
-BF_D uint2 stack_pop(trav_stack<S>& s) {
+template <uint32_t S, uint32_t LANES>
+BF_D uint2 stack_pop(trav_stack<S, LANES>& s) {
     if (S < BF_STACK_DEPTH && s.sp == 0u) {
         s.gsp -= S / 2;
 #pragma unroll
-        for (uint32_t k = 0; k < S / 2; ++k) s.lds[k * BF_TRACE_BLOCK] = s.spill[(size_t)(s.gsp + k) * s.spill_stride];
+        for (uint32_t k = 0; k < S / 2; ++k) s.lds[k * LANES] = s.spill[(size_t)(s.gsp + k) * s.spill_stride];
         s.sp = S / 2;
     }
     --s.sp;
-    return s.lds[s.sp * BF_TRACE_BLOCK];
+    return s.lds[s.sp * LANES];
 }
 
 /*
@@ -292,8 +292,8 @@ BF_D uint2 stack_pop(trav_stack<S>& s) {
  * live through the loop. The winning candidate keeps (t, ids, leaf); its barycentrics are
  * re-derived by closest_hit_barycentrics from the same deterministic triangle test.
  */
-template <bool ANY, bool COUNT, uint32_t S, typename WorldRay>
-BF_D uint32_t traverse(const bf_dev_scene& S_, const trav_ray& ray, WorldRay world, trav_stack<S>& stack, trav_hit& best,
+template <bool ANY, bool COUNT, uint32_t S, uint32_t LANES, typename WorldRay>
+BF_D uint32_t traverse(const bf_dev_scene& S_, const trav_ray& ray, WorldRay world, trav_stack<S, LANES>& stack, trav_hit& best,
                        bool& occluded, trav_counters& ctr) {
     best.present = false;
     best.t = ray.t_max;

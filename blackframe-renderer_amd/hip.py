@@ -55,6 +55,13 @@ def library():
     lib.blackframe_hip_launch_sample_stream_dump.argtypes = [u32, vp, vp, u32, vp, vp]
     lib.blackframe_hip_workspace_create.argtypes = [u64, C.POINTER(vp)]
     lib.blackframe_hip_workspace_destroy.argtypes = [vp]
+    lib.blackframe_hip_trace_wavefront_batch.argtypes = [vp, C.POINTER(abi.FrameJob), u32, C.POINTER(abi.TransportOptions),
+                                                         C.POINTER(abi.TransportReport), vp, vp]
+    lib.blackframe_hip_trace_wavefront_adjoint.argtypes = [vp, C.POINTER(abi.FrameJob), C.POINTER(abi.AdjointJob), u32,
+                                                           C.POINTER(abi.TransportOptions), C.POINTER(abi.TransportReport),
+                                                           vp]
+    lib.blackframe_hip_autotune_launch.argtypes = [vp, C.POINTER(abi.FrameJob), u32, C.POINTER(abi.TransportOptions), u32,
+                                                   C.POINTER(abi.LaunchTuning), vp]
     lib.blackframe_query_extension.restype = C.c_uint32
     lib.blackframe_query_extension.argtypes = [C.POINTER(abi.ExtensionQuery), C.POINTER(abi.ExtensionApi)]
     lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
@@ -83,6 +90,8 @@ class XpuScene:
         self._lib = library()
         self.handle = C.c_void_p()
         self.scene = scene  # keeps host arrays alive for the duration of the build
+        # texel counts per texture slot (gradient buffers must match them)
+        self.texture_sizes = [w * h * c for (w, h, c, _, _) in getattr(scene, "textures", [])]
         create = self._lib.blackframe_hip_scene_create if upload else self._lib.blackframe_hip_scene_prepare
         _check(create(C.byref(scene.desc), C.byref(self.handle)))
 
@@ -199,14 +208,36 @@ class Film:
                 "film_counts": self.counts.cpu().numpy().view(np.uint32).reshape(self.height, self.width)}
 
 
+class Tape:
+    """Per-path record of a forward render for the adjoint: final spectral radiance and the bit
+    mask of light samples that reached their light, in frame path order (device memory)."""
+
+    def __init__(self, path_count):
+        import torch
+
+        self.path_count = path_count
+        self.radiance = torch.zeros(path_count * 4, dtype=torch.float32, device="cuda")
+        self.visibility = torch.zeros(path_count, dtype=torch.int64, device="cuda")
+
+
 def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, want_paths=False, want_counters=False,
-                                  stream=None, synchronize=True):
+                                  stream=None, synchronize=True, tape=None):
     """Renders one frame; mirrors engine::trace_cuda_wavefront_transport (WavefrontTransport.hpp:197-208).
 
     want_paths: False, True (host ctypes array of PathResult) or "device" (a torch uint8 tensor of
     n_paths * 64 bytes left in device memory, for frames whose results are only sampled)."""
     import torch
 
+    if tape is not None:
+        # the single-frame entry point has the reference's signature; taped frames use the batch one
+        r = trace_xpu_wavefront_batch(workspace, [(scene, frame, film, tape)], options, want_paths=want_paths,
+                                      want_counters=want_counters, stream=stream, synchronize=synchronize)
+        out = {"film": r["films"][0], "report": r["report"]}
+        if want_counters:
+            out["traversal_counters"] = r["traversal_counters"]
+        if want_paths:
+            out["paths"] = r["paths"][0]
+        return out
     cam = frame.camera
     film = film if film is not None else Film(cam.width, cam.height)
     n_paths = cam.width * cam.height * frame.samples_per_pixel
@@ -229,6 +260,104 @@ def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, w
     elif want_paths:
         result["paths"] = (abi.PathResult * n_paths).from_buffer_copy(d_paths.cpu().numpy().tobytes())
     return result
+
+
+def trace_xpu_wavefront_batch(workspace, items, options, want_paths=False, want_counters=False, stream=None,
+                              synchronize=True):
+    """Renders many frames through one set of queues (blackframe_hip_trace_wavefront_batch).
+
+    items: sequence of (XpuScene, FrameDesc[, Film[, Tape]]). Returns {"films": [...], "report": ...,
+    "paths": [...] (per frame, as in trace_xpu_wavefront_transport), "traversal_counters": ...}."""
+    import torch
+
+    jobs = (abi.FrameJob * len(items))()
+    films, path_buffers = [], []
+    for k, item in enumerate(items):
+        scene, frame = item[0], item[1]
+        cam = frame.camera
+        film = item[2] if len(item) > 2 and item[2] is not None else Film(cam.width, cam.height)
+        films.append(film)
+        n_paths = cam.width * cam.height * frame.samples_per_pixel
+        d_paths = torch.zeros(n_paths * C.sizeof(abi.PathResult), dtype=torch.uint8, device="cuda") if want_paths else None
+        path_buffers.append((d_paths, n_paths))
+        jobs[k].scene = scene.handle.value
+        jobs[k].frame = frame
+        jobs[k].film = film.float_sum.data_ptr()
+        jobs[k].film_compensated = film.compensated.data_ptr()
+        jobs[k].film_counts = film.counts.data_ptr()
+        jobs[k].paths = d_paths.data_ptr() if want_paths else None
+        tape = item[3] if len(item) > 3 else None
+        if tape is not None:
+            if tape.path_count != n_paths:
+                raise XpuError(1, f"tape holds {tape.path_count} paths, frame {k} has {n_paths}")
+            jobs[k].tape_radiance = tape.radiance.data_ptr()
+            jobs[k].tape_visibility = tape.visibility.data_ptr()
+    report = abi.TransportReport()
+    counters = np.zeros(6, dtype=np.uint64)
+    _check(library().blackframe_hip_trace_wavefront_batch(
+        workspace.handle, jobs, len(items), C.byref(options), C.byref(report),
+        C.c_void_p(counters.ctypes.data) if want_counters else None, _stream_handle(stream)))
+    if synchronize:
+        torch.cuda.synchronize()
+    result = {"films": films, "report": report}
+    if want_counters:
+        result["traversal_counters"] = counters
+    if want_paths:
+        result["paths"] = [(abi.PathResult * n).from_buffer_copy(d.cpu().numpy().tobytes()) for d, n in path_buffers]
+    return result
+
+
+def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchronize=True):
+    """Path-replay adjoint (blackframe_hip_trace_wavefront_adjoint).
+
+    items: sequence of (XpuScene, FrameDesc, Tape, film_gradient, texture_gradients) with
+    film_gradient a float32 device tensor of dLoss/d(film RGB sums) [pixels * 3] and
+    texture_gradients a sequence (per scene texture slot) of float32 device tensors or None;
+    gradients are accumulated into them. Returns the report."""
+    import torch
+
+    jobs = (abi.FrameJob * len(items))()
+    adjoints = (abi.AdjointJob * len(items))()
+    for k, (scene, frame, tape, film_gradient, texture_gradients) in enumerate(items):
+        cam = frame.camera
+        if film_gradient.dtype != torch.float32 or film_gradient.numel() != cam.width * cam.height * 3:
+            raise XpuError(1, f"frame {k}: film gradient must be float32 [{cam.width * cam.height * 3}]")
+        if tape.path_count != cam.width * cam.height * frame.samples_per_pixel:
+            raise XpuError(1, f"frame {k}: tape holds {tape.path_count} paths")
+        jobs[k].scene = scene.handle.value
+        jobs[k].frame = frame
+        jobs[k].tape_radiance = tape.radiance.data_ptr()
+        jobs[k].tape_visibility = tape.visibility.data_ptr()
+        adjoints[k].film_gradient = film_gradient.data_ptr()
+        gradients = list(texture_gradients or [])
+        if len(gradients) > len(scene.texture_sizes):
+            raise XpuError(1, f"frame {k}: {len(gradients)} gradient buffers for {len(scene.texture_sizes)} textures")
+        for slot, g in enumerate(gradients):
+            if g is not None:
+                if g.dtype != torch.float32 or not g.is_contiguous() or g.numel() != scene.texture_sizes[slot]:
+                    raise XpuError(1, f"frame {k}: texture gradient {slot} must be contiguous float32 "
+                                      f"[{scene.texture_sizes[slot]}]")
+                adjoints[k].texture_gradients[slot] = g.data_ptr()
+    report = abi.TransportReport()
+    _check(library().blackframe_hip_trace_wavefront_adjoint(workspace.handle, jobs, adjoints, len(items), C.byref(options),
+                                                            C.byref(report), _stream_handle(stream)))
+    if synchronize:
+        torch.cuda.synchronize()
+    return report
+
+
+def autotune_launch(workspace, items, options, trial_passes=1, stream=None):
+    """Times launch shapes on the first sample passes of (XpuScene, FrameDesc) items
+    (blackframe_hip_autotune_launch); returns the LaunchTuning record. Set options.launch to
+    its .launch to use the choice."""
+    jobs = (abi.FrameJob * len(items))()
+    for k, (scene, frame) in enumerate(items):
+        jobs[k].scene = scene.handle.value
+        jobs[k].frame = frame
+    tuning = abi.LaunchTuning()
+    _check(library().blackframe_hip_autotune_launch(workspace.handle, jobs, len(items), C.byref(options), trial_passes,
+                                                    C.byref(tuning), _stream_handle(stream)))
+    return tuning
 
 
 def sample_stream_dump(sampler, indices, dimensions, stream=None):

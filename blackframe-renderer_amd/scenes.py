@@ -12,6 +12,9 @@ Workloads:
   icosphere_grid(...)   32x32 grid of seeded displaced icospheres (subdivision 3, 1280 triangles
                         each, 1.31 M triangles), per-object Lambert / GGX conductor / GGX dielectric,
                         64 emissive triangles and a constant environment.
+  textured_cornell(...) the Cornell box with a seeded RGB albedo texture (U[0.1, 0.9]) on the back
+                        wall and a GGX conductor box whose widths come from a roughness texture
+                        (alpha 0.3): the differentiable workload of the adjoint.
 """
 
 import ctypes as C
@@ -67,6 +70,7 @@ class Scene:
         self._positions, self._normals, self._uvs, self._indices = [], [], [], []
         self.geometries, self.instances, self.materials, self.closures = [], [], [], []
         self.spectra, self.spectrum_values = [], []
+        self.textures = []
         self.vertex_count = 0
         self.triangle_count = 0
         self.environment_spectrum = abi.INVALID
@@ -83,6 +87,21 @@ class Scene:
         if len(values) == 1:
             self._constant_cache[values[0]] = sid
         return sid
+
+    # -- textures -----------------------------------------------------------------------------
+    def add_texture(self, texels=None, device=None):
+        """A float32 image, (height, width, channels) host texels (copied into the scene) or
+        device=(pointer, width, height, channels) read in place by every render. Returns the slot."""
+        if texels is not None:
+            texels = np.ascontiguousarray(texels, dtype=np.float32)
+            if texels.ndim == 2:
+                texels = texels[:, :, None]
+            height, width, channels = texels.shape
+            self.textures.append((width, height, channels, texels, None))
+        else:
+            pointer, width, height, channels = device
+            self.textures.append((width, height, channels, None, int(pointer)))
+        return len(self.textures) - 1
 
     # -- geometry -----------------------------------------------------------------------------
     def add_geometry(self, gid, positions, normals, uvs, indices):
@@ -114,6 +133,8 @@ class Scene:
             rec.k_spectrum = self.spectrum(c["k"]) if c.get("k") is not None else abi.INVALID
             for i, p in enumerate(c.get("params", [])):
                 rec.parameters[i] = p
+            rec.weight_texture = c["weight_texture"] + 1 if c.get("weight_texture") is not None else 0
+            rec.roughness_texture = c["roughness_texture"] + 1 if c.get("roughness_texture") is not None else 0
             self.closures.append(rec)
             probabilities.append(float(c.get("probability", 1.0)))
         m = abi.MaterialDesc()
@@ -189,6 +210,14 @@ class Scene:
         d.spectra = C.addressof(self._spectrum_array)
         d.spectrum_values = self._spectrum_values.ctypes.data
         d.environment_spectrum = self.environment_spectrum
+        self._texture_array = (abi.TextureDesc * max(1, len(self.textures)))()
+        for i, (width, height, channels, texels, pointer) in enumerate(self.textures):
+            t = self._texture_array[i]
+            t.width, t.height, t.channels = width, height, channels
+            t.data = texels.ctypes.data if texels is not None else None
+            t.device_data = pointer
+        d.texture_count = len(self.textures)
+        d.textures = C.addressof(self._texture_array) if self.textures else None
         self.desc = d
         return self
 
@@ -371,6 +400,64 @@ def cornell(width=64, height=64, materials="lambert"):
     return s, camera
 
 
+def textured_cornell(width=256, height=256, texture_size=512, seed=0xC3_7E57, albedo=None, roughness=None,
+                     roughness_size=1, mixed=False):
+    """The differentiable Cornell box: the back wall's reflectance is an RGB texture (seeded
+    U[0.1, 0.9] texels unless `albedo` is given, host (h, w, 3) array or device tuple) and the tall
+    box is a gold GGX conductor whose widths come from a 1-channel roughness texture (0.3 unless
+    `roughness` is given). mixed=True adds every other binding: the back wall becomes a textured
+    Lambert + copper GGX mixture, the left wall an Oren-Nayar surface with a grey texture (slot 2)
+    and the short box a GGX dielectric with a 2-channel (alpha_x, alpha_y) texture (slot 3).
+    Returns (scene, camera, textures) with textures the host arrays used."""
+    rng = SplitMix64(seed)
+    s = Scene()
+    if albedo is None:
+        values = np.array([rng.uniform() for _ in range(texture_size * texture_size * 3)])
+        albedo = (0.1 + 0.8 * values).reshape(texture_size, texture_size, 3).astype(np.float32)
+    if roughness is None:
+        roughness = np.full((roughness_size, roughness_size, 1), 0.3, dtype=np.float32)
+    albedo_slot = s.add_texture(albedo) if isinstance(albedo, np.ndarray) else s.add_texture(device=albedo)
+    rough_slot = s.add_texture(roughness) if isinstance(roughness, np.ndarray) else s.add_texture(device=roughness)
+    white = s.add_material(21, [lambert(0.73)])
+    left = s.add_material(22, [lambert(0.12)])
+    right = s.add_material(23, [lambert(0.45)])
+    emitter = s.add_material(24, [lambert(0.0)], emission=17.0)
+    back = s.add_material(28, [dict(lambert(1.0), weight_texture=albedo_slot)])
+    short_material = white
+    textures = {"albedo": albedo, "roughness": roughness}
+    if mixed:
+        grey = (0.2 + 0.6 * np.array([rng.uniform() for _ in range(16)])).reshape(4, 4, 1).astype(np.float32)
+        widths = (0.15 + 0.3 * np.array([rng.uniform() for _ in range(32)])).reshape(4, 4, 2).astype(np.float32)
+        textures.update(grey=grey, widths=widths)
+        grey_slot = s.add_texture(grey)
+        widths_slot = s.add_texture(widths)
+        back = s.add_material(28, [dict(lambert(1.0, probability=0.7), weight_texture=albedo_slot),
+                                   rough_conductor(0.8, COPPER_ETA, COPPER_K, 0.15, 0.4, probability=0.3)])
+        left = s.add_material(27, [dict(rough_diffuse(0.9, 0.5), weight_texture=grey_slot)])
+        short_material = s.add_material(25, [dict(rough_dielectric(1.0, 1.0, 1.5, 0.2), roughness_texture=widths_slot)])
+    walls = [
+        (11, quad([[-1, -1, 0], [-1, -1, 2], [1, -1, 2], [1, -1, 0]], [0, 1, 0]), white),
+        (12, quad([[-1, 1, 0], [1, 1, 0], [1, 1, 2], [-1, 1, 2]], [0, -1, 0]), white),
+        (13, quad([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], [0, 0, 1]), back),
+        (14, quad([[-1, -1, 0], [-1, 1, 0], [-1, 1, 2], [-1, -1, 2]], [1, 0, 0]), left),
+        (15, quad([[1, -1, 0], [1, -1, 2], [1, 1, 2], [1, 1, 0]], [-1, 0, 0]), right),
+        (16, quad([[-0.32, 0.985, 0.62], [0.32, 0.985, 0.62], [0.32, 0.985, 1.18], [-0.32, 0.985, 1.18]],
+                  [0, -1, 0]), emitter),
+    ]
+    for k, (gid, shape, material) in enumerate(walls):
+        g = s.add_geometry(gid, *shape)
+        s.add_instance(31 + k, 1 + k, g, material)
+    box = s.add_geometry(17, *unit_box())
+    short = affine(rotation_y(math.radians(15.0)) @ np.diag([0.55, 0.55, 0.55]), (0.36, -0.725, 1.30))
+    tall = affine(rotation_y(math.radians(-18.0)) @ np.diag([0.55, 1.15, 0.55]), (-0.38, -0.425, 0.72))
+    tall_material = s.add_material(26, [dict(rough_conductor(1.0, GOLD_ETA, GOLD_K, 0.3), roughness_texture=rough_slot)])
+    s.add_instance(37, 7, box, short_material, short)
+    s.add_instance(38, 8, box, tall_material, tall)
+    s.finalize()
+    camera = look_at_camera((0.0, 0.0, 4.0), (0.0, -0.05, 0.92), width, height, 0.70)
+    return s, camera, textures
+
+
 def _icosphere(subdivisions):
     t = (1.0 + math.sqrt(5.0)) / 2.0
     verts = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
@@ -459,3 +546,21 @@ def icosphere_grid(width=1920, height=1080, grid=32, subdivisions=3, seed=0x5EED
     extent = spacing * grid / 2.0
     camera = look_at_camera((0.0, extent * 1.3, extent * 0.75), (0.0, -2.0, -extent * 0.1), width, height, 0.9)
     return s, camera
+
+
+def batch_scenes(count=32, seed=0xC2_5EED, min_triangles=1.0e4, max_triangles=5.0e5):
+    """Seeded batch of uneven scenes: triangle counts log-uniform in [min, max] (icosphere grids of
+    subdivision 3, 1 280 triangles per object, same material and light generator as
+    icosphere_grid), square resolutions 2^U{7..10} and spp 2^U{2..8}. Returns a list of
+    (Scene, camera, spp)."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        triangles = math.exp(math.log(min_triangles) + rng.uniform() * (math.log(max_triangles) - math.log(min_triangles)))
+        grid = max(1, int(round(math.sqrt(triangles / 1280.0))))
+        resolution = 1 << (7 + min(3, int(rng.uniform() * 4)))
+        spp = 1 << (2 + min(6, int(rng.uniform() * 7)))
+        scene, camera = icosphere_grid(resolution, resolution, grid=grid, subdivisions=3,
+                                       seed=rng.next_u64() & 0xFFFFFFFF)
+        out.append((scene, camera, spp))
+    return out

@@ -107,6 +107,86 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* workspace, const 
                                              bf_path_result* paths, bf_transport_report* report,
                                              uint64_t* traversal_counters, void* stream);
 
+/*
+ * One frame of a batch: a scene (on the current device), its frame description and its device
+ * film / optional per-path outputs, with the meaning they have in
+ * blackframe_hip_trace_wavefront_transport.
+ */
+typedef struct bf_frame_job {
+    const bf_hip_scene* scene;
+    bf_frame_desc frame;
+    float* film;
+    double* film_compensated;
+    uint32_t* film_counts;
+    bf_path_result* paths;
+    /* Optional tape for blackframe_hip_trace_wavefront_adjoint (both or neither), device memory
+     * in frame path order (path = pass * pixels + pixel): */
+    float* tape_radiance;      /* [paths * 4] final spectral radiance of each path */
+    uint64_t* tape_visibility; /* [paths] bit d: the shadow ray of path vertex d reached its light */
+} bf_frame_job;
+
+/*
+ * Renders many frames of (possibly) different scenes through one set of queues: every launch of
+ * the bounce loop carries paths of all frames that fit the workspace (whole sample passes, in
+ * job order), so small frames share the device instead of running one after another. Each
+ * frame's film, paths and failure semantics are exactly those of a separate
+ * blackframe_hip_trace_wavefront_transport call with the same options; the report aggregates
+ * the batch (per-bounce queue fingerprints are recorded for the first launch only).
+ * The reference has no batched entry point: its callers loop trace_cuda_wavefront_transport
+ * (WavefrontTransport.hpp:197-208) over frames, which this call replaces.
+ */
+int blackframe_hip_trace_wavefront_batch(bf_hip_workspace* workspace, const bf_frame_job* jobs, uint32_t job_count,
+                                         const bf_transport_options* options, bf_transport_report* report,
+                                         uint64_t* traversal_counters, void* stream);
+
+/*
+ * Gradient request of one frame for blackframe_hip_trace_wavefront_adjoint.
+ * film_gradient holds dLoss/d(film RGB sums), [pixels * 3], the layout of the float film.
+ * texture_gradients[k] receives dLoss/d(texels of scene texture k) in the texture's layout (it must
+ * hold width * height * channels floats) and is ACCUMULATED (+=) with float atomics in an
+ * unspecified order; null entries are skipped.
+ */
+typedef struct bf_adjoint_job {
+    const float* film_gradient;
+    float* texture_gradients[BF_MAX_TEXTURES];
+} bf_adjoint_job;
+
+/*
+ * Path-replay adjoint of blackframe_hip_trace_wavefront_batch (no counterpart in the reference,
+ * which is forward-only). jobs[i] must describe the frame exactly as it was rendered with a tape
+ * (same scene and texture values, frame, options); its film pointers are not used. Each path is
+ * replayed from its sample stream index through the same BVH decisions; shadow visibility and the
+ * final radiance come from the tape, so no shadow ray is traced. Sampling, MIS weights and
+ * Russian roulette are treated as constants (detached); derivatives flow through the textured
+ * closure weights and GGX widths. Requires diffuse + glossy + specular depth limits <= 63.
+ */
+int blackframe_hip_trace_wavefront_adjoint(bf_hip_workspace* workspace, const bf_frame_job* jobs,
+                                           const bf_adjoint_job* adjoints, uint32_t job_count,
+                                           const bf_transport_options* options, bf_transport_report* report,
+                                           void* stream);
+
+/* Result of blackframe_hip_autotune_launch. */
+typedef struct bf_launch_tuning {
+    uint32_t launch;         /* the fastest bf_transport_options.launch word found */
+    uint32_t candidates;     /* launch words timed */
+    float best_ms;           /* trial time of `launch` */
+    float default_ms;        /* trial time of launch = 0 (the library default) */
+    uint32_t tried[24];      /* the candidates, in the order timed */
+    float tried_ms[24];
+} bf_launch_tuning;
+
+/*
+ * Launch autotuner (no reference counterpart: the reference launches fixed shapes). Times the
+ * transport of `jobs` restricted to their first `trial_passes` sample passes (into scratch films
+ * the call allocates; the jobs' own buffers are not touched) for every combination of traversal
+ * workgroup size {64, 128, 256}, bounce group size {2, 4, 8} and graph replay {off, on} (graph
+ * replay only when `options` allow it), with hipEvents on `stream`, and returns the fastest
+ * launch word. options->launch is ignored; every other option is used as given. Synchronises.
+ */
+int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_job* jobs, uint32_t job_count,
+                                   const bf_transport_options* options, uint32_t trial_passes,
+                                   bf_launch_tuning* result, void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

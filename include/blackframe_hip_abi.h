@@ -32,6 +32,7 @@ extern "C" {
 #define BF_SPECTRUM_LANES 4u
 #define BF_MAX_CLOSURES 8u
 #define BF_CLOSURE_PARAMETERS 10u
+#define BF_MAX_TEXTURES 8u
 #define BF_INVALID_INDEX 0xFFFFFFFFu
 #define BF_TRAVERSAL_STACK_CAPACITY 64u
 
@@ -195,6 +196,15 @@ typedef struct bf_closure_record {
 /* ---- scene description (input to the host scene builder) ----------------------------------- */
 
 /* A spectrum is `count` samples on a uniform grid spanning [360, 830] nm; count == 1 is constant. */
+
+/* A float32 image: texel (x, y) channel c at data[(y * width + x) * channels + c], row 0 at v = 0.
+ * Lookups repeat in u and v and filter bilinearly between texel centres (x + 0.5) / width. */
+typedef struct bf_texture_desc {
+    uint32_t width, height, channels; /* channels 1..3 */
+    uint32_t reserved;
+    const float* data;   /* host texels, copied into the scene at creation ... */
+    float* device_data;  /* ... or caller-owned device texels, read in place by every render */
+} bf_texture_desc;
 typedef struct bf_spectrum_desc {
     uint32_t offset; /* into bf_scene_desc.spectrum_values */
     uint32_t count;
@@ -212,7 +222,15 @@ typedef struct bf_closure_desc {
     uint32_t weight_spectrum;
     uint32_t eta_spectrum;
     uint32_t k_spectrum;
-    uint32_t reserved[3];
+    /* Texture bindings, as texture slot + 1 (0: none, so zero-filled descriptors are untextured):
+     *  weight_texture     reflectance = weight_spectrum(lambda) * clamp01(sum_c t_c * B_c(lambda)),
+     *                     t = bilinear texel (3 channels: RGB; 1 channel: grey, t_r = t_g = t_b),
+     *                     B_c the RGB basis spectra of blackframe_texture.h;
+     *  roughness_texture  GGX widths of rough conductors / dielectrics (alpha_x, alpha_y) =
+     *                     (t_0, t_1) for 2 channels, (t_0, t_0) for 1 channel. */
+    uint32_t weight_texture;
+    uint32_t roughness_texture;
+    uint32_t reserved;
     float parameters[BF_CLOSURE_PARAMETERS];
     uint32_t reserved_tail[2];
 } bf_closure_desc;
@@ -263,7 +281,8 @@ typedef struct bf_scene_desc {
     const bf_spectrum_desc* spectra;
     const float* spectrum_values;
     uint32_t environment_spectrum;
-    uint32_t reserved;
+    uint32_t texture_count;                 /* <= BF_MAX_TEXTURES */
+    const bf_texture_desc* textures;
 } bf_scene_desc;
 
 /* ---- camera / render description ------------------------------------------------------------ */
@@ -294,8 +313,18 @@ typedef struct bf_transport_options {
     uint32_t record_queues; /* nonzero: fill bf_transport_report queue fingerprints */
     uint32_t instrumentation; /* bf_instrumentation_mode: BF_INSTRUMENT_STAGE_EVENTS fills stage_metrics */
     uint32_t failure_policy;  /* bf_failure_policy */
-    uint32_t reserved;
+    /* Launch shape, three fields:
+     *   bits 0-15  bounces issued between two host checks of the live-path count (0: 4);
+     *   bits 16-19 log2 of the traversal workgroup size: 6, 7 or 8 (0: 7, 128 lanes);
+     *   bit 31     BF_LAUNCH_GRAPH: each bounce group is one replayed hipGraph (size rounded up
+     *              to even). */
+    uint32_t launch;
 } bf_transport_options;
+
+#define BF_LAUNCH_GRAPH 0x80000000u
+#define BF_LAUNCH_GROUP_MASK 0x0000FFFFu
+#define BF_LAUNCH_TRACE_BLOCK_MASK 0x000F0000u
+#define BF_LAUNCH_TRACE_BLOCK(log2_lanes) ((uint32_t)(log2_lanes) << 16)
 
 /* What a numerically failed path (termination BF_TERM_FAILED) does to the call. The reference
  * aborts the batch on the first failing work item (WavefrontTransport.cpp:880-925), which is
@@ -346,7 +375,7 @@ typedef struct bf_transport_report {
     uint32_t bounces;
     uint32_t first_failure_slot;
     uint32_t first_failure_code;
-    uint32_t reserved;
+    uint32_t first_failure_job; /* frame of the first failure in a batched call */
     /* per bounce: FNV-1a-64 of the ray queue contents (slot order) and its size */
     uint64_t ray_queue_hash[BF_REPORT_MAX_BOUNCES];
     uint32_t ray_queue_size[BF_REPORT_MAX_BOUNCES];
@@ -386,6 +415,8 @@ BF_STATIC_ASSERT(sizeof(bf_bvh_blas) == 32, "SceneBvhBlas is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_instance_ref) == 32, "SceneBvhInstanceReference is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_wide_node) == 64, "wide node is one 64-byte line");
 BF_STATIC_ASSERT(sizeof(bf_closure_record) == 64, "ClosureRecord is 64 bytes");
+BF_STATIC_ASSERT(sizeof(bf_closure_desc) == 80, "closure descriptor is 80 bytes");
+BF_STATIC_ASSERT(sizeof(bf_texture_desc) == 32, "texture descriptor is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_transport_options) == 80, "transport options are 80 bytes");
 BF_STATIC_ASSERT(sizeof(bf_stage_metric) == 24, "stage metric is 24 bytes");
 BF_STATIC_ASSERT(sizeof(bf_transport_report) == 1808, "transport report is 1808 bytes");

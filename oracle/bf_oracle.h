@@ -32,6 +32,7 @@ uint64_t bfo_sobol_dimension_count(void);
 void bfo_cie_xyz(const float radiance[4], const float wavelengths[4], float pdf, float xyz[3]);
 void bfo_xyz_to_rgb(const float xyz[3], float rgb[3]);
 double bfo_cie_y_integral(void);
+void bfo_cie_cmf64(float wavelength, double out[3]);
 uint64_t bfo_cie_fingerprint(void);
 
 /* ---- scene ---------------------------------------------------------------------------------- */
@@ -73,6 +74,18 @@ int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame,
                uint32_t threads, bf_path_result* paths, float* film_float, double* film_compensated,
                uint32_t* film_counts, bf_transport_report* report, uint64_t* traversal_counters);
 
+/* Forward render that also writes the adjoint tape of each path (frame path order): final
+ * radiance [4 per path] and the bit mask of unoccluded light samples by path vertex. */
+int bfo_render_taped(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                     uint64_t path_begin, uint64_t path_count, uint32_t threads, float* film_float,
+                     double* film_compensated, uint32_t* film_counts, float* tape_radiance, uint64_t* tape_visibility);
+/* Path-replay adjoint of paths [path_begin, path_begin + path_count): texel gradients of
+ * dLoss/d(film RGB sums) = film_gradient, ACCUMULATED in binary64 into texture_gradients[k]
+ * (layout of scene texture k; NULL entries skipped). */
+int bfo_render_adjoint(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                       uint64_t path_begin, uint64_t path_count, uint32_t threads, const float* tape_radiance,
+                       const uint64_t* tape_visibility, const float* film_gradient, double* const* texture_gradients);
+
 /* ---- closure library (radiance transport) ------------------------------------------------
  * allowed: direction mask (BF_LOBE_REFLECTION / TRANSMISSION bits) after depth filtering.
  * Return values are the lobe statuses: 0 ok, 1 invalid, 2 not representable, 3 outside support. */
@@ -91,6 +104,14 @@ int bfo_smith_lambda(float ax, float ay, const float d[3], float* out);
 int bfo_vndf_pdf(float ax, float ay, const float wo[3], const float m[3], float* out);
 int bfo_sample_vndf(float ax, float ay, const float wo[3], float u, float v, float m[3], float* pdf);
 int bfo_ggx_widths_representable(float ax, float ay);
+/* texture lookup (bilinear, repeat) and RGB-basis reflectance of blackframe_texture.h */
+int bfo_texture_sample(uint32_t width, uint32_t height, uint32_t channels, const float* texels, float u, float v,
+                       float out[3], uint32_t index[4], float weight[4]);
+float bfo_texel_reflectance(const float* value, uint32_t channels, float wavelength);
+/* adjoint derivatives: d ln f / d(alpha_x, alpha_y) of GGX closures, d f / d w per lane */
+int bfo_closure_alpha_log_gradient(const bf_closure_record* closure, const float wo[3], const float wi[3], double out[2]);
+int bfo_closure_weight_derivative(const bf_closure_record* closure, uint32_t allowed, const float wo[3], const float wi[3],
+                                  double out[4]);
 /* MIS weight of `sampled` against `competing` (same measure); heuristic 0 balance, 1 power. */
 int bfo_mis_weight(uint32_t heuristic, float sampled, float competing, float* weight);
 /* Russian roulette on a 4-lane throughput (updated in place when the path survives). */

@@ -58,6 +58,9 @@ struct bfo_scene {
     bf_closure_desc* closures;
     bf_spectrum_desc* spectra;
     float* spectrum_values;
+    /* textures (host copies; desc.textures points at the owned descriptors) */
+    bf_texture_desc textures[BF_MAX_TEXTURES];
+    float* texels[BF_MAX_TEXTURES];
     /* canonical BVH */
     bf_bvh_blas* blases;
     bf_bvh_node* blas_nodes;

@@ -163,6 +163,19 @@ static void cmf(float wavelength, float out[3]) {
     }
 }
 
+/* The colour-matching functions at a wavelength in binary64 (same table and interpolation). */
+void bfo_cie_cmf64(float wavelength, double out[3]) {
+    const float coordinate = wavelength - 360.0f;
+    const size_t lower_index = (size_t)coordinate;
+    if (lower_index + 1 >= 471) {
+        for (int c = 0; c < 3; ++c) out[c] = cie_table[470][c];
+        return;
+    }
+    const double fraction = (double)coordinate - (double)lower_index;
+    for (int c = 0; c < 3; ++c)
+        out[c] = cie_table[lower_index][c] + (cie_table[lower_index + 1][c] - cie_table[lower_index][c]) * fraction;
+}
+
 /* Cie1931Sensor.cpp:104-124 */
 static float scaled_sensor_term(float contribution, float response, float normalization, float probability) {
     if (contribution == 0.0f || response == 0.0f) return 0.0f;

@@ -529,7 +529,37 @@ int bfo_scene_build(const bf_scene_desc* desc, bfo_scene** out) {
     DUP(s->closures, desc->closures, desc->closure_count, bf_closure_desc);
     DUP(s->spectra, desc->spectra, desc->spectrum_count, bf_spectrum_desc);
     DUP(s->spectrum_values, desc->spectrum_values, desc->spectrum_value_count, float);
-    int status = build_bvh(s);
+    /* textures: the rules of the product's scene builder; the oracle reads host texels only */
+    int status = BF_OK;
+    if (desc->texture_count > BF_MAX_TEXTURES || (desc->texture_count && !desc->textures)) status = BF_INVALID_ARGUMENT;
+    for (uint32_t t = 0; t < desc->texture_count && status == BF_OK; ++t) {
+        const bf_texture_desc* T = &desc->textures[t];
+        if (T->width == 0 || T->height == 0 || T->channels == 0 || T->channels > 3 || !T->data ||
+            (uint64_t)T->width * T->height * T->channels > 0xFFFFFFFFull) {
+            status = BF_INVALID_ARGUMENT;
+            break;
+        }
+        const size_t count = (size_t)T->width * T->height * T->channels;
+        DUP(s->texels[t], T->data, count, float);
+        s->textures[t] = *T;
+        s->textures[t].data = s->texels[t];
+        s->textures[t].device_data = NULL;
+    }
+    s->desc.textures = s->textures;
+    for (uint32_t c = 0; c < desc->closure_count && status == BF_OK; ++c) {
+        const bf_closure_desc* C = &desc->closures[c];
+        if (C->weight_texture) {
+            if (C->weight_texture > desc->texture_count) status = BF_INVALID_ARGUMENT;
+            else if (desc->textures[C->weight_texture - 1].channels == 2) status = BF_INVALID_ARGUMENT;
+        }
+        if (C->roughness_texture) {
+            if (C->roughness_texture > desc->texture_count ||
+                (C->kind != BF_CLOSURE_ROUGH_CONDUCTOR && C->kind != BF_CLOSURE_ROUGH_DIELECTRIC) ||
+                desc->textures[C->roughness_texture - 1].channels > 2)
+                status = BF_INVALID_ARGUMENT;
+        }
+    }
+    if (status == BF_OK) status = build_bvh(s);
     if (status == BF_OK) status = build_lights(s);
     if (status != BF_OK) {
         bfo_scene_free(s);
@@ -551,6 +581,7 @@ void bfo_scene_free(bfo_scene* s) {
     free(s->closures);
     free(s->spectra);
     free(s->spectrum_values);
+    for (uint32_t t = 0; t < BF_MAX_TEXTURES; ++t) free(s->texels[t]);
     free(s->blases);
     free(s->blas_nodes);
     free(s->primitive_refs);

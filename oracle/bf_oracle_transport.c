@@ -10,8 +10,15 @@
 #include <pthread.h>
 #include <stdio.h>
 
+#include "../include/blackframe_texture.h"
+#include "bf_oracle_adjoint.h"
 #include "bf_oracle_internal.h"
 #include "bf_oracle_lobes.h"
+
+/* RGB basis spectra of textured reflectance, [3][BF_RGB_BASIS_SAMPLES] */
+static const float rgb_basis[3 * BF_RGB_BASIS_SAMPLES] = {
+#include "../blackframe-renderer_amd/csrc/data/rgb_basis.inc"
+};
 
 /* ---- path lane ------------------------------------------------------------------------------ */
 typedef struct {
@@ -36,6 +43,12 @@ typedef struct {
     uint32_t termination, failed, fail_code;
     uint32_t route;
     uint32_t light_sampled, closure_sampled;
+    /* tape and adjoint replay */
+    uint64_t global_path;
+    uint32_t p_vertex;   /* path vertex of the pending light sample */
+    uint64_t visible;    /* bit d: the light sample of vertex d was unoccluded */
+    float total[4];      /* adjoint: the forward radiance of the path */
+    double adjoint[4];   /* adjoint: dLoss / d(radiance lane) */
 } lane;
 
 enum { ROUTE_NONE = 0, ROUTE_RAY = 1, ROUTE_HIT = 2, ROUTE_MISS = 3, ROUTE_SHADE = 4, ROUTE_SHADOW = 5, ROUTE_CONT = 6, ROUTE_TERM = 7 };
@@ -45,6 +58,12 @@ typedef struct {
     const bf_frame_desc* frame;
     const bf_transport_options* options;
     lane* lanes;
+    /* adjoint replay (film_gradient != NULL): tape inputs and binary64 texel gradients */
+    const float* tape_radiance;
+    const uint64_t* tape_visibility;
+    const float* film_gradient;
+    double* texture_gradients[BF_MAX_TEXTURES];
+    pthread_mutex_t* lock;
 } ctx;
 
 static void fail(lane* l, uint32_t code) {
@@ -68,7 +87,16 @@ static float spectrum_at(const bfo_scene* s, uint32_t id, float wavelength) {
     return fmaf(v[i + 1] - v[i], f, v[i]);
 }
 
-static void load_mixture(const bfo_scene* s, const bf_material_desc* m, const float* wl, lobe_mixture* mix) {
+/* Filtered texel value of a scene texture (blackframe_texture.h); 0 when not representable. */
+static int texture_value(const bfo_scene* s, uint32_t slot, const float* uv, float* value, bf_texel_footprint* f) {
+    const bf_texture_desc* T = &s->textures[slot];
+    if (!bf_texture_footprint(T->width, T->height, uv[0], uv[1], f)) return 0;
+    bf_texture_filter(T->data, T->channels, f, value);
+    return 1;
+}
+
+/* Backend extension: textured closure weights and GGX widths at the hit's texture coordinates. */
+static int load_mixture(const bfo_scene* s, const bf_material_desc* m, const float* wl, const float* uv, lobe_mixture* mix) {
     memset(mix, 0, sizeof *mix);
     mix->active_count = m->closure_count;
     for (uint32_t c = 0; c < m->closure_count; ++c) {
@@ -84,9 +112,26 @@ static void load_mixture(const bfo_scene* s, const bf_material_desc* m, const fl
                 r->parameters[4 + lane] = spectrum_at(s, d->k_spectrum, wl[lane]);
             }
         }
+        if (d->weight_texture) {
+            float t[3];
+            bf_texel_footprint f;
+            if (!texture_value(s, d->weight_texture - 1, uv, t, &f)) return 0;
+            const uint32_t channels = s->textures[d->weight_texture - 1].channels;
+            for (int lane = 0; lane < 4; ++lane) r->weight[lane] *= bf_texel_reflectance(rgb_basis, t, channels, wl[lane]);
+        }
+        if (d->roughness_texture) {
+            float t[3];
+            bf_texel_footprint f;
+            if (!texture_value(s, d->roughness_texture - 1, uv, t, &f)) return 0;
+            const float ay = s->textures[d->roughness_texture - 1].channels == 2 ? t[1] : t[0];
+            const int at = d->kind == BF_CLOSURE_ROUGH_CONDUCTOR ? 8 : 2;
+            r->parameters[at] = t[0];
+            r->parameters[at + 1] = ay;
+        }
         mix->probabilities[c] = m->probabilities[c];
     }
     for (uint32_t c = 0; c <= BF_MAX_CLOSURES; ++c) mix->cdf[c] = m->cdf[c];
+    return 1;
 }
 
 /* ---- camera (PinholeCamera.hpp:140-301, PixelJitter.hpp:37-63) ------------------------------ */
@@ -147,12 +192,32 @@ static void sample_wavelengths(float u, float* wl) {
 
 void bfo_sample_wavelengths(float u, float wavelengths[4]) { sample_wavelengths(u, wavelengths); }
 
+/* dLoss/d(radiance lane) of the lane's pixel in binary64: the film adds
+ * rgb = M sum_lanes L cmf(lambda) normalization / pdf per path (bfo_cie_xyz, bfo_xyz_to_rgb), so
+ * the derivative contracts the film gradient with M and the colour-matching functions. */
+static void film_adjoint(const ctx* c, lane* l) {
+    static const double M[3][3] = {{3.2404542, -1.5371385, -0.4985314},
+                                   {-0.9692660, 1.8760108, 0.0415560},
+                                   {0.0556434, -0.2040259, 1.0572252}};
+    const uint64_t pixel = (uint64_t)l->sample.pixel_y * c->frame->camera.width + l->sample.pixel_x;
+    double dxyz[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; ++k)
+        for (int ch = 0; ch < 3; ++ch) dxyz[k] += (double)c->film_gradient[3 * pixel + ch] * M[ch][k];
+    const double scale = 0.25 / bfo_cie_y_integral() * 470.0;
+    for (int lane = 0; lane < 4; ++lane) {
+        double r[3];
+        bfo_cie_cmf64(l->wl[lane], r);
+        l->adjoint[lane] = (dxyz[0] * r[0] + dxyz[1] * r[1] + dxyz[2] * r[2]) * scale;
+    }
+}
+
 static void camera_lane(const ctx* c, lane* l, uint64_t global_path) {
     const bf_frame_desc* f = c->frame;
     const uint64_t pixels = (uint64_t)f->camera.width * f->camera.height;
     const uint64_t s = global_path / pixels;
     const uint64_t rem = global_path % pixels;
     memset(l, 0, sizeof *l);
+    l->global_path = global_path;
     l->sample.pixel_x = (uint32_t)(rem % f->camera.width);
     l->sample.pixel_y = (uint32_t)(rem / f->camera.width);
     l->sample.sample_index = f->first_sample_index + s;
@@ -175,6 +240,11 @@ static void camera_lane(const ctx* c, lane* l, uint64_t global_path) {
     for (int k = 0; k < 4; ++k) l->beta[k] = 1.0f;
     l->eta_scale = 1.0f;
     l->route = ROUTE_RAY;
+    if (c->film_gradient) {
+        for (int k = 0; k < 4; ++k) l->total[k] = c->tape_radiance[4 * global_path + k];
+        l->visible = c->tape_visibility[global_path];
+        film_adjoint(c, l);
+    }
 }
 
 /* ---- geometry helpers (CudaWavefrontStages.cu) ---------------------------------------------- */
@@ -666,6 +736,134 @@ static int advance_depth(uint32_t lobes, lane* l) {
 
 static int is_zero4(const float* v) { return v[0] == 0.0f && v[1] == 0.0f && v[2] == 0.0f && v[3] == 0.0f; }
 
+/* ---- adjoint replay (no reference counterpart) -------------------------------------------------
+ * A parameter p of the closures at vertex j scales every term of the path estimate that passes
+ * through that vertex's BSDF value; with sampling, MIS weights and roulette held fixed,
+ *   dLoss/dp += sum_lanes a * (N * (df_l/dp) / f_l + R * (df_s/dp) / f_s),
+ * N the visible light-sample contribution (value f_l toward the light), R the radiance the
+ * continuation carried back (value f_s), a = dLoss/d(radiance lane). Accumulated in binary64. */
+static void rgb_basis64(float wavelength, double b[3]) {
+    const float step = 470.0f / (float)(BF_RGB_BASIS_SAMPLES - 1u);
+    const float t = (wavelength - 360.0f) / step;
+    uint32_t i = t <= 0.0f ? 0u : (uint32_t)t;
+    if (i > BF_RGB_BASIS_SAMPLES - 2u) i = BF_RGB_BASIS_SAMPLES - 2u;
+    const double f = (double)t - (double)i;
+    for (int ch = 0; ch < 3; ++ch) {
+        const float* v = rgb_basis + ch * BF_RGB_BASIS_SAMPLES;
+        b[ch] = (double)v[i] + ((double)v[i + 1] - (double)v[i]) * f;
+    }
+}
+
+static void vertex_gradients(const ctx* c, const lane* l, const bf_material_desc* mat, const lobe_mixture* mix,
+                             const lobe_filter_state* filtered, const float* uv, v3 wo, int light, v3 wi_light,
+                             const float* f_light, const float* N, int cont, v3 wi_cont, const float* f_cont) {
+    const bfo_scene* s = c->scene;
+    double R[4];
+    for (int k = 0; k < 4; ++k) R[k] = (double)l->total[k] - (double)l->L[k];
+    for (uint32_t i = 0; i < mix->active_count; ++i) {
+        const bf_closure_desc* D = &s->closures[mat->closure_offset + filtered->source_indices[i]];
+        double* wgrad = D->weight_texture ? c->texture_gradients[D->weight_texture - 1] : NULL;
+        double* rgrad = D->roughness_texture ? c->texture_gradients[D->roughness_texture - 1] : NULL;
+        if (!wgrad && !rgrad) continue;
+        const closure_rec* cr = &mix->closures[i];
+        const uint32_t allowed = filtered->allowed_directions[i];
+        double gl[4], gs[4];
+        for (int k = 0; k < 4; ++k) {
+            gl[k] = light && f_light[k] > 0.0f ? l->adjoint[k] * N[k] / f_light[k] : 0.0;
+            gs[k] = cont && f_cont[k] > 0.0f ? l->adjoint[k] * R[k] / f_cont[k] : 0.0;
+        }
+        if (wgrad) {
+            double dl[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0};
+            const int lw = light && oa_closure_weight_derivative(cr, allowed, wo, wi_light, dl);
+            const int cw = cont && oa_closure_weight_derivative(cr, allowed, wo, wi_cont, ds);
+            const uint32_t slot = D->weight_texture - 1;
+            const uint32_t channels = s->textures[slot].channels;
+            float t[3];
+            bf_texel_footprint f;
+            if (texture_value(s, slot, uv, t, &f)) {
+                double sc[3] = {0.0, 0.0, 0.0};
+                for (int k = 0; k < 4; ++k) {
+                    const double g = (lw ? gl[k] * dl[k] : 0.0) + (cw ? gs[k] * ds[k] : 0.0);
+                    if (g == 0.0) continue;
+                    const double w = g * spectrum_at(s, D->weight_spectrum, l->wl[k]);
+                    if (channels == 1) {
+                        if (t[0] > 0.0f && t[0] < 1.0f) sc[0] += w;
+                    } else {
+                        double b[3];
+                        rgb_basis64(l->wl[k], b);
+                        const double rho = t[0] * b[0] + t[1] * b[1] + t[2] * b[2];
+                        if (!(rho > 0.0) || !(rho < 1.0)) continue;
+                        for (int ch = 0; ch < 3; ++ch) sc[ch] += w * b[ch];
+                    }
+                }
+                pthread_mutex_lock(c->lock);
+                for (int q = 0; q < 4; ++q)
+                    for (uint32_t ch = 0; ch < channels; ++ch) wgrad[(size_t)f.index[q] * channels + ch] += f.weight[q] * sc[ch];
+                pthread_mutex_unlock(c->lock);
+            }
+        }
+        if (rgrad && (cr->kind == BF_CLOSURE_ROUGH_CONDUCTOR || cr->kind == BF_CLOSURE_ROUGH_DIELECTRIC)) {
+            double ql[2] = {0.0, 0.0}, qs[2] = {0.0, 0.0};
+            const int ul = light && oa_closure_alpha_log_gradient(cr, wo, wi_light, ql);
+            const int us = cont && oa_closure_alpha_log_gradient(cr, wo, wi_cont, qs);
+            float fl[4], fs[4];
+            memcpy(fl, f_light, sizeof fl);
+            memcpy(fs, f_cont, sizeof fs);
+            if (mix->active_count > 1) {
+                if (ul && lobe_record_eval(cr, allowed, wo, wi_light, fl) != LOBE_OK) continue;
+                if (us && lobe_record_eval(cr, allowed, wo, wi_cont, fs) != LOBE_OK) continue;
+            }
+            double h[2] = {0.0, 0.0};
+            for (int k = 0; k < 4; ++k) {
+                const double el = ul ? gl[k] * fl[k] : 0.0, es = us ? gs[k] * fs[k] : 0.0;
+                h[0] += el * ql[0] + es * qs[0];
+                h[1] += el * ql[1] + es * qs[1];
+            }
+            const uint32_t slot = D->roughness_texture - 1;
+            const uint32_t channels = s->textures[slot].channels;
+            if (channels == 1) h[0] += h[1];
+            float t[3];
+            bf_texel_footprint f;
+            if (texture_value(s, slot, uv, t, &f)) {
+                pthread_mutex_lock(c->lock);
+                for (int q = 0; q < 4; ++q)
+                    for (uint32_t ch = 0; ch < channels; ++ch) rgrad[(size_t)f.index[q] * channels + ch] += f.weight[q] * h[ch];
+                pthread_mutex_unlock(c->lock);
+            }
+        }
+    }
+}
+
+/* The adjoint's end of a shaded vertex: the light sample resolved from the tape (as shadow_lane
+ * would), gradients, then the route the shadow stage would have given. */
+static void adjoint_vertex(const ctx* c, lane* l, const bf_material_desc* mat, const lobe_mixture* mix,
+                           const lobe_filter_state* filtered, const float* uv, v3 wo, int has_shadow, v3 nee_wi,
+                           const float* nee_f, int continues, v3 wi_cont, const float* f_cont, uint32_t final_route,
+                           uint32_t final_term) {
+    float N[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const int visible = has_shadow && l->p_vertex < 64 && ((l->visible >> l->p_vertex) & 1u);
+    if (visible) {
+        for (int k = 0; k < 4; ++k) {
+            const float den[2] = {l->p_sel, l->p_cond};
+            const float coefficient = l->p_lambert ? BFO_INV_PI : 1.0f;
+            int status;
+            if (l->p_weight == 1.0f) {
+                const float num[6] = {l->p_beta[k], l->p_bsdf[k], coefficient, l->p_L[k], l->p_cos, l->p_corr};
+                status = lobe_checked_product_quotient(num, 6, den, 2, &N[k]);
+            } else {
+                const float num[7] = {l->p_beta[k], l->p_bsdf[k], coefficient, l->p_L[k], l->p_cos, l->p_corr, l->p_weight};
+                status = lobe_checked_product_quotient(num, 7, den, 2, &N[k]);
+            }
+            if (status != LOBE_OK) return fail(l, 41);
+            l->L[k] += N[k];
+            if (!bf_isfinite(l->L[k]) || l->L[k] < 0.0f) return fail(l, 41);
+        }
+    }
+    vertex_gradients(c, l, mat, mix, filtered, uv, wo, visible, nee_wi, nee_f, N, continues, wi_cont, f_cont);
+    l->route = final_route;
+    if (final_route == ROUTE_TERM) l->termination = final_term;
+}
+
 /* CudaWavefrontStages.cu:4107-4727 */
 static void shade_lane(const ctx* c, lane* l) {
     const bfo_scene* s = c->scene;
@@ -683,7 +881,7 @@ static void shade_lane(const ctx* c, lane* l) {
         if (!bf_isfinite(emitted[k]) || emitted[k] < 0.0f) return fail(l, 13);
     }
     lobe_mixture mix;
-    load_mixture(s, mat, l->wl, &mix);
+    if (!load_mixture(s, mat, l->wl, surf.uv, &mix)) return fail(l, 42);
     frame_t frame, unrotated;
     const int created = mat->frame_mode == 0 ? make_local_frame(surf.ns, &unrotated)
                                              : frame_from_tangent(surf.ns, surf.dpdu, &unrotated);
@@ -744,8 +942,12 @@ static void shade_lane(const ctx* c, lane* l) {
         if (!sample_light(c, l, &surf, first_dim, &inc)) return fail(l, 21);
     }
     int has_shadow = 0;
+    const uint32_t vertex = l->depth;
+    float nee_f[4] = {0.0f, 0.0f, 0.0f, 0.0f}; /* closure value toward the light (adjoint) */
+    v3 nee_wi = v3_make(0.0f, 0.0f, 0.0f);
     if (inc.kind != 0) {
         const v3 wi_local = to_local(&frame, inc.direction);
+        nee_wi = wi_local;
         if (wo_local.z != 0.0f && wi_local.z != 0.0f) {
             const int singleton_lambert = mix.active_count == 1 && mix.closures[0].kind == BF_CLOSURE_LAMBERTIAN;
             float bsdf[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -755,6 +957,7 @@ static void shade_lane(const ctx* c, lane* l) {
             } else if (lobe_filtered_mixture_eval(&mix, &filtered, wo_local, wi_local, bsdf) != LOBE_OK) {
                 return fail(l, 22);
             }
+            for (int k = 0; k < 4; ++k) nee_f[k] = singleton_lambert ? bsdf[k] * BFO_INV_PI : bsdf[k];
             if (!is_zero4(bsdf)) {
                 float corr;
                 if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, &corr)) return fail(l, 23);
@@ -773,6 +976,7 @@ static void shade_lane(const ctx* c, lane* l) {
                     l->p_corr = corr;
                     l->p_lambert = (uint32_t)singleton_lambert;
                     if (!prepare_shadow_ray(&surf, &inc, l)) return fail(l, 25);
+                    l->p_vertex = vertex;
                     has_shadow = 1;
                 }
             }
@@ -791,6 +995,9 @@ static void shade_lane(const ctx* c, lane* l) {
     if (!continuation) {
         l->cont_pending = 0;
         l->pending_term = BF_TERM_OUTSIDE_SUPPORT;
+        if (c->film_gradient)
+            return adjoint_vertex(c, l, mat, &mix, &filtered, surf.uv, wo_local, has_shadow, nee_wi, nee_f, 0, wi, bs.value,
+                                  ROUTE_TERM, BF_TERM_OUTSIDE_SUPPORT);
         if (has_shadow) {
             l->route = ROUTE_SHADOW;
             return;
@@ -858,6 +1065,9 @@ static void shade_lane(const ctx* c, lane* l) {
             pending_term = BF_TERM_RUSSIAN_ROULETTE;
         }
     }
+    if (c->film_gradient)
+        return adjoint_vertex(c, l, mat, &mix, &filtered, surf.uv, wo_local, has_shadow, nee_wi, nee_f, cont_pending,
+                              bs.incoming, bs.value, cont_pending ? ROUTE_CONT : ROUTE_TERM, pending_term);
     if (has_shadow) {
         l->cont_pending = (uint32_t)cont_pending;
         l->pending_term = pending_term;
@@ -876,6 +1086,7 @@ static void shade_lane(const ctx* c, lane* l) {
 static void shadow_lane(lane* l, uint32_t occlusion_status) {
     if (occlusion_status != BF_VISIBLE && occlusion_status != BF_OCCLUDED) return fail(l, 40);
     if (occlusion_status == BF_VISIBLE) {
+        l->visible |= 1ull << (l->p_vertex & 63u);
         for (int k = 0; k < 4; ++k) {
             const float den[2] = {l->p_sel, l->p_cond};
             float contribution;
@@ -1016,9 +1227,10 @@ static uint64_t queue_hash(const uint32_t* q, uint32_t n) {
 
 /* Queue routing of WavefrontTransport.cpp:2222-2649: every compaction is a stable filter of its
  * producer's order; shadow continuations are appended after the shade stage's continuations. */
-int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
-               uint64_t path_begin, uint64_t path_count, uint32_t threads, bf_path_result* paths, float* film_float,
-               double* film_compensated, uint32_t* film_counts, bf_transport_report* report, uint64_t* traversal_counters) {
+static int render(ctx* c0, const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                  uint64_t path_begin, uint64_t path_count, uint32_t threads, bf_path_result* paths, float* film_float,
+                  double* film_compensated, uint32_t* film_counts, bf_transport_report* report,
+                  uint64_t* traversal_counters, float* tape_radiance, uint64_t* tape_visibility) {
     if (path_count == 0 || path_count > 0xFFFFFFFFull) return BF_INVALID_ARGUMENT;
     const uint32_t n = (uint32_t)path_count;
     lane* lanes = (lane*)calloc(n, sizeof(lane));
@@ -1027,7 +1239,11 @@ int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_tran
     uint32_t* shadow_q = (uint32_t*)malloc(n * sizeof(uint32_t));
     uint32_t* cont_q = (uint32_t*)malloc(n * sizeof(uint32_t));
     if (!lanes || !ray_q || !hit_q || !shadow_q || !cont_q) return BF_RESOURCE_EXHAUSTED;
-    ctx c = {scene, frame, options, lanes};
+    ctx c = *c0;
+    c.scene = scene;
+    c.frame = frame;
+    c.options = options;
+    c.lanes = lanes;
     bf_transport_report rep;
     memset(&rep, 0, sizeof rep);
     rep.path_count = n;
@@ -1087,6 +1303,10 @@ int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_tran
             }
             rep.failed_paths++;
         }
+        if (tape_radiance) {
+            memcpy(tape_radiance + 4 * (path_begin + i), l->L, sizeof l->L);
+            tape_visibility[path_begin + i] = l->visible;
+        }
         if (paths) {
             bf_path_result* p = &paths[i];
             memcpy(p->radiance, l->L, sizeof p->radiance);
@@ -1128,4 +1348,76 @@ int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_tran
     free(shadow_q);
     free(cont_q);
     return BF_OK;
+}
+
+int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+               uint64_t path_begin, uint64_t path_count, uint32_t threads, bf_path_result* paths, float* film_float,
+               double* film_compensated, uint32_t* film_counts, bf_transport_report* report, uint64_t* traversal_counters) {
+    ctx c;
+    memset(&c, 0, sizeof c);
+    return render(&c, scene, frame, options, path_begin, path_count, threads, paths, film_float, film_compensated,
+                  film_counts, report, traversal_counters, NULL, NULL);
+}
+
+int bfo_render_taped(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                     uint64_t path_begin, uint64_t path_count, uint32_t threads, float* film_float,
+                     double* film_compensated, uint32_t* film_counts, float* tape_radiance, uint64_t* tape_visibility) {
+    if (!tape_radiance || !tape_visibility) return BF_INVALID_ARGUMENT;
+    ctx c;
+    memset(&c, 0, sizeof c);
+    return render(&c, scene, frame, options, path_begin, path_count, threads, NULL, film_float, film_compensated,
+                  film_counts, NULL, NULL, tape_radiance, tape_visibility);
+}
+
+int bfo_render_adjoint(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                       uint64_t path_begin, uint64_t path_count, uint32_t threads, const float* tape_radiance,
+                       const uint64_t* tape_visibility, const float* film_gradient, double* const* texture_gradients) {
+    if (!tape_radiance || !tape_visibility || !film_gradient) return BF_INVALID_ARGUMENT;
+    pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+    ctx c;
+    memset(&c, 0, sizeof c);
+    c.tape_radiance = tape_radiance;
+    c.tape_visibility = tape_visibility;
+    c.film_gradient = film_gradient;
+    for (uint32_t k = 0; k < BF_MAX_TEXTURES; ++k) c.texture_gradients[k] = texture_gradients ? texture_gradients[k] : NULL;
+    c.lock = &lock;
+    bf_transport_report report;
+    return render(&c, scene, frame, options, path_begin, path_count, threads, NULL, NULL, NULL, NULL, &report, NULL,
+                  NULL, NULL);
+}
+
+/* d ln f / d(alpha_x, alpha_y) of a GGX closure record in binary64 (0 where no gradient flows). */
+int bfo_closure_alpha_log_gradient(const bf_closure_record* closure, const float wo[3], const float wi[3], double out[2]) {
+    closure_rec c;
+    c.kind = closure->kind;
+    c.lobes = closure->lobes;
+    memcpy(c.weight, closure->weight, sizeof c.weight);
+    memcpy(c.parameters, closure->parameters, sizeof c.parameters);
+    return oa_closure_alpha_log_gradient(&c, v3_make(wo[0], wo[1], wo[2]), v3_make(wi[0], wi[1], wi[2]), out);
+}
+
+/* d f / d w per lane of a closure record in binary64. */
+int bfo_closure_weight_derivative(const bf_closure_record* closure, uint32_t allowed, const float wo[3], const float wi[3],
+                                  double out[4]) {
+    closure_rec c;
+    c.kind = closure->kind;
+    c.lobes = closure->lobes;
+    memcpy(c.weight, closure->weight, sizeof c.weight);
+    memcpy(c.parameters, closure->parameters, sizeof c.parameters);
+    return oa_closure_weight_derivative(&c, allowed, v3_make(wo[0], wo[1], wo[2]), v3_make(wi[0], wi[1], wi[2]), out);
+}
+
+/* Bilinear texture lookup and RGB reflectance of blackframe_texture.h (for tests). */
+int bfo_texture_sample(uint32_t width, uint32_t height, uint32_t channels, const float* texels, float u, float v,
+                       float out[3], uint32_t index[4], float weight[4]) {
+    bf_texel_footprint f;
+    if (!bf_texture_footprint(width, height, u, v, &f)) return 0;
+    bf_texture_filter(texels, channels, &f, out);
+    memcpy(index, f.index, sizeof f.index);
+    memcpy(weight, f.weight, sizeof f.weight);
+    return 1;
+}
+
+float bfo_texel_reflectance(const float* value, uint32_t channels, float wavelength) {
+    return bf_texel_reflectance(rgb_basis, value, channels, wavelength);
 }

@@ -78,6 +78,18 @@ def load(abi):
     lib.bfo_atan2f.restype = C.c_float
     lib.bfo_atan2f.argtypes = [C.c_float, C.c_float]
     lib.bfo_sincosf.argtypes = [C.c_float, f3, f3]
+    lib.bfo_render_taped.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
+                                     C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.bfo_render_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
+                                       C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    d2 = C.POINTER(C.c_double)
+    lib.bfo_closure_alpha_log_gradient.argtypes = [rec, f3, f3, d2]
+    lib.bfo_closure_weight_derivative.argtypes = [rec, C.c_uint32, f3, f3, d2]
+    lib.bfo_cie_cmf64.argtypes = [C.c_float, d2]
+    lib.bfo_texture_sample.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_float, C.c_float, f3,
+                                       C.POINTER(C.c_uint32), f3]
+    lib.bfo_texel_reflectance.restype = C.c_float
+    lib.bfo_texel_reflectance.argtypes = [f3, C.c_uint32, C.c_float]
     return lib
 
 
@@ -175,3 +187,38 @@ class OracleScene:
             result["film_compensated"] = comp[..., 0] + comp[..., 1]
             result["film_counts"] = counts.reshape(cam.height, cam.width)
         return result
+
+    def render_taped(self, frame, options, threads=1):
+        """Forward render of the whole frame plus its adjoint tape (radiance, visibility bits)."""
+        cam = frame.camera
+        pixels = cam.width * cam.height
+        total = pixels * frame.samples_per_pixel
+        film = np.zeros(pixels * 3, dtype=np.float32)
+        film_c = np.zeros(pixels * 6, dtype=np.float64)
+        counts = np.zeros(pixels, dtype=np.uint32)
+        radiance = np.zeros(total * 4, dtype=np.float32)
+        visibility = np.zeros(total, dtype=np.uint64)
+        status = self.lib.bfo_render_taped(self.handle, C.byref(frame), C.byref(options), 0, total, threads,
+                                           film.ctypes.data, film_c.ctypes.data, counts.ctypes.data,
+                                           radiance.ctypes.data, visibility.ctypes.data)
+        if status != 0:
+            raise RuntimeError(f"oracle taped render failed with status {status}")
+        comp = film_c.reshape(cam.height, cam.width, 3, 2)
+        return {"film": film.reshape(cam.height, cam.width, 3), "film_compensated": comp[..., 0] + comp[..., 1],
+                "film_counts": counts.reshape(cam.height, cam.width), "tape_radiance": radiance,
+                "tape_visibility": visibility}
+
+    def adjoint(self, frame, options, tape, film_gradient, texture_shapes, threads=1):
+        """Binary64 texel gradients of sum(film_gradient * film) (film = RGB sums), one array per
+        texture shape (None: not wanted)."""
+        cam = frame.camera
+        total = cam.width * cam.height * frame.samples_per_pixel
+        grads = [np.zeros(shape, dtype=np.float64) if shape is not None else None for shape in texture_shapes]
+        table = (C.c_void_p * 8)(*[g.ctypes.data if g is not None else None for g in grads] + [None] * (8 - len(grads)))
+        fg = np.ascontiguousarray(film_gradient, dtype=np.float32).ravel()
+        status = self.lib.bfo_render_adjoint(self.handle, C.byref(frame), C.byref(options), 0, total, threads,
+                                             tape["tape_radiance"].ctypes.data, tape["tape_visibility"].ctypes.data,
+                                             fg.ctypes.data, table)
+        if status != 0:
+            raise RuntimeError(f"oracle adjoint failed with status {status}")
+        return grads
