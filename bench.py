@@ -294,6 +294,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--batch-capacity", type=int, default=1 << 27, help="paths in flight per batched launch")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     ap.add_argument("--adjoint-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB mebibytes (HBM counter calibration)")
@@ -346,6 +347,15 @@ def main():
     rays = report.closest_hit_queries + report.shadow_queries
 
     options.instrumentation = bf.abi.INSTRUMENT_STAGE_EVENTS
+    tuning = None
+    if not args.no_autotune:
+        # launch shape for this workload, timed on its first sample pass (stage events stay on in
+        # the timed frames, which keeps graph replay out of the candidates)
+        t0 = time.perf_counter()
+        tuning = hip.autotune_launch(ws, [(xscene, frame)], options, trial_passes=1)
+        options.launch = tuning.launch
+        log(f"autotune: launch 0x{tuning.launch:08x} ({tuning.best_ms:.2f} ms vs default {tuning.default_ms:.2f} ms "
+            f"per trial pass, {tuning.candidates} candidates, {time.perf_counter() - t0:.1f} s)")
     stage_ns = np.zeros(8, dtype=np.float64)
     stage_dispatches = np.zeros(8, dtype=np.float64)
     stream = torch.cuda.current_stream()
@@ -389,6 +399,12 @@ def main():
         "dtype": "f32",
         "data": f"synthetic (seeded procedural scene, frame seed {args.seed})",
         "config": {"workload": desc, "mode": "plain", "paths_per_frame": path_count, "rays_per_frame": rays,
+                   "launch": f"0x{options.launch:08x}",
+                   "autotune": None if tuning is None else {
+                       "trial": "first sample pass", "best_ms": round(tuning.best_ms, 3),
+                       "default_ms": round(tuning.default_ms, 3),
+                       "tried": {f"0x{tuning.tried[k]:08x}": round(tuning.tried_ms[k], 3)
+                                 for k in range(min(tuning.candidates, 24))}},
                    "paths_in_flight": capacity, "bvh_build_and_upload_s": round(upload_s, 2),
                    "failed_paths_per_frame": report.failed_paths,
                    "first_failure": [report.first_failure_slot, report.first_failure_code] if report.failed_paths else None},

@@ -23,6 +23,12 @@
 #ifndef BF_TRACE_WAVES_PER_SIMD
 #define BF_TRACE_WAVES_PER_SIMD 6 /* register budget of the traversal kernels (waves per SIMD) */
 #endif
+#ifndef BF_TRACE_REFILL
+#define BF_TRACE_REFILL 16u /* idle lanes of a traversal wave that trigger a fetch of new rays */
+#endif
+#ifndef BF_SHADE_WAVES_PER_SIMD
+#define BF_SHADE_WAVES_PER_SIMD 2 /* register budget of the shade kernels (waves per SIMD) */
+#endif
 
 typedef struct bf_dev_blas {
     uint32_t root_ref;         /* wide-node index, or BF_WIDE_LEAF | leaf-triangle index */

@@ -796,6 +796,7 @@ struct bf_hip_workspace {
     uint32_t* failure = nullptr;   /* [0] count, [2..3] lowest (job << 48 | path << 8 | failure code) */
     uint32_t* sample_counts = nullptr;
     unsigned long long* traversal = nullptr; /* 6 counters */
+    uint32_t* fetch = nullptr;               /* traversal work counters (2 words after the counters) */
     unsigned long long* hashes = nullptr;    /* [bounce][ray, shadow] FNV-1a-64 of the queues */
     uint32_t* host_words = nullptr;          /* pinned readback */
     uint32_t* host_history = nullptr;        /* pinned readback of the history */
@@ -1031,6 +1032,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     w->failure = (uint32_t*)(b + o_fail);
     w->sample_counts = (uint32_t*)(b + o_samp);
     w->traversal = (unsigned long long*)(b + o_trav);
+    w->fetch = (uint32_t*)(b + o_trav + 48);
     w->hashes = (unsigned long long*)(b + o_hash);
     w->jobs = (bf_dev_job*)(b + o_jobs);
     w->spill = (uint2*)(b + o_spill);
@@ -1087,7 +1089,7 @@ int validate_options(const bf_transport_options* o) {
 int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transport_params& P, bool batch, bool instrument,
                  uint32_t n_max, uint32_t* ray_q, uint32_t* cont_q, unsigned long long* counters,
                  unsigned long long* hash_slot, hipStream_t stream) {
-    const bool adjoint = P.adjoint != 0u;
+    const bool adjoint = P.shade_mode == BF_SHADE_ADJOINT;
 #define STAGE(stage, dispatches, call) HIP_TRY(staged(w, rep, instrument, (stage), (dispatches), stream, [&] { return (call); }))
     if (hash_slot) STAGE(BF_STAGE_AUXILIARY, 1, bfk_queue_hash(ray_q, &w->sizes[0], hash_slot, stream));
     STAGE(BF_STAGE_CLOSEST_HIT, 1, bfk_trace_closest(&P, ray_q, &w->sizes[0], n_max, w->route, counters, batch, stream));
@@ -1095,7 +1097,7 @@ int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transpo
           bfk_compact(w->route, &w->sizes[0], n_max, ray_q, w->tile_counts, w->tile_offsets, w->queues[1], &w->sizes[1], 0u,
                       w->queues[2], &w->sizes[4], 0u, stream));
     STAGE(BF_STAGE_SHADE, 1,
-          bfk_shade(&P, w->queues[1], &w->sizes[1], n_max, w->route, w->sample_counts, batch, adjoint, stream));
+          bfk_shade(&P, w->queues[1], &w->sizes[1], n_max, w->route, w->sample_counts, batch, P.shade_mode, stream));
     STAGE(BF_STAGE_COMPACTION, 3,
           bfk_compact(w->route, &w->sizes[1], n_max, w->queues[1], w->tile_counts, w->tile_offsets, w->queues[2],
                       &w->sizes[2], 0u, cont_q, &w->sizes[3], 0u, stream));
@@ -1187,6 +1189,7 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     for (int k = 0; k < 4; ++k) P.packet[k] = o->packet_wavelengths[k];
     P.failure = w->failure;
     P.spill = w->spill;
+    P.fetch = w->fetch;
     P.spill_stride = w->spill_lanes;
     P.shade_blocks = w->shade_blocks;
     const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
@@ -1194,7 +1197,6 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     P.jobs = w->jobs;
     P.film_pdf = 1.0f / 470.0f;
     P.film_normalization = film_normalization();
-    P.adjoint = adjoint ? 1u : 0u;
 
     bf_transport_report rep{};
     rep.first_failure_slot = BF_INVALID_INDEX;
@@ -1268,6 +1270,9 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         while (cursor < job_count && next_pass[cursor] == jobs[cursor].frame.samples_per_pixel) ++cursor;
         const uint32_t n = slot;
         P.job_count = (uint32_t)chunk.size();
+        bool textured = false;
+        for (size_t c = 0; c < chunk.size(); ++c) textured = textured || jobs[chunk[c].job].scene->dev.has_textures != 0u;
+        P.shade_mode = adjoint ? BF_SHADE_ADJOINT : (textured ? BF_SHADE_TEXTURED : BF_SHADE_PLAIN);
         if (batch) {
             HIP_TRY(hipMemcpyAsync(w->jobs, w->host_jobs, chunk.size() * sizeof(bf_dev_job), hipMemcpyHostToDevice, stream));
         } else {

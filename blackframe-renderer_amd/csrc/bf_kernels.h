@@ -36,6 +36,10 @@ typedef struct bf_dev_job {
     float* texture_gradients[BF_MAX_TEXTURES];
 } bf_dev_job;
 
+#define BF_SHADE_PLAIN 0u
+#define BF_SHADE_TEXTURED 1u
+#define BF_SHADE_ADJOINT 2u
+
 typedef struct bf_transport_params {
     bf_dev_job job0;            /* the job of a single-frame launch (kernel argument, no gather) */
     const bf_dev_job* jobs;     /* the job table of a batched launch (device memory) */
@@ -54,10 +58,11 @@ typedef struct bf_transport_params {
     uint2* spill;               /* traversal stack spill: BF_STACK_DEPTH entries per persistent lane */
     uint32_t spill_stride;      /* persistent lanes = resident traversal workgroups * BF_TRACE_BLOCK */
     uint32_t shade_blocks;      /* resident shade workgroups (persistent grid) */
+    uint32_t* fetch;            /* traversal work counters [closest, shadow], cleared before each launch */
     uint32_t trace_block;       /* lanes per traversal workgroup: 64, 128 or 256 */
     float film_pdf;             /* wavelength pdf and sensor normalisation of the film (adjoint) */
     float film_normalization;
-    uint32_t adjoint;           /* launch mode (host bookkeeping: part of the graph-cache key) */
+    uint32_t shade_mode;        /* BF_SHADE_* of this launch (host bookkeeping: part of the graph-cache key) */
 } bf_transport_params;
 
 #ifdef __cplusplus
@@ -69,9 +74,10 @@ hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ra
                       hipStream_t s);
 hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                              uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s);
-/* adjoint: shadow visibility from the tape (route 1 is never produced) and texel gradients */
+/* mode BF_SHADE_ADJOINT: shadow visibility from the tape (route 1 is never produced) and texel
+ * gradients; BF_SHADE_TEXTURED: texture lookups compiled in; BF_SHADE_PLAIN: untextured scenes */
 hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
-                     uint8_t* route, uint32_t* sample_counts, bool batch, bool adjoint, hipStream_t s);
+                     uint8_t* route, uint32_t* sample_counts, bool batch, uint32_t mode, hipStream_t s);
 hipError_t bfk_trace_shadow(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                             uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s);
 /* 256-item tiles the compaction of max_count items uses (size of the tile count / offset arrays / 2). */

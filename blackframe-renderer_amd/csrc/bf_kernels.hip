@@ -221,34 +221,67 @@ __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint
  * taking 64 consecutive queue items at a time with a static stride, so that the per-lane stack
  * spill area is bounded by the grid and empty tail workgroups cost nothing. */
 
+/* One lane's ray between traversal steps. */
+struct lane_ray {
+    uint32_t item, slot;
+    float t_min, t_max;
+    trav_state ts;
+    trav_hit best;
+    bool occluded;
+};
+
+/* World-space ray of a slot, re-read from the path columns (closest-hit rays). */
+struct closest_world {
+    const bf_dev_paths* paths;
+    uint32_t slot;
+    BF_D void operator()(v3f& o, v3f& d) const {
+        const float4 a = paths->ray_origin[slot];
+        const float4 b = paths->ray_direction[slot];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+    }
+};
+
+/* Starts queue item i; returns the traversal status of the root test. */
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
-BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
-                             trav_stack<BF_SHORT_STACK, BLOCK>& stack, trav_counters& ctr) {
-    const uint32_t slot = queue[i];
-    const bf_dev_job& J = job_at<BATCH>(P, slot);
-    const float4 ro = P.paths.ray_origin[slot];
-    const float4 rd = P.paths.ray_direction[slot];
+BF_D uint32_t closest_start(const bf_transport_params& P, const uint32_t* queue, uint32_t i,
+                            trav_stack<BF_SHORT_STACK, BLOCK>& stack, lane_ray& L, trav_counters& ctr) {
+    L.item = i;
+    L.slot = queue[i];
+    const bf_dev_job& J = job_at<BATCH>(P, L.slot);
+    const float4 ro = P.paths.ray_origin[L.slot];
+    const float4 rd = P.paths.ray_direction[L.slot];
     trav_ray ray;
     ray.o = mk(ro.x, ro.y, ro.z);
     ray.d = mk(rd.x, rd.y, rd.z);
     ray.t_min = rd.w;
-    ray.t_max = (P.paths.depth[slot].y == 0u) ? J.camera.t_max : BF_INFINITY_F;
-    ray.mask = P.paths.visibility[slot];
-    trav_hit best;
+    ray.t_max = (P.paths.depth[L.slot].y == 0u) ? J.camera.t_max : BF_INFINITY_F;
+    ray.mask = P.paths.visibility[L.slot];
+    L.t_min = ray.t_min;
+    L.t_max = ray.t_max;
     bool occluded;
-    stack.sp = 0u;
-    stack.gsp = 0u;
-    const auto world = [&](v3f& o, v3f& d) {
-        const float4 a = P.paths.ray_origin[slot];
-        const float4 b = P.paths.ray_direction[slot];
-        o = mk(a.x, a.y, a.z);
-        d = mk(b.x, b.y, b.z);
-    };
-    uint32_t st = traverse<false, COUNT>(J.scene, ray, world, stack, best, occluded, ctr);
+    return traverse_begin<COUNT>(J.scene, ray, stack, L.ts, L.best, occluded, ctr);
+}
+
+template <bool COUNT, bool BATCH, uint32_t BLOCK>
+BF_D uint32_t closest_step(const bf_transport_params& P, trav_stack<BF_SHORT_STACK, BLOCK>& stack, lane_ray& L,
+                           trav_counters& ctr) {
+    const bf_dev_job& J = job_at<BATCH>(P, L.slot);
+    bool occluded;
+    return traverse_step<false, COUNT>(J.scene, closest_world{&P.paths, L.slot}, stack, L.ts, L.best, occluded, ctr);
+}
+
+/* Classifies a finished closest-hit query: hit (route 1, barycentrics re-derived for the winner),
+ * miss (environment radiance, path escapes) or traversal failure. */
+template <bool BATCH>
+BF_D void closest_finish(const bf_transport_params& P, uint8_t* route, lane_ray& L, uint32_t st) {
+    const uint32_t slot = L.slot;
+    const bf_dev_job& J = job_at<BATCH>(P, slot);
+    trav_hit& best = L.best;
     if (st == BF_HIT_MISS && best.present) {
         v3f o, d;
-        world(o, d);
-        st = closest_hit_barycentrics(J.scene, o, d, ray.t_min, ray.t_max, best);
+        closest_world{&P.paths, slot}(o, d);
+        st = closest_hit_barycentrics(J.scene, o, d, L.t_min, L.t_max, best);
     }
     uint8_t r = 0;
     if (st != BF_HIT_MISS) {
@@ -262,10 +295,10 @@ BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue
         if (J.scene.has_environment) {
             const float4 wl = P.paths.wavelengths[slot];
             const float4 beta = P.paths.beta[slot];
-            float4 L = P.paths.radiance[slot];
+            float4 Lr = P.paths.radiance[slot];
             const float w4[4] = {wl.x, wl.y, wl.z, wl.w};
             const float b4[4] = {beta.x, beta.y, beta.z, beta.w};
-            float l4[4] = {L.x, L.y, L.z, L.w};
+            float l4[4] = {Lr.x, Lr.y, Lr.z, Lr.w};
             bool ok = true;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -287,7 +320,51 @@ BF_D void trace_closest_item(const bf_transport_params& P, const uint32_t* queue
             P.paths.status[slot] = BF_TERM_ESCAPED;
         }
     }
-    route[i] = r;
+    route[L.item] = r;
+}
+
+/*
+ * The persistent lane loop shared by both traversal stages. Every lane holds one ray; a lane whose
+ * ray is done stays idle until BF_TRACE_REFILL lanes of its wave are (or all are), and then the
+ * wave claims that many queue items with one atomic on the launch's fetch counter. A wave
+ * therefore keeps working while any ray of the queue is unclaimed instead of waiting for its
+ * slowest ray before it moves on. Every ray's traversal is the same sequence of steps in any
+ * schedule, so results do not depend on it; the loop ends when the counter has passed the item
+ * count and every lane of the wave is idle.
+ */
+template <typename Start, typename Step, typename Finish>
+BF_D void lane_loop(uint32_t n, uint32_t* fetch, Start start, Step step, Finish finish) {
+    const uint32_t lane = threadIdx.x & 63u;
+    bool active = false, exhausted = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t want = (uint32_t)__popcll(idle);
+        if (!exhausted && (want >= BF_TRACE_REFILL || want == 64u)) {
+            uint32_t base = 0u;
+            if (lane == 0u) base = atomicAdd(fetch, want);
+            base = __shfl(base, 0, 64);
+            exhausted = base >= n || n - base <= want;
+            if (!active) {
+                const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (i < n && base < n) {
+                    const uint32_t st = start(i);
+                    if (st != BF_HIT_MISS || !step.pending()) finish(st);
+                    else active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active) {
+            const uint32_t st = step();
+            if (st != BF_HIT_MISS || !step.pending()) {
+                finish(st);
+                active = false;
+            }
+        }
+    }
 }
 
 BF_D void reduce_counters(const trav_counters& c, unsigned long long* counters) {
@@ -313,11 +390,18 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
     trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
                                             P.spill_stride, 0u, 0u};
     trav_counters ctr{0u, 0u, 0u};
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, waves = (gridDim.x * BLOCK) >> 6;
-    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
-        if (base + lane < n) trace_closest_item<COUNT, BATCH, BLOCK>(P, queue, base + lane, route, stack, ctr);
-    }
+    lane_ray L;
+    struct stepper {
+        const bf_transport_params* P;
+        trav_stack<BF_SHORT_STACK, BLOCK>* stack;
+        lane_ray* L;
+        trav_counters* ctr;
+        BF_D uint32_t operator()() const { return closest_step<COUNT, BATCH>(*P, *stack, *L, *ctr); }
+        BF_D bool pending() const { return stack_depth(*stack) != 0u; }
+    };
+    lane_loop(
+        n, P.fetch, [&](uint32_t i) { return closest_start<COUNT, BATCH>(P, queue, i, stack, L, ctr); },
+        stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { closest_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters);
 }
 
@@ -402,6 +486,7 @@ BF_D bool texture_value(const bf_dev_scene& S, uint32_t slot, const float* uv, f
 
 /* Closure records at the path's wavelengths; textured weights and GGX widths are looked up at the
  * hit's texture coordinates. False when a texture lookup is not representable. */
+template <bool TEX>
 BF_D bool load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const float* wl, const float* uv, mixture_t& mix) {
     mix.active_count = M.closure_count;
     for (uint32_t c = 0; c < M.closure_count; ++c) {
@@ -419,7 +504,7 @@ BF_D bool load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const f
                 r.parameters[4 + lane] = spectrum_at(S, D.k_spectrum, wl[lane]);
             }
         }
-        if (D.weight_texture) {
+        if (TEX && D.weight_texture) {
             float t[3];
             bf_texel_footprint f;
             if (!texture_value(S, D.weight_texture - 1u, uv, t, f)) return false;
@@ -427,7 +512,7 @@ BF_D bool load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const f
 #pragma unroll
             for (int lane = 0; lane < 4; ++lane) r.weight[lane] *= bf_texel_reflectance(c_rgb_basis, t, channels, wl[lane]);
         }
-        if (D.roughness_texture) {
+        if (TEX && D.roughness_texture) {
             float t[3];
             bf_texel_footprint f;
             if (!texture_value(S, D.roughness_texture - 1u, uv, t, f)) return false;
@@ -645,11 +730,14 @@ BF_D void texel_gradients(const bf_dev_job& J, const bf_dev_scene& S, const bf_m
     }
 }
 
-/* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation. The adjoint replay
- * (ADJ) resolves the light sample from the tape and never routes to the shadow queue. */
-template <bool BATCH, bool ADJ>
+/* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation. MODE is
+ * BF_SHADE_PLAIN (no texture lookups compiled in), BF_SHADE_TEXTURED, or BF_SHADE_ADJOINT: the
+ * adjoint replay resolves the light sample from the tape, never routes to the shadow queue and
+ * accumulates texel gradients. */
+template <bool BATCH, uint32_t MODE>
 BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
                      uint32_t* sample_counts) {
+    constexpr bool ADJ = MODE == BF_SHADE_ADJOINT;
     const uint32_t slot = queue[i];
     route[i] = 0;
     const bf_dev_job& J = job_at<BATCH>(P, slot);
@@ -689,7 +777,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
         if (!bf_isfinite(emitted[c]) || emitted[c] < 0.0f) return fail_path(P, J, slot, 13);
     }
     mixture_t mix;
-    if (!load_mixture(S, M, wl, surf.uv, mix)) return fail_path(P, J, slot, 42);
+    if (!load_mixture<MODE != BF_SHADE_PLAIN>(S, M, wl, surf.uv, mix)) return fail_path(P, J, slot, 42);
     frame3 frame, unrotated;
     const bool created = M.frame_mode == 0u ? make_local_frame(surf.ns, unrotated) : frame_from_tangent(surf.ns, surf.dpdu, unrotated);
     if (!created || !rotate_frame(unrotated, M.tangent_rotation, frame)) return fail_path(P, J, slot, 14);
@@ -910,58 +998,77 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
     route[i] = 2;
 }
 
-template <bool BATCH, bool ADJ>
-__global__ __launch_bounds__(BF_BLOCK) void k_shade(bf_transport_params P, const uint32_t* queue, const uint32_t* count_ptr,
-                                                    uint8_t* route, uint32_t* sample_counts) {
+template <bool BATCH, uint32_t MODE>
+__global__ __launch_bounds__(BF_BLOCK, BF_SHADE_WAVES_PER_SIMD) void k_shade(bf_transport_params P, const uint32_t* queue,
+                                                                             const uint32_t* count_ptr, uint8_t* route,
+                                                                             uint32_t* sample_counts) {
     const uint32_t n = *count_ptr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        shade_item<BATCH, ADJ>(P, queue, i, route, sample_counts);
+        shade_item<BATCH, MODE>(P, queue, i, route, sample_counts);
 }
 
+/* World-space shadow ray of a slot, re-read from the pending-shadow columns. */
+struct shadow_world {
+    const bf_dev_paths* paths;
+    uint32_t slot;
+    BF_D void operator()(v3f& o, v3f& d) const {
+        const float4 a = paths->shadow_origin[slot];
+        const float4 b = paths->shadow_direction[slot];
+        o = mk(a.x, a.y, a.z);
+        d = mk(b.x, b.y, b.z);
+    }
+};
+
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
-BF_D void trace_shadow_item(const bf_transport_params& P, const uint32_t* queue, uint32_t i, uint8_t* route,
-                            trav_stack<BF_SHORT_STACK, BLOCK>& stack, trav_counters& ctr) {
-    const uint32_t slot = queue[i];
-    const bf_dev_job& J = job_at<BATCH>(P, slot);
-    const float4 so = P.paths.shadow_origin[slot];
-    const float4 sd = P.paths.shadow_direction[slot];
+BF_D uint32_t shadow_start(const bf_transport_params& P, const uint32_t* queue, uint32_t i,
+                           trav_stack<BF_SHORT_STACK, BLOCK>& stack, lane_ray& L, trav_counters& ctr) {
+    L.item = i;
+    L.slot = queue[i];
+    const bf_dev_job& J = job_at<BATCH>(P, L.slot);
+    const float4 so = P.paths.shadow_origin[L.slot];
+    const float4 sd = P.paths.shadow_direction[L.slot];
     trav_ray ray;
     ray.o = mk(so.x, so.y, so.z);
     ray.d = mk(sd.x, sd.y, sd.z);
     ray.t_min = 0.0f;
     ray.t_max = so.w;
-    ray.mask = P.paths.visibility[slot];
-    trav_hit best;
-    bool occluded;
-    stack.sp = 0u;
-    stack.gsp = 0u;
-    const auto world = [&](v3f& o, v3f& d) {
-        const float4 a = P.paths.shadow_origin[slot];
-        const float4 b = P.paths.shadow_direction[slot];
-        o = mk(a.x, a.y, a.z);
-        d = mk(b.x, b.y, b.z);
-    };
-    const uint32_t st = traverse<true, COUNT>(J.scene, ray, world, stack, best, occluded, ctr);
-    const uint32_t packed = __float_as_uint(sd.w);
+    ray.mask = P.paths.visibility[L.slot];
+    return traverse_begin<COUNT>(J.scene, ray, stack, L.ts, L.best, L.occluded, ctr);
+}
+
+template <bool COUNT, bool BATCH, uint32_t BLOCK>
+BF_D uint32_t shadow_step(const bf_transport_params& P, trav_stack<BF_SHORT_STACK, BLOCK>& stack, lane_ray& L,
+                          trav_counters& ctr) {
+    const bf_dev_job& J = job_at<BATCH>(P, L.slot);
+    return traverse_step<true, COUNT>(J.scene, shadow_world{&P.paths, L.slot}, stack, L.ts, L.best, L.occluded, ctr);
+}
+
+/* process_shadow (:4772-4908): the frozen contribution is added when the light is visible (and the
+ * vertex's tape bit set), then the path continues or ends with its pending termination. */
+template <bool BATCH>
+BF_D void shadow_finish(const bf_transport_params& P, uint8_t* route, lane_ray& L, uint32_t st) {
+    const uint32_t slot = L.slot;
+    const bf_dev_job& J = job_at<BATCH>(P, slot);
+    const uint32_t packed = __float_as_uint(P.paths.shadow_direction[slot].w);
     uint8_t r = 0;
     if (st != BF_HIT_MISS) {
         fail_path(P, J, slot, 40);
     } else {
         bool ok = true;
-        if (!occluded) {
+        if (!L.occluded) {
             if (J.tape_visibility) J.tape_visibility[path_of(J, slot)] |= 1ull << ((packed >> 16) & 63u);
             const float4 c = P.paths.shadow_contribution[slot];
-            float4 L = P.paths.radiance[slot];
-            L.x += c.x;
-            L.y += c.y;
-            L.z += c.z;
-            L.w += c.w;
-            if (!bf_isfinite(L.x) || !bf_isfinite(L.y) || !bf_isfinite(L.z) || !bf_isfinite(L.w) || L.x < 0.0f ||
-                L.y < 0.0f || L.z < 0.0f || L.w < 0.0f) {
+            float4 Lr = P.paths.radiance[slot];
+            Lr.x += c.x;
+            Lr.y += c.y;
+            Lr.z += c.z;
+            Lr.w += c.w;
+            if (!bf_isfinite(Lr.x) || !bf_isfinite(Lr.y) || !bf_isfinite(Lr.z) || !bf_isfinite(Lr.w) || Lr.x < 0.0f ||
+                Lr.y < 0.0f || Lr.z < 0.0f || Lr.w < 0.0f) {
                 ok = false;
                 fail_path(P, J, slot, 41);
             } else {
-                P.paths.radiance[slot] = L;
+                P.paths.radiance[slot] = Lr;
             }
         }
         if (ok) {
@@ -969,24 +1076,31 @@ BF_D void trace_shadow_item(const bf_transport_params& P, const uint32_t* queue,
             else P.paths.status[slot] = (packed >> 8) & 0xFFu;
         }
     }
-    route[i] = r;
+    route[L.item] = r;
 }
 
 /* ---- shadow stage: gather + any-hit + process_shadow (:4729-4908) ---------------------------- */
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
 __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
-                                                                      const uint32_t* count_ptr, uint8_t* route,
-                                                                      unsigned long long* counters) {
+                                                                          const uint32_t* count_ptr, uint8_t* route,
+                                                                          unsigned long long* counters) {
     __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
     trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
                                             P.spill_stride, 0u, 0u};
     trav_counters ctr{0u, 0u, 0u};
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6, waves = (gridDim.x * BLOCK) >> 6;
-    for (uint32_t base = wave * 64u; base < n; base += waves * 64u) {
-        if (base + lane < n) trace_shadow_item<COUNT, BATCH, BLOCK>(P, queue, base + lane, route, stack, ctr);
-    }
+    lane_ray L;
+    struct stepper {
+        const bf_transport_params* P;
+        trav_stack<BF_SHORT_STACK, BLOCK>* stack;
+        lane_ray* L;
+        trav_counters* ctr;
+        BF_D uint32_t operator()() const { return shadow_step<COUNT, BATCH>(*P, *stack, *L, *ctr); }
+        BF_D bool pending() const { return stack_depth(*stack) != 0u; }
+    };
+    lane_loop(
+        n, P.fetch + 1, [&](uint32_t i) { return shadow_start<COUNT, BATCH>(P, queue, i, stack, L, ctr); },
+        stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { shadow_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters + 3);
 }
 
@@ -1377,6 +1491,7 @@ template <bool ANY, bool COUNT, bool BATCH, uint32_t BLOCK>
 static void launch_trace(const bf_transport_params* P, uint32_t max_count, const uint32_t* q, const uint32_t* n,
                          uint8_t* route, unsigned long long* counters, hipStream_t s) {
     const dim3 grid(trace_blocks(P, max_count, BLOCK));
+    hipLaunchKernelGGL(k_set_words, dim3(1), dim3(64), 0, s, P->fetch + (ANY ? 1 : 0), 1u, 0u);
     if (ANY)
         hipLaunchKernelGGL((k_trace_shadow<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, route, counters);
     else
@@ -1456,24 +1571,35 @@ uint32_t bfk_trace_resident_lanes(int device) {
 }
 
 uint32_t bfk_shade_resident_blocks(int device) {
-    const void* kernels[4] = {(const void*)k_shade<false, false>, (const void*)k_shade<true, false>,
-                              (const void*)k_shade<false, true>, (const void*)k_shade<true, true>};
-    return (uint32_t)max_resident(kernels, 4, BF_BLOCK, device);
+    const void* kernels[6] = {(const void*)k_shade<false, BF_SHADE_PLAIN>,    (const void*)k_shade<true, BF_SHADE_PLAIN>,
+                              (const void*)k_shade<false, BF_SHADE_TEXTURED>, (const void*)k_shade<true, BF_SHADE_TEXTURED>,
+                              (const void*)k_shade<false, BF_SHADE_ADJOINT>,  (const void*)k_shade<true, BF_SHADE_ADJOINT>};
+    return (uint32_t)max_resident(kernels, 6, BF_BLOCK, device);
 }
 
+} /* extern "C" */
+
+template <bool BATCH>
+static void launch_shade(const bf_transport_params* P, dim3 grid, const uint32_t* queue, const uint32_t* count_ptr,
+                         uint8_t* route, uint32_t* sample_counts, uint32_t mode, hipStream_t s) {
+    const dim3 block(BF_BLOCK);
+    if (mode == BF_SHADE_ADJOINT)
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_ADJOINT>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+    else if (mode == BF_SHADE_TEXTURED)
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_TEXTURED>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+    else
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_PLAIN>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+}
+
+extern "C" {
+
 hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
-                     uint8_t* route, uint32_t* sample_counts, bool batch, bool adjoint, hipStream_t s) {
+                     uint8_t* route, uint32_t* sample_counts, bool batch, uint32_t mode, hipStream_t s) {
     if (max_count == 0) return hipSuccess;
     const uint32_t needed = blocks_for(max_count, BF_BLOCK);
-    const dim3 grid(needed < P->shade_blocks ? needed : P->shade_blocks), block(BF_BLOCK);
-    if (batch && adjoint)
-        hipLaunchKernelGGL((k_shade<true, true>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
-    else if (batch)
-        hipLaunchKernelGGL((k_shade<true, false>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
-    else if (adjoint)
-        hipLaunchKernelGGL((k_shade<false, true>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
-    else
-        hipLaunchKernelGGL((k_shade<false, false>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+    const dim3 grid(needed < P->shade_blocks ? needed : P->shade_blocks);
+    if (batch) launch_shade<true>(P, grid, queue, count_ptr, route, sample_counts, mode, s);
+    else launch_shade<false>(P, grid, queue, count_ptr, route, sample_counts, mode, s);
     return hipGetLastError();
 }
 

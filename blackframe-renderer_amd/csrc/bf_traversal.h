@@ -292,124 +292,157 @@ BF_D uint2 stack_pop(trav_stack<S, LANES>& s) {
  * live through the loop. The winning candidate keeps (t, ids, leaf); its barycentrics are
  * re-derived by closest_hit_barycentrics from the same deterministic triangle test.
  */
-template <bool ANY, bool COUNT, uint32_t S, uint32_t LANES, typename WorldRay>
-BF_D uint32_t traverse(const bf_dev_scene& S_, const trav_ray& ray, WorldRay world, trav_stack<S, LANES>& stack, trav_hit& best,
-                       bool& occluded, trav_counters& ctr) {
+/* Loop state of one ray's traversal between steps (the stack lives in trav_stack). */
+struct trav_state {
+    trav_space space;
+    float t_min, t_max;
+    uint32_t mask;
+    uint32_t in_blas, blas_base;
+    uint32_t inst_id, scene_inst;
+};
+
+/* Starts a ray: empties the stack and pushes the TLAS root when the ray meets its box. On a normal
+ * return an empty stack means the ray is already done. */
+template <bool COUNT, uint32_t S, uint32_t LANES>
+BF_D uint32_t traverse_begin(const bf_dev_scene& S_, const trav_ray& ray, trav_stack<S, LANES>& stack, trav_state& ts,
+                             trav_hit& best, bool& occluded, trav_counters& ctr) {
     best.present = false;
     best.t = ray.t_max;
     occluded = false;
+    stack.sp = 0u;
+    stack.gsp = 0u;
     if (S_.tlas_root_ref == BF_INVALID_INDEX) return BF_HIT_MISS;
-    const float t_min = ray.t_min;
-    const float t_max = ray.t_max;
-    const uint32_t mask = ray.mask;
-    trav_space space;
-    make_space(ray.o, ray.d, space);
-    {
-        bool h;
-        double nv;
-        const uint32_t st = slab(S_.tlas_root_box[0], S_.tlas_root_box[1], S_.tlas_root_box[2], S_.tlas_root_box[3],
-                                 S_.tlas_root_box[4], S_.tlas_root_box[5], space, t_min, t_max, h, nv);
-        if (st != BF_HIT_MISS) return st;
-        if (!h) {
-            if (COUNT) ctr.nodes++;
-            return BF_HIT_MISS;
-        }
-        stack_push(stack, S_.tlas_root_ref, round_up_float(nv));
-    }
-    bool in_blas = false;
-    uint32_t blas_base = 0;
-    uint32_t inst_id = 0, scene_inst = 0;
-    while (stack_depth(stack) != 0u) {
-        if (in_blas && stack_depth(stack) <= blas_base) {
-            in_blas = false;
-            v3f o, d;
-            world(o, d);
-            make_space(o, d, space); /* back in world space (recomputing is exact) */
-        }
-        const uint2 entry = stack_pop(stack);
-        const uint32_t ref = entry.x;
-        const float nearv = __uint_as_float(entry.y);
+    ts.t_min = ray.t_min;
+    ts.t_max = ray.t_max;
+    ts.mask = ray.mask;
+    ts.in_blas = 0u;
+    ts.blas_base = 0u;
+    ts.inst_id = 0u;
+    ts.scene_inst = 0u;
+    make_space(ray.o, ray.d, ts.space);
+    bool h;
+    double nv;
+    const uint32_t st = slab(S_.tlas_root_box[0], S_.tlas_root_box[1], S_.tlas_root_box[2], S_.tlas_root_box[3],
+                             S_.tlas_root_box[4], S_.tlas_root_box[5], ts.space, ts.t_min, ts.t_max, h, nv);
+    if (st != BF_HIT_MISS) return st;
+    if (!h) {
         if (COUNT) ctr.nodes++;
-        const float tcur = ANY ? t_max : best.t;
-        if (!(nearv <= tcur)) continue;
-        if (ref & BF_WIDE_LEAF) {
-            const uint32_t leaf = ref & ~BF_WIDE_LEAF;
-            if (!in_blas) {
-                const bf_dev_tlas_leaf L = S_.tlas_leaves[leaf];
-                if ((mask & L.visibility_mask) == 0u) continue;
-                const v3f lo = xform_point(L.world_to_local, space.o);
-                const v3f ld = xform_vector(L.world_to_local, space.d);
-                if (!fin3(lo) || !fin3(ld) || (ld.x == 0.0f && ld.y == 0.0f && ld.z == 0.0f))
-                    return BF_HIT_NUMERICAL_FAILURE;
-                if (COUNT) ctr.instances++;
-                const bf_dev_blas B = S_.blases[L.blas_index];
-                trav_space local;
-                make_space(lo, ld, local);
-                bool h;
-                double nv;
-                const uint32_t st = slab(B.root_box[0], B.root_box[1], B.root_box[2], B.root_box[3], B.root_box[4],
-                                         B.root_box[5], local, t_min, tcur, h, nv);
-                if (st != BF_HIT_MISS) return st;
-                if (!h) {
-                    if (COUNT) ctr.nodes++;
-                    continue;
-                }
-                if (stack_depth(stack) >= BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
-                inst_id = L.instance_id;
-                scene_inst = L.scene_instance;
-                in_blas = true;
-                blas_base = stack_depth(stack);
-                space = local;
-                stack_push(stack, B.root_ref, round_up_float(nv));
-                continue;
-            }
-            const float4 q0 = S_.leaf_triangles[3 * (size_t)leaf + 0];
-            const float4 q1 = S_.leaf_triangles[3 * (size_t)leaf + 1];
-            const float4 q2 = S_.leaf_triangles[3 * (size_t)leaf + 2];
-            const uint32_t prim = __float_as_uint(q0.w);
-            if (COUNT) ctr.triangles++;
-            tri_result tr;
-            tr.hit = false;
-            const uint32_t st = watertight<false>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), space.o,
-                                                  space.d, t_min, tcur, tr);
+        return BF_HIT_MISS;
+    }
+    stack_push(stack, S_.tlas_root_ref, round_up_float(nv));
+    return BF_HIT_MISS;
+}
+
+/* One step: pops one stack entry and processes it (a node's two child boxes, an instance entry,
+ * or a triangle). Returns a traversal status (BF_HIT_MISS = no error); the ray is done when the
+ * stack is empty. `world(o, d)` re-reads the world-space ray when a BLAS is left, so that no copy
+ * of it stays live between steps. */
+template <bool ANY, bool COUNT, uint32_t S, uint32_t LANES, typename WorldRay>
+BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S, LANES>& stack, trav_state& ts,
+                            trav_hit& best, bool& occluded, trav_counters& ctr) {
+    if (ts.in_blas && stack_depth(stack) <= ts.blas_base) {
+        ts.in_blas = 0u;
+        v3f o, d;
+        world(o, d);
+        make_space(o, d, ts.space); /* back in world space (recomputing is exact) */
+    }
+    const uint2 entry = stack_pop(stack);
+    const uint32_t ref = entry.x;
+    const float nearv = __uint_as_float(entry.y);
+    if (COUNT) ctr.nodes++;
+    const float tcur = ANY ? ts.t_max : best.t;
+    if (!(nearv <= tcur)) return BF_HIT_MISS;
+    if (ref & BF_WIDE_LEAF) {
+        const uint32_t leaf = ref & ~BF_WIDE_LEAF;
+        if (!ts.in_blas) {
+            const bf_dev_tlas_leaf L = S_.tlas_leaves[leaf];
+            if ((ts.mask & L.visibility_mask) == 0u) return BF_HIT_MISS;
+            const v3f lo = xform_point(L.world_to_local, ts.space.o);
+            const v3f ld = xform_vector(L.world_to_local, ts.space.d);
+            if (!fin3(lo) || !fin3(ld) || (ld.x == 0.0f && ld.y == 0.0f && ld.z == 0.0f)) return BF_HIT_NUMERICAL_FAILURE;
+            if (COUNT) ctr.instances++;
+            const bf_dev_blas B = S_.blases[L.blas_index];
+            trav_space local;
+            make_space(lo, ld, local);
+            bool h;
+            double nv;
+            const uint32_t st = slab(B.root_box[0], B.root_box[1], B.root_box[2], B.root_box[3], B.root_box[4],
+                                     B.root_box[5], local, ts.t_min, tcur, h, nv);
             if (st != BF_HIT_MISS) return st;
-            if (!tr.hit) continue;
-            if (ANY) {
-                occluded = true;
-                stack.sp = 0;
-                stack.gsp = 0;
+            if (!h) {
+                if (COUNT) ctr.nodes++;
                 return BF_HIT_MISS;
             }
-            if (!precedes(tr.t, inst_id, prim, best)) continue;
-            best.present = true;
-            best.t = tr.t;
-            best.instance_id = inst_id;
-            best.primitive = prim;
-            best.scene_instance = scene_inst;
-            best.leaf = leaf;
-            continue;
+            if (stack_depth(stack) >= BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+            ts.inst_id = L.instance_id;
+            ts.scene_inst = L.scene_instance;
+            ts.in_blas = 1u;
+            ts.blas_base = stack_depth(stack);
+            ts.space = local;
+            stack_push(stack, B.root_ref, round_up_float(nv));
+            return BF_HIT_MISS;
         }
-        const bf_wide_node* nodes = in_blas ? S_.blas_nodes : S_.tlas_nodes;
-        const bf_wide_node N = nodes[ref];
-        bool h0, h1;
-        double n0 = 0.0, n1 = 0.0;
-        uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], space, t_min, tcur, h0, n0);
+        const float4 q0 = S_.leaf_triangles[3 * (size_t)leaf + 0];
+        const float4 q1 = S_.leaf_triangles[3 * (size_t)leaf + 1];
+        const float4 q2 = S_.leaf_triangles[3 * (size_t)leaf + 2];
+        const uint32_t prim = __float_as_uint(q0.w);
+        if (COUNT) ctr.triangles++;
+        tri_result tr;
+        tr.hit = false;
+        const uint32_t st = watertight<false>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), ts.space.o,
+                                              ts.space.d, ts.t_min, tcur, tr);
         if (st != BF_HIT_MISS) return st;
-        st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], space, t_min, tcur, h1, n1);
-        if (st != BF_HIT_MISS) return st;
-        if (!h0 && !h1) continue;
-        if (h0 && h1) {
-            if (stack_depth(stack) + 2 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
-            const float f0 = (float)n0, f1 = (float)n1;
-            const bool first_near = f0 <= f1; /* ties: the first child has the lower canonical index */
-            stack_push(stack, first_near ? N.child[1] : N.child[0], round_up_float(first_near ? n1 : n0));
-            stack_push(stack, first_near ? N.child[0] : N.child[1], round_up_float(first_near ? n0 : n1));
-        } else {
-            if (stack_depth(stack) + 1 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
-            stack_push(stack, h0 ? N.child[0] : N.child[1], round_up_float(h0 ? n0 : n1));
+        if (!tr.hit) return BF_HIT_MISS;
+        if (ANY) {
+            occluded = true;
+            stack.sp = 0;
+            stack.gsp = 0;
+            return BF_HIT_MISS;
         }
+        if (!precedes(tr.t, ts.inst_id, prim, best)) return BF_HIT_MISS;
+        best.present = true;
+        best.t = tr.t;
+        best.instance_id = ts.inst_id;
+        best.primitive = prim;
+        best.scene_instance = ts.scene_inst;
+        best.leaf = leaf;
+        return BF_HIT_MISS;
+    }
+    const bf_wide_node* nodes = ts.in_blas ? S_.blas_nodes : S_.tlas_nodes;
+    const bf_wide_node N = nodes[ref];
+    bool h0, h1;
+    double n0 = 0.0, n1 = 0.0;
+    uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], ts.space, ts.t_min, tcur, h0, n0);
+    if (st != BF_HIT_MISS) return st;
+    st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], ts.space, ts.t_min, tcur, h1, n1);
+    if (st != BF_HIT_MISS) return st;
+    if (!h0 && !h1) return BF_HIT_MISS;
+    if (h0 && h1) {
+        if (stack_depth(stack) + 2 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+        const float f0 = (float)n0, f1 = (float)n1;
+        const bool first_near = f0 <= f1; /* ties: the first child has the lower canonical index */
+        stack_push(stack, first_near ? N.child[1] : N.child[0], round_up_float(first_near ? n1 : n0));
+        stack_push(stack, first_near ? N.child[0] : N.child[1], round_up_float(first_near ? n0 : n1));
+    } else {
+        if (stack_depth(stack) + 1 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+        stack_push(stack, h0 ? N.child[0] : N.child[1], round_up_float(h0 ? n0 : n1));
     }
     return BF_HIT_MISS;
+}
+
+/*
+ * Traverses TLAS then BLASes depth-first in the reference's order. Returns a traversal status
+ * (BF_HIT_MISS = no error). The stack must be empty on entry and is empty on a normal return.
+ * The winning candidate keeps (t, ids, leaf); its barycentrics are re-derived by
+ * closest_hit_barycentrics from the same deterministic triangle test.
+ */
+template <bool ANY, bool COUNT, uint32_t S, uint32_t LANES, typename WorldRay>
+BF_D uint32_t traverse(const bf_dev_scene& S_, const trav_ray& ray, WorldRay world, trav_stack<S, LANES>& stack, trav_hit& best,
+                       bool& occluded, trav_counters& ctr) {
+    trav_state ts;
+    uint32_t st = traverse_begin<COUNT>(S_, ray, stack, ts, best, occluded, ctr);
+    while (st == BF_HIT_MISS && stack_depth(stack) != 0u) st = traverse_step<ANY, COUNT>(S_, world, stack, ts, best, occluded, ctr);
+    return st;
 }
 
 /* Barycentrics of the winning candidate: the triangle test is a pure function of the triangle, the
