@@ -295,6 +295,8 @@ def main():
     ap.add_argument("--batch-capacity", type=int, default=1 << 27, help="paths in flight per batched launch")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint/forward measurement (prints its JSON object; for profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=5)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB mebibytes (HBM counter calibration)")
@@ -320,6 +322,9 @@ def main():
         return 0
     bf = load_package()
     hip = importlib.import_module("blackframe_renderer_amd.hip")
+    if args.adjoint_only:
+        print(json.dumps(adjoint_section(bf, hip, args, log)), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     scene, cam, frame, options, desc = build_workload(bf, args)

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -378,6 +379,7 @@ struct bf_hip_scene {
     bf_dev_scene dev{};
     uint32_t max_blas_depth = 0, tlas_depth = 0;
     uint32_t light_count = 0;
+    std::vector<uint32_t> texture_floats; /* width * height * channels per texture slot */
 };
 
 namespace {
@@ -679,6 +681,9 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
         if ((T.data == nullptr) == (T.device_data == nullptr))
             return set_error(BF_INVALID_ARGUMENT, "texture %u needs exactly one of host data and device data", t);
     }
+    s->texture_floats.clear();
+    for (uint32_t t = 0; t < d->texture_count; ++t)
+        s->texture_floats.push_back(d->textures[t].width * d->textures[t].height * d->textures[t].channels);
     bool textured = false;
     for (uint32_t c = 0; c < d->closure_count; ++c) {
         const bf_closure_desc& C = d->closures[c];
@@ -797,6 +802,7 @@ struct bf_hip_workspace {
     uint32_t* sample_counts = nullptr;
     unsigned long long* traversal = nullptr; /* 6 counters */
     uint32_t* fetch = nullptr;               /* traversal work counters (2 words after the counters) */
+    float* replicas = nullptr;               /* BF_GRADIENT_REPLICAS x BF_GRADIENT_ROW adjoint scratch */
     unsigned long long* hashes = nullptr;    /* [bounce][ray, shadow] FNV-1a-64 of the queues */
     uint32_t* host_words = nullptr;          /* pinned readback */
     uint32_t* host_history = nullptr;        /* pinned readback of the history */
@@ -989,6 +995,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
                  o_sizes = A.add(nullptr, 64), o_hist = A.add(nullptr, BF_HISTORY_BOUNCES * 16), o_fail = A.add(nullptr, 16),
                  o_samp = A.add(nullptr, 16), o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, BF_REPORT_MAX_BOUNCES * 16),
                  o_jobs = A.add(nullptr, BF_MAX_BATCH_JOBS * sizeof(bf_dev_job)),
+                 o_replicas = A.add(nullptr, (size_t)BF_GRADIENT_REPLICAS * BF_GRADIENT_ROW * sizeof(float)),
                  o_spill = A.add(nullptr, (size_t)w->spill_lanes * BF_STACK_DEPTH * sizeof(uint2));
     hipError_t e = hipMalloc(&w->block, A.total);
     if (e != hipSuccess) {
@@ -1035,6 +1042,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     w->fetch = (uint32_t*)(b + o_trav + 48);
     w->hashes = (unsigned long long*)(b + o_hash);
     w->jobs = (bf_dev_job*)(b + o_jobs);
+    w->replicas = (float*)(b + o_replicas);
     w->spill = (uint2*)(b + o_spill);
     *out = w;
     return BF_OK;
@@ -1190,6 +1198,7 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     P.failure = w->failure;
     P.spill = w->spill;
     P.fetch = w->fetch;
+    P.gradient_replicas = w->replicas;
     P.spill_stride = w->spill_lanes;
     P.shade_blocks = w->shade_blocks;
     const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
@@ -1213,6 +1222,33 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     unsigned long long* counters = traversal_counters ? w->traversal : nullptr;
     const uint64_t bound = (uint64_t)o->diffuse_depth_limit + o->glossy_depth_limit + o->specular_depth_limit + 2u;
     const float normalization = film_normalization();
+
+    /* small gradient buffers accumulate through replica rows (one offset per distinct buffer) */
+    struct replica_slot {
+        float* dst;
+        uint32_t offset, count;
+    };
+    std::vector<replica_slot> replica_slots;
+    std::vector<std::array<uint32_t, BF_MAX_TEXTURES>> replica_offsets(adjoint ? job_count : 0u);
+    uint32_t replica_used = 0;
+    for (uint32_t j = 0; adjoint && j < job_count; ++j) {
+        for (uint32_t k = 0; k < BF_MAX_TEXTURES; ++k) {
+            replica_offsets[j][k] = BF_INVALID_INDEX;
+            float* dst = adjoints[j].texture_gradients[k];
+            if (!dst) continue;
+            const uint32_t count = jobs[j].scene->texture_floats[k];
+            for (const replica_slot& r : replica_slots)
+                if (r.dst == dst) replica_offsets[j][k] = r.offset;
+            if (replica_offsets[j][k] != BF_INVALID_INDEX || count > BF_GRADIENT_SMALL || replica_used + count > BF_GRADIENT_ROW)
+                continue;
+            replica_slots.push_back({dst, replica_used, count});
+            replica_offsets[j][k] = replica_used;
+            replica_used += count;
+        }
+    }
+    if (replica_used)
+        STAGE(BF_STAGE_AUXILIARY, 1,
+              bfk_set_words((uint32_t*)w->replicas, BF_GRADIENT_REPLICAS * BF_GRADIENT_ROW, 0u, stream));
 
     std::vector<uint32_t> next_pass(job_count, 0u);
     uint32_t cursor = 0;
@@ -1262,7 +1298,10 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
                 D.film_counts = nullptr;
                 D.paths = nullptr;
                 D.film_gradient = A.film_gradient;
-                for (uint32_t k = 0; k < BF_MAX_TEXTURES; ++k) D.texture_gradients[k] = A.texture_gradients[k];
+                for (uint32_t k = 0; k < BF_MAX_TEXTURES; ++k) {
+                    D.texture_gradients[k] = A.texture_gradients[k];
+                    D.gradient_replica[k] = replica_offsets[chunk[c].job][k];
+                }
             }
             slot += D.slot_count;
             next_pass[chunk[c].job] += chunk[c].passes;
@@ -1387,6 +1426,8 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         }
         first_launch = false;
     }
+    for (const replica_slot& r : replica_slots)
+        STAGE(BF_STAGE_AUXILIARY, 1, bfk_reduce_replicas(w->replicas, r.offset, r.count, r.dst, stream));
     for (uint32_t j = 0; j < job_count; ++j)
         rep.path_count += (uint64_t)jobs[j].frame.camera.width * jobs[j].frame.camera.height * jobs[j].frame.samples_per_pixel;
     HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 16, hipMemcpyDeviceToHost, stream));

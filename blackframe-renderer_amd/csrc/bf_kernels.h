@@ -34,7 +34,17 @@ typedef struct bf_dev_job {
     unsigned long long* tape_visibility;
     const float* film_gradient; /* adjoint: dLoss/d(film RGB sums) */
     float* texture_gradients[BF_MAX_TEXTURES];
+    /* offset of texture k's gradient in each replica row, or BF_INVALID_INDEX: accumulate into
+     * texture_gradients[k] directly */
+    uint32_t gradient_replica[BF_MAX_TEXTURES];
 } bf_dev_job;
+
+/* Gradients of small textures (<= BF_GRADIENT_SMALL floats, e.g. 1x1 per-material scalars) are
+ * accumulated into BF_GRADIENT_REPLICAS copies, one chosen per lane, and summed into the
+ * caller's buffer at the end of the call: same-address atomics from every lane would serialise. */
+#define BF_GRADIENT_REPLICAS 256u
+#define BF_GRADIENT_SMALL 4096u
+#define BF_GRADIENT_ROW 16384u
 
 #define BF_SHADE_PLAIN 0u
 #define BF_SHADE_TEXTURED 1u
@@ -63,6 +73,7 @@ typedef struct bf_transport_params {
     float film_pdf;             /* wavelength pdf and sensor normalisation of the film (adjoint) */
     float film_normalization;
     uint32_t shade_mode;        /* BF_SHADE_* of this launch (host bookkeeping: part of the graph-cache key) */
+    float* gradient_replicas;   /* BF_GRADIENT_REPLICAS rows of BF_GRADIENT_ROW floats */
 } bf_transport_params;
 
 #ifdef __cplusplus
@@ -101,6 +112,8 @@ hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t d
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s);
 hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s);
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
+/* dst[e] += sum over the replica rows of replicas[row * BF_GRADIENT_ROW + offset + e], e < count */
+hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count, float* dst, hipStream_t s);
 #ifdef __cplusplus
 }
 #endif

@@ -649,14 +649,26 @@ BF_D bool zero4(const float* v) { return v[0] == 0.0f && v[1] == 0.0f && v[2] ==
  *   dLoss/dp += sum_lanes a * (N * (df_l/dp) / f_l + R * (df_s/dp) / f_s),
  * a = dLoss/d(radiance lane) of the path's pixel. Texel gradients follow through the bilinear
  * weights and, for weights, the RGB basis spectra. */
-BF_D void texel_gradients(const bf_dev_job& J, const bf_dev_scene& S, const bf_material_desc& M, const mixture_t& mix,
+/* Adds v to texel-channel e of texture k's gradient, through a lane-chosen replica row when the
+ * texture is small. */
+BF_D void gradient_add(const bf_transport_params& P, const bf_dev_job& J, uint32_t k, size_t e, float v) {
+    const uint32_t offset = J.gradient_replica[k];
+    if (offset != BF_INVALID_INDEX) {
+        const uint32_t row = (blockIdx.x * blockDim.x + threadIdx.x) % BF_GRADIENT_REPLICAS;
+        unsafeAtomicAdd(&P.gradient_replicas[(size_t)row * BF_GRADIENT_ROW + offset + e], v);
+    } else {
+        unsafeAtomicAdd(&J.texture_gradients[k][e], v);
+    }
+}
+
+BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, const bf_dev_scene& S, const bf_material_desc& M, const mixture_t& mix,
                           const filter_t& filtered, const float* uv, const float* wl, const float* a, bool light, v3f wo,
                           v3f wi_light, const float* f_light, const float* N, bool cont, v3f wi_cont, const float* f_cont,
                           const float* R) {
     for (uint32_t i = 0; i < mix.active_count; ++i) {
         const bf_closure_desc D = S.closures[M.closure_offset + filtered.source_indices[i]];
-        float* wgrad = D.weight_texture ? J.texture_gradients[D.weight_texture - 1u] : nullptr;
-        float* rgrad = D.roughness_texture ? J.texture_gradients[D.roughness_texture - 1u] : nullptr;
+        const bool wgrad = D.weight_texture && J.texture_gradients[D.weight_texture - 1u];
+        const bool rgrad = D.roughness_texture && J.texture_gradients[D.roughness_texture - 1u];
         if (!wgrad && !rgrad) continue;
         const closure_t& c = mix.closures[i];
         float gl[4], gs[4];
@@ -692,7 +704,7 @@ BF_D void texel_gradients(const bf_dev_job& J, const bf_dev_scene& S, const bf_m
                 for (int q = 0; q < 4; ++q)
                     for (uint32_t ch = 0; ch < channels; ++ch)
                         if (f.weight[q] != 0.0f && sc[ch] != 0.0f)
-                            unsafeAtomicAdd(&wgrad[(size_t)f.index[q] * channels + ch], f.weight[q] * sc[ch]);
+                            gradient_add(P, J, slot, (size_t)f.index[q] * channels + ch, f.weight[q] * sc[ch]);
             }
         }
         if (rgrad && (c.kind == BF_CLOSURE_ROUGH_CONDUCTOR || c.kind == BF_CLOSURE_ROUGH_DIELECTRIC)) {
@@ -724,7 +736,7 @@ BF_D void texel_gradients(const bf_dev_job& J, const bf_dev_scene& S, const bf_m
                 for (int q = 0; q < 4; ++q)
                     for (uint32_t ch = 0; ch < channels; ++ch)
                         if (f.weight[q] != 0.0f && h[ch] != 0.0f)
-                            unsafeAtomicAdd(&rgrad[(size_t)f.index[q] * channels + ch], f.weight[q] * h[ch]);
+                            gradient_add(P, J, slot, (size_t)f.index[q] * channels + ch, f.weight[q] * h[ch]);
             }
         }
     }
@@ -973,7 +985,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
             const float4 a4 = P.paths.shadow_contribution[slot];
             const float a[4] = {a4.x, a4.y, a4.z, a4.w};
             const float R[4] = {total.x - L[0], total.y - L[1], total.z - L[2], total.w - L[3]};
-            texel_gradients(J, S, M, mix, filtered, surf.uv, wl, a, visible, wo_local, nee_wi, nee_f, contribution,
+            texel_gradients(P, J, S, M, mix, filtered, surf.uv, wl, a, visible, wo_local, nee_wi, nee_f, contribution,
                             cont_pending, bs.incoming, bs.value, R);
         }
         if (!cont_pending) {
@@ -1425,6 +1437,16 @@ __global__ void k_sample_dump(sampler_ctx sc, const bf_sample_stream_index* indi
 }
 
 /* ---- small utility kernels (used instead of hipMemsetAsync so the sequence stays capturable) - */
+/* one thread per gradient element: the replica rows in row order, then one add to the caller's buffer */
+__global__ __launch_bounds__(BF_BLOCK) void k_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count,
+                                                              float* dst) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count) return;
+    float sum = 0.0f;
+    for (uint32_t row = 0; row < BF_GRADIENT_REPLICAS; ++row) sum += replicas[(size_t)row * BF_GRADIENT_ROW + offset + e];
+    dst[e] += sum;
+}
+
 __global__ void k_set_words(uint32_t* dst, uint32_t count, uint32_t value) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = value;
@@ -1658,6 +1680,12 @@ hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t d
     sc.directions = directions;
     hipLaunchKernelGGL(k_sample_dump, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, sc, indices, dimensions, count,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count, float* dst, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_replicas, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, replicas, offset, count, dst);
     return hipGetLastError();
 }
 
