@@ -63,8 +63,8 @@ BF_D double quotient(double a, double b, double y) {
 
 /* CudaSceneClosestHit.cu:407-455 (boxes are validated finite and ordered by the host builder):
  * per axis the binary64 plane distances (lo - o) / d and (hi - o) / d, widened by one ulp. */
-BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, const trav_space& s, float t_min,
-                   float t_max, bool& hit, double& near_value) {
+BF_D uint32_t slab_exact(float lx, float ly, float lz, float hx, float hy, float hz, const trav_space& s, float t_min,
+                         float t_max, bool& hit, double& near_value) {
     double nv = (double)t_min;
     double fv = (double)t_max;
     const float lo[3] = {lx, ly, lz};
@@ -94,6 +94,63 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
     hit = true;
     near_value = nv;
     return BF_HIT_MISS;
+}
+
+/*
+ * The same decision with a binary32 filter in front. Each binary32 plane distance
+ * q~ = RN32(RN32(plane - o) * RN32(1/d)) is within 3 binary32 roundings of the exact quotient
+ * (relative 3u, u = 2^-24), and the binary64 quotient and its one-ulp widening add under 2^-51
+ * relative; subnormal differences add at most |1/d| 2^-149 absolutely. So every binary64
+ * interval end lies within e = 4u max|q~| + max |1/d| 2^-147 + 2^-126 of its binary32 estimate,
+ * and when the estimated interval is empty (or non-empty) by more than 2e the binary64 test
+ * reaches the same verdict. Only a box within that band, or a hit (whose binary64 near value is
+ * pushed), takes binary64 arithmetic, and a hit needs only the near plane of each axis: the
+ * smaller quotient is the lo plane for d > 0 and the hi plane for d < 0 (rounding is monotone).
+ */
+BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, const trav_space& s, float t_min,
+                   float t_max, bool& hit, double& near_value) {
+    const float lo[3] = {lx, ly, lz};
+    const float hi[3] = {hx, hy, hz};
+    const float oo[3] = {s.o.x, s.o.y, s.o.z};
+    const float dd[3] = {s.d.x, s.d.y, s.d.z};
+    float nv = t_min, fv = t_max, scale = 0.0f, inv_max = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (dd[a] == 0.0f) {
+            if (oo[a] < lo[a] || oo[a] > hi[a]) {
+                hit = false;
+                return BF_HIT_MISS;
+            }
+            continue;
+        }
+        const float inv = (float)s.inv[a];
+        const float q1 = (lo[a] - oo[a]) * inv;
+        const float q2 = (hi[a] - oo[a]) * inv;
+        nv = fmaxf(nv, fminf(q1, q2));
+        fv = fminf(fv, fmaxf(q1, q2));
+        scale = fmaxf(scale, fmaxf(fabsf(q1), fabsf(q2)));
+        inv_max = fmaxf(inv_max, fabsf(inv));
+    }
+    const float e = 4.0f * 0x1p-24f * scale + 0x1p-147f * inv_max + 0x1p-126f;
+    if (bf_isfinite(e) && bf_isfinite(nv) && bf_isfinite(scale)) {
+        if (nv - fv > 2.0f * e) {
+            hit = false;
+            return BF_HIT_MISS;
+        }
+        if (fv - nv > 2.0f * e) {
+            double n = (double)t_min;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (dd[a] == 0.0f) continue;
+                const float plane = dd[a] > 0.0f ? lo[a] : hi[a];
+                n = fmax(n, bf_next_down(quotient((double)plane - (double)oo[a], (double)dd[a], s.inv[a])));
+            }
+            hit = true;
+            near_value = n;
+            return BF_HIT_MISS;
+        }
+    }
+    return slab_exact(lx, ly, lz, hx, hy, hz, s, t_min, t_max, hit, near_value);
 }
 
 BF_D float round_up_float(double v) {
