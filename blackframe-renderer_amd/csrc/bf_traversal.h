@@ -264,6 +264,28 @@ BF_D uint32_t watertight(v3f v0, v3f v1, v3f v2, v3f o, v3f d, float t_min, floa
     if (!bf_isfinite(e0) || !bf_isfinite(e1) || !bf_isfinite(e2)) return BF_HIT_NUMERICAL_FAILURE;
     const bool neg = e0 < 0.0f || e1 < 0.0f || e2 < 0.0f;
     const bool pos = e0 > 0.0f || e1 > 0.0f || e2 > 0.0f;
+    if (neg && pos) {
+        /* Binary32 filter in front of the binary64 re-decision. Each sheared coordinate is within
+         * 4u (u = 2^-24) of its exact value relative to m = |t_x| + |s t_z|, so each edge function
+         * is within 10.001u (m_b m_c' + m_b' m_c) of the exact one, while the binary64
+         * re-decision is accurate to about 2^-50 of the same terms. When one edge is negative and
+         * another positive by more than 12u of that (plus subnormal floors), the binary64 signs
+         * are mixed too and the ray misses exactly as refine_edges would decide. */
+        const float fl = 0x1p-120f;
+        const float max_ = fabsf(comp3(t0, kx)) + fabsf(sx * comp3(t0, kz)) + fl;
+        const float may = fabsf(comp3(t0, ky)) + fabsf(sy * comp3(t0, kz)) + fl;
+        const float mbx = fabsf(comp3(t1, kx)) + fabsf(sx * comp3(t1, kz)) + fl;
+        const float mby = fabsf(comp3(t1, ky)) + fabsf(sy * comp3(t1, kz)) + fl;
+        const float mcx = fabsf(comp3(t2, kx)) + fabsf(sx * comp3(t2, kz)) + fl;
+        const float mcy = fabsf(comp3(t2, ky)) + fabsf(sy * comp3(t2, kz)) + fl;
+        const float k = 12.0f * 0x1p-24f;
+        const float b0 = k * (mbx * mcy + mby * mcx) + 0x1p-100f;
+        const float b1 = k * (mcx * may + mcy * max_) + 0x1p-100f;
+        const float b2 = k * (max_ * mby + may * mbx) + 0x1p-100f;
+        const bool sure_neg = e0 < -b0 || e1 < -b1 || e2 < -b2;
+        const bool sure_pos = e0 > b0 || e1 > b1 || e2 > b2;
+        if (sure_neg && sure_pos && bf_isfinite(b0) && bf_isfinite(b1) && bf_isfinite(b2)) return BF_HIT_MISS;
+    }
     if (e0 == 0.0f || e1 == 0.0f || e2 == 0.0f || (neg && pos))
         return refine_edges<BARY>(v0, v1, v2, o, d, kx, ky, kz, t_min, t_max, r);
     const float det = e0 + e1 + e2;
