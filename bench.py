@@ -294,7 +294,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--batch-capacity", type=int, default=1 << 27, help="paths in flight per batched launch")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the launch shape of --launch")
+    ap.add_argument("--launch", type=lambda v: int(v, 0), default=0,
+                    help="bf_transport_options.launch word used with --no-autotune (default 0)")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint/forward measurement (prints its JSON object; for profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=5)
@@ -352,6 +354,7 @@ def main():
     rays = report.closest_hit_queries + report.shadow_queries
 
     options.instrumentation = bf.abi.INSTRUMENT_STAGE_EVENTS
+    options.launch = args.launch
     tuning = None
     if not args.no_autotune:
         # launch shape for this workload, timed on its first sample pass (stage events stay on in

@@ -165,6 +165,7 @@ INSTRUMENT_DISABLED, INSTRUMENT_STAGE_EVENTS = 0, 1
 FAILURES_ABORT, FAILURES_COUNT = 0, 1
 LAUNCH_GRAPH = 0x80000000
 LAUNCH_GROUP_MASK = 0x0000FFFF
+LAUNCH_SHADE_ORDER = 0x00100000
 
 
 def launch_trace_block(lanes):
@@ -203,7 +204,7 @@ MAX_TEXTURES = 8
 class LaunchTuning(C.Structure):
     """bf_launch_tuning: the autotuner's choice and the timings behind it."""
     _fields_ = [("launch", C.c_uint32), ("candidates", C.c_uint32), ("best_ms", C.c_float), ("default_ms", C.c_float),
-                ("tried", C.c_uint32 * 24), ("tried_ms", C.c_float * 24)]
+                ("tried", C.c_uint32 * 48), ("tried_ms", C.c_float * 48)]
 
 
 class AdjointJob(C.Structure):
@@ -220,7 +221,7 @@ class PathResult(C.Structure):
 
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
-                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 208),
+                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
                        (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 

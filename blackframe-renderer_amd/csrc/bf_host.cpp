@@ -803,6 +803,8 @@ struct bf_hip_workspace {
     unsigned long long* traversal = nullptr; /* 6 counters */
     uint32_t* fetch = nullptr;               /* traversal work counters (2 words after the counters) */
     float* replicas = nullptr;               /* BF_GRADIENT_REPLICAS x BF_GRADIENT_ROW adjoint scratch */
+    uint32_t* bucket_counts = nullptr;       /* shade-order grouping: per-tile bucket counts / offsets */
+    uint32_t* bucket_offsets = nullptr;
     unsigned long long* hashes = nullptr;    /* [bounce][ray, shadow] FNV-1a-64 of the queues */
     uint32_t* host_words = nullptr;          /* pinned readback */
     uint32_t* host_history = nullptr;        /* pinned readback of the history */
@@ -991,6 +993,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
                  o_st = A.add(nullptr, n * 4), o_vis = A.add(nullptr, n * 4), o_job = A.add(nullptr, n * 2);
     size_t o_q[4];
     for (int k = 0; k < 4; ++k) o_q[k] = A.add(nullptr, n * 4);
+    const size_t o_bc = A.add(nullptr, (size_t)tiles * 32), o_bo = A.add(nullptr, (size_t)tiles * 32);
     const size_t o_route = A.add(nullptr, n), o_tc = A.add(nullptr, (size_t)tiles * 8), o_to = A.add(nullptr, (size_t)tiles * 8),
                  o_sizes = A.add(nullptr, 64), o_hist = A.add(nullptr, BF_HISTORY_BOUNCES * 16), o_fail = A.add(nullptr, 16),
                  o_samp = A.add(nullptr, 16), o_trav = A.add(nullptr, 64), o_hash = A.add(nullptr, BF_REPORT_MAX_BOUNCES * 16),
@@ -1033,6 +1036,8 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     for (int k = 0; k < 4; ++k) w->queues[k] = (uint32_t*)(b + o_q[k]);
     w->route = b + o_route;
     w->tile_counts = (uint32_t*)(b + o_tc);
+    w->bucket_counts = (uint32_t*)(b + o_bc);
+    w->bucket_offsets = (uint32_t*)(b + o_bo);
     w->tile_offsets = (uint32_t*)(b + o_to);
     w->sizes = (uint32_t*)(b + o_sizes);
     w->history = (uint32_t*)(b + o_hist);
@@ -1087,7 +1092,7 @@ int validate_options(const bf_transport_options* o) {
     const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
     if (block_log2 != 0 && (block_log2 < 6 || block_log2 > 8))
         return set_error(BF_INVALID_ARGUMENT, "traversal workgroups of 2^%u lanes are not available (64, 128 or 256)", block_log2);
-    if ((o->launch & ~(BF_LAUNCH_GRAPH | BF_LAUNCH_TRACE_BLOCK_MASK | BF_LAUNCH_GROUP_MASK)) != 0)
+    if ((o->launch & ~(BF_LAUNCH_GRAPH | BF_LAUNCH_TRACE_BLOCK_MASK | BF_LAUNCH_GROUP_MASK | BF_LAUNCH_SHADE_ORDER)) != 0)
         return set_error(BF_INVALID_ARGUMENT, "unknown launch bits 0x%08x", o->launch);
     return BF_OK;
 }
@@ -1104,7 +1109,7 @@ int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transpo
     STAGE(BF_STAGE_COMPACTION, 3,
           bfk_compact(w->route, &w->sizes[0], n_max, ray_q, w->tile_counts, w->tile_offsets, w->queues[1], &w->sizes[1], 0u,
                       w->queues[2], &w->sizes[4], 0u, stream));
-    STAGE(BF_STAGE_SHADE, 1,
+    STAGE(BF_STAGE_SHADE, P.shade_order ? 4 : 1,
           bfk_shade(&P, w->queues[1], &w->sizes[1], n_max, w->route, w->sample_counts, batch, P.shade_mode, stream));
     STAGE(BF_STAGE_COMPACTION, 3,
           bfk_compact(w->route, &w->sizes[1], n_max, w->queues[1], w->tile_counts, w->tile_offsets, w->queues[2],
@@ -1199,6 +1204,11 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     P.spill = w->spill;
     P.fetch = w->fetch;
     P.gradient_replicas = w->replicas;
+    if (o->launch & BF_LAUNCH_SHADE_ORDER) {
+        P.shade_order = w->queues[2]; /* free while shade runs: the shadow queue is built after it */
+        P.bucket_counts = w->bucket_counts;
+        P.bucket_offsets = w->bucket_offsets;
+    }
     P.spill_stride = w->spill_lanes;
     P.shade_blocks = w->shade_blocks;
     const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
@@ -1537,14 +1547,16 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* w, const bf_frame_job* jobs
         out.best_ms = out.default_ms;
         out.launch = 0u;
         const uint32_t blocks[3] = {6u, 7u, 8u}, groups[3] = {2u, 4u, 8u};
+        for (uint32_t order = 0; order < 2 && status == BF_OK; ++order)
         for (uint32_t b = 0; b < 3 && status == BF_OK; ++b)
             for (uint32_t g = 0; g < 3 && status == BF_OK; ++g)
                 for (uint32_t graph = 0; graph < (graphs ? 2u : 1u) && status == BF_OK; ++graph) {
-                    const uint32_t launch = groups[g] | BF_LAUNCH_TRACE_BLOCK(blocks[b]) | (graph ? BF_LAUNCH_GRAPH : 0u);
+                    const uint32_t launch = groups[g] | BF_LAUNCH_TRACE_BLOCK(blocks[b]) | (graph ? BF_LAUNCH_GRAPH : 0u) |
+                                            (order ? BF_LAUNCH_SHADE_ORDER : 0u);
                     float ms = 0.0f;
                     status = run(launch, ms);
                     if (status != BF_OK) break;
-                    if (out.candidates < 24) {
+                    if (out.candidates < 48) {
                         out.tried[out.candidates] = launch;
                         out.tried_ms[out.candidates] = ms;
                     }

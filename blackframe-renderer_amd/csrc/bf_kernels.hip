@@ -1012,11 +1012,11 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
 
 template <bool BATCH, uint32_t MODE>
 __global__ __launch_bounds__(BF_BLOCK, BF_SHADE_WAVES_PER_SIMD) void k_shade(bf_transport_params P, const uint32_t* queue,
-                                                                             const uint32_t* count_ptr, uint8_t* route,
-                                                                             uint32_t* sample_counts) {
+                                                                             const uint32_t* count_ptr, const uint32_t* order,
+                                                                             uint8_t* route, uint32_t* sample_counts) {
     const uint32_t n = *count_ptr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        shade_item<BATCH, MODE>(P, queue, i, route, sample_counts);
+        shade_item<BATCH, MODE>(P, queue, order ? order[i] : i, route, sample_counts);
 }
 
 /* World-space shadow ray of a slot, re-read from the pending-shadow columns. */
@@ -1214,6 +1214,124 @@ __global__ __launch_bounds__(BF_BLOCK) void k_compact_scatter(const uint8_t* rou
             for (uint32_t w = 0; w < wave; ++w) rank += wave_counts[k][w];
             const uint32_t position = tile_offsets[2 * tile + k] + rank;
             (k == 0 ? dst1 : dst2)[position] = src[i];
+        }
+        __syncthreads();
+    }
+}
+
+/* ---- shade order: hit-queue items grouped by the closure kind of their material ---------------
+ * Shading a wave of mixed closure kinds issues every kind's code under a partial mask. The shade
+ * stage can instead visit the hit queue through a permutation that groups items by kind (stable
+ * within a kind); each item still writes its route at its own queue position, so the compactions
+ * that follow see the queue order unchanged. Buckets: 0 multi-closure, 1..4 the closure kind. */
+#define BF_SHADE_BUCKETS 8u
+
+template <bool BATCH>
+BF_D uint32_t shade_bucket(const bf_transport_params& P, const uint32_t* queue, uint32_t i) {
+    const uint32_t slot = queue[i];
+    const bf_dev_scene& S = job_at<BATCH>(P, slot).scene;
+    const bf_material_desc& M = S.materials[S.instances[P.paths.hit_ids[slot].x].material_index];
+    if (M.closure_count != 1u) return 0u;
+    const uint32_t kind = S.closures[M.closure_offset].kind;
+    return kind < BF_SHADE_BUCKETS ? kind : 0u;
+}
+
+template <bool BATCH>
+__global__ __launch_bounds__(BF_BLOCK) void k_bucket_count(bf_transport_params P, const uint32_t* queue,
+                                                           const uint32_t* count_ptr, uint8_t* keys, uint32_t* tile_counts) {
+    __shared__ uint32_t wave_counts[BF_SHADE_BUCKETS][BF_BLOCK / 64];
+    const uint32_t n = *count_ptr;
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
+    const uint32_t wave = threadIdx.x >> 6;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t i = tile * BF_BLOCK + threadIdx.x;
+        const uint32_t key = i < n ? shade_bucket<BATCH>(P, queue, i) : BF_SHADE_BUCKETS;
+        if (i < n) keys[i] = (uint8_t)key;
+#pragma unroll
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) {
+            const uint64_t m = __ballot(key == b);
+            if ((threadIdx.x & 63u) == 0u) wave_counts[b][wave] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (threadIdx.x < BF_SHADE_BUCKETS) {
+            uint32_t total = 0;
+            for (uint32_t w = 0; w < BF_BLOCK / 64; ++w) total += wave_counts[threadIdx.x][w];
+            tile_counts[BF_SHADE_BUCKETS * tile + threadIdx.x] = total;
+        }
+        __syncthreads();
+    }
+}
+
+/* single workgroup: tile offsets = bucket base (items of lower buckets) + earlier tiles' counts */
+__global__ __launch_bounds__(1024) void k_bucket_scan(const uint32_t* count_ptr, const uint32_t* tile_counts,
+                                                      uint32_t* tile_offsets) {
+    __shared__ uint32_t partial[BF_SHADE_BUCKETS][1024];
+    __shared__ uint32_t base[BF_SHADE_BUCKETS];
+    const uint32_t t = threadIdx.x;
+    const uint32_t tiles = (*count_ptr + BF_BLOCK - 1u) / BF_BLOCK;
+    const uint32_t per = (tiles + 1023u) / 1024u;
+    const uint32_t begin = t * per;
+    const uint32_t end = begin + per < tiles ? begin + per : tiles;
+    uint32_t own[BF_SHADE_BUCKETS];
+#pragma unroll
+    for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) own[b] = 0u;
+    for (uint32_t tile = begin; tile < end; ++tile)
+#pragma unroll
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) own[b] += tile_counts[BF_SHADE_BUCKETS * tile + b];
+#pragma unroll
+    for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) partial[b][t] = own[b];
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        uint32_t add[BF_SHADE_BUCKETS];
+#pragma unroll
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) add[b] = t >= off ? partial[b][t - off] : 0u;
+        __syncthreads();
+#pragma unroll
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) partial[b][t] += add[b];
+        __syncthreads();
+    }
+    if (t == 0u) {
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) {
+            base[b] = run;
+            run += partial[b][1023];
+        }
+    }
+    __syncthreads();
+    uint32_t run[BF_SHADE_BUCKETS];
+#pragma unroll
+    for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) run[b] = base[b] + partial[b][t] - own[b];
+    for (uint32_t tile = begin; tile < end; ++tile)
+#pragma unroll
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) {
+            tile_offsets[BF_SHADE_BUCKETS * tile + b] = run[b];
+            run[b] += tile_counts[BF_SHADE_BUCKETS * tile + b];
+        }
+}
+
+__global__ __launch_bounds__(BF_BLOCK) void k_bucket_scatter(const uint8_t* keys, const uint32_t* count_ptr,
+                                                             const uint32_t* tile_offsets, uint32_t* order) {
+    __shared__ uint32_t wave_counts[BF_SHADE_BUCKETS][BF_BLOCK / 64];
+    const uint32_t n = *count_ptr;
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t i = tile * BF_BLOCK + threadIdx.x;
+        const uint32_t key = i < n ? keys[i] : BF_SHADE_BUCKETS;
+        uint64_t mine = 0ull;
+#pragma unroll
+        for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) {
+            const uint64_t m = __ballot(key == b);
+            if (key == b) mine = m;
+            if (lane == 0u) wave_counts[b][wave] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (key < BF_SHADE_BUCKETS) {
+            uint32_t rank = (uint32_t)__popcll(mine & below);
+            for (uint32_t w = 0; w < wave; ++w) rank += wave_counts[key][w];
+            order[tile_offsets[BF_SHADE_BUCKETS * tile + key] + rank] = i;
         }
         __syncthreads();
     }
@@ -1603,14 +1721,28 @@ uint32_t bfk_shade_resident_blocks(int device) {
 
 template <bool BATCH>
 static void launch_shade(const bf_transport_params* P, dim3 grid, const uint32_t* queue, const uint32_t* count_ptr,
-                         uint8_t* route, uint32_t* sample_counts, uint32_t mode, hipStream_t s) {
+                         const uint32_t* order, uint8_t* route, uint32_t* sample_counts, uint32_t mode, hipStream_t s) {
     const dim3 block(BF_BLOCK);
     if (mode == BF_SHADE_ADJOINT)
-        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_ADJOINT>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_ADJOINT>), grid, block, 0, s, *P, queue, count_ptr, order, route,
+                           sample_counts);
     else if (mode == BF_SHADE_TEXTURED)
-        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_TEXTURED>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_TEXTURED>), grid, block, 0, s, *P, queue, count_ptr, order, route,
+                           sample_counts);
     else
-        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_PLAIN>), grid, block, 0, s, *P, queue, count_ptr, route, sample_counts);
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_PLAIN>), grid, block, 0, s, *P, queue, count_ptr, order, route,
+                           sample_counts);
+}
+
+template <bool BATCH>
+static void launch_shade_order(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
+                               uint32_t max_count, uint8_t* keys, uint32_t* tile_counts, uint32_t* tile_offsets,
+                               uint32_t* order, hipStream_t s) {
+    const uint32_t tiles = bfk_compact_tiles(max_count);
+    const dim3 grid(tiles < 4096u ? tiles : 4096u);
+    hipLaunchKernelGGL(k_bucket_count<BATCH>, grid, dim3(BF_BLOCK), 0, s, *P, queue, count_ptr, keys, tile_counts);
+    hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, count_ptr, tile_counts, tile_offsets);
+    hipLaunchKernelGGL(k_bucket_scatter, grid, dim3(BF_BLOCK), 0, s, keys, count_ptr, tile_offsets, order);
 }
 
 extern "C" {
@@ -1618,10 +1750,21 @@ extern "C" {
 hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr, uint32_t max_count,
                      uint8_t* route, uint32_t* sample_counts, bool batch, uint32_t mode, hipStream_t s) {
     if (max_count == 0) return hipSuccess;
+    const uint32_t* order = nullptr;
+    if (P->shade_order) {
+        /* the route bytes of the hit queue are free until shade writes them: they hold the keys */
+        if (batch)
+            launch_shade_order<true>(P, queue, count_ptr, max_count, route, P->bucket_counts, P->bucket_offsets,
+                                     P->shade_order, s);
+        else
+            launch_shade_order<false>(P, queue, count_ptr, max_count, route, P->bucket_counts, P->bucket_offsets,
+                                      P->shade_order, s);
+        order = P->shade_order;
+    }
     const uint32_t needed = blocks_for(max_count, BF_BLOCK);
     const dim3 grid(needed < P->shade_blocks ? needed : P->shade_blocks);
-    if (batch) launch_shade<true>(P, grid, queue, count_ptr, route, sample_counts, mode, s);
-    else launch_shade<false>(P, grid, queue, count_ptr, route, sample_counts, mode, s);
+    if (batch) launch_shade<true>(P, grid, queue, count_ptr, order, route, sample_counts, mode, s);
+    else launch_shade<false>(P, grid, queue, count_ptr, order, route, sample_counts, mode, s);
     return hipGetLastError();
 }
 

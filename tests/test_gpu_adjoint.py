@@ -153,3 +153,53 @@ def test_autograd_step_is_seen_by_the_next_render(bf, hip):
             roughness.clamp_(0.05, 1.0)
     # darkening the back wall toward a black target lowers the loss
     assert losses[2] < losses[0]
+
+
+def test_batched_adjoint_equals_separate_frames(bf, hip):
+    """Three textured frames in one adjoint call (shared queues, per-frame film gradients, one
+    gradient buffer shared by two frames) give the gradients of three separate calls."""
+    import torch
+
+    rng = np.random.default_rng(9)
+    albedo_a = torch.from_numpy((0.1 + 0.8 * rng.random((8, 8, 3))).astype(np.float32)).cuda()
+    albedo_b = torch.from_numpy((0.1 + 0.8 * rng.random((16, 16, 3))).astype(np.float32)).cuda()
+    rough = torch.full((1, 1, 1), 0.3, dtype=torch.float32, device="cuda")
+    specs = [(albedo_a, 32, 24, 4, 0), (albedo_b, 16, 16, 8, 3), (albedo_a, 24, 24, 2, 5)]
+    options = bf.scenes.transport_options(max_depth=8)
+    frames = []
+    for albedo, w, h, spp, seed in specs:
+        scene, cam, _ = bf.scenes.textured_cornell(w, h, albedo=(albedo.data_ptr(),) + tuple(albedo.shape[1::-1]) + (3,),
+                                                   roughness=(rough.data_ptr(), 1, 1, 1))
+        xs = hip.XpuScene(scene)
+        frame = bf.scenes.frame_desc(cam, spp, seed=seed)
+        tape = hip.Tape(w * h * spp)
+        film_gradient = torch.from_numpy(rng.standard_normal(w * h * 3).astype(np.float32)).cuda()
+        frames.append((xs, frame, tape, film_gradient, albedo))
+    ws = hip.XpuWavefrontTransportWorkspace(1 << 15)
+    hip.trace_xpu_wavefront_batch(ws, [(xs, f, None, tape) for xs, f, tape, _, _ in frames], options)
+
+    def grads_for(k):
+        return [torch.zeros_like(frames[k][4]), torch.zeros_like(rough)]
+
+    separate = []
+    for k, (xs, f, tape, fg, _) in enumerate(frames):
+        g = grads_for(k)
+        hip.trace_xpu_wavefront_adjoint(ws, [(xs, f, tape, fg, g)], options)
+        separate.append(g)
+    # batched: frames 0 and 2 share the albedo texture and accumulate into one buffer
+    shared_albedo, albedo_b_grad, shared_rough = torch.zeros_like(albedo_a), torch.zeros_like(albedo_b), torch.zeros_like(rough)
+    items = [(frames[0][0], frames[0][1], frames[0][2], frames[0][3], [shared_albedo, shared_rough]),
+             (frames[1][0], frames[1][1], frames[1][2], frames[1][3], [albedo_b_grad, shared_rough]),
+             (frames[2][0], frames[2][1], frames[2][2], frames[2][3], [shared_albedo, shared_rough])]
+    report = hip.trace_xpu_wavefront_adjoint(ws, items, options)
+    assert report.failed_paths == 0
+
+    def close(got, want):
+        got, want = got.cpu().double().numpy(), want.cpu().double().numpy()
+        scale = np.abs(want).max()
+        assert scale > 0.0
+        assert np.all(np.abs(got - want) <= 1e-4 * scale + 1e-4 * np.abs(want)), np.abs(got - want).max() / scale
+
+    close(shared_albedo, separate[0][0] + separate[2][0])
+    close(albedo_b_grad, separate[1][0])
+    close(shared_rough, separate[0][1] + separate[1][1] + separate[2][1])
