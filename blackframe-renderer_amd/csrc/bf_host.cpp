@@ -1092,7 +1092,8 @@ int validate_options(const bf_transport_options* o) {
     const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
     if (block_log2 != 0 && (block_log2 < 6 || block_log2 > 8))
         return set_error(BF_INVALID_ARGUMENT, "traversal workgroups of 2^%u lanes are not available (64, 128 or 256)", block_log2);
-    if ((o->launch & ~(BF_LAUNCH_GRAPH | BF_LAUNCH_TRACE_BLOCK_MASK | BF_LAUNCH_GROUP_MASK | BF_LAUNCH_SHADE_ORDER)) != 0)
+    if ((o->launch & ~(BF_LAUNCH_GRAPH | BF_LAUNCH_TRACE_BLOCK_MASK | BF_LAUNCH_GROUP_MASK | BF_LAUNCH_SHADE_ORDER |
+                       BF_LAUNCH_TRACE_ORDER)) != 0)
         return set_error(BF_INVALID_ARGUMENT, "unknown launch bits 0x%08x", o->launch);
     return BF_OK;
 }
@@ -1204,10 +1205,14 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     P.spill = w->spill;
     P.fetch = w->fetch;
     P.gradient_replicas = w->replicas;
-    if (o->launch & BF_LAUNCH_SHADE_ORDER) {
-        P.shade_order = w->queues[2]; /* free while shade runs: the shadow queue is built after it */
-        P.bucket_counts = w->bucket_counts;
-        P.bucket_offsets = w->bucket_offsets;
+    P.bucket_counts = w->bucket_counts;
+    P.bucket_offsets = w->bucket_offsets;
+    /* orders live in queues that are free while the stage runs: the shadow queue is built after
+     * shade (and is empty during closest hit); the hit queue is consumed before the shadow stage */
+    if (o->launch & BF_LAUNCH_SHADE_ORDER) P.shade_order = w->queues[2];
+    if (o->launch & BF_LAUNCH_TRACE_ORDER) {
+        P.closest_order = w->queues[2];
+        P.shadow_order = w->queues[1];
     }
     P.spill_stride = w->spill_lanes;
     P.shade_blocks = w->shade_blocks;
@@ -1546,26 +1551,30 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* w, const bf_frame_job* jobs
         status = run(0u, out.default_ms);
         out.best_ms = out.default_ms;
         out.launch = 0u;
+        auto consider = [&](uint32_t launch) -> int {
+            float ms = 0.0f;
+            const int st = run(launch, ms);
+            if (st != BF_OK) return st;
+            if (out.candidates < 48) {
+                out.tried[out.candidates] = launch;
+                out.tried_ms[out.candidates] = ms;
+            }
+            ++out.candidates;
+            if (ms < out.best_ms) {
+                out.best_ms = ms;
+                out.launch = launch;
+            }
+            return BF_OK;
+        };
+        /* stage 1: launch shapes; stage 2: the work orders on the best shape */
         const uint32_t blocks[3] = {6u, 7u, 8u}, groups[3] = {2u, 4u, 8u};
-        for (uint32_t order = 0; order < 2 && status == BF_OK; ++order)
         for (uint32_t b = 0; b < 3 && status == BF_OK; ++b)
             for (uint32_t g = 0; g < 3 && status == BF_OK; ++g)
-                for (uint32_t graph = 0; graph < (graphs ? 2u : 1u) && status == BF_OK; ++graph) {
-                    const uint32_t launch = groups[g] | BF_LAUNCH_TRACE_BLOCK(blocks[b]) | (graph ? BF_LAUNCH_GRAPH : 0u) |
-                                            (order ? BF_LAUNCH_SHADE_ORDER : 0u);
-                    float ms = 0.0f;
-                    status = run(launch, ms);
-                    if (status != BF_OK) break;
-                    if (out.candidates < 48) {
-                        out.tried[out.candidates] = launch;
-                        out.tried_ms[out.candidates] = ms;
-                    }
-                    ++out.candidates;
-                    if (ms < out.best_ms) {
-                        out.best_ms = ms;
-                        out.launch = launch;
-                    }
-                }
+                for (uint32_t graph = 0; graph < (graphs ? 2u : 1u) && status == BF_OK; ++graph)
+                    status = consider(groups[g] | BF_LAUNCH_TRACE_BLOCK(blocks[b]) | (graph ? BF_LAUNCH_GRAPH : 0u));
+        const uint32_t shape = out.launch;
+        const uint32_t orders[3] = {BF_LAUNCH_SHADE_ORDER, BF_LAUNCH_TRACE_ORDER, BF_LAUNCH_SHADE_ORDER | BF_LAUNCH_TRACE_ORDER};
+        for (uint32_t k = 0; k < 3 && status == BF_OK; ++k) status = consider(shape | orders[k]);
     }
     if (start) (void)hipEventDestroy(start);
     if (stop) (void)hipEventDestroy(stop);

@@ -75,7 +75,9 @@ typedef struct bf_transport_params {
     uint32_t shade_mode;        /* BF_SHADE_* of this launch (host bookkeeping: part of the graph-cache key) */
     float* gradient_replicas;   /* BF_GRADIENT_REPLICAS rows of BF_GRADIENT_ROW floats */
     uint32_t* shade_order;      /* non-null: shade visits the hit queue grouped by closure kind (a free queue) */
-    uint32_t* bucket_counts;    /* per-tile bucket counts / offsets of that grouping */
+    uint32_t* closest_order;    /* non-null: traversal claims rays grouped by direction octant (free queues) */
+    uint32_t* shadow_order;
+    uint32_t* bucket_counts;    /* per-tile bucket counts / offsets of these groupings */
     uint32_t* bucket_offsets;
 } bf_transport_params;
 

@@ -383,8 +383,8 @@ BF_D void reduce_counters(const trav_counters& c, unsigned long long* counters) 
 
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
 __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closest(bf_transport_params P, const uint32_t* queue,
-                                                                      const uint32_t* count_ptr, uint8_t* route,
-                                                                      unsigned long long* counters) {
+                                                                      const uint32_t* count_ptr, const uint32_t* order,
+                                                                      uint8_t* route, unsigned long long* counters) {
     __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
     trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
         BF_D bool pending() const { return stack_depth(*stack) != 0u; }
     };
     lane_loop(
-        n, P.fetch, [&](uint32_t i) { return closest_start<COUNT, BATCH>(P, queue, i, stack, L, ctr); },
+        n, P.fetch, [&](uint32_t i) { return closest_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr); },
         stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { closest_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters);
 }
@@ -1094,8 +1094,8 @@ BF_D void shadow_finish(const bf_transport_params& P, uint8_t* route, lane_ray& 
 /* ---- shadow stage: gather + any-hit + process_shadow (:4729-4908) ---------------------------- */
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
 __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
-                                                                          const uint32_t* count_ptr, uint8_t* route,
-                                                                          unsigned long long* counters) {
+                                                                          const uint32_t* count_ptr, const uint32_t* order,
+                                                                          uint8_t* route, unsigned long long* counters) {
     __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
     trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
@@ -1111,7 +1111,7 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow
         BF_D bool pending() const { return stack_depth(*stack) != 0u; }
     };
     lane_loop(
-        n, P.fetch + 1, [&](uint32_t i) { return shadow_start<COUNT, BATCH>(P, queue, i, stack, L, ctr); },
+        n, P.fetch + 1, [&](uint32_t i) { return shadow_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr); },
         stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { shadow_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters + 3);
 }
@@ -1236,7 +1236,22 @@ BF_D uint32_t shade_bucket(const bf_transport_params& P, const uint32_t* queue, 
     return kind < BF_SHADE_BUCKETS ? kind : 0u;
 }
 
-template <bool BATCH>
+/* Traversal order: rays grouped by direction octant, so that a wave's rays tend to take the same
+ * near/far child order. */
+BF_D uint32_t octant(float4 d) { return (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u); }
+
+#define BF_KEY_SHADE 0u
+#define BF_KEY_RAY 1u
+#define BF_KEY_SHADOW 2u
+
+template <bool BATCH, uint32_t KEY>
+BF_D uint32_t bucket_key(const bf_transport_params& P, const uint32_t* queue, uint32_t i) {
+    if constexpr (KEY == BF_KEY_SHADE) return shade_bucket<BATCH>(P, queue, i);
+    else if constexpr (KEY == BF_KEY_RAY) return octant(P.paths.ray_direction[queue[i]]);
+    else return octant(P.paths.shadow_direction[queue[i]]);
+}
+
+template <bool BATCH, uint32_t KEY>
 __global__ __launch_bounds__(BF_BLOCK) void k_bucket_count(bf_transport_params P, const uint32_t* queue,
                                                            const uint32_t* count_ptr, uint8_t* keys, uint32_t* tile_counts) {
     __shared__ uint32_t wave_counts[BF_SHADE_BUCKETS][BF_BLOCK / 64];
@@ -1245,7 +1260,7 @@ __global__ __launch_bounds__(BF_BLOCK) void k_bucket_count(bf_transport_params P
     const uint32_t wave = threadIdx.x >> 6;
     for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const uint32_t i = tile * BF_BLOCK + threadIdx.x;
-        const uint32_t key = i < n ? shade_bucket<BATCH>(P, queue, i) : BF_SHADE_BUCKETS;
+        const uint32_t key = i < n ? bucket_key<BATCH, KEY>(P, queue, i) : BF_SHADE_BUCKETS;
         if (i < n) keys[i] = (uint8_t)key;
 #pragma unroll
         for (uint32_t b = 0; b < BF_SHADE_BUCKETS; ++b) {
@@ -1627,15 +1642,25 @@ static uint32_t trace_blocks(const bf_transport_params* P, uint32_t max_count, u
 
 } /* extern "C" */
 
+template <bool BATCH, uint32_t KEY>
+static void launch_shade_order(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
+                               uint32_t max_count, uint8_t* keys, uint32_t* tile_counts, uint32_t* tile_offsets,
+                               uint32_t* order, hipStream_t s);
+
 template <bool ANY, bool COUNT, bool BATCH, uint32_t BLOCK>
 static void launch_trace(const bf_transport_params* P, uint32_t max_count, const uint32_t* q, const uint32_t* n,
                          uint8_t* route, unsigned long long* counters, hipStream_t s) {
     const dim3 grid(trace_blocks(P, max_count, BLOCK));
     hipLaunchKernelGGL(k_set_words, dim3(1), dim3(64), 0, s, P->fetch + (ANY ? 1 : 0), 1u, 0u);
+    /* optional direction-octant order; the route bytes are free until the traversal writes them */
+    uint32_t* order = ANY ? P->shadow_order : P->closest_order;
+    if (order)
+        launch_shade_order<BATCH, ANY ? BF_KEY_SHADOW : BF_KEY_RAY>(P, q, n, max_count, route, P->bucket_counts,
+                                                                     P->bucket_offsets, order, s);
     if (ANY)
-        hipLaunchKernelGGL((k_trace_shadow<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, route, counters);
+        hipLaunchKernelGGL((k_trace_shadow<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, order, route, counters);
     else
-        hipLaunchKernelGGL((k_trace_closest<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, route, counters);
+        hipLaunchKernelGGL((k_trace_closest<COUNT, BATCH, BLOCK>), grid, dim3(BLOCK), 0, s, *P, q, n, order, route, counters);
 }
 
 template <bool ANY, bool COUNT, bool BATCH>
@@ -1734,13 +1759,13 @@ static void launch_shade(const bf_transport_params* P, dim3 grid, const uint32_t
                            sample_counts);
 }
 
-template <bool BATCH>
+template <bool BATCH, uint32_t KEY>
 static void launch_shade_order(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                                uint32_t max_count, uint8_t* keys, uint32_t* tile_counts, uint32_t* tile_offsets,
                                uint32_t* order, hipStream_t s) {
     const uint32_t tiles = bfk_compact_tiles(max_count);
     const dim3 grid(tiles < 4096u ? tiles : 4096u);
-    hipLaunchKernelGGL(k_bucket_count<BATCH>, grid, dim3(BF_BLOCK), 0, s, *P, queue, count_ptr, keys, tile_counts);
+    hipLaunchKernelGGL((k_bucket_count<BATCH, KEY>), grid, dim3(BF_BLOCK), 0, s, *P, queue, count_ptr, keys, tile_counts);
     hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, s, count_ptr, tile_counts, tile_offsets);
     hipLaunchKernelGGL(k_bucket_scatter, grid, dim3(BF_BLOCK), 0, s, keys, count_ptr, tile_offsets, order);
 }
@@ -1754,11 +1779,11 @@ hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const 
     if (P->shade_order) {
         /* the route bytes of the hit queue are free until shade writes them: they hold the keys */
         if (batch)
-            launch_shade_order<true>(P, queue, count_ptr, max_count, route, P->bucket_counts, P->bucket_offsets,
-                                     P->shade_order, s);
+            launch_shade_order<true, BF_KEY_SHADE>(P, queue, count_ptr, max_count, route, P->bucket_counts,
+                                                   P->bucket_offsets, P->shade_order, s);
         else
-            launch_shade_order<false>(P, queue, count_ptr, max_count, route, P->bucket_counts, P->bucket_offsets,
-                                      P->shade_order, s);
+            launch_shade_order<false, BF_KEY_SHADE>(P, queue, count_ptr, max_count, route, P->bucket_counts,
+                                                    P->bucket_offsets, P->shade_order, s);
         order = P->shade_order;
     }
     const uint32_t needed = blocks_for(max_count, BF_BLOCK);

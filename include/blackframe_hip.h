@@ -179,9 +179,10 @@ typedef struct bf_launch_tuning {
  * Launch autotuner (no reference counterpart: the reference launches fixed shapes). Times the
  * transport of `jobs` restricted to their first `trial_passes` sample passes (into scratch films
  * the call allocates; the jobs' own buffers are not touched) for every combination of traversal
- * workgroup size {64, 128, 256}, bounce group size {2, 4, 8}, graph replay {off, on} (graph
- * replay only when `options` allow it) and closure-kind shade order {off, on}, with hipEvents on
- * `stream`, and returns the fastest launch word. options->launch is ignored; every other option is used as given. Synchronises.
+ * workgroup size {64, 128, 256}, bounce group size {2, 4, 8} and graph replay {off, on} (graph
+ * replay only when `options` allow it), then the fastest of those with the closure-kind shade
+ * order, the direction-octant traversal order and both, with hipEvents on `stream`, and returns
+ * the fastest launch word. options->launch is ignored; every other option is used as given. Synchronises.
  */
 int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_job* jobs, uint32_t job_count,
                                    const bf_transport_options* options, uint32_t trial_passes,

@@ -317,6 +317,7 @@ typedef struct bf_transport_options {
      *   bits 0-15  bounces issued between two host checks of the live-path count (0: 4);
      *   bits 16-19 log2 of the traversal workgroup size: 6, 7 or 8 (0: 7, 128 lanes);
      *   bit 20     BF_LAUNCH_SHADE_ORDER: shade the hit queue grouped by closure kind;
+     *   bit 21     BF_LAUNCH_TRACE_ORDER: traverse rays grouped by direction octant;
      *   bit 31     BF_LAUNCH_GRAPH: each bounce group is one replayed hipGraph (size rounded up
      *              to even). */
     uint32_t launch;
@@ -327,6 +328,7 @@ typedef struct bf_transport_options {
 #define BF_LAUNCH_TRACE_BLOCK_MASK 0x000F0000u
 #define BF_LAUNCH_TRACE_BLOCK(log2_lanes) ((uint32_t)(log2_lanes) << 16)
 #define BF_LAUNCH_SHADE_ORDER 0x00100000u
+#define BF_LAUNCH_TRACE_ORDER 0x00200000u
 
 /* What a numerically failed path (termination BF_TERM_FAILED) does to the call. The reference
  * aborts the batch on the first failing work item (WavefrontTransport.cpp:880-925), which is

@@ -68,16 +68,16 @@ def test_single_frame_graph_replay_bit_exact(bf, oracle, hip):
         assert np.array_equal(got["film"].host()["film"], expected["film"])
 
 
-@pytest.mark.parametrize("lanes,order", [(64, False), (256, False), (128, True)])
+@pytest.mark.parametrize("lanes,order", [(64, 0), (256, 0), (128, 0x00100000), (128, 0x00200000), (256, 0x00300000)])
 def test_traversal_workgroup_sizes_bit_exact(bf, oracle, hip, lanes, order):
-    """The autotuner's traversal workgroup sizes and the closure-kind shade order change the
-    launch shape, never the result."""
+    """The autotuner's traversal workgroup sizes and the shade / traversal orders
+    (BF_LAUNCH_SHADE_ORDER, BF_LAUNCH_TRACE_ORDER) change the launch shape, never the result."""
     scene, cam = bf.scenes.icosphere_grid(48, 27, grid=4, subdivisions=2)
     frame = bf.scenes.frame_desc(cam, 2, seed=11)
     options = bf.scenes.transport_options(max_depth=16, mis=bf.abi.MIS_POWER)
     options.record_queues = 1
     expected = oracle.scene(scene).render(frame, options, threads=8)
-    options.launch = bf.abi.launch_trace_block(lanes) | (bf.abi.LAUNCH_SHADE_ORDER if order else 0)
+    options.launch = bf.abi.launch_trace_block(lanes) | order
     ws = hip.XpuWavefrontTransportWorkspace(48 * 27 * 2)
     got = hip.trace_xpu_wavefront_transport(ws, hip.XpuScene(scene), frame, options, want_paths=True, want_counters=True)
     assert np.array_equal(path_arrays(got["paths"]), path_arrays(expected["paths"]))
@@ -88,16 +88,16 @@ def test_traversal_workgroup_sizes_bit_exact(bf, oracle, hip, lanes, order):
 
 
 def test_autotuner_choice_is_valid_and_bit_exact(bf, hip):
-    """The autotuner times 36 launch shapes and returns the fastest; rendering with it gives the
-    same bits as the default launch."""
+    """The autotuner times 18 launch shapes, then three work orders on the best one, and returns
+    the fastest; rendering with it gives the same bits as the default launch."""
     items = frames(bf)
     options = bf.scenes.transport_options(max_depth=10, mis=bf.abi.MIS_POWER)
     xscenes = [hip.XpuScene(scene) for scene, _ in items]
     ws = hip.XpuWavefrontTransportWorkspace(1 << 16)
     tuning = hip.autotune_launch(ws, [(x, f) for x, (_, f) in zip(xscenes, items)], options, trial_passes=2)
-    assert tuning.candidates == 36
+    assert tuning.candidates == 21
     assert tuning.best_ms <= tuning.default_ms and tuning.best_ms > 0.0
-    assert tuning.launch == 0 or tuning.launch in list(tuning.tried)[:36]
+    assert tuning.launch == 0 or tuning.launch in list(tuning.tried)[:21]
     reference = hip.trace_xpu_wavefront_batch(ws, [(x, f) for x, (_, f) in zip(xscenes, items)], options, want_paths=True)
     options.launch = tuning.launch
     tuned = hip.trace_xpu_wavefront_batch(ws, [(x, f) for x, (_, f) in zip(xscenes, items)], options, want_paths=True)
