@@ -21,7 +21,10 @@
 #endif
 #define BF_TRACE_BLOCK 128u  /* lanes per traversal workgroup */
 #ifndef BF_TRACE_WAVES_PER_SIMD
-#define BF_TRACE_WAVES_PER_SIMD 6 /* register budget of the traversal kernels (waves per SIMD) */
+#define BF_TRACE_WAVES_PER_SIMD 6 /* register budget of the closest-hit traversal (waves per SIMD) */
+#endif
+#ifndef BF_SHADOW_WAVES_PER_SIMD
+#define BF_SHADOW_WAVES_PER_SIMD 8 /* the any-hit traversal keeps less state and gains from more waves */
 #endif
 #ifndef BF_TRACE_REFILL
 #define BF_TRACE_REFILL 16u /* idle lanes of a traversal wave that trigger a fetch of new rays */

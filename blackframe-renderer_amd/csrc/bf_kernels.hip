@@ -1093,7 +1093,7 @@ BF_D void shadow_finish(const bf_transport_params& P, uint8_t* route, lane_ray& 
 
 /* ---- shadow stage: gather + any-hit + process_shadow (:4729-4908) ---------------------------- */
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
-__global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
+__global__ __launch_bounds__(BLOCK, BF_SHADOW_WAVES_PER_SIMD) void k_trace_shadow(bf_transport_params P, const uint32_t* queue,
                                                                           const uint32_t* count_ptr, const uint32_t* order,
                                                                           uint8_t* route, unsigned long long* counters) {
     __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
