@@ -135,6 +135,16 @@ def batched_section(bf, hip, args, log):
     options.failure_policy = bf.abi.FAILURES_COUNT
     capacity = args.batch_capacity
     ws = hip.XpuWavefrontTransportWorkspace(capacity)
+    # one launch word for both modes (tuned on the batch's first sample passes), so that the
+    # comparison isolates batching
+    launch = bf.abi.LAUNCH_GRAPH | 4
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = hip.autotune_launch(ws, list(zip(xscenes, frames)), options, trial_passes=1)
+        launch = tuning.launch
+        log(f"batch: autotune launch 0x{launch:08x} ({tuning.best_ms:.1f} ms vs default {tuning.default_ms:.1f} ms, "
+            f"{time.perf_counter() - t0:.1f} s)")
+    options.launch = launch
 
     def plain():
         rays = 0
@@ -144,17 +154,13 @@ def batched_section(bf, hip, args, log):
         return rays
 
     def batched():
-        options.launch = bf.abi.LAUNCH_GRAPH | 4
-        try:
-            r = hip.trace_xpu_wavefront_batch(ws, [(x, f, film) for x, f, film in zip(xscenes, frames, films)], options,
-                                              synchronize=False)
-        finally:
-            options.launch = 0
+        r = hip.trace_xpu_wavefront_batch(ws, [(x, f, film) for x, f, film in zip(xscenes, frames, films)], options,
+                                          synchronize=False)
         return r["report"].closest_hit_queries + r["report"].shadow_queries
 
     result = {"workload": (f"{len(items)} seeded scenes, {sum(s.triangle_count for s, _, _ in items)} tris in total "
                            f"(10k-500k each), 128^2-1024^2, 4-256 spp, depth 16, power MIS"),
-              "paths": paths, "paths_in_flight": capacity}
+              "paths": paths, "paths_in_flight": capacity, "launch": f"0x{launch:08x}"}
     for name, fn in (("plain", plain), ("batched", batched)):
         rays = fn()  # warmup
         torch.cuda.synchronize()
@@ -275,6 +281,134 @@ def adjoint_section(bf, hip, args, log):
     return result
 
 
+def fit_section(bf, hip, args, log):
+    """Texture fitting over 16 seeded scenes: per scene an icosphere grid of 5e4-2e5 triangles
+    (uniform), a square resolution uniform in {256..512}, 32 spp, depth 8, balance MIS, roulette
+    off, and one shared material (Lambert + anisotropic gold GGX mixture) bound to its own 1024^2
+    RGB albedo texture and 1024^2 (alpha_x, alpha_y) texture. Targets are rendered with seeded
+    smooth target textures; the fit starts from albedo 0.5 and widths 0.25. One step = one batched
+    taped forward of all 16 frames + film loss + one batched adjoint + one Adam step over the 32
+    textures. Before the loop, the adjoint's directional derivative of the summed loss along a
+    random direction over all albedo texels is checked against a central finite difference
+    (eps 1e-2); the adjoint/forward ratio is reported only when that relative error is <= 1e-2."""
+    import numpy as np
+    import torch
+
+    from blackframe_renderer_amd.autodiff import TextureFit
+
+    t0 = time.perf_counter()
+    specs = bf.scenes.fit_specs(count=16)
+    options = bf.scenes.transport_options(max_depth=8, mis=bf.abi.MIS_BALANCE)
+    options.russian_roulette_enabled = 0
+    options.failure_policy = bf.abi.FAILURES_COUNT
+    size = 1024
+    frames, textures, items, triangles = [], [], [], 0
+    for k, spec in enumerate(specs):
+        albedo_target, widths_target = bf.scenes.fit_textures(spec["seed"], size=size)
+        albedo = torch.from_numpy(albedo_target).cuda().contiguous()
+        widths = torch.from_numpy(widths_target).cuda().contiguous()
+        res = spec["resolution"]
+        scene, cam = bf.scenes.icosphere_grid(res, res, grid=spec["grid"], subdivisions=3, seed=spec["seed"],
+                                              materials="textured", albedo=(albedo.data_ptr(), size, size, 3),
+                                              roughness=(widths.data_ptr(), size, size, 2))
+        triangles += scene.triangle_count
+        xs = hip.XpuScene(scene)
+        frame = bf.scenes.frame_desc(cam, 32, seed=args.seed + k)
+        frames.append([xs, frame, None, [albedo, widths]])
+        textures += [albedo, widths]
+    paths = sum(f[1].camera.width * f[1].camera.height * f[1].samples_per_pixel for f in frames)
+    ws = hip.XpuWavefrontTransportWorkspace(min(paths, args.batch_capacity))
+    log(f"fit: 16 scenes ({triangles} tris, {paths} paths per render) + 32 textures ready in "
+        f"{time.perf_counter() - t0:.1f} s")
+    # targets with the target textures, then the starting point
+    films = [hip.Film(f[1].camera.width, f[1].camera.height) for f in frames]
+    hip.trace_xpu_wavefront_batch(ws, [(f[0], f[1], film) for f, film in zip(frames, films)], options)
+    for f, film in zip(frames, films):
+        w, h = film.width, film.height
+        counts = film.counts.to(torch.float32).clamp_min(1.0).view(h, w, 1)
+        f[2] = (film.float_sum.view(h, w, 3) / counts).contiguous()
+    del films
+    with torch.no_grad():
+        for t in textures:
+            t.fill_(0.5 if t.shape[-1] == 3 else 0.25)
+    if not args.no_autotune:
+        tuning = hip.autotune_launch(ws, [(f[0], f[1]) for f in frames], options, trial_passes=1)
+        options.launch = tuning.launch
+    else:
+        options.launch = bf.abi.LAUNCH_GRAPH | 4
+    params = [(t, 0.02, 0.0, 1.0) if t.shape[-1] == 3 else (t, 0.01, 0.01, 1.0) for t in textures]
+    fit = TextureFit([tuple(f) for f in frames], options, params, workspace=ws)
+
+    # gradient check at the starting point
+    fit.forward()
+    fit.loss()
+    fit.adjoint()
+    albedos = [p for p in fit.parameters if p["value"].shape[-1] == 3]
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    directions = [torch.randn(p["value"].shape, generator=gen, device="cuda") for p in albedos]
+    ad = sum(float((p["gradient"].double() * d.double()).sum()) for p, d in zip(albedos, directions))
+    base = [p["value"].clone() for p in albedos]
+    eps = 1e-2
+
+    def loss_at(sign):
+        with torch.no_grad():
+            for p, b, d in zip(albedos, base, directions):
+                p["value"].copy_(b + sign * eps * d)
+        fit.forward()
+        return float(fit.loss().sum())
+
+    fd = (loss_at(1.0) - loss_at(-1.0)) / (2 * eps)
+    with torch.no_grad():
+        for p, b in zip(albedos, base):
+            p["value"].copy_(b)
+    err = abs(ad - fd) / abs(fd)
+    log(f"fit: directional derivative fd {fd:.6g} adjoint {ad:.6g} (rel err {err:.2e})")
+
+    steps = args.fit_steps
+    events = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+    losses = []
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for k in range(steps):
+        e = events[k]
+        e[0].record()
+        fit.forward()
+        e[1].record()
+        step_losses = fit.loss()
+        e[2].record()
+        fit.adjoint()
+        e[3].record()
+        fit.update()
+        e[4].record()
+        if k == 0 or k == steps - 1:
+            losses.append(step_losses.clone())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    phase = np.array([[events[k][i].elapsed_time(events[k][i + 1]) for i in range(4)] for k in range(steps)])
+    forward_ms, loss_ms, adjoint_ms, adam_ms = phase.mean(axis=0)
+    rays = fit.last_forward_report.closest_hit_queries + fit.last_forward_report.shadow_queries
+    first, last = float(losses[0].sum()), float(losses[-1].sum())
+    log(f"fit: {steps} steps in {elapsed:.1f} s ({elapsed / steps * 1e3:.1f} ms/step: forward {forward_ms:.1f}, "
+        f"loss {loss_ms:.2f}, adjoint {adjoint_ms:.1f}, Adam {adam_ms:.2f} ms); loss {first:.5g} -> {last:.5g}")
+    return {
+        "workload": (f"16 seeded scenes ({triangles} tris in total, 5e4-2e5 each), 256^2-512^2 x 32 spp ({paths} "
+                     "paths), per scene 1024^2 RGB albedo + 1024^2 (alpha_x, alpha_y) textures on a Lambert + "
+                     "anisotropic GGX mixture, depth 8, balance MIS, roulette off, L2 loss, Adam"),
+        "steps": steps,
+        "ms_per_step": round(elapsed / steps * 1e3, 3),
+        "forward_ms": round(float(forward_ms), 3),
+        "loss_ms": round(float(loss_ms), 3),
+        "adjoint_ms": round(float(adjoint_ms), 3),
+        "adam_ms": round(float(adam_ms), 3),
+        "forward_mrays_per_s": round(rays / (forward_ms * 1e-3) / 1e6, 2),
+        "launch": f"0x{options.launch:08x}",
+        "grad_rel_err": float(f"{err:.3e}"),
+        "adjoint_over_forward": round(float(adjoint_ms / forward_ms), 4) if err <= 1e-2 else None,
+        "loss_first_step": first,
+        "loss_last_step": last,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +434,10 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint/forward measurement (prints its JSON object; for profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=5)
+    ap.add_argument("--no-fit", action="store_true", help="skip the 16-scene texture fit")
+    ap.add_argument("--fit-steps", type=int, default=100, help="Adam steps of the texture fit")
+    ap.add_argument("--fit-only", action="store_true",
+                    help="run only the texture fit (prints its JSON object; for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -326,6 +464,9 @@ def main():
     hip = importlib.import_module("blackframe_renderer_amd.hip")
     if args.adjoint_only:
         print(json.dumps(adjoint_section(bf, hip, args, log)), flush=True)
+        return 0
+    if args.fit_only:
+        print(json.dumps(fit_section(bf, hip, args, log)), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -439,6 +580,10 @@ def main():
         line["batched"] = batched_section(bf, hip, args, log)
     if not args.no_adjoint and args.workload == "grid":
         line["adjoint"] = adjoint_section(bf, hip, args, log)
+        torch.cuda.empty_cache()
+    if not args.no_fit and args.workload == "grid":
+        line["fit"] = fit_section(bf, hip, args, log)
+        torch.cuda.empty_cache()
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"CPU baseline with {threads} threads")

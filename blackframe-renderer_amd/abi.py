@@ -212,6 +212,24 @@ class AdjointJob(C.Structure):
     _fields_ = [("film_gradient", C.c_void_p), ("texture_gradients", C.c_void_p * MAX_TEXTURES)]
 
 
+class FilmLossJob(C.Structure):
+    """bf_film_loss_job: one film of blackframe_hip_film_l2_loss (device pointers)."""
+    _fields_ = [("film", C.c_void_p), ("film_counts", C.c_void_p), ("target", C.c_void_p), ("film_gradient", C.c_void_p),
+                ("pixel_count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class AdamParam(C.Structure):
+    """bf_adam_param: one parameter array of blackframe_hip_adam_step (device pointers)."""
+    _fields_ = [("value", C.c_void_p), ("gradient", C.c_void_p), ("first_moment", C.c_void_p),
+                ("second_moment", C.c_void_p), ("count", C.c_uint64), ("learning_rate", C.c_float),
+                ("lower", C.c_float), ("upper", C.c_float), ("reserved", C.c_uint32)]
+
+
+class AdamOptions(C.Structure):
+    _fields_ = [("beta1", C.c_double), ("beta2", C.c_double), ("epsilon", C.c_double), ("step", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
 class PathResult(C.Structure):
     _fields_ = [
         ("radiance", C.c_float * 4), ("beta", C.c_float * 4), ("wavelengths", C.c_float * 4),
@@ -222,6 +240,7 @@ class PathResult(C.Structure):
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
+                       (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32),
                        (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 

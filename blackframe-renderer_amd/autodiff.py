@@ -71,3 +71,97 @@ class _Render(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_image):
         return (None, *ctx.frame.gradients(grad_image.contiguous()))
+
+
+class TextureFit:
+    """Fits scene textures to target images with projected Adam on the L2 film loss.
+
+    One step(), all on the device:
+      1. one batched taped forward render of every frame (blackframe_hip_trace_wavefront_batch);
+      2. per frame, the loss mean((image - target)^2) and its film gradient
+         (blackframe_hip_film_l2_loss, image = film sums / sample counts);
+      3. one batched path-replay adjoint of every frame (blackframe_hip_trace_wavefront_adjoint);
+      4. one projected Adam step over every fitted texture (blackframe_hip_adam_step).
+    The host waits only where the batched calls read live-path counts.
+
+        fit = TextureFit(frames, options, [(albedo, 0.02, 0.0, 1.0), (widths, 0.01, 0.01, 1.0)])
+        for _ in range(100):
+            losses = fit.step()
+
+    frames: sequence of (XpuScene, FrameDesc, target, textures) with target a (height, width, 3)
+    float32 CUDA tensor of per-pixel means and textures the float32 CUDA tensors bound to the
+    scene's texture slots, in slot order. parameters: (tensor, learning_rate, lower, upper) for
+    each texture to fit; bound textures not listed stay fixed. A tensor bound to several frames
+    gets the sum of their gradients.
+    """
+
+    def __init__(self, frames, options, parameters, workspace=None, betas=(0.9, 0.999), epsilon=1e-8):
+        self.options = options
+        self.betas, self.epsilon = betas, epsilon
+        self.parameters = []
+        by_pointer = {}
+        for tensor, lr, lower, upper in parameters:
+            if tensor.dtype != torch.float32 or not tensor.is_cuda or not tensor.is_contiguous():
+                raise ValueError("fitted textures must be contiguous float32 CUDA tensors")
+            entry = dict(value=tensor, gradient=torch.zeros_like(tensor), first=torch.zeros_like(tensor),
+                         second=torch.zeros_like(tensor), lr=float(lr), lower=float(lower), upper=float(upper))
+            self.parameters.append(entry)
+            by_pointer[tensor.data_ptr()] = entry
+        self.frames = []
+        paths = 0
+        for scene, frame, target, textures in frames:
+            cam = frame.camera
+            n = cam.width * cam.height * frame.samples_per_pixel
+            paths = max(paths, n)
+            if tuple(target.shape) != (cam.height, cam.width, 3) or target.dtype != torch.float32:
+                raise ValueError("targets are (height, width, 3) float32 CUDA tensors")
+            grads = [by_pointer[t.data_ptr()]["gradient"] if t.data_ptr() in by_pointer else None for t in textures]
+            self.frames.append(dict(scene=scene, frame=frame, target=target.contiguous().view(-1),
+                                    film=hip.Film(cam.width, cam.height), tape=hip.Tape(n),
+                                    film_gradient=torch.zeros(cam.width * cam.height * 3, dtype=torch.float32,
+                                                              device="cuda"),
+                                    gradients=grads, pixels=cam.width * cam.height))
+        self.workspace = workspace if workspace is not None else hip.XpuWavefrontTransportWorkspace(paths)
+        self.losses = torch.zeros(len(self.frames), dtype=torch.float64, device="cuda")
+        self.step_count = 0
+        self.last_forward_report = None
+        self.last_adjoint_report = None
+
+    def forward(self):
+        """Phase 1: the taped forward render of every frame (film buffers cleared first)."""
+        for f in self.frames:
+            for t in (f["film"].float_sum, f["film"].compensated, f["film"].counts):
+                t.zero_()
+        r = hip.trace_xpu_wavefront_batch(self.workspace, [(f["scene"], f["frame"], f["film"], f["tape"])
+                                                           for f in self.frames], self.options, synchronize=False)
+        self.last_forward_report = r["report"]
+
+    def loss(self):
+        """Phase 2: per-frame mean squared error and film gradients; returns the loss tensor."""
+        for k, f in enumerate(self.frames):
+            hip.film_l2_loss([(f["film"], f["target"], f["film_gradient"])], 1.0 / (3.0 * f["pixels"]),
+                             losses=self.losses[k:k + 1])
+        return self.losses
+
+    def adjoint(self):
+        """Phase 3: texture gradients of the summed loss, replayed from the tapes."""
+        for p in self.parameters:
+            p["gradient"].zero_()
+        self.last_adjoint_report = hip.trace_xpu_wavefront_adjoint(
+            self.workspace, [(f["scene"], f["frame"], f["tape"], f["film_gradient"], f["gradients"])
+                             for f in self.frames], self.options, synchronize=False)
+
+    def update(self):
+        """Phase 4: one projected Adam step of every fitted texture."""
+        self.step_count += 1
+        hip.adam_step([(p["value"], p["gradient"], p["first"], p["second"], p["lr"], p["lower"], p["upper"])
+                       for p in self.parameters], self.step_count, self.betas[0], self.betas[1], self.epsilon)
+
+    def step(self):
+        """forward, loss, adjoint, update; returns the per-frame losses of this step's render
+        (a CUDA float64 tensor, valid once the stream reaches it)."""
+        self.forward()
+        losses = self.loss()
+        self.adjoint()
+        self.update()
+        return losses

@@ -1593,4 +1593,63 @@ int blackframe_hip_trace_wavefront_adjoint(bf_hip_workspace* w, const bf_frame_j
     return trace_jobs(w, jobs, adjoints, job_count, o, report, nullptr, (hipStream_t)stream_handle);
 }
 
+int blackframe_hip_film_l2_loss(const bf_film_loss_job* jobs, uint32_t job_count, float scale, double* losses,
+                                void* stream_handle) {
+    if (job_count == 0) return BF_OK;
+    if (!jobs || !losses) return set_error(BF_INVALID_ARGUMENT, "NULL jobs or losses");
+    if (!std::isfinite(scale)) return set_error(BF_INVALID_ARGUMENT, "non-finite loss scale");
+    for (uint32_t j = 0; j < job_count; ++j) {
+        const bf_film_loss_job& J = jobs[j];
+        if (J.reserved != 0) return set_error(BF_INVALID_ARGUMENT, "film loss job %u: reserved word must be zero", j);
+        if (J.pixel_count && (!J.film || !J.film_counts || !J.target || !J.film_gradient))
+            return set_error(BF_INVALID_ARGUMENT, "film loss job %u: NULL buffer", j);
+    }
+    const hipStream_t s = (hipStream_t)stream_handle;
+    HIP_TRY(bfk_zero_doubles(losses, job_count, s));
+    const float two_scale = (float)(2.0 * (double)scale);
+    for (uint32_t j = 0; j < job_count; ++j) {
+        const bf_film_loss_job& J = jobs[j];
+        HIP_TRY(bfk_film_l2(J.film, J.film_counts, J.target, J.film_gradient, J.pixel_count, two_scale, (double)scale,
+                            losses + j, s));
+    }
+    return BF_OK;
+}
+
+int blackframe_hip_adam_step(const bf_adam_param* params, uint32_t param_count, const bf_adam_options* o,
+                             void* stream_handle) {
+    if (param_count == 0) return BF_OK;
+    if (!params || !o) return set_error(BF_INVALID_ARGUMENT, "NULL parameters or options");
+    if (o->step == 0) return set_error(BF_INVALID_ARGUMENT, "Adam steps count from 1");
+    if (o->reserved != 0) return set_error(BF_INVALID_ARGUMENT, "Adam options: reserved word must be zero");
+    if (!(o->beta1 >= 0.0 && o->beta1 < 1.0) || !(o->beta2 >= 0.0 && o->beta2 < 1.0))
+        return set_error(BF_INVALID_ARGUMENT, "Adam betas must lie in [0, 1)");
+    if (!(o->epsilon > 0.0) || !std::isfinite(o->epsilon)) return set_error(BF_INVALID_ARGUMENT, "Adam epsilon must be positive");
+    for (uint32_t k = 0; k < param_count; ++k) {
+        const bf_adam_param& P = params[k];
+        if (P.reserved != 0) return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: reserved word must be zero", k);
+        if (P.count && (!P.value || !P.gradient || !P.first_moment || !P.second_moment))
+            return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: NULL buffer", k);
+        if (!std::isfinite(P.learning_rate) || !(P.lower <= P.upper))
+            return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: bad learning rate or bounds", k);
+    }
+    const double bias1 = 1.0 - std::pow(o->beta1, (double)o->step);
+    const double bias2 = 1.0 - std::pow(o->beta2, (double)o->step);
+    if (!(bias1 > 0.0) || !(bias2 > 0.0)) return set_error(BF_INVALID_ARGUMENT, "Adam bias correction vanishes");
+    const hipStream_t s = (hipStream_t)stream_handle;
+    for (uint32_t k = 0; k < param_count; ++k) {
+        const bf_adam_param& P = params[k];
+        const bfk_adam_coefficients c{(float)o->beta1,
+                                      (float)(1.0 - o->beta1),
+                                      (float)o->beta2,
+                                      (float)(1.0 - o->beta2),
+                                      (float)((double)P.learning_rate / bias1),
+                                      (float)std::sqrt(bias2),
+                                      (float)o->epsilon,
+                                      P.lower,
+                                      P.upper};
+        HIP_TRY(bfk_adam(P.value, P.gradient, P.first_moment, P.second_moment, P.count, &c, s));
+    }
+    return BF_OK;
+}
+
 } /* extern "C" */

@@ -119,6 +119,16 @@ hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s);
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
 /* dst[e] += sum over the replica rows of replicas[row * BF_GRADIENT_ROW + offset + e], e < count */
 hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count, float* dst, hipStream_t s);
+/* Per-array coefficients of one Adam step, rounded once on the host. */
+typedef struct bfk_adam_coefficients {
+    float beta1, one_minus_beta1, beta2, one_minus_beta2, step_size, bias2_sqrt, epsilon, lower, upper;
+} bfk_adam_coefficients;
+hipError_t bfk_adam(float* value, const float* gradient, float* first, float* second, uint64_t count,
+                    const bfk_adam_coefficients* c, hipStream_t s);
+/* loss[0] += scale * sum d^2 and gradient = (two_scale * d) / max(count, 1), d = film / max(count, 1) - target */
+hipError_t bfk_film_l2(const float* film, const uint32_t* counts, const float* target, float* gradient, uint32_t pixels,
+                       float two_scale, double scale, double* loss, hipStream_t s);
+hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s);
 #ifdef __cplusplus
 }
 #endif

@@ -1580,6 +1580,79 @@ __global__ __launch_bounds__(BF_BLOCK) void k_reduce_replicas(const float* repli
     dst[e] += sum;
 }
 
+/* ---- texture fitting: L2 film loss and Adam ------------------------------------------------- */
+
+__device__ inline void adam_element(float& p, float g, float& m, float& v, const bfk_adam_coefficients& c) {
+    m = c.beta1 * m + c.one_minus_beta1 * g;
+    v = c.beta2 * v + c.one_minus_beta2 * (g * g);
+    const float den = sqrtf(v) / c.bias2_sqrt + c.epsilon;
+    p = p - c.step_size * (m / den);
+    p = fminf(fmaxf(p, c.lower), c.upper);
+}
+
+/* VEC: the four arrays are 16-byte aligned and the body runs on float4 (the tail of count % 4
+ * elements is done by the first threads of the grid). */
+template <bool VEC>
+__global__ __launch_bounds__(BF_BLOCK) void k_adam(float* value, const float* gradient, float* first, float* second,
+                                                   uint64_t count, bfk_adam_coefficients c) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t scalar_begin = 0;
+    if (VEC) {
+        const uint64_t quads = count / 4u;
+        for (uint64_t i = t; i < quads; i += stride) {
+            float4 p = reinterpret_cast<float4*>(value)[i];
+            const float4 g = reinterpret_cast<const float4*>(gradient)[i];
+            float4 m = reinterpret_cast<float4*>(first)[i];
+            float4 v = reinterpret_cast<float4*>(second)[i];
+            adam_element(p.x, g.x, m.x, v.x, c);
+            adam_element(p.y, g.y, m.y, v.y, c);
+            adam_element(p.z, g.z, m.z, v.z, c);
+            adam_element(p.w, g.w, m.w, v.w, c);
+            reinterpret_cast<float4*>(value)[i] = p;
+            reinterpret_cast<float4*>(first)[i] = m;
+            reinterpret_cast<float4*>(second)[i] = v;
+        }
+        scalar_begin = quads * 4u;
+    }
+    for (uint64_t i = scalar_begin + t; i < count; i += stride) {
+        float p = value[i], m = first[i], v = second[i];
+        adam_element(p, gradient[i], m, v, c);
+        value[i] = p;
+        first[i] = m;
+        second[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(BF_BLOCK) void k_film_l2(const float* film, const uint32_t* counts, const float* target,
+                                                      float* gradient, uint32_t pixels, float two_scale, double scale,
+                                                      double* loss) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sum = 0.0;
+    if (i < pixels) {
+        const float c = (float)(counts[i] > 1u ? counts[i] : 1u);
+        for (uint32_t ch = 0; ch < 3u; ++ch) {
+            const float d = film[(size_t)i * 3u + ch] / c - target[(size_t)i * 3u + ch];
+            gradient[(size_t)i * 3u + ch] = (two_scale * d) / c;
+            sum += (double)d * (double)d;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o, 64);
+    __shared__ double partial[BF_BLOCK / 64];
+    if ((threadIdx.x & 63u) == 0) partial[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double block = 0.0;
+        for (uint32_t w = 0; w < BF_BLOCK / 64; ++w) block += partial[w];
+        atomicAdd(loss, scale * block);
+    }
+}
+
+__global__ void k_zero_doubles(double* dst, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = 0.0;
+}
+
 __global__ void k_set_words(uint32_t* dst, uint32_t count, uint32_t value) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = value;
@@ -1854,6 +1927,34 @@ hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t d
 hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count, float* dst, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_replicas, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, replicas, offset, count, dst);
+    return hipGetLastError();
+}
+
+hipError_t bfk_adam(float* value, const float* gradient, float* first, float* second, uint64_t count,
+                    const bfk_adam_coefficients* c, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    const bool vec = ((reinterpret_cast<uintptr_t>(value) | reinterpret_cast<uintptr_t>(gradient) |
+                       reinterpret_cast<uintptr_t>(first) | reinterpret_cast<uintptr_t>(second)) & 15u) == 0;
+    const uint64_t work = vec ? (count + 3u) / 4u : count;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + BF_BLOCK - 1u) / BF_BLOCK, 16384u);
+    if (vec)
+        hipLaunchKernelGGL(k_adam<true>, dim3(blocks), dim3(BF_BLOCK), 0, s, value, gradient, first, second, count, *c);
+    else
+        hipLaunchKernelGGL(k_adam<false>, dim3(blocks), dim3(BF_BLOCK), 0, s, value, gradient, first, second, count, *c);
+    return hipGetLastError();
+}
+
+hipError_t bfk_film_l2(const float* film, const uint32_t* counts, const float* target, float* gradient, uint32_t pixels,
+                       float two_scale, double scale, double* loss, hipStream_t s) {
+    if (pixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_film_l2, dim3(blocks_for(pixels, BF_BLOCK)), dim3(BF_BLOCK), 0, s, film, counts, target, gradient,
+                       pixels, two_scale, scale, loss);
+    return hipGetLastError();
+}
+
+hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_doubles, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, dst, count);
     return hipGetLastError();
 }
 

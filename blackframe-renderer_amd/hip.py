@@ -62,6 +62,8 @@ def library():
                                                            vp]
     lib.blackframe_hip_autotune_launch.argtypes = [vp, C.POINTER(abi.FrameJob), u32, C.POINTER(abi.TransportOptions), u32,
                                                    C.POINTER(abi.LaunchTuning), vp]
+    lib.blackframe_hip_film_l2_loss.argtypes = [C.POINTER(abi.FilmLossJob), u32, C.c_float, vp, vp]
+    lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
     lib.blackframe_query_extension.restype = C.c_uint32
     lib.blackframe_query_extension.argtypes = [C.POINTER(abi.ExtensionQuery), C.POINTER(abi.ExtensionApi)]
     lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
@@ -344,6 +346,51 @@ def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchron
     if synchronize:
         torch.cuda.synchronize()
     return report
+
+
+def film_l2_loss(items, scale, losses=None, stream=None):
+    """L2 loss of films against target images and dLoss/d(film sums) (blackframe_hip_film_l2_loss).
+
+    items: sequence of (Film, target, film_gradient) with target and film_gradient float32 CUDA
+    tensors of width * height * 3 elements. Returns the float64 CUDA tensor of per-film losses
+    (`losses` if given), without synchronising."""
+    import torch
+
+    jobs = (abi.FilmLossJob * len(items))()
+    for k, (film, target, gradient) in enumerate(items):
+        n = film.width * film.height
+        for t in (target, gradient):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.numel() != n * 3:
+                raise XpuError(1, f"film loss item {k}: target and gradient must be contiguous float32 CUDA tensors of {n * 3} elements")
+        jobs[k].film = film.float_sum.data_ptr()
+        jobs[k].film_counts = film.counts.data_ptr()
+        jobs[k].target = target.data_ptr()
+        jobs[k].film_gradient = gradient.data_ptr()
+        jobs[k].pixel_count = n
+    if losses is None:
+        losses = torch.empty(len(items), dtype=torch.float64, device="cuda")
+    _check(library().blackframe_hip_film_l2_loss(jobs, len(items), float(scale), C.c_void_p(losses.data_ptr()),
+                                                 _stream_handle(stream)))
+    return losses
+
+
+def adam_step(params, step, beta1=0.9, beta2=0.999, epsilon=1e-8, stream=None):
+    """One projected Adam step (blackframe_hip_adam_step). params: sequence of (value, gradient,
+    first_moment, second_moment, learning_rate, lower, upper) with float32 contiguous CUDA tensors
+    of one shape per entry; step counts from 1. Does not synchronise."""
+    import torch
+
+    records = (abi.AdamParam * len(params))()
+    for k, (value, gradient, m, v, lr, lower, upper) in enumerate(params):
+        for t in (value, gradient, m, v):
+            if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.numel() != value.numel():
+                raise XpuError(1, f"Adam parameter {k}: tensors must be contiguous float32 CUDA tensors of one size")
+        records[k].value, records[k].gradient = value.data_ptr(), gradient.data_ptr()
+        records[k].first_moment, records[k].second_moment = m.data_ptr(), v.data_ptr()
+        records[k].count = value.numel()
+        records[k].learning_rate, records[k].lower, records[k].upper = lr, lower, upper
+    options = abi.AdamOptions(beta1, beta2, epsilon, step, 0)
+    _check(library().blackframe_hip_adam_step(records, len(params), C.byref(options), _stream_handle(stream)))
 
 
 def autotune_launch(workspace, items, options, trial_passes=1, stream=None):

@@ -494,14 +494,25 @@ COPPER_K = [1.94, 2.33, 2.58, 2.58, 3.28, 3.90, 4.42, 4.95]
 
 
 def icosphere_grid(width=1920, height=1080, grid=32, subdivisions=3, seed=0x5EED_1C05, emitter_count=64,
-                   materials="mixed"):
+                   materials="mixed", albedo=None, roughness=None):
     """32x32 displaced icospheres (see module docstring). materials='lambert' keeps every object
-    Lambertian (used where a closure kind is not yet available)."""
+    Lambertian (used where a closure kind is not yet available). materials='textured' gives every
+    object one shared material bound to two textures: a 50/50 mixture of a Lambert closure and an
+    anisotropic gold GGX conductor, both weighted by the RGB `albedo` texture (slot 0), with the
+    conductor's widths (alpha_x, alpha_y) read from the 2-channel `roughness` texture (slot 1);
+    each texture is a host (h, w, c) array or a device tuple (pointer, width, height, channels).
+    The geometry does not depend on `materials`."""
     rng = SplitMix64(seed)
     s = Scene()
     base_vertices, faces = _icosphere(subdivisions)
     spacing = 2.6
     emitter_material = s.add_material(900, [lambert(0.0)], emission=40.0)
+    shared = None
+    if materials == "textured":
+        slots = [s.add_texture(t) if isinstance(t, np.ndarray) else s.add_texture(device=t) for t in (albedo, roughness)]
+        shared = s.add_material(1999, [dict(lambert(1.0, probability=0.5), weight_texture=slots[0]),
+                                       dict(rough_conductor(1.0, GOLD_ETA, GOLD_K, 0.3, 0.3, probability=0.5),
+                                            weight_texture=slots[0], roughness_texture=slots[1])])
     for gy in range(grid):
         for gx in range(grid):
             obj = gy * grid + gx
@@ -529,7 +540,7 @@ def icosphere_grid(width=1920, height=1080, grid=32, subdivisions=3, seed=0x5EED
                                             roughness * roughness)]
             else:
                 closures = [rough_dielectric(1.0, 1.0, 1.5, roughness * roughness)]
-            m = s.add_material(2000 + obj, closures)
+            m = shared if shared is not None else s.add_material(2000 + obj, closures)
             x = (gx - (grid - 1) / 2.0) * spacing
             z = (gy - (grid - 1) / 2.0) * spacing
             s.add_instance(3000 + obj, 3000 + obj, g, m, affine(translation=(x, 0.0, z)))
@@ -564,3 +575,37 @@ def batch_scenes(count=32, seed=0xC2_5EED, min_triangles=1.0e4, max_triangles=5.
                                        seed=rng.next_u64() & 0xFFFFFFFF)
         out.append((scene, camera, spp))
     return out
+
+
+def fit_specs(count=16, seed=0xC4_5EED, min_triangles=5.0e4, max_triangles=2.0e5, min_resolution=256,
+              max_resolution=512):
+    """Seeded texture-fitting batch: per scene a triangle count uniform in [min, max] (icosphere
+    grids of subdivision 3, 1 280 triangles per object), a square resolution uniform in
+    {min_resolution..max_resolution} and a scene seed. Returns a list of dicts (grid, resolution,
+    seed); build each with icosphere_grid(..., materials="textured")."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        triangles = min_triangles + rng.uniform() * (max_triangles - min_triangles)
+        grid = max(1, int(round(math.sqrt(triangles / 1280.0))))
+        resolution = min_resolution + min(max_resolution - min_resolution, int(rng.uniform() * (max_resolution - min_resolution + 1)))
+        out.append(dict(grid=grid, resolution=resolution, seed=rng.next_u64() & 0xFFFFFFFF))
+    return out
+
+
+def fit_textures(seed, size=1024):
+    """Seeded smooth target textures of a texture fit: RGB albedo in [0.1, 0.9] and GGX widths
+    (alpha_x, alpha_y) in [0.05, 0.6], each channel a product of two seeded sinusoids over (u, v).
+    Returns float32 arrays (size, size, 3) and (size, size, 2)."""
+    rng = SplitMix64(seed)
+    u = (np.arange(size, dtype=np.float64) + 0.5) / size
+
+    def channel(lo, hi):
+        fu, fv = 1 + int(rng.uniform() * 6), 1 + int(rng.uniform() * 6)
+        pu, pv = rng.uniform(), rng.uniform()
+        pattern = np.outer(np.cos(2 * math.pi * (fv * u + pv)), np.sin(2 * math.pi * (fu * u + pu)))
+        return lo + (hi - lo) * (0.5 + 0.5 * pattern)
+
+    albedo = np.stack([channel(0.1, 0.9) for _ in range(3)], axis=2).astype(np.float32)
+    widths = np.stack([channel(0.05, 0.6) for _ in range(2)], axis=2).astype(np.float32)
+    return albedo, widths

@@ -188,8 +188,69 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_j
                                    const bf_transport_options* options, uint32_t trial_passes,
                                    bf_launch_tuning* result, void* stream);
 
+/* One frame of blackframe_hip_film_l2_loss (device pointers; 40 bytes). */
+typedef struct bf_film_loss_job {
+    const float* film;            /* [pixels * 3] float RGB sums, the film of a transport call */
+    const uint32_t* film_counts;  /* [pixels] samples per pixel */
+    const float* target;          /* [pixels * 3] target image (per-pixel means) */
+    float* film_gradient;         /* [pixels * 3] out: dLoss/d(film sums), the adjoint's input */
+    uint32_t pixel_count;
+    uint32_t reserved;            /* must be 0 */
+} bf_film_loss_job;
+
+/*
+ * L2 image loss of rendered films and its gradient (no counterpart in the reference, which is
+ * forward-only; this is the loss the adjoint's texture fits use). Per frame j, with
+ * mean = film / max(count, 1) and d = mean - target per channel:
+ *   losses[j]        = scale * sum(d * d)            (binary64, summed in an unspecified order)
+ *   film_gradient[i] = ((2 * scale) * d) / max(count, 1)   (binary32, in this order)
+ * `losses` is device memory of job_count doubles and is overwritten. Kernels only (no memset, no
+ * allocation, no synchronisation), so a caller may capture the call in a graph.
+ */
+int blackframe_hip_film_l2_loss(const bf_film_loss_job* jobs, uint32_t job_count, float scale, double* losses,
+                                void* stream);
+
+/* One parameter array of blackframe_hip_adam_step (device pointers; 56 bytes). */
+typedef struct bf_adam_param {
+    float* value;                 /* [count] parameters, updated in place */
+    const float* gradient;        /* [count] */
+    float* first_moment;          /* [count] m, zero before step 1 */
+    float* second_moment;         /* [count] v, zero before step 1 */
+    uint64_t count;
+    float learning_rate;
+    float lower;                  /* the update is projected onto [lower, upper] */
+    float upper;
+    uint32_t reserved;            /* must be 0 */
+} bf_adam_param;
+
+typedef struct bf_adam_options {
+    double beta1;
+    double beta2;
+    double epsilon;
+    uint32_t step;                /* 1 for the first step */
+    uint32_t reserved;            /* must be 0 */
+} bf_adam_options;
+
+/*
+ * One Adam step (Kingma & Ba 2015, the bias-corrected form torch.optim.Adam uses) with a box
+ * projection, over every parameter array (no counterpart in the reference). Per element, in
+ * binary32 and in this order:
+ *   m = beta1 * m + (1 - beta1) * g
+ *   v = beta2 * v + (1 - beta2) * (g * g)
+ *   p = p - step_size * (m / (sqrt(v) / sqrt(1 - beta2^step) + epsilon)),  p = min(max(p, lower), upper)
+ * with step_size = learning_rate / (1 - beta1^step). beta1, beta2, 1 - beta1, 1 - beta2, epsilon,
+ * step_size and sqrt(1 - beta2^step) are computed in binary64 on the host and rounded once to
+ * binary32. One kernel launch per array; arrays must not overlap.
+ */
+int blackframe_hip_adam_step(const bf_adam_param* params, uint32_t param_count, const bf_adam_options* options,
+                             void* stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
+
+BF_STATIC_ASSERT(sizeof(bf_film_loss_job) == 40, "film loss job is 40 bytes");
+BF_STATIC_ASSERT(sizeof(bf_adam_param) == 56, "Adam parameter record is 56 bytes");
+BF_STATIC_ASSERT(sizeof(bf_adam_options) == 32, "Adam options are 32 bytes");
 
 #endif /* BLACKFRAME_HIP_H */

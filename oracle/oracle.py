@@ -93,6 +93,44 @@ def load(abi):
     return lib
 
 
+# ---- texture fitting: no reference counterpart (the reference renders forward only) -----------
+# These restate the binary32 operation order of blackframe_hip_film_l2_loss and
+# blackframe_hip_adam_step. They are pinned by torch's own implementations (torch.nn.functional
+# .mse_loss and torch.optim.Adam) within a few ulps, and by closed forms (tests/test_fit_oracle.py).
+
+
+def film_l2_loss(film, counts, target, scale):
+    """film, target: float32 [pixels * 3]; counts: uint32 [pixels]. Returns (loss as float64,
+    film gradient as float32) with d = film / max(count, 1) - target:
+    loss = scale * sum(d^2) (binary64), gradient = ((2 * scale) * d) / max(count, 1) (binary32)."""
+    scale = float(np.float32(scale))  # the ABI takes the scale as a binary32
+    c = np.repeat(np.maximum(np.asarray(counts, dtype=np.uint32), 1).astype(np.float32), 3)
+    d = np.asarray(film, dtype=np.float32) / c - np.asarray(target, dtype=np.float32)
+    two_scale = np.float32(2.0 * scale)
+    gradient = (two_scale * d) / c
+    dd = d.astype(np.float64)
+    return scale * float(np.sum(dd * dd)), gradient
+
+
+def adam_step(value, gradient, first, second, learning_rate, step, beta1=0.9, beta2=0.999, epsilon=1e-8,
+              lower=-np.inf, upper=np.inf):
+    """One projected Adam step on float32 arrays; returns new (value, first, second)."""
+    f32 = np.float32
+    beta1, beta2 = float(beta1), float(beta2)
+    b1, b2 = f32(beta1), f32(beta2)
+    bias1 = 1.0 - beta1 ** step
+    bias2 = 1.0 - beta2 ** step
+    omb1, omb2 = f32(1.0 - beta1), f32(1.0 - beta2)
+    step_size, bias2_sqrt = f32(float(f32(learning_rate)) / bias1), f32(np.sqrt(bias2))
+    g = np.asarray(gradient, dtype=np.float32)
+    m = b1 * np.asarray(first, dtype=np.float32) + omb1 * g
+    v = b2 * np.asarray(second, dtype=np.float32) + omb2 * (g * g)
+    den = np.sqrt(v) / bias2_sqrt + f32(epsilon)
+    p = np.asarray(value, dtype=np.float32) - step_size * (m / den)
+    p = np.minimum(np.maximum(p, f32(lower)), f32(upper))
+    return p.astype(np.float32), m, v
+
+
 class Oracle:
     def __init__(self, abi):
         self.abi = abi

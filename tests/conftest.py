@@ -49,6 +49,11 @@ def oracle(bf):
 
 
 @pytest.fixture(scope="session")
+def oracle_module():
+    return load_oracle_module()
+
+
+@pytest.fixture(scope="session")
 def hip(bf):
     import importlib
 
