@@ -23,6 +23,15 @@
 #ifndef BF_TRACE_WAVES_PER_SIMD
 #define BF_TRACE_WAVES_PER_SIMD 6 /* register budget of the closest-hit traversal (waves per SIMD) */
 #endif
+#ifndef BF_LEAF_PHASE
+/* Traversal waves step interior nodes and leaves in separate iterations: leaves when at least this
+ * many lanes (or every active lane) have one next, interior nodes otherwise (0: every kind in
+ * every iteration, each under its own lane mask). 16 measured best of {1, 4, 8, 12, 16, 32, 64}. */
+#define BF_LEAF_PHASE 16u
+#endif
+#ifndef BF_WORLD_INV_CACHE
+#define BF_WORLD_INV_CACHE 1 /* closest-hit lanes keep their world-space reciprocals in LDS */
+#endif
 #ifndef BF_SHADOW_WAVES_PER_SIMD
 #define BF_SHADOW_WAVES_PER_SIMD 8 /* the any-hit traversal keeps less state and gains from more waves */
 #endif

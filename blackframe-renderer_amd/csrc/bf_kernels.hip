@@ -230,10 +230,13 @@ struct lane_ray {
     bool occluded;
 };
 
-/* World-space ray of a slot, re-read from the path columns (closest-hit rays). */
+/* World-space ray of a slot, re-read from the path columns (closest-hit rays). `inv` (optional)
+ * holds the lane's world-space reciprocals, entry k at inv[k * stride]. */
 struct closest_world {
     const bf_dev_paths* paths;
     uint32_t slot;
+    double* inv;
+    uint32_t stride;
     BF_D void operator()(v3f& o, v3f& d) const {
         const float4 a = paths->ray_origin[slot];
         const float4 b = paths->ray_direction[slot];
@@ -241,6 +244,17 @@ struct closest_world {
         d = mk(b.x, b.y, b.z);
     }
 };
+
+BF_D void world_space(const closest_world& world, trav_space& s) {
+    world(s.o, s.d);
+    if (world.inv) {
+        s.inv[0] = world.inv[0];
+        s.inv[1] = world.inv[world.stride];
+        s.inv[2] = world.inv[2u * world.stride];
+    } else {
+        make_space(s.o, s.d, s);
+    }
+}
 
 /* Starts queue item i; returns the traversal status of the root test. */
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
@@ -265,10 +279,11 @@ BF_D uint32_t closest_start(const bf_transport_params& P, const uint32_t* queue,
 
 template <bool COUNT, bool BATCH, uint32_t BLOCK>
 BF_D uint32_t closest_step(const bf_transport_params& P, trav_stack<BF_SHORT_STACK, BLOCK>& stack, lane_ray& L,
-                           trav_counters& ctr) {
+                           trav_counters& ctr, double* inv) {
     const bf_dev_job& J = job_at<BATCH>(P, L.slot);
     bool occluded;
-    return traverse_step<false, COUNT>(J.scene, closest_world{&P.paths, L.slot}, stack, L.ts, L.best, occluded, ctr);
+    return traverse_step<false, COUNT>(J.scene, closest_world{&P.paths, L.slot, inv, BLOCK}, stack, L.ts, L.best,
+                                       occluded, ctr);
 }
 
 /* Classifies a finished closest-hit query: hit (route 1, barycentrics re-derived for the winner),
@@ -280,7 +295,7 @@ BF_D void closest_finish(const bf_transport_params& P, uint8_t* route, lane_ray&
     trav_hit& best = L.best;
     if (st == BF_HIT_MISS && best.present) {
         v3f o, d;
-        closest_world{&P.paths, slot}(o, d);
+        closest_world{&P.paths, slot, nullptr, 0u}(o, d);
         st = closest_hit_barycentrics(J.scene, o, d, L.t_min, L.t_max, best);
     }
     uint8_t r = 0;
@@ -328,9 +343,11 @@ BF_D void closest_finish(const bf_transport_params& P, uint8_t* route, lane_ray&
  * ray is done stays idle until BF_TRACE_REFILL lanes of its wave are (or all are), and then the
  * wave claims that many queue items with one atomic on the launch's fetch counter. A wave
  * therefore keeps working while any ray of the queue is unclaimed instead of waiting for its
- * slowest ray before it moves on. Every ray's traversal is the same sequence of steps in any
- * schedule, so results do not depend on it; the loop ends when the counter has passed the item
- * count and every lane of the wave is idle.
+ * slowest ray before it moves on. Within a wave, an iteration steps either the lanes whose next
+ * entry is an interior node or those whose next entry is a leaf (BF_LEAF_PHASE), so the wave issues
+ * one kind of step code at a time instead of every kind under partial masks. Every ray's
+ * traversal is the same sequence of steps in any schedule, so results do not depend on it; the
+ * loop ends when the counter has passed the item count and every lane of the wave is idle.
  */
 template <typename Start, typename Step, typename Finish>
 BF_D void lane_loop(uint32_t n, uint32_t* fetch, Start start, Step step, Finish finish) {
@@ -357,7 +374,16 @@ BF_D void lane_loop(uint32_t n, uint32_t* fetch, Start start, Step step, Finish 
             if (exhausted) break;
             continue;
         }
-        if (active) {
+        bool go = active;
+        if (BF_LEAF_PHASE != 0u) {
+            /* one kind of step per iteration: interior nodes, unless BF_LEAF_PHASE lanes (or all
+             * active ones) have a leaf next */
+            const bool leaf = active && step.leaf_next();
+            const uint64_t leaves = __ballot(leaf);
+            const bool leaf_phase = (uint32_t)__popcll(leaves) >= BF_LEAF_PHASE || leaves == __ballot(active);
+            go = active && leaf == leaf_phase;
+        }
+        if (go) {
             const uint32_t st = step();
             if (st != BF_HIT_MISS || !step.pending()) {
                 finish(st);
@@ -396,12 +422,30 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
         trav_stack<BF_SHORT_STACK, BLOCK>* stack;
         lane_ray* L;
         trav_counters* ctr;
-        BF_D uint32_t operator()() const { return closest_step<COUNT, BATCH>(*P, *stack, *L, *ctr); }
+        double* inv;
+        BF_D uint32_t operator()() const { return closest_step<COUNT, BATCH>(*P, *stack, *L, *ctr, inv); }
         BF_D bool pending() const { return stack_depth(*stack) != 0u; }
+        BF_D bool leaf_next() const { return ::leaf_next(*stack); }
     };
+#if BF_WORLD_INV_CACHE
+    /* the lane's world-space reciprocals, kept for the return from each BLAS */
+    __shared__ double s_inv[3 * BLOCK];
+    double* inv = s_inv + threadIdx.x;
+#else
+    double* inv = nullptr;
+#endif
     lane_loop(
-        n, P.fetch, [&](uint32_t i) { return closest_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr); },
-        stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { closest_finish<BATCH>(P, route, L, st); });
+        n, P.fetch,
+        [&](uint32_t i) {
+            const uint32_t st = closest_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr);
+            if (inv) {
+                inv[0] = L.ts.space.inv[0];
+                inv[BLOCK] = L.ts.space.inv[1];
+                inv[2u * BLOCK] = L.ts.space.inv[2];
+            }
+            return st;
+        },
+        stepper{&P, &stack, &L, &ctr, inv}, [&](uint32_t st) { closest_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters);
 }
 
@@ -1109,6 +1153,7 @@ __global__ __launch_bounds__(BLOCK, BF_SHADOW_WAVES_PER_SIMD) void k_trace_shado
         trav_counters* ctr;
         BF_D uint32_t operator()() const { return shadow_step<COUNT, BATCH>(*P, *stack, *L, *ctr); }
         BF_D bool pending() const { return stack_depth(*stack) != 0u; }
+        BF_D bool leaf_next() const { return ::leaf_next(*stack); }
     };
     lane_loop(
         n, P.fetch + 1, [&](uint32_t i) { return shadow_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr); },

@@ -371,6 +371,22 @@ BF_D uint2 stack_pop(trav_stack<S, LANES>& s) {
  * live through the loop. The winning candidate keeps (t, ids, leaf); its barycentrics are
  * re-derived by closest_hit_barycentrics from the same deterministic triangle test.
  */
+/* The world-space ray of a traversal, re-read through its functor when a BLAS is left. A functor
+ * type may provide its own overload (found by argument-dependent lookup) that keeps the
+ * reciprocals instead of recomputing them. */
+template <typename WorldRay>
+BF_D void world_space(const WorldRay& world, trav_space& s) {
+    v3f o, d;
+    world(o, d);
+    make_space(o, d, s);
+}
+
+/* True when the next step pops a leaf (a triangle or an instance) rather than an interior node. */
+template <uint32_t S, uint32_t LANES>
+BF_D bool leaf_next(const trav_stack<S, LANES>& s) {
+    return s.sp != 0u && (s.lds[(s.sp - 1u) * LANES].x & BF_WIDE_LEAF) != 0u;
+}
+
 /* Loop state of one ray's traversal between steps (the stack lives in trav_stack). */
 struct trav_state {
     trav_space space;
@@ -421,9 +437,7 @@ BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S
                             trav_hit& best, bool& occluded, trav_counters& ctr) {
     if (ts.in_blas && stack_depth(stack) <= ts.blas_base) {
         ts.in_blas = 0u;
-        v3f o, d;
-        world(o, d);
-        make_space(o, d, ts.space); /* back in world space (recomputing is exact) */
+        world_space(world, ts.space); /* back in world space (recomputing is exact) */
     }
     const uint2 entry = stack_pop(stack);
     const uint32_t ref = entry.x;
