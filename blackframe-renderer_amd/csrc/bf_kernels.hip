@@ -705,10 +705,42 @@ BF_D void gradient_add(const bf_transport_params& P, const bf_dev_job& J, uint32
     }
 }
 
+/* Texel gradients of one texture at one footprint, summed over the closures that bind it before
+ * they go out as atomics: closures of a material share the hit's texture coordinate, so
+ * consecutive closures bound to the same texture add into one set of texels. */
+struct texel_pending {
+    uint32_t slot, channels;
+    bf_texel_footprint f;
+    float sc[3];
+};
+
+BF_D void texel_flush(const bf_transport_params& P, const bf_dev_job& J, texel_pending& p) {
+    if (p.slot == BF_INVALID_INDEX) return;
+    for (int q = 0; q < 4; ++q)
+        for (uint32_t ch = 0; ch < p.channels; ++ch)
+            if (p.f.weight[q] != 0.0f && p.sc[ch] != 0.0f)
+                gradient_add(P, J, p.slot, (size_t)p.f.index[q] * p.channels + ch, p.f.weight[q] * p.sc[ch]);
+    p.slot = BF_INVALID_INDEX;
+}
+
+BF_D void texel_add(const bf_transport_params& P, const bf_dev_job& J, texel_pending& p, uint32_t slot, uint32_t channels,
+                    const bf_texel_footprint& f, const float* sc) {
+    if (p.slot != slot) {
+        texel_flush(P, J, p);
+        p.slot = slot;
+        p.channels = channels;
+        p.f = f;
+        p.sc[0] = p.sc[1] = p.sc[2] = 0.0f;
+    }
+    for (uint32_t ch = 0; ch < channels; ++ch) p.sc[ch] += sc[ch];
+}
+
 BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, const bf_dev_scene& S, const bf_material_desc& M, const mixture_t& mix,
                           const filter_t& filtered, const float* uv, const float* wl, const float* a, bool light, v3f wo,
                           v3f wi_light, const float* f_light, const float* N, bool cont, v3f wi_cont, const float* f_cont,
                           const float* R) {
+    texel_pending weights{BF_INVALID_INDEX, 0u, {}, {0.0f, 0.0f, 0.0f}};
+    texel_pending widths{BF_INVALID_INDEX, 0u, {}, {0.0f, 0.0f, 0.0f}};
     for (uint32_t i = 0; i < mix.active_count; ++i) {
         const bf_closure_desc D = S.closures[M.closure_offset + filtered.source_indices[i]];
         const bool wgrad = D.weight_texture && J.texture_gradients[D.weight_texture - 1u];
@@ -745,10 +777,7 @@ BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, con
                         for (int ch = 0; ch < 3; ++ch) sc[ch] += w * b[ch];
                     }
                 }
-                for (int q = 0; q < 4; ++q)
-                    for (uint32_t ch = 0; ch < channels; ++ch)
-                        if (f.weight[q] != 0.0f && sc[ch] != 0.0f)
-                            gradient_add(P, J, slot, (size_t)f.index[q] * channels + ch, f.weight[q] * sc[ch]);
+                texel_add(P, J, weights, slot, channels, f, sc);
             }
         }
         if (rgrad && (c.kind == BF_CLOSURE_ROUGH_CONDUCTOR || c.kind == BF_CLOSURE_ROUGH_DIELECTRIC)) {
@@ -776,14 +805,13 @@ BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, con
             float t[3];
             bf_texel_footprint f;
             if (texture_value(S, slot, uv, t, f)) {
-                const float h[2] = {channels == 2u ? hx : hx + hy, hy};
-                for (int q = 0; q < 4; ++q)
-                    for (uint32_t ch = 0; ch < channels; ++ch)
-                        if (f.weight[q] != 0.0f && h[ch] != 0.0f)
-                            gradient_add(P, J, slot, (size_t)f.index[q] * channels + ch, f.weight[q] * h[ch]);
+                const float h[3] = {channels == 2u ? hx : hx + hy, hy, 0.0f};
+                texel_add(P, J, widths, slot, channels, f, h);
             }
         }
     }
+    texel_flush(P, J, weights);
+    texel_flush(P, J, widths);
 }
 
 /* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation. MODE is
