@@ -251,6 +251,7 @@ BF_D void world_space(const closest_world& world, trav_space& s) {
         s.inv[0] = world.inv[0];
         s.inv[1] = world.inv[world.stride];
         s.inv[2] = world.inv[2u * world.stride];
+        finish_space(s);
     } else {
         make_space(s.o, s.d, s);
     }

@@ -37,12 +37,63 @@ struct trav_counters {
     uint32_t nodes, triangles, instances;
 };
 
+/* Shear of the watertight triangle test for one direction (CudaSceneClosestHit.cu:694-740): the
+ * dominant axis kz, the other two in winding order, and rz = 1 / d_kz, sx = d_kx rz, sy = d_ky rz.
+ * `ok` is false when one of them is not finite (every triangle test then fails numerically). */
+struct trav_shear {
+    uint32_t kx, ky, kz;
+    float rz, sx, sy;
+    bool ok;
+};
+
+BF_D void make_shear(v3f d, trav_shear& s) {
+    uint32_t kz = 0;
+    float maxd = fabsf(d.x);
+    if (fabsf(d.y) > maxd) {
+        kz = 1;
+        maxd = fabsf(d.y);
+    }
+    if (fabsf(d.z) > maxd) kz = 2;
+    uint32_t kx = kz + 1;
+    if (kx == 3) kx = 0;
+    uint32_t ky = kx + 1;
+    if (ky == 3) ky = 0;
+    if (comp3(d, kz) < 0.0f) {
+        const uint32_t t = kx;
+        kx = ky;
+        ky = t;
+    }
+    s.kx = kx;
+    s.ky = ky;
+    s.kz = kz;
+    s.rz = 1.0f / comp3(d, kz);
+    s.sx = comp3(d, kx) * s.rz;
+    s.sy = comp3(d, ky) * s.rz;
+    s.ok = bf_isfinite(s.rz) && bf_isfinite(s.sx) && bf_isfinite(s.sy);
+}
+
 /* A ray expressed in one space (world for the TLAS, an instance's object space for its BLAS),
- * with the correctly rounded binary64 reciprocal of every non-zero direction component. */
+ * with the correctly rounded binary64 reciprocal of every non-zero direction component and the
+ * binary32 roundings the slab filter uses (finv, and efloor = 2^-147 max |finv|), formed once
+ * instead of per node. (Keeping the triangle test's shear here as well measured slower: the
+ * extra live registers spill in the traversal kernels.) */
 struct trav_space {
     v3f o, d;
     double inv[3];
+    float finv[3];
+    float efloor;
 };
+
+/* The derived binary32 constants of a space whose o, d and inv are set. */
+BF_D void finish_space(trav_space& s) {
+    float inv_max = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.finv[a] = (float)s.inv[a];
+        inv_max = fmaxf(inv_max, fabsf(s.finv[a]));
+    }
+    s.efloor = 0x1p-147f * inv_max;
+}
 
 BF_D void make_space(v3f o, v3f d, trav_space& s) {
     s.o = o;
@@ -50,6 +101,7 @@ BF_D void make_space(v3f o, v3f d, trav_space& s) {
     s.inv[0] = d.x != 0.0f ? 1.0 / (double)d.x : 0.0;
     s.inv[1] = d.y != 0.0f ? 1.0 / (double)d.y : 0.0;
     s.inv[2] = d.z != 0.0f ? 1.0 / (double)d.z : 0.0;
+    finish_space(s);
 }
 
 /* a / b rounded to nearest from y = RN(1/b): q0 = RN(a y), r = a - b q0 (exact through the fma),
@@ -113,7 +165,7 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
     const float hi[3] = {hx, hy, hz};
     const float oo[3] = {s.o.x, s.o.y, s.o.z};
     const float dd[3] = {s.d.x, s.d.y, s.d.z};
-    float nv = t_min, fv = t_max, scale = 0.0f, inv_max = 0.0f;
+    float nv = t_min, fv = t_max, scale = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (dd[a] == 0.0f) {
@@ -123,15 +175,14 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
             }
             continue;
         }
-        const float inv = (float)s.inv[a];
+        const float inv = s.finv[a];
         const float q1 = (lo[a] - oo[a]) * inv;
         const float q2 = (hi[a] - oo[a]) * inv;
         nv = fmaxf(nv, fminf(q1, q2));
         fv = fminf(fv, fmaxf(q1, q2));
         scale = fmaxf(scale, fmaxf(fabsf(q1), fabsf(q2)));
-        inv_max = fmaxf(inv_max, fabsf(inv));
     }
-    const float e = 4.0f * 0x1p-24f * scale + 0x1p-147f * inv_max + 0x1p-126f;
+    const float e = 4.0f * 0x1p-24f * scale + s.efloor + 0x1p-126f;
     if (bf_isfinite(e) && bf_isfinite(nv) && bf_isfinite(scale)) {
         if (nv - fv > 2.0f * e) {
             hit = false;
@@ -221,31 +272,15 @@ BF_D uint32_t refine_edges(v3f v0, v3f v1, v3f v2, v3f o, v3f d, uint32_t kx, ui
     return BF_HIT_MISS;
 }
 
-/* CudaSceneClosestHit.cu:694-824 */
+/* CudaSceneClosestHit.cu:694-824, with the direction's shear formed by make_shear */
 template <bool BARY>
-BF_D uint32_t watertight(v3f v0, v3f v1, v3f v2, v3f o, v3f d, float t_min, float t_max, tri_result& r) {
-    uint32_t kz = 0;
-    float maxd = fabsf(d.x);
-    if (fabsf(d.y) > maxd) {
-        kz = 1;
-        maxd = fabsf(d.y);
-    }
-    if (fabsf(d.z) > maxd) kz = 2;
-    uint32_t kx = kz + 1;
-    if (kx == 3) kx = 0;
-    uint32_t ky = kx + 1;
-    if (ky == 3) ky = 0;
-    if (comp3(d, kz) < 0.0f) {
-        const uint32_t t = kx;
-        kx = ky;
-        ky = t;
-    }
+BF_D uint32_t watertight(v3f v0, v3f v1, v3f v2, v3f o, v3f d, const trav_shear& sh, float t_min, float t_max,
+                         tri_result& r) {
+    const uint32_t kx = sh.kx, ky = sh.ky, kz = sh.kz;
     const v3f t0 = sub3(v0, o), t1 = sub3(v1, o), t2 = sub3(v2, o);
     if (!fin3(t0) || !fin3(t1) || !fin3(t2)) return BF_HIT_NUMERICAL_FAILURE;
-    const float rz = 1.0f / comp3(d, kz);
-    const float sx = comp3(d, kx) * rz;
-    const float sy = comp3(d, ky) * rz;
-    if (!bf_isfinite(rz) || !bf_isfinite(sx) || !bf_isfinite(sy)) return BF_HIT_NUMERICAL_FAILURE;
+    const float rz = sh.rz, sx = sh.sx, sy = sh.sy;
+    if (!sh.ok) return BF_HIT_NUMERICAL_FAILURE;
     const float ax = fmaf(-sx, comp3(t0, kz), comp3(t0, kx));
     const float ay = fmaf(-sy, comp3(t0, kz), comp3(t0, ky));
     const float bx = fmaf(-sx, comp3(t1, kz), comp3(t1, kx));
@@ -482,8 +517,10 @@ BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S
         if (COUNT) ctr.triangles++;
         tri_result tr;
         tr.hit = false;
+        trav_shear sh;
+        make_shear(ts.space.d, sh);
         const uint32_t st = watertight<false>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), ts.space.o,
-                                              ts.space.d, ts.t_min, tcur, tr);
+                                              ts.space.d, sh, ts.t_min, tcur, tr);
         if (st != BF_HIT_MISS) return st;
         if (!tr.hit) return BF_HIT_MISS;
         if (ANY) {
@@ -550,8 +587,10 @@ BF_D uint32_t closest_hit_barycentrics(const bf_dev_scene& S, v3f o, v3f d, floa
     const float4 q2 = S.leaf_triangles[3 * (size_t)best.leaf + 2];
     tri_result tr;
     tr.hit = false;
-    const uint32_t st = watertight<true>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), lo, ld, t_min,
-                                         t_max, tr);
+    trav_shear sh;
+    make_shear(ld, sh);
+    const uint32_t st = watertight<true>(mk(q0.x, q0.y, q0.z), mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z), lo, ld, sh,
+                                         t_min, t_max, tr);
     if (st != BF_HIT_MISS) return st;
     if (!tr.hit || tr.t != best.t) return BF_HIT_NUMERICAL_FAILURE;
     best.b0 = tr.b0;
