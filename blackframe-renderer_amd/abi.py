@@ -212,6 +212,25 @@ class AdjointJob(C.Structure):
     _fields_ = [("film_gradient", C.c_void_p), ("texture_gradients", C.c_void_p * MAX_TEXTURES)]
 
 
+QUEUE_COUNT = 7
+QUEUE_ABI_MAJOR, QUEUE_ABI_MINOR = 1, 0
+COMPACTION_SCRATCH_ALIGNMENT = 256
+COMPACTION_SUCCESS, COMPACTION_INVALID_CONTRACT, COMPACTION_CAPACITY_EXHAUSTED = 0, 1, 2
+ROUTE_RAY, ROUTE_HIT, ROUTE_MISS, ROUTE_SHADE, ROUTE_SHADOW, ROUTE_CONTINUATION, ROUTE_TERMINATED = 1, 2, 3, 4, 5, 6, 7
+
+
+class QueueSoa(C.Structure):
+    """bf_queue_soa: WavefrontQueueDeviceSoa (device pointers)."""
+    _fields_ = [("headers", C.c_void_p), ("path_slots", C.c_void_p), ("queue_count", C.c_uint32),
+                ("slot_stride", C.c_uint32)]
+
+
+class CompactionResult(C.Structure):
+    _fields_ = [("status", C.c_uint32), ("queue_kind", C.c_uint32), ("route", C.c_uint32), ("input_count", C.c_uint32),
+                ("initial_size", C.c_uint32), ("selected_count", C.c_uint32), ("published_count", C.c_uint32),
+                ("rejected_count", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
 class FilmLossJob(C.Structure):
     """bf_film_loss_job: one film of blackframe_hip_film_l2_loss (device pointers)."""
     _fields_ = [("film", C.c_void_p), ("film_counts", C.c_void_p), ("target", C.c_void_p), ("film_gradient", C.c_void_p),
@@ -240,7 +259,7 @@ class PathResult(C.Structure):
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
-                       (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32),
+                       (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48),
                        (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 

@@ -1593,6 +1593,33 @@ int blackframe_hip_trace_wavefront_adjoint(bf_hip_workspace* w, const bf_frame_j
     return trace_jobs(w, jobs, adjoints, job_count, o, report, nullptr, (hipStream_t)stream_handle);
 }
 
+int blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(uint32_t max_input_count, size_t* scratch_bytes) {
+    if (!scratch_bytes) return set_error(BF_INVALID_ARGUMENT, "NULL scratch_bytes");
+    *scratch_bytes = bfk_queue_compaction_scratch(max_input_count);
+    return BF_OK;
+}
+
+int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const bf_stage_outcome* outcomes,
+                                                     uint32_t input_count, uint32_t route, void* scratch,
+                                                     size_t scratch_bytes, bf_compaction_result* device_result,
+                                                     void* stream_handle) {
+    /* CudaWavefrontQueueCompaction.cu:253-273: every rejection happens before any launch */
+    if (!queues.headers || queues.queue_count != BF_QUEUE_COUNT || (queues.slot_stride != 0u && !queues.path_slots))
+        return set_error(BF_INVALID_ARGUMENT, "invalid queue view");
+    if (route < 1u || route > 6u) return set_error(BF_INVALID_ARGUMENT, "route %u is not a compactable queue kind", route);
+    if (input_count > queues.slot_stride)
+        return set_error(BF_INVALID_ARGUMENT, "%u outcomes exceed the queue capacity %u", input_count, queues.slot_stride);
+    if (!device_result || (input_count != 0u && !outcomes)) return set_error(BF_INVALID_ARGUMENT, "NULL outcomes or result");
+    const size_t required = bfk_queue_compaction_scratch(input_count);
+    if (scratch_bytes < required ||
+        (required != 0u && (!scratch || reinterpret_cast<uintptr_t>(scratch) % BF_COMPACTION_SCRATCH_ALIGNMENT != 0u)))
+        return set_error(BF_INVALID_ARGUMENT, "compaction scratch: %zu bytes at %p, %zu aligned bytes required",
+                         scratch_bytes, scratch, required);
+    HIP_TRY(bfk_queue_compaction(&queues, outcomes, input_count, route, scratch, device_result,
+                                 (hipStream_t)stream_handle));
+    return BF_OK;
+}
+
 int blackframe_hip_film_l2_loss(const bf_film_loss_job* jobs, uint32_t job_count, float scale, double* losses,
                                 void* stream_handle) {
     if (job_count == 0) return BF_OK;

@@ -1727,6 +1727,155 @@ __global__ void k_zero_doubles(double* dst, uint32_t count) {
     if (i < count) dst[i] = 0.0;
 }
 
+/* ---- queue compaction: CudaWavefrontQueueCompaction.cu:55-250 -------------------------------- */
+/* A selected outcome succeeded and names the route; the selected slots keep their outcome order
+ * (a stable partition), the append is all or nothing, and a header is used only while its
+ * immutable fields and counters satisfy the queue contract (WavefrontQueueDevice.cuh:77-85). */
+BF_D bool qc_header_ok(const bf_queue_header& h, uint32_t kind, uint32_t stride) {
+    return h.abi_major == BF_QUEUE_ABI_MAJOR && h.abi_minor == BF_QUEUE_ABI_MINOR &&
+           h.struct_size == (uint32_t)sizeof(bf_queue_header) && h.queue_kind == kind && h.capacity == stride &&
+           h.reserved == 0u;
+}
+
+BF_D bool qc_selected(const bf_stage_outcome& o, uint32_t route) { return o.status == 0u && o.route == route; }
+
+BF_D void qc_saturating_add(uint32_t* counter, uint32_t value) {
+    uint32_t observed = *counter;
+    for (;;) {
+        const uint32_t next = value > 0xFFFFFFFFu - observed ? 0xFFFFFFFFu : observed + value;
+        const uint32_t previous = atomicCAS(counter, observed, next);
+        if (previous == observed) return;
+        observed = previous;
+    }
+}
+
+__global__ void k_qc_init(bf_queue_soa q, uint32_t route, uint32_t n, bf_compaction_result* result) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+    bf_compaction_result r;
+    memset(&r, 0, sizeof r);
+    r.status = BF_COMPACTION_SUCCESS;
+    r.queue_kind = route;
+    r.route = route;
+    r.input_count = n;
+    const bf_queue_header h = q.headers[route];
+    r.initial_size = h.size;
+    if (!qc_header_ok(h, route, q.slot_stride) || h.size > h.capacity || h.overflow_count != 0u || h.rejected_count != 0u)
+        r.status = BF_COMPACTION_INVALID_CONTRACT;
+    *result = r;
+}
+
+__global__ void k_qc_validate(const bf_stage_outcome* outcomes, uint32_t n, uint32_t route, uint32_t capacity,
+                              bf_compaction_result* result) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const bf_stage_outcome o = outcomes[i];
+    if (qc_selected(o, route) && o.path_slot >= capacity)
+        atomicCAS(&result->status, (uint32_t)BF_COMPACTION_SUCCESS, (uint32_t)BF_COMPACTION_INVALID_CONTRACT);
+}
+
+/* selected outcomes per BF_BLOCK-item tile, from wave ballots */
+__global__ __launch_bounds__(BF_BLOCK) void k_qc_count(const bf_stage_outcome* outcomes, uint32_t n, uint32_t route,
+                                                       uint32_t* tile_counts) {
+    __shared__ uint32_t waves[BF_BLOCK / 64u];
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t i = tile * BF_BLOCK + threadIdx.x;
+        const bool sel = i < n && qc_selected(outcomes[i], route);
+        const uint32_t c = (uint32_t)__popcll(__ballot(sel));
+        if ((threadIdx.x & 63u) == 0u) waves[threadIdx.x >> 6] = c;
+        __syncthreads();
+        if (threadIdx.x == 0u) {
+            uint32_t sum = 0u;
+            for (uint32_t w = 0; w < BF_BLOCK / 64u; ++w) sum += waves[w];
+            tile_counts[tile] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+/* one workgroup: exclusive scan of the tile counts, and their total */
+__global__ __launch_bounds__(1024) void k_qc_scan(const uint32_t* tile_counts, uint32_t tiles, uint32_t* tile_offsets,
+                                                  uint32_t* total) {
+    __shared__ uint32_t partial[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (tiles + 1023u) / 1024u;
+    const uint32_t begin = t * per < tiles ? t * per : tiles;
+    const uint32_t end = begin + per < tiles ? begin + per : tiles;
+    uint32_t own = 0u;
+    for (uint32_t k = begin; k < end; ++k) own += tile_counts[k];
+    partial[t] = own;
+    __syncthreads();
+    for (uint32_t off = 1u; off < 1024u; off <<= 1) {
+        const uint32_t add = t >= off ? partial[t - off] : 0u;
+        __syncthreads();
+        partial[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = partial[t] - own;
+    for (uint32_t k = begin; k < end; ++k) {
+        tile_offsets[k] = run;
+        run += tile_counts[k];
+    }
+    if (t == 1023u) *total = partial[1023];
+}
+
+/* the transactional capacity decision (CudaWavefrontQueueCompaction.cu:155-192) */
+__global__ void k_qc_decide(bf_queue_soa q, const uint32_t* total, bf_compaction_result* result) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+    const uint32_t selected = *total;
+    result->selected_count = selected;
+    if (result->status != BF_COMPACTION_SUCCESS) return;
+    bf_queue_header& h = q.headers[result->queue_kind];
+    if (!qc_header_ok(h, result->queue_kind, q.slot_stride) || h.size != result->initial_size || h.overflow_count != 0u ||
+        h.rejected_count != 0u) {
+        result->status = BF_COMPACTION_INVALID_CONTRACT;
+        return;
+    }
+    if (selected > h.capacity - h.size) {
+        qc_saturating_add(&h.overflow_count, selected);
+        qc_saturating_add(&h.rejected_count, selected);
+        result->status = BF_COMPACTION_CAPACITY_EXHAUSTED;
+        result->rejected_count = selected;
+    }
+}
+
+/* stable scatter: rank = tile offset + earlier waves of the tile + earlier lanes of the wave */
+__global__ __launch_bounds__(BF_BLOCK) void k_qc_scatter(bf_queue_soa q, const bf_stage_outcome* outcomes, uint32_t n,
+                                                         uint32_t route, const uint32_t* tile_offsets,
+                                                         const bf_compaction_result* result) {
+    __shared__ uint32_t waves[BF_BLOCK / 64u];
+    if (result->status != BF_COMPACTION_SUCCESS) return;
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
+    const uint32_t lane = threadIdx.x & 63u;
+    const size_t base = (size_t)result->queue_kind * q.slot_stride + result->initial_size;
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint32_t i = tile * BF_BLOCK + threadIdx.x;
+        bf_stage_outcome o{0u, 0u, 0u, 0u};
+        if (i < n) o = outcomes[i];
+        const bool sel = i < n && qc_selected(o, route);
+        const uint64_t mask = __ballot(sel);
+        if (lane == 0u) waves[threadIdx.x >> 6] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        uint32_t rank = tile_offsets[tile];
+        for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) rank += waves[w];
+        rank += (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (sel) q.path_slots[base + rank] = o.path_slot;
+        __syncthreads();
+    }
+}
+
+__global__ void k_qc_publish(bf_queue_soa q, bf_compaction_result* result) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u || result->status != BF_COMPACTION_SUCCESS) return;
+    bf_queue_header& h = q.headers[result->queue_kind];
+    if (!qc_header_ok(h, result->queue_kind, q.slot_stride) || h.size != result->initial_size || h.overflow_count != 0u ||
+        h.rejected_count != 0u || result->selected_count > h.capacity - h.size) {
+        result->status = BF_COMPACTION_INVALID_CONTRACT;
+        return;
+    }
+    h.size += result->selected_count;
+    result->published_count = result->selected_count;
+}
+
 __global__ void k_set_words(uint32_t* dst, uint32_t count, uint32_t value) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = value;
@@ -2029,6 +2178,38 @@ hipError_t bfk_film_l2(const float* film, const uint32_t* counts, const float* t
 hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_doubles, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, dst, count);
+    return hipGetLastError();
+}
+
+static size_t align_scratch(size_t bytes) {
+    return (bytes + BF_COMPACTION_SCRATCH_ALIGNMENT - 1u) / BF_COMPACTION_SCRATCH_ALIGNMENT * BF_COMPACTION_SCRATCH_ALIGNMENT;
+}
+
+size_t bfk_queue_compaction_scratch(uint32_t n) {
+    if (n == 0u) return 0u;
+    const size_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
+    return 2u * align_scratch(tiles * sizeof(uint32_t)) + BF_COMPACTION_SCRATCH_ALIGNMENT;
+}
+
+hipError_t bfk_queue_compaction(const bf_queue_soa* queues, const bf_stage_outcome* outcomes, uint32_t n, uint32_t route,
+                                void* scratch, bf_compaction_result* result, hipStream_t s) {
+    const bf_queue_soa q = *queues;
+    hipLaunchKernelGGL(k_qc_init, dim3(1), dim3(64), 0, s, q, route, n, result);
+    if (n == 0u) return hipGetLastError();
+    const uint32_t tiles = (n + BF_BLOCK - 1u) / BF_BLOCK;
+    const size_t part = align_scratch((size_t)tiles * sizeof(uint32_t));
+    uint8_t* base = static_cast<uint8_t*>(scratch);
+    uint32_t* tile_counts = reinterpret_cast<uint32_t*>(base);
+    uint32_t* tile_offsets = reinterpret_cast<uint32_t*>(base + part);
+    uint32_t* total = reinterpret_cast<uint32_t*>(base + 2u * part);
+    const dim3 grid(tiles < 4096u ? tiles : 4096u);
+    hipLaunchKernelGGL(k_qc_validate, dim3(blocks_for(n, BF_BLOCK)), dim3(BF_BLOCK), 0, s, outcomes, n, route, q.slot_stride,
+                       result);
+    hipLaunchKernelGGL(k_qc_count, grid, dim3(BF_BLOCK), 0, s, outcomes, n, route, tile_counts);
+    hipLaunchKernelGGL(k_qc_scan, dim3(1), dim3(1024), 0, s, tile_counts, tiles, tile_offsets, total);
+    hipLaunchKernelGGL(k_qc_decide, dim3(1), dim3(64), 0, s, q, total, result);
+    hipLaunchKernelGGL(k_qc_scatter, grid, dim3(BF_BLOCK), 0, s, q, outcomes, n, route, tile_offsets, result);
+    hipLaunchKernelGGL(k_qc_publish, dim3(1), dim3(64), 0, s, q, result);
     return hipGetLastError();
 }
 

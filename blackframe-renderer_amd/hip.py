@@ -63,6 +63,8 @@ def library():
     lib.blackframe_hip_autotune_launch.argtypes = [vp, C.POINTER(abi.FrameJob), u32, C.POINTER(abi.TransportOptions), u32,
                                                    C.POINTER(abi.LaunchTuning), vp]
     lib.blackframe_hip_film_l2_loss.argtypes = [C.POINTER(abi.FilmLossJob), u32, C.c_float, vp, vp]
+    lib.blackframe_hip_query_wavefront_queue_compaction_scratch_bytes.argtypes = [u32, C.POINTER(C.c_size_t)]
+    lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
     lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
     lib.blackframe_query_extension.restype = C.c_uint32
     lib.blackframe_query_extension.argtypes = [C.POINTER(abi.ExtensionQuery), C.POINTER(abi.ExtensionApi)]
@@ -346,6 +348,41 @@ def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchron
     if synchronize:
         torch.cuda.synchronize()
     return report
+
+
+def queue_compaction_scratch_bytes(max_input_count):
+    """blackframe_hip_query_wavefront_queue_compaction_scratch_bytes."""
+    out = C.c_size_t()
+    _check(library().blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(max_input_count, C.byref(out)))
+    return out.value
+
+
+def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None):
+    """Runs blackframe_hip_launch_wavefront_queue_compaction on host arrays copied to the device:
+    headers uint32 [7, 8] (QueueHeader words), slots uint32 [7 * slot_stride], outcomes uint32
+    [n, 4] (status, route, path_slot, detail). Returns (headers, slots, result) read back, result a
+    uint32 [12] array (WavefrontQueueCompactionResult)."""
+    import torch
+
+    headers = np.ascontiguousarray(headers, dtype=np.uint32).reshape(abi.QUEUE_COUNT, 8)
+    slots = np.ascontiguousarray(slots, dtype=np.uint32)
+    outcomes = np.ascontiguousarray(outcomes, dtype=np.uint32).reshape(-1, 4)
+    stride = slots.size // abi.QUEUE_COUNT
+    d_headers = torch.from_numpy(headers.view(np.int32).copy()).cuda()
+    d_slots = torch.from_numpy(slots.view(np.int32).copy()).cuda() if slots.size else None
+    d_outcomes = torch.from_numpy(outcomes.view(np.int32).copy()).cuda() if outcomes.size else None
+    d_result = torch.full((12,), -1, dtype=torch.int32, device="cuda")
+    need = queue_compaction_scratch_bytes(len(outcomes))
+    size = need if scratch_bytes is None else scratch_bytes
+    d_scratch = torch.zeros(max(size, 1), dtype=torch.uint8, device="cuda")
+    view = abi.QueueSoa(d_headers.data_ptr(), d_slots.data_ptr() if d_slots is not None else None, abi.QUEUE_COUNT, stride)
+    _check(library().blackframe_hip_launch_wavefront_queue_compaction(
+        view, C.c_void_p(d_outcomes.data_ptr()) if d_outcomes is not None else None, len(outcomes), route,
+        C.c_void_p(d_scratch.data_ptr()), size, C.c_void_p(d_result.data_ptr()), _stream_handle()))
+    torch.cuda.synchronize()
+    out_slots = d_slots.cpu().numpy().view(np.uint32) if d_slots is not None else slots.copy()
+    return (d_headers.cpu().numpy().view(np.uint32).reshape(abi.QUEUE_COUNT, 8), out_slots,
+            d_result.cpu().numpy().view(np.uint32))
 
 
 def film_l2_loss(items, scale, losses=None, stream=None):

@@ -188,6 +188,32 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_j
                                    const bf_transport_options* options, uint32_t trial_passes,
                                    bf_launch_tuning* result, void* stream);
 
+/*
+ * Minimum scratch (device bytes, BF_COMPACTION_SCRATCH_ALIGNMENT-aligned) of
+ * blackframe_hip_launch_wavefront_queue_compaction for any input_count <= max_input_count (0 for 0).
+ * Replaces: blackframe_cuda_query_wavefront_queue_compaction_scratch_bytes
+ *           (XPU/CUDA/Include/Blackframe/XPU/CUDA/WavefrontQueueCompaction.hpp:52-59).
+ */
+int blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(uint32_t max_input_count, size_t* scratch_bytes);
+
+/*
+ * Stable append of the successful outcomes whose route equals `route` (1..6) to the queue of that
+ * kind: their path slots go, in outcome order, to queue[route][initial size ...], and the header's
+ * size grows by their count. Transactional: when they do not all fit, nothing is appended, the
+ * header's overflow and rejected counters grow (saturating) by their count and the result says
+ * capacity_exhausted. A header that breaks its contract (ABI version, struct size, kind, capacity
+ * equal to slot_stride, size within capacity, zero counters) or a selected path slot >=
+ * slot_stride gives invalid_contract and leaves the queue unchanged. Invalid launch arguments
+ * (queue view, route, input_count > slot_stride, null result, short or misaligned scratch) are
+ * rejected synchronously with BF_INVALID_ARGUMENT. Kernels only, on `stream`; no allocation.
+ * Replaces: blackframe_cuda_launch_wavefront_queue_compaction (WavefrontQueueCompaction.hpp:61-72,
+ *           Source/CudaWavefrontQueueCompaction.cu:253-330).
+ */
+int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const bf_stage_outcome* outcomes,
+                                                     uint32_t input_count, uint32_t route, void* scratch,
+                                                     size_t scratch_bytes, bf_compaction_result* device_result,
+                                                     void* stream);
+
 /* One frame of blackframe_hip_film_l2_loss (device pointers; 40 bytes). */
 typedef struct bf_film_loss_job {
     const float* film;            /* [pixels * 3] float RGB sums, the film of a transport call */

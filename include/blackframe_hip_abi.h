@@ -149,6 +149,49 @@ typedef struct bf_queue_header {
     uint32_t reserved;
 } bf_queue_header;
 
+/* ---- queue compaction (the reference's WavefrontQueueCompaction.hpp contract) --------------- */
+#define BF_QUEUE_COUNT 7u                    /* camera, ray, hit, miss, shade, shadow, continuation */
+#define BF_QUEUE_ABI_MAJOR 1u                /* HostDeviceTransportAbiMajor (TransportAbi.hpp:12) */
+#define BF_QUEUE_ABI_MINOR 0u
+#define BF_COMPACTION_SCRATCH_ALIGNMENT 256u
+
+/* WavefrontStageOutcome (WavefrontStageKernel.hpp:107-112): status 0 = success, route 1..6 names a
+ * queue kind (ray, hit, miss, shade, shadow, continuation; 7 = terminated). */
+typedef struct bf_stage_outcome {
+    uint32_t status;
+    uint32_t route;
+    uint32_t path_slot;
+    uint32_t detail;
+} bf_stage_outcome;
+
+/* WavefrontQueueDeviceSoa (WavefrontQueueKernel.hpp:22-27): BF_QUEUE_COUNT headers and queue-major
+ * path slots, queue k at path_slots[k * slot_stride]. */
+typedef struct bf_queue_soa {
+    bf_queue_header* headers;
+    uint32_t* path_slots;
+    uint32_t queue_count;
+    uint32_t slot_stride;
+} bf_queue_soa;
+
+enum bf_compaction_status {
+    BF_COMPACTION_SUCCESS = 0,
+    BF_COMPACTION_INVALID_CONTRACT = 1,
+    BF_COMPACTION_CAPACITY_EXHAUSTED = 2
+};
+
+/* WavefrontQueueCompactionResult (WavefrontQueueCompaction.hpp:21-31), written on the device. */
+typedef struct bf_compaction_result {
+    uint32_t status;
+    uint32_t queue_kind;
+    uint32_t route;
+    uint32_t input_count;
+    uint32_t initial_size;
+    uint32_t selected_count;
+    uint32_t published_count;
+    uint32_t rejected_count;
+    uint32_t reserved[4];
+} bf_compaction_result;
+
 /* Canonical BVH node, bit-identical to the reference's 48-byte SceneBvhNode. */
 typedef struct bf_bvh_node {
     float min_x, min_y, min_z;
@@ -414,6 +457,9 @@ BF_STATIC_ASSERT(sizeof(bf_closest_hit) == 64, "ClosestHit is 64 bytes");
 BF_STATIC_ASSERT(sizeof(bf_closest_hit_result) == 80, "SceneClosestHitResult is 80 bytes");
 BF_STATIC_ASSERT(sizeof(bf_occlusion_result) == 16, "SceneOcclusionResult is 16 bytes");
 BF_STATIC_ASSERT(sizeof(bf_queue_header) == 32, "QueueHeader is 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_stage_outcome) == 16, "WavefrontStageOutcome is 16 bytes");
+BF_STATIC_ASSERT(sizeof(bf_queue_soa) == 24, "queue view is 24 bytes");
+BF_STATIC_ASSERT(sizeof(bf_compaction_result) == 48, "WavefrontQueueCompactionResult is 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_node) == 48, "SceneBvhNode is 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_blas) == 32, "SceneBvhBlas is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_instance_ref) == 32, "SceneBvhInstanceReference is 32 bytes");

@@ -93,6 +93,62 @@ def load(abi):
     return lib
 
 
+# ---- queue compaction: Blackframe/XPU/CUDA/Source/CudaWavefrontQueueCompaction.cu:55-330 ---------
+QUEUE_COUNT = 7
+
+
+def queue_headers(capacity, sizes=None):
+    """Canonical QueueHeader words [7, 8] (CudaWavefrontQueues::create): abi 1.0, struct size 32,
+    kind k, capacity, size, zero overflow / rejected / reserved."""
+    h = np.zeros((QUEUE_COUNT, 8), dtype=np.uint32)
+    for k in range(QUEUE_COUNT):
+        h[k] = [1 | (0 << 16), 32, k, capacity, 0 if sizes is None else sizes[k], 0, 0, 0]
+    return h
+
+
+def queue_compaction(headers, slots, outcomes, route):
+    """The compaction contract on host arrays (words as in the device records). Returns new
+    (headers, slots, result[12]); arguments the launcher rejects synchronously are not modelled."""
+    headers = np.array(headers, dtype=np.uint32).reshape(QUEUE_COUNT, 8)
+    slots = np.array(slots, dtype=np.uint32)
+    outcomes = np.asarray(outcomes, dtype=np.uint32).reshape(-1, 4)
+    stride = slots.size // QUEUE_COUNT
+    n = len(outcomes)
+
+    def header_ok(h, kind):
+        return (h[0] & 0xFFFF) == 1 and (h[0] >> 16) == 0 and h[1] == 32 and h[2] == kind and h[3] == stride and h[7] == 0
+
+    result = np.zeros(12, dtype=np.uint32)
+    h = headers[route]
+    result[1] = result[2] = route
+    result[3] = n
+    result[4] = h[4]
+    status = 0
+    if not header_ok(h, route) or h[4] > h[3] or h[5] != 0 or h[6] != 0:
+        status = 1
+    if n == 0:
+        result[0] = status
+        return headers, slots, result
+    selected = (outcomes[:, 0] == 0) & (outcomes[:, 1] == route)
+    if np.any(selected & (outcomes[:, 2] >= stride)):
+        status = 1
+    count = int(selected.sum())
+    result[5] = count
+    if status == 0:
+        if count > int(h[3]) - int(h[4]):
+            headers[route, 5] = min(0xFFFFFFFF, int(headers[route, 5]) + count)
+            headers[route, 6] = min(0xFFFFFFFF, int(headers[route, 6]) + count)
+            status = 2
+            result[7] = count
+        else:
+            begin = route * stride + int(h[4])
+            slots[begin:begin + count] = outcomes[selected, 2]
+            headers[route, 4] += count
+            result[6] = count
+    result[0] = status
+    return headers, slots, result
+
+
 # ---- texture fitting: no reference counterpart (the reference renders forward only) -----------
 # These restate the binary32 operation order of blackframe_hip_film_l2_loss and
 # blackframe_hip_adam_step. They are pinned by torch's own implementations (torch.nn.functional
