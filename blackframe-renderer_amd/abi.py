@@ -219,6 +219,16 @@ COMPACTION_SUCCESS, COMPACTION_INVALID_CONTRACT, COMPACTION_CAPACITY_EXHAUSTED =
 ROUTE_RAY, ROUTE_HIT, ROUTE_MISS, ROUTE_SHADE, ROUTE_SHADOW, ROUTE_CONTINUATION, ROUTE_TERMINATED = 1, 2, 3, 4, 5, 6, 7
 
 
+LAYOUT_VALUE_COUNT = 73
+
+
+class LayoutManifest(C.Structure):
+    """bf_layout_manifest: LayoutManifest (TransportAbi.hpp:136-238)."""
+    _fields_ = [("abi_major", C.c_uint16), ("abi_minor", C.c_uint16), ("device_cxx_standard", C.c_uint32),
+                ("value_count", C.c_uint32), ("reserved_header", C.c_uint32),
+                ("values", C.c_uint32 * LAYOUT_VALUE_COUNT), ("reserved_tail", C.c_uint32 * 3)]
+
+
 class QueueSoa(C.Structure):
     """bf_queue_soa: WavefrontQueueDeviceSoa (device pointers)."""
     _fields_ = [("headers", C.c_void_p), ("path_slots", C.c_void_p), ("queue_count", C.c_uint32),
@@ -259,7 +269,7 @@ class PathResult(C.Structure):
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
-                       (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48),
+                       (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48), (LayoutManifest, 320),
                        (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 

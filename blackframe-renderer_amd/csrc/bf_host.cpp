@@ -1593,6 +1593,36 @@ int blackframe_hip_trace_wavefront_adjoint(bf_hip_workspace* w, const bf_frame_j
     return trace_jobs(w, jobs, adjoints, job_count, o, report, nullptr, (hipStream_t)stream_handle);
 }
 
+int blackframe_hip_launch_scene_soa_hash(const uint8_t* scene_bytes, size_t scene_byte_count, uint64_t* output_hash) {
+    if (!scene_bytes || !output_hash || scene_byte_count < BF_SCENE_SOA_HEADER_SIZE)
+        return set_error(BF_INVALID_ARGUMENT, "scene hash needs a blob of at least %u bytes and an output word",
+                         BF_SCENE_SOA_HEADER_SIZE);
+    HIP_TRY(bfk_scene_hash(scene_bytes, scene_byte_count, output_hash, nullptr));
+    return BF_OK;
+}
+
+int blackframe_hip_query_transport_abi_layout(bf_layout_manifest* output, int* device_count) {
+    if (!output || !device_count) return set_error(BF_INVALID_ARGUMENT, "NULL manifest or device count");
+    std::memset(output, 0, sizeof *output);
+    *device_count = 0;
+    HIP_TRY(hipGetDeviceCount(device_count));
+    if (*device_count == 0) return set_error(BF_UNAVAILABLE, "no HIP device");
+    int previous = 0;
+    HIP_TRY(hipGetDevice(&previous));
+    HIP_TRY(hipSetDevice(0));
+    bf_layout_manifest* d = nullptr;
+    hipError_t e = hipMalloc(&d, sizeof *d);
+    if (e == hipSuccess) e = bfk_layout_manifest(d, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(output, d, sizeof *output, hipMemcpyDeviceToHost);
+    const hipError_t fe = d ? hipFree(d) : hipSuccess;
+    const hipError_t se = hipSetDevice(previous);
+    if (e != hipSuccess) return hip_error(e, "transport layout probe");
+    if (fe != hipSuccess) return hip_error(fe, "hipFree(layout manifest)");
+    if (se != hipSuccess) return hip_error(se, "hipSetDevice(previous)");
+    return BF_OK;
+}
+
 int blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(uint32_t max_input_count, size_t* scratch_bytes) {
     if (!scratch_bytes) return set_error(BF_INVALID_ARGUMENT, "NULL scratch_bytes");
     *scratch_bytes = bfk_queue_compaction_scratch(max_input_count);

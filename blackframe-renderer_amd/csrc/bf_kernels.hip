@@ -1727,6 +1727,81 @@ __global__ void k_zero_doubles(double* dst, uint32_t count) {
     if (i < count) dst[i] = 0.0;
 }
 
+/* ---- scene blob hash: FNV-1a-64 with the content-hash field read as zero ---------------------- */
+/* The byte chain is sequential by definition, so one lane walks it; it reads 16-byte words where
+ * the blob is 16-byte aligned and single bytes otherwise. */
+__global__ void k_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+    uint64_t h = BF_FNV1A64_OFFSET_BASIS;
+    const auto step = [&](size_t index, uint32_t byte) {
+        const uint32_t v = index >= BF_SCENE_SOA_HASH_OFFSET && index < BF_SCENE_SOA_HASH_OFFSET + 8u ? 0u : byte;
+        h ^= v;
+        h *= BF_FNV1A64_PRIME;
+    };
+    size_t i = 0;
+    if ((reinterpret_cast<uintptr_t>(bytes) & 15u) == 0u) {
+        for (; i + 16u <= count; i += 16u) {
+            const uint4 w = *reinterpret_cast<const uint4*>(bytes + i);
+            const uint32_t words[4] = {w.x, w.y, w.z, w.w};
+            for (int k = 0; k < 4; ++k)
+                for (int b = 0; b < 4; ++b) step(i + 4u * k + b, (words[k] >> (8 * b)) & 0xFFu);
+        }
+    }
+    for (; i < count; ++i) step(i, bytes[i]);
+    *out = h;
+}
+
+/* ---- transport record layout, as device code sees it (CudaTransportAbiProbe.cu:33-160) ------- */
+__global__ void k_layout_manifest(bf_layout_manifest* out) {
+    if (blockIdx.x != 0u || threadIdx.x != 0u) return;
+    bf_layout_manifest m;
+    memset(&m, 0, sizeof m);
+    m.abi_major = BF_QUEUE_ABI_MAJOR;
+    m.abi_minor = BF_QUEUE_ABI_MINOR;
+    m.device_cxx_standard = (uint32_t)__cplusplus;
+    m.value_count = BF_LAYOUT_VALUE_COUNT;
+    uint32_t k = 0;
+    const auto put = [&](size_t v) { m.values[k++] = (uint32_t)v; };
+    /* the reference's LayoutValue order */
+    put(sizeof(bf_transport_spectrum)); put(alignof(bf_transport_spectrum)); put(offsetof(bf_transport_spectrum, values));
+    put(sizeof(bf_transport_ray)); put(alignof(bf_transport_ray));
+    put(offsetof(bf_transport_ray, origin_x)); put(offsetof(bf_transport_ray, origin_y));
+    put(offsetof(bf_transport_ray, origin_z)); put(offsetof(bf_transport_ray, t_min));
+    put(offsetof(bf_transport_ray, direction_x)); put(offsetof(bf_transport_ray, direction_y));
+    put(offsetof(bf_transport_ray, direction_z)); put(offsetof(bf_transport_ray, t_max));
+    put(offsetof(bf_transport_ray, time)); put(offsetof(bf_transport_ray, visibility_mask));
+    put(offsetof(bf_transport_ray, current_medium)); put(offsetof(bf_transport_ray, reserved));
+    put(sizeof(bf_sample_stream_index)); put(alignof(bf_sample_stream_index));
+    put(offsetof(bf_sample_stream_index, pixel_x)); put(offsetof(bf_sample_stream_index, pixel_y));
+    put(offsetof(bf_sample_stream_index, sample_index)); put(offsetof(bf_sample_stream_index, seed));
+    put(sizeof(bf_path_slot)); put(alignof(bf_path_slot)); put(offsetof(bf_path_slot, value));
+    put(sizeof(bf_queue_header)); put(alignof(bf_queue_header));
+    put(offsetof(bf_queue_header, abi_major)); put(offsetof(bf_queue_header, abi_minor));
+    put(offsetof(bf_queue_header, struct_size)); put(offsetof(bf_queue_header, queue_kind));
+    put(offsetof(bf_queue_header, capacity)); put(offsetof(bf_queue_header, size));
+    put(offsetof(bf_queue_header, overflow_count)); put(offsetof(bf_queue_header, rejected_count));
+    put(offsetof(bf_queue_header, reserved));
+    put(sizeof(bf_surface_identifiers)); put(alignof(bf_surface_identifiers));
+    put(offsetof(bf_surface_identifiers, object)); put(offsetof(bf_surface_identifiers, instance));
+    put(offsetof(bf_surface_identifiers, geometry)); put(offsetof(bf_surface_identifiers, primitive));
+    put(offsetof(bf_surface_identifiers, material)); put(offsetof(bf_surface_identifiers, reserved));
+    put(sizeof(bf_closest_hit)); put(alignof(bf_closest_hit));
+    put(offsetof(bf_closest_hit, parameter)); put(offsetof(bf_closest_hit, barycentric0));
+    put(offsetof(bf_closest_hit, barycentric1)); put(offsetof(bf_closest_hit, barycentric2));
+    put(offsetof(bf_closest_hit, normal_x)); put(offsetof(bf_closest_hit, normal_y));
+    put(offsetof(bf_closest_hit, normal_z)); put(offsetof(bf_closest_hit, reserved)); put(offsetof(bf_closest_hit, ids));
+    put(sizeof(bf_path_state_lane)); put(alignof(bf_path_state_lane));
+    put(offsetof(bf_path_state_lane, beta)); put(offsetof(bf_path_state_lane, accumulated_radiance));
+    put(offsetof(bf_path_state_lane, wavelength_nanometers)); put(offsetof(bf_path_state_lane, wavelength_pdf_values));
+    put(offsetof(bf_path_state_lane, diffuse_depth)); put(offsetof(bf_path_state_lane, glossy_depth));
+    put(offsetof(bf_path_state_lane, specular_depth)); put(offsetof(bf_path_state_lane, transmission_depth));
+    put(offsetof(bf_path_state_lane, volume_depth)); put(offsetof(bf_path_state_lane, depth));
+    put(offsetof(bf_path_state_lane, eta_scale)); put(offsetof(bf_path_state_lane, current_medium));
+    put(offsetof(bf_path_state_lane, delta_flags)); put(offsetof(bf_path_state_lane, wavelength_pdf_measures));
+    put(offsetof(bf_path_state_lane, reserved));
+    *out = m;
+}
+
 /* ---- queue compaction: CudaWavefrontQueueCompaction.cu:55-250 -------------------------------- */
 /* A selected outcome succeeded and names the route; the selected slots keep their outcome order
  * (a stable partition), the append is all or nothing, and a header is used only while its
@@ -2178,6 +2253,16 @@ hipError_t bfk_film_l2(const float* film, const uint32_t* counts, const float* t
 hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_doubles, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t bfk_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_scene_hash, dim3(1), dim3(64), 0, s, bytes, count, out);
+    return hipGetLastError();
+}
+
+hipError_t bfk_layout_manifest(bf_layout_manifest* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_layout_manifest, dim3(1), dim3(64), 0, s, out);
     return hipGetLastError();
 }
 

@@ -64,6 +64,8 @@ def library():
                                                    C.POINTER(abi.LaunchTuning), vp]
     lib.blackframe_hip_film_l2_loss.argtypes = [C.POINTER(abi.FilmLossJob), u32, C.c_float, vp, vp]
     lib.blackframe_hip_query_wavefront_queue_compaction_scratch_bytes.argtypes = [u32, C.POINTER(C.c_size_t)]
+    lib.blackframe_hip_query_transport_abi_layout.argtypes = [C.POINTER(abi.LayoutManifest), C.POINTER(C.c_int)]
+    lib.blackframe_hip_launch_scene_soa_hash.argtypes = [vp, C.c_size_t, vp]
     lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
     lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
     lib.blackframe_query_extension.restype = C.c_uint32
@@ -348,6 +350,29 @@ def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchron
     if synchronize:
         torch.cuda.synchronize()
     return report
+
+
+def scene_soa_hash(blob, offset=0):
+    """blackframe_hip_launch_scene_soa_hash over host bytes copied to the device (starting `offset`
+    bytes into the device copy, to exercise unaligned blobs). Returns the hash."""
+    import torch
+
+    data = np.frombuffer(bytes(blob), dtype=np.uint8)
+    d = torch.zeros(len(data) + offset, dtype=torch.uint8, device="cuda")
+    d[offset:] = torch.from_numpy(data.copy()).cuda()
+    out = torch.zeros(1, dtype=torch.int64, device="cuda")
+    _check(library().blackframe_hip_launch_scene_soa_hash(C.c_void_p(d.data_ptr() + offset), len(data),
+                                                           C.c_void_p(out.data_ptr())))
+    torch.cuda.synchronize()
+    return int(out.cpu().numpy().view(np.uint64)[0])
+
+
+def transport_abi_layout():
+    """blackframe_hip_query_transport_abi_layout: (LayoutManifest, device count)."""
+    manifest = abi.LayoutManifest()
+    count = C.c_int()
+    _check(library().blackframe_hip_query_transport_abi_layout(C.byref(manifest), C.byref(count)))
+    return manifest, count.value
 
 
 def queue_compaction_scratch_bytes(max_input_count):

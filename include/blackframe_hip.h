@@ -189,6 +189,25 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_j
                                    bf_launch_tuning* result, void* stream);
 
 /*
+ * FNV-1a-64 of `scene_byte_count` device bytes of a scene blob, with the 8 bytes of its
+ * content-hash field (offset BF_SCENE_SOA_HASH_OFFSET) read as zero, written to the device word
+ * `output_hash` on the default stream. Rejects a blob shorter than its header
+ * (BF_SCENE_SOA_HEADER_SIZE) or null pointers before any launch.
+ * Replaces: blackframe_cuda_launch_scene_soa_hash (XPU/CUDA/Include/Blackframe/XPU/CUDA/SceneSoaHash.hpp:9,
+ *           Source/CudaSceneSoaHash.cu:13-51).
+ */
+int blackframe_hip_launch_scene_soa_hash(const uint8_t* scene_bytes, size_t scene_byte_count, uint64_t* output_hash);
+
+/*
+ * Fills `output` from device code on device 0 with the transport records' sizes, alignments and
+ * field offsets (BF_LAYOUT_VALUE_COUNT values in the reference's LayoutValue order) and sets
+ * `device_count`. Allocates and frees one manifest of device memory and synchronises.
+ * Replaces: blackframe_cuda_query_transport_abi_layout (XPU/CUDA/Include/Blackframe/XPU/CUDA/
+ *           TransportAbiProbe.hpp, Source/CudaTransportAbiProbe.cu:168-209).
+ */
+int blackframe_hip_query_transport_abi_layout(bf_layout_manifest* output, int* device_count);
+
+/*
  * Minimum scratch (device bytes, BF_COMPACTION_SCRATCH_ALIGNMENT-aligned) of
  * blackframe_hip_launch_wavefront_queue_compaction for any input_count <= max_input_count (0 for 0).
  * Replaces: blackframe_cuda_query_wavefront_queue_compaction_scratch_bytes

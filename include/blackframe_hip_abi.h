@@ -98,6 +98,17 @@ enum bf_termination {
 };
 
 /* ---- frozen records ------------------------------------------------------------------------ */
+/* Records the reference declares alignas(16) (TransportAbi.hpp:16-134) carry the same alignment. */
+#define BF_ALIGN16 __attribute__((aligned(16)))
+
+typedef struct BF_ALIGN16 bf_transport_spectrum {
+    float values[BF_SPECTRUM_LANES];
+} bf_transport_spectrum;
+
+typedef struct bf_path_slot {
+    uint32_t value;
+} bf_path_slot;
+
 typedef struct bf_sample_stream_index {
     uint32_t pixel_x;
     uint32_t pixel_y;
@@ -105,7 +116,7 @@ typedef struct bf_sample_stream_index {
     uint64_t seed;
 } bf_sample_stream_index;
 
-typedef struct bf_transport_ray {
+typedef struct BF_ALIGN16 bf_transport_ray {
     float origin_x, origin_y, origin_z, t_min;
     float direction_x, direction_y, direction_z, t_max;
     float time;
@@ -114,12 +125,12 @@ typedef struct bf_transport_ray {
     uint32_t reserved;
 } bf_transport_ray;
 
-typedef struct bf_surface_identifiers {
+typedef struct BF_ALIGN16 bf_surface_identifiers {
     uint32_t object, instance, geometry, primitive, material;
     uint32_t reserved[3];
 } bf_surface_identifiers;
 
-typedef struct bf_closest_hit {
+typedef struct BF_ALIGN16 bf_closest_hit {
     float parameter;
     float barycentric0, barycentric1, barycentric2;
     float normal_x, normal_y, normal_z;
@@ -138,7 +149,7 @@ typedef struct bf_occlusion_result {
     uint32_t reserved[3];
 } bf_occlusion_result;
 
-typedef struct bf_queue_header {
+typedef struct BF_ALIGN16 bf_queue_header {
     uint16_t abi_major, abi_minor;
     uint32_t struct_size;
     uint32_t queue_kind;
@@ -148,6 +159,38 @@ typedef struct bf_queue_header {
     uint32_t rejected_count;
     uint32_t reserved;
 } bf_queue_header;
+
+/* TransportPathStateLane (TransportAbi.hpp:110-134): one float path lane across the boundary. */
+typedef struct BF_ALIGN16 bf_path_state_lane {
+    bf_transport_spectrum beta;
+    bf_transport_spectrum accumulated_radiance;
+    bf_transport_spectrum wavelength_nanometers;
+    bf_transport_spectrum wavelength_pdf_values;
+    uint32_t diffuse_depth, glossy_depth, specular_depth, transmission_depth, volume_depth, depth;
+    float eta_scale;
+    uint32_t current_medium;
+    uint32_t delta_flags;
+    uint8_t wavelength_pdf_measures[BF_SPECTRUM_LANES];
+    uint32_t reserved[6];
+} bf_path_state_lane;
+
+/* LayoutManifest (TransportAbi.hpp:136-238): 73 sizes, alignments and offsets of the transport
+ * records, indexed in the reference's LayoutValue order, filled from device code. */
+#define BF_LAYOUT_VALUE_COUNT 73u
+typedef struct BF_ALIGN16 bf_layout_manifest {
+    uint16_t abi_major, abi_minor;
+    uint32_t device_cxx_standard;
+    uint32_t value_count;
+    uint32_t reserved_header;
+    uint32_t values[BF_LAYOUT_VALUE_COUNT];
+    uint32_t reserved_tail[3];
+} bf_layout_manifest;
+
+/* ---- scene blob hash (SceneSoaAbi.hpp:11-16, 350) --------------------------------------------- */
+#define BF_SCENE_SOA_HEADER_SIZE 4848u          /* sizeof(SceneSoaHeader) */
+#define BF_SCENE_SOA_HASH_OFFSET 32u            /* offsetof(SceneSoaHeader, content_hash) */
+#define BF_FNV1A64_OFFSET_BASIS 14695981039346656037ull
+#define BF_FNV1A64_PRIME 1099511628211ull
 
 /* ---- queue compaction (the reference's WavefrontQueueCompaction.hpp contract) --------------- */
 #define BF_QUEUE_COUNT 7u                    /* camera, ray, hit, miss, shade, shadow, continuation */
@@ -458,6 +501,11 @@ BF_STATIC_ASSERT(sizeof(bf_closest_hit_result) == 80, "SceneClosestHitResult is 
 BF_STATIC_ASSERT(sizeof(bf_occlusion_result) == 16, "SceneOcclusionResult is 16 bytes");
 BF_STATIC_ASSERT(sizeof(bf_queue_header) == 32, "QueueHeader is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_stage_outcome) == 16, "WavefrontStageOutcome is 16 bytes");
+BF_STATIC_ASSERT(sizeof(bf_transport_spectrum) == 16, "TransportSpectrum is 16 bytes");
+BF_STATIC_ASSERT(sizeof(bf_path_slot) == 4, "PathSlot is 4 bytes");
+BF_STATIC_ASSERT(sizeof(bf_path_state_lane) == 128, "TransportPathStateLane is 128 bytes");
+BF_STATIC_ASSERT(offsetof(bf_path_state_lane, wavelength_pdf_measures) == 100, "path state measures at 100");
+BF_STATIC_ASSERT(sizeof(bf_layout_manifest) == 320, "LayoutManifest is 320 bytes");
 BF_STATIC_ASSERT(sizeof(bf_queue_soa) == 24, "queue view is 24 bytes");
 BF_STATIC_ASSERT(sizeof(bf_compaction_result) == 48, "WavefrontQueueCompactionResult is 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_node) == 48, "SceneBvhNode is 48 bytes");

@@ -93,6 +93,19 @@ def load(abi):
     return lib
 
 
+# ---- scene blob hash: Blackframe/XPU/CUDA/Source/CudaSceneSoaHash.cu:13-36 -----------------------
+def fnv1a64(data, zero_begin=None, zero_end=None):
+    """FNV-1a-64 (offset basis 14695981039346656037, prime 1099511628211) of the bytes, those in
+    [zero_begin, zero_end) read as zero."""
+    h = 14695981039346656037
+    for i, b in enumerate(bytes(data)):
+        if zero_begin is not None and zero_begin <= i < zero_end:
+            b = 0
+        h ^= b
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
 # ---- queue compaction: Blackframe/XPU/CUDA/Source/CudaWavefrontQueueCompaction.cu:55-330 ---------
 QUEUE_COUNT = 7
 
