@@ -1623,6 +1623,17 @@ int blackframe_hip_query_transport_abi_layout(bf_layout_manifest* output, int* d
     return BF_OK;
 }
 
+int blackframe_hip_launch_wavefront_queue_pushes(bf_queue_header* headers, uint32_t* path_slots, uint32_t queue_count,
+                                                 uint32_t slot_stride, const bf_queue_push* requests,
+                                                 uint32_t request_count, uint32_t* outcomes) {
+    if (!headers || queue_count != BF_QUEUE_COUNT || (slot_stride != 0u && !path_slots) ||
+        (request_count != 0u && (!requests || !outcomes)))
+        return set_error(BF_INVALID_ARGUMENT, "invalid queue view or push arrays");
+    const bf_queue_soa q{headers, path_slots, queue_count, slot_stride};
+    HIP_TRY(bfk_queue_pushes(&q, requests, request_count, outcomes, nullptr));
+    return BF_OK;
+}
+
 int blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(uint32_t max_input_count, size_t* scratch_bytes) {
     if (!scratch_bytes) return set_error(BF_INVALID_ARGUMENT, "NULL scratch_bytes");
     *scratch_bytes = bfk_queue_compaction_scratch(max_input_count);

@@ -132,6 +132,8 @@ hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s);
 /* Scratch bytes of bfk_queue_compaction for n outcomes, and the compaction itself (arguments
  * validated by the caller; scratch at least bfk_queue_compaction_scratch(n) bytes, 256-aligned). */
 hipError_t bfk_layout_manifest(bf_layout_manifest* out, hipStream_t s);
+hipError_t bfk_queue_pushes(const bf_queue_soa* queues, const bf_queue_push* requests, uint32_t n, uint32_t* outcomes,
+                            hipStream_t s);
 hipError_t bfk_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out, hipStream_t s);
 size_t bfk_queue_compaction_scratch(uint32_t n);
 hipError_t bfk_queue_compaction(const bf_queue_soa* queues, const bf_stage_outcome* outcomes, uint32_t n, uint32_t route,

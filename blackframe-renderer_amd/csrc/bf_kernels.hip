@@ -1824,6 +1824,65 @@ BF_D void qc_saturating_add(uint32_t* counter, uint32_t value) {
     }
 }
 
+/* Queue pushes (WavefrontQueueDevice.cuh:118-197): the requests of one wave that name one queue
+ * kind reserve a contiguous run with one compare-and-swap on its size, leader lane first; the
+ * lanes past the room that was left are rejected and counted. */
+__global__ __launch_bounds__(BF_BLOCK) void k_queue_push(bf_queue_soa q, const bf_queue_push* requests, uint32_t n,
+                                                         uint32_t* outcomes) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const bool live = i < n;
+    bf_queue_push r{0xFFFFFFFFu, 0u};
+    if (live) r = requests[i];
+    uint32_t status = BF_PUSH_INVALID_CONTRACT;
+    const bool grouped = live && r.queue_kind < BF_QUEUE_COUNT && qc_header_ok(q.headers[r.queue_kind], r.queue_kind, q.slot_stride);
+    uint64_t pending = __ballot(grouped);
+    while (pending != 0ull) {
+        const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1u;
+        const uint32_t kind = __shfl(r.queue_kind, (int)leader, 64);
+        const uint64_t group = __ballot(grouped && r.queue_kind == kind) & pending;
+        pending &= ~group;
+        const uint32_t size = (uint32_t)__popcll(group);
+        uint32_t base = 0u, accepted = 0u, valid = 1u;
+        if (lane == leader) {
+            bf_queue_header& h = q.headers[kind];
+            uint32_t observed = __hip_atomic_load(&h.size, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (;;) {
+                if (observed > h.capacity) {
+                    valid = 0u;
+                    break;
+                }
+                const uint32_t remaining = h.capacity - observed;
+                accepted = remaining < size ? remaining : size;
+                base = observed;
+                if (accepted == 0u) break;
+                const uint32_t previous = atomicCAS(&h.size, observed, observed + accepted);
+                if (previous == observed) break;
+                observed = previous;
+            }
+            if (valid) {
+                qc_saturating_add(&h.overflow_count, size - accepted);
+                qc_saturating_add(&h.rejected_count, size - accepted);
+            }
+        }
+        base = __shfl(base, (int)leader, 64);
+        accepted = __shfl(accepted, (int)leader, 64);
+        valid = __shfl(valid, (int)leader, 64);
+        if ((group >> lane) & 1ull) {
+            const uint32_t rank = (uint32_t)__popcll(group & ((1ull << lane) - 1ull));
+            if (!valid) {
+                status = BF_PUSH_INVALID_CONTRACT;
+            } else if (rank >= accepted) {
+                status = BF_PUSH_CAPACITY_EXHAUSTED;
+            } else {
+                q.path_slots[(size_t)kind * q.slot_stride + base + rank] = r.slot;
+                status = BF_PUSH_PUSHED;
+            }
+        }
+    }
+    if (live) outcomes[i] = status;
+}
+
 __global__ void k_qc_init(bf_queue_soa q, uint32_t route, uint32_t n, bf_compaction_result* result) {
     if (blockIdx.x != 0u || threadIdx.x != 0u) return;
     bf_compaction_result r;
@@ -2258,6 +2317,13 @@ hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s) {
 
 hipError_t bfk_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out, hipStream_t s) {
     hipLaunchKernelGGL(k_scene_hash, dim3(1), dim3(64), 0, s, bytes, count, out);
+    return hipGetLastError();
+}
+
+hipError_t bfk_queue_pushes(const bf_queue_soa* queues, const bf_queue_push* requests, uint32_t n, uint32_t* outcomes,
+                            hipStream_t s) {
+    if (n == 0u) return hipSuccess;
+    hipLaunchKernelGGL(k_queue_push, dim3(blocks_for(n, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *queues, requests, n, outcomes);
     return hipGetLastError();
 }
 

@@ -66,6 +66,7 @@ def library():
     lib.blackframe_hip_query_wavefront_queue_compaction_scratch_bytes.argtypes = [u32, C.POINTER(C.c_size_t)]
     lib.blackframe_hip_query_transport_abi_layout.argtypes = [C.POINTER(abi.LayoutManifest), C.POINTER(C.c_int)]
     lib.blackframe_hip_launch_scene_soa_hash.argtypes = [vp, C.c_size_t, vp]
+    lib.blackframe_hip_launch_wavefront_queue_pushes.argtypes = [vp, vp, u32, u32, vp, u32, vp]
     lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
     lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
     lib.blackframe_query_extension.restype = C.c_uint32
@@ -373,6 +374,29 @@ def transport_abi_layout():
     count = C.c_int()
     _check(library().blackframe_hip_query_transport_abi_layout(C.byref(manifest), C.byref(count)))
     return manifest, count.value
+
+
+def queue_pushes(headers, slots, requests):
+    """Runs blackframe_hip_launch_wavefront_queue_pushes on host arrays copied to the device:
+    headers uint32 [7, 8], slots uint32 [7 * slot_stride], requests uint32 [n, 2] (queue kind,
+    slot). Returns (headers, slots, outcomes) read back."""
+    import torch
+
+    headers = np.ascontiguousarray(headers, dtype=np.uint32).reshape(abi.QUEUE_COUNT, 8)
+    slots = np.ascontiguousarray(slots, dtype=np.uint32)
+    requests = np.ascontiguousarray(requests, dtype=np.uint32).reshape(-1, 2)
+    stride = slots.size // abi.QUEUE_COUNT
+    d_headers = torch.from_numpy(headers.view(np.int32).copy()).cuda()
+    d_slots = torch.from_numpy(slots.view(np.int32).copy()).cuda()
+    d_requests = torch.from_numpy(requests.view(np.int32).copy()).cuda() if requests.size else None
+    d_outcomes = torch.full((max(len(requests), 1),), -1, dtype=torch.int32, device="cuda")
+    _check(library().blackframe_hip_launch_wavefront_queue_pushes(
+        C.c_void_p(d_headers.data_ptr()), C.c_void_p(d_slots.data_ptr()), abi.QUEUE_COUNT, stride,
+        C.c_void_p(d_requests.data_ptr()) if d_requests is not None else None, len(requests),
+        C.c_void_p(d_outcomes.data_ptr())))
+    torch.cuda.synchronize()
+    return (d_headers.cpu().numpy().view(np.uint32).reshape(abi.QUEUE_COUNT, 8), d_slots.cpu().numpy().view(np.uint32),
+            d_outcomes.cpu().numpy().view(np.uint32)[:len(requests)])
 
 
 def queue_compaction_scratch_bytes(max_input_count):

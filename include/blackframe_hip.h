@@ -208,6 +208,20 @@ int blackframe_hip_launch_scene_soa_hash(const uint8_t* scene_bytes, size_t scen
 int blackframe_hip_query_transport_abi_layout(bf_layout_manifest* output, int* device_count);
 
 /*
+ * Pushes requests[i].slot onto queue requests[i].queue_kind and writes outcomes[i] (bf_push_status):
+ * pushed, capacity_exhausted (the queue's overflow and rejected counters grow, saturating), or
+ * invalid_contract (queue kind >= BF_QUEUE_COUNT, a header that breaks the queue contract, or a
+ * size beyond capacity). Requests of one wave and kind reserve their slots with one atomic; the
+ * order of slots and which requests win under overflow are unspecified, as in the reference.
+ * Default stream, asynchronous; invalid launch arguments are rejected before any launch.
+ * Replaces: blackframe_cuda_launch_wavefront_queue_pushes (WavefrontQueueKernel.hpp:48-57,
+ *           Source/CudaWavefrontQueues.cu:22-68, WavefrontQueueDevice.cuh:118-197).
+ */
+int blackframe_hip_launch_wavefront_queue_pushes(bf_queue_header* headers, uint32_t* path_slots, uint32_t queue_count,
+                                                 uint32_t slot_stride, const bf_queue_push* requests,
+                                                 uint32_t request_count, uint32_t* outcomes);
+
+/*
  * Minimum scratch (device bytes, BF_COMPACTION_SCRATCH_ALIGNMENT-aligned) of
  * blackframe_hip_launch_wavefront_queue_compaction for any input_count <= max_input_count (0 for 0).
  * Replaces: blackframe_cuda_query_wavefront_queue_compaction_scratch_bytes

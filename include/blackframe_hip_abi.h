@@ -216,6 +216,13 @@ typedef struct bf_queue_soa {
     uint32_t slot_stride;
 } bf_queue_soa;
 
+/* WavefrontQueueDevicePush (WavefrontQueueKernel.hpp:29-32) and its per-request status. */
+typedef struct bf_queue_push {
+    uint32_t queue_kind;
+    uint32_t slot;
+} bf_queue_push;
+enum bf_push_status { BF_PUSH_PUSHED = 0, BF_PUSH_CAPACITY_EXHAUSTED = 1, BF_PUSH_INVALID_CONTRACT = 2 };
+
 enum bf_compaction_status {
     BF_COMPACTION_SUCCESS = 0,
     BF_COMPACTION_INVALID_CONTRACT = 1,
@@ -501,6 +508,7 @@ BF_STATIC_ASSERT(sizeof(bf_closest_hit_result) == 80, "SceneClosestHitResult is 
 BF_STATIC_ASSERT(sizeof(bf_occlusion_result) == 16, "SceneOcclusionResult is 16 bytes");
 BF_STATIC_ASSERT(sizeof(bf_queue_header) == 32, "QueueHeader is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_stage_outcome) == 16, "WavefrontStageOutcome is 16 bytes");
+BF_STATIC_ASSERT(sizeof(bf_queue_push) == 8, "WavefrontQueueDevicePush is 8 bytes");
 BF_STATIC_ASSERT(sizeof(bf_transport_spectrum) == 16, "TransportSpectrum is 16 bytes");
 BF_STATIC_ASSERT(sizeof(bf_path_slot) == 4, "PathSlot is 4 bytes");
 BF_STATIC_ASSERT(sizeof(bf_path_state_lane) == 128, "TransportPathStateLane is 128 bytes");
