@@ -118,6 +118,66 @@ def cpu_baseline(bf, scene, frame, options, target_seconds, threads):
     }
 
 
+def cornell_section(bf, hip, args, log):
+    """The reference's own CPU-runnable case: Cornell box 64x64, 16 spp (65 536 paths), depth 8,
+    NEE with balance MIS, Russian roulette from depth 3, Sobol-Owen seed 0. GPU frames are timed
+    back to back; the oracle renders the same whole frame on the host as the threaded CPU
+    wavefront and as the single-threaded scalar path. The float film of one GPU frame is compared
+    with the compensated binary64 film of the same frame (mean and largest relative error over
+    pixels with radiance)."""
+    import numpy as np
+    import torch
+
+    scene, cam = bf.scenes.cornell(64, 64)
+    frame = bf.scenes.frame_desc(cam, 16, seed=0)
+    options = bf.scenes.transport_options(max_depth=8, mis=bf.abi.MIS_BALANCE)
+    options.failure_policy = bf.abi.FAILURES_COUNT
+    xs = hip.XpuScene(scene)
+    paths = cam.width * cam.height * frame.samples_per_pixel
+    ws = hip.XpuWavefrontTransportWorkspace(paths)
+    film = hip.Film(cam.width, cam.height)
+    r = hip.trace_xpu_wavefront_transport(ws, xs, frame, options, film=film)
+    rays = r["report"].closest_hit_queries + r["report"].shadow_queries
+    if not args.no_autotune:
+        # a frame this small is bound by launches: the tuner times the whole frame
+        options.launch = hip.autotune_launch(ws, [(xs, frame)], options, trial_passes=frame.samples_per_pixel).launch
+    host = film.host()
+    f32, f64 = host["film"].astype(np.float64), host["film_compensated"]
+    lit = np.abs(f64) > 0
+    rel = np.abs(f32[lit] - f64[lit]) / np.abs(f64[lit])
+    steps = 50
+    for _ in range(3):
+        hip.trace_xpu_wavefront_transport(ws, xs, frame, options, film=film, synchronize=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hip.trace_xpu_wavefront_transport(ws, xs, frame, options, film=film, synchronize=False)
+    torch.cuda.synchronize()
+    gpu_s = (time.perf_counter() - t0) / steps
+    result = {"workload": "Cornell box 64x64x16 spp (65536 paths), depth 8, balance MIS, roulette from depth 3, "
+                          "Sobol-Owen seed 0",
+              "launch": f"0x{options.launch:08x}", "rays_per_frame": rays, "gpu_ms_per_frame": round(gpu_s * 1e3, 3),
+              "gpu_mrays_per_s": round(rays / gpu_s / 1e6, 2),
+              "film_float_vs_compensated_mean_rel_err": float(f"{rel.mean():.3e}"),
+              "film_float_vs_compensated_max_rel_err": float(f"{rel.max():.3e}")}
+    log(f"cornell: GPU {gpu_s * 1e3:.2f} ms per frame ({rays / gpu_s / 1e6:.1f} Mrays/s); float film vs "
+        f"compensated film: mean rel err {rel.mean():.2e}, max {rel.max():.2e}")
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        oracle_module = importlib.import_module("oracle")
+        oscene = oracle_module.Oracle(bf.abi).scene(scene)
+        threads = min(16, len(os.sched_getaffinity(0)))
+        for key, n in (("cpu_wavefront", threads), ("cpu_scalar", 1)):
+            t0 = time.perf_counter()
+            rep = oscene.render(frame, options, threads=n, want_paths=False, want_film=False)["report"]
+            cpu_s = time.perf_counter() - t0
+            cpu_rays = rep.closest_hit_queries + rep.shadow_queries
+            result[key] = {"value": round(cpu_rays / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": n, "kind": "port",
+                           "sample": "the whole frame", "seconds": round(cpu_s, 3)}
+            log(f"cornell: oracle on {n} host thread(s): {cpu_s:.2f} s, {cpu_rays / cpu_s / 1e6:.2f} Mrays/s")
+    return result
+
+
 def batched_section(bf, hip, args, log):
     """Plain vs batched on the uneven-batch workload: 32 seeded scenes (10k-500k triangles,
     128^2-1024^2, 4-256 spp, the same material and light generator), rendered once frame by frame
@@ -429,6 +489,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the plain-vs-batched measurement")
+    ap.add_argument("--no-cornell", action="store_true", help="skip the 64x64 Cornell measurement")
     ap.add_argument("--batch-steps", type=int, default=1)
     ap.add_argument("--batch-capacity", type=int, default=1 << 27, help="paths in flight per batched launch")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
@@ -582,6 +643,8 @@ def main():
         "stages": stages,
         "cpu_baseline": None,
     }
+    if not args.no_cornell and args.workload == "grid":
+        line["cornell"] = cornell_section(bf, hip, args, log)
     if not args.no_batched and args.workload == "grid":
         del ws
         torch.cuda.empty_cache()
