@@ -8,6 +8,16 @@ INVALID = 0xFFFFFFFF
 
 # closure kinds / lobes (TransportLobesDevice.cuh:25-44)
 LOBE_DIFFUSE, LOBE_GLOSSY, LOBE_SPECULAR, LOBE_REFLECTION, LOBE_TRANSMISSION = 0x1, 0x2, 0x4, 0x8, 0x10
+# bf_status
+OK, INVALID_ARGUMENT, INCOMPATIBLE, RESOURCE_EXHAUSTED, UNAVAILABLE, DEVICE_ERROR, INTERNAL_ERROR, TRANSPORT_FAILURE = range(8)
+# point-sampled spectra: count = SPECTRUM_SAMPLED | 4, values = 4 wavelengths then 4 values
+SPECTRUM_SAMPLED = 0x80000000
+# scene blob "BLKFSOA4" (bf_scene_soa_status, BF_SCENE_SOA_VERIFY_HASH)
+SCENE_SOA_VALID, SCENE_SOA_BAD_MAGIC, SCENE_SOA_INCOMPATIBLE_VERSION, SCENE_SOA_BAD_HEADER_SIZE = 0, 1, 2, 3
+SCENE_SOA_BAD_COLUMN_COUNT, SCENE_SOA_UNSUPPORTED_HASH_ALGORITHM, SCENE_SOA_NONZERO_RESERVED = 4, 5, 6
+SCENE_SOA_INVALID_ENVIRONMENT_COUNT, SCENE_SOA_INVALID_COLUMN_DESCRIPTOR, SCENE_SOA_SIZE_OVERFLOW = 7, 8, 9
+SCENE_SOA_BAD_TOTAL_SIZE, SCENE_SOA_TRUNCATED, SCENE_SOA_HASH_MISMATCH = 10, 11, 12
+SCENE_SOA_VERIFY_HASH = 0x1
 CLOSURE_LAMBERTIAN, CLOSURE_ROUGH_DIFFUSE, CLOSURE_ROUGH_CONDUCTOR, CLOSURE_ROUGH_DIELECTRIC = 1, 2, 3, 4
 CLOSURE_SPECULAR_REFLECTION, CLOSURE_SPECULAR_TRANSMISSION = 5, 6
 

@@ -380,6 +380,8 @@ struct bf_hip_scene {
     uint32_t max_blas_depth = 0, tlas_depth = 0;
     uint32_t light_count = 0;
     std::vector<uint32_t> texture_floats; /* width * height * channels per texture slot */
+    bool sampled = false;                 /* holds point-sampled spectra at sampled_wavelengths */
+    float sampled_wavelengths[BF_SPECTRUM_LANES] = {};
 };
 
 namespace {
@@ -612,8 +614,10 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
         if (m.emission_spectrum >= d->spectrum_count) return set_error(BF_INVALID_ARGUMENT, "emission spectrum out of range");
         const bf_spectrum_desc& sp = d->spectra[m.emission_spectrum];
         bool black = true;
-        for (uint32_t k = 0; k < sp.count; ++k)
-            if (d->spectrum_values[sp.offset + k] != 0.0f) black = false;
+        const bool sampled = (sp.count & BF_SPECTRUM_SAMPLED) != 0;
+        const uint32_t first = sampled ? BF_SPECTRUM_LANES : 0u, count = sampled ? BF_SPECTRUM_LANES : sp.count;
+        for (uint32_t k = 0; k < count; ++k)
+            if (d->spectrum_values[sp.offset + first + k] != 0.0f) black = false;
         if (black) continue;
         const bf_geometry_desc& geo = d->geometries[inst.geometry_index];
         const uint32_t n = (uint32_t)geo.triangle_count;
@@ -701,6 +705,26 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
             const uint32_t ch = d->textures[C.roughness_texture - 1].channels;
             if (ch != 1 && ch != 2) return set_error(BF_INVALID_ARGUMENT, "closure %u: roughness textures have 1 or 2 channels", c);
             textured = true;
+        }
+    }
+    /* -- spectra: grid spectra in range; point-sampled ones share one wavelength packet -- */
+    s->sampled = false;
+    for (uint32_t i = 0; i < d->spectrum_count; ++i) {
+        const bf_spectrum_desc& sp = d->spectra[i];
+        const bool sampled = (sp.count & BF_SPECTRUM_SAMPLED) != 0;
+        const uint64_t span = sampled ? 2u * BF_SPECTRUM_LANES : (sp.count ? sp.count : 1u);
+        if ((sampled && sp.count != (BF_SPECTRUM_SAMPLED | BF_SPECTRUM_LANES)) || (uint64_t)sp.offset + span > d->spectrum_value_count)
+            return set_error(BF_INVALID_ARGUMENT, "spectrum %u has an invalid value range", i);
+        if (!sampled) continue;
+        const float* wl = d->spectrum_values + sp.offset;
+        for (uint32_t k = 0; k < BF_SPECTRUM_LANES; ++k)
+            if (!std::isfinite(wl[k]) || !(wl[k] > 0.0f))
+                return set_error(BF_INVALID_ARGUMENT, "spectrum %u: sampled wavelengths must be finite and positive", i);
+        if (!s->sampled) {
+            std::memcpy(s->sampled_wavelengths, wl, sizeof s->sampled_wavelengths);
+            s->sampled = true;
+        } else if (std::memcmp(s->sampled_wavelengths, wl, sizeof s->sampled_wavelengths) != 0) {
+            return set_error(BF_INVALID_ARGUMENT, "spectrum %u: every point-sampled spectrum of a scene shares one wavelength packet", i);
         }
     }
     if (!upload) return BF_OK;
@@ -1154,6 +1178,10 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         if (!J.scene->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
         if (J.scene->device != device)
             return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", J.scene->device, device);
+        if (J.scene->sampled && (o->wavelength_mode != BF_WAVELENGTHS_FRAME_PACKET ||
+                                 std::memcmp(o->packet_wavelengths, J.scene->sampled_wavelengths, sizeof J.scene->sampled_wavelengths) != 0))
+            return set_error(BF_INVALID_ARGUMENT,
+                             "job %u: the scene's spectra are point-sampled; render in frame-packet mode at their wavelengths", j);
         const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
         if (pixels == 0 || J.frame.samples_per_pixel == 0) return set_error(BF_INVALID_ARGUMENT, "job %u: empty frame", j);
         if (pixels > w->capacity)
@@ -1721,3 +1749,14 @@ int blackframe_hip_adam_step(const bf_adam_param* params, uint32_t param_count, 
 }
 
 } /* extern "C" */
+
+/* The error channel of blackframe_hip_last_error for the library's other host sources. */
+int bf_report_error(int status, const char* fmt, ...) {
+    char buffer[512];
+    va_list args;
+    va_start(args, fmt);
+    std::vsnprintf(buffer, sizeof buffer, fmt, args);
+    va_end(args);
+    g_last_error = buffer;
+    return status;
+}

@@ -66,6 +66,14 @@ BF_D float spectrum_at(const bf_dev_scene& S, uint32_t id, float wavelength) {
     if (id == BF_INVALID_INDEX) return 0.0f;
     const bf_spectrum_desc sp = S.spectra[id];
     const float* v = S.spectrum_values + sp.offset;
+    if (sp.count & BF_SPECTRUM_SAMPLED) {
+        /* four (wavelength, value) pairs; the host admits only frames whose packet is these
+         * wavelengths, so one lane always matches */
+#pragma unroll
+        for (int lane = 0; lane < 4; ++lane)
+            if (v[lane] == wavelength) return v[4 + lane];
+        return __builtin_nanf("");
+    }
     if (sp.count <= 1u) return v[0];
     const float step = 470.0f / (float)(sp.count - 1u);
     const float t = (wavelength - 360.0f) / step;

@@ -1,0 +1,494 @@
+/*
+ * The reference's serialized scene ("BLKFSOA4", ABI 4.0) at the C ABI: header validation and the
+ * blob -> bf_scene_desc mapping behind blackframe_hip_scene_create_from_soa. Host code only; the
+ * device scene is then built by blackframe_hip_scene_create.
+ *
+ * Column schema: XPU/Shared/Include/Blackframe/XPU/Shared/SceneSoaAbi.hpp:22-276.
+ * Column contents: Backends/GPU/CUDA/Source/SceneSoA.cpp:333-1178.
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <vector>
+
+#include "../../include/blackframe_hip.h"
+
+int bf_report_error(int status, const char* fmt, ...);
+
+namespace {
+
+/* First column of each group (SceneSoaAbi.hpp:24-132). */
+enum : uint32_t {
+    C_OBJECT_ID = 0,
+    C_GEOMETRY_ID = 1, C_GEOMETRY_VERTEX_OFFSET = 2, C_GEOMETRY_VERTEX_COUNT = 3,
+    C_GEOMETRY_TRIANGLE_OFFSET = 4, C_GEOMETRY_TRIANGLE_COUNT = 5,
+    C_POSITION_X = 6, C_NORMAL_X = 9, C_TEXCOORD_X = 12, C_TRIANGLE_VERTEX_0 = 14,
+    C_MATERIAL_ID = 17, C_MATERIAL_SPECTRAL_PRESENT = 18, C_MATERIAL_WAVELENGTH_NM = 19,
+    C_MATERIAL_WAVELENGTH_PDF = 23, C_MATERIAL_WAVELENGTH_MEASURE = 27, C_MATERIAL_CLOSURE_OFFSET = 31,
+    C_MATERIAL_CLOSURE_COUNT = 32, C_MATERIAL_FRAME_MODE = 33, C_MATERIAL_TANGENT_ROTATION = 34,
+    C_MATERIAL_EMITTED_RADIANCE = 35,
+    C_CLOSURE_KIND = 39, C_CLOSURE_LOBES = 40, C_CLOSURE_WEIGHT = 41, C_CLOSURE_PARAMETERS = 45,
+    C_CLOSURE_PROBABILITY = 55,
+    C_INSTANCE_ID = 56, C_INSTANCE_PARENT_PRESENT = 57, C_INSTANCE_PARENT_ID = 58, C_INSTANCE_OBJECT_ID = 59,
+    C_INSTANCE_GEOMETRY_ID = 60, C_INSTANCE_MATERIAL_ID = 61, C_INSTANCE_VISIBILITY_MASK = 62,
+    C_INSTANCE_LOCAL_TO_PARENT = 63, C_INSTANCE_PARENT_TO_LOCAL = 79, C_INSTANCE_LOCAL_TO_WORLD = 95,
+    C_INSTANCE_WORLD_TO_LOCAL = 111,
+    C_PUNCTUAL_KIND = 127, C_MESH_AREA_LIGHT_INSTANCE_ID = 143,
+    C_ENVIRONMENT_WAVELENGTH_NM = 144, C_ENVIRONMENT_WAVELENGTH_PDF = 148, C_ENVIRONMENT_WAVELENGTH_MEASURE = 152,
+    C_ENVIRONMENT_RADIANCE = 156,
+    C_TEXTURE_ID = 160, C_TEXTURE_KIND = 161, C_TEXTURE_VALUE = 162,
+    C_NORMAL_MAP_PRESENT = 166, C_BUMP_MAP_PRESENT = 176, C_BUMP_MAP_SCALE = 179,
+    C_IMAGE_TEXTURE_ID = 184, C_IMAGE_TEXTURE_MIP_OFFSET = 185, C_IMAGE_TEXTURE_MIP_COUNT = 186,
+    C_IMAGE_MIP_WIDTH = 189, C_IMAGE_MIP_TEXEL_OFFSET = 191, C_IMAGE_MIP_TEXEL_COUNT = 192,
+    C_IMAGE_TEXEL_VALUE = 193,
+};
+
+constexpr uint8_t kMeasureWavelength = 5u; /* ProbabilityMeasure::wavelength (TransportConventions.hpp:11-18) */
+
+/* scene_soa_column_count (SceneSoaAbi.hpp:193-240): the header count that sizes column c. */
+uint64_t column_count(const bf_scene_soa_header& h, uint32_t c) {
+    if (c == C_OBJECT_ID) return h.object_count;
+    if (c >= C_GEOMETRY_ID && c <= C_GEOMETRY_TRIANGLE_COUNT) return h.geometry_count;
+    if (c >= C_POSITION_X && c < C_TRIANGLE_VERTEX_0) return h.vertex_count;
+    if (c >= C_TRIANGLE_VERTEX_0 && c < C_MATERIAL_ID) return h.triangle_count;
+    if (c >= C_MATERIAL_ID && c < C_CLOSURE_KIND) return h.material_count;
+    if (c >= C_CLOSURE_KIND && c < C_INSTANCE_ID) return h.closure_count;
+    if (c >= C_INSTANCE_ID && c < C_PUNCTUAL_KIND) return h.instance_count;
+    if (c >= C_PUNCTUAL_KIND && c < C_MESH_AREA_LIGHT_INSTANCE_ID) return h.punctual_light_count;
+    if (c == C_MESH_AREA_LIGHT_INSTANCE_ID) return h.mesh_area_light_count;
+    if (c >= C_ENVIRONMENT_WAVELENGTH_NM && c < C_TEXTURE_ID) return h.environment_count;
+    if (c >= C_TEXTURE_ID && c < C_NORMAL_MAP_PRESENT) return h.texture_count;
+    if (c >= C_NORMAL_MAP_PRESENT && c < C_IMAGE_TEXTURE_ID) return h.material_count;
+    if (c >= C_IMAGE_TEXTURE_ID && c < C_IMAGE_MIP_WIDTH) return h.image_texture_count;
+    if (c >= C_IMAGE_MIP_WIDTH && c < C_IMAGE_TEXEL_VALUE) return h.image_mip_count;
+    if (c == C_IMAGE_TEXEL_VALUE) return h.image_texel_count;
+    return 0;
+}
+
+/* scene_soa_column_element_size (SceneSoaAbi.hpp:242-276). */
+uint32_t column_element_size(uint32_t c) {
+    if (c >= BF_SCENE_SOA_COLUMN_COUNT) return 0;
+    if ((c >= C_GEOMETRY_VERTEX_OFFSET && c <= C_GEOMETRY_TRIANGLE_COUNT) || c == C_MATERIAL_CLOSURE_OFFSET ||
+        c == C_MATERIAL_CLOSURE_COUNT)
+        return 8;
+    if (c == C_MATERIAL_SPECTRAL_PRESENT || (c >= C_MATERIAL_WAVELENGTH_MEASURE && c < C_MATERIAL_CLOSURE_OFFSET) ||
+        c == C_MATERIAL_FRAME_MODE || c == C_INSTANCE_PARENT_PRESENT || c == C_NORMAL_MAP_PRESENT ||
+        c == C_BUMP_MAP_PRESENT || (c >= C_ENVIRONMENT_WAVELENGTH_MEASURE && c < C_ENVIRONMENT_RADIANCE))
+        return 1;
+    if ((c >= C_POSITION_X && c < C_TRIANGLE_VERTEX_0) ||
+        (c >= C_MATERIAL_WAVELENGTH_NM && c < C_MATERIAL_WAVELENGTH_MEASURE) || c == C_MATERIAL_TANGENT_ROTATION ||
+        (c >= C_MATERIAL_EMITTED_RADIANCE && c < C_CLOSURE_KIND) || (c >= C_CLOSURE_WEIGHT && c < C_INSTANCE_ID) ||
+        (c >= C_INSTANCE_LOCAL_TO_PARENT && c < C_PUNCTUAL_KIND) ||
+        (c > C_PUNCTUAL_KIND && c < C_MESH_AREA_LIGHT_INSTANCE_ID) ||
+        (c >= C_ENVIRONMENT_WAVELENGTH_NM && c < C_ENVIRONMENT_WAVELENGTH_MEASURE) ||
+        (c >= C_ENVIRONMENT_RADIANCE && c < C_TEXTURE_ID) || (c >= C_TEXTURE_VALUE && c < C_NORMAL_MAP_PRESENT) ||
+        c == C_BUMP_MAP_SCALE || c == C_IMAGE_TEXEL_VALUE)
+        return 4;
+    if (c == C_IMAGE_TEXTURE_MIP_OFFSET || c == C_IMAGE_TEXTURE_MIP_COUNT || c == C_IMAGE_MIP_TEXEL_OFFSET ||
+        c == C_IMAGE_MIP_TEXEL_COUNT)
+        return 8;
+    return 4;
+}
+
+uint64_t align_up(uint64_t v) {
+    const uint64_t r = v % BF_SCENE_SOA_COLUMN_ALIGNMENT;
+    return r == 0 ? v : v + (BF_SCENE_SOA_COLUMN_ALIGNMENT - r);
+}
+
+/* validate_scene_soa_header (SceneSoaAbi.hpp:283-341), check for check. */
+uint32_t validate_header(const bf_scene_soa_header& h) {
+    if (h.magic != BF_SCENE_SOA_MAGIC) return BF_SCENE_SOA_BAD_MAGIC;
+    if (h.abi_major != BF_SCENE_SOA_ABI_MAJOR || h.abi_minor != BF_SCENE_SOA_ABI_MINOR)
+        return BF_SCENE_SOA_INCOMPATIBLE_VERSION;
+    if (h.header_size != BF_SCENE_SOA_HEADER_SIZE) return BF_SCENE_SOA_BAD_HEADER_SIZE;
+    if (h.column_count != BF_SCENE_SOA_COLUMN_COUNT) return BF_SCENE_SOA_BAD_COLUMN_COUNT;
+    if (h.hash_algorithm != BF_SCENE_SOA_HASH_FNV1A64) return BF_SCENE_SOA_UNSUPPORTED_HASH_ALGORITHM;
+    for (uint64_t r : h.reserved)
+        if (r != 0) return BF_SCENE_SOA_NONZERO_RESERVED;
+    if (h.environment_count > 1) return BF_SCENE_SOA_INVALID_ENVIRONMENT_COUNT;
+    if (h.total_size_bytes < BF_SCENE_SOA_HEADER_SIZE) return BF_SCENE_SOA_BAD_TOTAL_SIZE;
+    constexpr uint64_t top = std::numeric_limits<uint64_t>::max();
+    uint64_t cursor = align_up(BF_SCENE_SOA_HEADER_SIZE);
+    for (uint32_t c = 0; c < BF_SCENE_SOA_COLUMN_COUNT; ++c) {
+        const bf_scene_soa_column& d = h.columns[c];
+        const uint64_t count = column_count(h, c);
+        const uint32_t size = column_element_size(c);
+        if (d.element_count != count || d.element_size != size || d.reserved != 0)
+            return BF_SCENE_SOA_INVALID_COLUMN_DESCRIPTOR;
+        if (count == 0) {
+            if (d.offset_bytes != 0) return BF_SCENE_SOA_INVALID_COLUMN_DESCRIPTOR;
+            continue;
+        }
+        if (cursor > top - (BF_SCENE_SOA_COLUMN_ALIGNMENT - 1)) return BF_SCENE_SOA_SIZE_OVERFLOW;
+        cursor = align_up(cursor);
+        if (d.offset_bytes != cursor) return BF_SCENE_SOA_INVALID_COLUMN_DESCRIPTOR;
+        if (count > top / size) return BF_SCENE_SOA_SIZE_OVERFLOW;
+        const uint64_t bytes = count * size;
+        if (cursor > top - bytes) return BF_SCENE_SOA_SIZE_OVERFLOW;
+        cursor += bytes;
+    }
+    if (h.total_size_bytes != cursor) return BF_SCENE_SOA_BAD_TOTAL_SIZE;
+    return BF_SCENE_SOA_VALID;
+}
+
+/* FNV-1a-64 with the content-hash bytes read as zero (SceneSoA.cpp:226-236). */
+uint64_t normalized_hash(const uint8_t* bytes, uint64_t n) {
+    uint64_t h = BF_FNV1A64_OFFSET_BASIS;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint8_t v = (i >= BF_SCENE_SOA_HASH_OFFSET && i < BF_SCENE_SOA_HASH_OFFSET + 8u) ? 0u : bytes[i];
+        h = (h ^ v) * BF_FNV1A64_PRIME;
+    }
+    return h;
+}
+
+uint32_t check_blob(const void* blob, size_t byte_count, uint32_t flags, bf_scene_soa_header& h) {
+    if (byte_count < BF_SCENE_SOA_HEADER_SIZE) return BF_SCENE_SOA_TRUNCATED;
+    std::memcpy(&h, blob, sizeof h);
+    const uint32_t st = validate_header(h);
+    if (st != BF_SCENE_SOA_VALID) return st;
+    if (h.total_size_bytes > byte_count) return BF_SCENE_SOA_TRUNCATED;
+    if ((flags & BF_SCENE_SOA_VERIFY_HASH) &&
+        normalized_hash(static_cast<const uint8_t*>(blob), h.total_size_bytes) != h.content_hash)
+        return BF_SCENE_SOA_HASH_MISMATCH;
+    return BF_SCENE_SOA_VALID;
+}
+
+const char* status_name(uint32_t s) {
+    static const char* names[] = {"valid", "bad magic", "incompatible version", "bad header size", "bad column count",
+                                  "unsupported hash algorithm", "nonzero reserved", "invalid environment count",
+                                  "invalid column descriptor", "size overflow", "bad total size", "truncated blob",
+                                  "content hash mismatch"};
+    return s < sizeof names / sizeof names[0] ? names[s] : "unknown";
+}
+
+/* A validated blob's columns, copied out element by element (the blob may be unaligned). */
+struct Blob {
+    const uint8_t* bytes;
+    const bf_scene_soa_header& h;
+    template <class T>
+    std::vector<T> column(uint32_t c) const {
+        std::vector<T> out(h.columns[c].element_count);
+        if (!out.empty()) std::memcpy(out.data(), bytes + h.columns[c].offset_bytes, out.size() * sizeof(T));
+        return out;
+    }
+};
+
+/* Closure selection probabilities on the 2^-24 grid, identical to the Python scene builder's
+ * project_probabilities: float64 shares of the float64 sum (sequential, or numpy's eight-way
+ * pairwise order at eight terms), floors, then the missing units to the largest remainders, the
+ * lowest index first on ties. */
+void project_probabilities(const float* weights, uint32_t n, float* probabilities, float* cdf) {
+    double sum = 0.0;
+    if (n == 8) {
+        double r[8];
+        for (uint32_t i = 0; i < 8; ++i) r[i] = weights[i];
+        sum = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    } else {
+        for (uint32_t i = 0; i < n; ++i) sum += (double)weights[i];
+    }
+    const double scale = 16777216.0;
+    int64_t masses[BF_MAX_CLOSURES];
+    double remainders[BF_MAX_CLOSURES];
+    int64_t total = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const double scaled = (double)weights[i] / sum * scale;
+        masses[i] = (int64_t)std::floor(scaled);
+        remainders[i] = scaled - (double)masses[i];
+        total += masses[i];
+    }
+    uint32_t order[BF_MAX_CLOSURES];
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order, order + n, [&](uint32_t a, uint32_t b) { return remainders[a] > remainders[b]; });
+    for (int64_t u = 0; u < (int64_t)scale - total && u < (int64_t)n; ++u) masses[order[u]] += 1;
+    double running = 0.0;
+    cdf[0] = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        probabilities[i] = (float)((double)masses[i] / scale);
+        running += (double)masses[i] / scale;
+        cdf[i + 1] = (float)running;
+    }
+}
+
+struct Spectra {
+    std::vector<bf_spectrum_desc> descs;
+    std::vector<float> values;
+    float wavelengths[BF_SPECTRUM_LANES];
+    uint32_t add(const float lanes[BF_SPECTRUM_LANES]) {
+        const uint32_t id = (uint32_t)descs.size();
+        descs.push_back(bf_spectrum_desc{(uint32_t)values.size(), BF_SPECTRUM_SAMPLED | BF_SPECTRUM_LANES});
+        values.insert(values.end(), wavelengths, wavelengths + BF_SPECTRUM_LANES);
+        values.insert(values.end(), lanes, lanes + BF_SPECTRUM_LANES);
+        return id;
+    }
+};
+
+template <class T>
+bool strictly_increasing(const std::vector<T>& v) {
+    for (size_t i = 1; i < v.size(); ++i)
+        if (!(v[i - 1] < v[i])) return false;
+    return true;
+}
+
+int index_of(const std::vector<uint32_t>& sorted_ids, uint32_t id, uint32_t& index) {
+    const auto it = std::lower_bound(sorted_ids.begin(), sorted_ids.end(), id);
+    if (it == sorted_ids.end() || *it != id) return 0;
+    index = (uint32_t)(it - sorted_ids.begin());
+    return 1;
+}
+
+int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t flags, bf_hip_scene** out) {
+    (void)flags;
+    const Blob B{bytes, h};
+    if (h.punctual_light_count != 0)
+        return bf_report_error(BF_UNAVAILABLE, "scene blob: punctual lights are not rendered by this transport");
+    for (uint8_t present : B.column<uint8_t>(C_NORMAL_MAP_PRESENT))
+        if (present) return bf_report_error(BF_UNAVAILABLE, "scene blob: normal maps are not rendered by this transport");
+    for (uint8_t present : B.column<uint8_t>(C_BUMP_MAP_PRESENT))
+        if (present) return bf_report_error(BF_UNAVAILABLE, "scene blob: bump maps are not rendered by this transport");
+    if (h.geometry_count == 0 || h.instance_count == 0 || h.material_count == 0)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: a renderable scene needs geometries, materials and instances");
+    if (h.geometry_count > 0xFFFFFFFFull || h.material_count > 0xFFFFFFFFull || h.instance_count > 0xFFFFFFFFull ||
+        h.closure_count > 0xFFFFFFFFull)
+        return bf_report_error(BF_RESOURCE_EXHAUSTED, "scene blob: record counts exceed 32 bits");
+
+    /* canonical identifier order (FrameScene.cpp:413-420) */
+    const auto object_ids = B.column<uint32_t>(C_OBJECT_ID);
+    const auto geometry_ids = B.column<uint32_t>(C_GEOMETRY_ID);
+    const auto material_ids = B.column<uint32_t>(C_MATERIAL_ID);
+    const auto instance_ids = B.column<uint32_t>(C_INSTANCE_ID);
+    if (!strictly_increasing(object_ids) || !strictly_increasing(geometry_ids) || !strictly_increasing(material_ids) ||
+        !strictly_increasing(instance_ids))
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: identifiers are not in strictly increasing order");
+
+    /* geometry ranges: consecutive, covering every vertex and triangle (SceneSoA.cpp:496-508) */
+    const uint32_t G = (uint32_t)h.geometry_count;
+    const auto vo = B.column<uint64_t>(C_GEOMETRY_VERTEX_OFFSET), vc = B.column<uint64_t>(C_GEOMETRY_VERTEX_COUNT);
+    const auto to = B.column<uint64_t>(C_GEOMETRY_TRIANGLE_OFFSET), tc = B.column<uint64_t>(C_GEOMETRY_TRIANGLE_COUNT);
+    std::vector<bf_geometry_desc> geometries(G);
+    uint64_t vrun = 0, trun = 0;
+    for (uint32_t g = 0; g < G; ++g) {
+        if (vo[g] != vrun || to[g] != trun || vc[g] > h.vertex_count - vrun || tc[g] > h.triangle_count - trun)
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: geometry %u has an inconsistent range", g);
+        geometries[g] = bf_geometry_desc{geometry_ids[g], 0u, vo[g], vc[g], to[g], tc[g]};
+        vrun += vc[g];
+        trun += tc[g];
+    }
+    if (vrun != h.vertex_count || trun != h.triangle_count)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: geometry ranges do not cover the mesh columns");
+    const uint64_t V = h.vertex_count, T = h.triangle_count;
+    std::vector<float> positions(3 * V), normals(3 * V), uvs(2 * V);
+    std::vector<uint32_t> indices(3 * T);
+    for (uint32_t a = 0; a < 3; ++a) {
+        const auto p = B.column<float>(C_POSITION_X + a), n = B.column<float>(C_NORMAL_X + a);
+        const auto t = B.column<uint32_t>(C_TRIANGLE_VERTEX_0 + a);
+        for (uint64_t v = 0; v < V; ++v) {
+            positions[3 * v + a] = p[v];
+            normals[3 * v + a] = n[v];
+        }
+        for (uint64_t i = 0; i < T; ++i) indices[3 * i + a] = t[i];
+    }
+    for (uint32_t a = 0; a < 2; ++a) {
+        const auto u = B.column<float>(C_TEXCOORD_X + a);
+        for (uint64_t v = 0; v < V; ++v) uvs[2 * v + a] = u[v];
+    }
+
+    /* one wavelength packet for every spectral material and the environment (spectra_match,
+     * CudaWavefrontStages.cu:1170-1187, against the uniform visible sampler's pdf) */
+    Spectra spectra;
+    bool have_packet = false;
+    const float uniform_pdf = 1.0f / 470.0f;
+    auto admit_packet = [&](const float* nm, const float* pdf, const uint8_t* measure, const char* what) -> int {
+        for (uint32_t k = 0; k < BF_SPECTRUM_LANES; ++k)
+            if (!std::isfinite(nm[k]) || !(nm[k] > 0.0f) || pdf[k] != uniform_pdf || measure[k] != kMeasureWavelength)
+                return bf_report_error(BF_INVALID_ARGUMENT,
+                                       "scene blob: %s wavelengths are not a uniform visible packet (pdf 1/470, wavelength measure)", what);
+        if (!have_packet) {
+            std::memcpy(spectra.wavelengths, nm, sizeof spectra.wavelengths);
+            have_packet = true;
+        } else if (std::memcmp(spectra.wavelengths, nm, sizeof spectra.wavelengths) != 0) {
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: %s wavelengths differ from the scene's packet", what);
+        }
+        return BF_OK;
+    };
+
+    /* materials and closures (SceneSoA.cpp:594-734) */
+    const uint32_t M = (uint32_t)h.material_count, K = (uint32_t)h.closure_count;
+    const auto present = B.column<uint8_t>(C_MATERIAL_SPECTRAL_PRESENT);
+    std::vector<float> nm[4], pdf[4], emitted[4];
+    std::vector<uint8_t> measure[4];
+    for (uint32_t k = 0; k < 4; ++k) {
+        nm[k] = B.column<float>(C_MATERIAL_WAVELENGTH_NM + k);
+        pdf[k] = B.column<float>(C_MATERIAL_WAVELENGTH_PDF + k);
+        measure[k] = B.column<uint8_t>(C_MATERIAL_WAVELENGTH_MEASURE + k);
+        emitted[k] = B.column<float>(C_MATERIAL_EMITTED_RADIANCE + k);
+    }
+    const auto closure_offset = B.column<uint64_t>(C_MATERIAL_CLOSURE_OFFSET);
+    const auto closure_count = B.column<uint64_t>(C_MATERIAL_CLOSURE_COUNT);
+    const auto frame_mode = B.column<uint8_t>(C_MATERIAL_FRAME_MODE);
+    const auto rotation = B.column<float>(C_MATERIAL_TANGENT_ROTATION);
+    const auto kinds = B.column<uint32_t>(C_CLOSURE_KIND), lobes = B.column<uint32_t>(C_CLOSURE_LOBES);
+    const auto probability = B.column<float>(C_CLOSURE_PROBABILITY);
+    std::vector<float> weight[4], params[BF_CLOSURE_PARAMETERS];
+    for (uint32_t k = 0; k < 4; ++k) weight[k] = B.column<float>(C_CLOSURE_WEIGHT + k);
+    for (uint32_t p = 0; p < BF_CLOSURE_PARAMETERS; ++p) params[p] = B.column<float>(C_CLOSURE_PARAMETERS + p);
+
+    std::vector<bf_material_desc> materials(M);
+    std::vector<bf_closure_desc> closures(K);
+    uint64_t crun = 0;
+    for (uint32_t m = 0; m < M; ++m) {
+        bf_material_desc& md = materials[m];
+        md = bf_material_desc{};
+        md.id = material_ids[m];
+        md.emission_spectrum = BF_INVALID_INDEX;
+        if (closure_offset[m] != crun || closure_count[m] > h.closure_count - crun)
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u has an inconsistent closure range", m);
+        if (!present[m]) {
+            if (closure_count[m] != 0)
+                return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u has closures but no spectral data", m);
+            md.closure_offset = (uint32_t)crun;
+            continue;
+        }
+        if (closure_count[m] > BF_MAX_CLOSURES)
+            return bf_report_error(BF_UNAVAILABLE, "scene blob: material %u has more than %u closures", m, BF_MAX_CLOSURES);
+        const float lane_nm[4] = {nm[0][m], nm[1][m], nm[2][m], nm[3][m]};
+        const float lane_pdf[4] = {pdf[0][m], pdf[1][m], pdf[2][m], pdf[3][m]};
+        const uint8_t lane_measure[4] = {measure[0][m], measure[1][m], measure[2][m], measure[3][m]};
+        if (const int st = admit_packet(lane_nm, lane_pdf, lane_measure, "material"); st != BF_OK) return st;
+        md.closure_offset = (uint32_t)crun;
+        md.closure_count = (uint32_t)closure_count[m];
+        md.frame_mode = frame_mode[m];
+        md.tangent_rotation = rotation[m];
+        const float e[4] = {emitted[0][m], emitted[1][m], emitted[2][m], emitted[3][m]};
+        if (e[0] != 0.0f || e[1] != 0.0f || e[2] != 0.0f || e[3] != 0.0f) md.emission_spectrum = spectra.add(e);
+        float w[BF_MAX_CLOSURES];
+        for (uint32_t j = 0; j < md.closure_count; ++j) {
+            const uint64_t c = crun + j;
+            bf_closure_desc& cd = closures[c];
+            cd = bf_closure_desc{};
+            cd.kind = kinds[c];
+            cd.lobes = lobes[c];
+            const float lane_weight[4] = {weight[0][c], weight[1][c], weight[2][c], weight[3][c]};
+            cd.weight_spectrum = spectra.add(lane_weight);
+            cd.eta_spectrum = cd.k_spectrum = BF_INVALID_INDEX;
+            for (uint32_t p = 0; p < BF_CLOSURE_PARAMETERS; ++p) cd.parameters[p] = params[p][c];
+            if (cd.kind == BF_CLOSURE_ROUGH_CONDUCTOR) {
+                /* ClosureSet.hpp:203-209: parameters[0..3] relative eta, [4..7] relative k */
+                cd.eta_spectrum = spectra.add(cd.parameters);
+                cd.k_spectrum = spectra.add(cd.parameters + BF_SPECTRUM_LANES);
+            }
+            w[j] = probability[c];
+        }
+        project_probabilities(w, md.closure_count, md.probabilities, md.cdf);
+        crun += md.closure_count;
+    }
+    if (crun != h.closure_count)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material closure ranges do not cover the closure columns");
+
+    /* instances: world transforms as serialized (SceneSoA.cpp:736-815) */
+    const uint32_t I = (uint32_t)h.instance_count;
+    const auto inst_object = B.column<uint32_t>(C_INSTANCE_OBJECT_ID);
+    const auto inst_geometry = B.column<uint32_t>(C_INSTANCE_GEOMETRY_ID);
+    const auto inst_material = B.column<uint32_t>(C_INSTANCE_MATERIAL_ID);
+    const auto inst_mask = B.column<uint32_t>(C_INSTANCE_VISIBILITY_MASK);
+    std::vector<float> l2w[16], w2l[16];
+    for (uint32_t e = 0; e < 16; ++e) {
+        l2w[e] = B.column<float>(C_INSTANCE_LOCAL_TO_WORLD + e);
+        w2l[e] = B.column<float>(C_INSTANCE_WORLD_TO_LOCAL + e);
+    }
+    std::vector<bf_instance_desc> instances(I);
+    std::vector<uint32_t> emissive_ids;
+    for (uint32_t i = 0; i < I; ++i) {
+        bf_instance_desc& id = instances[i];
+        id = bf_instance_desc{};
+        id.id = instance_ids[i];
+        id.object_id = inst_object[i];
+        id.visibility_mask = inst_mask[i];
+        uint32_t unused = 0;
+        if (!index_of(object_ids, inst_object[i], unused) || !index_of(geometry_ids, inst_geometry[i], id.geometry_index) ||
+            !index_of(material_ids, inst_material[i], id.material_index))
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: instance %u references an unknown object, geometry or material", i);
+        if (!present[id.material_index])
+            return bf_report_error(BF_UNAVAILABLE, "scene blob: instance %u uses a material without spectral data", i);
+        if (l2w[12][i] != 0.0f || l2w[13][i] != 0.0f || l2w[14][i] != 0.0f || l2w[15][i] != 1.0f ||
+            w2l[12][i] != 0.0f || w2l[13][i] != 0.0f || w2l[14][i] != 0.0f || w2l[15][i] != 1.0f)
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: instance %u transform is not affine", i);
+        for (uint32_t e = 0; e < 12; ++e) {
+            id.local_to_world[e] = l2w[e][i];
+            id.world_to_local[e] = w2l[e][i];
+        }
+        if (materials[id.material_index].emission_spectrum != BF_INVALID_INDEX) emissive_ids.push_back(id.id);
+    }
+    /* the serialized light registry is the emissive instances in instance order (FrameScene.cpp:201-262) */
+    if (B.column<uint32_t>(C_MESH_AREA_LIGHT_INSTANCE_ID) != emissive_ids)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: mesh-area-light ids disagree with the emissive instances");
+
+    /* environment (SceneSoA.cpp:880-906) */
+    bf_scene_desc d{};
+    if (h.environment_count == 1) {
+        float env_nm[4], env_pdf[4], env_radiance[4];
+        uint8_t env_measure[4];
+        for (uint32_t k = 0; k < 4; ++k) {
+            env_nm[k] = B.column<float>(C_ENVIRONMENT_WAVELENGTH_NM + k)[0];
+            env_pdf[k] = B.column<float>(C_ENVIRONMENT_WAVELENGTH_PDF + k)[0];
+            env_measure[k] = B.column<uint8_t>(C_ENVIRONMENT_WAVELENGTH_MEASURE + k)[0];
+            env_radiance[k] = B.column<float>(C_ENVIRONMENT_RADIANCE + k)[0];
+        }
+        if (const int st = admit_packet(env_nm, env_pdf, env_measure, "environment"); st != BF_OK) return st;
+        d.has_environment = 1;
+        d.environment_spectrum = spectra.add(env_radiance);
+    } else {
+        d.environment_spectrum = BF_INVALID_INDEX;
+    }
+
+    if (spectra.values.empty()) spectra.values.push_back(0.0f); /* the builder wants a value array */
+    if (spectra.descs.empty()) spectra.descs.push_back(bf_spectrum_desc{0u, 1u});
+    d.geometry_count = G;
+    d.instance_count = I;
+    d.material_count = M;
+    d.closure_count = K;
+    d.spectrum_count = (uint32_t)spectra.descs.size();
+    d.vertex_count = V;
+    d.triangle_count = T;
+    d.spectrum_value_count = spectra.values.size();
+    d.positions = positions.data();
+    d.normals = normals.data();
+    d.uvs = uvs.data();
+    d.indices = indices.data();
+    d.geometries = geometries.data();
+    d.instances = instances.data();
+    d.materials = materials.data();
+    d.closures = closures.empty() ? nullptr : closures.data();
+    d.spectra = spectra.descs.data();
+    d.spectrum_values = spectra.values.data();
+    d.texture_count = 0;
+    d.textures = nullptr;
+    return blackframe_hip_scene_create(&d, out);
+}
+
+} // namespace
+
+extern "C" int blackframe_hip_scene_soa_validate(const void* blob, size_t byte_count, uint32_t flags, uint32_t* status) {
+    if (!blob || !status) return bf_report_error(BF_INVALID_ARGUMENT, "scene blob or status is NULL");
+    if (flags & ~BF_SCENE_SOA_VERIFY_HASH) return bf_report_error(BF_INVALID_ARGUMENT, "unknown scene blob flags 0x%x", flags);
+    bf_scene_soa_header h;
+    *status = check_blob(blob, byte_count, flags, h);
+    return BF_OK;
+}
+
+extern "C" int blackframe_hip_scene_create_from_soa(const void* blob, size_t byte_count, uint32_t flags, bf_hip_scene** out) {
+    if (!blob || !out) return bf_report_error(BF_INVALID_ARGUMENT, "scene blob or output is NULL");
+    *out = nullptr;
+    if (flags & ~BF_SCENE_SOA_VERIFY_HASH) return bf_report_error(BF_INVALID_ARGUMENT, "unknown scene blob flags 0x%x", flags);
+    bf_scene_soa_header h;
+    const uint32_t st = check_blob(blob, byte_count, flags, h);
+    if (st != BF_SCENE_SOA_VALID)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob rejected: %s (status %u)", status_name(st), st);
+    try {
+        return build_from_blob(static_cast<const uint8_t*>(blob), h, flags, out);
+    } catch (const std::bad_alloc&) {
+        return bf_report_error(BF_RESOURCE_EXHAUSTED, "host allocation failed while reading the scene blob");
+    }
+}

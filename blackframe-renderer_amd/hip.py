@@ -66,6 +66,8 @@ def library():
     lib.blackframe_hip_query_wavefront_queue_compaction_scratch_bytes.argtypes = [u32, C.POINTER(C.c_size_t)]
     lib.blackframe_hip_query_transport_abi_layout.argtypes = [C.POINTER(abi.LayoutManifest), C.POINTER(C.c_int)]
     lib.blackframe_hip_launch_scene_soa_hash.argtypes = [vp, C.c_size_t, vp]
+    lib.blackframe_hip_scene_soa_validate.argtypes = [vp, C.c_size_t, u32, C.POINTER(C.c_uint32)]
+    lib.blackframe_hip_scene_create_from_soa.argtypes = [vp, C.c_size_t, u32, C.POINTER(vp)]
     lib.blackframe_hip_launch_wavefront_queue_pushes.argtypes = [vp, vp, u32, u32, vp, u32, vp]
     lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
     lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
@@ -101,6 +103,21 @@ class XpuScene:
         self.texture_sizes = [w * h * c for (w, h, c, _, _) in getattr(scene, "textures", [])]
         create = self._lib.blackframe_hip_scene_create if upload else self._lib.blackframe_hip_scene_prepare
         _check(create(C.byref(scene.desc), C.byref(self.handle)))
+
+    @classmethod
+    def from_soa(cls, blob, verify_hash=True):
+        """A device scene from the reference's serialized scene blob ("BLKFSOA4" bytes):
+        blackframe_hip_scene_create_from_soa. Render it in frame-packet mode at the blob's
+        wavelength packet (soa_wavelengths)."""
+        self = cls.__new__(cls)
+        self._lib = library()
+        self.handle = C.c_void_p()
+        self.scene = None
+        self.texture_sizes = []
+        data = bytes(blob)
+        _check(self._lib.blackframe_hip_scene_create_from_soa(data, len(data), abi.SCENE_SOA_VERIFY_HASH if verify_hash else 0,
+                                                              C.byref(self.handle)))
+        return self
 
     def close(self):
         if self.handle:
@@ -366,6 +383,15 @@ def scene_soa_hash(blob, offset=0):
                                                            C.c_void_p(out.data_ptr())))
     torch.cuda.synchronize()
     return int(out.cpu().numpy().view(np.uint64)[0])
+
+
+def scene_soa_validate(blob, verify_hash=True):
+    """blackframe_hip_scene_soa_validate: the bf_scene_soa_status of host blob bytes (host only)."""
+    data = bytes(blob)
+    status = C.c_uint32()
+    _check(library().blackframe_hip_scene_soa_validate(data, len(data), abi.SCENE_SOA_VERIFY_HASH if verify_hash else 0,
+                                                       C.byref(status)))
+    return status.value
 
 
 def transport_abi_layout():

@@ -63,6 +63,10 @@ def affine(linear=None, translation=(0.0, 0.0, 0.0)):
     return m
 
 
+class SpectrumId(int):
+    """An already-built spectrum, accepted wherever Scene.add_material takes spectrum values."""
+
+
 class Scene:
     """Owns the float32 / uint32 arrays behind one bf_scene_desc."""
 
@@ -78,6 +82,8 @@ class Scene:
 
     # -- spectra ------------------------------------------------------------------------------
     def spectrum(self, values):
+        if isinstance(values, SpectrumId):
+            return int(values)
         values = [float(v) for v in np.atleast_1d(values)]
         if len(values) == 1 and values[0] in self._constant_cache:
             return self._constant_cache[values[0]]
@@ -87,6 +93,14 @@ class Scene:
         if len(values) == 1:
             self._constant_cache[values[0]] = sid
         return sid
+
+    def sampled_spectrum(self, wavelengths, values):
+        """A point-sampled spectrum: four wavelengths (nm) and the value at each; the scene then
+        renders in frame-packet mode at exactly those wavelengths."""
+        sid = len(self.spectra)
+        self.spectra.append((len(self.spectrum_values), abi.SPECTRUM_SAMPLED | 4))
+        self.spectrum_values.extend(float(np.float32(v)) for v in list(wavelengths) + list(values))
+        return SpectrumId(sid)
 
     # -- textures -----------------------------------------------------------------------------
     def add_texture(self, texels=None, device=None):

@@ -199,6 +199,33 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_j
 int blackframe_hip_launch_scene_soa_hash(const uint8_t* scene_bytes, size_t scene_byte_count, uint64_t* output_hash);
 
 /*
+ * Checks a host scene blob: validate_scene_soa_header's checks in its order, then (no reference
+ * counterpart) that `byte_count` covers total_size_bytes and, with BF_SCENE_SOA_VERIFY_HASH, that
+ * content_hash is the FNV-1a-64 of the bytes with the hash field read as zero. *status receives a
+ * bf_scene_soa_status; the call returns BF_OK whenever it could decide. Host only, no GPU call.
+ * Replaces: validate_scene_soa_header (XPU/Shared/.../SceneSoaAbi.hpp:283-341), the check every
+ *           reference stage makes of the uploaded header (CudaWavefrontStages.cu:818-880).
+ */
+int blackframe_hip_scene_soa_validate(const void* blob, size_t byte_count, uint32_t flags, uint32_t* status);
+
+/*
+ * Builds a device scene from a host "BLKFSOA4" blob, the reference's serialized FrameScene
+ * (SceneSoA.cpp:333-1178): the same BVH, tables and upload as blackframe_hip_scene_create.
+ * Materials, closures and the environment arrive sampled at one wavelength packet and become
+ * point-sampled spectra (BF_SPECTRUM_SAMPLED): render the scene in frame-packet mode with
+ * packet_wavelengths equal to the blob's wavelengths, as the reference's stages require
+ * (spectra_match, CudaWavefrontStages.cu:1170-1187). Rejects with BF_INVALID_ARGUMENT an invalid
+ * header (see blackframe_hip_scene_soa_validate), non-canonical ids, inconsistent ranges,
+ * unknown references, a mesh-area-light column that disagrees with the emissive instances, or
+ * wavelengths other than one uniform-visible packet (pdf 1/470 per nm, wavelength measure); with
+ * BF_UNAVAILABLE punctual lights, normal or bump maps and instanced materials without spectral
+ * data, which this transport does not render. Constant and image textures are accepted and unused.
+ * Replaces: CudaSceneSoA::upload (Backends/GPU/CUDA/Include/Blackframe/Backends/GPU/CUDA/SceneSoA.hpp:33,
+ *           Source/SceneSoA.cpp:1180-1210).
+ */
+int blackframe_hip_scene_create_from_soa(const void* blob, size_t byte_count, uint32_t flags, bf_hip_scene** out);
+
+/*
  * Fills `output` from device code on device 0 with the transport records' sizes, alignments and
  * field offsets (BF_LAYOUT_VALUE_COUNT values in the reference's LayoutValue order) and sets
  * `device_count`. Allocates and frees one manifest of device memory and synchronises.

@@ -186,11 +186,63 @@ typedef struct BF_ALIGN16 bf_layout_manifest {
     uint32_t reserved_tail[3];
 } bf_layout_manifest;
 
-/* ---- scene blob hash (SceneSoaAbi.hpp:11-16, 350) --------------------------------------------- */
+/* ---- scene blob "BLKFSOA4" (SceneSoaAbi.hpp:11-19, 141-189, 350) ------------------------------ */
 #define BF_SCENE_SOA_HEADER_SIZE 4848u          /* sizeof(SceneSoaHeader) */
 #define BF_SCENE_SOA_HASH_OFFSET 32u            /* offsetof(SceneSoaHeader, content_hash) */
 #define BF_FNV1A64_OFFSET_BASIS 14695981039346656037ull
 #define BF_FNV1A64_PRIME 1099511628211ull
+#define BF_SCENE_SOA_MAGIC 0x34414F53464B4C42ull /* "BLKFSOA4" */
+#define BF_SCENE_SOA_ABI_MAJOR 4u
+#define BF_SCENE_SOA_ABI_MINOR 0u
+#define BF_SCENE_SOA_HASH_FNV1A64 1u
+#define BF_SCENE_SOA_COLUMN_COUNT 194u
+#define BF_SCENE_SOA_COLUMN_ALIGNMENT 16u
+
+/* SceneSoaHeaderValidationStatus (SceneSoaAbi.hpp:177-189), same values and check order. */
+enum bf_scene_soa_status {
+    BF_SCENE_SOA_VALID = 0,
+    BF_SCENE_SOA_BAD_MAGIC = 1,
+    BF_SCENE_SOA_INCOMPATIBLE_VERSION = 2,
+    BF_SCENE_SOA_BAD_HEADER_SIZE = 3,
+    BF_SCENE_SOA_BAD_COLUMN_COUNT = 4,
+    BF_SCENE_SOA_UNSUPPORTED_HASH_ALGORITHM = 5,
+    BF_SCENE_SOA_NONZERO_RESERVED = 6,
+    BF_SCENE_SOA_INVALID_ENVIRONMENT_COUNT = 7,
+    BF_SCENE_SOA_INVALID_COLUMN_DESCRIPTOR = 8,
+    BF_SCENE_SOA_SIZE_OVERFLOW = 9,
+    BF_SCENE_SOA_BAD_TOTAL_SIZE = 10,
+    /* beyond the header (no reference counterpart): checked by the blob entry points */
+    BF_SCENE_SOA_TRUNCATED = 11,   /* fewer bytes than the header or than total_size_bytes */
+    BF_SCENE_SOA_HASH_MISMATCH = 12 /* content_hash differs from the FNV-1a-64 of the bytes */
+};
+
+/* Flags of blackframe_hip_scene_soa_validate / blackframe_hip_scene_create_from_soa. */
+#define BF_SCENE_SOA_VERIFY_HASH 0x1u
+
+/* SceneSoaColumnDescriptor and SceneSoaHeader (SceneSoaAbi.hpp:141-175): the blob's first
+ * 4848 bytes, little-endian; column c holds element_count elements of element_size bytes at
+ * offset_bytes, 16-byte aligned, in column order, empty columns at offset 0. */
+typedef struct bf_scene_soa_column {
+    uint64_t offset_bytes;
+    uint64_t element_count;
+    uint32_t element_size;
+    uint32_t reserved;
+} bf_scene_soa_column;
+
+typedef struct BF_ALIGN16 bf_scene_soa_header {
+    uint64_t magic;
+    uint16_t abi_major, abi_minor;
+    uint32_t header_size;
+    uint32_t column_count;
+    uint32_t hash_algorithm;
+    uint64_t total_size_bytes;
+    uint64_t content_hash;
+    uint64_t object_count, geometry_count, vertex_count, triangle_count, material_count, closure_count;
+    uint64_t instance_count, punctual_light_count, mesh_area_light_count, environment_count, texture_count;
+    uint64_t image_texture_count, image_mip_count, image_texel_count;
+    bf_scene_soa_column columns[BF_SCENE_SOA_COLUMN_COUNT];
+    uint64_t reserved[5];
+} bf_scene_soa_header;
 
 /* ---- queue compaction (the reference's WavefrontQueueCompaction.hpp contract) --------------- */
 #define BF_QUEUE_COUNT 7u                    /* camera, ray, hit, miss, shade, shadow, continuation */
@@ -298,6 +350,12 @@ typedef struct bf_texture_desc {
     const float* data;   /* host texels, copied into the scene at creation ... */
     float* device_data;  /* ... or caller-owned device texels, read in place by every render */
 } bf_texture_desc;
+/* A point-sampled spectrum: count == (BF_SPECTRUM_SAMPLED | BF_SPECTRUM_LANES) and its eight
+ * values are four wavelengths (nm) then the value at each. It is defined only at those wavelengths,
+ * so a scene holding one renders in frame-packet mode with packet_wavelengths equal to them (every
+ * sampled spectrum of a scene shares one wavelength packet). This is how the reference's scene
+ * blob stores materials, closures and the environment (SceneSoA.cpp:606-663, 880-906). */
+#define BF_SPECTRUM_SAMPLED 0x80000000u
 typedef struct bf_spectrum_desc {
     uint32_t offset; /* into bf_scene_desc.spectrum_values */
     uint32_t count;
@@ -514,6 +572,12 @@ BF_STATIC_ASSERT(sizeof(bf_path_slot) == 4, "PathSlot is 4 bytes");
 BF_STATIC_ASSERT(sizeof(bf_path_state_lane) == 128, "TransportPathStateLane is 128 bytes");
 BF_STATIC_ASSERT(offsetof(bf_path_state_lane, wavelength_pdf_measures) == 100, "path state measures at 100");
 BF_STATIC_ASSERT(sizeof(bf_layout_manifest) == 320, "LayoutManifest is 320 bytes");
+BF_STATIC_ASSERT(sizeof(bf_scene_soa_column) == 24, "SceneSoaColumnDescriptor is 24 bytes");
+BF_STATIC_ASSERT(sizeof(bf_scene_soa_header) == BF_SCENE_SOA_HEADER_SIZE, "SceneSoaHeader is 4848 bytes");
+BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, content_hash) == BF_SCENE_SOA_HASH_OFFSET, "content hash at 32");
+BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, object_count) == 40, "counts at 40");
+BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, columns) == 152, "columns at 152");
+BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, reserved) == 4808, "reserved at 4808");
 BF_STATIC_ASSERT(sizeof(bf_queue_soa) == 24, "queue view is 24 bytes");
 BF_STATIC_ASSERT(sizeof(bf_compaction_result) == 48, "WavefrontQueueCompactionResult is 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_node) == 48, "SceneBvhNode is 48 bytes");

@@ -466,8 +466,10 @@ static int build_lights(bfo_scene* scene) {
         if (m->emission_spectrum == BF_INVALID_INDEX) continue;
         const bf_spectrum_desc* sp = &scene->spectra[m->emission_spectrum];
         int black = 1;
-        for (uint32_t k = 0; k < sp->count; ++k)
-            if (scene->spectrum_values[sp->offset + k] != 0.0f) black = 0;
+        const int sampled = (sp->count & BF_SPECTRUM_SAMPLED) != 0;
+        const uint32_t first = sampled ? 4u : 0u, count = sampled ? 4u : sp->count;
+        for (uint32_t k = 0; k < count; ++k)
+            if (scene->spectrum_values[sp->offset + first + k] != 0.0f) black = 0;
         if (black) continue;
         bfo_light* light = &scene->lights[scene->light_count];
         light->instance = i;

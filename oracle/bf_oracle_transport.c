@@ -78,6 +78,11 @@ static float spectrum_at(const bfo_scene* s, uint32_t id, float wavelength) {
     if (id == BF_INVALID_INDEX) return 0.0f;
     const bf_spectrum_desc* sp = &s->spectra[id];
     const float* v = s->spectrum_values + sp->offset;
+    if (sp->count & BF_SPECTRUM_SAMPLED) {
+        for (int lane = 0; lane < 4; ++lane)
+            if (v[lane] == wavelength) return v[4 + lane];
+        return NAN;
+    }
     if (sp->count <= 1) return v[0];
     const float step = 470.0f / (float)(sp->count - 1);
     const float t = (wavelength - 360.0f) / step;
