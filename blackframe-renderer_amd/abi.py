@@ -135,7 +135,18 @@ class SceneDesc(C.Structure):
         ("geometries", C.c_void_p), ("instances", C.c_void_p), ("materials", C.c_void_p), ("closures", C.c_void_p),
         ("spectra", C.c_void_p), ("spectrum_values", C.c_void_p),
         ("environment_spectrum", C.c_uint32), ("texture_count", C.c_uint32), ("textures", C.c_void_p),
+        ("punctual_count", C.c_uint32), ("reserved_punctual", C.c_uint32), ("punctuals", C.c_void_p),
     ]
+
+
+PUNCTUAL_POINT, PUNCTUAL_DIRECTIONAL, PUNCTUAL_SPOT = 0, 1, 2
+
+
+class PunctualDesc(C.Structure):
+    """bf_punctual_desc: a point, directional or spot light (radiant intensity / irradiance spectrum)."""
+    _fields_ = [("kind", C.c_uint32), ("spectrum", C.c_uint32), ("position", C.c_float * 3),
+                ("position_error", C.c_float * 3), ("direction", C.c_float * 3), ("inner_half_angle", C.c_float),
+                ("outer_half_angle", C.c_float), ("reserved", C.c_uint32)]
 
 
 class TextureDesc(C.Structure):
@@ -278,7 +289,7 @@ class PathResult(C.Structure):
 
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
-                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 144), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
+                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 160), (PunctualDesc, 56), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
                        (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48), (LayoutManifest, 320),
                        (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))

@@ -42,6 +42,14 @@
 #define BF_SHADE_WAVES_PER_SIMD 2 /* register budget of the shade kernels (waves per SIMD) */
 #endif
 
+/* A punctual light with its spot-cone cosines formed once on the host. */
+typedef struct bf_dev_punctual {
+    uint32_t kind, spectrum;
+    float position[3], position_error[3], direction[3];
+    float cosine_inner, cosine_outer;
+    uint32_t smooth; /* spot with inner != outer: smoothstep falloff between the cones */
+} bf_dev_punctual;
+
 typedef struct bf_dev_blas {
     uint32_t root_ref;         /* wide-node index, or BF_WIDE_LEAF | leaf-triangle index */
     uint32_t geometry_index;
@@ -121,7 +129,9 @@ typedef struct bf_dev_scene {
     const float* light_cumulative;
     const float* light_next;
     const float4* light_normals;       /* exact world normal of every emitter triangle */
-    uint32_t light_count;
+    const bf_dev_punctual* punctuals;
+    uint32_t punctual_count;           /* registered lights 0 .. punctual_count-1 */
+    uint32_t light_count;              /* punctual + mesh-area lights; lights[i - punctual_count] */
     uint32_t has_environment;
     uint32_t environment_spectrum;
     uint32_t material_count;

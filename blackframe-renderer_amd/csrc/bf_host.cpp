@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <array>
 #include <cmath>
 #include <cstdarg>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "../../include/blackframe_hip.h"
+#include "../../include/blackframe_math.h"
 #include "bf_kernels.h"
 
 namespace {
@@ -672,7 +674,54 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
 
     if (d->has_environment && d->environment_spectrum >= d->spectrum_count)
         return set_error(BF_INVALID_ARGUMENT, "environment spectrum out of range");
-    s->light_count = (uint32_t)lights.size();
+
+    /* -- punctual lights: the scene checks sample_punctual_light makes per sample
+     *    (CudaWavefrontStages.cu:2246-2340) are made once here, with the spot-cone cosines -- */
+    if (d->punctual_count && !d->punctuals) return set_error(BF_INVALID_ARGUMENT, "punctual lights without an array");
+    std::vector<bf_dev_punctual> punctuals(d->punctual_count);
+    for (uint32_t i = 0; i < d->punctual_count; ++i) {
+        const bf_punctual_desc& L = d->punctuals[i];
+        bf_dev_punctual& D = punctuals[i];
+        D = bf_dev_punctual{};
+        D.kind = L.kind;
+        D.spectrum = L.spectrum;
+        std::memcpy(D.position, L.position, sizeof D.position);
+        std::memcpy(D.position_error, L.position_error, sizeof D.position_error);
+        std::memcpy(D.direction, L.direction, sizeof D.direction);
+        if (L.kind > BF_PUNCTUAL_SPOT) return set_error(BF_INVALID_ARGUMENT, "punctual light %u: unknown kind %u", i, L.kind);
+        if (L.spectrum >= d->spectrum_count) return set_error(BF_INVALID_ARGUMENT, "punctual light %u: spectrum out of range", i);
+        if (L.kind != BF_PUNCTUAL_DIRECTIONAL)
+            for (int a = 0; a < 3; ++a)
+                if (!std::isfinite(L.position[a]) || !std::isfinite(L.position_error[a]) || !(L.position_error[a] >= 0.0f))
+                    return set_error(BF_INVALID_ARGUMENT, "punctual light %u: position and its error must be finite, error >= 0", i);
+        if (L.kind != BF_PUNCTUAL_POINT) {
+            /* transport::unit_vector (TransportDevice.cuh:325-333) */
+            const float* v = L.direction;
+            const float sq = std::fma(v[0], v[0], std::fma(v[1], v[1], v[2] * v[2]));
+            if (!std::isfinite(v[0]) || !std::isfinite(v[1]) || !std::isfinite(v[2]) || !std::isfinite(sq) ||
+                !(std::fabs(sq - 1.0f) <= 128.0f * FLT_EPSILON))
+                return set_error(BF_INVALID_ARGUMENT, "punctual light %u: direction must be a unit vector", i);
+        }
+        if (L.kind == BF_PUNCTUAL_SPOT) {
+            const float pi = 3.14159265358979323846f, inner = L.inner_half_angle, outer = L.outer_half_angle;
+            if (!std::isfinite(inner) || !std::isfinite(outer) || inner < 0.0f || inner > outer || !(outer > 0.0f) ||
+                !(outer < pi))
+                return set_error(BF_INVALID_ARGUMENT, "punctual light %u: spot angles need 0 <= inner <= outer < pi, outer > 0", i);
+            /* exact_special_cosine (:515-526) */
+            auto special_cosine = [pi](float angle) {
+                if (angle == 0.0f) return 1.0f;
+                if (angle == pi / 2.0f) return 0.0f;
+                return bf_cosf(angle);
+            };
+            D.cosine_inner = special_cosine(inner);
+            D.cosine_outer = special_cosine(outer);
+            if (!std::isfinite(D.cosine_inner) || !std::isfinite(D.cosine_outer) || D.cosine_inner < D.cosine_outer)
+                return set_error(BF_INVALID_ARGUMENT, "punctual light %u: spot cone cosines are not representable", i);
+            D.smooth = inner != outer ? 1u : 0u;
+        }
+    }
+    if (lights.size() + punctuals.size() > (1u << 24)) return set_error(BF_RESOURCE_EXHAUSTED, "more than 2^24 lights");
+    s->light_count = (uint32_t)(lights.size() + punctuals.size());
 
     /* -- textures: albedo / roughness images bound to closures (no reference counterpart) -- */
     if (d->texture_count > BF_MAX_TEXTURES || (d->texture_count && !d->textures))
@@ -751,6 +800,7 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     const size_t o_cum = A.add(cumul.data(), cumul.size() * sizeof(float));
     const size_t o_nex = A.add(nexts.data(), nexts.size() * sizeof(float));
     const size_t o_lno = A.add(normals4.data(), normals4.size() * sizeof(float));
+    const size_t o_pun = A.add(punctuals.data(), punctuals.size() * sizeof(bf_dev_punctual));
     const size_t o_tex = A.add(nullptr, d->texture_count * sizeof(bf_dev_texture));
     size_t o_texels[BF_MAX_TEXTURES] = {};
     for (uint32_t t = 0; t < d->texture_count; ++t) {
@@ -796,7 +846,9 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     S.light_cumulative = (const float*)(base + o_cum);
     S.light_next = (const float*)(base + o_nex);
     S.light_normals = (const float4*)(base + o_lno);
-    S.light_count = (uint32_t)lights.size();
+    S.punctuals = (const bf_dev_punctual*)(base + o_pun);
+    S.punctual_count = (uint32_t)punctuals.size();
+    S.light_count = (uint32_t)(lights.size() + punctuals.size());
     S.has_environment = d->has_environment ? 1u : 0u;
     S.environment_spectrum = d->environment_spectrum;
     S.material_count = d->material_count;

@@ -581,12 +581,110 @@ BF_D bool load_mixture(const bf_dev_scene& S, const bf_material_desc& M, const f
     return true;
 }
 
+/* IncidentLightKind of the reference: a mesh-area sample (solid-angle pdf, MIS), a point or
+ * spot light (discrete, shadow ray to the contracted point) or a directional light (discrete,
+ * shadow ray to infinity). */
+enum : uint32_t { BF_INCIDENT_SURFACE = 0u, BF_INCIDENT_POINT = 1u, BF_INCIDENT_INFINITE = 2u };
+
 struct incident_t {
     float radiance[4];
     v3f direction, position, error, normal;
     float conditional, selection;
+    uint32_t kind;
     bool present;
 };
+
+/* sample_punctual_light (:2240-2376): the scene checks it makes per sample are made by the host. */
+BF_D bool sample_punctual(const bf_dev_scene& S, uint32_t index, const float* wl, const surface_t& surf, float sel,
+                          incident_t& inc) {
+    const bf_dev_punctual L = S.punctuals[index];
+    float spectrum[4];
+    bool black = true;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        spectrum[c] = spectrum_at(S, L.spectrum, wl[c]);
+        if (!bf_isfinite(spectrum[c]) || spectrum[c] < 0.0f) return false;
+        if (spectrum[c] != 0.0f) black = false;
+    }
+    const v3f axis = mk(L.direction[0], L.direction[1], L.direction[2]);
+    if (L.kind == BF_PUNCTUAL_DIRECTIONAL) {
+        if (black) return true;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) inc.radiance[c] = spectrum[c];
+        inc.direction = mul3(axis, -1.0f);
+        inc.position = inc.error = inc.normal = mk(0.0f, 0.0f, 0.0f);
+        inc.conditional = 1.0f;
+        inc.selection = sel;
+        inc.kind = BF_INCIDENT_INFINITE;
+        inc.present = true;
+        return true;
+    }
+    const v3f position = mk(L.position[0], L.position[1], L.position[2]);
+    segment_t seg;
+    if (!scaled_segment(sub3(position, surf.position), seg)) return false;
+    const float ssd = dot3(seg.scaled, seg.scaled);
+    if (!bf_isfinite(ssd) || !(ssd > 0.0f)) return false;
+    float transition = 1.0f, smooth = 1.0f;
+    if (L.kind == BF_PUNCTUAL_SPOT) {
+        const float cone = dot3(axis, mul3(seg.direction, -1.0f));
+        if (!bf_isfinite(cone)) return false;
+        if (cone <= L.cosine_outer) return true;
+        if (L.smooth && cone < L.cosine_inner) {
+            transition = (cone - L.cosine_outer) / (L.cosine_inner - L.cosine_outer);
+            smooth = 3.0f - 2.0f * transition;
+            if (!bf_isfinite(transition) || !(transition > 0.0f) || !(transition < 1.0f) || !bf_isfinite(smooth) ||
+                !(smooth > 0.0f))
+                return false;
+        }
+    }
+    float radiance[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float num[4] = {spectrum[c], transition, transition, smooth};
+        const float den[3] = {seg.scale, seg.scale, ssd};
+        if (checked_product_quotient(num, den, radiance[c]) != BF_L_OK) return false;
+    }
+    if (black) return true;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) inc.radiance[c] = radiance[c];
+    inc.direction = seg.direction;
+    inc.position = position;
+    inc.error = mk(L.position_error[0], L.position_error[1], L.position_error[2]);
+    inc.normal = mk(0.0f, 0.0f, 0.0f);
+    inc.conditional = 1.0f;
+    inc.selection = sel;
+    inc.kind = BF_INCIDENT_POINT;
+    inc.present = true;
+    return true;
+}
+
+/* contract_point_endpoint (:2630-2682): the shadow ray of a point light stops short of the
+ * light's position by its error bound along the direction to the light. */
+BF_D bool endpoint_support(float direction, float error, float& contraction) {
+    const float candidate = fmaf(fabsf(direction), error, contraction);
+    if (!bf_isfinite(candidate) || candidate < contraction) return false;
+    if (direction != 0.0f && error != 0.0f) {
+        contraction = bf_next_up_f(candidate);
+        return bf_isfinite(contraction);
+    }
+    contraction = candidate;
+    return true;
+}
+
+BF_D bool contract_point_endpoint(v3f p, v3f err, v3f to_light, v3f& out) {
+    if (!fin3(p) || !fin3(err) || err.x < 0.0f || err.y < 0.0f || err.z < 0.0f || !fin3(to_light)) return false;
+    float contraction = 0.0f;
+    if (!endpoint_support(to_light.x, err.x, contraction) || !endpoint_support(to_light.y, err.y, contraction) ||
+        !endpoint_support(to_light.z, err.z, contraction))
+        return false;
+    out = p;
+    if (contraction == 0.0f) return true;
+    out = mk(fmaf(-to_light.x, contraction, p.x), fmaf(-to_light.y, contraction, p.y), fmaf(-to_light.z, contraction, p.z));
+    if (to_light.x != 0.0f) out.x = to_light.x > 0.0f ? bf_next_down_f(out.x) : bf_next_up_f(out.x);
+    if (to_light.y != 0.0f) out.y = to_light.y > 0.0f ? bf_next_down_f(out.y) : bf_next_up_f(out.y);
+    if (to_light.z != 0.0f) out.z = to_light.z > 0.0f ? bf_next_down_f(out.z) : bf_next_up_f(out.z);
+    return fin3(out);
+}
 
 /* sample_registered_light (:2378-2460) + sample_mesh_area_light (:2044-2201); the per-light area
  * CDF of :2085-2147 is a scene constant evaluated by the host, searched here in O(log n). */
@@ -597,7 +695,8 @@ BF_D bool sample_light(const bf_transport_params& P, const bf_dev_scene& S, cons
     uint32_t li;
     float sel;
     if (!uniform_light_selection(S.light_count, u_sel, li, sel)) return false;
-    const bf_dev_light L = S.lights[li];
+    if (li < S.punctual_count) return sample_punctual(S, li, wl, surf, sel, inc);
+    const bf_dev_light L = S.lights[li - S.punctual_count];
     const bf_dev_instance& I = S.instances[L.instance];
     if ((I.visibility_mask & visibility) == 0u) return true;
     const float area_sample = sample_1d(P.sampler, k, first_dim + 1);
@@ -665,6 +764,7 @@ BF_D bool sample_light(const bf_transport_params& P, const bf_dev_scene& S, cons
     inc.normal = n;
     inc.conditional = cond;
     inc.selection = sel;
+    inc.kind = BF_INCIDENT_SURFACE;
     inc.present = true;
     return true;
 }
@@ -677,7 +777,7 @@ BF_D bool emission_weight(const bf_transport_params& P, const bf_dev_scene& S, u
     const int32_t li = S.instances[inst].light_index;
     if (li < 0) return false;
     float sel;
-    if (!uniform_light_probability(S.light_count, (uint32_t)li, sel)) return false;
+    if (!uniform_light_probability(S.light_count, S.punctual_count + (uint32_t)li, sel)) return false;
     const float inv_area = S.lights[li].inverse_area;
     segment_t seg;
     if (!scaled_segment(sub3(surf.position, mk(prev.x, prev.y, prev.z)), seg)) return false;
@@ -948,10 +1048,13 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
                 float corr;
                 if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, corr)) return fail_path(P, J, slot, 23);
                 if (corr > 0.0f) {
-                    float joint, bsdf_pdf, w;
-                    if (!joint_light_pdf(inc.selection, inc.conditional, joint)) return fail_path(P, J, slot, 24);
-                    if (mixture_pdf(mix, filtered, wo_local, wi_local, bsdf_pdf) != BF_L_OK) return fail_path(P, J, slot, 24);
-                    if (!mis_weight(P.mis, joint, bsdf_pdf, w)) return fail_path(P, J, slot, 24);
+                    float w = 1.0f; /* punctual lights are discrete: no MIS (:4437-4470) */
+                    if (inc.kind == BF_INCIDENT_SURFACE) {
+                        float joint, bsdf_pdf;
+                        if (!joint_light_pdf(inc.selection, inc.conditional, joint)) return fail_path(P, J, slot, 24);
+                        if (mixture_pdf(mix, filtered, wo_local, wi_local, bsdf_pdf) != BF_L_OK) return fail_path(P, J, slot, 24);
+                        if (!mis_weight(P.mis, joint, bsdf_pdf, w)) return fail_path(P, J, slot, 24);
+                    }
                     /* the visible-shadow contribution of process_shadow_kernel (:4772-4908),
                      * evaluated now from the same frozen factors */
                     const float coefficient = singleton_lambert ? BF_INV_PI : 1.0f;
@@ -970,15 +1073,22 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
                     }
                     /* prepare_shadow_ray (:4042-4105) */
                     v3f src, endpoint;
-                    segment_t seg;
-                    if (!offset_point(surf.position, surf.error, surf.ng, inc.direction, src) ||
-                        !offset_point(inc.position, inc.error, inc.normal, mul3(inc.direction, -1.0f), endpoint) ||
-                        !scaled_segment(sub3(endpoint, src), seg) || !(dot3(seg.direction, inc.direction) > 0.0f))
-                        return fail_path(P, J, slot, 25);
-                    const float tmax = bf_next_toward_zero_f(seg.length);
-                    if (!(tmax > 0.0f)) return fail_path(P, J, slot, 25);
-                    shadow_o = make_float4(src.x, src.y, src.z, tmax);
-                    shadow_d = make_float4(seg.direction.x, seg.direction.y, seg.direction.z, 0.0f);
+                    if (!offset_point(surf.position, surf.error, surf.ng, inc.direction, src)) return fail_path(P, J, slot, 25);
+                    if (inc.kind == BF_INCIDENT_INFINITE) {
+                        shadow_o = make_float4(src.x, src.y, src.z, BF_INFINITY_F);
+                        shadow_d = make_float4(inc.direction.x, inc.direction.y, inc.direction.z, 0.0f);
+                    } else {
+                        segment_t seg;
+                        const bool ends = inc.kind == BF_INCIDENT_POINT
+                                              ? contract_point_endpoint(inc.position, inc.error, inc.direction, endpoint)
+                                              : offset_point(inc.position, inc.error, inc.normal, mul3(inc.direction, -1.0f), endpoint);
+                        if (!ends || !scaled_segment(sub3(endpoint, src), seg) || !(dot3(seg.direction, inc.direction) > 0.0f))
+                            return fail_path(P, J, slot, 25);
+                        const float tmax = bf_next_toward_zero_f(seg.length);
+                        if (!(tmax > 0.0f)) return fail_path(P, J, slot, 25);
+                        shadow_o = make_float4(src.x, src.y, src.z, tmax);
+                        shadow_d = make_float4(seg.direction.x, seg.direction.y, seg.direction.z, 0.0f);
+                    }
                     has_shadow = true;
                 }
             }

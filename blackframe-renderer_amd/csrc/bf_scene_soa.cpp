@@ -36,7 +36,8 @@ enum : uint32_t {
     C_INSTANCE_GEOMETRY_ID = 60, C_INSTANCE_MATERIAL_ID = 61, C_INSTANCE_VISIBILITY_MASK = 62,
     C_INSTANCE_LOCAL_TO_PARENT = 63, C_INSTANCE_PARENT_TO_LOCAL = 79, C_INSTANCE_LOCAL_TO_WORLD = 95,
     C_INSTANCE_WORLD_TO_LOCAL = 111,
-    C_PUNCTUAL_KIND = 127, C_MESH_AREA_LIGHT_INSTANCE_ID = 143,
+    C_PUNCTUAL_KIND = 127, C_PUNCTUAL_POSITION_X = 128, C_PUNCTUAL_POSITION_ERROR_X = 131, C_PUNCTUAL_DIRECTION_X = 134,
+    C_PUNCTUAL_INNER = 137, C_PUNCTUAL_OUTER = 138, C_PUNCTUAL_SPECTRUM = 139, C_MESH_AREA_LIGHT_INSTANCE_ID = 143,
     C_ENVIRONMENT_WAVELENGTH_NM = 144, C_ENVIRONMENT_WAVELENGTH_PDF = 148, C_ENVIRONMENT_WAVELENGTH_MEASURE = 152,
     C_ENVIRONMENT_RADIANCE = 156,
     C_TEXTURE_ID = 160, C_TEXTURE_KIND = 161, C_TEXTURE_VALUE = 162,
@@ -242,8 +243,6 @@ int index_of(const std::vector<uint32_t>& sorted_ids, uint32_t id, uint32_t& ind
 int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t flags, bf_hip_scene** out) {
     (void)flags;
     const Blob B{bytes, h};
-    if (h.punctual_light_count != 0)
-        return bf_report_error(BF_UNAVAILABLE, "scene blob: punctual lights are not rendered by this transport");
     for (uint8_t present : B.column<uint8_t>(C_NORMAL_MAP_PRESENT))
         if (present) return bf_report_error(BF_UNAVAILABLE, "scene blob: normal maps are not rendered by this transport");
     for (uint8_t present : B.column<uint8_t>(C_BUMP_MAP_PRESENT))
@@ -443,6 +442,38 @@ int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t
         d.environment_spectrum = BF_INVALID_INDEX;
     }
 
+    /* punctual lights, in registry order (SceneSoA.cpp:818-878); their spectra are the path's lanes */
+    const uint64_t P = h.punctual_light_count;
+    if (P > 0xFFFFFFFFull) return bf_report_error(BF_RESOURCE_EXHAUSTED, "scene blob: punctual light count exceeds 32 bits");
+    if (P != 0 && !have_packet)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: punctual lights need the scene's wavelength packet");
+    std::vector<bf_punctual_desc> punctuals(P);
+    {
+        const auto kind = B.column<uint32_t>(C_PUNCTUAL_KIND);
+        std::vector<float> pos[3], err[3], dir[3], lanes[4];
+        for (uint32_t a = 0; a < 3; ++a) {
+            pos[a] = B.column<float>(C_PUNCTUAL_POSITION_X + a);
+            err[a] = B.column<float>(C_PUNCTUAL_POSITION_ERROR_X + a);
+            dir[a] = B.column<float>(C_PUNCTUAL_DIRECTION_X + a);
+        }
+        for (uint32_t k = 0; k < 4; ++k) lanes[k] = B.column<float>(C_PUNCTUAL_SPECTRUM + k);
+        const auto inner = B.column<float>(C_PUNCTUAL_INNER), outer = B.column<float>(C_PUNCTUAL_OUTER);
+        for (uint64_t i = 0; i < P; ++i) {
+            bf_punctual_desc& L = punctuals[i];
+            L = bf_punctual_desc{};
+            L.kind = kind[i];
+            for (uint32_t a = 0; a < 3; ++a) {
+                L.position[a] = pos[a][i];
+                L.position_error[a] = err[a][i];
+                L.direction[a] = dir[a][i];
+            }
+            L.inner_half_angle = inner[i];
+            L.outer_half_angle = outer[i];
+            const float v[4] = {lanes[0][i], lanes[1][i], lanes[2][i], lanes[3][i]};
+            L.spectrum = spectra.add(v);
+        }
+    }
+
     if (spectra.values.empty()) spectra.values.push_back(0.0f); /* the builder wants a value array */
     if (spectra.descs.empty()) spectra.descs.push_back(bf_spectrum_desc{0u, 1u});
     d.geometry_count = G;
@@ -465,6 +496,8 @@ int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t
     d.spectrum_values = spectra.values.data();
     d.texture_count = 0;
     d.textures = nullptr;
+    d.punctual_count = (uint32_t)P;
+    d.punctuals = punctuals.empty() ? nullptr : punctuals.data();
     return blackframe_hip_scene_create(&d, out);
 }
 

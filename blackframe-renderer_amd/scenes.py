@@ -75,6 +75,7 @@ class Scene:
         self.geometries, self.instances, self.materials, self.closures = [], [], [], []
         self.spectra, self.spectrum_values = [], []
         self.textures = []
+        self.punctuals = []
         self.vertex_count = 0
         self.triangle_count = 0
         self.environment_spectrum = abi.INVALID
@@ -188,6 +189,28 @@ class Scene:
     def set_environment(self, radiance):
         self.environment_spectrum = self.spectrum(radiance)
 
+    # -- punctual lights ------------------------------------------------------------------------
+    def add_point_light(self, position, intensity, position_error=(0.0, 0.0, 0.0)):
+        return self._punctual(abi.PUNCTUAL_POINT, intensity, position, position_error, (0.0, 0.0, 0.0), 0.0, 0.0)
+
+    def add_spot_light(self, position, axis, inner_half_angle, outer_half_angle, intensity,
+                       position_error=(0.0, 0.0, 0.0)):
+        return self._punctual(abi.PUNCTUAL_SPOT, intensity, position, position_error, axis, inner_half_angle,
+                              outer_half_angle)
+
+    def add_directional_light(self, propagation, irradiance):
+        return self._punctual(abi.PUNCTUAL_DIRECTIONAL, irradiance, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), propagation, 0.0, 0.0)
+
+    def _punctual(self, kind, spectrum, position, error, direction, inner, outer):
+        rec = abi.PunctualDesc()
+        rec.kind = kind
+        rec.spectrum = self.spectrum(spectrum)
+        for a in range(3):
+            rec.position[a], rec.position_error[a], rec.direction[a] = position[a], error[a], direction[a]
+        rec.inner_half_angle, rec.outer_half_angle = inner, outer
+        self.punctuals.append(rec)
+        return len(self.punctuals) - 1
+
     # -- packing ------------------------------------------------------------------------------
     def finalize(self):
         self.positions = np.ascontiguousarray(np.concatenate(self._positions)).astype(np.float32)
@@ -232,6 +255,9 @@ class Scene:
             t.device_data = pointer
         d.texture_count = len(self.textures)
         d.textures = C.addressof(self._texture_array) if self.textures else None
+        self._punctual_array = (abi.PunctualDesc * max(1, len(self.punctuals)))(*self.punctuals)
+        d.punctual_count = len(self.punctuals)
+        d.punctuals = C.addressof(self._punctual_array) if self.punctuals else None
         self.desc = d
         return self
 

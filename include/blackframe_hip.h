@@ -218,8 +218,9 @@ int blackframe_hip_scene_soa_validate(const void* blob, size_t byte_count, uint3
  * header (see blackframe_hip_scene_soa_validate), non-canonical ids, inconsistent ranges,
  * unknown references, a mesh-area-light column that disagrees with the emissive instances, or
  * wavelengths other than one uniform-visible packet (pdf 1/470 per nm, wavelength measure); with
- * BF_UNAVAILABLE punctual lights, normal or bump maps and instanced materials without spectral
- * data, which this transport does not render. Constant and image textures are accepted and unused.
+ * BF_UNAVAILABLE normal or bump maps and instanced materials without spectral data, which this
+ * transport does not render. Punctual lights keep their registry order, and their spectra apply
+ * lane by lane. Constant and image textures are accepted and unused.
  * Replaces: CudaSceneSoA::upload (Backends/GPU/CUDA/Include/Blackframe/Backends/GPU/CUDA/SceneSoA.hpp:33,
  *           Source/SceneSoA.cpp:1180-1210).
  */

@@ -417,6 +417,22 @@ typedef struct bf_instance_desc {
     float world_to_local[12];
 } bf_instance_desc;
 
+/* A punctual light (the reference's ScenePointLight / SceneSpotLight / SceneDirectionalLight,
+ * serialized as SceneSoA.cpp:818-878 and sampled by sample_punctual_light,
+ * CudaWavefrontStages.cu:2240-2376). Lights are registered punctual first, then mesh-area lights
+ * (sample_registered_light, :2378-2403), and one is selected uniformly. */
+enum bf_punctual_kind { BF_PUNCTUAL_POINT = 0, BF_PUNCTUAL_DIRECTIONAL = 1, BF_PUNCTUAL_SPOT = 2 };
+typedef struct bf_punctual_desc {
+    uint32_t kind;              /* bf_punctual_kind */
+    uint32_t spectrum;          /* radiant intensity (point; spot on its axis) or irradiance (directional) */
+    float position[3];          /* point and spot */
+    float position_error[3];    /* absolute error bound of position, every component >= 0 */
+    float direction[3];         /* unit: propagation (directional) or emission axis (spot) */
+    float inner_half_angle;     /* spot, radians: 0 <= inner <= outer, 0 < outer < pi */
+    float outer_half_angle;
+    uint32_t reserved;
+} bf_punctual_desc;
+
 typedef struct bf_scene_desc {
     uint32_t geometry_count, instance_count, material_count, closure_count, spectrum_count;
     uint32_t has_environment;
@@ -434,6 +450,9 @@ typedef struct bf_scene_desc {
     uint32_t environment_spectrum;
     uint32_t texture_count;                 /* <= BF_MAX_TEXTURES */
     const bf_texture_desc* textures;
+    uint32_t punctual_count;
+    uint32_t reserved_punctual;
+    const bf_punctual_desc* punctuals;
 } bf_scene_desc;
 
 /* ---- camera / render description ------------------------------------------------------------ */
@@ -572,6 +591,7 @@ BF_STATIC_ASSERT(sizeof(bf_path_slot) == 4, "PathSlot is 4 bytes");
 BF_STATIC_ASSERT(sizeof(bf_path_state_lane) == 128, "TransportPathStateLane is 128 bytes");
 BF_STATIC_ASSERT(offsetof(bf_path_state_lane, wavelength_pdf_measures) == 100, "path state measures at 100");
 BF_STATIC_ASSERT(sizeof(bf_layout_manifest) == 320, "LayoutManifest is 320 bytes");
+BF_STATIC_ASSERT(sizeof(bf_punctual_desc) == 56, "punctual light record is 56 bytes");
 BF_STATIC_ASSERT(sizeof(bf_scene_soa_column) == 24, "SceneSoaColumnDescriptor is 24 bytes");
 BF_STATIC_ASSERT(sizeof(bf_scene_soa_header) == BF_SCENE_SOA_HEADER_SIZE, "SceneSoaHeader is 4848 bytes");
 BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, content_hash) == BF_SCENE_SOA_HASH_OFFSET, "content hash at 32");

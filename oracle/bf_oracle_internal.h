@@ -73,6 +73,10 @@ struct bfo_scene {
     uint32_t light_count;
     bfo_light* lights;
     int32_t* instance_light; /* instance -> light index or -1 */
+    /* punctual lights, registered before the mesh-area lights; spot-cone cosines (inner, outer) */
+    uint32_t punctual_count;
+    bf_punctual_desc* punctuals;
+    float* punctual_cosines;
 };
 
 /* vector helpers restated from Blackframe/XPU/CUDA/Source/CudaWavefrontStages.cu:298-349 */

@@ -561,6 +561,21 @@ int bfo_scene_build(const bf_scene_desc* desc, bfo_scene** out) {
                 status = BF_INVALID_ARGUMENT;
         }
     }
+    /* punctual lights: the exact_special_cosine of CudaWavefrontStages.cu:515-526 for spot cones */
+    s->punctual_count = desc->punctual_count;
+    s->desc.punctuals = NULL;
+    if (desc->punctual_count) {
+        DUP(s->punctuals, desc->punctuals, desc->punctual_count, bf_punctual_desc);
+        s->punctual_cosines = (float*)calloc(2 * (size_t)desc->punctual_count, sizeof(float));
+        s->desc.punctuals = s->punctuals;
+        const float pi = 3.14159265358979323846f;
+        for (uint32_t i = 0; i < desc->punctual_count; ++i) {
+            const float angles[2] = {s->punctuals[i].inner_half_angle, s->punctuals[i].outer_half_angle};
+            for (int k = 0; k < 2; ++k)
+                s->punctual_cosines[2 * i + k] =
+                    angles[k] == 0.0f ? 1.0f : (angles[k] == pi / 2.0f ? 0.0f : bf_cosf(angles[k]));
+        }
+    }
     if (status == BF_OK) status = build_bvh(s);
     if (status == BF_OK) status = build_lights(s);
     if (status != BF_OK) {
@@ -583,6 +598,8 @@ void bfo_scene_free(bfo_scene* s) {
     free(s->closures);
     free(s->spectra);
     free(s->spectrum_values);
+    free(s->punctuals);
+    free(s->punctual_cosines);
     for (uint32_t t = 0; t < BF_MAX_TEXTURES; ++t) free(s->texels[t]);
     free(s->blases);
     free(s->blas_nodes);

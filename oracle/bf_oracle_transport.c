@@ -597,20 +597,83 @@ typedef struct {
     float radiance[4];
     v3 direction, position, error, normal;
     float distance, conditional, selection;
-    uint32_t kind; /* 0 none, 2 finite surface */
+    uint32_t kind; /* IncidentLightKind: 0 none, 1 infinite, 2 finite surface, 3 finite point */
 } incident_t;
 
-/* :2044-2201 + :2378-2460 (mesh-area lights; the scene has no punctual lights) */
+enum { INCIDENT_NONE = 0, INCIDENT_INFINITE = 1, INCIDENT_SURFACE = 2, INCIDENT_POINT = 3 };
+
+/* :2240-2376 (the scene checks are the product host's; a valid scene passes them) */
+static int sample_punctual_light(const bfo_scene* s, uint32_t index, const lane* l, const surface_t* surf, float sel,
+                                 incident_t* inc) {
+    const bf_punctual_desc* L = &s->punctuals[index];
+    float spectrum[4];
+    int black = 1;
+    for (int k = 0; k < 4; ++k) {
+        spectrum[k] = spectrum_at(s, L->spectrum, l->wl[k]);
+        if (!bf_isfinite(spectrum[k]) || spectrum[k] < 0.0f) return 0;
+        if (spectrum[k] != 0.0f) black = 0;
+    }
+    const v3 axis = v3_make(L->direction[0], L->direction[1], L->direction[2]);
+    if (L->kind == BF_PUNCTUAL_DIRECTIONAL) {
+        if (black) return 1;
+        memcpy(inc->radiance, spectrum, sizeof spectrum);
+        inc->direction = v3_mul(axis, -1.0f);
+        inc->distance = INFINITY;
+        inc->conditional = 1.0f;
+        inc->selection = sel;
+        inc->kind = INCIDENT_INFINITE;
+        return 1;
+    }
+    const v3 position = v3_make(L->position[0], L->position[1], L->position[2]);
+    segment_t seg;
+    if (!scaled_segment(v3_sub(position, surf->position), &seg)) return 0;
+    const float ssd = v3_dot(seg.scaled, seg.scaled);
+    if (!bf_isfinite(ssd) || !(ssd > 0.0f)) return 0;
+    float transition = 1.0f, smooth = 1.0f;
+    if (L->kind == BF_PUNCTUAL_SPOT) {
+        const float cos_inner = s->punctual_cosines[2 * index], cos_outer = s->punctual_cosines[2 * index + 1];
+        const float cone = v3_dot(axis, v3_mul(seg.direction, -1.0f));
+        if (!bf_isfinite(cone)) return 0;
+        if (cone <= cos_outer) return 1;
+        if (L->inner_half_angle != L->outer_half_angle && cone < cos_inner) {
+            transition = (cone - cos_outer) / (cos_inner - cos_outer);
+            smooth = 3.0f - 2.0f * transition;
+            if (!bf_isfinite(transition) || !(transition > 0.0f) || !(transition < 1.0f) || !bf_isfinite(smooth) ||
+                !(smooth > 0.0f))
+                return 0;
+        }
+    }
+    float radiance[4];
+    for (int k = 0; k < 4; ++k) {
+        const float num[4] = {spectrum[k], transition, transition, smooth};
+        const float den[3] = {seg.scale, seg.scale, ssd};
+        if (lobe_checked_product_quotient(num, 4, den, 3, &radiance[k]) != LOBE_OK) return 0;
+    }
+    if (black) return 1;
+    memcpy(inc->radiance, radiance, sizeof radiance);
+    inc->direction = seg.direction;
+    inc->position = position;
+    inc->error = v3_make(L->position_error[0], L->position_error[1], L->position_error[2]);
+    inc->distance = seg.length;
+    inc->conditional = 1.0f;
+    inc->selection = sel;
+    inc->kind = INCIDENT_POINT;
+    return 1;
+}
+
+/* :2044-2201 + :2378-2460: punctual lights first, then the mesh-area lights */
 static int sample_light(const ctx* c, const lane* l, const surface_t* surf, uint64_t first_dim, incident_t* inc) {
     const bfo_scene* s = c->scene;
     memset(inc, 0, sizeof *inc);
-    const uint32_t count = s->light_count;
+    const uint32_t count = s->punctual_count + s->light_count;
     if (count == 0) return 0;
     const uint32_t sampler = c->options->sampler;
     const float u_sel = bfo_sample_1d(sampler, &l->sample, first_dim + 0);
     uint32_t li;
     float sel;
     if (!uniform_light_selection(count, u_sel, &li, &sel)) return 0;
+    if (li < s->punctual_count) return sample_punctual_light(s, li, l, surf, sel, inc);
+    li -= s->punctual_count;
     const bfo_light* light = &s->lights[li];
     const bf_instance_desc* id = &s->instances[light->instance];
     if ((id->visibility_mask & l->ray.visibility_mask) == 0u) return 1;
@@ -674,7 +737,7 @@ static int sample_light(const ctx* c, const lane* l, const surface_t* surf, uint
     inc->distance = seg.length;
     inc->conditional = cond;
     inc->selection = sel;
-    inc->kind = 2;
+    inc->kind = INCIDENT_SURFACE;
     return 1;
 }
 
@@ -686,7 +749,7 @@ static int emissive_hit_mis_weight(const ctx* c, const lane* l, const surface_t*
     const int32_t li = s->instance_light[l->hit_instance];
     if (li < 0) return 0;
     float sel;
-    if (!uniform_light_probability(s->light_count, (uint32_t)li, &sel)) return 0;
+    if (!uniform_light_probability(s->punctual_count + s->light_count, s->punctual_count + (uint32_t)li, &sel)) return 0;
     const float inv_area = s->lights[li].inverse_area;
     segment_t seg;
     if (!scaled_segment(v3_sub(surf->position, v3_make(l->prev_ctx[0], l->prev_ctx[1], l->prev_ctx[2])), &seg)) return 0;
@@ -706,19 +769,61 @@ static int emissive_hit_mis_weight(const ctx* c, const lane* l, const surface_t*
     return 1;
 }
 
+/* :2630-2682 */
+static int accumulate_point_endpoint_support(float direction, float error, float* contraction) {
+    const float candidate = fmaf(fabsf(direction), error, *contraction);
+    if (!bf_isfinite(candidate) || candidate < *contraction) return 0;
+    if (direction != 0.0f && error != 0.0f) {
+        *contraction = bf_next_up_f(candidate);
+        return bf_isfinite(*contraction);
+    }
+    *contraction = candidate;
+    return 1;
+}
+
+static int contract_point_endpoint(v3 p, v3 err, v3 to_light, v3* out) {
+    if (!v3_finite(p) || !v3_finite(err) || err.x < 0.0f || err.y < 0.0f || err.z < 0.0f || !v3_finite(to_light)) return 0;
+    float contraction = 0.0f;
+    if (!accumulate_point_endpoint_support(to_light.x, err.x, &contraction) ||
+        !accumulate_point_endpoint_support(to_light.y, err.y, &contraction) ||
+        !accumulate_point_endpoint_support(to_light.z, err.z, &contraction))
+        return 0;
+    *out = p;
+    if (contraction == 0.0f) return 1;
+    v3 q = v3_make(fmaf(-to_light.x, contraction, p.x), fmaf(-to_light.y, contraction, p.y), fmaf(-to_light.z, contraction, p.z));
+    if (to_light.x != 0.0f) q.x = to_light.x > 0.0f ? bf_next_down_f(q.x) : bf_next_up_f(q.x);
+    if (to_light.y != 0.0f) q.y = to_light.y > 0.0f ? bf_next_down_f(q.y) : bf_next_up_f(q.y);
+    if (to_light.z != 0.0f) q.z = to_light.z > 0.0f ? bf_next_down_f(q.z) : bf_next_up_f(q.z);
+    *out = q;
+    return v3_finite(q);
+}
+
 /* :4042-4105 */
 static int prepare_shadow_ray(const surface_t* surf, const incident_t* inc, lane* l) {
     v3 src, endpoint;
     if (!offset_point(surf->position, surf->error, surf->ng, inc->direction, &src)) return 0;
-    if (!offset_point(inc->position, inc->error, inc->normal, v3_mul(inc->direction, -1.0f), &endpoint)) return 0;
-    segment_t seg;
-    if (!scaled_segment(v3_sub(endpoint, src), &seg) || !(v3_dot(seg.direction, inc->direction) > 0.0f)) return 0;
     bf_transport_ray* r = &l->shadow_ray;
     memset(r, 0, sizeof *r);
     r->origin_x = src.x;
     r->origin_y = src.y;
     r->origin_z = src.z;
     r->t_min = 0.0f;
+    r->time = l->ray.time;
+    r->visibility_mask = l->ray.visibility_mask;
+    if (inc->kind == INCIDENT_INFINITE) {
+        r->direction_x = inc->direction.x;
+        r->direction_y = inc->direction.y;
+        r->direction_z = inc->direction.z;
+        r->t_max = INFINITY;
+        return 1;
+    }
+    if (inc->kind == INCIDENT_POINT) {
+        if (!contract_point_endpoint(inc->position, inc->error, inc->direction, &endpoint)) return 0;
+    } else if (!offset_point(inc->position, inc->error, inc->normal, v3_mul(inc->direction, -1.0f), &endpoint)) {
+        return 0;
+    }
+    segment_t seg;
+    if (!scaled_segment(v3_sub(endpoint, src), &seg) || !(v3_dot(seg.direction, inc->direction) > 0.0f)) return 0;
     r->direction_x = seg.direction.x;
     r->direction_y = seg.direction.y;
     r->direction_z = seg.direction.z;
@@ -942,7 +1047,7 @@ static void shade_lane(const ctx* c, lane* l) {
     }
     incident_t inc;
     memset(&inc, 0, sizeof inc);
-    if (s->light_count != 0 && lobe_mixture_has_nonblack_weight(&mix)) {
+    if (s->punctual_count + s->light_count != 0 && lobe_mixture_has_nonblack_weight(&mix)) {
         l->light_sampled = 1;
         if (!sample_light(c, l, &surf, first_dim, &inc)) return fail(l, 21);
     }
@@ -967,10 +1072,14 @@ static void shade_lane(const ctx* c, lane* l) {
                 float corr;
                 if (!shading_correction(surf.ng, surf.ns, wo, inc.direction, &corr)) return fail(l, 23);
                 if (corr > 0.0f) {
-                    float joint, bsdf_pdf, w;
-                    if (!lobe_joint_light_pdf(inc.selection, inc.conditional, &joint)) return fail(l, 24);
-                    if (lobe_filtered_mixture_pdf(&mix, &filtered, wo_local, wi_local, &bsdf_pdf) != LOBE_OK) return fail(l, 24);
-                    if (!lobe_mis_weight(o->mis_heuristic, joint, bsdf_pdf, &w)) return fail(l, 24);
+                    float w = 1.0f; /* discrete (punctual) incident light: no MIS (:4437-4470) */
+                    if (inc.kind == INCIDENT_SURFACE) {
+                        float joint, bsdf_pdf;
+                        if (!lobe_joint_light_pdf(inc.selection, inc.conditional, &joint)) return fail(l, 24);
+                        if (lobe_filtered_mixture_pdf(&mix, &filtered, wo_local, wi_local, &bsdf_pdf) != LOBE_OK)
+                            return fail(l, 24);
+                        if (!lobe_mis_weight(o->mis_heuristic, joint, bsdf_pdf, &w)) return fail(l, 24);
+                    }
                     memcpy(l->p_beta, l->beta, sizeof l->p_beta);
                     memcpy(l->p_bsdf, bsdf, sizeof bsdf);
                     memcpy(l->p_L, inc.radiance, sizeof l->p_L);

@@ -144,6 +144,20 @@ def scene_from_canonical(bf, canon, constant_spectra=False):
             rec.local_to_world[e] = float(l2w[e])
             rec.world_to_local[e] = float(w2l[e])
         s.instances.append(rec)
+    kinds = {"point": abi.PUNCTUAL_POINT, "directional": abi.PUNCTUAL_DIRECTIONAL, "spot": abi.PUNCTUAL_SPOT}
+    for light in canon["punctual_lights"]:
+        rec = abi.PunctualDesc()
+        rec.kind = kinds[light["kind"]]
+        rec.spectrum = int(spectrum(packet, light["spectrum"]))
+        for a in range(3):
+            if light["kind"] != "directional":
+                rec.position[a] = light["position"][a]
+                rec.position_error[a] = light["position_error"][a]
+            if light["kind"] != "point":
+                rec.direction[a] = light["direction"][a]
+        if light["kind"] == "spot":
+            rec.inner_half_angle, rec.outer_half_angle = light["inner"], light["outer"]
+        s.punctuals.append(rec)
     if canon["environment"] is not None:
         s.environment_spectrum = int(spectrum(packet, canon["environment"]["radiance"]))
     return s.finalize()

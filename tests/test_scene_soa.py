@@ -225,8 +225,14 @@ def ingest_status(hip, blob, verify_hash=False):
 
 def test_blob_ingest_rejections_before_any_device_work(bf, hip, abi):
     """Each case is refused by host checks that run before the device scene is built."""
-    status, message = ingest_status(hip, rich_blob())
-    assert status == abi.UNAVAILABLE and "punctual" in message
+    rich = rich_blob()
+    # the spot light's axis is no longer a unit vector
+    status, message = ingest_status(hip, patched(rich, soa.PUNCTUAL_DIRECTION_X + 2, 0, -0.5))
+    assert status == abi.INVALID_ARGUMENT and "unit vector" in message
+    # inner cone wider than the outer one
+    assert ingest_status(hip, patched(rich, soa.PUNCTUAL_INNER, 0, 0.75))[0] == abi.INVALID_ARGUMENT
+    # negative position error
+    assert ingest_status(hip, patched(rich, soa.PUNCTUAL_POSITION_ERROR_X, 1, -0.01))[0] == abi.INVALID_ARGUMENT
     d, _ = cornell_description(bf)
     blob = soa.serialize(soa.frame_scene(d))
     corrupt = bytearray(blob)
