@@ -106,6 +106,153 @@ BF_D void empty_sample(lobe_sample_t& s, int status) {
 
 #include "bf_device_microfacet.h"
 
+/* ---- specular delta closures (TransportLobesDevice.cuh:1689-1960) ---------------------------
+ * Their value and density toward any given direction are zero; sampling returns the one mirror
+ * or refracted direction with a discrete probability of one. */
+BF_D int specular_zero(const closure_t& c, v3f wo, v3f wi) {
+    if (c.kind == BF_CLOSURE_SPECULAR_TRANSMISSION &&
+        (!bf_isfinite(c.parameters[0]) || !(c.parameters[0] > 0.0f) || !bf_isfinite(c.parameters[1]) ||
+         !(c.parameters[1] > 0.0f)))
+        return BF_L_INVALID;
+    return reflectance_ok(c.weight) && unit_vector(wo) && unit_vector(wi) ? BF_L_OK : BF_L_INVALID;
+}
+
+/* checked_positive_product_quotient (:1756-1782): left * right / denominator on significands */
+BF_D int positive_product_quotient(float left, float right, float denominator, float& out) {
+    out = 0.0f;
+    if (left == 0.0f) return BF_L_OK;
+    if (!bf_isfinite(left) || !(left > 0.0f) || !bf_isfinite(right) || !(right > 0.0f) || !bf_isfinite(denominator) ||
+        !(denominator > 0.0f))
+        return BF_L_INVALID;
+    int le, re, de, ne;
+    const float ls = bf_frexpf(left, &le), rs = bf_frexpf(right, &re), ds = bf_frexpf(denominator, &de);
+    const float ns = bf_frexpf((ls * rs) / ds, &ne);
+    const float r = bf_ldexpf(ns, le + re - de + ne);
+    if (!bf_isfinite(r) || !(r > 0.0f)) return BF_L_NOT_REPRESENTABLE;
+    out = r;
+    return BF_L_OK;
+}
+
+/* scaled_delta_value (:1784-1797) */
+BF_D int delta_value(const float* coefficient, float factor, float cosine, float* out) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int st = positive_product_quotient(coefficient[k], factor, cosine, out[k]);
+        if (st != BF_L_OK) return st;
+    }
+    return BF_L_OK;
+}
+
+/* checked_squared_ratio (:1799-1819) */
+BF_D int squared_ratio(float numerator, float denominator, float& out) {
+    out = 0.0f;
+    if (!bf_isfinite(numerator) || !(numerator > 0.0f) || !bf_isfinite(denominator) || !(denominator > 0.0f))
+        return BF_L_INVALID;
+    int ne, de, re;
+    const float ns = bf_frexpf(numerator, &ne), ds = bf_frexpf(denominator, &de);
+    const float q = ns / ds;
+    const float rs = bf_frexpf(q * q, &re);
+    const float r = bf_ldexpf(rs, 2 * (ne - de) + re);
+    if (!bf_isfinite(r) || !(r > 0.0f)) return BF_L_NOT_REPRESENTABLE;
+    out = r;
+    return BF_L_OK;
+}
+
+BF_D void delta_sample(lobe_sample_t& out, v3f wi, const float* value, uint32_t lobes, float eta_scale) {
+    copy4(out.value, value);
+    out.incoming = wi;
+    out.pdf = 1.0f;
+    out.discrete = 1u;
+    out.lobes = lobes;
+    out.eta_scale = eta_scale;
+    out.status = BF_L_OK;
+}
+
+/* sample_specular_reflection (:1821-1845) */
+BF_D void specular_reflection_sample(const float* w, v3f wo, lobe_sample_t& out) {
+    empty_sample(out, BF_L_INVALID);
+    if (!reflectance_ok(w) || !unit_vector(wo)) return;
+    if (wo.z == 0.0f) {
+        out.status = BF_L_OUTSIDE;
+        return;
+    }
+    const v3f wi = mk(-wo.x, -wo.y, wo.z);
+    float value[4];
+    const int st = delta_value(w, 1.0f, fabsf(wi.z), value);
+    if (st != BF_L_OK) {
+        out.status = st;
+        return;
+    }
+    delta_sample(out, wi, value, BF_LOBE_SPECULAR | BF_LOBE_REFLECTION, 1.0f);
+}
+
+/* sample_specular_transmission (:1847-1958), radiance transport mode */
+BF_D void specular_transmission_sample(const float* w, float exterior, float interior, v3f wo, lobe_sample_t& out) {
+    empty_sample(out, BF_L_INVALID);
+    if (!reflectance_ok(w) || !bf_isfinite(exterior) || !(exterior > 0.0f) || !bf_isfinite(interior) || !(interior > 0.0f) ||
+        !unit_vector(wo))
+        return;
+    if (wo.z == 0.0f) {
+        out.status = BF_L_OUTSIDE;
+        return;
+    }
+    const bool entering = wo.z > 0.0f;
+    const float eta_i = entering ? exterior : interior, eta_t = entering ? interior : exterior;
+    float value[4];
+    if (eta_i == eta_t) {
+        const v3f wi = mk(-wo.x, -wo.y, -wo.z);
+        const int st = delta_value(w, 1.0f, fabsf(wi.z), value);
+        if (st != BF_L_OK) {
+            out.status = st;
+            return;
+        }
+        delta_sample(out, wi, value, BF_LOBE_SPECULAR | BF_LOBE_TRANSMISSION, 1.0f);
+        return;
+    }
+    out.status = BF_L_NOT_REPRESENTABLE;
+    const float raw_sine = bf_hypotf(wo.x, wo.y);
+    if (!bf_isfinite(raw_sine)) return;
+    float sine = raw_sine, tangent_normalization = 1.0f;
+    if (raw_sine > 1.0f) {
+        const float cosine = fabsf(wo.z);
+        if (cosine > 1.0f) return;
+        sine = sqrtf((1.0f - cosine) * (1.0f + cosine));
+        tangent_normalization = sine / raw_sine;
+        if (!bf_isfinite(sine) || !bf_isfinite(tangent_normalization) || !(tangent_normalization > 0.0f)) return;
+    }
+    if (sine > 0.0f) {
+        const float critical = eta_t / eta_i;
+        if (critical <= 1.0f && sine >= critical) {
+            out.status = BF_L_OUTSIDE;
+            return;
+        }
+    }
+    const float ratio = eta_i / eta_t;
+    if (!bf_isfinite(ratio) || !(ratio > 0.0f)) return;
+    const float ix = -ratio * tangent_normalization * wo.x;
+    const float iy = -ratio * tangent_normalization * wo.y;
+    if (!bf_isfinite(ix) || !bf_isfinite(iy) || (wo.x != 0.0f && ix == 0.0f) || (wo.y != 0.0f && iy == 0.0f)) return;
+    const float t_sine = bf_hypotf(ix, iy);
+    if (!bf_isfinite(t_sine) || !(t_sine < 1.0f)) return;
+    const float t_cosine = sqrtf((1.0f - t_sine) * (1.0f + t_sine));
+    if (!bf_isfinite(t_cosine) || !(t_cosine > 0.0f)) return;
+    const v3f wi = mk(ix, iy, entering ? -t_cosine : t_cosine);
+    if (!unit_vector(wi)) return;
+    float factor, eta_scale;
+    int st = squared_ratio(eta_i, eta_t, factor);
+    const int st2 = squared_ratio(eta_t, eta_i, eta_scale);
+    if (st != BF_L_OK || st2 != BF_L_OK) {
+        out.status = st != BF_L_OK ? st : st2;
+        return;
+    }
+    st = delta_value(w, factor, t_cosine, value);
+    if (st != BF_L_OK) {
+        out.status = st;
+        return;
+    }
+    delta_sample(out, wi, value, BF_LOBE_SPECULAR | BF_LOBE_TRANSMISSION, eta_scale);
+}
+
 /* ---- record dispatch (TransportLobesDevice.cuh:2092-2260); transport mode radiance ---------- */
 BF_D int record_eval(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float* out) {
     if (!mf_valid_closure(&c)) return BF_L_INVALID;
@@ -119,6 +266,10 @@ BF_D int record_eval(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float
         return mf_conductor_eval(c.weight, p, p + 4, p[8], p[9], wo, wi, out);
     case BF_CLOSURE_ROUGH_DIELECTRIC:
         return mf_dielectric_eval_masked(c.weight, p[0], p[1], p[2], p[3], wo, wi, BF_TRANSPORT_RADIANCE, allowed, out);
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return specular_zero(c, wo, wi);
     default:
         return BF_L_INVALID;
     }
@@ -135,6 +286,10 @@ BF_D int record_pdf(const closure_t& c, uint32_t allowed, v3f wo, v3f wi, float&
         return mf_conductor_pdf(p[8], p[9], wo, wi, &pdf);
     case BF_CLOSURE_ROUGH_DIELECTRIC:
         return mf_dielectric_pdf_masked(c.weight, p[0], p[1], p[2], p[3], wo, wi, BF_TRANSPORT_RADIANCE, allowed, &pdf);
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        pdf = 0.0f;
+        return specular_zero(c, wo, wi);
     default:
         return BF_L_INVALID;
     }
@@ -186,6 +341,12 @@ BF_D void record_sample(const closure_t& c, uint32_t allowed, v3f wo, float even
     case BF_CLOSURE_ROUGH_DIELECTRIC:
         mf_dielectric_sample_masked(c.weight, p[0], p[1], p[2], p[3], wo, event, u, v, BF_TRANSPORT_RADIANCE, allowed,
                                     &out);
+        return;
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+        specular_reflection_sample(c.weight, wo, out);
+        return;
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        specular_transmission_sample(c.weight, p[0], p[1], wo, out);
         return;
     default:
         return;
@@ -461,8 +622,56 @@ BF_D void sample_mixture(const mixture_t& m, const filter_t& f, v3f wo, float co
         for (int k = 0; k < 4; ++k) out.value[k] = value[k];
         out.pdf = pdf;
     } else {
-        out.status = BF_L_NOT_REPRESENTABLE; /* delta atoms: not produced by the supported closures */
-        return;
+        /* aggregate_delta_atom (:2394-2480): every delta closure of the (filtered) mixture that
+         * sends wo to the sampled direction adds its value and its selection-weighted probability */
+        float sums[4] = {0.0f, 0.0f, 0.0f, 0.0f}, corrections[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float psum = 0.0f, pcorrection = 0.0f;
+        uint32_t matching = 0;
+        for (uint32_t i = 0; i < m.active_count; ++i) {
+            const uint32_t kind = m.closures[i].kind;
+            if (kind != BF_CLOSURE_SPECULAR_REFLECTION && kind != BF_CLOSURE_SPECULAR_TRANSMISSION) continue;
+            lobe_sample_t c;
+            record_sample(m.closures[i], BF_LOBE_REFLECTION | BF_LOBE_TRANSMISSION, wo, 0.0f, 0.0f, 0.0f, c);
+            if (c.status == BF_L_OUTSIDE) continue;
+            if (c.status != BF_L_OK) {
+                out.status = c.status;
+                return;
+            }
+            if (c.incoming.x != s.incoming.x || c.incoming.y != s.incoming.y || c.incoming.z != s.incoming.z) continue;
+            if (!c.discrete || !bf_isfinite(c.pdf) || !(c.pdf > 0.0f) || c.pdf > 1.0f) {
+                out.status = BF_L_NOT_REPRESENTABLE;
+                return;
+            }
+            for (int k = 0; k < 4; ++k)
+                if (!compensated_add(c.value[k], sums[k], corrections[k])) {
+                    out.status = BF_L_NOT_REPRESENTABLE;
+                    return;
+                }
+            float weighted;
+            if (!checked_product(m.probabilities[i], c.pdf, weighted)) {
+                out.status = BF_L_NOT_REPRESENTABLE;
+                return;
+            }
+            if (!compensated_add(weighted, psum, pcorrection)) {
+                out.status = BF_L_NOT_REPRESENTABLE;
+                return;
+            }
+            ++matching;
+        }
+        const float probability = psum + pcorrection;
+        if (matching == 0 || !bf_isfinite(probability) || !(probability > 0.0f) || probability > 1.0f) {
+            out.status = BF_L_NOT_REPRESENTABLE;
+            return;
+        }
+        for (int k = 0; k < 4; ++k) {
+            out.value[k] = sums[k] + corrections[k];
+            if (!bf_isfinite(out.value[k]) || out.value[k] < 0.0f) {
+                out.status = BF_L_NOT_REPRESENTABLE;
+                return;
+            }
+        }
+        out.pdf = probability;
+        out.discrete = 1u;
     }
     out.selected = f.source_indices[sel];
     out.status = BF_L_OK;

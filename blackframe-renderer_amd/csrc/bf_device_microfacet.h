@@ -983,6 +983,18 @@ BF_D int mf_valid_closure(const closure_t* c) {
         for (int i = 4; i < 10; ++i)
             if (p[i] != 0.0f) return 0;
         return 1;
+    case BF_CLOSURE_SPECULAR_REFLECTION: /* TransportLobesDevice.cuh:2033-2042 */
+        if (c->lobes != (BF_LOBE_SPECULAR | BF_LOBE_REFLECTION)) return 0;
+        for (int i = 0; i < 10; ++i)
+            if (p[i] != 0.0f) return 0;
+        return 1;
+    case BF_CLOSURE_SPECULAR_TRANSMISSION: /* :2043-2055: exterior and interior eta */
+        if (c->lobes != (BF_LOBE_SPECULAR | BF_LOBE_TRANSMISSION) || !bf_isfinite(p[0]) || !(p[0] > 0.0f) ||
+            !bf_isfinite(p[1]) || !(p[1] > 0.0f))
+            return 0;
+        for (int i = 2; i < 10; ++i)
+            if (p[i] != 0.0f) return 0;
+        return 1;
     default:
         return 0;
     }

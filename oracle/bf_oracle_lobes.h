@@ -257,6 +257,150 @@ static inline void lobe_empty_sample(lobe_sample* s, uint32_t status) {
 
 #include "bf_oracle_microfacet.h"
 
+/* ---- specular delta closures (TransportLobesDevice.cuh:1689-1960) --------------------------- */
+static inline int lobe_specular_zero(const closure_rec* c, v3 wo, v3 wi) {
+    if (c->kind == BF_CLOSURE_SPECULAR_TRANSMISSION &&
+        (!bf_isfinite(c->parameters[0]) || !(c->parameters[0] > 0.0f) || !bf_isfinite(c->parameters[1]) ||
+         !(c->parameters[1] > 0.0f)))
+        return LOBE_INVALID;
+    return lobe_reflectance(c->weight) && lobe_unit_vector(wo) && lobe_unit_vector(wi) ? LOBE_OK : LOBE_INVALID;
+}
+
+/* checked_positive_product_quotient (:1756-1782) */
+static inline int lobe_positive_product_quotient(float left, float right, float denominator, float* out) {
+    *out = 0.0f;
+    if (left == 0.0f) return LOBE_OK;
+    if (!bf_isfinite(left) || !(left > 0.0f) || !bf_isfinite(right) || !(right > 0.0f) || !bf_isfinite(denominator) ||
+        !(denominator > 0.0f))
+        return LOBE_INVALID;
+    int le, re, de, ne;
+    const float ls = bf_frexpf(left, &le), rs = bf_frexpf(right, &re), ds = bf_frexpf(denominator, &de);
+    const float ns = bf_frexpf((ls * rs) / ds, &ne);
+    const float r = bf_ldexpf(ns, le + re - de + ne);
+    if (!bf_isfinite(r) || !(r > 0.0f)) return LOBE_NOT_REPRESENTABLE;
+    *out = r;
+    return LOBE_OK;
+}
+
+/* scaled_delta_value (:1784-1797) */
+static inline int lobe_delta_value(const float* coefficient, float factor, float cosine, float* out) {
+    for (int k = 0; k < 4; ++k) {
+        const int st = lobe_positive_product_quotient(coefficient[k], factor, cosine, &out[k]);
+        if (st != LOBE_OK) return st;
+    }
+    return LOBE_OK;
+}
+
+/* checked_squared_ratio (:1799-1819) */
+static inline int lobe_squared_ratio(float numerator, float denominator, float* out) {
+    *out = 0.0f;
+    if (!bf_isfinite(numerator) || !(numerator > 0.0f) || !bf_isfinite(denominator) || !(denominator > 0.0f))
+        return LOBE_INVALID;
+    int ne, de, re;
+    const float ns = bf_frexpf(numerator, &ne), ds = bf_frexpf(denominator, &de);
+    const float q = ns / ds;
+    const float rs = bf_frexpf(q * q, &re);
+    const float r = bf_ldexpf(rs, 2 * (ne - de) + re);
+    if (!bf_isfinite(r) || !(r > 0.0f)) return LOBE_NOT_REPRESENTABLE;
+    *out = r;
+    return LOBE_OK;
+}
+
+static inline void lobe_delta_sample(lobe_sample* out, v3 wi, const float* value, uint32_t lobes, float eta_scale) {
+    memcpy(out->value, value, 4 * sizeof(float));
+    out->incoming = wi;
+    out->pdf = 1.0f;
+    out->measure = LOBE_MEASURE_DISCRETE;
+    out->lobes = lobes;
+    out->eta_scale = eta_scale;
+    out->status = LOBE_OK;
+}
+
+/* sample_specular_reflection (:1821-1845) */
+static inline void lobe_specular_reflection_sample(const float* w, v3 wo, lobe_sample* out) {
+    lobe_empty_sample(out, LOBE_INVALID);
+    if (!lobe_reflectance(w) || !lobe_unit_vector(wo)) return;
+    if (wo.z == 0.0f) {
+        out->status = LOBE_OUTSIDE_SUPPORT;
+        return;
+    }
+    const v3 wi = v3_make(-wo.x, -wo.y, wo.z);
+    float value[4];
+    const int st = lobe_delta_value(w, 1.0f, fabsf(wi.z), value);
+    if (st != LOBE_OK) {
+        out->status = st;
+        return;
+    }
+    lobe_delta_sample(out, wi, value, BF_LOBE_SPECULAR | BF_LOBE_REFLECTION, 1.0f);
+}
+
+/* sample_specular_transmission (:1847-1958), radiance transport mode */
+static inline void lobe_specular_transmission_sample(const float* w, float exterior, float interior, v3 wo, lobe_sample* out) {
+    lobe_empty_sample(out, LOBE_INVALID);
+    if (!lobe_reflectance(w) || !bf_isfinite(exterior) || !(exterior > 0.0f) || !bf_isfinite(interior) ||
+        !(interior > 0.0f) || !lobe_unit_vector(wo))
+        return;
+    if (wo.z == 0.0f) {
+        out->status = LOBE_OUTSIDE_SUPPORT;
+        return;
+    }
+    const int entering = wo.z > 0.0f;
+    const float eta_i = entering ? exterior : interior, eta_t = entering ? interior : exterior;
+    float value[4];
+    if (eta_i == eta_t) {
+        const v3 wi = v3_make(-wo.x, -wo.y, -wo.z);
+        const int st = lobe_delta_value(w, 1.0f, fabsf(wi.z), value);
+        if (st != LOBE_OK) {
+            out->status = st;
+            return;
+        }
+        lobe_delta_sample(out, wi, value, BF_LOBE_SPECULAR | BF_LOBE_TRANSMISSION, 1.0f);
+        return;
+    }
+    out->status = LOBE_NOT_REPRESENTABLE;
+    const float raw_sine = bf_hypotf(wo.x, wo.y);
+    if (!bf_isfinite(raw_sine)) return;
+    float sine = raw_sine, tangent_normalization = 1.0f;
+    if (raw_sine > 1.0f) {
+        const float cosine = fabsf(wo.z);
+        if (cosine > 1.0f) return;
+        sine = sqrtf((1.0f - cosine) * (1.0f + cosine));
+        tangent_normalization = sine / raw_sine;
+        if (!bf_isfinite(sine) || !bf_isfinite(tangent_normalization) || !(tangent_normalization > 0.0f)) return;
+    }
+    if (sine > 0.0f) {
+        const float critical = eta_t / eta_i;
+        if (critical <= 1.0f && sine >= critical) {
+            out->status = LOBE_OUTSIDE_SUPPORT;
+            return;
+        }
+    }
+    const float ratio = eta_i / eta_t;
+    if (!bf_isfinite(ratio) || !(ratio > 0.0f)) return;
+    const float ix = -ratio * tangent_normalization * wo.x;
+    const float iy = -ratio * tangent_normalization * wo.y;
+    if (!bf_isfinite(ix) || !bf_isfinite(iy) || (wo.x != 0.0f && ix == 0.0f) || (wo.y != 0.0f && iy == 0.0f)) return;
+    const float t_sine = bf_hypotf(ix, iy);
+    if (!bf_isfinite(t_sine) || !(t_sine < 1.0f)) return;
+    const float t_cosine = sqrtf((1.0f - t_sine) * (1.0f + t_sine));
+    if (!bf_isfinite(t_cosine) || !(t_cosine > 0.0f)) return;
+    const v3 wi = v3_make(ix, iy, entering ? -t_cosine : t_cosine);
+    if (!lobe_unit_vector(wi)) return;
+    float factor, eta_scale;
+    int st = lobe_squared_ratio(eta_i, eta_t, &factor);
+    const int st2 = lobe_squared_ratio(eta_t, eta_i, &eta_scale);
+    if (st != LOBE_OK || st2 != LOBE_OK) {
+        out->status = st != LOBE_OK ? st : st2;
+        return;
+    }
+    st = lobe_delta_value(w, factor, t_cosine, value);
+    if (st != LOBE_OK) {
+        out->status = st;
+        return;
+    }
+    lobe_delta_sample(out, wi, value, BF_LOBE_SPECULAR | BF_LOBE_TRANSMISSION, eta_scale);
+}
+
 /* ---- closure record dispatch (TransportLobesDevice.cuh:2092-2260); transport mode radiance --- */
 static inline int lobe_record_eval(const closure_rec* c, uint32_t allowed, v3 wo, v3 wi, float* out) {
     if (!mf_valid_closure(c)) return LOBE_INVALID;
@@ -270,6 +414,10 @@ static inline int lobe_record_eval(const closure_rec* c, uint32_t allowed, v3 wo
         return mf_conductor_eval(c->weight, p, p + 4, p[8], p[9], wo, wi, out);
     case BF_CLOSURE_ROUGH_DIELECTRIC:
         return mf_dielectric_eval_masked(c->weight, p[0], p[1], p[2], p[3], wo, wi, BFO_TRANSPORT_RADIANCE, allowed, out);
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        out[0] = out[1] = out[2] = out[3] = 0.0f;
+        return lobe_specular_zero(c, wo, wi);
     default:
         return LOBE_INVALID;
     }
@@ -286,6 +434,10 @@ static inline int lobe_record_pdf(const closure_rec* c, uint32_t allowed, v3 wo,
         return mf_conductor_pdf(p[8], p[9], wo, wi, pdf);
     case BF_CLOSURE_ROUGH_DIELECTRIC:
         return mf_dielectric_pdf_masked(c->weight, p[0], p[1], p[2], p[3], wo, wi, BFO_TRANSPORT_RADIANCE, allowed, pdf);
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        *pdf = 0.0f;
+        return lobe_specular_zero(c, wo, wi);
     default:
         return LOBE_INVALID;
     }
@@ -337,6 +489,12 @@ static inline void lobe_record_sample(const closure_rec* c, uint32_t allowed, v3
         return;
     case BF_CLOSURE_ROUGH_DIELECTRIC:
         mf_dielectric_sample_masked(c->weight, p[0], p[1], p[2], p[3], wo, event, u, v, BFO_TRANSPORT_RADIANCE, allowed, out);
+        return;
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+        lobe_specular_reflection_sample(c->weight, wo, out);
+        return;
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        lobe_specular_transmission_sample(c->weight, p[0], p[1], wo, out);
         return;
     default:
         return;
@@ -597,8 +755,52 @@ static inline void lobe_sample_filtered_mixture(const lobe_mixture* m, const lob
         out->pdf = pdf;
         out->measure = LOBE_MEASURE_SOLID_ANGLE;
     } else {
-        out->status = LOBE_NOT_REPRESENTABLE; /* delta atoms: not exercised by the supported closures */
-        return;
+        /* aggregate_delta_atom (:2394-2480) over the filtered mixture */
+        float sums[4] = {0.0f, 0.0f, 0.0f, 0.0f}, corrections[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        float psum = 0.0f, pcorrection = 0.0f;
+        uint32_t matching = 0;
+        for (uint32_t i = 0; i < m->active_count; ++i) {
+            const uint32_t kind = m->closures[i].kind;
+            if (kind != BF_CLOSURE_SPECULAR_REFLECTION && kind != BF_CLOSURE_SPECULAR_TRANSMISSION) continue;
+            lobe_sample c;
+            lobe_record_sample(&m->closures[i], BF_LOBE_REFLECTION | BF_LOBE_TRANSMISSION, wo, 0.0f, 0.0f, 0.0f, &c);
+            if (c.status == LOBE_OUTSIDE_SUPPORT) continue;
+            if (c.status != LOBE_OK) {
+                out->status = c.status;
+                return;
+            }
+            if (c.incoming.x != s.incoming.x || c.incoming.y != s.incoming.y || c.incoming.z != s.incoming.z) continue;
+            float weighted;
+            if (c.measure != LOBE_MEASURE_DISCRETE || !bf_isfinite(c.pdf) || !(c.pdf > 0.0f) || c.pdf > 1.0f) {
+                out->status = LOBE_NOT_REPRESENTABLE;
+                return;
+            }
+            for (int k = 0; k < 4; ++k)
+                if (!lobe_compensated_add(c.value[k], &sums[k], &corrections[k])) {
+                    out->status = LOBE_NOT_REPRESENTABLE;
+                    return;
+                }
+            if (!lobe_checked_product(m->probabilities[i], c.pdf, &weighted) ||
+                !lobe_compensated_add(weighted, &psum, &pcorrection)) {
+                out->status = LOBE_NOT_REPRESENTABLE;
+                return;
+            }
+            ++matching;
+        }
+        const float probability = psum + pcorrection;
+        if (matching == 0 || !bf_isfinite(probability) || !(probability > 0.0f) || probability > 1.0f) {
+            out->status = LOBE_NOT_REPRESENTABLE;
+            return;
+        }
+        for (int k = 0; k < 4; ++k) {
+            out->value[k] = sums[k] + corrections[k];
+            if (!bf_isfinite(out->value[k]) || out->value[k] < 0.0f) {
+                out->status = LOBE_NOT_REPRESENTABLE;
+                return;
+            }
+        }
+        out->pdf = probability;
+        out->measure = LOBE_MEASURE_DISCRETE;
     }
     out->incoming = s.incoming;
     out->selected = f->source_indices[selected];
