@@ -630,8 +630,14 @@ BF_D void sample_mixture(const mixture_t& m, const filter_t& f, v3f wo, float co
         for (uint32_t i = 0; i < m.active_count; ++i) {
             const uint32_t kind = m.closures[i].kind;
             if (kind != BF_CLOSURE_SPECULAR_REFLECTION && kind != BF_CLOSURE_SPECULAR_TRANSMISSION) continue;
+            /* record_sample restricted to the two delta kinds: the microfacet samplers are not
+             * inlined a second time here */
             lobe_sample_t c;
-            record_sample(m.closures[i], BF_LOBE_REFLECTION | BF_LOBE_TRANSMISSION, wo, 0.0f, 0.0f, 0.0f, c);
+            empty_sample(c, BF_L_INVALID);
+            if (mf_valid_closure(&m.closures[i])) {
+                if (kind == BF_CLOSURE_SPECULAR_REFLECTION) specular_reflection_sample(m.closures[i].weight, wo, c);
+                else specular_transmission_sample(m.closures[i].weight, m.closures[i].parameters[0], m.closures[i].parameters[1], wo, c);
+            }
             if (c.status == BF_L_OUTSIDE) continue;
             if (c.status != BF_L_OK) {
                 out.status = c.status;
