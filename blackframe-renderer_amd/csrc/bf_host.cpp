@@ -1681,6 +1681,16 @@ int blackframe_hip_launch_scene_soa_hash(const uint8_t* scene_bytes, size_t scen
     return BF_OK;
 }
 
+int blackframe_hip_launch_scene_soa_validate(const uint8_t* scene_bytes, size_t scene_byte_count, uint32_t* output_status) {
+    if (!scene_bytes || !output_status || scene_byte_count < BF_SCENE_SOA_HEADER_SIZE)
+        return set_error(BF_INVALID_ARGUMENT, "scene validation needs a blob of at least %u bytes and an output word",
+                         BF_SCENE_SOA_HEADER_SIZE);
+    if ((reinterpret_cast<uintptr_t>(scene_bytes) & 15u) != 0)
+        return set_error(BF_INVALID_ARGUMENT, "the device blob must be 16-byte aligned, as its header is");
+    HIP_TRY(bfk_scene_soa_validate(scene_bytes, scene_byte_count, output_status, nullptr));
+    return BF_OK;
+}
+
 int blackframe_hip_query_transport_abi_layout(bf_layout_manifest* output, int* device_count) {
     if (!output || !device_count) return set_error(BF_INVALID_ARGUMENT, "NULL manifest or device count");
     std::memset(output, 0, sizeof *output);

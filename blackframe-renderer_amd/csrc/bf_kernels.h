@@ -135,6 +135,7 @@ hipError_t bfk_layout_manifest(bf_layout_manifest* out, hipStream_t s);
 hipError_t bfk_queue_pushes(const bf_queue_soa* queues, const bf_queue_push* requests, uint32_t n, uint32_t* outcomes,
                             hipStream_t s);
 hipError_t bfk_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out, hipStream_t s);
+hipError_t bfk_scene_soa_validate(const uint8_t* bytes, size_t count, uint32_t* status, hipStream_t s);
 size_t bfk_queue_compaction_scratch(uint32_t n);
 hipError_t bfk_queue_compaction(const bf_queue_soa* queues, const bf_stage_outcome* outcomes, uint32_t n, uint32_t route,
                                 void* scratch, bf_compaction_result* result, hipStream_t s);

@@ -15,6 +15,7 @@
 #include "../../include/blackframe_texture.h"
 #include "bf_device_adjoint.h"
 #include "bf_kernels.h"
+#include "bf_scene_soa.h"
 #include "bf_shading.h"
 #include "bf_traversal.h"
 
@@ -1848,6 +1849,14 @@ __global__ void k_zero_doubles(double* dst, uint32_t count) {
 /* ---- scene blob hash: FNV-1a-64 with the content-hash field read as zero ---------------------- */
 /* The byte chain is sequential by definition, so one lane walks it; it reads 16-byte words where
  * the blob is 16-byte aligned and single bytes otherwise. */
+/* validate_scene_soa_header on the device copy of a blob (the check every reference stage makes
+ * of the uploaded header, CudaWavefrontStages.cu:818-880), one thread; plus the byte count. */
+__global__ void k_scene_soa_validate(const bf_scene_soa_header* header, uint64_t byte_count, uint32_t* status) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t st = bf_soa::validate_header(*header);
+    *status = st != BF_SCENE_SOA_VALID ? st : (header->total_size_bytes > byte_count ? (uint32_t)BF_SCENE_SOA_TRUNCATED : st);
+}
+
 __global__ void k_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out) {
     if (blockIdx.x != 0u || threadIdx.x != 0u) return;
     uint64_t h = BF_FNV1A64_OFFSET_BASIS;
@@ -2430,6 +2439,11 @@ hipError_t bfk_film_l2(const float* film, const uint32_t* counts, const float* t
 hipError_t bfk_zero_doubles(double* dst, uint32_t count, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_doubles, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t bfk_scene_soa_validate(const uint8_t* bytes, size_t count, uint32_t* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_scene_soa_validate, dim3(1), dim3(64), 0, s, (const bf_scene_soa_header*)bytes, (uint64_t)count, status);
     return hipGetLastError();
 }
 

@@ -67,6 +67,7 @@ def library():
     lib.blackframe_hip_query_transport_abi_layout.argtypes = [C.POINTER(abi.LayoutManifest), C.POINTER(C.c_int)]
     lib.blackframe_hip_launch_scene_soa_hash.argtypes = [vp, C.c_size_t, vp]
     lib.blackframe_hip_scene_soa_validate.argtypes = [vp, C.c_size_t, u32, C.POINTER(C.c_uint32)]
+    lib.blackframe_hip_launch_scene_soa_validate.argtypes = [vp, C.c_size_t, vp]
     lib.blackframe_hip_scene_create_from_soa.argtypes = [vp, C.c_size_t, u32, C.POINTER(vp)]
     lib.blackframe_hip_launch_wavefront_queue_pushes.argtypes = [vp, vp, u32, u32, vp, u32, vp]
     lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
@@ -392,6 +393,19 @@ def scene_soa_validate(blob, verify_hash=True):
     _check(library().blackframe_hip_scene_soa_validate(data, len(data), abi.SCENE_SOA_VERIFY_HASH if verify_hash else 0,
                                                        C.byref(status)))
     return status.value
+
+
+def scene_soa_validate_on_device(blob):
+    """blackframe_hip_launch_scene_soa_validate over host bytes copied to the device."""
+    import torch
+
+    data = bytes(blob)
+    d = torch.frombuffer(bytearray(data), dtype=torch.uint8).cuda()
+    status = torch.full((1,), 0xFFFF, dtype=torch.int32, device="cuda")
+    _check(library().blackframe_hip_launch_scene_soa_validate(C.c_void_p(d.data_ptr()), len(data),
+                                                              C.c_void_p(status.data_ptr())))
+    torch.cuda.synchronize()
+    return int(status.item())
 
 
 def transport_abi_layout():

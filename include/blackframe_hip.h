@@ -199,6 +199,16 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_j
 int blackframe_hip_launch_scene_soa_hash(const uint8_t* scene_bytes, size_t scene_byte_count, uint64_t* output_hash);
 
 /*
+ * validate_scene_soa_header run by the device on a device copy of a blob (16-byte aligned), then
+ * BF_SCENE_SOA_TRUNCATED when total_size_bytes exceeds `scene_byte_count`: the bf_scene_soa_status
+ * is written to the device word `output_status` on the default stream. Rejects a blob shorter than
+ * its header, a misaligned blob or null pointers before any launch.
+ * Replaces: the header check each reference stage makes of the uploaded blob
+ *           (CudaWavefrontStages.cu:818-880, 3966-3971; validate_scene_soa_header, SceneSoaAbi.hpp:283-341).
+ */
+int blackframe_hip_launch_scene_soa_validate(const uint8_t* scene_bytes, size_t scene_byte_count, uint32_t* output_status);
+
+/*
  * Checks a host scene blob: validate_scene_soa_header's checks in its order, then (no reference
  * counterpart) that `byte_count` covers total_size_bytes and, with BF_SCENE_SOA_VERIFY_HASH, that
  * content_hash is the FNV-1a-64 of the bytes with the hash field read as zero. *status receives a

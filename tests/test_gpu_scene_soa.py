@@ -76,3 +76,15 @@ def test_blob_scene_renders_only_at_its_packet(bf, hip, abi):
             hip.trace_xpu_wavefront_transport(ws, scene, frame, options)
         assert e.value.status == abi.INVALID_ARGUMENT
     hip.trace_xpu_wavefront_transport(ws, scene, frame, packet_options(bf, wl))
+
+
+def test_device_header_validation_matches_the_oracle(hip, abi):
+    """The device-side header check returns the status the oracle's restatement of
+    validate_scene_soa_header gives, for the reference's rejection cases and the rich blob."""
+    from test_scene_soa import malformed_headers
+    for header, expected in malformed_headers():
+        assert soa.validate_header(header) == expected
+        assert hip.scene_soa_validate_on_device(soa.pack_header(header)) == expected
+    blob = rich_blob()
+    assert hip.scene_soa_validate_on_device(blob) == abi.SCENE_SOA_VALID
+    assert hip.scene_soa_validate_on_device(blob[:-16]) == abi.SCENE_SOA_TRUNCATED
