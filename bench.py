@@ -82,6 +82,22 @@ def algorithmic_bytes(report, tc, path_count):
     return closest, shadow, frame
 
 
+def host_cpu():
+    """The host CPU the baseline ran on: model name, logical CPUs of the machine, and the CPUs this
+    process may use (the GPU box gives one GPU's job a share of 16; the worker count is that share,
+    the reference scheduler's hardware_concurrency() of the job)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "machine_logical_cpus": os.cpu_count(), "job_cpus": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(bf, scene, frame, options, target_seconds, threads):
     """Times the oracle's threaded CPU wavefront on a contiguous range of path slots starting at the
     middle pixel row of the first sample pass (no film), growing the range until the run lasts about
@@ -112,6 +128,7 @@ def cpu_baseline(bf, scene, frame, options, target_seconds, threads):
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
+        "host": host_cpu(),
         "sample": (f"path slots {begin}..{begin + count - 1} of the frame (from pixel row {cam.height // 2} of pass 0 "
                    f"on; {count} paths, {rays} rays) in {elapsed:.1f} s; paths/s {count / elapsed:.0f}; "
                    f"oracle BVH build {build_s:.1f} s"),

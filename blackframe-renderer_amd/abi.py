@@ -17,6 +17,7 @@ SCENE_SOA_VALID, SCENE_SOA_BAD_MAGIC, SCENE_SOA_INCOMPATIBLE_VERSION, SCENE_SOA_
 SCENE_SOA_BAD_COLUMN_COUNT, SCENE_SOA_UNSUPPORTED_HASH_ALGORITHM, SCENE_SOA_NONZERO_RESERVED = 4, 5, 6
 SCENE_SOA_INVALID_ENVIRONMENT_COUNT, SCENE_SOA_INVALID_COLUMN_DESCRIPTOR, SCENE_SOA_SIZE_OVERFLOW = 7, 8, 9
 SCENE_SOA_BAD_TOTAL_SIZE, SCENE_SOA_TRUNCATED, SCENE_SOA_HASH_MISMATCH = 10, 11, 12
+SCENE_SOA_SIZE_MISMATCH, SCENE_SOA_COUNT_OVERFLOW, SCENE_SOA_TOO_MANY_MESH_LIGHTS = 13, 14, 15
 SCENE_SOA_VERIFY_HASH = 0x1
 CLOSURE_LAMBERTIAN, CLOSURE_ROUGH_DIFFUSE, CLOSURE_ROUGH_CONDUCTOR, CLOSURE_ROUGH_DIELECTRIC = 1, 2, 3, 4
 CLOSURE_SPECULAR_REFLECTION, CLOSURE_SPECULAR_TRANSMISSION = 5, 6
@@ -216,7 +217,11 @@ class FrameJob(C.Structure):
     """One frame of a batched call (bf_frame_job): scene handle, frame, device film and path buffers."""
     _fields_ = [("scene", C.c_void_p), ("frame", FrameDesc), ("film", C.c_void_p), ("film_compensated", C.c_void_p),
                 ("film_counts", C.c_void_p), ("paths", C.c_void_p), ("tape_radiance", C.c_void_p),
-                ("tape_visibility", C.c_void_p)]
+                ("tape_visibility", C.c_void_p), ("tape_survival", C.c_void_p), ("survival_mode", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+SURVIVAL_NONE, SURVIVAL_RECORD, SURVIVAL_REPLAY = 0, 1, 2
 
 
 MAX_TEXTURES = 8
@@ -231,6 +236,40 @@ class LaunchTuning(C.Structure):
 class AdjointJob(C.Structure):
     """bf_adjoint_job: dLoss/d(film RGB sums) in, accumulated texel gradients out (device pointers)."""
     _fields_ = [("film_gradient", C.c_void_p), ("texture_gradients", C.c_void_p * MAX_TEXTURES)]
+
+
+SAMPLE_DIMENSION_SCHEMA = 1
+SAMPLE_MAXIMUM_BOUNCE = 0xFFFFFFFF
+SAMPLE_DUMP_SUCCESS, SAMPLE_DUMP_BOUNCE_OUT_OF_RANGE = 0, 1
+
+
+class SamplePrimaryDimensions(C.Structure):
+    """SampleStreamPrimaryDimensions (SampleStreamKernel.hpp:15-22)."""
+    _fields_ = [(n, C.c_uint64) for n in ("camera_raster_x", "camera_raster_y", "lens_u", "lens_v", "time", "wavelength")]
+
+
+class SampleBounceDimensions(C.Structure):
+    """SampleStreamBounceDimensions (SampleStreamKernel.hpp:24-35)."""
+    _fields_ = [(n, C.c_uint64) for n in ("light_selection", "light_u", "light_v", "bsdf_component", "bsdf_u", "bsdf_v",
+                                          "medium_distance", "medium_phase_u", "medium_phase_v", "russian_roulette")]
+
+
+class SampleDimensionManifest(C.Structure):
+    """SampleStreamDimensionManifest (SampleStreamKernel.hpp:37-47)."""
+    _fields_ = [("schema_version", C.c_uint32), ("reserved", C.c_uint32), ("primary_dimensions", SamplePrimaryDimensions),
+                ("first_bounce_dimension", C.c_uint64), ("dimensions_per_bounce", C.c_uint64),
+                ("maximum_bounce_index", C.c_uint64), ("bounce_offsets", SampleBounceDimensions)]
+
+
+class SampleDumpRequest(C.Structure):
+    """SampleStreamDumpRequest (SampleStreamKernel.hpp:49-53)."""
+    _fields_ = [("sample_stream", SampleStreamIndex), ("dimension", C.c_uint64), ("bounce_index", C.c_uint64)]
+
+
+class SampleDumpResult(C.Structure):
+    """SampleStreamDumpResult (SampleStreamKernel.hpp:55-62)."""
+    _fields_ = [("status", C.c_uint32), ("reserved", C.c_uint32), ("indexed_bits", C.c_uint64), ("sample", C.c_float),
+                ("reserved_tail", C.c_uint32), ("bounce_dimensions", SampleBounceDimensions)]
 
 
 QUEUE_COUNT = 7
@@ -291,7 +330,8 @@ for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitR
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 160), (PunctualDesc, 56), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
                        (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48), (LayoutManifest, 320),
-                       (TransportOptions, 80), (FrameDesc, 112), (FrameJob, 168), (TransportReport, 1808), (StageMetric, 24)):
+                       (TransportOptions, 80), (FrameDesc, 112), (SampleDimensionManifest, 160), (SampleDumpRequest, 40),
+                       (SampleDumpResult, 104), (FrameJob, 184), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 
 

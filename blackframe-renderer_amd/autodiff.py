@@ -20,9 +20,11 @@ from . import hip
 
 
 class DifferentiableFrame:
-    def __init__(self, scene, frame, options, textures, workspace=None):
+    def __init__(self, scene, frame, options, textures, workspace=None, record_survival=False):
         """scene: an XpuScene whose texture slots k were created from textures[k] (device=(ptr, ...));
-        textures: the float32 CUDA tensors bound to those slots, in slot order."""
+        textures: the float32 CUDA tensors bound to those slots, in slot order. record_survival:
+        render() also records every Russian-roulette survival probability, so that render_frozen()
+        can replay the roulette with moved texels (finite-difference checks of the adjoint)."""
         self.scene, self.frame, self.options = scene, frame, options
         self.textures = list(textures)
         cam = frame.camera
@@ -31,6 +33,7 @@ class DifferentiableFrame:
         self.workspace = workspace if workspace is not None else hip.XpuWavefrontTransportWorkspace(self.path_count)
         self.film = hip.Film(cam.width, cam.height)
         self.tape = hip.Tape(self.path_count)
+        self.survival = hip.Survival(self.path_count, options) if record_survival else None
         self.last_forward_report = None
         self.last_adjoint_report = None
 
@@ -39,10 +42,24 @@ class DifferentiableFrame:
         for t in (self.film.float_sum, self.film.compensated, self.film.counts):
             t.zero_()
         r = hip.trace_xpu_wavefront_transport(self.workspace, self.scene, self.frame, self.options, film=self.film,
-                                              tape=self.tape, synchronize=False)
+                                              tape=self.tape, survival=self.survival, synchronize=False)
         self.last_forward_report = r["report"]
         counts = self.film.counts.to(torch.float32).clamp_min(1.0).view(self.height, self.width, 1)
         return self.film.float_sum.view(self.height, self.width, 3) / counts
+
+    def render_frozen(self):
+        """Forward render (no tape) whose roulette decides with the survival probabilities the last
+        render() recorded, so that only the texels' values differ from that render; returns the
+        binary64 (height, width, 3) mean of the compensated film. The adjoint differentiates exactly
+        this function of the texels."""
+        if self.survival is None:
+            raise ValueError("render_frozen() needs a frame created with record_survival=True")
+        film = hip.Film(self.width, self.height)
+        hip.trace_xpu_wavefront_transport(self.workspace, self.scene, self.frame, self.options, film=film,
+                                          survival=self.survival.replaying(), synchronize=False)
+        compensated = film.compensated.view(self.height, self.width, 3, 2)
+        counts = film.counts.to(torch.float64).clamp_min(1.0).view(self.height, self.width, 1)
+        return (compensated[..., 0] + compensated[..., 1]) / counts
 
     def gradients(self, image_gradient):
         """Texel gradients of sum(image_gradient * image) for the last render."""
@@ -56,6 +73,8 @@ class DifferentiableFrame:
     def image(self, *textures):
         """The film mean as a differentiable function of the bound textures (pass them in slot
         order; they must be the tensors the scene reads)."""
+        if len(textures) != len(self.textures):
+            raise ValueError(f"image() takes all {len(self.textures)} bound textures, got {len(textures)}")
         for given, bound in zip(textures, self.textures):
             if given.data_ptr() != bound.data_ptr():
                 raise ValueError("image() takes the textures bound to the scene, in slot order")

@@ -112,4 +112,22 @@ BF_D int closure_weight_derivative(const closure_t& c, uint32_t allowed, v3f wo,
     return BF_L_OK;
 }
 
+/* d f / d w per lane of a specular (delta) closure along the delta direction `wi`: its atom's value
+ * at unit weight, which the atom scales linearly (scaled_delta_value, TransportLobesDevice.cuh:
+ * 1784-1797). False when the closure does not send wo to wi, i.e. when it is not part of the
+ * sampled delta atom: then the continuation's throughput does not depend on its weight. */
+BF_D bool delta_weight_derivative(const closure_t& c, v3f wo, v3f wi, float* out) {
+    if (c.kind != BF_CLOSURE_SPECULAR_REFLECTION && c.kind != BF_CLOSURE_SPECULAR_TRANSMISSION) return false;
+    const float unit[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    lobe_sample_t s;
+    empty_sample(s, BF_L_INVALID);
+    if (c.kind == BF_CLOSURE_SPECULAR_REFLECTION) specular_reflection_sample(unit, wo, s);
+    else specular_transmission_sample(unit, c.parameters[0], c.parameters[1], wo, s);
+    if (s.status != BF_L_OK || !s.discrete || s.incoming.x != wi.x || s.incoming.y != wi.y || s.incoming.z != wi.z)
+        return false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = s.value[k];
+    return true;
+}
+
 #endif

@@ -226,23 +226,14 @@ BF_D bool mis_weight(uint32_t heuristic, float sampled, float competing, float& 
 }
 
 /* TransportDevice.cuh:711-782 */
-BF_D bool russian_roulette(float4& t, float eta, float u, float minimum, float maximum, bool& survived) {
-    if (!bf_isfinite(eta) || !(eta > 0.0f) || !bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return false;
+/* The roulette decision with survival probability p: u < p survives and divides the throughput
+ * by p. */
+BF_D bool russian_roulette_at(float4& t, float p, float u, bool& survived) {
+    if (!bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return false;
     const float v[4] = {t.x, t.y, t.z, t.w};
-    float m = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 4; ++k)
         if (!bf_isfinite(v[k]) || v[k] < 0.0f) return false;
-        m = fmaxf(m, v[k]);
-    }
-    if (m == 0.0f) return false;
-    float p;
-    if (m >= maximum / eta) p = maximum;
-    else if (m <= minimum / eta) p = minimum;
-    else {
-        const float c = m * eta;
-        p = c < minimum ? minimum : (maximum < c ? maximum : c);
-    }
     if (!bf_isfinite(p) || !(p > 0.0f) || p > 1.0f) return false;
     if (!(u < p)) {
         survived = false;
@@ -257,6 +248,26 @@ BF_D bool russian_roulette(float4& t, float eta, float u, float minimum, float m
     t = make_float4(o[0], o[1], o[2], o[3]);
     survived = true;
     return true;
+}
+
+/* The survival probability clamp(max(throughput) * eta, minimum, maximum), then the decision. */
+BF_D bool russian_roulette(float4& t, float eta, float u, float minimum, float maximum, bool& survived, float& p) {
+    if (!bf_isfinite(eta) || !(eta > 0.0f) || !bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return false;
+    const float v[4] = {t.x, t.y, t.z, t.w};
+    float m = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!bf_isfinite(v[k]) || v[k] < 0.0f) return false;
+        m = fmaxf(m, v[k]);
+    }
+    if (m == 0.0f) return false;
+    if (m >= maximum / eta) p = maximum;
+    else if (m <= minimum / eta) p = minimum;
+    else {
+        const float c = m * eta;
+        p = c < minimum ? minimum : (maximum < c ? maximum : c);
+    }
+    return russian_roulette_at(t, p, u, survived);
 }
 
 /* TransportDevice.cuh:327-333 */

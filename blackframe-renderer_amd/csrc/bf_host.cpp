@@ -1033,8 +1033,19 @@ int blackframe_hip_launch_scene_occlusion(const bf_hip_scene* s, const bf_transp
     return BF_OK;
 }
 
-int blackframe_hip_launch_sample_stream_dump(uint32_t sampler, const bf_sample_stream_index* indices,
-                                             const uint64_t* dimensions, uint32_t count, float* samples, void* stream) {
+int blackframe_hip_launch_sample_stream_dump(uint32_t requested_schema_version, const bf_sample_dump_request* requests,
+                                             uint32_t request_count, bf_sample_dimension_manifest* manifest,
+                                             bf_sample_dump_result* results) {
+    /* CudaSampleStream.cu:52-56: every rejection before any launch */
+    if (requested_schema_version != BF_SAMPLE_DIMENSION_SCHEMA || !requests || request_count == 0u || !manifest || !results)
+        return set_error(BF_INVALID_ARGUMENT, "sample-stream dump: schema %u, %u requests, or a null pointer",
+                         requested_schema_version, request_count);
+    HIP_TRY(bfk_sample_stream_dump(requests, request_count, manifest, results));
+    return BF_OK;
+}
+
+int blackframe_hip_launch_sample_dump(uint32_t sampler, const bf_sample_stream_index* indices, const uint64_t* dimensions,
+                                      uint32_t count, float* samples, void* stream) {
     if (sampler > BF_SAMPLER_SOBOL_OWEN) return set_error(BF_INVALID_ARGUMENT, "unknown sampler %u", sampler);
     if (count != 0 && (!indices || !dimensions || !samples)) return set_error(BF_INVALID_ARGUMENT, "NULL argument");
     const uint64_t* directions = nullptr;
@@ -1245,6 +1256,11 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
             return set_error(BF_INVALID_ARGUMENT, "job %u: a tape needs both its radiance and visibility buffers", j);
         if ((J.tape_radiance || adjoint) && vertices > 64u)
             return set_error(BF_INVALID_ARGUMENT, "taped frames need diffuse + glossy + specular depth limits <= 63");
+        if (J.reserved != 0u) return set_error(BF_INVALID_ARGUMENT, "job %u: reserved word must be zero", j);
+        if (J.survival_mode > BF_SURVIVAL_REPLAY) return set_error(BF_INVALID_ARGUMENT, "job %u: unknown survival mode", j);
+        if (J.survival_mode != BF_SURVIVAL_NONE && (!J.tape_survival || vertices > 64u))
+            return set_error(BF_INVALID_ARGUMENT,
+                             "job %u: a survival mode needs its buffer and diffuse + glossy + specular depth limits <= 63", j);
         if (adjoint) {
             if (!J.tape_radiance) return set_error(BF_INVALID_ARGUMENT, "job %u: the adjoint needs the frame's tape", j);
             if (!adjoints[j].film_gradient) return set_error(BF_INVALID_ARGUMENT, "job %u: no film gradient", j);
@@ -1386,6 +1402,11 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
             D.paths = J.paths;
             D.tape_radiance = (float4*)J.tape_radiance;
             D.tape_visibility = (unsigned long long*)J.tape_visibility;
+            if (J.survival_mode != BF_SURVIVAL_NONE) {
+                D.tape_survival = J.tape_survival;
+                D.survival_stride = (uint32_t)vertices;
+                D.survival_mode = J.survival_mode;
+            }
             if (adjoint) {
                 const bf_adjoint_job& A = adjoints[chunk[c].job];
                 D.film = nullptr;
@@ -1595,6 +1616,7 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* w, const bf_frame_job* jobs
         J.paths = nullptr;
         J.tape_radiance = nullptr;
         J.tape_visibility = nullptr;
+        if (J.survival_mode == BF_SURVIVAL_RECORD) J.survival_mode = BF_SURVIVAL_NONE; /* trials leave it untouched */
     }
     void* scratch = nullptr;
     HIP_TRY(hipMalloc(&scratch, film_doubles * 8 + film_floats * 4 + film_words * 4));

@@ -37,6 +37,9 @@ typedef struct bf_dev_job {
     /* offset of texture k's gradient in each replica row, or BF_INVALID_INDEX: accumulate into
      * texture_gradients[k] directly */
     uint32_t gradient_replica[BF_MAX_TEXTURES];
+    float* tape_survival;       /* roulette survival probabilities [path * survival_stride + vertex] */
+    uint32_t survival_stride;
+    uint32_t survival_mode;     /* bf_survival_mode */
 } bf_dev_job;
 
 /* Gradients of small textures (<= BF_GRADIENT_SMALL floats, e.g. 1x1 per-material scalars) are
@@ -114,6 +117,8 @@ hipError_t bfk_scene_occlusion(const bf_dev_scene* S, const bf_transport_ray* ra
                                bf_occlusion_result* results, hipStream_t s);
 hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t dims, const bf_sample_stream_index* indices,
                            const uint64_t* dimensions, uint32_t count, float* out, hipStream_t s);
+hipError_t bfk_sample_stream_dump(const bf_sample_dump_request* requests, uint32_t count,
+                                  bf_sample_dimension_manifest* manifest, bf_sample_dump_result* results);
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s);
 hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s);
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);

@@ -847,8 +847,8 @@ BF_D void texel_add(const bf_transport_params& P, const bf_dev_job& J, texel_pen
 
 BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, const bf_dev_scene& S, const bf_material_desc& M, const mixture_t& mix,
                           const filter_t& filtered, const float* uv, const float* wl, const float* a, bool light, v3f wo,
-                          v3f wi_light, const float* f_light, const float* N, bool cont, v3f wi_cont, const float* f_cont,
-                          const float* R) {
+                          v3f wi_light, const float* f_light, const float* N, bool cont, bool cont_discrete, v3f wi_cont,
+                          const float* f_cont, const float* R) {
     texel_pending weights{BF_INVALID_INDEX, 0u, {}, {0.0f, 0.0f, 0.0f}};
     texel_pending widths{BF_INVALID_INDEX, 0u, {}, {0.0f, 0.0f, 0.0f}};
     for (uint32_t i = 0; i < mix.active_count; ++i) {
@@ -866,7 +866,10 @@ BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, con
         if (wgrad) {
             float dl[4] = {0.0f, 0.0f, 0.0f, 0.0f}, ds[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             const bool lw = light && closure_weight_derivative(c, filtered.allowed[i], wo, wi_light, dl) == BF_L_OK;
-            const bool cw = cont && closure_weight_derivative(c, filtered.allowed[i], wo, wi_cont, ds) == BF_L_OK;
+            /* a delta continuation carries the sum of the matching delta atoms: only their weights
+             * move it, and each by its unit-weight atom value */
+            const bool cw = cont && (cont_discrete ? delta_weight_derivative(c, wo, wi_cont, ds)
+                                                   : closure_weight_derivative(c, filtered.allowed[i], wo, wi_cont, ds) == BF_L_OK);
             const uint32_t slot = D.weight_texture - 1u;
             const uint32_t channels = S.textures[slot].channels;
             float t[3];
@@ -893,7 +896,7 @@ BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, con
         if (rgrad && (c.kind == BF_CLOSURE_ROUGH_CONDUCTOR || c.kind == BF_CLOSURE_ROUGH_DIELECTRIC)) {
             alpha_gradient ql{0.0f, 0.0f}, qs{0.0f, 0.0f};
             const bool use_l = light && closure_alpha_log_gradient(c, wo, wi_light, ql);
-            const bool use_s = cont && closure_alpha_log_gradient(c, wo, wi_cont, qs);
+            const bool use_s = cont && !cont_discrete && closure_alpha_log_gradient(c, wo, wi_cont, qs);
             /* the closure's own value at each direction: the mixture value when it is alone */
             float fl[4], fs[4];
             for (int k = 0; k < 4; ++k) {
@@ -1149,7 +1152,14 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
         } else if (P.rr_enabled && depth >= P.rr_first) {
             const float u = sample_1d(P.sampler, key, first_dim + 9);
             bool survived;
-            if (!russian_roulette(beta4, eta_scale, u, P.rr_min, P.rr_max, survived)) return fail_path(P, J, slot, 32);
+            float* frozen = J.tape_survival ? J.tape_survival + path_of(J, slot) * J.survival_stride + vertex : nullptr;
+            if (frozen && J.survival_mode == BF_SURVIVAL_REPLAY) {
+                if (!russian_roulette_at(beta4, *frozen, u, survived)) return fail_path(P, J, slot, 32);
+            } else {
+                float p;
+                if (!russian_roulette(beta4, eta_scale, u, P.rr_min, P.rr_max, survived, p)) return fail_path(P, J, slot, 32);
+                if (frozen && !ADJ) *frozen = p;
+            }
             if (!survived) {
                 cont_pending = false;
                 pending_term = BF_TERM_RUSSIAN_ROULETTE;
@@ -1178,7 +1188,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
             const float a[4] = {a4.x, a4.y, a4.z, a4.w};
             const float R[4] = {total.x - L[0], total.y - L[1], total.z - L[2], total.w - L[3]};
             texel_gradients(P, J, S, M, mix, filtered, surf.uv, wl, a, visible, wo_local, nee_wi, nee_f, contribution,
-                            cont_pending, bs.incoming, bs.value, R);
+                            cont_pending, bs.discrete != 0u, bs.incoming, bs.value, R);
         }
         if (!cont_pending) {
             P.paths.status[slot] = pending_term;
@@ -1762,6 +1772,48 @@ __global__ void k_sample_dump(sampler_ctx sc, const bf_sample_stream_index* indi
     out[i] = sample_1d(sc, k, dims[i]);
 }
 
+/* The reference's dimension map (SampleDimensionMap.cpp:13-69, SampleStreamKernel.hpp:60-91):
+ * bounce b uses dimensions 4 + 10 b + {light selection, light u/v, bsdf component/u/v, medium x3,
+ * roulette} for b <= 2^32 - 1. */
+__global__ void k_sample_stream_dump(const bf_sample_dump_request* requests, uint32_t count,
+                                     bf_sample_dimension_manifest* manifest, bf_sample_dump_result* results) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) {
+        bf_sample_dimension_manifest m;
+        m.schema_version = BF_SAMPLE_DIMENSION_SCHEMA;
+        m.reserved = 0u;
+        m.primary_dimensions = bf_sample_primary_dimensions{BF_DIM_RASTER_X, BF_DIM_RASTER_Y, 0u, 1u, 2u, 3u};
+        m.first_bounce_dimension = BF_SAMPLE_FIRST_BOUNCE_DIMENSION;
+        m.dimensions_per_bounce = BF_SAMPLE_DIMENSIONS_PER_BOUNCE;
+        m.maximum_bounce_index = BF_SAMPLE_MAXIMUM_BOUNCE;
+        m.bounce_offsets = bf_sample_bounce_dimensions{0u, 1u, 2u, 3u, 4u, 5u, 6u, 7u, 8u, 9u};
+        *manifest = m;
+    }
+    if (i >= count) return;
+    const bf_sample_dump_request r = requests[i];
+    sample_key k;
+    k.px = r.sample_stream.pixel_x;
+    k.py = r.sample_stream.pixel_y;
+    k.s = r.sample_stream.sample_index;
+    k.seed = r.sample_stream.seed;
+    bf_sample_dump_result out;
+    out.reserved = 0u;
+    out.reserved_tail = 0u;
+    out.indexed_bits = indexed_bits(k, r.dimension);
+    out.sample = (float)(out.indexed_bits >> 40) * 0x1p-24f;
+    uint64_t d[10];
+    if (r.bounce_index > BF_SAMPLE_MAXIMUM_BOUNCE) {
+        out.status = BF_SAMPLE_DUMP_BOUNCE_OUT_OF_RANGE;
+        for (int k2 = 0; k2 < 10; ++k2) d[k2] = 0u;
+    } else {
+        out.status = BF_SAMPLE_DUMP_SUCCESS;
+        const uint64_t first = BF_SAMPLE_FIRST_BOUNCE_DIMENSION + r.bounce_index * BF_SAMPLE_DIMENSIONS_PER_BOUNCE;
+        for (int k2 = 0; k2 < 10; ++k2) d[k2] = first + (uint64_t)k2;
+    }
+    out.bounce_dimensions = bf_sample_bounce_dimensions{d[0], d[1], d[2], d[3], d[4], d[5], d[6], d[7], d[8], d[9]};
+    results[i] = out;
+}
+
 /* ---- small utility kernels (used instead of hipMemsetAsync so the sequence stays capturable) - */
 /* one thread per gradient element: the replica rows in row order, then one add to the caller's buffer */
 __global__ __launch_bounds__(BF_BLOCK) void k_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count,
@@ -1851,10 +1903,22 @@ __global__ void k_zero_doubles(double* dst, uint32_t count) {
  * the blob is 16-byte aligned and single bytes otherwise. */
 /* validate_scene_soa_header on the device copy of a blob (the check every reference stage makes
  * of the uploaded header, CudaWavefrontStages.cu:818-880), one thread; plus the byte count. */
+/* validate_scene_soa_header (SceneSoaAbi.hpp:283-341), then the stage check's further conditions
+ * (CudaWavefrontStages.cu:818-862): the header's total size equals the device byte count, every
+ * record count fits 32 bits, and the mesh-area lights fit the uniform light registry (2^24). */
 __global__ void k_scene_soa_validate(const bf_scene_soa_header* header, uint64_t byte_count, uint32_t* status) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t st = bf_soa::validate_header(*header);
-    *status = st != BF_SCENE_SOA_VALID ? st : (header->total_size_bytes > byte_count ? (uint32_t)BF_SCENE_SOA_TRUNCATED : st);
+    const bf_scene_soa_header& h = *header;
+    uint32_t st = bf_soa::validate_header(h);
+    if (st == BF_SCENE_SOA_VALID && h.total_size_bytes > byte_count) st = BF_SCENE_SOA_TRUNCATED;
+    if (st == BF_SCENE_SOA_VALID && h.total_size_bytes != byte_count) st = BF_SCENE_SOA_SIZE_MISMATCH;
+    const uint64_t counts[12] = {h.object_count, h.geometry_count, h.vertex_count, h.triangle_count,
+                                 h.material_count, h.closure_count, h.instance_count, h.mesh_area_light_count,
+                                 h.punctual_light_count, h.texture_count, h.image_texture_count, h.image_mip_count};
+    for (int k = 0; k < 12 && st == BF_SCENE_SOA_VALID; ++k)
+        if (counts[k] > 0xFFFFFFFFull) st = BF_SCENE_SOA_COUNT_OVERFLOW;
+    if (st == BF_SCENE_SOA_VALID && h.mesh_area_light_count > (1ull << 24)) st = BF_SCENE_SOA_TOO_MANY_MESH_LIGHTS;
+    *status = st;
 }
 
 __global__ void k_scene_hash(const uint8_t* bytes, size_t count, uint64_t* out) {
@@ -2405,6 +2469,13 @@ hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t d
     sc.directions = directions;
     hipLaunchKernelGGL(k_sample_dump, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, sc, indices, dimensions, count,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t bfk_sample_stream_dump(const bf_sample_dump_request* requests, uint32_t count,
+                                  bf_sample_dimension_manifest* manifest, bf_sample_dump_result* results) {
+    hipLaunchKernelGGL(k_sample_stream_dump, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, nullptr, requests, count,
+                       manifest, results);
     return hipGetLastError();
 }
 

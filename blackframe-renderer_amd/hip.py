@@ -52,7 +52,8 @@ def library():
     lib.blackframe_hip_scene_wide_export.argtypes = [vp, C.POINTER(u64), vp, vp]
     lib.blackframe_hip_launch_scene_closest_hit.argtypes = [vp, vp, u32, vp, vp, vp]
     lib.blackframe_hip_launch_scene_occlusion.argtypes = [vp, vp, u32, vp, vp]
-    lib.blackframe_hip_launch_sample_stream_dump.argtypes = [u32, vp, vp, u32, vp, vp]
+    lib.blackframe_hip_launch_sample_dump.argtypes = [u32, vp, vp, u32, vp, vp]
+    lib.blackframe_hip_launch_sample_stream_dump.argtypes = [u32, vp, u32, vp, vp]
     lib.blackframe_hip_workspace_create.argtypes = [u64, C.POINTER(vp)]
     lib.blackframe_hip_workspace_destroy.argtypes = [vp]
     lib.blackframe_hip_trace_wavefront_batch.argtypes = [vp, C.POINTER(abi.FrameJob), u32, C.POINTER(abi.TransportOptions),
@@ -245,17 +246,46 @@ class Tape:
         self.visibility = torch.zeros(path_count, dtype=torch.int64, device="cuda")
 
 
+class Survival:
+    """Russian-roulette survival probabilities by path vertex (bf_frame_job.tape_survival), device
+    memory of path_count * (diffuse + glossy + specular depth limits + 1) floats. mode is
+    abi.SURVIVAL_RECORD (a render writes them) or abi.SURVIVAL_REPLAY (a render decides with them)."""
+
+    def __init__(self, path_count, options, mode=abi.SURVIVAL_RECORD):
+        import torch
+
+        self.path_count = path_count
+        self.stride = options.diffuse_depth_limit + options.glossy_depth_limit + options.specular_depth_limit + 1
+        self.values = torch.zeros(path_count * self.stride, dtype=torch.float32, device="cuda")
+        self.mode = mode
+
+    def replaying(self):
+        """The same probabilities, decided with instead of recorded."""
+        other = Survival.__new__(Survival)
+        other.path_count, other.stride, other.values, other.mode = self.path_count, self.stride, self.values, abi.SURVIVAL_REPLAY
+        return other
+
+
+def _bind_survival(job, survival, n_paths, k):
+    if survival is None:
+        return
+    if survival.path_count != n_paths:
+        raise XpuError(1, f"survival tape holds {survival.path_count} paths, frame {k} has {n_paths}")
+    job.tape_survival = survival.values.data_ptr()
+    job.survival_mode = survival.mode
+
+
 def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, want_paths=False, want_counters=False,
-                                  stream=None, synchronize=True, tape=None):
+                                  stream=None, synchronize=True, tape=None, survival=None):
     """Renders one frame; mirrors engine::trace_cuda_wavefront_transport (WavefrontTransport.hpp:197-208).
 
     want_paths: False, True (host ctypes array of PathResult) or "device" (a torch uint8 tensor of
     n_paths * 64 bytes left in device memory, for frames whose results are only sampled)."""
     import torch
 
-    if tape is not None:
+    if tape is not None or survival is not None:
         # the single-frame entry point has the reference's signature; taped frames use the batch one
-        r = trace_xpu_wavefront_batch(workspace, [(scene, frame, film, tape)], options, want_paths=want_paths,
+        r = trace_xpu_wavefront_batch(workspace, [(scene, frame, film, tape, survival)], options, want_paths=want_paths,
                                       want_counters=want_counters, stream=stream, synchronize=synchronize)
         out = {"film": r["films"][0], "report": r["report"]}
         if want_counters:
@@ -291,7 +321,7 @@ def trace_xpu_wavefront_batch(workspace, items, options, want_paths=False, want_
                               synchronize=True):
     """Renders many frames through one set of queues (blackframe_hip_trace_wavefront_batch).
 
-    items: sequence of (XpuScene, FrameDesc[, Film[, Tape]]). Returns {"films": [...], "report": ...,
+    items: sequence of (XpuScene, FrameDesc[, Film[, Tape[, Survival]]]). Returns {"films": [...], "report": ...,
     "paths": [...] (per frame, as in trace_xpu_wavefront_transport), "traversal_counters": ...}."""
     import torch
 
@@ -317,6 +347,7 @@ def trace_xpu_wavefront_batch(workspace, items, options, want_paths=False, want_
                 raise XpuError(1, f"tape holds {tape.path_count} paths, frame {k} has {n_paths}")
             jobs[k].tape_radiance = tape.radiance.data_ptr()
             jobs[k].tape_visibility = tape.visibility.data_ptr()
+        _bind_survival(jobs[k], item[4] if len(item) > 4 else None, n_paths, k)
     report = abi.TransportReport()
     counters = np.zeros(6, dtype=np.uint64)
     _check(library().blackframe_hip_trace_wavefront_batch(
@@ -335,7 +366,7 @@ def trace_xpu_wavefront_batch(workspace, items, options, want_paths=False, want_
 def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchronize=True):
     """Path-replay adjoint (blackframe_hip_trace_wavefront_adjoint).
 
-    items: sequence of (XpuScene, FrameDesc, Tape, film_gradient, texture_gradients) with
+    items: sequence of (XpuScene, FrameDesc, Tape, film_gradient, texture_gradients[, Survival]) with
     film_gradient a float32 device tensor of dLoss/d(film RGB sums) [pixels * 3] and
     texture_gradients a sequence (per scene texture slot) of float32 device tensors or None;
     gradients are accumulated into them. Returns the report."""
@@ -343,7 +374,8 @@ def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchron
 
     jobs = (abi.FrameJob * len(items))()
     adjoints = (abi.AdjointJob * len(items))()
-    for k, (scene, frame, tape, film_gradient, texture_gradients) in enumerate(items):
+    for k, item in enumerate(items):
+        scene, frame, tape, film_gradient, texture_gradients = item[:5]
         cam = frame.camera
         if film_gradient.dtype != torch.float32 or film_gradient.numel() != cam.width * cam.height * 3:
             raise XpuError(1, f"frame {k}: film gradient must be float32 [{cam.width * cam.height * 3}]")
@@ -353,6 +385,7 @@ def trace_xpu_wavefront_adjoint(workspace, items, options, stream=None, synchron
         jobs[k].frame = frame
         jobs[k].tape_radiance = tape.radiance.data_ptr()
         jobs[k].tape_visibility = tape.visibility.data_ptr()
+        _bind_survival(jobs[k], item[5] if len(item) > 5 else None, tape.path_count, k)
         adjoints[k].film_gradient = film_gradient.data_ptr()
         gradients = list(texture_gradients or [])
         if len(gradients) > len(scene.texture_sizes):
@@ -395,14 +428,17 @@ def scene_soa_validate(blob, verify_hash=True):
     return status.value
 
 
-def scene_soa_validate_on_device(blob):
-    """blackframe_hip_launch_scene_soa_validate over host bytes copied to the device."""
+def scene_soa_validate_on_device(blob, byte_count=None):
+    """blackframe_hip_launch_scene_soa_validate over host bytes copied to the device. byte_count
+    (default: the bytes given) is the device byte count the call is told; the check reads the
+    header only, so a header can be checked against the size its columns claim."""
     import torch
 
     data = bytes(blob)
     d = torch.frombuffer(bytearray(data), dtype=torch.uint8).cuda()
     status = torch.full((1,), 0xFFFF, dtype=torch.int32, device="cuda")
-    _check(library().blackframe_hip_launch_scene_soa_validate(C.c_void_p(d.data_ptr()), len(data),
+    _check(library().blackframe_hip_launch_scene_soa_validate(C.c_void_p(d.data_ptr()),
+                                                              len(data) if byte_count is None else byte_count,
                                                               C.c_void_p(status.data_ptr())))
     torch.cuda.synchronize()
     return int(status.item())
@@ -463,7 +499,6 @@ def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None):
     d_result = torch.full((12,), -1, dtype=torch.int32, device="cuda")
     need = queue_compaction_scratch_bytes(len(outcomes))
     size = need if scratch_bytes is None else scratch_bytes
-    d_scratch = torch.zeros(max(size, 1), dtype=torch.uint8, device="cuda")
     view = abi.QueueSoa(d_headers.data_ptr(), d_slots.data_ptr() if d_slots is not None else None, abi.QUEUE_COUNT, stride)
     _check(library().blackframe_hip_launch_wavefront_queue_compaction(
         view, C.c_void_p(d_outcomes.data_ptr()) if d_outcomes is not None else None, len(outcomes), route,
@@ -541,11 +576,28 @@ def sample_stream_dump(sampler, indices, dimensions, stream=None):
     d_idx = torch.frombuffer(bytearray(memoryview(indices).cast("B")), dtype=torch.uint8).cuda()
     d_dim = torch.from_numpy(np.ascontiguousarray(dimensions, dtype=np.uint64).view(np.int64)).cuda()
     d_out = torch.zeros(n, dtype=torch.float32, device="cuda")
-    _check(library().blackframe_hip_launch_sample_stream_dump(sampler, C.c_void_p(d_idx.data_ptr()),
-                                                              C.c_void_p(d_dim.data_ptr()), n,
-                                                              C.c_void_p(d_out.data_ptr()), _stream_handle(stream)))
+    _check(library().blackframe_hip_launch_sample_dump(sampler, C.c_void_p(d_idx.data_ptr()), C.c_void_p(d_dim.data_ptr()),
+                                                       n, C.c_void_p(d_out.data_ptr()), _stream_handle(stream)))
     torch.cuda.synchronize()
     return d_out.cpu().numpy()
+
+
+def sample_stream_dump_requests(requests, schema=abi.SAMPLE_DIMENSION_SCHEMA):
+    """blackframe_hip_launch_sample_stream_dump on a ctypes array of SampleDumpRequest copied to the
+    device. Returns (SampleDimensionManifest, ctypes array of SampleDumpResult)."""
+    import torch
+
+    n = len(requests)
+    d_req = torch.frombuffer(bytearray(memoryview(requests).cast("B")), dtype=torch.uint8).cuda()
+    d_manifest = torch.zeros(C.sizeof(abi.SampleDimensionManifest), dtype=torch.uint8, device="cuda")
+    d_results = torch.zeros(n * C.sizeof(abi.SampleDumpResult), dtype=torch.uint8, device="cuda")
+    _check(library().blackframe_hip_launch_sample_stream_dump(schema, C.c_void_p(d_req.data_ptr()), n,
+                                                              C.c_void_p(d_manifest.data_ptr()),
+                                                              C.c_void_p(d_results.data_ptr())))
+    torch.cuda.synchronize()
+    manifest = abi.SampleDimensionManifest.from_buffer_copy(d_manifest.cpu().numpy().tobytes())
+    results = (abi.SampleDumpResult * n).from_buffer_copy(d_results.cpu().numpy().tobytes())
+    return manifest, results
 
 
 def xpu_devices():

@@ -377,6 +377,11 @@ def lambert(reflectance, probability=1.0):
             "probability": probability}
 
 
+def specular_reflection(reflectance, probability=1.0):
+    return {"kind": abi.CLOSURE_SPECULAR_REFLECTION, "lobes": abi.LOBE_SPECULAR | abi.LOBE_REFLECTION,
+            "weight": reflectance, "probability": probability}
+
+
 def rough_diffuse(reflectance, roughness, probability=1.0):
     """Energy-preserving Oren–Nayar; parameters[0] = roughness in [0, 1]."""
     return {"kind": abi.CLOSURE_ROUGH_DIFFUSE, "lobes": abi.LOBE_DIFFUSE | abi.LOBE_REFLECTION, "weight": reflectance,
@@ -441,13 +446,16 @@ def cornell(width=64, height=64, materials="lambert"):
 
 
 def textured_cornell(width=256, height=256, texture_size=512, seed=0xC3_7E57, albedo=None, roughness=None,
-                     roughness_size=1, mixed=False):
+                     roughness_size=1, mixed=False, specular=None, tint=None):
     """The differentiable Cornell box: the back wall's reflectance is an RGB texture (seeded
     U[0.1, 0.9] texels unless `albedo` is given, host (h, w, 3) array or device tuple) and the tall
     box is a gold GGX conductor whose widths come from a 1-channel roughness texture (0.3 unless
     `roughness` is given). mixed=True adds every other binding: the back wall becomes a textured
     Lambert + copper GGX mixture, the left wall an Oren-Nayar surface with a grey texture (slot 2)
     and the short box a GGX dielectric with a 2-channel (alpha_x, alpha_y) texture (slot 3).
+    specular="mirror" makes the back wall a mirror whose tint is the albedo texture;
+    specular="mixture" makes it a Lambert (albedo texture, probability 0.6) + mirror (probability
+    0.4) mixture whose mirror tint is a 4x4 RGB texture (`tint`, slot 2).
     Returns (scene, camera, textures) with textures the host arrays used."""
     rng = SplitMix64(seed)
     s = Scene()
@@ -465,6 +473,17 @@ def textured_cornell(width=256, height=256, texture_size=512, seed=0xC3_7E57, al
     back = s.add_material(28, [dict(lambert(1.0), weight_texture=albedo_slot)])
     short_material = white
     textures = {"albedo": albedo, "roughness": roughness}
+    if specular == "mirror":
+        back = s.add_material(28, [dict(specular_reflection(1.0), weight_texture=albedo_slot)])
+    elif specular == "mixture":
+        if tint is None:
+            tint = (0.3 + 0.6 * np.array([rng.uniform() for _ in range(48)])).reshape(4, 4, 3).astype(np.float32)
+        textures["tint"] = tint
+        tint_slot = s.add_texture(tint)
+        back = s.add_material(28, [dict(lambert(1.0, probability=0.6), weight_texture=albedo_slot),
+                                   dict(specular_reflection(1.0, probability=0.4), weight_texture=tint_slot)])
+    elif specular is not None:
+        raise ValueError(f"unknown specular variant {specular!r}")
     if mixed:
         grey = (0.2 + 0.6 * np.array([rng.uniform() for _ in range(16)])).reshape(4, 4, 1).astype(np.float32)
         widths = (0.15 + 0.3 * np.array([rng.uniform() for _ in range(32)])).reshape(4, 4, 2).astype(np.float32)

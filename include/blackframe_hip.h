@@ -70,12 +70,25 @@ int blackframe_hip_launch_scene_occlusion(const bf_hip_scene* scene, const bf_tr
                                           bf_occlusion_result* results, void* stream);
 
 /*
- * Evaluates sample_1d for `count` (index, dimension) pairs with the chosen sampler
- * (bf_sampler_kind) into device floats.
- * Replaces: blackframe_cuda_launch_sample_stream_dump (SampleStreamKernel.hpp:170-176).
+ * For every device request: the indexed-hash bits and binary32 sample of (sample_stream, dimension)
+ * and the ten transport dimensions of bounce_index (status bounce_out_of_range with zero dimensions
+ * past BF_SAMPLE_MAXIMUM_BOUNCE); thread 0 also writes the dimension manifest. A schema other than
+ * BF_SAMPLE_DIMENSION_SCHEMA, a zero count or a null pointer returns BF_INVALID_ARGUMENT (the value
+ * of the reference's cudaErrorInvalidValue) before any launch. Default stream, asynchronous.
+ * Replaces: blackframe_cuda_launch_sample_stream_dump (XPU/CUDA/Include/Blackframe/XPU/CUDA/
+ *           SampleStreamKernel.hpp:170-176, Source/CudaSampleStream.cu:23-63).
  */
-int blackframe_hip_launch_sample_stream_dump(uint32_t sampler, const bf_sample_stream_index* indices,
-                                             const uint64_t* dimensions, uint32_t count, float* samples, void* stream);
+int blackframe_hip_launch_sample_stream_dump(uint32_t requested_schema_version, const bf_sample_dump_request* requests,
+                                             uint32_t request_count, bf_sample_dimension_manifest* manifest,
+                                             bf_sample_dump_result* results);
+
+/*
+ * sample_1d of `count` (index, dimension) pairs with either sampler (bf_sampler_kind: the
+ * reference's indexed hash, or the Sobol-Owen sequence the transport can use instead) into device
+ * floats, on `stream`. No reference counterpart: the reference dumps the indexed hash only.
+ */
+int blackframe_hip_launch_sample_dump(uint32_t sampler, const bf_sample_stream_index* indices, const uint64_t* dimensions,
+                                      uint32_t count, float* samples, void* stream);
 
 /*
  * Fixed-capacity transport workspace: path-state columns, the four queues, compaction scratch.
@@ -123,7 +136,18 @@ typedef struct bf_frame_job {
      * in frame path order (path = pass * pixels + pixel): */
     float* tape_radiance;      /* [paths * 4] final spectral radiance of each path */
     uint64_t* tape_visibility; /* [paths] bit d: the shadow ray of path vertex d reached its light */
+    /* Optional Russian-roulette survival probabilities by path vertex, device memory
+     * [paths * (diffuse + glossy + specular depth limits + 1)] floats in frame path order (no
+     * reference counterpart). survival_mode BF_SURVIVAL_RECORD writes the probability of every
+     * roulette decision; BF_SURVIVAL_REPLAY decides with the recorded probability instead of the
+     * one the path's throughput gives (u < p survives, the throughput is divided by p). Replay holds
+     * the roulette fixed while texels move, so a finite difference of replayed renders is the
+     * derivative the adjoint computes. */
+    float* tape_survival;
+    uint32_t survival_mode;    /* bf_survival_mode; BF_SURVIVAL_NONE ignores tape_survival */
+    uint32_t reserved;         /* must be 0 */
 } bf_frame_job;
+
 
 /*
  * Renders many frames of (possibly) different scenes through one set of queues: every launch of
@@ -346,6 +370,7 @@ int blackframe_hip_adam_step(const bf_adam_param* params, uint32_t param_count, 
 } /* extern "C" */
 #endif
 
+BF_STATIC_ASSERT(sizeof(bf_frame_job) == 184, "frame job is 184 bytes");
 BF_STATIC_ASSERT(sizeof(bf_film_loss_job) == 40, "film loss job is 40 bytes");
 BF_STATIC_ASSERT(sizeof(bf_adam_param) == 56, "Adam parameter record is 56 bytes");
 BF_STATIC_ASSERT(sizeof(bf_adam_options) == 32, "Adam options are 32 bytes");

@@ -213,7 +213,12 @@ enum bf_scene_soa_status {
     BF_SCENE_SOA_BAD_TOTAL_SIZE = 10,
     /* beyond the header (no reference counterpart): checked by the blob entry points */
     BF_SCENE_SOA_TRUNCATED = 11,   /* fewer bytes than the header or than total_size_bytes */
-    BF_SCENE_SOA_HASH_MISMATCH = 12 /* content_hash differs from the FNV-1a-64 of the bytes */
+    BF_SCENE_SOA_HASH_MISMATCH = 12, /* content_hash differs from the FNV-1a-64 of the bytes */
+    /* the further checks each reference stage makes of the uploaded blob (validate_scene,
+     * CudaWavefrontStages.cu:818-862), reported by blackframe_hip_launch_scene_soa_validate: */
+    BF_SCENE_SOA_SIZE_MISMATCH = 13,       /* total_size_bytes differs from the blob's byte count (padding) */
+    BF_SCENE_SOA_COUNT_OVERFLOW = 14,      /* a record count above 2^32 - 1 */
+    BF_SCENE_SOA_TOO_MANY_MESH_LIGHTS = 15 /* mesh_area_light_count above 2^24 (unsupported_transport) */
 };
 
 /* Flags of blackframe_hip_scene_soa_validate / blackframe_hip_scene_create_from_soa. */
@@ -243,6 +248,48 @@ typedef struct BF_ALIGN16 bf_scene_soa_header {
     bf_scene_soa_column columns[BF_SCENE_SOA_COLUMN_COUNT];
     uint64_t reserved[5];
 } bf_scene_soa_header;
+
+/* ---- sample-stream dump (the reference's SampleStreamKernel.hpp:10-176 contract) --------------- */
+#define BF_SAMPLE_DIMENSION_SCHEMA 1u              /* sample_stream::DimensionMapSchemaVersion */
+#define BF_SAMPLE_MAXIMUM_BOUNCE 0xFFFFFFFFull     /* MaximumBounceIndex */
+#define BF_SAMPLE_FIRST_BOUNCE_DIMENSION 4ull
+#define BF_SAMPLE_DIMENSIONS_PER_BOUNCE 10ull
+enum bf_sample_dump_status { BF_SAMPLE_DUMP_SUCCESS = 0, BF_SAMPLE_DUMP_BOUNCE_OUT_OF_RANGE = 1 };
+
+/* SampleStreamPrimaryDimensions (48 B) and SampleStreamBounceDimensions (80 B) */
+typedef struct bf_sample_primary_dimensions {
+    uint64_t camera_raster_x, camera_raster_y, lens_u, lens_v, time, wavelength;
+} bf_sample_primary_dimensions;
+typedef struct bf_sample_bounce_dimensions {
+    uint64_t light_selection, light_u, light_v, bsdf_component, bsdf_u, bsdf_v;
+    uint64_t medium_distance, medium_phase_u, medium_phase_v, russian_roulette;
+} bf_sample_bounce_dimensions;
+
+/* SampleStreamDimensionManifest (160 B) */
+typedef struct bf_sample_dimension_manifest {
+    uint32_t schema_version;
+    uint32_t reserved;
+    bf_sample_primary_dimensions primary_dimensions;
+    uint64_t first_bounce_dimension;
+    uint64_t dimensions_per_bounce;
+    uint64_t maximum_bounce_index;
+    bf_sample_bounce_dimensions bounce_offsets;
+} bf_sample_dimension_manifest;
+
+/* SampleStreamDumpRequest (40 B) and SampleStreamDumpResult (104 B) */
+typedef struct bf_sample_dump_request {
+    bf_sample_stream_index sample_stream;
+    uint64_t dimension;
+    uint64_t bounce_index;
+} bf_sample_dump_request;
+typedef struct bf_sample_dump_result {
+    uint32_t status; /* bf_sample_dump_status */
+    uint32_t reserved;
+    uint64_t indexed_bits;
+    float sample;
+    uint32_t reserved_tail;
+    bf_sample_bounce_dimensions bounce_dimensions; /* zero when status is bounce_out_of_range */
+} bf_sample_dump_result;
 
 /* ---- queue compaction (the reference's WavefrontQueueCompaction.hpp contract) --------------- */
 #define BF_QUEUE_COUNT 7u                    /* camera, ray, hit, miss, shade, shadow, continuation */
@@ -508,6 +555,10 @@ typedef struct bf_transport_options {
  * restatement. */
 enum bf_failure_policy { BF_FAILURES_ABORT = 0, BF_FAILURES_COUNT = 1 };
 
+/* What a frame does with its Russian-roulette survival tape (bf_frame_job.tape_survival): nothing,
+ * record every decision's probability, or decide with the recorded probabilities. */
+enum bf_survival_mode { BF_SURVIVAL_NONE = 0, BF_SURVIVAL_RECORD = 1, BF_SURVIVAL_REPLAY = 2 };
+
 /* One frame of work: spp passes over width*height pixels; path slot p = s*W*H + y*W + x. */
 typedef struct bf_frame_desc {
     bf_camera_desc camera;
@@ -599,6 +650,13 @@ BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, object_count) == 40, "counts at 4
 BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, columns) == 152, "columns at 152");
 BF_STATIC_ASSERT(offsetof(bf_scene_soa_header, reserved) == 4808, "reserved at 4808");
 BF_STATIC_ASSERT(sizeof(bf_queue_soa) == 24, "queue view is 24 bytes");
+BF_STATIC_ASSERT(sizeof(bf_sample_primary_dimensions) == 48, "SampleStreamPrimaryDimensions is 48 bytes");
+BF_STATIC_ASSERT(sizeof(bf_sample_bounce_dimensions) == 80, "SampleStreamBounceDimensions is 80 bytes");
+BF_STATIC_ASSERT(sizeof(bf_sample_dimension_manifest) == 160, "SampleStreamDimensionManifest is 160 bytes");
+BF_STATIC_ASSERT(offsetof(bf_sample_dimension_manifest, bounce_offsets) == 80, "manifest bounce offsets at 80");
+BF_STATIC_ASSERT(sizeof(bf_sample_dump_request) == 40, "SampleStreamDumpRequest is 40 bytes");
+BF_STATIC_ASSERT(sizeof(bf_sample_dump_result) == 104, "SampleStreamDumpResult is 104 bytes");
+BF_STATIC_ASSERT(offsetof(bf_sample_dump_result, bounce_dimensions) == 24, "dump result dimensions at 24");
 BF_STATIC_ASSERT(sizeof(bf_compaction_result) == 48, "WavefrontQueueCompactionResult is 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_node) == 48, "SceneBvhNode is 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_blas) == 32, "SceneBvhBlas is 32 bytes");

@@ -27,6 +27,9 @@ uint64_t bfo_sobol_bits(uint64_t sample_index, uint64_t sobol_dimension);
 uint64_t bfo_owen_bits(const bf_sample_stream_index* index, uint64_t sobol_dimension);
 float bfo_sample_1d(uint32_t sampler, const bf_sample_stream_index* index, uint64_t dimension);
 uint64_t bfo_sobol_dimension_count(void);
+/* Transport dimensions of bounce b (light selection, light u/v, bsdf component/u/v, medium x3,
+ * roulette): returns 0, or 1 with zeros past the mapped range. */
+int bfo_bounce_dimensions(uint64_t bounce_index, uint64_t out[10]);
 
 /* ---- film sensor -------------------------------------------------------------------------- */
 void bfo_cie_xyz(const float radiance[4], const float wavelengths[4], float pdf, float xyz[3]);
@@ -79,6 +82,13 @@ int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame,
 int bfo_render_taped(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
                      uint64_t path_begin, uint64_t path_count, uint32_t threads, float* film_float,
                      double* film_compensated, uint32_t* film_counts, float* tape_radiance, uint64_t* tape_visibility);
+/* Forward render with the Russian-roulette survival probabilities of each path vertex recorded
+ * into (BF_SURVIVAL_RECORD) or decided by (BF_SURVIVAL_REPLAY) `survival`, [global path * (diffuse
+ * + glossy + specular limits + 1) + vertex]; tape buffers optional (both or neither). */
+int bfo_render_survival(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                        uint64_t path_begin, uint64_t path_count, uint32_t threads, uint32_t survival_mode, float* survival,
+                        bf_path_result* paths, float* film_float, double* film_compensated, uint32_t* film_counts,
+                        float* tape_radiance, uint64_t* tape_visibility);
 /* Path-replay adjoint of paths [path_begin, path_begin + path_count): texel gradients of
  * dLoss/d(film RGB sums) = film_gradient, ACCUMULATED in binary64 into texture_gradients[k]
  * (layout of scene texture k; NULL entries skipped). */

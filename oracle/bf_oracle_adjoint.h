@@ -100,4 +100,20 @@ static int oa_closure_weight_derivative(const closure_rec* c, uint32_t allowed, 
     return 1;
 }
 
+/* d f / d w per lane of a specular (delta) closure along the delta direction wi: the value of its
+ * atom at unit weight (the atom is linear in the weight, scaled_delta_value,
+ * TransportLobesDevice.cuh:1784-1797); 0 when the closure's atom does not point along wi. */
+static int oa_delta_weight_derivative(const closure_rec* c, v3 wo, v3 wi, double out[4]) {
+    if (c->kind != BF_CLOSURE_SPECULAR_REFLECTION && c->kind != BF_CLOSURE_SPECULAR_TRANSMISSION) return 0;
+    const float unit[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    lobe_sample s;
+    if (c->kind == BF_CLOSURE_SPECULAR_REFLECTION) lobe_specular_reflection_sample(unit, wo, &s);
+    else lobe_specular_transmission_sample(unit, c->parameters[0], c->parameters[1], wo, &s);
+    if (s.status != LOBE_OK || s.measure != LOBE_MEASURE_DISCRETE || s.incoming.x != wi.x || s.incoming.y != wi.y ||
+        s.incoming.z != wi.z)
+        return 0;
+    for (int k = 0; k < 4; ++k) out[k] = s.value[k];
+    return 1;
+}
+
 #endif

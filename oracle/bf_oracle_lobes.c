@@ -68,5 +68,6 @@ int bfo_mis_weight(uint32_t heuristic, float sampled, float competing, float* we
 }
 
 int bfo_russian_roulette(float throughput[4], float eta_scale, float u, float minimum, float maximum, int* survived) {
-    return lobe_russian_roulette(throughput, eta_scale, u, minimum, maximum, survived);
+    float p;
+    return lobe_russian_roulette(throughput, eta_scale, u, minimum, maximum, survived, &p);
 }

@@ -155,9 +155,30 @@ static inline int lobe_mis_weight(uint32_t heuristic, float sampled, float compe
     return 1;
 }
 
-/* TransportDevice.cuh:711-782 (policy validated by the host) */
+/* The decision with a given survival probability p: u < p survives, the throughput is divided by p
+ * (RussianRoulette.hpp; a recorded p replays the decision with the texels moved). */
+static inline int lobe_russian_roulette_at(float* throughput, float p, float u, int* survived) {
+    if (!bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return 0;
+    for (int k = 0; k < 4; ++k)
+        if (!bf_isfinite(throughput[k]) || throughput[k] < 0.0f) return 0;
+    if (!bf_isfinite(p) || !(p > 0.0f) || p > 1.0f) return 0;
+    if (!(u < p)) {
+        *survived = 0;
+        return 1;
+    }
+    float out[4];
+    for (int k = 0; k < 4; ++k) {
+        out[k] = throughput[k] / p;
+        if (!bf_isfinite(out[k]) || (throughput[k] != 0.0f && out[k] == 0.0f)) return 0;
+    }
+    memcpy(throughput, out, sizeof out);
+    *survived = 1;
+    return 1;
+}
+
+/* TransportDevice.cuh:711-782 (policy validated by the host): p = clamp(max(throughput) * eta, minimum, maximum), then the decision. */
 static inline int lobe_russian_roulette(float* throughput, float eta_scale, float u, float minimum, float maximum,
-                                        int* survived) {
+                                        int* survived, float* probability) {
     if (!bf_isfinite(eta_scale) || !(eta_scale > 0.0f) || !bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return 0;
     float m = 0.0f;
     for (int k = 0; k < 4; ++k) {
@@ -172,19 +193,8 @@ static inline int lobe_russian_roulette(float* throughput, float eta_scale, floa
         const float candidate = m * eta_scale;
         p = candidate < minimum ? minimum : (maximum < candidate ? maximum : candidate);
     }
-    if (!bf_isfinite(p) || !(p > 0.0f) || p > 1.0f) return 0;
-    if (!(u < p)) {
-        *survived = 0;
-        return 1;
-    }
-    float out[4];
-    for (int k = 0; k < 4; ++k) {
-        out[k] = throughput[k] / p;
-        if (!bf_isfinite(out[k]) || (throughput[k] != 0.0f && out[k] == 0.0f)) return 0;
-    }
-    memcpy(throughput, out, sizeof out);
-    *survived = 1;
-    return 1;
+    *probability = p;
+    return lobe_russian_roulette_at(throughput, p, u, survived);
 }
 
 /* ---- Lambert (TransportDevice.cuh:335-478) ------------------------------------------------- */

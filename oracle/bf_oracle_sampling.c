@@ -207,3 +207,15 @@ void bfo_xyz_to_rgb(const float xyz[3], float rgb[3]) {
     rgb[1] = -0.9692660f * xyz[0] + 1.8760108f * xyz[1] + 0.0415560f * xyz[2];
     rgb[2] = 0.0556434f * xyz[0] - 0.2040259f * xyz[1] + 1.0572252f * xyz[2];
 }
+
+/* Renderer/Source/SampleDimensionMap.cpp:47-69: the ten transport dimensions of a bounce,
+ * 4 + 10 b + offset; 1 (and zeros) past the largest mapped bounce index 2^32 - 1. */
+int bfo_bounce_dimensions(uint64_t bounce_index, uint64_t out[10]) {
+    if (bounce_index > 0xFFFFFFFFull) {
+        for (int k = 0; k < 10; ++k) out[k] = 0u;
+        return 1;
+    }
+    const uint64_t first = BFO_FIRST_BOUNCE_DIM + bounce_index * BFO_DIMS_PER_BOUNCE;
+    for (int k = 0; k < 10; ++k) out[k] = first + (uint64_t)k;
+    return 0;
+}

@@ -64,6 +64,9 @@ typedef struct {
     const float* film_gradient;
     double* texture_gradients[BF_MAX_TEXTURES];
     pthread_mutex_t* lock;
+    /* roulette survival probabilities by path vertex [global path * stride + vertex] */
+    float* survival;
+    uint32_t survival_stride, survival_mode;
 } ctx;
 
 static void fail(lane* l, uint32_t code) {
@@ -866,7 +869,8 @@ static void rgb_basis64(float wavelength, double b[3]) {
 
 static void vertex_gradients(const ctx* c, const lane* l, const bf_material_desc* mat, const lobe_mixture* mix,
                              const lobe_filter_state* filtered, const float* uv, v3 wo, int light, v3 wi_light,
-                             const float* f_light, const float* N, int cont, v3 wi_cont, const float* f_cont) {
+                             const float* f_light, const float* N, int cont, int cont_discrete, v3 wi_cont,
+                             const float* f_cont) {
     const bfo_scene* s = c->scene;
     double R[4];
     for (int k = 0; k < 4; ++k) R[k] = (double)l->total[k] - (double)l->L[k];
@@ -885,7 +889,10 @@ static void vertex_gradients(const ctx* c, const lane* l, const bf_material_desc
         if (wgrad) {
             double dl[4] = {0.0, 0.0, 0.0, 0.0}, ds[4] = {0.0, 0.0, 0.0, 0.0};
             const int lw = light && oa_closure_weight_derivative(cr, allowed, wo, wi_light, dl);
-            const int cw = cont && oa_closure_weight_derivative(cr, allowed, wo, wi_cont, ds);
+            /* a delta continuation carries the sum of the matching delta atoms: only their weights
+             * move it, each by its unit-weight atom value */
+            const int cw = cont && (cont_discrete ? oa_delta_weight_derivative(cr, wo, wi_cont, ds)
+                                                  : oa_closure_weight_derivative(cr, allowed, wo, wi_cont, ds));
             const uint32_t slot = D->weight_texture - 1;
             const uint32_t channels = s->textures[slot].channels;
             float t[3];
@@ -915,7 +922,7 @@ static void vertex_gradients(const ctx* c, const lane* l, const bf_material_desc
         if (rgrad && (cr->kind == BF_CLOSURE_ROUGH_CONDUCTOR || cr->kind == BF_CLOSURE_ROUGH_DIELECTRIC)) {
             double ql[2] = {0.0, 0.0}, qs[2] = {0.0, 0.0};
             const int ul = light && oa_closure_alpha_log_gradient(cr, wo, wi_light, ql);
-            const int us = cont && oa_closure_alpha_log_gradient(cr, wo, wi_cont, qs);
+            const int us = cont && !cont_discrete && oa_closure_alpha_log_gradient(cr, wo, wi_cont, qs);
             float fl[4], fs[4];
             memcpy(fl, f_light, sizeof fl);
             memcpy(fs, f_cont, sizeof fs);
@@ -948,8 +955,8 @@ static void vertex_gradients(const ctx* c, const lane* l, const bf_material_desc
  * would), gradients, then the route the shadow stage would have given. */
 static void adjoint_vertex(const ctx* c, lane* l, const bf_material_desc* mat, const lobe_mixture* mix,
                            const lobe_filter_state* filtered, const float* uv, v3 wo, int has_shadow, v3 nee_wi,
-                           const float* nee_f, int continues, v3 wi_cont, const float* f_cont, uint32_t final_route,
-                           uint32_t final_term) {
+                           const float* nee_f, int continues, int cont_discrete, v3 wi_cont, const float* f_cont,
+                           uint32_t final_route, uint32_t final_term) {
     float N[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     const int visible = has_shadow && l->p_vertex < 64 && ((l->visible >> l->p_vertex) & 1u);
     if (visible) {
@@ -969,7 +976,7 @@ static void adjoint_vertex(const ctx* c, lane* l, const bf_material_desc* mat, c
             if (!bf_isfinite(l->L[k]) || l->L[k] < 0.0f) return fail(l, 41);
         }
     }
-    vertex_gradients(c, l, mat, mix, filtered, uv, wo, visible, nee_wi, nee_f, N, continues, wi_cont, f_cont);
+    vertex_gradients(c, l, mat, mix, filtered, uv, wo, visible, nee_wi, nee_f, N, continues, cont_discrete, wi_cont, f_cont);
     l->route = final_route;
     if (final_route == ROUTE_TERM) l->termination = final_term;
 }
@@ -1110,8 +1117,8 @@ static void shade_lane(const ctx* c, lane* l) {
         l->cont_pending = 0;
         l->pending_term = BF_TERM_OUTSIDE_SUPPORT;
         if (c->film_gradient)
-            return adjoint_vertex(c, l, mat, &mix, &filtered, surf.uv, wo_local, has_shadow, nee_wi, nee_f, 0, wi, bs.value,
-                                  ROUTE_TERM, BF_TERM_OUTSIDE_SUPPORT);
+            return adjoint_vertex(c, l, mat, &mix, &filtered, surf.uv, wo_local, has_shadow, nee_wi, nee_f, 0, 0, wi,
+                                  bs.value, ROUTE_TERM, BF_TERM_OUTSIDE_SUPPORT);
         if (has_shadow) {
             l->route = ROUTE_SHADOW;
             return;
@@ -1171,9 +1178,16 @@ static void shade_lane(const ctx* c, lane* l) {
     } else if (o->russian_roulette_enabled && l->depth >= o->russian_roulette_first_depth) {
         const float u = bfo_sample_1d(sampler, &l->sample, first_dim + 9);
         int survived;
-        if (!lobe_russian_roulette(l->beta, l->eta_scale, u, o->russian_roulette_minimum, o->russian_roulette_maximum,
-                                   &survived))
-            return fail(l, 32);
+        float* frozen = c->survival ? c->survival + l->global_path * c->survival_stride + vertex : NULL;
+        if (frozen && c->survival_mode == BF_SURVIVAL_REPLAY) {
+            if (!lobe_russian_roulette_at(l->beta, *frozen, u, &survived)) return fail(l, 32);
+        } else {
+            float p;
+            if (!lobe_russian_roulette(l->beta, l->eta_scale, u, o->russian_roulette_minimum, o->russian_roulette_maximum,
+                                       &survived, &p))
+                return fail(l, 32);
+            if (frozen && !c->film_gradient) *frozen = p;
+        }
         if (!survived) {
             cont_pending = 0;
             pending_term = BF_TERM_RUSSIAN_ROULETTE;
@@ -1181,7 +1195,8 @@ static void shade_lane(const ctx* c, lane* l) {
     }
     if (c->film_gradient)
         return adjoint_vertex(c, l, mat, &mix, &filtered, surf.uv, wo_local, has_shadow, nee_wi, nee_f, cont_pending,
-                              bs.incoming, bs.value, cont_pending ? ROUTE_CONT : ROUTE_TERM, pending_term);
+                              bs.measure == LOBE_MEASURE_DISCRETE, bs.incoming, bs.value,
+                              cont_pending ? ROUTE_CONT : ROUTE_TERM, pending_term);
     if (has_shadow) {
         l->cont_pending = (uint32_t)cont_pending;
         l->pending_term = pending_term;
@@ -1481,6 +1496,25 @@ int bfo_render_taped(const bfo_scene* scene, const bf_frame_desc* frame, const b
     memset(&c, 0, sizeof c);
     return render(&c, scene, frame, options, path_begin, path_count, threads, NULL, film_float, film_compensated,
                   film_counts, NULL, NULL, tape_radiance, tape_visibility);
+}
+
+int bfo_render_survival(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                        uint64_t path_begin, uint64_t path_count, uint32_t threads, uint32_t survival_mode, float* survival,
+                        bf_path_result* paths, float* film_float, double* film_compensated, uint32_t* film_counts,
+                        float* tape_radiance, uint64_t* tape_visibility) {
+    const uint64_t stride =
+        (uint64_t)options->diffuse_depth_limit + options->glossy_depth_limit + options->specular_depth_limit + 1u;
+    if (!survival || survival_mode == BF_SURVIVAL_NONE || survival_mode > BF_SURVIVAL_REPLAY || stride > 64u ||
+        (tape_radiance == NULL) != (tape_visibility == NULL))
+        return BF_INVALID_ARGUMENT;
+    ctx c;
+    memset(&c, 0, sizeof c);
+    c.survival = survival;
+    c.survival_stride = (uint32_t)stride;
+    c.survival_mode = survival_mode;
+    bf_transport_report report;
+    return render(&c, scene, frame, options, path_begin, path_count, threads, paths, film_float, film_compensated,
+                  film_counts, &report, NULL, tape_radiance, tape_visibility);
 }
 
 int bfo_render_adjoint(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,

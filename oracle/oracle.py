@@ -80,6 +80,10 @@ def load(abi):
     lib.bfo_sincosf.argtypes = [C.c_float, f3, f3]
     lib.bfo_render_taped.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
                                      C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.bfo_bounce_dimensions.argtypes = [C.c_uint64, C.POINTER(C.c_uint64)]
+    lib.bfo_render_survival.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
+                                        C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
     lib.bfo_render_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
                                        C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     d2 = C.POINTER(C.c_double)
@@ -314,6 +318,35 @@ class OracleScene:
         return {"film": film.reshape(cam.height, cam.width, 3), "film_compensated": comp[..., 0] + comp[..., 1],
                 "film_counts": counts.reshape(cam.height, cam.width), "tape_radiance": radiance,
                 "tape_visibility": visibility}
+
+    def render_survival(self, frame, options, survival, replay, taped=False, threads=1):
+        """Forward render of the whole frame whose Russian-roulette survival probabilities are
+        recorded into (replay=False) or decided by (replay=True) `survival`, a float32 array of
+        paths * (diffuse + glossy + specular limits + 1) entries; taped=True adds the adjoint tape."""
+        cam = frame.camera
+        pixels = cam.width * cam.height
+        total = pixels * frame.samples_per_pixel
+        film = np.zeros(pixels * 3, dtype=np.float32)
+        film_c = np.zeros(pixels * 6, dtype=np.float64)
+        counts = np.zeros(pixels, dtype=np.uint32)
+        abi = self.abi
+        paths = (abi.PathResult * total)()
+        radiance = np.zeros(total * 4, dtype=np.float32) if taped else None
+        visibility = np.zeros(total, dtype=np.uint64) if taped else None
+        assert survival.dtype == np.float32 and survival.flags.c_contiguous
+        status = self.lib.bfo_render_survival(self.handle, C.byref(frame), C.byref(options), 0, total, threads,
+                                              abi.SURVIVAL_REPLAY if replay else abi.SURVIVAL_RECORD,
+                                              survival.ctypes.data, paths, film.ctypes.data, film_c.ctypes.data,
+                                              counts.ctypes.data, radiance.ctypes.data if taped else None,
+                                              visibility.ctypes.data if taped else None)
+        if status != 0:
+            raise RuntimeError(f"oracle survival render failed with status {status}")
+        comp = film_c.reshape(cam.height, cam.width, 3, 2)
+        out = {"film": film.reshape(cam.height, cam.width, 3), "film_compensated": comp[..., 0] + comp[..., 1],
+               "film_counts": counts.reshape(cam.height, cam.width), "paths": paths}
+        if taped:
+            out.update(tape_radiance=radiance, tape_visibility=visibility)
+        return out
 
     def adjoint(self, frame, options, tape, film_gradient, texture_shapes, threads=1):
         """Binary64 texel gradients of sum(film_gradient * film) (film = RGB sums), one array per

@@ -88,3 +88,34 @@ def test_device_header_validation_matches_the_oracle(hip, abi):
     blob = rich_blob()
     assert hip.scene_soa_validate_on_device(blob) == abi.SCENE_SOA_VALID
     assert hip.scene_soa_validate_on_device(blob[:-16]) == abi.SCENE_SOA_TRUNCATED
+    # the stage check's further conditions (CudaWavefrontStages.cu:818-862): a padded blob, a record
+    # count above 2^32 - 1 and more than 2^24 mesh-area lights, each on a header whose columns are
+    # consistent with its counts (the check reads the header only)
+    assert hip.scene_soa_validate_on_device(blob + bytes(16)) == abi.SCENE_SOA_SIZE_MISMATCH
+    for field, count, expected in (("object", 1 << 32, abi.SCENE_SOA_COUNT_OVERFLOW),
+                                   ("mesh_area_light", (1 << 24) + 1, abi.SCENE_SOA_TOO_MANY_MESH_LIGHTS),
+                                   ("mesh_area_light", 1 << 24, abi.SCENE_SOA_VALID)):
+        header = consistent_header({field: count})
+        assert soa.validate_header(header) == soa.VALID
+        assert hip.scene_soa_validate_on_device(soa.pack_header(header), header["total_size_bytes"]) == expected
+
+
+def consistent_header(counts):
+    """A header whose column descriptors and total size follow `counts` (others zero)."""
+    import numpy as np
+
+    from test_scene_soa import bare_header
+    h = bare_header()
+    h["counts"].update(counts)
+    cursor = soa.align_up(soa.HEADER_SIZE)
+    for c in range(soa.COLUMN_COUNT):
+        n = h["counts"][soa.column_domain(c)]
+        size = np.dtype(soa.column_dtype(c)).itemsize
+        if n == 0:
+            h["columns"][c] = (0, 0, size, 0)
+            continue
+        cursor = soa.align_up(cursor)
+        h["columns"][c] = (cursor, n, size, 0)
+        cursor += n * size
+    h["total_size_bytes"] = cursor
+    return h

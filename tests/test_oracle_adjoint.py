@@ -238,3 +238,95 @@ def test_adjoint_width_gradient_agrees_with_finite_differences_statistically(bf,
     assert np.all(adjoint < 0) and np.all(fd < 0)
     spread = np.std(np.concatenate([adjoint, fd])) / np.sqrt(len(adjoint))
     assert abs(adjoint.mean() - fd.mean()) < 3.0 * spread + 0.05 * abs(fd.mean()), (adjoint, fd)
+
+
+@pytest.mark.parametrize("variant", ["mirror", "mixture"])
+def test_adjoint_gradient_through_delta_continuations(bf, oracle, variant):
+    """A textured mirror back wall, and a textured Lambert + textured mirror mixture: when the
+    continuation is the mirror's delta atom, only the mirror's weight moves it (by its unit-weight
+    atom value) and the Lambert's weight must not. Central differences of forward renders (roulette
+    off, so every sampling decision is independent of the texels) pin both texture gradients."""
+    rng = np.random.default_rng(21)
+    albedo = (0.2 + 0.6 * rng.random((4, 4, 3))).astype(np.float32)
+    tint = (0.3 + 0.6 * rng.random((4, 4, 3))).astype(np.float32)
+    roughness = np.full((1, 1, 1), 0.3, np.float32)
+
+    def build(a, t):
+        scene, cam, _ = bf.scenes.textured_cornell(16, 16, albedo=a, roughness=roughness, specular=variant, tint=t)
+        return scene, cam
+
+    scene, cam = build(albedo, tint)
+    frame = bf.scenes.frame_desc(cam, 8, seed=0)
+    options = bf.scenes.transport_options(max_depth=6)
+    options.russian_roulette_enabled = 0
+    weights = rng.standard_normal((16, 16, 3)).astype(np.float32)
+    s = oracle.scene(scene)
+    tape = s.render_taped(frame, options, threads=8)
+    shapes = [albedo.shape, roughness.shape] + ([tint.shape] if variant == "mixture" else [])
+    grads = s.adjoint(frame, options, tape, weights, shapes, threads=8)
+
+    def loss(a, t):
+        sc, _ = build(a, t)
+        return float((oracle.scene(sc).render(frame, options, threads=8, want_paths=False)["film_compensated"] *
+                      weights).sum())
+
+    eps = 1e-2
+    checks = [(0, albedo)] + ([(2, tint)] if variant == "mixture" else [])
+    for slot, texture in checks:
+        g = grads[slot]
+        scale = np.abs(g).max()
+        assert scale > 0.0, (variant, slot)
+        for y, x, c in [(0, 0, 0), (1, 2, 1), (3, 3, 2), (2, 1, 0), (1, 1, 1), (2, 2, 2)]:
+            up, dn = texture.copy(), texture.copy()
+            up[y, x, c] += eps
+            dn[y, x, c] -= eps
+            if slot == 0:
+                fd = (loss(up, tint) - loss(dn, tint)) / (2 * eps)
+            else:
+                fd = (loss(albedo, up) - loss(albedo, dn)) / (2 * eps)
+            assert abs(fd - g[y, x, c]) <= 2e-3 * abs(fd) + 1e-3 * scale, (variant, slot, (y, x, c), fd, g[y, x, c])
+
+
+def test_roulette_adjoint_matches_frozen_survival_differences(bf, oracle):
+    """Roulette on (from depth 2): recording every decision's survival probability leaves the render
+    unchanged, replaying them with the same texels reproduces it bit for bit, and central differences
+    of replayed renders (the roulette held fixed while the texels move, which is what the adjoint
+    differentiates) match the adjoint's texel gradients. Without the frozen probabilities a decision
+    that flips between the two renders adds a whole path tail to the difference quotient."""
+    rng = np.random.default_rng(31)
+    albedo = (0.1 + 0.8 * rng.random((4, 4, 3))).astype(np.float32)
+    roughness = np.full((1, 1, 1), 0.3, np.float32)
+    scene, cam = _textured(bf, 16, 16, albedo, roughness)
+    frame = bf.scenes.frame_desc(cam, 8, seed=0)
+    options = bf.scenes.transport_options(max_depth=6, rr_first_depth=2)
+    stride = options.diffuse_depth_limit + options.glossy_depth_limit + options.specular_depth_limit + 1
+    paths = 16 * 16 * 8
+    s = oracle.scene(scene)
+    survival = np.zeros(paths * stride, np.float32)
+    recorded = s.render_survival(frame, options, survival, replay=False, taped=True, threads=8)
+    plain = s.render(frame, options, threads=8)
+    assert np.array_equal(recorded["film"], plain["film"])
+    terminations = np.array([p.termination for p in plain["paths"]])
+    assert (terminations == bf.abi.TERM_RUSSIAN_ROULETTE).sum() > 100  # roulette decides many paths
+    assert np.all((survival == 0.0) | ((survival >= options.russian_roulette_minimum) & (survival <= 1.0)))
+    assert (survival > 0.0).sum() > 1000
+    replayed = s.render_survival(frame, options, survival, replay=True, threads=8)
+    assert np.array_equal(replayed["film"], plain["film"])
+
+    weights = rng.standard_normal((16, 16, 3)).astype(np.float32)
+    g_albedo, _ = s.adjoint(frame, options, recorded, weights, [albedo.shape, roughness.shape], threads=8)
+
+    def loss(a):
+        sc, _ = _textured(bf, 16, 16, a, roughness)
+        frozen = survival.copy()
+        return float((oracle.scene(sc).render_survival(frame, options, frozen, replay=True, threads=8)["film_compensated"] *
+                      weights).sum())
+
+    eps = 1e-2
+    scale = np.abs(g_albedo).max()
+    for y, x, c in [(0, 0, 0), (1, 2, 1), (3, 3, 2), (2, 1, 0), (0, 3, 1), (3, 0, 2)]:
+        up, dn = albedo.copy(), albedo.copy()
+        up[y, x, c] += eps
+        dn[y, x, c] -= eps
+        fd = (loss(up) - loss(dn)) / (2 * eps)
+        assert abs(fd - g_albedo[y, x, c]) <= 2e-3 * abs(fd) + 1e-3 * scale, ((y, x, c), fd, g_albedo[y, x, c])

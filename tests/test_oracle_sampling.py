@@ -120,3 +120,29 @@ def test_xyz_to_rgb_of_white_point(oracle):
     rgb = (C.c_float * 3)()
     oracle.lib.bfo_xyz_to_rgb(xyz, rgb)
     assert np.allclose(list(rgb), [1.0, 1.0, 1.0], atol=2e-4)
+
+
+def test_oracle_dump_matches_the_frozen_canonical_fingerprint(oracle, abi):
+    """Tests/CUDA/SampleStreamTest.cpp:401-414: the canonical text of the reference's dump requests
+    (indexed bits, binary32 sample, bounce dimensions, manifest), computed from the oracle, hashes
+    to the reference's frozen FNV-1a-64. A wrong salt, sample shift or dimension offset changes it."""
+    import sample_dump as sd
+
+    reqs = sd.requests()
+    results = []
+    for px, py, s, seed, dim, bounce in reqs:
+        i = abi.SampleStreamIndex(px, py, s, seed)
+        bits = oracle.lib.bfo_indexed_bits(C.byref(i), dim)
+        sample = oracle.lib.bfo_sample_1d(abi.SAMPLER_INDEXED_HASH, C.byref(i), dim)
+        dims = (C.c_uint64 * 10)()
+        status = oracle.lib.bfo_bounce_dimensions(bounce, dims)
+        results.append((status, bits, sd.float_bits(sample), list(dims)))
+    text = sd.canonical_text(sd.manifest_values(), reqs, results)
+    assert sd.fnv1a64(text) == sd.FROZEN_CANONICAL_DUMP_FNV1A, text
+    # the known bits and samples of the first four requests (:455-474)
+    assert [r[1] for r in results[:4]] == [0x2F37B19CDAB08AA0, 0xDE27E521323C8AFF, 0xDC4661423B6A00FA, 0x8F4FD2AABC4AC602]
+    assert [r[2] for r in results[:4]] == [0x3E3CDEC4, 0x3F5E27E5, 0x3F5C4661, 0x3F0F4FD2]
+    assert results[4][0] == 1 and results[4][3] == [0] * 10
+    # a planted off-by-one in the bounce map changes the fingerprint
+    planted = [(st, b, f, [d + 1 if d else 0 for d in dims]) for st, b, f, dims in results]
+    assert sd.fnv1a64(sd.canonical_text(sd.manifest_values(), reqs, planted)) != sd.FROZEN_CANONICAL_DUMP_FNV1A

@@ -182,3 +182,72 @@ def test_direct_lighting_form_factor(bf, abi, oracle, mis):
     expected = rho * radiance * 4 * rectangle_form_factor(half / h, half / h)
     error = values.std() / math.sqrt(values.size)
     assert abs(values.mean() - expected) < 4 * error + 1e-3 * expected, (values.mean(), expected, error)
+
+
+RASTER_X, RASTER_Y, WAVELENGTH = 0xA24BAED4963EE407, 0x9FB21C651E98DF25, 3
+
+
+def test_wavelength_known_answer(oracle, abi):
+    """Tests/Renderer/WavelengthSamplingTest.cpp:156-170: the default SampleStreamIndex's
+    wavelength dimension and its four stratified wavelengths, exactly."""
+    i = abi.SampleStreamIndex(0, 0, 0, 0)
+    u = oracle.lib.bfo_sample_1d(abi.SAMPLER_INDEXED_HASH, C.byref(i), WAVELENGTH)
+    assert np.float32(u) == np.float32(0.23662316799163818)
+    out = fbuf([0] * 4)
+    oracle.lib.bfo_sample_wavelengths(u, out)
+    assert [np.float32(v) for v in out] == [np.float32(v) for v in (471.212890625, 588.712890625, 706.212890625,
+                                                                    823.712890625)]
+
+
+def test_pixel_jitter_known_answer(oracle, abi):
+    """Tests/Renderer/PixelJitterTest.cpp:142-152: uniform jitter of pixel (17, 29), sample 27,
+    seed 42 on the raster dimensions, exactly."""
+    i = abi.SampleStreamIndex(17, 29, 27, 42)
+    ox = oracle.lib.bfo_sample_1d(abi.SAMPLER_INDEXED_HASH, C.byref(i), RASTER_X)
+    oy = oracle.lib.bfo_sample_1d(abi.SAMPLER_INDEXED_HASH, C.byref(i), RASTER_Y)
+    assert np.float32(ox) == np.float32(0x4D67A7 * 2.0**-24)
+    assert np.float32(oy) == np.float32(0x6B1427 * 2.0**-24)
+
+
+def _camera(abi, origin, tangent, bitangent, normal, width, height, fov, t_min=0.0, t_max=float("inf"), time=0.0,
+            mask=0xFFFFFFFF):
+    cam = abi.CameraDesc()
+    for k in range(3):
+        cam.origin[k], cam.tangent[k], cam.bitangent[k], cam.normal[k] = origin[k], tangent[k], bitangent[k], normal[k]
+    cam.half_height = float(np.float32(math.tan(fov / 2.0)))
+    cam.half_width = float(np.float32(cam.half_height * width / height))
+    cam.t_min, cam.t_max, cam.time = t_min, t_max, time
+    cam.width, cam.height, cam.visibility_mask, cam.jitter_mode = width, height, mask, abi.JITTER_CENTER
+    return cam
+
+
+def _ray(oracle, abi, cam, px, py):
+    i = abi.SampleStreamIndex(px, py, 0, 0)
+    ray = abi.TransportRay()
+    assert oracle.lib.bfo_camera_ray(C.byref(cam), abi.SAMPLER_INDEXED_HASH, C.byref(i), C.byref(ray)) == 1
+    return ray
+
+
+def test_pinhole_camera_known_answers(oracle, abi):
+    """Tests/Renderer/PinholeCameraTest.cpp:19-76, with the pixel centre (jitter off) standing for
+    the raster samples (1.5, 1.5), (0.5, 0.5) and (1.5, 1.5):
+      - the centre of a 3x3 image looks exactly along -normal from the origin, with the camera's
+        t range, time and visibility mask;
+      - on a 2x2 image with a 90 degree field of view the upper-left sample maps to -x, +y, -z and
+        the lower-right one to the exact mirror in x and y (an x or y flip fails here);
+      - the camera basis maps to world space: normal (1, 0, 0) gives the centre ray (-1, 0, 0)."""
+    cam = _camera(abi, (1.0, -2.0, 3.0), (1, 0, 0), (0, 1, 0), (0, 0, 1), 3, 3, 1.0, 0.25, 100.0, 0.75, 0x00FF00FF)
+    r = _ray(oracle, abi, cam, 1, 1)
+    assert (r.origin_x, r.origin_y, r.origin_z) == (1.0, -2.0, 3.0)
+    assert (r.direction_x, r.direction_y, r.direction_z) == (0.0, 0.0, -1.0)
+    assert (np.float32(r.t_min), np.float32(r.t_max), np.float32(r.time), r.visibility_mask) == \
+        (np.float32(0.25), np.float32(100.0), np.float32(0.75), 0x00FF00FF)
+    cam = _camera(abi, (0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), 2, 2, math.pi / 2)
+    upper_left, lower_right = _ray(oracle, abi, cam, 0, 0), _ray(oracle, abi, cam, 1, 1)
+    assert upper_left.direction_x < 0 and upper_left.direction_y > 0 and upper_left.direction_z < 0
+    assert lower_right.direction_x == -upper_left.direction_x and lower_right.direction_y == -upper_left.direction_y
+    assert lower_right.direction_z == upper_left.direction_z
+    cam = _camera(abi, (4.0, 5.0, 6.0), (0, 1, 0), (0, 0, 1), (1, 0, 0), 1, 1, 1.0, 0.0, 1.0)
+    r = _ray(oracle, abi, cam, 0, 0)
+    assert (r.origin_x, r.origin_y, r.origin_z) == (4.0, 5.0, 6.0)
+    assert (r.direction_x, r.direction_y, r.direction_z) == (-1.0, 0.0, 0.0)
