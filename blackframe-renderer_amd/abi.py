@@ -18,6 +18,12 @@ SCENE_SOA_BAD_COLUMN_COUNT, SCENE_SOA_UNSUPPORTED_HASH_ALGORITHM, SCENE_SOA_NONZ
 SCENE_SOA_INVALID_ENVIRONMENT_COUNT, SCENE_SOA_INVALID_COLUMN_DESCRIPTOR, SCENE_SOA_SIZE_OVERFLOW = 7, 8, 9
 SCENE_SOA_BAD_TOTAL_SIZE, SCENE_SOA_TRUNCATED, SCENE_SOA_HASH_MISMATCH = 10, 11, 12
 SCENE_SOA_SIZE_MISMATCH, SCENE_SOA_COUNT_OVERFLOW, SCENE_SOA_TOO_MANY_MESH_LIGHTS = 13, 14, 15
+# serialized BVH "BLKFBVH1" (bf_bvh_status, BF_BVH_VERIFY_HASH)
+(BVH_VALID, BVH_BAD_MAGIC, BVH_INCOMPATIBLE_VERSION, BVH_BAD_HEADER_SIZE, BVH_BAD_ARRAY_COUNT,
+ BVH_UNSUPPORTED_HASH_ALGORITHM, BVH_NONZERO_RESERVED, BVH_BAD_LEAF_CAPACITY, BVH_INVALID_TOPOLOGY_COUNTS,
+ BVH_INVALID_TLAS_ROOT, BVH_INVALID_ARRAY_DESCRIPTOR, BVH_SIZE_OVERFLOW, BVH_BAD_TOTAL_SIZE, BVH_TRUNCATED,
+ BVH_HASH_MISMATCH) = range(15)
+BVH_VERIFY_HASH = 0x1
 SCENE_SOA_VERIFY_HASH = 0x1
 CLOSURE_LAMBERTIAN, CLOSURE_ROUGH_DIFFUSE, CLOSURE_ROUGH_CONDUCTOR, CLOSURE_ROUGH_DIELECTRIC = 1, 2, 3, 4
 CLOSURE_SPECULAR_REFLECTION, CLOSURE_SPECULAR_TRANSMISSION = 5, 6
@@ -41,6 +47,42 @@ class TransportRay(C.Structure):
         ("direction_x", C.c_float), ("direction_y", C.c_float), ("direction_z", C.c_float), ("t_max", C.c_float),
         ("time", C.c_float), ("visibility_mask", C.c_uint32), ("current_medium", C.c_uint32), ("reserved", C.c_uint32),
     ]
+
+
+class PathStateLane(C.Structure):
+    """bf_path_state_lane: TransportPathStateLane (TransportAbi.hpp:110-134), 128 bytes."""
+    _fields_ = [("beta", C.c_float * 4), ("accumulated_radiance", C.c_float * 4), ("wavelength_nanometers", C.c_float * 4),
+                ("wavelength_pdf_values", C.c_float * 4), ("diffuse_depth", C.c_uint32), ("glossy_depth", C.c_uint32),
+                ("specular_depth", C.c_uint32), ("transmission_depth", C.c_uint32), ("volume_depth", C.c_uint32),
+                ("depth", C.c_uint32), ("eta_scale", C.c_float), ("current_medium", C.c_uint32), ("delta_flags", C.c_uint32),
+                ("wavelength_pdf_measures", C.c_uint8 * 4), ("reserved", C.c_uint32 * 6)]
+
+
+class RayCone(C.Structure):
+    """bf_ray_cone: WavefrontRayCone (WavefrontStageKernel.hpp:146-150)."""
+    _fields_ = [("width", C.c_float), ("spread", C.c_float)]
+
+
+class LaneControl(C.Structure):
+    """bf_lane_control: WavefrontLaneControl (WavefrontStageKernel.hpp:139-144)."""
+    _fields_ = [("phase", C.c_uint32), ("termination", C.c_uint32), ("flags", C.c_uint32),
+                ("blocked_depth_limits", C.c_uint32)]
+
+
+class PathInputs(C.Structure):
+    """bf_path_inputs: WavefrontCameraInputDeviceSoa (device column pointers)."""
+    _fields_ = [("sample_streams", C.c_void_p), ("rays", C.c_void_p), ("ray_cones", C.c_void_p), ("path_states", C.c_void_p),
+                ("count", C.c_uint32), ("reserved", C.c_uint32), ("reserved_tail", C.c_uint32 * 2)]
+
+
+class PathOutputs(C.Structure):
+    """bf_path_outputs: the terminal path states, rays, cones and lane controls (device columns)."""
+    _fields_ = [("path_states", C.c_void_p), ("terminal_rays", C.c_void_p), ("terminal_cones", C.c_void_p),
+                ("controls", C.c_void_p), ("count", C.c_uint32), ("reserved", C.c_uint32), ("reserved_tail", C.c_uint32 * 2)]
+
+
+PHASE_TERMINATED = 8
+WAVELENGTH_MEASURE = 5
 
 
 class ClosestHit(C.Structure):
@@ -331,7 +373,8 @@ for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitR
                        (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 160), (PunctualDesc, 56), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
                        (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48), (LayoutManifest, 320),
                        (TransportOptions, 80), (FrameDesc, 112), (SampleDimensionManifest, 160), (SampleDumpRequest, 40),
-                       (SampleDumpResult, 104), (FrameJob, 184), (TransportReport, 1808), (StageMetric, 24)):
+                       (SampleDumpResult, 104), (PathStateLane, 128), (RayCone, 8), (LaneControl, 16),
+                       (PathInputs, 48), (PathOutputs, 48), (FrameJob, 184), (TransportReport, 1808), (StageMetric, 24)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 
 

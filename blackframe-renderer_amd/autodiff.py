@@ -156,7 +156,8 @@ class TextureFit:
         self.last_forward_report = r["report"]
 
     def loss(self):
-        """Phase 2: per-frame mean squared error and film gradients; returns the loss tensor."""
+        """Phase 2: per-frame mean squared error and film gradients; returns the loss tensor (one
+        buffer, overwritten by the next call: clone it to keep it)."""
         for k, f in enumerate(self.frames):
             hip.film_l2_loss([(f["film"], f["target"], f["film_gradient"])], 1.0 / (3.0 * f["pixels"]),
                              losses=self.losses[k:k + 1])
@@ -183,4 +184,4 @@ class TextureFit:
         losses = self.loss()
         self.adjoint()
         self.update()
-        return losses
+        return losses.clone()

@@ -157,6 +157,7 @@ typedef struct bf_dev_paths {
     uint32_t* status;        /* termination | failure code << 8 */
     uint32_t* visibility;    /* ray visibility mask */
     uint16_t* job;           /* job (frame) of the slot in a batched launch */
+    float2* cone;            /* ray cone (width, spread) of path-input frames */
 } bf_dev_paths;
 
 #endif

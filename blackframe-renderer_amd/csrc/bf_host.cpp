@@ -384,7 +384,10 @@ struct bf_hip_scene {
     std::vector<uint32_t> texture_floats; /* width * height * channels per texture slot */
     bool sampled = false;                 /* holds point-sampled spectra at sampled_wavelengths */
     float sampled_wavelengths[BF_SPECTRUM_LANES] = {};
+    uint64_t source_hash = 0;             /* content hash of the scene blob it was made from, if any */
 };
+
+void bf_scene_set_source_hash(bf_hip_scene* s, uint64_t hash) { s->source_hash = hash; }
 
 namespace {
 
@@ -1007,6 +1010,146 @@ int blackframe_hip_scene_bvh_export(const bf_hip_scene* s, uint64_t counts[5], b
     return BF_OK;
 }
 
+} /* extern "C" */
+
+namespace {
+
+uint64_t bvh_array_count(const bf_bvh_header& h, uint32_t a) {
+    const uint64_t counts[BF_BVH_ARRAY_COUNT] = {h.blas_count, h.blas_node_count, h.primitive_reference_count,
+                                                 h.tlas_node_count, h.instance_reference_count};
+    return counts[a];
+}
+
+uint32_t bvh_array_element_size(uint32_t a) {
+    const uint32_t sizes[BF_BVH_ARRAY_COUNT] = {sizeof(bf_bvh_blas), sizeof(bf_bvh_node), sizeof(uint32_t),
+                                                sizeof(bf_bvh_node), sizeof(bf_bvh_instance_ref)};
+    return sizes[a];
+}
+
+uint64_t bvh_align(uint64_t v) { return (v + BF_BVH_ARRAY_ALIGNMENT - 1u) / BF_BVH_ARRAY_ALIGNMENT * BF_BVH_ARRAY_ALIGNMENT; }
+
+/* validate_scene_bvh_header (SceneBvhAbi.hpp:146-229), in its check order */
+uint32_t validate_bvh_header(const bf_bvh_header& h) {
+    if (h.magic != BF_BVH_MAGIC) return BF_BVH_BAD_MAGIC;
+    if (h.abi_major != BF_BVH_ABI_MAJOR || h.abi_minor != BF_BVH_ABI_MINOR) return BF_BVH_INCOMPATIBLE_VERSION;
+    if (h.header_size != BF_BVH_HEADER_SIZE) return BF_BVH_BAD_HEADER_SIZE;
+    if (h.array_count != BF_BVH_ARRAY_COUNT) return BF_BVH_BAD_ARRAY_COUNT;
+    if (h.hash_algorithm != BF_BVH_HASH_FNV1A64) return BF_BVH_UNSUPPORTED_HASH_ALGORITHM;
+    for (uint64_t r : h.reserved)
+        if (r != 0u) return BF_BVH_NONZERO_RESERVED;
+    if (h.leaf_capacity != BF_BVH_LEAF_CAPACITY) return BF_BVH_BAD_LEAF_CAPACITY;
+    const uint64_t u32max = 0xFFFFFFFFull;
+    if (h.blas_count > u32max || h.blas_node_count > u32max || h.primitive_reference_count > u32max ||
+        h.tlas_node_count > u32max || h.instance_reference_count > u32max ||
+        (h.blas_count == 0u && (h.blas_node_count != 0u || h.primitive_reference_count != 0u)) ||
+        (h.blas_count != 0u && (h.primitive_reference_count < h.blas_count ||
+                                h.blas_node_count != h.primitive_reference_count * 2u - h.blas_count)) ||
+        (h.instance_reference_count == 0u && h.tlas_node_count != 0u) ||
+        (h.instance_reference_count != 0u && h.tlas_node_count != h.instance_reference_count * 2u - 1u))
+        return BF_BVH_INVALID_TOPOLOGY_COUNTS;
+    if ((h.tlas_node_count == 0u && h.tlas_root_node != BF_INVALID_INDEX) || (h.tlas_node_count != 0u && h.tlas_root_node != 0u))
+        return BF_BVH_INVALID_TLAS_ROOT;
+    if (h.total_size_bytes < BF_BVH_HEADER_SIZE) return BF_BVH_BAD_TOTAL_SIZE;
+    const uint64_t top = ~0ull;
+    uint64_t cursor = bvh_align(BF_BVH_HEADER_SIZE);
+    for (uint32_t a = 0; a < BF_BVH_ARRAY_COUNT; ++a) {
+        const bf_bvh_array_descriptor& d = h.arrays[a];
+        const uint64_t count = bvh_array_count(h, a);
+        const uint32_t size = bvh_array_element_size(a);
+        if (d.element_count != count || d.element_size != size || d.reserved != 0u) return BF_BVH_INVALID_ARRAY_DESCRIPTOR;
+        if (count == 0u) {
+            if (d.offset_bytes != 0u) return BF_BVH_INVALID_ARRAY_DESCRIPTOR;
+            continue;
+        }
+        if (cursor > top - (BF_BVH_ARRAY_ALIGNMENT - 1u)) return BF_BVH_SIZE_OVERFLOW;
+        cursor = bvh_align(cursor);
+        if (d.offset_bytes != cursor) return BF_BVH_INVALID_ARRAY_DESCRIPTOR;
+        if (count > top / size) return BF_BVH_SIZE_OVERFLOW;
+        const uint64_t bytes = count * size;
+        if (cursor > top - bytes) return BF_BVH_SIZE_OVERFLOW;
+        cursor += bytes;
+    }
+    return h.total_size_bytes == cursor ? (uint32_t)BF_BVH_VALID : (uint32_t)BF_BVH_BAD_TOTAL_SIZE;
+}
+
+/* FNV-1a-64 with the content-hash field read as zero (SceneBvh.cpp:153-163) */
+uint64_t bvh_blob_hash(const uint8_t* bytes, size_t n) {
+    uint64_t h = BF_FNV1A64_OFFSET_BASIS;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= (i >= BF_BVH_HASH_OFFSET && i < BF_BVH_HASH_OFFSET + 8u) ? 0u : bytes[i];
+        h *= BF_FNV1A64_PRIME;
+    }
+    return h;
+}
+
+} // namespace
+
+extern "C" {
+
+int blackframe_hip_scene_bvh_serialize(const bf_hip_scene* s, void* out, size_t capacity, size_t* size) {
+    if (!s || !size) return set_error(BF_INVALID_ARGUMENT, "scene or size is NULL");
+    /* SceneBvh.cpp:882-1071: header, then the arrays at aligned offsets in their fixed order */
+    bf_bvh_header h{};
+    h.magic = BF_BVH_MAGIC;
+    h.abi_major = BF_BVH_ABI_MAJOR;
+    h.abi_minor = BF_BVH_ABI_MINOR;
+    h.header_size = BF_BVH_HEADER_SIZE;
+    h.array_count = BF_BVH_ARRAY_COUNT;
+    h.hash_algorithm = BF_BVH_HASH_FNV1A64;
+    h.source_scene_hash = s->source_hash;
+    h.blas_count = s->blases.size();
+    h.blas_node_count = s->blas_nodes.size();
+    h.primitive_reference_count = s->prim_refs.size();
+    h.tlas_node_count = s->tlas_nodes.size();
+    h.instance_reference_count = s->inst_refs.size();
+    h.tlas_root_node = s->tlas_nodes.empty() ? BF_INVALID_INDEX : s->tlas_root;
+    h.leaf_capacity = BF_BVH_LEAF_CAPACITY;
+    const void* data[BF_BVH_ARRAY_COUNT] = {s->blases.data(), s->blas_nodes.data(), s->prim_refs.data(),
+                                            s->tlas_nodes.data(), s->inst_refs.data()};
+    uint64_t cursor = bvh_align(BF_BVH_HEADER_SIZE);
+    for (uint32_t a = 0; a < BF_BVH_ARRAY_COUNT; ++a) {
+        bf_bvh_array_descriptor& d = h.arrays[a];
+        d.element_count = bvh_array_count(h, a);
+        d.element_size = bvh_array_element_size(a);
+        if (d.element_count == 0u) continue;
+        cursor = bvh_align(cursor);
+        d.offset_bytes = cursor;
+        cursor += d.element_count * d.element_size;
+    }
+    h.total_size_bytes = cursor;
+    *size = (size_t)cursor;
+    if (!out) return BF_OK;
+    if (capacity < cursor) return set_error(BF_RESOURCE_EXHAUSTED, "BVH blob needs %zu bytes, %zu given", (size_t)cursor, capacity);
+    uint8_t* bytes = static_cast<uint8_t*>(out);
+    std::memset(bytes, 0, (size_t)cursor);
+    for (uint32_t a = 0; a < BF_BVH_ARRAY_COUNT; ++a)
+        if (h.arrays[a].element_count)
+            std::memcpy(bytes + h.arrays[a].offset_bytes, data[a], (size_t)(h.arrays[a].element_count * h.arrays[a].element_size));
+    std::memcpy(bytes, &h, sizeof h);
+    h.content_hash = bvh_blob_hash(bytes, (size_t)cursor);
+    std::memcpy(bytes, &h, sizeof h);
+    if (validate_bvh_header(h) != BF_BVH_VALID) return set_error(BF_INTERNAL_ERROR, "serialized BVH header does not validate");
+    return BF_OK;
+}
+
+int blackframe_hip_scene_bvh_validate(const void* blob, size_t byte_count, uint32_t flags, uint32_t* status) {
+    if (!blob || !status) return set_error(BF_INVALID_ARGUMENT, "BVH blob or status is NULL");
+    if (flags & ~BF_BVH_VERIFY_HASH) return set_error(BF_INVALID_ARGUMENT, "unknown BVH blob flags 0x%x", flags);
+    if (byte_count < BF_BVH_HEADER_SIZE) {
+        *status = BF_BVH_TRUNCATED;
+        return BF_OK;
+    }
+    bf_bvh_header h;
+    std::memcpy(&h, blob, sizeof h);
+    uint32_t st = validate_bvh_header(h);
+    if (st == BF_BVH_VALID && h.total_size_bytes > byte_count) st = BF_BVH_TRUNCATED;
+    if (st == BF_BVH_VALID && (flags & BF_BVH_VERIFY_HASH) &&
+        bvh_blob_hash(static_cast<const uint8_t*>(blob), (size_t)h.total_size_bytes) != h.content_hash)
+        st = BF_BVH_HASH_MISMATCH;
+    *status = st;
+    return BF_OK;
+}
+
 int blackframe_hip_scene_wide_export(const bf_hip_scene* s, uint64_t counts[2], bf_wide_node* blas_wide,
                                      bf_wide_node* tlas_wide) {
     if (!s || !counts) return set_error(BF_INVALID_ARGUMENT, "scene or counts is NULL");
@@ -1077,7 +1220,8 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
                  o_rad = A.add(nullptr, n * 16), o_wl = A.add(nullptr, n * 16), o_dep = A.add(nullptr, n * 16),
                  o_prev = A.add(nullptr, n * 16), o_hit = A.add(nullptr, n * 16), o_ids = A.add(nullptr, n * 8),
                  o_so = A.add(nullptr, n * 16), o_sd = A.add(nullptr, n * 16), o_sc = A.add(nullptr, n * 16),
-                 o_st = A.add(nullptr, n * 4), o_vis = A.add(nullptr, n * 4), o_job = A.add(nullptr, n * 2);
+                 o_st = A.add(nullptr, n * 4), o_vis = A.add(nullptr, n * 4), o_job = A.add(nullptr, n * 2),
+                 o_cone = A.add(nullptr, n * 8);
     size_t o_q[4];
     for (int k = 0; k < 4; ++k) o_q[k] = A.add(nullptr, n * 4);
     const size_t o_bc = A.add(nullptr, (size_t)tiles * 32), o_bo = A.add(nullptr, (size_t)tiles * 32);
@@ -1120,6 +1264,7 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     w->paths.status = (uint32_t*)(b + o_st);
     w->paths.visibility = (uint32_t*)(b + o_vis);
     w->paths.job = (uint16_t*)(b + o_job);
+    w->paths.cone = (float2*)(b + o_cone);
     for (int k = 0; k < 4; ++k) w->queues[k] = (uint32_t*)(b + o_q[k]);
     w->route = b + o_route;
     w->tile_counts = (uint32_t*)(b + o_tc);
@@ -1223,7 +1368,7 @@ int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transpo
  */
 int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_job* adjoints, uint32_t job_count,
                const bf_transport_options* o, bf_transport_report* report, uint64_t* traversal_counters,
-               hipStream_t stream) {
+               hipStream_t stream, const bf_path_inputs* inputs = nullptr, const bf_path_outputs* outputs = nullptr) {
     if (!w || !jobs || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, jobs or options");
     if (job_count == 0) return set_error(BF_INVALID_ARGUMENT, "empty batch");
     const int ost = validate_options(o);
@@ -1250,7 +1395,7 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         if (pixels > w->capacity)
             return set_error(BF_RESOURCE_EXHAUSTED, "workspace capacity %llu is below one sample pass (%llu paths)",
                              (unsigned long long)w->capacity, (unsigned long long)pixels);
-        if (!adjoint && (!J.film || !J.film_compensated || !J.film_counts))
+        if (!adjoint && !inputs && (!J.film || !J.film_compensated || !J.film_counts))
             return set_error(BF_INVALID_ARGUMENT, "job %u: film buffers are required", j);
         if ((J.tape_radiance == nullptr) != (J.tape_visibility == nullptr))
             return set_error(BF_INVALID_ARGUMENT, "job %u: a tape needs both its radiance and visibility buffers", j);
@@ -1402,6 +1547,16 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
             D.paths = J.paths;
             D.tape_radiance = (float4*)J.tape_radiance;
             D.tape_visibility = (unsigned long long*)J.tape_visibility;
+            if (inputs) {
+                D.in_samples = inputs->sample_streams;
+                D.in_rays = inputs->rays;
+                D.in_cones = inputs->ray_cones;
+                D.in_states = inputs->path_states;
+                D.out_states = outputs->path_states;
+                D.out_rays = outputs->terminal_rays;
+                D.out_cones = outputs->terminal_cones;
+                D.out_controls = outputs->controls;
+            }
             if (J.survival_mode != BF_SURVIVAL_NONE) {
                 D.tape_survival = J.tape_survival;
                 D.survival_stride = (uint32_t)vertices;
@@ -1435,13 +1590,21 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         }
         uint32_t* ray_q = w->queues[0];
         uint32_t* cont_q = w->queues[3];
-        STAGE(BF_STAGE_CAMERA, 1, bfk_camera(&P, n, ray_q, batch, adjoint, stream));
+        STAGE(BF_STAGE_CAMERA, 1, bfk_camera(&P, n, inputs ? cont_q : ray_q, w->route, batch, adjoint, stream));
         rep.stage_metrics[BF_STAGE_CAMERA].kernel_lanes += n;
         STAGE(BF_STAGE_AUXILIARY, 2, [&] {
             hipError_t e = bfk_set_words(w->sizes, 1, n, stream);
             if (e == hipSuccess) e = bfk_set_words(w->sizes + 1, 7, 0u, stream);
             return e;
         }());
+        if (inputs) {
+            /* path inputs: only the paths whose inputs validated enter the ray queue (sizes[6] holds
+             * the slot count the compaction reads) */
+            STAGE(BF_STAGE_AUXILIARY, 1, bfk_set_words(w->sizes + 6, 1, n, stream));
+            STAGE(BF_STAGE_COMPACTION, 3,
+                  bfk_compact(w->route, &w->sizes[6], n, cont_q, w->tile_counts, w->tile_offsets, ray_q, &w->sizes[0], 0u,
+                              w->queues[2], &w->sizes[4], 0u, stream));
+        }
         const bool record = o->record_queues && first_launch;
         uint32_t bounce = 0;
         bool live = true;
@@ -1536,6 +1699,10 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         rep.bounces = std::max<uint32_t>(rep.bounces, active_bounces);
         for (size_t c = 0; c < chunk.size() && !adjoint; ++c) {
             const bf_dev_job& D = w->host_jobs[c];
+            if (inputs) {
+                STAGE(BF_STAGE_AUXILIARY, 1, bfk_export_path_records(&w->paths, &D, stream));
+                continue;
+            }
             STAGE(BF_STAGE_FILM, 1, bfk_film(&w->paths, &D, P.film_pdf, normalization, stream));
             rep.stage_metrics[BF_STAGE_FILM].kernel_lanes += D.slot_count;
             if (D.paths) STAGE(BF_STAGE_AUXILIARY, 1, bfk_export_paths(&w->paths, &D, stream));
@@ -1588,6 +1755,33 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* w, const bf_hip_s
     job.film_counts = film_counts;
     job.paths = paths;
     return trace_jobs(w, &job, nullptr, 1, o, report, traversal_counters, (hipStream_t)stream_handle);
+}
+
+int blackframe_hip_trace_wavefront_paths(bf_hip_workspace* w, const bf_hip_scene* s, const bf_path_inputs* inputs,
+                                         const bf_transport_options* o, const bf_path_outputs* outputs,
+                                         bf_transport_report* report, void* stream_handle) {
+    if (!w || !s || !inputs || !o || !outputs)
+        return set_error(BF_INVALID_ARGUMENT, "NULL workspace, scene, inputs, options or outputs");
+    if (!inputs->sample_streams || !inputs->rays || !inputs->ray_cones || !inputs->path_states)
+        return set_error(BF_INVALID_ARGUMENT, "path inputs need all four columns");
+    if (!outputs->path_states || !outputs->terminal_rays || !outputs->terminal_cones || !outputs->controls)
+        return set_error(BF_INVALID_ARGUMENT, "path outputs need all four columns");
+    if (inputs->reserved || inputs->reserved_tail[0] || inputs->reserved_tail[1] || outputs->reserved ||
+        outputs->reserved_tail[0] || outputs->reserved_tail[1])
+        return set_error(BF_INVALID_ARGUMENT, "reserved words of the path inputs and outputs must be zero");
+    if (inputs->count == 0) return set_error(BF_INVALID_ARGUMENT, "no paths");
+    if (outputs->count != inputs->count)
+        return set_error(BF_INVALID_ARGUMENT, "outputs hold %u paths, inputs %u", outputs->count, inputs->count);
+    if (inputs->count > w->capacity)
+        return set_error(BF_RESOURCE_EXHAUSTED, "%u paths exceed the workspace capacity %llu", inputs->count,
+                         (unsigned long long)w->capacity);
+    /* a one-row frame of `count` pixels, one pass: only its extent is read (slot packing) */
+    bf_frame_job job{};
+    job.scene = s;
+    job.frame.camera.width = inputs->count;
+    job.frame.camera.height = 1;
+    job.frame.samples_per_pixel = 1;
+    return trace_jobs(w, &job, nullptr, 1, o, report, nullptr, (hipStream_t)stream_handle, inputs, outputs);
 }
 
 int blackframe_hip_trace_wavefront_batch(bf_hip_workspace* w, const bf_frame_job* jobs, uint32_t job_count,

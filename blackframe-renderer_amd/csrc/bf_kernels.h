@@ -40,7 +40,21 @@ typedef struct bf_dev_job {
     float* tape_survival;       /* roulette survival probabilities [path * survival_stride + vertex] */
     uint32_t survival_stride;
     uint32_t survival_mode;     /* bf_survival_mode */
+    /* path-input frames (blackframe_hip_trace_wavefront_paths): the inputs replace the camera, the
+     * outputs receive the terminal records; all null for camera frames */
+    const bf_sample_stream_index* in_samples;
+    const bf_transport_ray* in_rays;
+    const bf_ray_cone* in_cones;
+    const bf_path_state_lane* in_states;
+    bf_path_state_lane* out_states;
+    bf_transport_ray* out_rays;
+    bf_ray_cone* out_cones;
+    bf_lane_control* out_controls;
 } bf_dev_job;
+
+/* depth.z bit: the slot's ray is still the path's first ray, whose t_max is the camera's or the
+ * input ray's (every later ray runs to infinity) */
+#define BF_FIRST_RAY 0x80000000u
 
 /* Gradients of small textures (<= BF_GRADIENT_SMALL floats, e.g. 1x1 per-material scalars) are
  * accumulated into BF_GRADIENT_REPLICAS copies, one chosen per lane, and summed into the
@@ -88,9 +102,11 @@ typedef struct bf_transport_params {
 extern "C" {
 #endif
 /* adjoint: regenerate the primary rays of a taped frame and load each path's tape and film
- * gradient instead of clearing the tape */
-hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, bool batch, bool adjoint,
-                      hipStream_t s);
+ * gradient instead of clearing the tape. Path-input frames: every slot's input is validated, the
+ * slot written to ray_queue and route[slot] = 1 for a valid input, 0 for a failed one (the caller
+ * compacts the queue with these routes). */
+hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, uint8_t* route, bool batch,
+                      bool adjoint, hipStream_t s);
 hipError_t bfk_trace_closest(const bf_transport_params* P, const uint32_t* queue, const uint32_t* count_ptr,
                              uint32_t max_count, uint8_t* route, unsigned long long* counters, bool batch, hipStream_t s);
 /* mode BF_SHADE_ADJOINT: shadow visibility from the tape (route 1 is never produced) and texel
@@ -122,6 +138,8 @@ hipError_t bfk_sample_stream_dump(const bf_sample_dump_request* requests, uint32
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s);
 hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s);
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
+/* the terminal records of a path-input frame's slots: path state, ray, cone and lane control */
+hipError_t bfk_export_path_records(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
 /* dst[e] += sum over the replica rows of replicas[row * BF_GRADIENT_ROW + offset + e], e < count */
 hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t count, float* dst, hipStream_t s);
 /* Per-array coefficients of one Adam step, rounded once on the host. */

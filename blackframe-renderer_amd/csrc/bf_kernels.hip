@@ -39,6 +39,10 @@ BF_D uint64_t path_of(const bf_dev_job& J, uint32_t slot) { return J.path_base +
 
 BF_D sample_key key_for_slot(const bf_dev_job& J, uint32_t slot) {
     const uint64_t global = path_of(J, slot);
+    if (J.in_samples) {
+        const bf_sample_stream_index in = J.in_samples[global];
+        return sample_key{in.pixel_x, in.pixel_y, in.sample_index, in.seed};
+    }
     const uint64_t s = global / J.pixels;
     const uint32_t rem = (uint32_t)(global - s * J.pixels);
     sample_key k;
@@ -158,8 +162,164 @@ BF_D float4 film_adjoint(const bf_transport_params& P, const bf_dev_job& J, cons
     return make_float4(a[0], a[1], a[2], a[3]);
 }
 
+/* camera_stage_kernel's path-input form (CudaWavefrontStages.cu:1347-1405, validity checks
+ * :528-575): the slot takes its path's ray, ray cone and path state. Depth counters above 255 (kept
+ * in bytes here) and participating media fail the path. */
+BF_D bool valid_input_ray(const bf_transport_ray& r) {
+    const float d2 = r.direction_x * r.direction_x + r.direction_y * r.direction_y + r.direction_z * r.direction_z;
+    return bf_isfinite(r.origin_x) && bf_isfinite(r.origin_y) && bf_isfinite(r.origin_z) && bf_isfinite(r.t_min) &&
+           r.t_min >= 0.0f && !isnan(r.t_max) && r.t_max >= r.t_min && bf_isfinite(r.time) && r.time >= 0.0f &&
+           r.time <= 1.0f && bf_isfinite(r.direction_x) && bf_isfinite(r.direction_y) && bf_isfinite(r.direction_z) &&
+           d2 > 0.0f && r.reserved == 0u;
+}
+
+BF_D bool valid_cone(float w, float s) { return bf_isfinite(w) && w >= 0.0f && bf_isfinite(s) && s >= 0.0f; }
+
+BF_D bool valid_input_state(const bf_path_state_lane& s) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (!bf_isfinite(s.beta.values[k]) || s.beta.values[k] < 0.0f || !bf_isfinite(s.accumulated_radiance.values[k]) ||
+            s.accumulated_radiance.values[k] < 0.0f)
+            return false;
+    if (!bf_isfinite(s.eta_scale) || !(s.eta_scale > 0.0f)) return false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float w = s.wavelength_nanometers.values[k], pdf = s.wavelength_pdf_values.values[k];
+        if (!bf_isfinite(w) || w < 360.0f || w > 830.0f || !bf_isfinite(pdf) || !(pdf > 0.0f) || s.wavelength_pdf_measures[k] != 5u)
+            return false;
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        if (s.reserved[k] != 0u) return false;
+    const uint64_t total = (uint64_t)s.diffuse_depth + s.glossy_depth + s.specular_depth + s.volume_depth;
+    const uint64_t surface = (uint64_t)s.diffuse_depth + s.glossy_depth + s.specular_depth;
+    return total == s.depth && s.transmission_depth <= surface && (s.delta_flags & ~3u) == 0u;
+}
+
+BF_D bool input_lane(const bf_transport_params& P, const bf_dev_job& J, uint32_t slot) {
+    const uint64_t path = path_of(J, slot);
+    const bf_transport_ray r = J.in_rays[path];
+    const bf_ray_cone c = J.in_cones[path];
+    const bf_path_state_lane& s = J.in_states[path];
+    P.paths.status[slot] = BF_TERM_NONE;
+    P.paths.previous[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint32_t code = 0u;
+    if (!valid_input_ray(r)) code = 3u;
+    else if (!valid_cone(c.width, c.spread) || !valid_input_state(s) || r.current_medium != s.current_medium) code = 4u;
+    else if (s.current_medium != 0u || s.volume_depth != 0u) code = 5u;
+    else if (s.diffuse_depth > 255u || s.glossy_depth > 255u || s.specular_depth > 255u || s.transmission_depth > 255u)
+        code = 6u;
+    if (code != 0u) {
+        fail_path(P, J, slot, code);
+        return false;
+    }
+    P.paths.beta[slot] = make_float4(s.beta.values[0], s.beta.values[1], s.beta.values[2], s.beta.values[3]);
+    P.paths.radiance[slot] = make_float4(s.accumulated_radiance.values[0], s.accumulated_radiance.values[1],
+                                         s.accumulated_radiance.values[2], s.accumulated_radiance.values[3]);
+    P.paths.wavelengths[slot] = make_float4(s.wavelength_nanometers.values[0], s.wavelength_nanometers.values[1],
+                                            s.wavelength_nanometers.values[2], s.wavelength_nanometers.values[3]);
+    P.paths.depth[slot] = make_uint4(s.diffuse_depth | (s.glossy_depth << 8) | (s.specular_depth << 16) |
+                                         (s.transmission_depth << 24),
+                                     s.depth, s.delta_flags | BF_FIRST_RAY, __float_as_uint(s.eta_scale));
+    P.paths.visibility[slot] = r.visibility_mask;
+    P.paths.cone[slot] = make_float2(c.width, c.spread);
+    P.paths.ray_origin[slot] = make_float4(r.origin_x, r.origin_y, r.origin_z, r.time);
+    P.paths.ray_direction[slot] = make_float4(r.direction_x, r.direction_y, r.direction_z, r.t_min);
+    return true;
+}
+
+/* ---- ray cones (CudaWavefrontStages.cu:2816-2867, 3675-3782) ---------------------------------- */
+BF_D bool cone_product(float a, float b, float& out) {
+    out = a * b;
+    return bf_isfinite(out) && !(a != 0.0f && b != 0.0f && out == 0.0f);
+}
+
+BF_D bool cone_ratio(float n, float d, float& out) {
+    if (!bf_isfinite(n) || !(n > 0.0f) || !bf_isfinite(d) || !(d > 0.0f)) return false;
+    out = n / d;
+    return bf_isfinite(out) && out > 0.0f;
+}
+
+/* the cone at the hit: width + spread x the distance travelled along the (unnormalised) ray */
+BF_D bool advance_cone(float2 cone, v3f d, float t, float2& out) {
+    if (!valid_cone(cone.x, cone.y) || !bf_isfinite(t) || t < 0.0f) return false;
+    const float scale = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+    if (!bf_isfinite(scale) || !(scale > 0.0f)) return false;
+    const float inv = 1.0f / scale;
+    const float sx = d.x * inv, sy = d.y * inv, sz = d.z * inv;
+    const float nl = sqrtf(sx * sx + sy * sy + sz * sz);
+    float length, distance, growth;
+    if (!bf_isfinite(nl) || !(nl > 0.0f) || !cone_product(scale, nl, length) || !(length > 0.0f)) return false;
+    if (!cone_product(t, length, distance) || distance < 0.0f) return false;
+    if (!cone_product(cone.y, distance, growth)) return false;
+    const float width = fmaf(cone.y, distance, cone.x);
+    if (!bf_isfinite(width) || width < cone.x || (growth != 0.0f && width == cone.x)) return false;
+    out = make_float2(width, cone.y);
+    return valid_cone(out.x, out.y);
+}
+
+/* the cone leaving a scattering event: the spread is floored by the sampled closure's roughness
+ * (1 for diffuse closures); refraction scales the spread by the eta and cosine ratios and the width
+ * by the inverse cosine ratio */
+BF_D bool propagate_cone(float2 cone, const closure_t& c, uint32_t event, v3f wo, v3f wi, float2& out) {
+    if (!valid_cone(cone.x, cone.y) || !valid_surface_event(event) || !unit_vector(wo) || !unit_vector(wi) ||
+        wo.z == 0.0f || wi.z == 0.0f)
+        return false;
+    const bool reflection = (event & BF_LOBE_REFLECTION) != 0u, transmission = (event & BF_LOBE_TRANSMISSION) != 0u;
+    const bool same = (wo.z > 0.0f) == (wi.z > 0.0f);
+    if ((reflection && !same) || (transmission && same)) return false;
+    float floor_spread = 0.0f, spread_scale = 1.0f, width_scale = 1.0f;
+    switch (c.kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        if (!reflection || !(event & BF_LOBE_DIFFUSE)) return false;
+        floor_spread = 1.0f;
+        break;
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        if (!reflection || !(event & BF_LOBE_GLOSSY)) return false;
+        if (!bf_isfinite(c.parameters[8]) || c.parameters[8] < 0.0f || !bf_isfinite(c.parameters[9]) ||
+            c.parameters[9] < 0.0f)
+            return false;
+        floor_spread = fmaxf(c.parameters[8], c.parameters[9]);
+        break;
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        if (!(event & BF_LOBE_GLOSSY)) return false;
+        if (!bf_isfinite(c.parameters[2]) || c.parameters[2] < 0.0f || !bf_isfinite(c.parameters[3]) ||
+            c.parameters[3] < 0.0f)
+            return false;
+        floor_spread = fmaxf(c.parameters[2], c.parameters[3]);
+        break;
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+        if (!reflection || !(event & BF_LOBE_SPECULAR)) return false;
+        break;
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        if (!transmission || !(event & BF_LOBE_SPECULAR)) return false;
+        break;
+    default:
+        return false;
+    }
+    if (!bf_isfinite(floor_spread) || floor_spread < 0.0f) return false;
+    if (transmission) {
+        if (c.kind != BF_CLOSURE_ROUGH_DIELECTRIC && c.kind != BF_CLOSURE_SPECULAR_TRANSMISSION) return false;
+        const float incident = wo.z > 0.0f ? c.parameters[0] : c.parameters[1];
+        const float transmitted = wo.z > 0.0f ? c.parameters[1] : c.parameters[0];
+        float eta_ratio, cosine_ratio, inverse_ratio, meridional;
+        if (!cone_ratio(incident, transmitted, eta_ratio) || !cone_ratio(fabsf(wo.z), fabsf(wi.z), cosine_ratio) ||
+            !cone_ratio(fabsf(wi.z), fabsf(wo.z), inverse_ratio))
+            return false;
+        if (!cone_product(eta_ratio, cosine_ratio, meridional)) return false;
+        spread_scale = fmaxf(eta_ratio, meridional);
+        width_scale = fmaxf(1.0f, inverse_ratio);
+    }
+    float width, spread;
+    if (!cone_product(cone.x, width_scale, width) || !cone_product(cone.y, spread_scale, spread)) return false;
+    out = make_float2(width, fmaxf(spread, floor_spread));
+    return valid_cone(out.x, out.y);
+}
+
 template <bool BATCH, bool ADJ>
-__global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint32_t count, uint32_t* ray_queue) {
+__global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint32_t count, uint32_t* ray_queue,
+                                                     uint8_t* route) {
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     if (slot >= count) return;
     uint32_t j = 0;
@@ -175,13 +335,19 @@ __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint
         P.paths.job[slot] = (uint16_t)j;
     }
     const bf_dev_job& J = BATCH ? P.jobs[j] : P.job0;
+    if (J.in_rays) {
+        /* the ray queue is compacted from these routes: paths whose inputs failed never trace */
+        ray_queue[slot] = slot;
+        route[slot] = input_lane(P, J, slot) ? 1u : 0u;
+        return;
+    }
     const sample_key k = key_for_slot(J, slot);
     v3f o, d;
     ray_queue[slot] = slot;
     P.paths.status[slot] = BF_TERM_NONE;
     P.paths.beta[slot] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
     P.paths.radiance[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    P.paths.depth[slot] = make_uint4(0u, 0u, 0u, __float_as_uint(1.0f));
+    P.paths.depth[slot] = make_uint4(0u, 0u, BF_FIRST_RAY, __float_as_uint(1.0f));
     P.paths.previous[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     P.paths.visibility[slot] = J.camera.visibility_mask;
     float wl[4];
@@ -279,7 +445,9 @@ BF_D uint32_t closest_start(const bf_transport_params& P, const uint32_t* queue,
     ray.o = mk(ro.x, ro.y, ro.z);
     ray.d = mk(rd.x, rd.y, rd.z);
     ray.t_min = rd.w;
-    ray.t_max = (P.paths.depth[L.slot].y == 0u) ? J.camera.t_max : BF_INFINITY_F;
+    ray.t_max = (P.paths.depth[L.slot].z & BF_FIRST_RAY) == 0u
+                    ? BF_INFINITY_F
+                    : (J.in_rays ? J.in_rays[path_of(J, L.slot)].t_max : J.camera.t_max);
     ray.mask = P.paths.visibility[L.slot];
     L.t_min = ray.t_min;
     L.t_max = ray.t_max;
@@ -954,7 +1122,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
     uint4 dstate = P.paths.depth[slot];
     uint32_t diffuse = dstate.x & 0xFFu, glossy = (dstate.x >> 8) & 0xFFu, specular = (dstate.x >> 16) & 0xFFu,
              transmission = (dstate.x >> 24) & 0xFFu;
-    uint32_t depth = dstate.y, delta_flags = dstate.z;
+    uint32_t depth = dstate.y, delta_flags = dstate.z & 3u;
     const uint32_t vertex = depth;
     float eta_scale = __uint_as_float(dstate.w);
     const float4 prev = P.paths.previous[slot];
@@ -1005,7 +1173,8 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
     filter_by_depth(mix, limits, counters, depth, filtered);
     if (filtered.status != BF_L_OK) return fail_path(P, J, slot, 18);
     if (mix.active_count == 0 && filtered.source_count != 0) {
-        P.paths.status[slot] = BF_TERM_DEPTH_LIMIT;
+        /* the depth categories that blocked the path ride above the termination byte */
+        P.paths.status[slot] = BF_TERM_DEPTH_LIMIT | (filtered.blocked << 8);
         return;
     }
     const v3f wo_local = to_local(frame, wo);
@@ -1142,6 +1311,21 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
         beta4 = make_float4(nb[0], nb[1], nb[2], nb[3]);
         P.paths.previous[slot] = delta_event ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
                                              : make_float4(surf.position.x, surf.position.y, surf.position.z, bs.pdf);
+        if (J.in_cones) {
+            /* the sampled closure's compact index, then the cone advanced to the hit and scattered */
+            uint32_t compact = mix.active_count, matches = 0u;
+            for (uint32_t k = 0; k < mix.active_count; ++k)
+                if (filtered.source_indices[k] == bs.selected) {
+                    compact = k;
+                    ++matches;
+                }
+            float2 advanced, next;
+            if (matches != 1u || compact >= mix.active_count) return fail_path(P, J, slot, 43);
+            if (!advance_cone(P.paths.cone[slot], rd, hit.x, advanced)) return fail_path(P, J, slot, 44);
+            if (!propagate_cone(advanced, mix.closures[compact], bs.lobes, wo_local, bs.incoming, next))
+                return fail_path(P, J, slot, 45);
+            P.paths.cone[slot] = next;
+        }
         P.paths.ray_origin[slot] = make_float4(origin.x, origin.y, origin.z, ro4.w);
         P.paths.ray_direction[slot] = make_float4(wi.x, wi.y, wi.z, 0.0f);
         cont_pending = true;
@@ -2237,19 +2421,88 @@ __global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, b
     out[k] = r;
 }
 
+/* The terminal records of the path-input mode (WavefrontTransport.cpp:2753-2786): the path state
+ * from the slot's columns (wavelength pdfs and measures pass through from the input), the ray the
+ * path ended on (its first ray keeps the input's t_max, every later one runs to infinity), the ray
+ * cone and a terminated lane control. A failed path returns zero records and termination NONE; its
+ * failure is in the report. */
+__global__ __launch_bounds__(BF_BLOCK) void k_export_path_records(bf_dev_paths paths, bf_dev_job J) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= J.slot_count) return;
+    const uint32_t i = J.slot_begin + k;
+    const uint64_t path = J.path_base + k;
+    const uint32_t status = paths.status[i];
+    const uint32_t termination = status & 0xFFu;
+    bf_path_state_lane st;
+    bf_transport_ray ray;
+    float2 cone = make_float2(0.0f, 0.0f);
+    memset(&st, 0, sizeof st);
+    memset(&ray, 0, sizeof ray);
+    bf_lane_control ctl;
+    ctl.phase = BF_PHASE_TERMINATED;
+    ctl.termination = 0u;
+    ctl.flags = 0u;
+    ctl.blocked_depth_limits = 0u;
+    if (termination != BF_TERM_FAILED) {
+        const bf_path_state_lane& in = J.in_states[path];
+        const float4 B = paths.beta[i], L = paths.radiance[i], W = paths.wavelengths[i];
+        const uint4 D = paths.depth[i];
+        st.beta.values[0] = B.x;
+        st.beta.values[1] = B.y;
+        st.beta.values[2] = B.z;
+        st.beta.values[3] = B.w;
+        st.accumulated_radiance.values[0] = L.x;
+        st.accumulated_radiance.values[1] = L.y;
+        st.accumulated_radiance.values[2] = L.z;
+        st.accumulated_radiance.values[3] = L.w;
+        st.wavelength_nanometers.values[0] = W.x;
+        st.wavelength_nanometers.values[1] = W.y;
+        st.wavelength_nanometers.values[2] = W.z;
+        st.wavelength_nanometers.values[3] = W.w;
+        st.wavelength_pdf_values = in.wavelength_pdf_values;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) st.wavelength_pdf_measures[w] = in.wavelength_pdf_measures[w];
+        st.diffuse_depth = D.x & 0xFFu;
+        st.glossy_depth = (D.x >> 8) & 0xFFu;
+        st.specular_depth = (D.x >> 16) & 0xFFu;
+        st.transmission_depth = D.x >> 24;
+        st.depth = D.y;
+        st.eta_scale = __uint_as_float(D.w);
+        st.delta_flags = D.z & 3u;
+        const float4 o = paths.ray_origin[i], d = paths.ray_direction[i];
+        ray.origin_x = o.x;
+        ray.origin_y = o.y;
+        ray.origin_z = o.z;
+        ray.t_min = d.w;
+        ray.direction_x = d.x;
+        ray.direction_y = d.y;
+        ray.direction_z = d.z;
+        ray.t_max = (D.z & BF_FIRST_RAY) != 0u ? J.in_rays[path].t_max : BF_INFINITY_F;
+        ray.time = o.w;
+        ray.visibility_mask = paths.visibility[i];
+        cone = paths.cone[i];
+        ctl.termination = termination;
+        ctl.blocked_depth_limits = termination == BF_TERM_DEPTH_LIMIT ? status >> 8 : 0u;
+    }
+    J.out_states[path] = st;
+    J.out_rays[path] = ray;
+    J.out_cones[path] = bf_ray_cone{cone.x, cone.y};
+    J.out_controls[path] = ctl;
+}
+
 /* ---- host-side launch helpers (stream-ordered, never synchronising) -------------------------- */
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
 extern "C" {
 
-hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, bool batch, bool adjoint,
-                      hipStream_t s) {
+hipError_t bfk_camera(const bf_transport_params* P, uint32_t count, uint32_t* ray_queue, uint8_t* route, bool batch,
+                      bool adjoint, hipStream_t s) {
     if (count == 0) return hipSuccess;
     const dim3 grid(blocks_for(count, BF_BLOCK)), block(BF_BLOCK);
-    if (batch && adjoint) hipLaunchKernelGGL((k_camera<true, true>), grid, block, 0, s, *P, count, ray_queue);
-    else if (batch) hipLaunchKernelGGL((k_camera<true, false>), grid, block, 0, s, *P, count, ray_queue);
-    else if (adjoint) hipLaunchKernelGGL((k_camera<false, true>), grid, block, 0, s, *P, count, ray_queue);
-    else hipLaunchKernelGGL((k_camera<false, false>), grid, block, 0, s, *P, count, ray_queue);
+    if (batch && adjoint) hipLaunchKernelGGL((k_camera<true, true>), grid, block, 0, s, *P, count, ray_queue, route);
+    else if (batch) hipLaunchKernelGGL((k_camera<true, false>), grid, block, 0, s, *P, count, ray_queue, route);
+    else if (adjoint) hipLaunchKernelGGL((k_camera<false, true>), grid, block, 0, s, *P, count, ray_queue, route);
+    else hipLaunchKernelGGL((k_camera<false, false>), grid, block, 0, s, *P, count, ray_queue, route);
     return hipGetLastError();
 }
 
@@ -2581,6 +2834,13 @@ hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s) {
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s) {
     if (job->slot_count == 0 || !job->paths) return hipSuccess;
     hipLaunchKernelGGL(k_export_paths, dim3(blocks_for(job->slot_count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, *job);
+    return hipGetLastError();
+}
+
+hipError_t bfk_export_path_records(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s) {
+    if (job->slot_count == 0 || !job->out_states) return hipSuccess;
+    hipLaunchKernelGGL(k_export_path_records, dim3(blocks_for(job->slot_count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths,
+                       *job);
     return hipGetLastError();
 }
 

@@ -20,6 +20,8 @@
 using namespace bf_soa;
 
 int bf_report_error(int status, const char* fmt, ...);
+/* Records the content hash of the blob a scene was made from (bf_host.cpp). */
+void bf_scene_set_source_hash(bf_hip_scene* s, uint64_t hash);
 
 namespace {
 
@@ -386,7 +388,9 @@ int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t
     d.textures = nullptr;
     d.punctual_count = (uint32_t)P;
     d.punctuals = punctuals.empty() ? nullptr : punctuals.data();
-    return blackframe_hip_scene_create(&d, out);
+    const int st = blackframe_hip_scene_create(&d, out);
+    if (st == BF_OK) bf_scene_set_source_hash(*out, h.content_hash);
+    return st;
 }
 
 } // namespace

@@ -50,6 +50,8 @@ def library():
     lib.blackframe_hip_scene_destroy.argtypes = [vp]
     lib.blackframe_hip_scene_bvh_export.argtypes = [vp, C.POINTER(u64), vp, vp, vp, vp, vp, C.POINTER(u32)]
     lib.blackframe_hip_scene_wide_export.argtypes = [vp, C.POINTER(u64), vp, vp]
+    lib.blackframe_hip_scene_bvh_serialize.argtypes = [vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    lib.blackframe_hip_scene_bvh_validate.argtypes = [vp, C.c_size_t, u32, C.POINTER(u32)]
     lib.blackframe_hip_launch_scene_closest_hit.argtypes = [vp, vp, u32, vp, vp, vp]
     lib.blackframe_hip_launch_scene_occlusion.argtypes = [vp, vp, u32, vp, vp]
     lib.blackframe_hip_launch_sample_dump.argtypes = [u32, vp, vp, u32, vp, vp]
@@ -78,6 +80,9 @@ def library():
     lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
                                                              C.POINTER(abi.TransportOptions), vp, vp, vp, vp,
                                                              C.POINTER(abi.TransportReport), vp, vp]
+    lib.blackframe_hip_trace_wavefront_paths.argtypes = [vp, vp, C.POINTER(abi.PathInputs),
+                                                         C.POINTER(abi.TransportOptions), C.POINTER(abi.PathOutputs),
+                                                         C.POINTER(abi.TransportReport), vp]
     _lib = lib
     return lib
 
@@ -153,6 +158,14 @@ class XpuScene:
             "instance_refs": as_bytes(irefs, counts[4], 32),
             "tlas_root": root.value,
         }
+
+    def bvh_blob(self):
+        """The canonical BVH as the reference's serialized "BLKFBVH1" blob (blackframe_hip_scene_bvh_serialize)."""
+        size = C.c_size_t()
+        _check(self._lib.blackframe_hip_scene_bvh_serialize(self.handle, None, 0, C.byref(size)))
+        out = (C.c_uint8 * size.value)()
+        _check(self._lib.blackframe_hip_scene_bvh_serialize(self.handle, out, size.value, C.byref(size)))
+        return bytes(out)
 
     def wide_nodes(self):
         counts = (C.c_uint64 * 2)()
@@ -317,11 +330,59 @@ def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, w
     return result
 
 
+def _device_records(records):
+    """A ctypes record array copied into a CUDA uint8 tensor, or a CUDA tensor passed through."""
+    import torch
+
+    if isinstance(records, torch.Tensor):
+        if not records.is_cuda or not records.is_contiguous():
+            raise ValueError("device record columns must be contiguous CUDA tensors")
+        return records
+    raw = np.frombuffer(bytes(memoryview(records).cast("B")), dtype=np.uint8)
+    return torch.from_numpy(raw.copy()).cuda()
+
+
+def trace_xpu_wavefront_paths(workspace, scene, inputs, options, stream=None, synchronize=True, want_host=True):
+    """The path-input transport (blackframe_hip_trace_wavefront_paths); mirrors
+    engine::trace_cuda_wavefront_transport(workspace, scene, bvh, inputs, light_sampler, options)
+    (WavefrontTransport.hpp:197-208).
+
+    inputs: (sample streams, rays, ray cones, path states), each a ctypes array of SampleStreamIndex /
+    TransportRay / RayCone / PathStateLane (copied to the device) or a CUDA uint8 tensor of those
+    records. Returns {"states", "rays", "cones", "controls"} (ctypes arrays when want_host, else CUDA
+    uint8 tensors), "device" (the four output tensors) and "report"."""
+    import torch
+
+    samples, rays, cones, states = inputs
+    sizes = (C.sizeof(abi.SampleStreamIndex), C.sizeof(abi.TransportRay), C.sizeof(abi.RayCone),
+             C.sizeof(abi.PathStateLane))
+    columns = [_device_records(x) for x in (samples, rays, cones, states)]
+    n = columns[1].numel() // sizes[1]
+    for col, size in zip(columns, sizes):
+        if col.numel() != n * size:
+            raise ValueError("path input columns differ in length")
+    record = (abi.PathStateLane, abi.TransportRay, abi.RayCone, abi.LaneControl)
+    outs = [torch.empty(n * C.sizeof(t), dtype=torch.uint8, device="cuda") for t in record]
+    pin = abi.PathInputs(columns[0].data_ptr(), columns[1].data_ptr(), columns[2].data_ptr(), columns[3].data_ptr(), n)
+    pout = abi.PathOutputs(outs[0].data_ptr(), outs[1].data_ptr(), outs[2].data_ptr(), outs[3].data_ptr(), n)
+    report = abi.TransportReport()
+    _check(library().blackframe_hip_trace_wavefront_paths(workspace.handle, scene.handle, C.byref(pin), C.byref(options),
+                                                          C.byref(pout), C.byref(report), _stream_handle(stream)))
+    if synchronize or want_host:
+        torch.cuda.synchronize()
+    result = {"device": outs, "report": report}
+    for name, t, out in zip(("states", "rays", "cones", "controls"), record, outs):
+        result[name] = (t * n).from_buffer_copy(out.cpu().numpy().tobytes()) if want_host else out
+    return result
+
+
 def trace_xpu_wavefront_batch(workspace, items, options, want_paths=False, want_counters=False, stream=None,
                               synchronize=True):
     """Renders many frames through one set of queues (blackframe_hip_trace_wavefront_batch).
 
-    items: sequence of (XpuScene, FrameDesc[, Film[, Tape[, Survival]]]). Returns {"films": [...], "report": ...,
+    items: sequence of (XpuScene, FrameDesc[, Film[, Tape[, Survival]]]). want_paths as in
+    trace_xpu_wavefront_transport ("device": one torch uint8 tensor of n_paths * 64 bytes per frame).
+    Returns {"films": [...], "report": ...,
     "paths": [...] (per frame, as in trace_xpu_wavefront_transport), "traversal_counters": ...}."""
     import torch
 
@@ -358,7 +419,9 @@ def trace_xpu_wavefront_batch(workspace, items, options, want_paths=False, want_
     result = {"films": films, "report": report}
     if want_counters:
         result["traversal_counters"] = counters
-    if want_paths:
+    if want_paths == "device":
+        result["paths"] = [d for d, _ in path_buffers]
+    elif want_paths:
         result["paths"] = [(abi.PathResult * n).from_buffer_copy(d.cpu().numpy().tobytes()) for d, n in path_buffers]
     return result
 
@@ -424,6 +487,15 @@ def scene_soa_validate(blob, verify_hash=True):
     data = bytes(blob)
     status = C.c_uint32()
     _check(library().blackframe_hip_scene_soa_validate(data, len(data), abi.SCENE_SOA_VERIFY_HASH if verify_hash else 0,
+                                                       C.byref(status)))
+    return status.value
+
+
+def scene_bvh_validate(blob, verify_hash=True):
+    """blackframe_hip_scene_bvh_validate on host bytes: a bf_bvh_status value."""
+    data = bytes(blob)
+    status = C.c_uint32()
+    _check(library().blackframe_hip_scene_bvh_validate(data, len(data), abi.BVH_VERIFY_HASH if verify_hash else 0,
                                                        C.byref(status)))
     return status.value
 
@@ -500,6 +572,7 @@ def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None):
     need = queue_compaction_scratch_bytes(len(outcomes))
     size = need if scratch_bytes is None else scratch_bytes
     view = abi.QueueSoa(d_headers.data_ptr(), d_slots.data_ptr() if d_slots is not None else None, abi.QUEUE_COUNT, stride)
+    d_scratch = torch.zeros(max(size, 1), dtype=torch.uint8, device="cuda")
     _check(library().blackframe_hip_launch_wavefront_queue_compaction(
         view, C.c_void_p(d_outcomes.data_ptr()) if d_outcomes is not None else None, len(outcomes), route,
         C.c_void_p(d_scratch.data_ptr()), size, C.c_void_p(d_result.data_ptr()), _stream_handle()))

@@ -53,6 +53,25 @@ int blackframe_hip_scene_bvh_export(const bf_hip_scene* scene, uint64_t counts[5
                                     bf_bvh_node* blas_nodes, uint32_t* primitive_refs, bf_bvh_node* tlas_nodes,
                                     bf_bvh_instance_ref* instance_refs, uint32_t* tlas_root);
 
+/*
+ * Writes the scene's canonical BVH as the reference's serialized "BLKFBVH1" blob (host memory):
+ * header, the five arrays at 16-byte aligned offsets, content hash. source_scene_hash is the
+ * content hash of the scene blob for scenes made by blackframe_hip_scene_create_from_soa, 0 for
+ * scenes made from a bf_scene_desc. *size receives the blob's byte count; out == NULL queries it
+ * only, and a capacity below it returns BF_RESOURCE_EXHAUSTED. Host only.
+ * Replaces: the bytes engine::CudaSceneBvh::build serializes and uploads (Backends/GPU/CUDA/Source/
+ *           SceneBvh.cpp:996-1071, 1109-1146; layout SceneBvhAbi.hpp:37-98).
+ */
+int blackframe_hip_scene_bvh_serialize(const bf_hip_scene* scene, void* out, size_t capacity, size_t* size);
+
+/*
+ * Checks a host "BLKFBVH1" blob: validate_scene_bvh_header's checks in its order, then (no
+ * reference counterpart) that `byte_count` covers total_size_bytes and, with BF_BVH_VERIFY_HASH, the
+ * content hash. *status receives a bf_bvh_status; the call returns BF_OK whenever it could decide.
+ * Replaces: validate_scene_bvh_header (XPU/Shared/Include/Blackframe/XPU/Shared/SceneBvhAbi.hpp:146-229).
+ */
+int blackframe_hip_scene_bvh_validate(const void* blob, size_t byte_count, uint32_t flags, uint32_t* status);
+
 /* Copies the device-resident 64-byte wide nodes (host memory): counts = {blas wide, tlas wide}. */
 int blackframe_hip_scene_wide_export(const bf_hip_scene* scene, uint64_t counts[2], bf_wide_node* blas_wide,
                                      bf_wide_node* tlas_wide);
@@ -119,6 +138,29 @@ int blackframe_hip_trace_wavefront_transport(bf_hip_workspace* workspace, const 
                                              float* film, double* film_compensated, uint32_t* film_counts,
                                              bf_path_result* paths, bf_transport_report* report,
                                              uint64_t* traversal_counters, void* stream);
+
+/*
+ * The path-input form of the transport: per path a sample stream address, a primary ray, its ray
+ * cone and an initial path state (throughput, radiance, four wavelengths with their pdfs and
+ * measures, depth counters, eta scale, delta flags) come in as device columns; the camera stage
+ * validates and loads them, the bounce loop is the frame transport's, and every path returns its
+ * terminal path state (wavelength pdfs and measures passed through), the ray it ended on (its
+ * sampled continuation when roulette or zero throughput ended it; its last traced ray otherwise),
+ * its ray cone advanced to each hit and scattered (spread floored by the sampled closure's
+ * roughness, 1 for diffuse; refraction scales by the eta and cosine ratios) and a lane control
+ * (phase terminated, termination, the depth categories that blocked a depth-limited path).
+ * Invalid inputs (negative t_min, non-unit wavelength measures, media) fail only their paths,
+ * which return zero records and termination NONE; failures are counted in `report` under the
+ * options' failure policy. inputs->count == outputs->count <= the workspace capacity; the light
+ * sampler is the options' strategy. No film is written.
+ * Replaces: engine::trace_cuda_wavefront_transport(workspace, scene, bvh, inputs, light_sampler,
+ *           options) (Backends/GPU/CUDA/Include/Blackframe/Backends/GPU/CUDA/WavefrontTransport.hpp:
+ *           197-208), whose CudaWavefrontPathInput span and CudaWavefrontTransportBatch (paths and
+ *           terminal cones) these columns carry.
+ */
+int blackframe_hip_trace_wavefront_paths(bf_hip_workspace* workspace, const bf_hip_scene* scene,
+                                         const bf_path_inputs* inputs, const bf_transport_options* options,
+                                         const bf_path_outputs* outputs, bf_transport_report* report, void* stream);
 
 /*
  * One frame of a batch: a scene (on the current device), its frame description and its device

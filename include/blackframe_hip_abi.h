@@ -186,6 +186,118 @@ typedef struct BF_ALIGN16 bf_layout_manifest {
     uint32_t reserved_tail[3];
 } bf_layout_manifest;
 
+/* ---- wavefront stage records (WavefrontStageKernel.hpp:12-203) ------------------------------------
+ * The reference's per-stage lane records, layout-identical. The fused kernels of this library keep
+ * the same information in their own SoA columns; these records are the form the path-input entry
+ * point (blackframe_hip_trace_wavefront_paths) takes and returns and the stage audit reports. */
+enum bf_stage_status {
+    BF_STAGE_SUCCESS = 0, BF_STAGE_INVALID_CONTRACT = 1, BF_STAGE_INVALID_PATH_SLOT = 2, BF_STAGE_INVALID_LANE_STATE = 3,
+    BF_STAGE_INVALID_RAY = 4, BF_STAGE_INVALID_SCENE = 5, BF_STAGE_UNSUPPORTED_TRANSPORT = 6,
+    BF_STAGE_NUMERICAL_FAILURE = 7, BF_STAGE_QUEUE_OVERFLOW = 8, BF_STAGE_TRAVERSAL_ERROR = 9
+};
+enum bf_stage_route {
+    BF_ROUTE_NONE = 0, BF_ROUTE_RAY = 1, BF_ROUTE_HIT = 2, BF_ROUTE_MISS = 3, BF_ROUTE_SHADE = 4, BF_ROUTE_SHADOW = 5,
+    BF_ROUTE_CONTINUATION = 6, BF_ROUTE_TERMINATED = 7
+};
+enum bf_lane_phase {
+    BF_PHASE_EMPTY = 0, BF_PHASE_CAMERA = 1, BF_PHASE_RAY = 2, BF_PHASE_HIT = 3, BF_PHASE_MISS = 4, BF_PHASE_SHADE = 5,
+    BF_PHASE_SHADOW = 6, BF_PHASE_CONTINUATION = 7, BF_PHASE_TERMINATED = 8, BF_PHASE_PROCESSING = 9
+};
+enum bf_stage_kind {
+    BF_STAGE_KIND_CAMERA_SEED = 0, BF_STAGE_KIND_CAMERA = 1, BF_STAGE_KIND_INTERSECTION_GATHER = 2,
+    BF_STAGE_KIND_INTERSECTION_CLASSIFY = 3, BF_STAGE_KIND_HIT = 4, BF_STAGE_KIND_MISS = 5, BF_STAGE_KIND_SHADE = 6,
+    BF_STAGE_KIND_SHADOW_GATHER = 7, BF_STAGE_KIND_SHADOW_PROCESS = 8, BF_STAGE_KIND_CONTINUATION = 9
+};
+#define BF_LANE_CONTINUATION_PENDING (1u << 0)
+#define BF_LANE_SHADOW_PENDING (1u << 1)
+
+/* WavefrontTransportConfig (64 B) */
+typedef struct BF_ALIGN16 bf_transport_config {
+    uint16_t abi_major, abi_minor;
+    uint32_t struct_size;
+    uint32_t mis_heuristic;
+    uint32_t light_sampling_strategy;
+    uint32_t light_count;
+    uint32_t diffuse_depth_limit, glossy_depth_limit, specular_depth_limit, transmission_depth_limit, volume_depth_limit;
+    uint32_t russian_roulette_mode;
+    uint32_t russian_roulette_first_depth;
+    float russian_roulette_minimum_probability;
+    float russian_roulette_maximum_probability;
+    uint32_t reserved[2];
+} bf_transport_config;
+
+/* WavefrontStageAudit (64 B): the first failure is the failing outcome with the smallest work index
+ * (UINT32_MAX when the stage succeeded). */
+typedef struct BF_ALIGN16 bf_stage_audit {
+    uint16_t abi_major, abi_minor;
+    uint32_t struct_size;
+    uint32_t stage_kind;
+    uint32_t expected_work_count;
+    uint32_t inspected_work_count;
+    uint32_t first_failure_work_index;
+    uint32_t closure_samples;
+    uint32_t light_samples;
+    uint32_t first_failure[4]; /* bf_stage_outcome */
+    uint32_t reserved[4];
+} bf_stage_audit;
+
+/* WavefrontLaneControl (16 B): phase, termination (bf_termination), flags, blocked depth limits
+ * (the lobes the depth filter removed when the path ended at its depth limit). */
+typedef struct BF_ALIGN16 bf_lane_control {
+    uint32_t phase;
+    uint32_t termination;
+    uint32_t flags;
+    uint32_t blocked_depth_limits;
+} bf_lane_control;
+
+/* WavefrontRayCone (8 B): width at the ray origin and spread angle. */
+typedef struct bf_ray_cone {
+    float width;
+    float spread;
+} bf_ray_cone;
+
+/* WavefrontPendingShadow (128 B) */
+typedef struct BF_ALIGN16 bf_pending_shadow {
+    bf_transport_ray ray;
+    bf_transport_spectrum beta, bsdf_factor, incident_radiance;
+    float absolute_incoming_cosine, estimator_weight, selection_probability, conditional_probability;
+    float shading_normal_correction;
+    uint32_t continuation_pending, termination, bsdf_encoding;
+} bf_pending_shadow;
+
+/* WavefrontPreviousBsdfSample (48 B) */
+typedef struct BF_ALIGN16 bf_previous_bsdf_sample {
+    float context_x, context_y, context_z, context_time;
+    float incoming_x, incoming_y, incoming_z;
+    float probability_value;
+    uint32_t probability_measure;
+    uint32_t valid;
+    uint32_t reserved[2];
+} bf_previous_bsdf_sample;
+
+/* WavefrontCameraInputDeviceSoa (48 B): the path inputs as four device columns. */
+typedef struct BF_ALIGN16 bf_path_inputs {
+    const bf_sample_stream_index* sample_streams;
+    const bf_transport_ray* rays;
+    const bf_ray_cone* ray_cones;
+    const bf_path_state_lane* path_states;
+    uint32_t count;
+    uint32_t reserved;
+    uint32_t reserved_tail[2];
+} bf_path_inputs;
+
+/* WavefrontStageDeviceSoa restricted to what a finished path returns (the per-path outputs of the
+ * path-input entry point, four device columns of `count` records). */
+typedef struct BF_ALIGN16 bf_path_outputs {
+    bf_path_state_lane* path_states;
+    bf_transport_ray* terminal_rays;
+    bf_ray_cone* terminal_cones;
+    bf_lane_control* controls;
+    uint32_t count;
+    uint32_t reserved;
+    uint32_t reserved_tail[2];
+} bf_path_outputs;
+
 /* ---- scene blob "BLKFSOA4" (SceneSoaAbi.hpp:11-19, 141-189, 350) ------------------------------ */
 #define BF_SCENE_SOA_HEADER_SIZE 4848u          /* sizeof(SceneSoaHeader) */
 #define BF_SCENE_SOA_HASH_OFFSET 32u            /* offsetof(SceneSoaHeader, content_hash) */
@@ -362,6 +474,63 @@ typedef struct bf_bvh_instance_ref {
     uint32_t blas_index, visibility_mask;
     uint32_t reserved[2];
 } bf_bvh_instance_ref;
+
+/* ---- serialized BVH "BLKFBVH1" (SceneBvhAbi.hpp:11-98, 116-229) -------------------------------
+ * header, then five 16-byte aligned arrays in this order: BLAS, BLAS nodes, primitive references,
+ * TLAS nodes, instance references; FNV-1a-64 of every byte with content_hash read as zero. */
+#define BF_BVH_MAGIC 0x31485642464B4C42ull /* "BLKFBVH1" */
+#define BF_BVH_ABI_MAJOR 1u
+#define BF_BVH_ABI_MINOR 0u
+#define BF_BVH_HASH_FNV1A64 1u
+#define BF_BVH_ARRAY_ALIGNMENT 16u
+#define BF_BVH_LEAF_CAPACITY 1u
+#define BF_BVH_ARRAY_COUNT 5u
+#define BF_BVH_HEADER_SIZE 256u
+#define BF_BVH_HASH_OFFSET 32u
+
+typedef struct bf_bvh_array_descriptor {
+    uint64_t offset_bytes;
+    uint64_t element_count;
+    uint32_t element_size;
+    uint32_t reserved;
+} bf_bvh_array_descriptor;
+
+typedef struct BF_ALIGN16 bf_bvh_header {
+    uint64_t magic;
+    uint16_t abi_major, abi_minor;
+    uint32_t header_size;
+    uint32_t array_count;
+    uint32_t hash_algorithm;
+    uint64_t total_size_bytes;
+    uint64_t content_hash;
+    uint64_t source_scene_hash; /* content hash of the scene blob the BVH was built from */
+    uint64_t blas_count, blas_node_count, primitive_reference_count, tlas_node_count, instance_reference_count;
+    uint32_t tlas_root_node;
+    uint32_t leaf_capacity;
+    bf_bvh_array_descriptor arrays[BF_BVH_ARRAY_COUNT];
+    uint64_t reserved[5];
+} bf_bvh_header;
+
+/* SceneBvhHeaderValidationStatus (SceneBvhAbi.hpp:104-118), same values and check order, then the
+ * blob checks of blackframe_hip_scene_bvh_validate (no reference counterpart). */
+enum bf_bvh_status {
+    BF_BVH_VALID = 0,
+    BF_BVH_BAD_MAGIC = 1,
+    BF_BVH_INCOMPATIBLE_VERSION = 2,
+    BF_BVH_BAD_HEADER_SIZE = 3,
+    BF_BVH_BAD_ARRAY_COUNT = 4,
+    BF_BVH_UNSUPPORTED_HASH_ALGORITHM = 5,
+    BF_BVH_NONZERO_RESERVED = 6,
+    BF_BVH_BAD_LEAF_CAPACITY = 7,
+    BF_BVH_INVALID_TOPOLOGY_COUNTS = 8,
+    BF_BVH_INVALID_TLAS_ROOT = 9,
+    BF_BVH_INVALID_ARRAY_DESCRIPTOR = 10,
+    BF_BVH_SIZE_OVERFLOW = 11,
+    BF_BVH_BAD_TOTAL_SIZE = 12,
+    BF_BVH_TRUNCATED = 13,    /* fewer bytes than the header or than total_size_bytes */
+    BF_BVH_HASH_MISMATCH = 14 /* content_hash differs from the FNV-1a-64 of the bytes */
+};
+#define BF_BVH_VERIFY_HASH 0x1u
 
 /*
  * Device-resident wide node (this backend's layout). One 64-byte record per INTERNAL canonical
@@ -658,7 +827,30 @@ BF_STATIC_ASSERT(sizeof(bf_sample_dump_request) == 40, "SampleStreamDumpRequest 
 BF_STATIC_ASSERT(sizeof(bf_sample_dump_result) == 104, "SampleStreamDumpResult is 104 bytes");
 BF_STATIC_ASSERT(offsetof(bf_sample_dump_result, bounce_dimensions) == 24, "dump result dimensions at 24");
 BF_STATIC_ASSERT(sizeof(bf_compaction_result) == 48, "WavefrontQueueCompactionResult is 48 bytes");
+BF_STATIC_ASSERT(sizeof(bf_transport_config) == 64, "WavefrontTransportConfig is 64 bytes");
+BF_STATIC_ASSERT(offsetof(bf_transport_config, russian_roulette_mode) == 40, "config roulette mode at 40");
+BF_STATIC_ASSERT(offsetof(bf_transport_config, reserved) == 56, "config reserved at 56");
+BF_STATIC_ASSERT(sizeof(bf_stage_audit) == 64, "WavefrontStageAudit is 64 bytes");
+BF_STATIC_ASSERT(offsetof(bf_stage_audit, first_failure) == 32, "audit first failure at 32");
+BF_STATIC_ASSERT(sizeof(bf_lane_control) == 16, "WavefrontLaneControl is 16 bytes");
+BF_STATIC_ASSERT(sizeof(bf_ray_cone) == 8, "WavefrontRayCone is 8 bytes");
+BF_STATIC_ASSERT(sizeof(bf_pending_shadow) == 128, "WavefrontPendingShadow is 128 bytes");
+BF_STATIC_ASSERT(offsetof(bf_pending_shadow, absolute_incoming_cosine) == 96, "pending shadow cosine at 96");
+BF_STATIC_ASSERT(offsetof(bf_pending_shadow, bsdf_encoding) == 124, "pending shadow encoding at 124");
+BF_STATIC_ASSERT(sizeof(bf_previous_bsdf_sample) == 48, "WavefrontPreviousBsdfSample is 48 bytes");
+BF_STATIC_ASSERT(offsetof(bf_previous_bsdf_sample, valid) == 36, "previous sample valid at 36");
+BF_STATIC_ASSERT(sizeof(bf_path_inputs) == 48, "WavefrontCameraInputDeviceSoa is 48 bytes");
+BF_STATIC_ASSERT(offsetof(bf_path_inputs, count) == 32, "path inputs count at 32");
+BF_STATIC_ASSERT(sizeof(bf_path_outputs) == 48, "path outputs are 48 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_node) == 48, "SceneBvhNode is 48 bytes");
+BF_STATIC_ASSERT(sizeof(bf_bvh_array_descriptor) == 24, "SceneBvhArrayDescriptor is 24 bytes");
+BF_STATIC_ASSERT(sizeof(bf_bvh_header) == BF_BVH_HEADER_SIZE, "SceneBvhHeader is 256 bytes");
+BF_STATIC_ASSERT(offsetof(bf_bvh_header, total_size_bytes) == 24, "BVH total size at 24");
+BF_STATIC_ASSERT(offsetof(bf_bvh_header, content_hash) == BF_BVH_HASH_OFFSET, "BVH content hash at 32");
+BF_STATIC_ASSERT(offsetof(bf_bvh_header, source_scene_hash) == 40, "BVH source hash at 40");
+BF_STATIC_ASSERT(offsetof(bf_bvh_header, tlas_root_node) == 88, "BVH TLAS root at 88");
+BF_STATIC_ASSERT(offsetof(bf_bvh_header, arrays) == 96, "BVH arrays at 96");
+BF_STATIC_ASSERT(offsetof(bf_bvh_header, reserved) == 216, "BVH reserved at 216");
 BF_STATIC_ASSERT(sizeof(bf_bvh_blas) == 32, "SceneBvhBlas is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_bvh_instance_ref) == 32, "SceneBvhInstanceReference is 32 bytes");
 BF_STATIC_ASSERT(sizeof(bf_wide_node) == 64, "wide node is one 64-byte line");

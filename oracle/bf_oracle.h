@@ -89,6 +89,13 @@ int bfo_render_survival(const bfo_scene* scene, const bf_frame_desc* frame, cons
                         uint64_t path_begin, uint64_t path_count, uint32_t threads, uint32_t survival_mode, float* survival,
                         bf_path_result* paths, float* film_float, double* film_compensated, uint32_t* film_counts,
                         float* tape_radiance, uint64_t* tape_visibility);
+/* The path-input transport (trace_cuda_wavefront_transport's contract): `count` paths start from
+ * the given sample streams, rays, ray cones and path states (host arrays) instead of the camera and
+ * end in the terminal path states, rays, cones and lane controls. */
+int bfo_render_paths(const bfo_scene* scene, const bf_transport_options* options, uint64_t count, uint32_t threads,
+                     const bf_sample_stream_index* samples, const bf_transport_ray* rays, const bf_ray_cone* cones,
+                     const bf_path_state_lane* states, bf_path_state_lane* out_states, bf_transport_ray* out_rays,
+                     bf_ray_cone* out_cones, bf_lane_control* out_controls, bf_transport_report* report);
 /* Path-replay adjoint of paths [path_begin, path_begin + path_count): texel gradients of
  * dLoss/d(film RGB sums) = film_gradient, ACCUMULATED in binary64 into texture_gradients[k]
  * (layout of scene texture k; NULL entries skipped). */

@@ -28,6 +28,12 @@ typedef struct {
     uint32_t diffuse, glossy, specular, transmission, volume, depth;
     float eta_scale;
     uint32_t delta_flags;
+    /* path-input mode: the ray cone, the wavelength pdfs and measures carried through, and the
+     * lobes the depth filter removed when the path ended at its depth limit */
+    bf_ray_cone cone;
+    float wl_pdfs[4];
+    uint8_t wl_measures[4];
+    uint32_t blocked;
     /* closest hit */
     bf_closest_hit hit;
     uint32_t hit_instance, hit_geometry;
@@ -67,6 +73,15 @@ typedef struct {
     /* roulette survival probabilities by path vertex [global path * stride + vertex] */
     float* survival;
     uint32_t survival_stride, survival_mode;
+    /* path-input mode: per-path inputs replace the camera, outputs receive the terminal records */
+    const bf_sample_stream_index* in_samples;
+    const bf_transport_ray* in_rays;
+    const bf_ray_cone* in_cones;
+    const bf_path_state_lane* in_states;
+    bf_path_state_lane* out_states;
+    bf_transport_ray* out_rays;
+    bf_ray_cone* out_cones;
+    bf_lane_control* out_controls;
 } ctx;
 
 static void fail(lane* l, uint32_t code) {
@@ -253,6 +268,161 @@ static void camera_lane(const ctx* c, lane* l, uint64_t global_path) {
         l->visible = c->tape_visibility[global_path];
         film_adjoint(c, l);
     }
+}
+
+/* ---- path inputs (CudaWavefrontStages.cu:528-575, 1347-1405) ---------------------------------- */
+static int valid_input_ray(const bf_transport_ray* r) {
+    const float d2 = r->direction_x * r->direction_x + r->direction_y * r->direction_y + r->direction_z * r->direction_z;
+    return bf_isfinite(r->origin_x) && bf_isfinite(r->origin_y) && bf_isfinite(r->origin_z) && bf_isfinite(r->t_min) &&
+           r->t_min >= 0.0f && !isnan(r->t_max) && r->t_max >= r->t_min && bf_isfinite(r->time) && r->time >= 0.0f &&
+           r->time <= 1.0f && bf_isfinite(r->direction_x) && bf_isfinite(r->direction_y) && bf_isfinite(r->direction_z) &&
+           d2 > 0.0f && r->reserved == 0u;
+}
+
+static int valid_cone(bf_ray_cone c) {
+    return bf_isfinite(c.width) && c.width >= 0.0f && bf_isfinite(c.spread) && c.spread >= 0.0f;
+}
+
+static int valid_input_state(const bf_path_state_lane* s) {
+    for (int k = 0; k < 4; ++k) {
+        if (!bf_isfinite(s->beta.values[k]) || s->beta.values[k] < 0.0f || !bf_isfinite(s->accumulated_radiance.values[k]) ||
+            s->accumulated_radiance.values[k] < 0.0f)
+            return 0;
+    }
+    if (!bf_isfinite(s->eta_scale) || !(s->eta_scale > 0.0f)) return 0;
+    for (int k = 0; k < 4; ++k) {
+        const float w = s->wavelength_nanometers.values[k], pdf = s->wavelength_pdf_values.values[k];
+        if (!bf_isfinite(w) || w < 360.0f || w > 830.0f || !bf_isfinite(pdf) || !(pdf > 0.0f) ||
+            s->wavelength_pdf_measures[k] != 5u)
+            return 0;
+    }
+    for (int k = 0; k < 6; ++k)
+        if (s->reserved[k] != 0u) return 0;
+    const uint64_t total = (uint64_t)s->diffuse_depth + s->glossy_depth + s->specular_depth + s->volume_depth;
+    const uint64_t surface = (uint64_t)s->diffuse_depth + s->glossy_depth + s->specular_depth;
+    return total == s->depth && s->transmission_depth <= surface && (s->delta_flags & ~3u) == 0u;
+}
+
+/* camera_stage_kernel in input mode: the lane takes its path's sample stream, ray, cone and state.
+ * Depth counters above 255 (this backend keeps them in bytes) fail the path. */
+static void input_lane(const ctx* c, lane* l, uint64_t index) {
+    memset(l, 0, sizeof *l);
+    l->global_path = index;
+    l->sample = c->in_samples[index];
+    const bf_transport_ray* r = &c->in_rays[index];
+    const bf_path_state_lane* s = &c->in_states[index];
+    if (!valid_input_ray(r)) return fail(l, 3);
+    if (!valid_cone(c->in_cones[index]) || !valid_input_state(s) || r->current_medium != s->current_medium)
+        return fail(l, 4);
+    if (s->current_medium != 0u || s->volume_depth != 0u) return fail(l, 5);
+    if (s->diffuse_depth > 255u || s->glossy_depth > 255u || s->specular_depth > 255u || s->transmission_depth > 255u)
+        return fail(l, 6);
+    l->ray = *r;
+    l->cone = c->in_cones[index];
+    memcpy(l->beta, s->beta.values, sizeof l->beta);
+    memcpy(l->L, s->accumulated_radiance.values, sizeof l->L);
+    memcpy(l->wl, s->wavelength_nanometers.values, sizeof l->wl);
+    memcpy(l->wl_pdfs, s->wavelength_pdf_values.values, sizeof l->wl_pdfs);
+    memcpy(l->wl_measures, s->wavelength_pdf_measures, sizeof l->wl_measures);
+    l->wl_pdf = l->wl_pdfs[0];
+    l->diffuse = s->diffuse_depth;
+    l->glossy = s->glossy_depth;
+    l->specular = s->specular_depth;
+    l->transmission = s->transmission_depth;
+    l->depth = s->depth;
+    l->eta_scale = s->eta_scale;
+    l->delta_flags = s->delta_flags;
+    l->route = ROUTE_RAY;
+}
+
+/* ---- ray cones (CudaWavefrontStages.cu:2816-2867, 3675-3782) ------------------------------------ */
+static int cone_product(float a, float b, float* out) {
+    *out = a * b;
+    return bf_isfinite(*out) && !(a != 0.0f && b != 0.0f && *out == 0.0f);
+}
+
+static int cone_ratio(float n, float d, float* out) {
+    if (!bf_isfinite(n) || !(n > 0.0f) || !bf_isfinite(d) || !(d > 0.0f)) return 0;
+    *out = n / d;
+    return bf_isfinite(*out) && *out > 0.0f;
+}
+
+/* the cone at the hit: width + spread * distance along the (unnormalised) ray */
+static int advance_cone(bf_ray_cone cone, const bf_transport_ray* ray, float t, bf_ray_cone* out) {
+    if (!valid_cone(cone) || !bf_isfinite(t) || t < 0.0f) return 0;
+    const float scale = fmaxf(fabsf(ray->direction_x), fmaxf(fabsf(ray->direction_y), fabsf(ray->direction_z)));
+    if (!bf_isfinite(scale) || !(scale > 0.0f)) return 0;
+    const float inv = 1.0f / scale;
+    const float sx = ray->direction_x * inv, sy = ray->direction_y * inv, sz = ray->direction_z * inv;
+    const float nl = sqrtf(sx * sx + sy * sy + sz * sz);
+    float length, distance, growth;
+    if (!bf_isfinite(nl) || !(nl > 0.0f) || !cone_product(scale, nl, &length) || !(length > 0.0f)) return 0;
+    if (!cone_product(t, length, &distance) || distance < 0.0f) return 0;
+    if (!cone_product(cone.spread, distance, &growth)) return 0;
+    const float width = fmaf(cone.spread, distance, cone.width);
+    if (!bf_isfinite(width) || width < cone.width || (growth != 0.0f && width == cone.width)) return 0;
+    out->width = width;
+    out->spread = cone.spread;
+    return valid_cone(*out);
+}
+
+/* the cone leaving a scattering event: the spread is floored by the closure's roughness (1 for
+ * diffuse closures) and refraction scales spread and width by the eta and cosine ratios */
+static int propagate_cone(bf_ray_cone cone, const closure_rec* c, uint32_t event, v3 wo, v3 wi, bf_ray_cone* out) {
+    if (!valid_cone(cone) || !lobe_valid_surface_event(event) || !lobe_unit_vector(wo) || !lobe_unit_vector(wi) ||
+        wo.z == 0.0f || wi.z == 0.0f)
+        return 0;
+    const int reflection = (event & BF_LOBE_REFLECTION) != 0, transmission = (event & BF_LOBE_TRANSMISSION) != 0;
+    const int same = (wo.z > 0.0f) == (wi.z > 0.0f);
+    if ((reflection && !same) || (transmission && same)) return 0;
+    float floor_spread = 0.0f, spread_scale = 1.0f, width_scale = 1.0f;
+    switch (c->kind) {
+    case BF_CLOSURE_LAMBERTIAN:
+    case BF_CLOSURE_ROUGH_DIFFUSE:
+        if (!reflection || !(event & BF_LOBE_DIFFUSE)) return 0;
+        floor_spread = 1.0f;
+        break;
+    case BF_CLOSURE_ROUGH_CONDUCTOR:
+        if (!reflection || !(event & BF_LOBE_GLOSSY)) return 0;
+        if (!bf_isfinite(c->parameters[8]) || c->parameters[8] < 0.0f || !bf_isfinite(c->parameters[9]) ||
+            c->parameters[9] < 0.0f)
+            return 0;
+        floor_spread = fmaxf(c->parameters[8], c->parameters[9]);
+        break;
+    case BF_CLOSURE_ROUGH_DIELECTRIC:
+        if (!(event & BF_LOBE_GLOSSY)) return 0;
+        if (!bf_isfinite(c->parameters[2]) || c->parameters[2] < 0.0f || !bf_isfinite(c->parameters[3]) ||
+            c->parameters[3] < 0.0f)
+            return 0;
+        floor_spread = fmaxf(c->parameters[2], c->parameters[3]);
+        break;
+    case BF_CLOSURE_SPECULAR_REFLECTION:
+        if (!reflection || !(event & BF_LOBE_SPECULAR)) return 0;
+        break;
+    case BF_CLOSURE_SPECULAR_TRANSMISSION:
+        if (!transmission || !(event & BF_LOBE_SPECULAR)) return 0;
+        break;
+    default:
+        return 0;
+    }
+    if (!bf_isfinite(floor_spread) || floor_spread < 0.0f) return 0;
+    if (transmission) {
+        if (c->kind != BF_CLOSURE_ROUGH_DIELECTRIC && c->kind != BF_CLOSURE_SPECULAR_TRANSMISSION) return 0;
+        const float incident = wo.z > 0.0f ? c->parameters[0] : c->parameters[1];
+        const float transmitted = wo.z > 0.0f ? c->parameters[1] : c->parameters[0];
+        float eta_ratio, cosine_ratio, inverse_ratio, meridional;
+        if (!cone_ratio(incident, transmitted, &eta_ratio) || !cone_ratio(fabsf(wo.z), fabsf(wi.z), &cosine_ratio) ||
+            !cone_ratio(fabsf(wi.z), fabsf(wo.z), &inverse_ratio))
+            return 0;
+        if (!cone_product(eta_ratio, cosine_ratio, &meridional)) return 0;
+        spread_scale = fmaxf(eta_ratio, meridional);
+        width_scale = fmaxf(1.0f, inverse_ratio);
+    }
+    float width, spread;
+    if (!cone_product(cone.width, width_scale, &width) || !cone_product(cone.spread, spread_scale, &spread)) return 0;
+    out->width = width;
+    out->spread = fmaxf(spread, floor_spread);
+    return valid_cone(*out);
 }
 
 /* ---- geometry helpers (CudaWavefrontStages.cu) ---------------------------------------------- */
@@ -1032,6 +1202,7 @@ static void shade_lane(const ctx* c, lane* l) {
     if (filtered.status != LOBE_OK) return fail(l, 18);
     if (mix.active_count == 0 && filtered.source_count != 0) {
         l->termination = BF_TERM_DEPTH_LIMIT;
+        l->blocked = filtered.blocked_lobes;
         l->route = ROUTE_TERM;
         return;
     }
@@ -1162,6 +1333,20 @@ static void shade_lane(const ctx* c, lane* l) {
         l->prev_ctx[0] = l->prev_ctx[1] = l->prev_ctx[2] = 0.0f;
         l->prev_pdf = 0.0f;
     }
+    if (c->in_cones) {
+        /* the sampled closure's compact index, then the cone advanced to the hit and scattered */
+        uint32_t compact = mix.active_count, matches = 0;
+        for (uint32_t k = 0; k < mix.active_count; ++k)
+            if (filtered.source_indices[k] == bs.selected) {
+                compact = k;
+                ++matches;
+            }
+        bf_ray_cone advanced, next;
+        if (matches != 1u || compact >= mix.active_count) return fail(l, 43);
+        if (!advance_cone(l->cone, &l->ray, l->hit.parameter, &advanced)) return fail(l, 44);
+        if (!propagate_cone(advanced, &mix.closures[compact], bs.lobes, wo_local, bs.incoming, &next)) return fail(l, 45);
+        l->cone = next;
+    }
     l->ray.origin_x = origin.x;
     l->ray.origin_y = origin.y;
     l->ray.origin_z = origin.z;
@@ -1272,7 +1457,8 @@ static void run_item(job* j, uint32_t item) {
     lane* l = &c->lanes[item];
     switch (j->stage) {
     case 0:
-        camera_lane(c, l, j->global_begin + item);
+        if (c->in_rays) input_lane(c, l, j->global_begin + item);
+        else camera_lane(c, l, j->global_begin + item);
         break;
     case 1: {
         bf_closest_hit_result r;
@@ -1432,6 +1618,38 @@ static int render(ctx* c0, const bfo_scene* scene, const bf_frame_desc* frame, c
             }
             rep.failed_paths++;
         }
+        if (c.out_states && l->failed) {
+            /* a failed path returns zero records and termination NONE (its failure is in the report) */
+            memset(&c.out_states[path_begin + i], 0, sizeof c.out_states[0]);
+            memset(&c.out_rays[path_begin + i], 0, sizeof c.out_rays[0]);
+            memset(&c.out_cones[path_begin + i], 0, sizeof c.out_cones[0]);
+            bf_lane_control* ctl = &c.out_controls[path_begin + i];
+            memset(ctl, 0, sizeof *ctl);
+            ctl->phase = BF_PHASE_TERMINATED;
+        } else if (c.out_states) {
+            /* the terminal records of the path-input mode (WavefrontTransport.cpp:2753-2786) */
+            bf_path_state_lane* st = &c.out_states[path_begin + i];
+            memset(st, 0, sizeof *st);
+            memcpy(st->beta.values, l->beta, sizeof l->beta);
+            memcpy(st->accumulated_radiance.values, l->L, sizeof l->L);
+            memcpy(st->wavelength_nanometers.values, l->wl, sizeof l->wl);
+            memcpy(st->wavelength_pdf_values.values, l->wl_pdfs, sizeof l->wl_pdfs);
+            memcpy(st->wavelength_pdf_measures, l->wl_measures, sizeof l->wl_measures);
+            st->diffuse_depth = l->diffuse;
+            st->glossy_depth = l->glossy;
+            st->specular_depth = l->specular;
+            st->transmission_depth = l->transmission;
+            st->depth = l->depth;
+            st->eta_scale = l->eta_scale;
+            st->delta_flags = l->delta_flags;
+            c.out_rays[path_begin + i] = l->ray;
+            c.out_cones[path_begin + i] = l->cone;
+            bf_lane_control* ctl = &c.out_controls[path_begin + i];
+            memset(ctl, 0, sizeof *ctl);
+            ctl->phase = BF_PHASE_TERMINATED;
+            ctl->termination = l->termination;
+            ctl->blocked_depth_limits = l->termination == BF_TERM_DEPTH_LIMIT ? l->blocked : 0u;
+        }
         if (tape_radiance) {
             memcpy(tape_radiance + 4 * (path_begin + i), l->L, sizeof l->L);
             tape_visibility[path_begin + i] = l->visible;
@@ -1515,6 +1733,31 @@ int bfo_render_survival(const bfo_scene* scene, const bf_frame_desc* frame, cons
     bf_transport_report report;
     return render(&c, scene, frame, options, path_begin, path_count, threads, paths, film_float, film_compensated,
                   film_counts, &report, NULL, tape_radiance, tape_visibility);
+}
+
+int bfo_render_paths(const bfo_scene* scene, const bf_transport_options* options, uint64_t count, uint32_t threads,
+                     const bf_sample_stream_index* samples, const bf_transport_ray* rays, const bf_ray_cone* cones,
+                     const bf_path_state_lane* states, bf_path_state_lane* out_states, bf_transport_ray* out_rays,
+                     bf_ray_cone* out_cones, bf_lane_control* out_controls, bf_transport_report* report) {
+    if (!samples || !rays || !cones || !states || !out_states || !out_rays || !out_cones || !out_controls)
+        return BF_INVALID_ARGUMENT;
+    ctx c;
+    memset(&c, 0, sizeof c);
+    c.in_samples = samples;
+    c.in_rays = rays;
+    c.in_cones = cones;
+    c.in_states = states;
+    c.out_states = out_states;
+    c.out_rays = out_rays;
+    c.out_cones = out_cones;
+    c.out_controls = out_controls;
+    /* a one-row frame of `count` pixels: only its extent is read, for the film, which is not written */
+    bf_frame_desc frame;
+    memset(&frame, 0, sizeof frame);
+    frame.camera.width = (uint32_t)count;
+    frame.camera.height = 1;
+    frame.samples_per_pixel = 1;
+    return render(&c, scene, &frame, options, 0, count, threads, NULL, NULL, NULL, NULL, report, NULL, NULL, NULL);
 }
 
 int bfo_render_adjoint(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,

@@ -84,6 +84,8 @@ def load(abi):
     lib.bfo_render_survival.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
                                         C.c_uint64, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.bfo_render_paths.argtypes = [C.c_void_p, C.POINTER(abi.TransportOptions), C.c_uint64, C.c_uint32] + [C.c_void_p] * 8 + \
+        [C.POINTER(abi.TransportReport)]
     lib.bfo_render_adjoint.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
                                        C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     d2 = C.POINTER(C.c_double)
@@ -347,6 +349,21 @@ class OracleScene:
         if taped:
             out.update(tape_radiance=radiance, tape_visibility=visibility)
         return out
+
+    def render_paths(self, inputs, options, threads=1):
+        """The path-input transport: inputs = (samples, rays, cones, states) ctypes arrays of equal
+        length. Returns (states, rays, cones, controls, report)."""
+        abi = self.abi
+        samples, rays, cones, states = inputs
+        n = len(rays)
+        outs = ((abi.PathStateLane * n)(), (abi.TransportRay * n)(), (abi.RayCone * n)(), (abi.LaneControl * n)())
+        report = abi.TransportReport()
+        status = self.lib.bfo_render_paths(self.handle, C.byref(options), n, threads, C.addressof(samples), C.addressof(rays),
+                                           C.addressof(cones), C.addressof(states), *[C.addressof(o) for o in outs],
+                                           C.byref(report))
+        if status != 0:
+            raise RuntimeError(f"oracle path-input render failed with status {status}")
+        return (*outs, report)
 
     def adjoint(self, frame, options, tape, film_gradient, texture_shapes, threads=1):
         """Binary64 texel gradients of sum(film_gradient * film) (film = RGB sums), one array per

@@ -172,3 +172,116 @@ def scene_bvh(geometries, instances):
             "blas": blas_bytes, "blas_nodes": pack_nodes(blas_nodes),
             "primitive_refs": np.array(prim_refs, dtype="<u4").tobytes(), "tlas_nodes": pack_nodes(tlas_nodes),
             "instance_refs": inst_bytes, "tlas_root": root}
+
+
+# ---- the serialized blob (SceneBvhAbi.hpp:11-229, SceneBvh.cpp:882-1071) ---------------------
+MAGIC = 0x31485642464B4C42
+HEADER_SIZE = 256
+ELEMENT_SIZES = (32, 48, 4, 48, 32)  # BLAS, BLAS node, primitive reference, TLAS node, instance reference
+ARRAY_KEYS = ("blas", "blas_nodes", "primitive_refs", "tlas_nodes", "instance_refs")
+(VALID, BAD_MAGIC, INCOMPATIBLE_VERSION, BAD_HEADER_SIZE, BAD_ARRAY_COUNT, UNSUPPORTED_HASH_ALGORITHM, NONZERO_RESERVED,
+ BAD_LEAF_CAPACITY, INVALID_TOPOLOGY_COUNTS, INVALID_TLAS_ROOT, INVALID_ARRAY_DESCRIPTOR, SIZE_OVERFLOW,
+ BAD_TOTAL_SIZE) = range(13)
+TOP = (1 << 64) - 1
+
+
+def align16(v):
+    return (v + 15) // 16 * 16
+
+
+def header(counts, source_hash=0):
+    """A header dict laid out as prepare_header_layout does (SceneBvh.cpp:882-925)."""
+    h = {"magic": MAGIC, "abi_major": 1, "abi_minor": 0, "header_size": HEADER_SIZE, "array_count": 5,
+         "hash_algorithm": 1, "total_size_bytes": 0, "content_hash": 0, "source_scene_hash": source_hash,
+         "counts": list(counts), "tlas_root_node": INVALID if counts[3] == 0 else 0, "leaf_capacity": 1,
+         "arrays": [], "reserved": [0] * 5}
+    cursor = align16(HEADER_SIZE)
+    for a in range(5):
+        if counts[a] == 0:
+            h["arrays"].append([0, 0, ELEMENT_SIZES[a], 0])
+            continue
+        cursor = align16(cursor)
+        h["arrays"].append([cursor, counts[a], ELEMENT_SIZES[a], 0])
+        cursor += counts[a] * ELEMENT_SIZES[a]
+    h["total_size_bytes"] = cursor
+    return h
+
+
+def pack_header(h):
+    import struct
+    out = struct.pack("<QHHIIIQQQ", h["magic"], h["abi_major"], h["abi_minor"], h["header_size"], h["array_count"],
+                      h["hash_algorithm"], h["total_size_bytes"], h["content_hash"], h["source_scene_hash"])
+    out += struct.pack("<5QII", *h["counts"], h["tlas_root_node"], h["leaf_capacity"])
+    for d in h["arrays"]:
+        out += struct.pack("<QQII", *d)
+    out += struct.pack("<5Q", *h["reserved"])
+    assert len(out) == HEADER_SIZE
+    return out
+
+
+def fnv1a64_normalized(blob):
+    h = 0xCBF29CE484222325
+    for i, byte in enumerate(blob):
+        h = ((h ^ (0 if 32 <= i < 40 else byte)) * 0x100000001B3) & TOP
+    return h
+
+
+def serialize(bvh, source_hash=0):
+    """The blob of the canonical arrays (as scene_bvh returns them)."""
+    h = header(bvh["counts"], source_hash)
+    blob = bytearray(h["total_size_bytes"])
+    for a, key in enumerate(ARRAY_KEYS):
+        if h["counts"][a]:
+            off = h["arrays"][a][0]
+            blob[off:off + len(bvh[key])] = bvh[key]
+    blob[:HEADER_SIZE] = pack_header(h)
+    h["content_hash"] = fnv1a64_normalized(blob)
+    blob[:HEADER_SIZE] = pack_header(h)
+    return bytes(blob)
+
+
+def validate_header(h):
+    """validate_scene_bvh_header (SceneBvhAbi.hpp:146-229) in its check order."""
+    if h["magic"] != MAGIC:
+        return BAD_MAGIC
+    if h["abi_major"] != 1 or h["abi_minor"] != 0:
+        return INCOMPATIBLE_VERSION
+    if h["header_size"] != HEADER_SIZE:
+        return BAD_HEADER_SIZE
+    if h["array_count"] != 5:
+        return BAD_ARRAY_COUNT
+    if h["hash_algorithm"] != 1:
+        return UNSUPPORTED_HASH_ALGORITHM
+    if any(h["reserved"]):
+        return NONZERO_RESERVED
+    if h["leaf_capacity"] != 1:
+        return BAD_LEAF_CAPACITY
+    blas, blas_nodes, prims, tlas_nodes, irefs = h["counts"]
+    if (max(h["counts"]) > INVALID or (blas == 0 and (blas_nodes or prims)) or
+            (blas and (prims < blas or blas_nodes != prims * 2 - blas)) or (irefs == 0 and tlas_nodes) or
+            (irefs and tlas_nodes != irefs * 2 - 1)):
+        return INVALID_TOPOLOGY_COUNTS
+    if (tlas_nodes == 0 and h["tlas_root_node"] != INVALID) or (tlas_nodes and h["tlas_root_node"] != 0):
+        return INVALID_TLAS_ROOT
+    if h["total_size_bytes"] < HEADER_SIZE:
+        return BAD_TOTAL_SIZE
+    cursor = align16(HEADER_SIZE)
+    for a in range(5):
+        off, count, size, reserved = h["arrays"][a]
+        if count != h["counts"][a] or size != ELEMENT_SIZES[a] or reserved:
+            return INVALID_ARRAY_DESCRIPTOR
+        if count == 0:
+            if off:
+                return INVALID_ARRAY_DESCRIPTOR
+            continue
+        if cursor > TOP - 15:
+            return SIZE_OVERFLOW
+        cursor = align16(cursor)
+        if off != cursor:
+            return INVALID_ARRAY_DESCRIPTOR
+        if count > TOP // size:
+            return SIZE_OVERFLOW
+        if cursor > TOP - count * size:
+            return SIZE_OVERFLOW
+        cursor += count * size
+    return VALID if h["total_size_bytes"] == cursor else BAD_TOTAL_SIZE

@@ -331,3 +331,68 @@ def test_reference_bvh_invariants(bf, hip, oracle, builder, name):
         assert len(tlas_nodes) == 1 and float(tlas_nodes[0]["lo"][0]) <= 0.0 and float(tlas_nodes[0]["hi"][0]) >= 1.0
         for v in range(3):
             assert exact_transform(instances[0][2], geometries[0][1][v], 0) == 1
+
+
+@pytest.mark.parametrize("seed", list(range(0, 60, 5)) + ["reference", "reference-permuted", "precision"])
+def test_serialized_blob_equals_the_restated_serializer(bf, hip, seed):
+    """blackframe_hip_scene_bvh_serialize against the restated layout of SceneBvh.cpp:882-1071
+    (header fields, aligned array offsets, array order, FNV-1a-64 with the hash field zeroed) over
+    the independently built arrays: the whole blob, byte for byte."""
+    geometries, instances = random_scene(seed) if isinstance(seed, int) else invariant_scene(seed)
+    blob = hip.XpuScene(build_scene(bf, geometries, instances), upload=False).bvh_blob()
+    assert blob == ref.serialize(ref.scene_bvh(geometries, instances), source_hash=0)
+    assert hip.scene_bvh_validate(blob) == ref.VALID
+
+
+def test_blob_bytes_ignore_insertion_order(bf, hip):
+    """CanonicalBytesIgnoreInsertionAndAllocationIdentity (SceneBvhTest.cpp:682-700)."""
+    a = hip.XpuScene(build_scene(bf, *reference_scene(False)), upload=False).bvh_blob()
+    b = hip.XpuScene(build_scene(bf, *reference_scene(True)), upload=False).bvh_blob()
+    assert a == b and ref.fnv1a64_normalized(a) == int.from_bytes(a[32:40], "little")
+
+
+def bvh_header_cases():
+    """The reference's EncodesEmptySceneAndRejectsBudgetOrAbiMismatch checks (SceneBvhTest.cpp:702-731)
+    plus one case per remaining status reachable with 32-bit counts (size_overflow needs byte
+    counts beyond 2^64, which counts below 2^32 cannot reach)."""
+    empty = ref.header([0, 0, 0, 0, 0])
+    cases = [(empty, ref.VALID)]
+    h = dict(empty, counts=[0, 0, 0, 0, 1])
+    cases.append((h, ref.INVALID_TOPOLOGY_COUNTS))
+    cases.append((dict(empty, tlas_root_node=0), ref.INVALID_TLAS_ROOT))
+    full = ref.header([2, 12, 7, 5, 3])
+    cases += [(full, ref.VALID), (dict(full, magic=0), ref.BAD_MAGIC), (dict(full, abi_minor=1), ref.INCOMPATIBLE_VERSION),
+              (dict(full, header_size=255), ref.BAD_HEADER_SIZE), (dict(full, array_count=4), ref.BAD_ARRAY_COUNT),
+              (dict(full, hash_algorithm=2), ref.UNSUPPORTED_HASH_ALGORITHM),
+              (dict(full, reserved=[0, 0, 1, 0, 0]), ref.NONZERO_RESERVED), (dict(full, leaf_capacity=2), ref.BAD_LEAF_CAPACITY),
+              (dict(full, counts=[2, 13, 7, 5, 3]), ref.INVALID_TOPOLOGY_COUNTS),
+              (dict(full, tlas_root_node=3), ref.INVALID_TLAS_ROOT),
+              (dict(full, total_size_bytes=100), ref.BAD_TOTAL_SIZE),
+              (dict(full, total_size_bytes=full["total_size_bytes"] + 16), ref.BAD_TOTAL_SIZE)]
+    shifted = dict(full, arrays=[list(d) for d in full["arrays"]])
+    shifted["arrays"][2][0] += 16
+    cases.append((shifted, ref.INVALID_ARRAY_DESCRIPTOR))
+    resized = dict(full, arrays=[list(d) for d in full["arrays"]])
+    resized["arrays"][0][2] = 16
+    cases.append((resized, ref.INVALID_ARRAY_DESCRIPTOR))
+    return cases
+
+
+@pytest.mark.parametrize("case", range(17))
+def test_bvh_header_validation_order(hip, case):
+    h, expected = bvh_header_cases()[case]
+    assert ref.validate_header(h) == expected
+    blob = ref.pack_header(h)
+    status = hip.scene_bvh_validate(blob + bytes(max(0, h["total_size_bytes"] - len(blob))), verify_hash=False)
+    assert status == expected, (case, status, expected)
+
+
+def test_bvh_blob_integrity_statuses(bf, hip, abi):
+    blob = hip.XpuScene(build_scene(bf, *reference_scene(False)), upload=False).bvh_blob()
+    assert hip.scene_bvh_validate(blob) == abi.BVH_VALID
+    assert hip.scene_bvh_validate(blob[:-1]) == abi.BVH_TRUNCATED
+    assert hip.scene_bvh_validate(blob[:255]) == abi.BVH_TRUNCATED
+    flipped = bytearray(blob)
+    flipped[300] ^= 1
+    assert hip.scene_bvh_validate(bytes(flipped)) == abi.BVH_HASH_MISMATCH
+    assert hip.scene_bvh_validate(bytes(flipped), verify_hash=False) == abi.BVH_VALID

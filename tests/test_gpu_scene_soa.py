@@ -119,3 +119,15 @@ def consistent_header(counts):
         cursor += n * size
     h["total_size_bytes"] = cursor
     return h
+
+
+def test_bvh_blob_of_a_blob_scene_names_its_source(hip, abi):
+    """The serialized BVH of a scene made from a scene blob carries that blob's content hash as its
+    source_scene_hash (SceneBvh.cpp:1081, CanonicalBytesIgnoreInsertionAndAllocationIdentity) and
+    validates with its own content hash."""
+    import struct
+
+    blob = rich_blob()
+    bvh = hip.XpuScene.from_soa(blob).bvh_blob()
+    assert hip.scene_bvh_validate(bvh) == abi.BVH_VALID
+    assert struct.unpack_from("<Q", bvh, 40)[0] == struct.unpack_from("<Q", blob, 32)[0]
