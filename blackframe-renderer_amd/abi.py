@@ -317,6 +317,7 @@ class SampleDumpResult(C.Structure):
 QUEUE_COUNT = 7
 QUEUE_ABI_MAJOR, QUEUE_ABI_MINOR = 1, 0
 COMPACTION_SCRATCH_ALIGNMENT = 256
+STATIC_COMPACTION_MAX_INPUT = 1 << 24  # BF_STATIC_COMPACTION_MAX_INPUT
 COMPACTION_SUCCESS, COMPACTION_INVALID_CONTRACT, COMPACTION_CAPACITY_EXHAUSTED = 0, 1, 2
 ROUTE_RAY, ROUTE_HIT, ROUTE_MISS, ROUTE_SHADE, ROUTE_SHADOW, ROUTE_CONTINUATION, ROUTE_TERMINATED = 1, 2, 3, 4, 5, 6, 7
 

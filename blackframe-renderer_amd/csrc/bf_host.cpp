@@ -1967,6 +1967,25 @@ int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const 
     return BF_OK;
 }
 
+int blackframe_hip_launch_wavefront_queue_compaction_static(bf_queue_soa queues, const bf_stage_outcome* outcomes,
+                                                            uint32_t input_count, uint32_t route,
+                                                            bf_compaction_result* device_result, void* stream_handle) {
+    if (!queues.headers || queues.queue_count != BF_QUEUE_COUNT || (queues.slot_stride != 0u && !queues.path_slots))
+        return set_error(BF_INVALID_ARGUMENT, "invalid queue view");
+    if (route < 1u || route > 6u) return set_error(BF_INVALID_ARGUMENT, "route %u is not a compactable queue kind", route);
+    if (input_count > queues.slot_stride)
+        return set_error(BF_INVALID_ARGUMENT, "%u outcomes exceed the queue capacity %u", input_count, queues.slot_stride);
+    if (input_count > BF_STATIC_COMPACTION_MAX_INPUT)
+        return set_error(BF_INVALID_ARGUMENT, "%u outcomes exceed the library-owned scratch (at most %u)", input_count,
+                         BF_STATIC_COMPACTION_MAX_INPUT);
+    if (!device_result || (input_count != 0u && !outcomes)) return set_error(BF_INVALID_ARGUMENT, "NULL outcomes or result");
+    void* scratch = nullptr;
+    HIP_TRY(bfk_static_compaction_scratch(&scratch));
+    HIP_TRY(bfk_queue_compaction(&queues, outcomes, input_count, route, scratch, device_result,
+                                 (hipStream_t)stream_handle));
+    return BF_OK;
+}
+
 int blackframe_hip_film_l2_loss(const bf_film_loss_job* jobs, uint32_t job_count, float scale, double* losses,
                                 void* stream_handle) {
     if (job_count == 0) return BF_OK;

@@ -162,6 +162,10 @@ hipError_t bfk_scene_soa_validate(const uint8_t* bytes, size_t count, uint32_t* 
 size_t bfk_queue_compaction_scratch(uint32_t n);
 hipError_t bfk_queue_compaction(const bf_queue_soa* queues, const bf_stage_outcome* outcomes, uint32_t n, uint32_t route,
                                 void* scratch, bf_compaction_result* result, hipStream_t s);
+/* Device address, on the current device, of the library-owned compaction scratch (at least
+ * bfk_queue_compaction_scratch(BF_STATIC_COMPACTION_MAX_INPUT) bytes, 256-aligned); looked up once
+ * per device with hipGetSymbolAddress. */
+hipError_t bfk_static_compaction_scratch(void** out);
 #ifdef __cplusplus
 }
 #endif

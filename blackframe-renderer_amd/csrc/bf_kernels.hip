@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/blackframe_texture.h"
 #include "bf_device_adjoint.h"
@@ -2819,6 +2820,45 @@ hipError_t bfk_queue_compaction(const bf_queue_soa* queues, const bf_stage_outco
     hipLaunchKernelGGL(k_qc_publish, dim3(1), dim3(64), 0, s, q, result);
     return hipGetLastError();
 }
+
+/*
+ * Library-owned compaction scratch for callers that cannot allocate: one copy per device, held
+ * until the process ends. Its size serves BF_STATIC_COMPACTION_MAX_INPUT outcomes. In host code
+ * `scratch` names a host-side stand-in, never the device array: every host use goes through the
+ * address hipGetSymbolAddress returned for the current device. Nothing in it must start at zero
+ * (the count kernel writes every tile it uses, the scan writes the total), and every call on a
+ * device shares it, so calls that overlap on one device race.
+ */
+#define BF_STATIC_COMPACTION_SCRATCH_BYTES                                                                  \
+    (2u * (((BF_STATIC_COMPACTION_MAX_INPUT / BF_BLOCK) * 4u + BF_COMPACTION_SCRATCH_ALIGNMENT - 1u) /     \
+           BF_COMPACTION_SCRATCH_ALIGNMENT * BF_COMPACTION_SCRATCH_ALIGNMENT) +                             \
+     BF_COMPACTION_SCRATCH_ALIGNMENT)
+static __device__ __align__(256) uint8_t scratch[BF_STATIC_COMPACTION_SCRATCH_BYTES];
+
+namespace {
+constexpr int kStaticScratchDevices = 64;
+std::mutex g_static_scratch_lock;
+void* g_static_scratch[kStaticScratchDevices] = {};
+} // namespace
+
+hipError_t bfk_static_compaction_scratch(void** out) {
+    *out = nullptr;
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kStaticScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(g_static_scratch_lock);
+    if (g_static_scratch[device] == nullptr) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_static_scratch[device] = ptr;
+    }
+    *out = g_static_scratch[device];
+    return hipSuccess;
+}
+
+static_assert(BF_STATIC_COMPACTION_MAX_INPUT % BF_BLOCK == 0u, "the static scratch covers whole tiles");
 
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s) {
     if (count == 0) return hipSuccess;

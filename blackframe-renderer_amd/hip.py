@@ -74,6 +74,7 @@ def library():
     lib.blackframe_hip_scene_create_from_soa.argtypes = [vp, C.c_size_t, u32, C.POINTER(vp)]
     lib.blackframe_hip_launch_wavefront_queue_pushes.argtypes = [vp, vp, u32, u32, vp, u32, vp]
     lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
+    lib.blackframe_hip_launch_wavefront_queue_compaction_static.argtypes = [abi.QueueSoa, vp, u32, u32, vp, vp]
     lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
     lib.blackframe_query_extension.restype = C.c_uint32
     lib.blackframe_query_extension.argtypes = [C.POINTER(abi.ExtensionQuery), C.POINTER(abi.ExtensionApi)]
@@ -554,11 +555,12 @@ def queue_compaction_scratch_bytes(max_input_count):
     return out.value
 
 
-def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None):
+def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None, library_scratch=False):
     """Runs blackframe_hip_launch_wavefront_queue_compaction on host arrays copied to the device:
     headers uint32 [7, 8] (QueueHeader words), slots uint32 [7 * slot_stride], outcomes uint32
     [n, 4] (status, route, path_slot, detail). Returns (headers, slots, result) read back, result a
-    uint32 [12] array (WavefrontQueueCompactionResult)."""
+    uint32 [12] array (WavefrontQueueCompactionResult). library_scratch=True calls
+    blackframe_hip_launch_wavefront_queue_compaction_static instead (no scratch argument)."""
     import torch
 
     headers = np.ascontiguousarray(headers, dtype=np.uint32).reshape(abi.QUEUE_COUNT, 8)
@@ -572,10 +574,15 @@ def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None):
     need = queue_compaction_scratch_bytes(len(outcomes))
     size = need if scratch_bytes is None else scratch_bytes
     view = abi.QueueSoa(d_headers.data_ptr(), d_slots.data_ptr() if d_slots is not None else None, abi.QUEUE_COUNT, stride)
-    d_scratch = torch.zeros(max(size, 1), dtype=torch.uint8, device="cuda")
-    _check(library().blackframe_hip_launch_wavefront_queue_compaction(
-        view, C.c_void_p(d_outcomes.data_ptr()) if d_outcomes is not None else None, len(outcomes), route,
-        C.c_void_p(d_scratch.data_ptr()), size, C.c_void_p(d_result.data_ptr()), _stream_handle()))
+    d_out = C.c_void_p(d_outcomes.data_ptr()) if d_outcomes is not None else None
+    if library_scratch:
+        _check(library().blackframe_hip_launch_wavefront_queue_compaction_static(
+            view, d_out, len(outcomes), route, C.c_void_p(d_result.data_ptr()), _stream_handle()))
+    else:
+        d_scratch = torch.zeros(max(size, 1), dtype=torch.uint8, device="cuda")
+        _check(library().blackframe_hip_launch_wavefront_queue_compaction(
+            view, d_out, len(outcomes), route, C.c_void_p(d_scratch.data_ptr()), size, C.c_void_p(d_result.data_ptr()),
+            _stream_handle()))
     torch.cuda.synchronize()
     out_slots = d_slots.cpu().numpy().view(np.uint32) if d_slots is not None else slots.copy()
     return (d_headers.cpu().numpy().view(np.uint32).reshape(abi.QUEUE_COUNT, 8), out_slots,

@@ -351,6 +351,20 @@ int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const 
                                                      size_t scratch_bytes, bf_compaction_result* device_result,
                                                      void* stream);
 
+/*
+ * blackframe_hip_launch_wavefront_queue_compaction without a scratch argument: the same kernels,
+ * the same contract and results, with their scratch in a library-owned device array of fixed size
+ * (about 512 KiB, one per device, held until the process ends). It calls no allocator, so callers
+ * that cannot allocate device memory (code under stream capture, tight serving loops, harnesses
+ * that forbid it) can compact. input_count above BF_STATIC_COMPACTION_MAX_INPUT is rejected with
+ * BF_INVALID_ARGUMENT before any launch. Every call on one device shares that array: calls that
+ * overlap on one device (concurrent streams, or threads) race and must be serialised by the caller.
+ * No counterpart in the reference, whose launcher always takes caller scratch.
+ */
+int blackframe_hip_launch_wavefront_queue_compaction_static(bf_queue_soa queues, const bf_stage_outcome* outcomes,
+                                                            uint32_t input_count, uint32_t route,
+                                                            bf_compaction_result* device_result, void* stream);
+
 /* One frame of blackframe_hip_film_l2_loss (device pointers; 40 bytes). */
 typedef struct bf_film_loss_job {
     const float* film;            /* [pixels * 3] float RGB sums, the film of a transport call */

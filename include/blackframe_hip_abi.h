@@ -408,6 +408,9 @@ typedef struct bf_sample_dump_result {
 #define BF_QUEUE_ABI_MAJOR 1u                /* HostDeviceTransportAbiMajor (TransportAbi.hpp:12) */
 #define BF_QUEUE_ABI_MINOR 0u
 #define BF_COMPACTION_SCRATCH_ALIGNMENT 256u
+/* Largest input_count of blackframe_hip_launch_wavefront_queue_compaction_static (its scratch is
+ * library-owned and of fixed size: about 512 KiB per device). */
+#define BF_STATIC_COMPACTION_MAX_INPUT (1u << 24)
 
 /* WavefrontStageOutcome (WavefrontStageKernel.hpp:107-112): status 0 = success, route 1..6 names a
  * queue kind (ray, hit, miss, shade, shadow, continuation; 7 = terminated). */
