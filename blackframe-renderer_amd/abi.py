@@ -320,6 +320,16 @@ COMPACTION_SCRATCH_ALIGNMENT = 256
 STATIC_COMPACTION_MAX_INPUT = 1 << 24  # BF_STATIC_COMPACTION_MAX_INPUT
 COMPACTION_SUCCESS, COMPACTION_INVALID_CONTRACT, COMPACTION_CAPACITY_EXHAUSTED = 0, 1, 2
 ROUTE_RAY, ROUTE_HIT, ROUTE_MISS, ROUTE_SHADE, ROUTE_SHADOW, ROUTE_CONTINUATION, ROUTE_TERMINATED = 1, 2, 3, 4, 5, 6, 7
+STAGE_KIND_SHADE, STAGE_KIND_CONTINUATION = 6, 9
+SHADE_DETAIL_CLOSURE_SAMPLED, SHADE_DETAIL_LIGHT_SAMPLED = 1 << 30, 1 << 31
+
+
+class StageAudit(C.Structure):
+    """bf_stage_audit: WavefrontStageAudit (64 B)."""
+    _fields_ = [("abi_major", C.c_uint16), ("abi_minor", C.c_uint16), ("struct_size", C.c_uint32),
+                ("stage_kind", C.c_uint32), ("expected_work_count", C.c_uint32), ("inspected_work_count", C.c_uint32),
+                ("first_failure_work_index", C.c_uint32), ("closure_samples", C.c_uint32),
+                ("light_samples", C.c_uint32), ("first_failure", C.c_uint32 * 4), ("reserved", C.c_uint32 * 4)]
 
 
 LAYOUT_VALUE_COUNT = 73

@@ -15,7 +15,19 @@
 
 #include "../../include/blackframe_hip_abi.h"
 
-#define BF_STACK_DEPTH 32u   /* traversal stack entries per lane (TLAS + BLAS) */
+/* The reference keeps one 64-entry stack for the TLAS and one for each BLAS it enters
+ * (CudaSceneClosestHit.cu:44, 941, 1092) and reports stack_overflow for a lane that needs more.
+ * Here one stack per lane holds both parts: BF_STRUCTURE_STACK entries per structure, checked per
+ * structure, BF_STACK_DEPTH entries in all. */
+#define BF_STRUCTURE_STACK 64u
+#define BF_STACK_DEPTH 128u  /* traversal stack entries per lane (TLAS + BLAS) */
+#define BF_API_STACK 64u     /* LDS entries per lane of the stand-alone traversal launchers (no spill) */
+#define BF_API_BLOCK 64u     /* lanes per workgroup of the stand-alone traversal launchers */
+/* A traversal stack entry is (reference, near word). Reference bits: 31 BF_WIDE_LEAF, 30
+ * BF_NEAR_EXACT (the near word is the binary64 near value rounded up to binary32; otherwise it
+ * holds bounds of it), 0-29 the wide-node or leaf index. */
+#define BF_NEAR_EXACT 0x40000000u
+#define BF_REF_MASK 0x3FFFFFFFu
 #ifndef BF_SHORT_STACK
 #define BF_SHORT_STACK 8u    /* of which the transport kernels keep this many in LDS */
 #endif
@@ -109,6 +121,12 @@ typedef struct bf_dev_scene {
     const bf_dev_blas* blases;
     const float4* leaf_triangles;      /* 3 float4 per BLAS leaf; [0].w = primitive (bits) */
     const bf_dev_tlas_leaf* tlas_leaves;
+    /* every wide node's and leaf's own box (6 floats: lo xyz, hi xyz), by wide index and by leaf
+     * reference: read only when the near bounds a stack entry keeps cannot decide its pop test */
+    const float* blas_node_boxes;
+    const float* blas_leaf_boxes;
+    const float* tlas_node_boxes;
+    const float* tlas_leaf_boxes;
     uint32_t tlas_root_ref;            /* BF_INVALID_INDEX when the scene has no instances */
     uint32_t instance_count;
     float tlas_root_box[6];
@@ -132,6 +150,7 @@ typedef struct bf_dev_scene {
     const bf_dev_punctual* punctuals;
     uint32_t punctual_count;           /* registered lights 0 .. punctual_count-1 */
     uint32_t light_count;              /* punctual + mesh-area lights; lights[i - punctual_count] */
+    const float* light_cdf;            /* power-weighted selection CDF (light_count + 1), or null: uniform */
     uint32_t has_environment;
     uint32_t environment_spectrum;
     uint32_t material_count;

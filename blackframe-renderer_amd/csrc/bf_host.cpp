@@ -12,8 +12,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <string>
 #include <utility>
 #include <vector>
@@ -253,43 +255,51 @@ struct Item {
     bf_bvh_instance_ref ref;
 };
 
+/* Where a build writes: the whole node array and the whole primitive- or instance-reference
+ * array of its kind (indices are absolute). */
 struct Builder {
-    std::vector<bf_bvh_node>* nodes;
-    std::vector<uint32_t>* prim_refs;
-    std::vector<bf_bvh_instance_ref>* inst_refs;
+    bf_bvh_node* nodes;
+    uint32_t* prim_refs;            /* BLAS builds */
+    bf_bvh_instance_ref* inst_refs; /* the TLAS build */
 };
 
-uint32_t build_node(std::vector<Item>& items, size_t begin, size_t end, Builder& b) {
-    const uint32_t index = (uint32_t)b.nodes->size();
-    b.nodes->emplace_back();
+/* Ranges of at least this many items build their two halves on two threads. */
+constexpr size_t kParallelBuildItems = 16384;
+
+/*
+ * Builds items[0, n) as the subtree whose root is node `index` and whose leaves take references
+ * ref .. ref + n - 1. With one item per leaf a subtree of m items holds 2m - 1 nodes in pre-order,
+ * so the first child's subtree starts at index + 1 and the second's at index + 2 m_first: every
+ * node's position is known before its subtree is built, and the two halves (disjoint item ranges,
+ * disjoint node and reference ranges) can be built concurrently with the bytes a sequential
+ * pre-order build writes. `spawn` levels below this one may still split onto new threads.
+ */
+void build_range(Item* items, size_t n, uint32_t index, uint32_t ref, const Builder& b, int spawn) {
     auto init = [](const float* box) {
-        bf_bvh_node n{};
-        n.min_x = box[0];
-        n.min_y = box[1];
-        n.min_z = box[2];
-        n.max_x = box[3];
-        n.max_y = box[4];
-        n.max_z = box[5];
-        n.split_axis = n.first_child = n.second_child = BF_INVALID_INDEX;
-        return n;
+        bf_bvh_node node{};
+        node.min_x = box[0];
+        node.min_y = box[1];
+        node.min_z = box[2];
+        node.max_x = box[3];
+        node.max_y = box[4];
+        node.max_z = box[5];
+        node.split_axis = node.first_child = node.second_child = BF_INVALID_INDEX;
+        return node;
     };
-    if (end - begin == 1) {
-        bf_bvh_node n = init(items[begin].b);
-        n.kind = 1;
-        if (b.prim_refs) {
-            n.reference_offset = (uint32_t)b.prim_refs->size();
-            b.prim_refs->push_back(items[begin].key);
-        } else {
-            n.reference_offset = (uint32_t)b.inst_refs->size();
-            b.inst_refs->push_back(items[begin].ref);
-        }
-        n.reference_count = 1;
-        (*b.nodes)[index] = n;
-        return index;
+    if (n == 1) {
+        bf_bvh_node leaf = init(items[0].b);
+        leaf.kind = 1;
+        leaf.reference_offset = ref;
+        leaf.reference_count = 1;
+        if (b.prim_refs) b.prim_refs[ref] = items[0].key;
+        else b.inst_refs[ref] = items[0].ref;
+        b.nodes[index] = leaf;
+        return;
     }
+    /* split axis: the largest binary64 centroid extent, the first axis on ties */
     double mn[3], mx[3];
-    for (int a = 0; a < 3; ++a) mn[a] = mx[a] = items[begin].c[a];
-    for (size_t i = begin + 1; i < end; ++i)
+    for (int a = 0; a < 3; ++a) mn[a] = mx[a] = items[0].c[a];
+    for (size_t i = 1; i < n; ++i)
         for (int a = 0; a < 3; ++a) {
             mn[a] = std::min(mn[a], items[i].c[a]);
             mx[a] = std::max(mx[a], items[i].c[a]);
@@ -301,24 +311,63 @@ uint32_t build_node(std::vector<Item>& items, size_t begin, size_t end, Builder&
             largest = mx[a] - mn[a];
             axis = a;
         }
-    std::sort(items.begin() + (ptrdiff_t)begin, items.begin() + (ptrdiff_t)end, [axis](const Item& l, const Item& r) {
+    /* a strict total order (centroid, then key): any correct sort gives the same permutation */
+    std::sort(items, items + n, [axis](const Item& l, const Item& r) {
         if (l.c[axis] < r.c[axis]) return true;
         if (r.c[axis] < l.c[axis]) return false;
         return l.key < r.key;
     });
-    const size_t middle = begin + (end - begin) / 2;
-    const uint32_t first = build_node(items, begin, middle, b);
-    const uint32_t second = build_node(items, middle, end, b);
-    const bf_bvh_node f = (*b.nodes)[first], s = (*b.nodes)[second];
-    const float box[6] = {canonical_zero(std::min(f.min_x, s.min_x)), canonical_zero(std::min(f.min_y, s.min_y)),
-                          canonical_zero(std::min(f.min_z, s.min_z)), canonical_zero(std::max(f.max_x, s.max_x)),
-                          canonical_zero(std::max(f.max_y, s.max_y)), canonical_zero(std::max(f.max_z, s.max_z))};
-    bf_bvh_node n = init(box);
-    n.split_axis = axis;
-    n.first_child = first;
-    n.second_child = second;
-    (*b.nodes)[index] = n;
-    return index;
+    const size_t half = n / 2;
+    const uint32_t first = index + 1, second = index + (uint32_t)(2 * half);
+    if (spawn > 0 && n >= kParallelBuildItems) {
+        std::thread worker([&] { build_range(items, half, first, ref, b, spawn - 1); });
+        build_range(items + half, n - half, second, ref + (uint32_t)half, b, spawn - 1);
+        worker.join();
+    } else {
+        build_range(items, half, first, ref, b, 0);
+        build_range(items + half, n - half, second, ref + (uint32_t)half, b, 0);
+    }
+    const bf_bvh_node f = b.nodes[first], g = b.nodes[second];
+    const float box[6] = {canonical_zero(std::min(f.min_x, g.min_x)), canonical_zero(std::min(f.min_y, g.min_y)),
+                          canonical_zero(std::min(f.min_z, g.min_z)), canonical_zero(std::max(f.max_x, g.max_x)),
+                          canonical_zero(std::max(f.max_y, g.max_y)), canonical_zero(std::max(f.max_z, g.max_z))};
+    bf_bvh_node node = init(box);
+    node.split_axis = axis;
+    node.first_child = first;
+    node.second_child = second;
+    b.nodes[index] = node;
+}
+
+/* Host threads for scene preparation. */
+uint32_t build_threads() {
+    const unsigned t = std::thread::hardware_concurrency();
+    return t == 0 ? 1u : std::min(t, 64u);
+}
+
+/* levels of build_range that may split onto new threads so that about `threads` run at once */
+int spawn_levels(uint32_t threads) {
+    int levels = 0;
+    while ((1u << levels) < threads && levels < 6) ++levels;
+    return levels;
+}
+
+/* Runs task(0 .. count-1) on up to `threads` threads (dynamic assignment; tasks are independent). */
+template <typename Task>
+void parallel_for(uint32_t count, uint32_t threads, Task task) {
+    const uint32_t workers = std::min(threads, count);
+    if (workers <= 1) {
+        for (uint32_t i = 0; i < count; ++i) task(i);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    auto run = [&] {
+        for (uint32_t i = next.fetch_add(1); i < count; i = next.fetch_add(1)) task(i);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(workers - 1);
+    for (uint32_t w = 1; w < workers; ++w) pool.emplace_back(run);
+    run();
+    for (std::thread& t : pool) t.join();
 }
 
 void item_centroid(Item& it) {
@@ -378,6 +427,7 @@ struct bf_hip_scene {
     uint32_t tlas_root = BF_INVALID_INDEX;
     std::vector<bf_wide_node> blas_wide, tlas_wide;
     void* device_block = nullptr;
+    float* light_cdf = nullptr; /* device copy of the power-weighted selection CDF, if bound */
     bf_dev_scene dev{};
     uint32_t max_blas_depth = 0, tlas_depth = 0;
     uint32_t light_count = 0;
@@ -448,26 +498,60 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
         return set_error(BF_INVALID_ARGUMENT, "scene description is missing arrays or counts");
     if (d->vertex_count > 0xFFFFFFFFull || d->triangle_count > 0x7FFFFFFFull || d->instance_count > 0x7FFFFFFFu)
         return set_error(BF_RESOURCE_EXHAUSTED, "scene exceeds the 31-bit reference domain of the wide BVH");
-    /* -- BLAS per geometry -- */
+    /* -- BLAS per geometry -- *
+     * Geometries are checked in order first; their BLASes then build concurrently into node and
+     * reference ranges fixed in advance (2k - 1 nodes and k references for k triangles), which gives
+     * the bytes of a build in geometry order. The error reported is the first geometry's in order. */
+    const uint32_t threads = build_threads();
     uint32_t prev_id = 0;
+    std::vector<uint64_t> node_base(d->geometry_count), ref_base(d->geometry_count);
+    uint64_t total_nodes = 0, total_refs = 0;
+    uint32_t checked = d->geometry_count; /* geometries before this one passed the header checks */
+    std::string header_error;
     for (uint32_t g = 0; g < d->geometry_count; ++g) {
         const bf_geometry_desc& geo = d->geometries[g];
-        if (g > 0 && geo.id <= prev_id)
-            return set_error(BF_INVALID_ARGUMENT, "geometry identifiers must be strictly increasing");
+        if (g > 0 && geo.id <= prev_id) {
+            checked = g;
+            header_error = "geometry identifiers must be strictly increasing";
+            break;
+        }
         prev_id = geo.id;
         if (geo.triangle_count == 0 || geo.vertex_count == 0 || geo.vertex_offset + geo.vertex_count > d->vertex_count ||
-            geo.triangle_offset + geo.triangle_count > d->triangle_count)
-            return set_error(BF_INVALID_ARGUMENT, "geometry %u has an invalid range", g);
+            geo.triangle_offset + geo.triangle_count > d->triangle_count) {
+            checked = g;
+            header_error = "geometry " + std::to_string(g) + " has an invalid range";
+            break;
+        }
+        node_base[g] = total_nodes;
+        ref_base[g] = total_refs;
+        total_nodes += 2 * geo.triangle_count - 1;
+        total_refs += geo.triangle_count;
+    }
+    if (total_nodes > 0x7FFFFFFFull) return set_error(BF_RESOURCE_EXHAUSTED, "scene exceeds the 31-bit BVH node domain");
+    s->blas_nodes.assign(total_nodes, bf_bvh_node{});
+    s->prim_refs.assign(total_refs, 0u);
+    s->blases.assign(checked, bf_bvh_blas{});
+    std::vector<std::string> geometry_error(checked);
+    const Builder bb{s->blas_nodes.data(), s->prim_refs.data(), nullptr};
+    const bool few = checked < threads;
+    parallel_for(checked, few ? 1u : threads, [&](uint32_t g) {
+        const bf_geometry_desc& geo = d->geometries[g];
         std::vector<Item> items(geo.triangle_count);
         for (uint32_t p = 0; p < geo.triangle_count; ++p) {
             const uint64_t t = geo.triangle_offset + p;
             Item& it = items[p];
             for (int c = 0; c < 3; ++c) {
                 const uint32_t vi = d->indices[3 * t + c];
-                if (vi >= geo.vertex_count) return set_error(BF_INVALID_ARGUMENT, "triangle index out of range");
+                if (vi >= geo.vertex_count) {
+                    geometry_error[g] = "triangle index out of range";
+                    return;
+                }
                 const float* pos = d->positions + 3 * (geo.vertex_offset + vi);
                 for (int a = 0; a < 3; ++a) {
-                    if (!std::isfinite(pos[a])) return set_error(BF_INVALID_ARGUMENT, "non-finite vertex position");
+                    if (!std::isfinite(pos[a])) {
+                        geometry_error[g] = "non-finite vertex position";
+                        return;
+                    }
                     if (c == 0) it.b[a] = it.b[3 + a] = canonical_zero(pos[a]);
                     else {
                         it.b[a] = canonical_zero(std::min(it.b[a], pos[a]));
@@ -478,32 +562,45 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
             item_centroid(it);
             it.key = p;
         }
-        const uint32_t node_offset = (uint32_t)s->blas_nodes.size();
-        const uint32_t ref_offset = (uint32_t)s->prim_refs.size();
-        Builder b{&s->blas_nodes, &s->prim_refs, nullptr};
-        const uint32_t root = build_node(items, 0, items.size(), b);
-        bf_bvh_blas blas{};
+        const uint32_t root = (uint32_t)node_base[g];
+        build_range(items.data(), items.size(), root, (uint32_t)ref_base[g], bb, few ? spawn_levels(threads) : 0);
+        bf_bvh_blas& blas = s->blases[g];
         blas.geometry_id = geo.id;
         blas.root_node = root;
-        blas.node_offset = node_offset;
-        blas.node_count = (uint32_t)(s->blas_nodes.size() - node_offset);
-        blas.primitive_reference_offset = ref_offset;
+        blas.node_offset = root;
+        blas.node_count = (uint32_t)(2 * geo.triangle_count - 1);
+        blas.primitive_reference_offset = (uint32_t)ref_base[g];
         blas.primitive_reference_count = (uint32_t)geo.triangle_count;
-        s->blases.push_back(blas);
-        s->max_blas_depth = std::max(s->max_blas_depth, tree_depth(s->blas_nodes, root));
-    }
-    /* -- TLAS over instances (SceneBvh.cpp:603-736) -- */
+    });
+    for (uint32_t g = 0; g < checked; ++g)
+        if (!geometry_error[g].empty()) return set_error(BF_INVALID_ARGUMENT, "%s", geometry_error[g].c_str());
+    if (checked < d->geometry_count) return set_error(BF_INVALID_ARGUMENT, "%s", header_error.c_str());
+    for (uint32_t g = 0; g < d->geometry_count; ++g)
+        s->max_blas_depth = std::max(s->max_blas_depth, tree_depth(s->blas_nodes, s->blases[g].root_node));
+    /* -- TLAS over instances (SceneBvh.cpp:603-736); instance bounds are formed concurrently -- */
     std::vector<Item> items(d->instance_count);
     prev_id = 0;
-    for (uint32_t i = 0; i < d->instance_count; ++i) {
+    uint32_t valid = d->instance_count; /* instances before this one passed their checks */
+    std::string instance_error;
+    for (uint32_t i = 0; i < d->instance_count && valid == d->instance_count; ++i) {
         const bf_instance_desc& inst = d->instances[i];
-        if (i > 0 && inst.id <= prev_id) return set_error(BF_INVALID_ARGUMENT, "instance identifiers must be strictly increasing");
+        char message[128] = {};
+        if (i > 0 && inst.id <= prev_id) std::snprintf(message, sizeof message, "instance identifiers must be strictly increasing");
+        else if (inst.geometry_index >= d->geometry_count || inst.material_index >= d->material_count)
+            std::snprintf(message, sizeof message, "instance %u references a missing geometry or material", i);
+        else
+            for (int k = 0; k < 12; ++k)
+                if (!std::isfinite(inst.local_to_world[k]) || !std::isfinite(inst.world_to_local[k]))
+                    std::snprintf(message, sizeof message, "instance %u has a non-finite transform", i);
         prev_id = inst.id;
-        if (inst.geometry_index >= d->geometry_count || inst.material_index >= d->material_count)
-            return set_error(BF_INVALID_ARGUMENT, "instance %u references a missing geometry or material", i);
-        for (int k = 0; k < 12; ++k)
-            if (!std::isfinite(inst.local_to_world[k]) || !std::isfinite(inst.world_to_local[k]))
-                return set_error(BF_INVALID_ARGUMENT, "instance %u has a non-finite transform", i);
+        if (message[0]) {
+            valid = i;
+            instance_error = message;
+        }
+    }
+    std::vector<uint8_t> bounds_ok(valid, 1u);
+    parallel_for(valid, threads, [&](uint32_t i) {
+        const bf_instance_desc& inst = d->instances[i];
         const bf_geometry_desc& geo = d->geometries[inst.geometry_index];
         bool have = false;
         float box[6];
@@ -511,8 +608,10 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
             for (int c = 0; c < 3; ++c) {
                 const float* pos = d->positions + 3 * (geo.vertex_offset + d->indices[3 * (geo.triangle_offset + p) + c]);
                 float pb[6];
-                if (!point_bounds(inst.local_to_world, pos, pb))
-                    return set_error(BF_INVALID_ARGUMENT, "instance %u bounds are not representable", i);
+                if (!point_bounds(inst.local_to_world, pos, pb)) {
+                    bounds_ok[i] = 0u;
+                    return;
+                }
                 if (!have) {
                     std::memcpy(box, pb, sizeof box);
                     have = true;
@@ -533,13 +632,16 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
         it.ref.geometry_id = geo.id;
         it.ref.blas_index = inst.geometry_index;
         it.ref.visibility_mask = inst.visibility_mask;
-    }
-    Builder tb{&s->tlas_nodes, nullptr, &s->inst_refs};
-    s->tlas_root = build_node(items, 0, items.size(), tb);
+    });
+    for (uint32_t i = 0; i < valid; ++i)
+        if (!bounds_ok[i]) return set_error(BF_INVALID_ARGUMENT, "instance %u bounds are not representable", i);
+    if (valid < d->instance_count) return set_error(BF_INVALID_ARGUMENT, "%s", instance_error.c_str());
+    s->tlas_nodes.assign(2 * (size_t)d->instance_count - 1, bf_bvh_node{});
+    s->inst_refs.assign(d->instance_count, bf_bvh_instance_ref{});
+    const Builder tb{s->tlas_nodes.data(), nullptr, s->inst_refs.data()};
+    build_range(items.data(), items.size(), 0u, 0u, tb, spawn_levels(threads));
+    s->tlas_root = 0u;
     s->tlas_depth = tree_depth(s->tlas_nodes, s->tlas_root);
-    if (s->tlas_depth + s->max_blas_depth + 2u > BF_STACK_DEPTH)
-        return set_error(BF_RESOURCE_EXHAUSTED, "BVH depth %u + %u exceeds the %u-entry traversal stack", s->tlas_depth,
-                         s->max_blas_depth, BF_STACK_DEPTH);
     for (const auto& n : s->blas_nodes)
         if (!valid_box(n)) return set_error(BF_INVALID_ARGUMENT, "BLAS node box is not finite and ordered");
     for (const auto& n : s->tlas_nodes)
@@ -561,6 +663,25 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
         std::memcpy(db.root_box, rb, sizeof rb);
     }
     relayout(s->tlas_nodes, 0, (uint32_t)s->tlas_nodes.size(), 1, s->tlas_wide, tlas_ref);
+    if (s->blas_wide.size() > BF_REF_MASK || s->prim_refs.size() > BF_REF_MASK || s->tlas_wide.size() > BF_REF_MASK ||
+        s->inst_refs.size() > BF_REF_MASK)
+        return set_error(BF_RESOURCE_EXHAUSTED, "scene exceeds the 30-bit node and leaf references of the traversal");
+    /* own boxes of wide nodes and leaves, for the traversal's exact pop-test fallback */
+    auto own_boxes = [](const std::vector<bf_bvh_node>& nodes, const std::vector<uint32_t>& refs, size_t wide_count,
+                        size_t leaf_count, std::vector<float>& node_boxes, std::vector<float>& leaf_boxes) {
+        node_boxes.assign(6 * wide_count, 0.0f);
+        leaf_boxes.assign(6 * leaf_count, 0.0f);
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const bf_bvh_node& n = nodes[i];
+            const float box[6] = {n.min_x, n.min_y, n.min_z, n.max_x, n.max_y, n.max_z};
+            const uint32_t ref = refs[i];
+            float* dst = (ref & BF_WIDE_LEAF) ? &leaf_boxes[6 * (size_t)(ref & BF_REF_MASK)] : &node_boxes[6 * (size_t)ref];
+            std::memcpy(dst, box, sizeof box);
+        }
+    };
+    std::vector<float> blas_node_boxes, blas_leaf_boxes, tlas_node_boxes, tlas_leaf_boxes;
+    own_boxes(s->blas_nodes, blas_ref, s->blas_wide.size(), s->prim_refs.size(), blas_node_boxes, blas_leaf_boxes);
+    own_boxes(s->tlas_nodes, tlas_ref, s->tlas_wide.size(), s->inst_refs.size(), tlas_node_boxes, tlas_leaf_boxes);
     /* leaf triangles in BLAS leaf order: object-space vertices + primitive index */
     std::vector<float> leaf_tris(s->prim_refs.size() * 12);
     for (uint32_t g = 0; g < d->geometry_count; ++g) {
@@ -786,6 +907,10 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     const size_t o_blasw = A.add(s->blas_wide.data(), s->blas_wide.size() * sizeof(bf_wide_node));
     const size_t o_blas = A.add(dev_blases.data(), dev_blases.size() * sizeof(bf_dev_blas));
     const size_t o_tris = A.add(leaf_tris.data(), leaf_tris.size() * sizeof(float));
+    const size_t o_bnb = A.add(blas_node_boxes.data(), blas_node_boxes.size() * sizeof(float));
+    const size_t o_blb = A.add(blas_leaf_boxes.data(), blas_leaf_boxes.size() * sizeof(float));
+    const size_t o_tnb = A.add(tlas_node_boxes.data(), tlas_node_boxes.size() * sizeof(float));
+    const size_t o_tlb = A.add(tlas_leaf_boxes.data(), tlas_leaf_boxes.size() * sizeof(float));
     const size_t o_leaves = A.add(tlas_leaves.data(), tlas_leaves.size() * sizeof(bf_dev_tlas_leaf));
     const size_t o_inst = A.add(instances.data(), instances.size() * sizeof(bf_dev_instance));
     const size_t o_geo = A.add(geometries.data(), geometries.size() * sizeof(bf_dev_geometry));
@@ -827,6 +952,10 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     S.blas_nodes = (const bf_wide_node*)(base + o_blasw);
     S.blases = (const bf_dev_blas*)(base + o_blas);
     S.leaf_triangles = (const float4*)(base + o_tris);
+    S.blas_node_boxes = (const float*)(base + o_bnb);
+    S.blas_leaf_boxes = (const float*)(base + o_blb);
+    S.tlas_node_boxes = (const float*)(base + o_tnb);
+    S.tlas_leaf_boxes = (const float*)(base + o_tlb);
     S.tlas_leaves = (const bf_dev_tlas_leaf*)(base + o_leaves);
     S.tlas_root_ref = tlas_ref[s->tlas_root];
     S.instance_count = d->instance_count;
@@ -984,10 +1113,146 @@ int blackframe_hip_scene_create(const bf_scene_desc* desc, bf_hip_scene** out) {
 
 int blackframe_hip_scene_prepare(const bf_scene_desc* desc, bf_hip_scene** out) { return create_scene(desc, out, false); }
 
+/* ---- power-weighted light selection (Renderer/Source/LightSampler.cpp) ------------------------ */
+namespace {
+
+/* add_compensated (:85-102): a running binary32 sum with its Neumaier correction; false when a
+ * term is negative or not finite, or the sum overflows */
+bool add_compensated(float value, float& sum, float& correction) {
+    if (!std::isfinite(value) || value < 0.0f) return false;
+    const float next = sum + value;
+    if (!std::isfinite(next)) return false;
+    correction += sum >= value ? (sum - next) + value : (value - next) + sum;
+    if (!std::isfinite(correction)) return false;
+    sum = next;
+    return true;
+}
+
+} // namespace
+
+extern "C" int blackframe_hip_light_power_cdf(const float* powers, uint32_t count, float* boundaries) {
+    if (!powers || !boundaries) return set_error(BF_INVALID_ARGUMENT, "NULL powers or boundaries");
+    if (count == 0) return set_error(BF_INVALID_ARGUMENT, "power-weighted light sampling needs at least one light");
+    /* power_weights (:201-249): lanes scaled by the global peak, each light's four lanes summed */
+    float peak = 0.0f;
+    for (uint64_t k = 0; k < 4ull * count; ++k) {
+        if (!std::isfinite(powers[k]) || powers[k] < 0.0f)
+            return set_error(BF_INVALID_ARGUMENT, "light %llu: spectral power lanes must be finite and non-negative",
+                             (unsigned long long)(k / 4));
+        peak = std::max(peak, powers[k]);
+    }
+    if (peak == 0.0f) return set_error(BF_INVALID_ARGUMENT, "power-weighted light sampling needs positive power");
+    std::vector<float> weights(count);
+    for (uint32_t i = 0; i < count; ++i) {
+        float sum = 0.0f, correction = 0.0f;
+        bool positive = false;
+        for (int lane = 0; lane < 4; ++lane) {
+            const float v = powers[4ull * i + lane];
+            const float ratio = v / peak;
+            if (v > 0.0f) {
+                positive = true;
+                if (!(ratio > 0.0f))
+                    return set_error(BF_RESOURCE_EXHAUSTED, "light %u: a positive lane underflows after scaling", i);
+            }
+            if (!add_compensated(ratio, sum, correction))
+                return set_error(BF_RESOURCE_EXHAUSTED, "light %u: the selection weight is not representable", i);
+        }
+        const float w = sum + correction;
+        if (!std::isfinite(w) || w < 0.0f || (positive && !(w > 0.0f)))
+            return set_error(BF_RESOURCE_EXHAUSTED, "light %u: the selection weight is not representable", i);
+        weights[i] = w == 0.0f ? 0.0f : w;
+    }
+    /* build_power_cdf (:116-199): compensated prefix sums over the compensated total; the last
+     * light with power closes at exactly 1, lights without power repeat the previous boundary */
+    float total_sum = 0.0f, total_correction = 0.0f;
+    for (float w : weights)
+        if (!add_compensated(w, total_sum, total_correction))
+            return set_error(BF_RESOURCE_EXHAUSTED, "the total light power is not representable");
+    float total = total_sum + total_correction;
+    if (!std::isfinite(total) || total < 0.0f) return set_error(BF_RESOURCE_EXHAUSTED, "the total light power is not representable");
+    if (total == 0.0f) return set_error(BF_INVALID_ARGUMENT, "power-weighted light sampling needs positive power");
+    uint32_t last_positive = count;
+    for (uint32_t i = count; i > 0; --i)
+        if (weights[i - 1] > 0.0f) {
+            last_positive = i - 1;
+            break;
+        }
+    if (last_positive == count) return set_error(BF_INTERNAL_ERROR, "positive power has no light");
+    boundaries[0] = 0.0f;
+    float prefix_sum = 0.0f, prefix_correction = 0.0f, previous = 0.0f;
+    for (uint32_t i = 0; i < count; ++i) {
+        const float w = weights[i];
+        if (w == 0.0f) {
+            boundaries[i + 1] = previous;
+            continue;
+        }
+        if (!add_compensated(w, prefix_sum, prefix_correction))
+            return set_error(BF_RESOURCE_EXHAUSTED, "a light power prefix is not representable");
+        const float prefix = prefix_sum + prefix_correction;
+        if (!std::isfinite(prefix) || !(prefix > 0.0f))
+            return set_error(BF_RESOURCE_EXHAUSTED, "a light power prefix is not representable");
+        float boundary = 1.0f;
+        if (i != last_positive) {
+            boundary = prefix / total;
+            if (!std::isfinite(boundary) || !(boundary > previous) || !(boundary < 1.0f))
+                return set_error(BF_RESOURCE_EXHAUSTED, "light %u: its selection probability is not representable", i);
+        } else {
+            const float nominal = w / total;
+            const float candidate = previous + nominal;
+            if (!std::isfinite(nominal) || !(nominal > 0.0f) || !std::isfinite(candidate) || !(candidate > previous))
+                return set_error(BF_RESOURCE_EXHAUSTED, "light %u: its selection probability is not representable", i);
+        }
+        boundaries[i + 1] = boundary;
+        previous = boundary;
+    }
+    if (boundaries[count] != 1.0f) return set_error(BF_INTERNAL_ERROR, "the light selection CDF does not end at 1");
+    for (uint32_t i = 0; i < count; ++i) {
+        const float p = boundaries[i + 1] - boundaries[i];
+        if (!std::isfinite(p) || p < 0.0f || ((weights[i] > 0.0f) != (p > 0.0f)))
+            return set_error(BF_RESOURCE_EXHAUSTED, "light %u: the selection CDF lost or created support", i);
+    }
+    return BF_OK;
+}
+
+extern "C" int blackframe_hip_scene_set_light_powers(bf_hip_scene* s, const float* powers, uint32_t count) {
+    if (!s) return set_error(BF_INVALID_ARGUMENT, "NULL scene");
+    if (!s->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
+    if (powers && count != s->light_count)
+        return set_error(BF_INVALID_ARGUMENT, "%u light powers for a scene with %u registered lights", count, s->light_count);
+    float* cdf = nullptr;
+    if (powers) {
+        std::vector<float> host(count + 1);
+        const int st = blackframe_hip_light_power_cdf(powers, count, host.data());
+        if (st != BF_OK) return st;
+        int device = 0;
+        HIP_TRY(hipGetDevice(&device));
+        if (device != s->device)
+            return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", s->device, device);
+        HIP_TRY(hipMalloc(&cdf, host.size() * sizeof(float)));
+        const hipError_t e = hipMemcpy(cdf, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(cdf);
+            return hip_error(e, "hipMemcpy(light cdf)");
+        }
+    }
+    if (s->light_cdf) {
+        /* a launch still reading the old table must finish before it goes */
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipFree(s->light_cdf));
+    }
+    s->light_cdf = cdf;
+    s->dev.light_cdf = cdf;
+    return BF_OK;
+}
+
 int blackframe_hip_scene_destroy(bf_hip_scene* s) {
     if (!s) return BF_OK;
     hipError_t e = hipSuccess;
-    if (s->device_block) e = hipFree(s->device_block);
+    if (s->light_cdf) e = hipFree(s->light_cdf);
+    if (s->device_block) {
+        const hipError_t f = hipFree(s->device_block);
+        if (e == hipSuccess) e = f;
+    }
     delete s;
     return e == hipSuccess ? BF_OK : hip_error(e, "hipFree(scene)");
 }
@@ -1964,6 +2229,16 @@ int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const 
                          scratch_bytes, scratch, required);
     HIP_TRY(bfk_queue_compaction(&queues, outcomes, input_count, route, scratch, device_result,
                                  (hipStream_t)stream_handle));
+    return BF_OK;
+}
+
+int blackframe_hip_launch_wavefront_audit_stage(const bf_stage_outcome* outcomes, uint32_t work_count,
+                                                uint32_t allowed_route_mask, uint32_t path_capacity,
+                                                uint32_t stage_kind, bf_stage_audit* audit, void* stream_handle) {
+    if (!audit || (work_count != 0u && !outcomes)) return set_error(BF_INVALID_ARGUMENT, "NULL audit or outcomes");
+    if (stage_kind > BF_STAGE_KIND_CONTINUATION) return set_error(BF_INVALID_ARGUMENT, "unknown stage kind %u", stage_kind);
+    HIP_TRY(bfk_audit_stage(outcomes, work_count, allowed_route_mask, path_capacity, stage_kind, audit,
+                            (hipStream_t)stream_handle));
     return BF_OK;
 }
 

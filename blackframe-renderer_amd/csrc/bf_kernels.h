@@ -138,6 +138,9 @@ hipError_t bfk_sample_stream_dump(const bf_sample_dump_request* requests, uint32
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s);
 hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s);
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
+/* stage outcome audit: init + per-workgroup partials + one finalising workgroup */
+hipError_t bfk_audit_stage(const bf_stage_outcome* outcomes, uint32_t work_count, uint32_t allowed_route_mask,
+                           uint32_t path_capacity, uint32_t stage_kind, bf_stage_audit* audit, hipStream_t s);
 /* the terminal records of a path-input frame's slots: path state, ray, cone and lane control */
 hipError_t bfk_export_path_records(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s);
 /* dst[e] += sum over the replica rows of replicas[row * BF_GRADIENT_ROW + offset + e], e < count */

@@ -593,7 +593,7 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
     __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
     trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
-                                            P.spill_stride, 0u, 0u};
+                                            P.spill_stride, 0u, 0u, BF_STACK_DEPTH};
     trav_counters ctr{0u, 0u, 0u};
     lane_ray L;
     struct stepper {
@@ -864,7 +864,9 @@ BF_D bool sample_light(const bf_transport_params& P, const bf_dev_scene& S, cons
     const float u_sel = sample_1d(P.sampler, k, first_dim + 0);
     uint32_t li;
     float sel;
-    if (!uniform_light_selection(S.light_count, u_sel, li, sel)) return false;
+    if (S.light_cdf ? !cdf_light_selection(S.light_cdf, S.light_count, u_sel, li, sel)
+                    : !uniform_light_selection(S.light_count, u_sel, li, sel))
+        return false;
     if (li < S.punctual_count) return sample_punctual(S, li, wl, surf, sel, inc);
     const bf_dev_light L = S.lights[li - S.punctual_count];
     const bf_dev_instance& I = S.instances[L.instance];
@@ -947,7 +949,10 @@ BF_D bool emission_weight(const bf_transport_params& P, const bf_dev_scene& S, u
     const int32_t li = S.instances[inst].light_index;
     if (li < 0) return false;
     float sel;
-    if (!uniform_light_probability(S.light_count, S.punctual_count + (uint32_t)li, sel)) return false;
+    const uint32_t slot = S.punctual_count + (uint32_t)li;
+    if (S.light_cdf ? !cdf_light_probability(S.light_cdf, S.light_count, slot, sel)
+                    : !uniform_light_probability(S.light_count, slot, sel))
+        return false;
     const float inv_area = S.lights[li].inverse_area;
     segment_t seg;
     if (!scaled_segment(sub3(surf.position, mk(prev.x, prev.y, prev.z)), seg)) return false;
@@ -1486,7 +1491,7 @@ __global__ __launch_bounds__(BLOCK, BF_SHADOW_WAVES_PER_SIMD) void k_trace_shado
     __shared__ uint2 s_stack[BF_SHORT_STACK * BLOCK];
     const uint32_t n = *count_ptr;
     trav_stack<BF_SHORT_STACK, BLOCK> stack{s_stack + threadIdx.x, P.spill + blockIdx.x * BLOCK + threadIdx.x,
-                                            P.spill_stride, 0u, 0u};
+                                            P.spill_stride, 0u, 0u, BF_STACK_DEPTH};
     trav_counters ctr{0u, 0u, 0u};
     lane_ray L;
     struct stepper {
@@ -1849,11 +1854,14 @@ __global__ void k_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, u
 }
 
 /* ---- stand-alone traversal (blackframe_hip_launch_scene_closest_hit / _occlusion) ----------- */
-__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_scene S, const bf_transport_ray* rays,
-                                                                      uint32_t count, bf_closest_hit_result* results,
-                                                                      uint32_t* counters) {
-    __shared__ uint2 s_stack[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    trav_stack<BF_STACK_DEPTH, BF_TRACE_BLOCK> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
+/* One ray per lane with a 64-entry LDS stack and no spill area: a lane whose TLAS and BLAS entries
+ * together exceed 64 reports stack overflow (the reference allows 64 per structure; a median-split
+ * BVH never holds more than its depth + 1 entries per structure). */
+__global__ __launch_bounds__(BF_API_BLOCK) void k_scene_closest_hit(bf_dev_scene S, const bf_transport_ray* rays,
+                                                                    uint32_t count, bf_closest_hit_result* results,
+                                                                    uint32_t* counters) {
+    __shared__ uint2 s_stack[BF_API_STACK * BF_API_BLOCK];
+    trav_stack<BF_API_STACK, BF_API_BLOCK> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u, BF_API_STACK};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const bf_transport_ray r = rays[i];
@@ -1913,10 +1921,10 @@ __global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_closest_hit(bf_dev_sce
     }
 }
 
-__global__ __launch_bounds__(BF_TRACE_BLOCK) void k_scene_occlusion(bf_dev_scene S, const bf_transport_ray* rays,
-                                                                    uint32_t count, bf_occlusion_result* results) {
-    __shared__ uint2 s_stack[BF_STACK_DEPTH * BF_TRACE_BLOCK];
-    trav_stack<BF_STACK_DEPTH, BF_TRACE_BLOCK> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u};
+__global__ __launch_bounds__(BF_API_BLOCK) void k_scene_occlusion(bf_dev_scene S, const bf_transport_ray* rays,
+                                                                  uint32_t count, bf_occlusion_result* results) {
+    __shared__ uint2 s_stack[BF_API_STACK * BF_API_BLOCK];
+    trav_stack<BF_API_STACK, BF_API_BLOCK> stack{s_stack + threadIdx.x, nullptr, 0u, 0u, 0u, BF_API_STACK};
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const bf_transport_ray r = rays[i];
@@ -2491,6 +2499,79 @@ __global__ __launch_bounds__(BF_BLOCK) void k_export_path_records(bf_dev_paths p
     J.out_controls[path] = ctl;
 }
 
+/* ---- stage audit (CudaWavefrontStages.cu:4945-5033) -------------------------------------------
+ * Each workgroup reduces its outcomes with wave ballots and shuffles into one partial (smallest
+ * failing index, closure and light sample counts), written to the audit's reserved words through
+ * atomics; a second one-wave launch on the same stream closes the audit (inspected count, first
+ * failing outcome) and clears the words it used. */
+BF_D bool outcome_failed(const bf_stage_outcome& o, uint32_t allowed_route_mask, uint32_t path_capacity) {
+    if (o.status != BF_STAGE_SUCCESS || o.route > BF_ROUTE_TERMINATED) return true;
+    return (allowed_route_mask & (1u << o.route)) == 0u || o.path_slot >= path_capacity;
+}
+
+__global__ void k_audit_init(uint32_t work_count, uint32_t stage_kind, bf_stage_audit* audit) {
+    if (threadIdx.x != 0u) return;
+    bf_stage_audit a;
+    memset(&a, 0, sizeof a);
+    a.abi_major = BF_STAGE_AUDIT_ABI_MAJOR;
+    a.abi_minor = BF_STAGE_AUDIT_ABI_MINOR;
+    a.struct_size = sizeof(bf_stage_audit);
+    a.stage_kind = stage_kind;
+    a.expected_work_count = work_count;
+    a.first_failure_work_index = 0xFFFFFFFFu;
+    *audit = a;
+}
+
+__global__ __launch_bounds__(BF_BLOCK) void k_audit_reduce(const bf_stage_outcome* outcomes, uint32_t work_count,
+                                                           uint32_t allowed_route_mask, uint32_t path_capacity,
+                                                           uint32_t shade, bf_stage_audit* audit) {
+    __shared__ uint32_t wave_first[BF_BLOCK / 64], wave_closure[BF_BLOCK / 64], wave_light[BF_BLOCK / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t first = 0xFFFFFFFFu, closure = 0u, light = 0u;
+    for (uint32_t i = blockIdx.x * BF_BLOCK + threadIdx.x; i < work_count; i += gridDim.x * BF_BLOCK) {
+        const bf_stage_outcome o = outcomes[i];
+        if (outcome_failed(o, allowed_route_mask, path_capacity)) first = min(first, i);
+        if (shade) {
+            closure += (o.detail & BF_SHADE_DETAIL_CLOSURE_SAMPLED) ? 1u : 0u;
+            light += (o.detail & BF_SHADE_DETAIL_LIGHT_SAMPLED) ? 1u : 0u;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        first = min(first, (uint32_t)__shfl_xor((int)first, off, 64));
+        closure += (uint32_t)__shfl_xor((int)closure, off, 64);
+        light += (uint32_t)__shfl_xor((int)light, off, 64);
+    }
+    if (lane == 0u) {
+        wave_first[wave] = first;
+        wave_closure[wave] = closure;
+        wave_light[wave] = light;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        for (uint32_t w = 1; w < BF_BLOCK / 64; ++w) {
+            first = min(first, wave_first[w]);
+            closure += wave_closure[w];
+            light += wave_light[w];
+        }
+        if (first != 0xFFFFFFFFu) atomicMin(&audit->first_failure_work_index, first);
+        if (closure) atomicAdd(&audit->closure_samples, closure);
+        if (light) atomicAdd(&audit->light_samples, light);
+    }
+}
+
+__global__ void k_audit_close(const bf_stage_outcome* outcomes, uint32_t work_count, bf_stage_audit* audit) {
+    if (threadIdx.x != 0u) return;
+    audit->inspected_work_count = work_count;
+    const uint32_t f = audit->first_failure_work_index;
+    if (f < work_count) {
+        const bf_stage_outcome o = outcomes[f];
+        audit->first_failure[0] = o.status;
+        audit->first_failure[1] = o.route;
+        audit->first_failure[2] = o.path_slot;
+        audit->first_failure[3] = o.detail;
+    }
+}
+
 /* ---- host-side launch helpers (stream-ordered, never synchronising) -------------------------- */
 static inline uint32_t blocks_for(uint32_t n, uint32_t b) { return (n + b - 1u) / b; }
 
@@ -2701,7 +2782,7 @@ hipError_t bfk_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, unsi
 hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
                                  bf_closest_hit_result* results, uint32_t* counters, hipStream_t s) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scene_closest_hit, dim3(blocks_for(count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s, *S, rays,
+    hipLaunchKernelGGL(k_scene_closest_hit, dim3(blocks_for(count, BF_API_BLOCK)), dim3(BF_API_BLOCK), 0, s, *S, rays,
                        count, results, counters);
     return hipGetLastError();
 }
@@ -2709,7 +2790,7 @@ hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* 
 hipError_t bfk_scene_occlusion(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
                                bf_occlusion_result* results, hipStream_t s) {
     if (count == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scene_occlusion, dim3(blocks_for(count, BF_TRACE_BLOCK)), dim3(BF_TRACE_BLOCK), 0, s, *S, rays,
+    hipLaunchKernelGGL(k_scene_occlusion, dim3(blocks_for(count, BF_API_BLOCK)), dim3(BF_API_BLOCK), 0, s, *S, rays,
                        count, results);
     return hipGetLastError();
 }
@@ -2874,6 +2955,17 @@ hipError_t bfk_copy_word(const uint32_t* src, uint32_t* dst, hipStream_t s) {
 hipError_t bfk_export_paths(const bf_dev_paths* paths, const bf_dev_job* job, hipStream_t s) {
     if (job->slot_count == 0 || !job->paths) return hipSuccess;
     hipLaunchKernelGGL(k_export_paths, dim3(blocks_for(job->slot_count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, *paths, *job);
+    return hipGetLastError();
+}
+
+hipError_t bfk_audit_stage(const bf_stage_outcome* outcomes, uint32_t work_count, uint32_t allowed_route_mask,
+                           uint32_t path_capacity, uint32_t stage_kind, bf_stage_audit* audit, hipStream_t s) {
+    hipLaunchKernelGGL(k_audit_init, dim3(1), dim3(64), 0, s, work_count, stage_kind, audit);
+    if (work_count == 0u) return hipGetLastError();
+    const uint32_t blocks = std::min<uint32_t>(blocks_for(work_count, BF_BLOCK), 2048u);
+    hipLaunchKernelGGL(k_audit_reduce, dim3(blocks), dim3(BF_BLOCK), 0, s, outcomes, work_count, allowed_route_mask,
+                       path_capacity, stage_kind == BF_STAGE_KIND_SHADE ? 1u : 0u, audit);
+    hipLaunchKernelGGL(k_audit_close, dim3(1), dim3(64), 0, s, outcomes, work_count, audit);
     return hipGetLastError();
 }
 

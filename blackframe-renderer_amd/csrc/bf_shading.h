@@ -265,4 +265,31 @@ BF_D bool uniform_light_probability(uint32_t count, uint32_t i, float& probabili
     return probability > 0.0f && probability <= 1.0f;
 }
 
+/* Power-weighted selection over the host-built CDF (LightSampler::sample, Renderer/Source/
+ * LightSampler.cpp:1023-1057): the first boundary above u closes the selected light's interval. */
+BF_D bool cdf_light_selection(const float* cdf, uint32_t count, float u, uint32_t& index, float& probability) {
+    if (count == 0 || !bf_isfinite(u) || u < 0.0f || !(u < 1.0f)) return false;
+    uint32_t first = 1, last = count + 1;
+    while (first < last) {
+        const uint32_t mid = first + (last - first) / 2;
+        if (!(u < cdf[mid])) first = mid + 1;
+        else last = mid;
+    }
+    if (first > count) return false;
+    const float p = cdf[first] - cdf[first - 1];
+    if (!bf_isfinite(p) || !(p > 0.0f) || p > 1.0f) return false;
+    index = first - 1;
+    probability = p;
+    return true;
+}
+
+/* LightSampler::probability (:1133-1151): a light without power has probability 0 */
+BF_D bool cdf_light_probability(const float* cdf, uint32_t count, uint32_t i, float& probability) {
+    if (i >= count) return false;
+    const float p = cdf[i + 1] - cdf[i];
+    if (!bf_isfinite(p) || p < 0.0f || p > 1.0f) return false;
+    probability = p;
+    return true;
+}
+
 #endif

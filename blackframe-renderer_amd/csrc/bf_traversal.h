@@ -11,8 +11,8 @@
  * passed its push-time test, so that pop test reduces to near <= t_max; the stack stores near
  * rounded UP to binary32 (exactly equivalent for a binary32 t_max), which lets a popped leaf go
  * straight to its triangle without fetching a node. The reference keeps one 64-entry stack per
- * structure; here TLAS and BLAS share one stack of BF_STACK_DEPTH entries and the host rejects
- * scenes whose TLAS depth + BLAS depth + 2 exceeds it. Slab quotients are formed from a per-ray
+ * structure; here TLAS and BLAS share one stack of BF_STACK_DEPTH entries, each part held to the
+ * reference's 64, so a lane reports stack overflow exactly where the reference's would. Slab quotients are formed from a per-ray
  * binary64 reciprocal with one correction step, which yields the correctly rounded quotient.
  */
 #ifndef BF_TRAVERSAL_H
@@ -148,6 +148,40 @@ BF_D uint32_t slab_exact(float lx, float ly, float lz, float hx, float hy, float
     return BF_HIT_MISS;
 }
 
+/* A child a node step pushes: its stack entry and bounds [key_lo, key_hi] of the binary32 rounding
+ * of its binary64 near value (the reference orders siblings by that rounding). */
+struct child_near {
+    uint2 entry;
+    float key_lo, key_hi;
+};
+
+BF_D float round_up_float(double v) {
+    float f = (float)v;
+    if ((double)f < v) f = bf_next_up_f(f);
+    return f;
+}
+
+/* An entry whose near word is the exact binary64 near value rounded up to binary32: visited iff it
+ * is <= the current t_max, exactly the reference's pop test. */
+BF_D void exact_child(uint32_t ref, double v, child_near& c) {
+    c.entry = make_uint2(ref | BF_NEAR_EXACT, __float_as_uint(round_up_float(v)));
+    c.key_lo = c.key_hi = (float)v;
+}
+
+/*
+ * An entry that keeps bounds lo <= near <= hi of the binary64 near value (0 <= lo): bits 8-31 of
+ * the near word hold hi rounded up to a multiple of 256 ulps (still an upper bound) and bits 0-7
+ * k with 2^(k-127) >= hi' - lo, so that hi' - 2^(k-127) is a lower bound (k = 255: none kept).
+ * With E the exponent field of RN(hi' - lo), 2^(E+1-127) exceeds hi' - lo itself.
+ */
+BF_D void bounded_child(uint32_t ref, float lo, float hi, child_near& c) {
+    const uint32_t hb = (__float_as_uint(hi) + 0xFFu) & ~0xFFu;
+    const uint32_t k = min((__float_as_uint(__uint_as_float(hb) - lo) >> 23) + 1u, 255u);
+    c.entry = make_uint2(ref, hb | k);
+    c.key_lo = lo;
+    c.key_hi = hi;
+}
+
 /*
  * The same decision with a binary32 filter in front. Each binary32 plane distance
  * q~ = RN32(RN32(plane - o) * RN32(1/d)) is within 3 binary32 roundings of the exact quotient
@@ -155,12 +189,12 @@ BF_D uint32_t slab_exact(float lx, float ly, float lz, float hx, float hy, float
  * relative; subnormal differences add at most |1/d| 2^-149 absolutely. So every binary64
  * interval end lies within e = 4u max|q~| + max |1/d| 2^-147 + 2^-126 of its binary32 estimate,
  * and when the estimated interval is empty (or non-empty) by more than 2e the binary64 test
- * reaches the same verdict. Only a box within that band, or a hit (whose binary64 near value is
- * pushed), takes binary64 arithmetic, and a hit needs only the near plane of each axis: the
- * smaller quotient is the lo plane for d > 0 and the hi plane for d < 0 (rounding is monotone).
+ * reaches the same verdict. Only a box within that band takes binary64 arithmetic; a certain hit
+ * returns its near value as the binary32 interval [nv - e, nv + e] (widened outward by one ulp
+ * each side, and never below t_min, which bounds the binary64 value from below).
  */
 BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, const trav_space& s, float t_min,
-                   float t_max, bool& hit, double& near_value) {
+                   float t_max, uint32_t ref, bool& hit, child_near& near) {
     const float lo[3] = {lx, ly, lz};
     const float hi[3] = {hx, hy, hz};
     const float oo[3] = {s.o.x, s.o.y, s.o.z};
@@ -188,26 +222,45 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
             hit = false;
             return BF_HIT_MISS;
         }
-        if (fv - nv > 2.0f * e) {
-            double n = (double)t_min;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                if (dd[a] == 0.0f) continue;
-                const float plane = dd[a] > 0.0f ? lo[a] : hi[a];
-                n = fmax(n, bf_next_down(quotient((double)plane - (double)oo[a], (double)dd[a], s.inv[a])));
-            }
+        const float up = nv + e; /* > 0: nv >= t_min >= 0, e > 0 */
+        if (fv - nv > 2.0f * e && up < 0x1p127f) {
+            const float low = nv - e;
             hit = true;
-            near_value = n;
+            /* one ulp outward on each side (both positive where they bind) */
+            bounded_child(ref, low > t_min ? __uint_as_float(__float_as_uint(low) - 1u) : t_min,
+                          __uint_as_float(__float_as_uint(up) + 1u), near);
             return BF_HIT_MISS;
         }
     }
-    return slab_exact(lx, ly, lz, hx, hy, hz, s, t_min, t_max, hit, near_value);
+    double v = 0.0;
+    const uint32_t st = slab_exact(lx, ly, lz, hx, hy, hz, s, t_min, t_max, hit, v);
+    exact_child(ref, v, near);
+    return st;
 }
 
-BF_D float round_up_float(double v) {
-    float f = (float)v;
-    if ((double)f < v) f = bf_next_up_f(f);
-    return f;
+/* The binary64 near value of a box the ray is known to meet: per axis the near plane (lo for
+ * d > 0, hi for d < 0; rounding is monotone) widened down by one ulp, and t_min. */
+BF_D double near_exact(const float* box, const trav_space& s, float t_min) {
+    const float oo[3] = {s.o.x, s.o.y, s.o.z};
+    const float dd[3] = {s.d.x, s.d.y, s.d.z};
+    double n = (double)t_min;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (dd[a] == 0.0f) continue;
+        const float plane = dd[a] > 0.0f ? box[a] : box[3 + a];
+        n = fmax(n, bf_next_down(quotient((double)plane - (double)oo[a], (double)dd[a], s.inv[a])));
+    }
+    return n;
+}
+
+/* 1: the entry's near value is <= t (visit), 0: it is > t (skip), 2: its bounds cannot tell */
+BF_D uint32_t near_decision(uint2 entry, float t) {
+    if (entry.x & BF_NEAR_EXACT) return __uint_as_float(entry.y) <= t ? 1u : 0u;
+    const float hi = __uint_as_float(entry.y & ~0xFFu);
+    if (hi <= t) return 1u;
+    const uint32_t k = entry.y & 0xFFu;
+    if (k != 255u && bf_next_down_f(hi - __uint_as_float(k << 23)) > t) return 0u;
+    return 2u;
 }
 
 struct tri_result {
@@ -368,13 +421,14 @@ struct trav_stack {
     uint2* spill;           /* entry k at spill[k * spill_stride] */
     uint32_t spill_stride;
     uint32_t sp, gsp;       /* entries in LDS, entries spilled */
+    uint32_t cap;           /* entries the lane may hold in all: S without a spill area */
 };
 
 template <uint32_t S, uint32_t LANES>
 BF_D uint32_t stack_depth(const trav_stack<S, LANES>& s) { return s.sp + s.gsp; }
 
 template <uint32_t S, uint32_t LANES>
-BF_D void stack_push(trav_stack<S, LANES>& s, uint32_t ref, float nearv) {
+BF_D void stack_push(trav_stack<S, LANES>& s, uint2 entry) {
     if (S < BF_STACK_DEPTH && s.sp == S) {
 #pragma unroll
         for (uint32_t k = 0; k < S / 2; ++k) s.spill[(size_t)(s.gsp + k) * s.spill_stride] = s.lds[k * LANES];
@@ -383,7 +437,7 @@ BF_D void stack_push(trav_stack<S, LANES>& s, uint32_t ref, float nearv) {
         s.gsp += S / 2;
         s.sp = S / 2;
     }
-    s.lds[s.sp * LANES] = make_uint2(ref, __float_as_uint(nearv));
+    s.lds[s.sp * LANES] = entry;
     ++s.sp;
 }
 
@@ -451,15 +505,16 @@ BF_D uint32_t traverse_begin(const bf_dev_scene& S_, const trav_ray& ray, trav_s
     ts.scene_inst = 0u;
     make_space(ray.o, ray.d, ts.space);
     bool h;
-    double nv;
+    child_near nv;
     const uint32_t st = slab(S_.tlas_root_box[0], S_.tlas_root_box[1], S_.tlas_root_box[2], S_.tlas_root_box[3],
-                             S_.tlas_root_box[4], S_.tlas_root_box[5], ts.space, ts.t_min, ts.t_max, h, nv);
+                             S_.tlas_root_box[4], S_.tlas_root_box[5], ts.space, ts.t_min, ts.t_max, S_.tlas_root_ref,
+                             h, nv);
     if (st != BF_HIT_MISS) return st;
     if (!h) {
         if (COUNT) ctr.nodes++;
         return BF_HIT_MISS;
     }
-    stack_push(stack, S_.tlas_root_ref, round_up_float(nv));
+    stack_push(stack, nv.entry);
     return BF_HIT_MISS;
 }
 
@@ -475,13 +530,20 @@ BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S
         world_space(world, ts.space); /* back in world space (recomputing is exact) */
     }
     const uint2 entry = stack_pop(stack);
-    const uint32_t ref = entry.x;
-    const float nearv = __uint_as_float(entry.y);
+    const uint32_t ref = entry.x & ~BF_NEAR_EXACT;
     if (COUNT) ctr.nodes++;
     const float tcur = ANY ? ts.t_max : best.t;
-    if (!(nearv <= tcur)) return BF_HIT_MISS;
+    /* the reference's pop test: the entry's binary64 near value against the current t_max */
+    uint32_t visit = near_decision(entry, tcur);
+    if (visit == 2u) {
+        const uint32_t index = ref & BF_REF_MASK;
+        const float* box = (ref & BF_WIDE_LEAF) ? (ts.in_blas ? S_.blas_leaf_boxes : S_.tlas_leaf_boxes)
+                                                : (ts.in_blas ? S_.blas_node_boxes : S_.tlas_node_boxes);
+        visit = near_exact(box + 6 * (size_t)index, ts.space, ts.t_min) <= (double)tcur ? 1u : 0u;
+    }
+    if (visit == 0u) return BF_HIT_MISS;
     if (ref & BF_WIDE_LEAF) {
-        const uint32_t leaf = ref & ~BF_WIDE_LEAF;
+        const uint32_t leaf = ref & BF_REF_MASK;
         if (!ts.in_blas) {
             const bf_dev_tlas_leaf L = S_.tlas_leaves[leaf];
             if ((ts.mask & L.visibility_mask) == 0u) return BF_HIT_MISS;
@@ -493,21 +555,22 @@ BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S
             trav_space local;
             make_space(lo, ld, local);
             bool h;
-            double nv;
+            child_near nv;
             const uint32_t st = slab(B.root_box[0], B.root_box[1], B.root_box[2], B.root_box[3], B.root_box[4],
-                                     B.root_box[5], local, ts.t_min, tcur, h, nv);
+                                     B.root_box[5], local, ts.t_min, tcur, B.root_ref, h, nv);
             if (st != BF_HIT_MISS) return st;
             if (!h) {
                 if (COUNT) ctr.nodes++;
                 return BF_HIT_MISS;
             }
-            if (stack_depth(stack) >= BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
+            /* the BLAS root opens the BLAS part of the stack (a fresh stack in the reference) */
+            if (stack_depth(stack) + 1u > stack.cap) return BF_HIT_STACK_OVERFLOW;
             ts.inst_id = L.instance_id;
             ts.scene_inst = L.scene_instance;
             ts.in_blas = 1u;
             ts.blas_base = stack_depth(stack);
             ts.space = local;
-            stack_push(stack, B.root_ref, round_up_float(nv));
+            stack_push(stack, nv.entry);
             return BF_HIT_MISS;
         }
         const float4 q0 = S_.leaf_triangles[3 * (size_t)leaf + 0];
@@ -541,21 +604,36 @@ BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S
     const bf_wide_node* nodes = ts.in_blas ? S_.blas_nodes : S_.tlas_nodes;
     const bf_wide_node N = nodes[ref];
     bool h0, h1;
-    double n0 = 0.0, n1 = 0.0;
-    uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], ts.space, ts.t_min, tcur, h0, n0);
+    child_near n0, n1;
+    uint32_t st = slab(N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0], ts.space, ts.t_min, tcur,
+                       N.child[0], h0, n0);
     if (st != BF_HIT_MISS) return st;
-    st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], ts.space, ts.t_min, tcur, h1, n1);
+    st = slab(N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1], ts.space, ts.t_min, tcur, N.child[1], h1,
+              n1);
     if (st != BF_HIT_MISS) return st;
     if (!h0 && !h1) return BF_HIT_MISS;
+    /* entries of the current structure (TLAS, or the BLAS above blas_base) against its own cap */
+    const uint32_t depth = stack_depth(stack), used = ts.in_blas ? depth - ts.blas_base : depth;
+    const uint32_t pushes = h0 && h1 ? 2u : 1u;
+    if (used + pushes > BF_STRUCTURE_STACK || depth + pushes > stack.cap) return BF_HIT_STACK_OVERFLOW;
     if (h0 && h1) {
-        if (stack_depth(stack) + 2 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
-        const float f0 = (float)n0, f1 = (float)n1;
-        const bool first_near = f0 <= f1; /* ties: the first child has the lower canonical index */
-        stack_push(stack, first_near ? N.child[1] : N.child[0], round_up_float(first_near ? n1 : n0));
-        stack_push(stack, first_near ? N.child[0] : N.child[1], round_up_float(first_near ? n0 : n1));
+        /* near child first: RN32 of the binary64 near values, ties to the first child (the lower
+         * canonical index). Bounds of both roundings decide it unless they overlap; then both
+         * near values are formed exactly. */
+        bool first_near;
+        if (n0.key_hi <= n1.key_lo) first_near = true;
+        else if (n0.key_lo > n1.key_hi) first_near = false;
+        else {
+            const float b0[6] = {N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0]};
+            const float b1[6] = {N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1]};
+            exact_child(N.child[0], near_exact(b0, ts.space, ts.t_min), n0);
+            exact_child(N.child[1], near_exact(b1, ts.space, ts.t_min), n1);
+            first_near = n0.key_lo <= n1.key_lo;
+        }
+        stack_push(stack, first_near ? n1.entry : n0.entry);
+        stack_push(stack, first_near ? n0.entry : n1.entry);
     } else {
-        if (stack_depth(stack) + 1 > BF_STACK_DEPTH) return BF_HIT_STACK_OVERFLOW;
-        stack_push(stack, h0 ? N.child[0] : N.child[1], round_up_float(h0 ? n0 : n1));
+        stack_push(stack, h0 ? n0.entry : n1.entry);
     }
     return BF_HIT_MISS;
 }

@@ -81,6 +81,9 @@ def library():
     lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
                                                              C.POINTER(abi.TransportOptions), vp, vp, vp, vp,
                                                              C.POINTER(abi.TransportReport), vp, vp]
+    lib.blackframe_hip_launch_wavefront_audit_stage.argtypes = [vp, u32, u32, u32, u32, vp, vp]
+    lib.blackframe_hip_light_power_cdf.argtypes = [vp, u32, vp]
+    lib.blackframe_hip_scene_set_light_powers.argtypes = [vp, vp, u32]
     lib.blackframe_hip_trace_wavefront_paths.argtypes = [vp, vp, C.POINTER(abi.PathInputs),
                                                          C.POINTER(abi.TransportOptions), C.POINTER(abi.PathOutputs),
                                                          C.POINTER(abi.TransportReport), vp]
@@ -125,6 +128,17 @@ class XpuScene:
         data = bytes(blob)
         _check(self._lib.blackframe_hip_scene_create_from_soa(data, len(data), abi.SCENE_SOA_VERIFY_HASH if verify_hash else 0,
                                                               C.byref(self.handle)))
+        return self
+
+    def set_light_powers(self, powers):
+        """Power-weighted light selection (blackframe_hip_scene_set_light_powers) from [lights][4]
+        spectral powers in registry order (punctual lights, then mesh-area lights); None returns
+        the scene to uniform selection."""
+        if powers is None:
+            _check(self._lib.blackframe_hip_scene_set_light_powers(self.handle, None, 0))
+        else:
+            p = np.ascontiguousarray(powers, dtype=np.float32).reshape(-1, 4)
+            _check(self._lib.blackframe_hip_scene_set_light_powers(self.handle, C.c_void_p(p.ctypes.data), len(p)))
         return self
 
     def close(self):
@@ -329,6 +343,16 @@ def trace_xpu_wavefront_transport(workspace, scene, frame, options, film=None, w
     elif want_paths:
         result["paths"] = (abi.PathResult * n_paths).from_buffer_copy(d_paths.cpu().numpy().tobytes())
     return result
+
+
+def light_power_cdf(powers):
+    """blackframe_hip_light_power_cdf: the power-weighted selection CDF (count + 1 boundaries) of
+    [count][4] spectral powers; raises XpuError with the reference's status on rejection."""
+    p = np.ascontiguousarray(powers, dtype=np.float32).reshape(-1, 4)
+    out = np.zeros(len(p) + 1, dtype=np.float32)
+    _check(library().blackframe_hip_light_power_cdf(C.c_void_p(p.ctypes.data) if len(p) else C.c_void_p(out.ctypes.data),
+                                                     len(p), C.c_void_p(out.ctypes.data)))
+    return out
 
 
 def _device_records(records):
@@ -553,6 +577,21 @@ def queue_compaction_scratch_bytes(max_input_count):
     out = C.c_size_t()
     _check(library().blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(max_input_count, C.byref(out)))
     return out.value
+
+
+def audit_stage(outcomes, allowed_route_mask, path_capacity, stage_kind):
+    """blackframe_hip_launch_wavefront_audit_stage on host outcomes (uint32 [n, 4]: status, route,
+    path_slot, detail) copied to the device; returns the StageAudit read back."""
+    import torch
+
+    outcomes = np.ascontiguousarray(outcomes, dtype=np.uint32).reshape(-1, 4)
+    d_out = torch.from_numpy(outcomes.view(np.int32).copy()).cuda() if outcomes.size else None
+    d_audit = torch.full((C.sizeof(abi.StageAudit) // 4,), -1, dtype=torch.int32, device="cuda")
+    _check(library().blackframe_hip_launch_wavefront_audit_stage(
+        C.c_void_p(d_out.data_ptr()) if d_out is not None else None, len(outcomes), allowed_route_mask, path_capacity,
+        stage_kind, C.c_void_p(d_audit.data_ptr()), _stream_handle()))
+    torch.cuda.synchronize()
+    return abi.StageAudit.from_buffer_copy(d_audit.cpu().numpy().tobytes())
 
 
 def queue_compaction(headers, slots, outcomes, route, scratch_bytes=None, library_scratch=False):

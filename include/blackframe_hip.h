@@ -72,6 +72,33 @@ int blackframe_hip_scene_bvh_serialize(const bf_hip_scene* scene, void* out, siz
  */
 int blackframe_hip_scene_bvh_validate(const void* blob, size_t byte_count, uint32_t flags, uint32_t* status);
 
+/*
+ * The power-weighted light selection CDF of `count` lights from their four-lane spectral powers
+ * (host, [count][4] floats, registry order: punctual lights, then mesh-area lights): lanes scaled by
+ * the global peak and summed per light, compensated binary32 prefix sums over the compensated
+ * total, the last light with power closing at exactly 1, lights without power repeating the
+ * previous boundary. `boundaries` (host) receives count + 1 floats; light i is selected with
+ * probability boundaries[i + 1] - boundaries[i]. Rejections carry the reference's status classes
+ * (no lights, black or non-finite powers: BF_INVALID_ARGUMENT; unrepresentable weights or
+ * intervals: BF_RESOURCE_EXHAUSTED). Host only.
+ * Replaces: renderer::LightSampler::create_power_weighted (Renderer/Include/Blackframe/Renderer/
+ *           LightSampler.hpp:112-113; Renderer/Source/LightSampler.cpp:116-249, 605-624).
+ */
+int blackframe_hip_light_power_cdf(const float* spectral_powers, uint32_t count, float* boundaries);
+
+/*
+ * Binds a power-weighted light sampler to a device scene: the transport then selects the light of
+ * each light sample from the CDF above (LightSampler::sample, LightSampler.cpp:1023-1057) and
+ * weighs emissive hits with that light's selection probability (::probability, :1133-1151; an
+ * emitter without power fails its path there, as the reference's scalar MIS does). count must be
+ * the scene's registered light count; spectral_powers == NULL returns the scene to uniform
+ * selection. Synchronises the device when it replaces a bound table.
+ * Replaces: the CudaWavefrontLightSampler argument of trace_cuda_wavefront_transport
+ *           (WavefrontTransport.hpp:160-161, 197-208), whose CUDA path accepts only the uniform
+ *           strategy (WavefrontTransport.cpp:575-630).
+ */
+int blackframe_hip_scene_set_light_powers(bf_hip_scene* scene, const float* spectral_powers, uint32_t count);
+
 /* Copies the device-resident 64-byte wide nodes (host memory): counts = {blas wide, tlas wide}. */
 int blackframe_hip_scene_wide_export(const bf_hip_scene* scene, uint64_t counts[2], bf_wide_node* blas_wide,
                                      bf_wide_node* tlas_wide);
@@ -350,6 +377,20 @@ int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const 
                                                      uint32_t input_count, uint32_t route, void* scratch,
                                                      size_t scratch_bytes, bf_compaction_result* device_result,
                                                      void* stream);
+
+/*
+ * Audits `work_count` device stage outcomes into one device bf_stage_audit: an outcome fails when
+ * its status is not success, its route is unknown or outside allowed_route_mask (bit = route), or
+ * its path slot is not below path_capacity; first_failure_work_index is the smallest failing index
+ * (UINT32_MAX when none) and first_failure that outcome; for the shade stage the outcomes' closure-
+ * and light-sample detail bits are counted. An invalid stage kind or a null pointer returns
+ * BF_INVALID_ARGUMENT before any launch. Asynchronous on `stream`.
+ * Replaces: blackframe_cuda_launch_wavefront_audit_stage (XPU/CUDA/Include/Blackframe/XPU/CUDA/
+ *           WavefrontStageKernel.hpp:398-401; Source/CudaWavefrontStages.cu:4945-5033, 5274-5293).
+ */
+int blackframe_hip_launch_wavefront_audit_stage(const bf_stage_outcome* outcomes, uint32_t work_count,
+                                                uint32_t allowed_route_mask, uint32_t path_capacity,
+                                                uint32_t stage_kind, bf_stage_audit* audit, void* stream);
 
 /*
  * blackframe_hip_launch_wavefront_queue_compaction without a scratch argument: the same kernels,

@@ -208,6 +208,11 @@ enum bf_stage_kind {
     BF_STAGE_KIND_INTERSECTION_CLASSIFY = 3, BF_STAGE_KIND_HIT = 4, BF_STAGE_KIND_MISS = 5, BF_STAGE_KIND_SHADE = 6,
     BF_STAGE_KIND_SHADOW_GATHER = 7, BF_STAGE_KIND_SHADOW_PROCESS = 8, BF_STAGE_KIND_CONTINUATION = 9
 };
+#define BF_STAGE_AUDIT_ABI_MAJOR 1u
+#define BF_STAGE_AUDIT_ABI_MINOR 0u
+/* bf_stage_outcome.detail bits of a shade outcome (WavefrontStageKernel.hpp:82-83) */
+#define BF_SHADE_DETAIL_CLOSURE_SAMPLED (1u << 30)
+#define BF_SHADE_DETAIL_LIGHT_SAMPLED (1u << 31)
 #define BF_LANE_CONTINUATION_PENDING (1u << 0)
 #define BF_LANE_SHADOW_PENDING (1u << 1)
 

@@ -48,6 +48,22 @@ int bfo_scene_bvh(const bfo_scene* scene, uint64_t counts[5], bf_bvh_blas* blase
 uint32_t bfo_scene_tlas_root(const bfo_scene* scene);
 /* mesh-area-light tables: per light (instance index, triangle count, inverse area) */
 uint32_t bfo_scene_light_count(const bfo_scene* scene);
+/* The scalar per-path integrator (Engine/Source/SceneMisPathLoop.cpp:632-1016): one path at a time,
+ * brute-force scene queries (checked against the BVH; near-coplanar ties take the BVH's answer and
+ * are counted per path in path_ties, optional), the light sample's shadow ray resolved at its
+ * vertex. One thread. A query where brute force and BVH disagree otherwise fails the path (70). */
+int bfo_render_scalar(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                      uint64_t path_begin, uint64_t path_count, bf_path_result* paths, bf_transport_report* report,
+                      uint32_t* path_ties);
+/* Power-weighted light selection (Renderer/Source/LightSampler.cpp:116-249, 605-624): cdf receives
+ * n + 1 boundaries from n four-lane spectral powers; 0 or a bf_status. */
+int bfo_light_power_cdf(const float* powers, uint32_t n, float* cdf);
+/* Binds the power CDF of [count][4] powers (registry order: punctual, then mesh lights) to the
+ * scene's transport; powers == NULL returns it to uniform selection. */
+int bfo_scene_set_light_powers(bfo_scene* scene, const float* powers, uint32_t count);
+/* LightSampler::sample (:1023-1057) and ::probability (:1133-1151) over a CDF; 1 = ok. */
+int bfo_light_cdf_sample(const float* cdf, uint32_t n, float u, uint32_t* index, float* probability);
+int bfo_light_cdf_probability(const float* cdf, uint32_t n, uint32_t i, float* probability);
 int bfo_scene_light(const bfo_scene* scene, uint32_t light, uint32_t* instance, uint32_t* triangle_count,
                     float* inverse_area, float* areas, float* probabilities, float* next_probabilities);
 int bfo_triangle_area(const float vertices[9], float* area, float normal[3]);

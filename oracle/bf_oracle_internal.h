@@ -77,6 +77,8 @@ struct bfo_scene {
     uint32_t punctual_count;
     bf_punctual_desc* punctuals;
     float* punctual_cosines;
+    /* power-weighted light selection CDF (punctual_count + light_count + 1 boundaries), or NULL */
+    float* light_cdf;
 };
 
 /* vector helpers restated from Blackframe/XPU/CUDA/Source/CudaWavefrontStages.cu:298-349 */

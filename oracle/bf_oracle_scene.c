@@ -38,12 +38,14 @@ static void centroid_of(const float* b, double* c) {
     for (int a = 0; a < 3; ++a) c[a] = ((double)b[a] + (double)b[3 + a]) * 0.5;
 }
 
-static uint32_t sort_axis;
-static int compare_items(const void* lp, const void* rp) {
+/* (centroid on the split axis, stable key): the axis comes in as the comparator's argument, so
+ * builds on several threads do not share state */
+static int compare_items(const void* lp, const void* rp, void* axis_arg) {
+    const uint32_t axis = *(const uint32_t*)axis_arg;
     const bfo_build_item* l = (const bfo_build_item*)lp;
     const bfo_build_item* r = (const bfo_build_item*)rp;
-    if (l->centroid[sort_axis] < r->centroid[sort_axis]) return -1;
-    if (r->centroid[sort_axis] < l->centroid[sort_axis]) return 1;
+    if (l->centroid[axis] < r->centroid[axis]) return -1;
+    if (r->centroid[axis] < l->centroid[axis]) return 1;
     return l->stable_key < r->stable_key ? -1 : (l->stable_key > r->stable_key ? 1 : 0);
 }
 
@@ -105,8 +107,7 @@ static uint32_t build_node(bfo_build_item* items, size_t begin, size_t end, buil
             axis = a;
         }
     }
-    sort_axis = axis;
-    qsort(items + begin, end - begin, sizeof(bfo_build_item), compare_items);
+    qsort_r(items + begin, end - begin, sizeof(bfo_build_item), compare_items, &axis);
     const size_t middle = begin + (end - begin) / 2;
     const uint32_t first = build_node(items, begin, middle, ctx);
     const uint32_t second = build_node(items, middle, end, ctx);
@@ -600,6 +601,7 @@ void bfo_scene_free(bfo_scene* s) {
     free(s->spectrum_values);
     free(s->punctuals);
     free(s->punctual_cosines);
+    free(s->light_cdf);
     for (uint32_t t = 0; t < BF_MAX_TEXTURES; ++t) free(s->texels[t]);
     free(s->blases);
     free(s->blas_nodes);

@@ -844,7 +844,9 @@ static int sample_light(const ctx* c, const lane* l, const surface_t* surf, uint
     const float u_sel = bfo_sample_1d(sampler, &l->sample, first_dim + 0);
     uint32_t li;
     float sel;
-    if (!uniform_light_selection(count, u_sel, &li, &sel)) return 0;
+    if (s->light_cdf ? !bfo_light_cdf_sample(s->light_cdf, count, u_sel, &li, &sel)
+                     : !uniform_light_selection(count, u_sel, &li, &sel))
+        return 0;
     if (li < s->punctual_count) return sample_punctual_light(s, li, l, surf, sel, inc);
     li -= s->punctual_count;
     const bfo_light* light = &s->lights[li];
@@ -922,7 +924,10 @@ static int emissive_hit_mis_weight(const ctx* c, const lane* l, const surface_t*
     const int32_t li = s->instance_light[l->hit_instance];
     if (li < 0) return 0;
     float sel;
-    if (!uniform_light_probability(s->punctual_count + s->light_count, s->punctual_count + (uint32_t)li, &sel)) return 0;
+    const uint32_t count = s->punctual_count + s->light_count, slot = s->punctual_count + (uint32_t)li;
+    if (s->light_cdf ? !bfo_light_cdf_probability(s->light_cdf, count, slot, &sel)
+                     : !uniform_light_probability(count, slot, &sel))
+        return 0;
     const float inv_area = s->lights[li].inverse_area;
     segment_t seg;
     if (!scaled_segment(v3_sub(surf->position, v3_make(l->prev_ctx[0], l->prev_ctx[1], l->prev_ctx[2])), &seg)) return 0;
@@ -1695,6 +1700,109 @@ static int render(ctx* c0, const bfo_scene* scene, const bf_frame_desc* frame, c
     free(shadow_q);
     free(cont_q);
     return BF_OK;
+}
+
+/* ---- the scalar per-path integrator (Engine/Source/SceneMisPathLoop.cpp:632-1016) -------------
+ * The reference's second opinion on its wavefront: each path runs to its end before the next one
+ * starts, and every scene query is brute force over every triangle of every instance
+ * (Engine/Source/AccelBackend.cpp:20-140) instead of a BVH. The per-vertex steps (emission with
+ * MIS, depth filter, closure and light samples, throughput update, roulette) are the shared
+ * renderer functions both loops call; what differs is the order of work and the scene queries: the
+ * light sample's shadow ray is resolved at its vertex, before the continuation, where the
+ * wavefront defers it to a later queue. An occlusion query is a brute-force closest hit in the
+ * shadow ray's range. */
+/* A brute-force closest hit checked against the canonical BVH traversal of the same ray: they agree
+ * except on near-coplanar ties (both hit, parameters within 4 ulps), which the traversal resolves
+ * by its visit order (its slab test prunes against exact plane distances while the triangle test
+ * returns a rounded parameter); the reference's answer is the traversal's, so a tie takes it and is
+ * counted. Any other disagreement returns 0. */
+static int scene_query(const bfo_scene* s, const bf_transport_ray* ray, bf_closest_hit_result* r, uint32_t* ties) {
+    bf_closest_hit_result bvh;
+    uint32_t ctr[3];
+    bfo_closest_hit_brute_force(s, ray, 1, r);
+    bfo_closest_hit(s, ray, 1, &bvh, ctr);
+    if (memcmp(r, &bvh, sizeof bvh) == 0) return 1;
+    if (r->status != BF_HIT_HIT || bvh.status != BF_HIT_HIT) return 0;
+    uint32_t a, b;
+    memcpy(&a, &r->hit.parameter, 4);
+    memcpy(&b, &bvh.hit.parameter, 4);
+    if ((a > b ? a - b : b - a) > 4u) return 0;
+    *r = bvh;
+    ++*ties;
+    return 1;
+}
+
+static void scalar_path(const ctx* c, lane* l, uint64_t global_path, uint64_t* queries, uint32_t* ties) {
+    camera_lane(c, l, global_path);
+    while (!l->failed && l->route == ROUTE_RAY) {
+        bf_closest_hit_result r;
+        ++queries[0];
+        if (!scene_query(c->scene, &l->ray, &r, ties)) return fail(l, 70);
+        if (r.status == BF_HIT_MISS) {
+            miss_lane(c, l);
+            return;
+        }
+        if (r.status != BF_HIT_HIT) return fail(l, 60 + r.status);
+        l->hit = r.hit;
+        for (uint32_t i = 0; i < c->scene->desc.instance_count; ++i)
+            if (c->scene->instances[i].id == r.hit.ids.instance) {
+                l->hit_instance = i;
+                l->hit_geometry = c->scene->instances[i].geometry_index;
+                break;
+            }
+        shade_lane(c, l);
+        if (l->failed) return;
+        if (l->route == ROUTE_SHADOW) {
+            bf_closest_hit_result o;
+            ++queries[1];
+            if (!scene_query(c->scene, &l->shadow_ray, &o, ties)) return fail(l, 70);
+            shadow_lane(l, o.status == BF_HIT_HIT ? BF_OCCLUDED : (o.status == BF_HIT_MISS ? BF_VISIBLE : o.status));
+        }
+        if (l->route == ROUTE_CONT) l->route = ROUTE_RAY;
+    }
+}
+
+int bfo_render_scalar(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,
+                      uint64_t path_begin, uint64_t path_count, bf_path_result* paths, bf_transport_report* report,
+                      uint32_t* path_ties) {
+    if (!scene || !frame || !options || !paths || path_count == 0) return BF_INVALID_ARGUMENT;
+    ctx c;
+    memset(&c, 0, sizeof c);
+    c.scene = scene;
+    c.frame = frame;
+    c.options = options;
+    bf_transport_report rep;
+    memset(&rep, 0, sizeof rep);
+    rep.path_count = path_count;
+    rep.first_failure_slot = BF_INVALID_INDEX;
+    uint64_t queries[2] = {0, 0};
+    for (uint64_t i = 0; i < path_count; ++i) {
+        lane l;
+        memset(&l, 0, sizeof l);
+        uint32_t ties = 0;
+        scalar_path(&c, &l, path_begin + i, queries, &ties);
+        if (path_ties) path_ties[i] = ties;
+        if (l.failed) {
+            if (rep.failed_paths == 0) {
+                rep.first_failure_slot = (uint32_t)i;
+                rep.first_failure_code = l.fail_code;
+            }
+            rep.failed_paths++;
+        }
+        bf_path_result* p = &paths[i];
+        memcpy(p->radiance, l.L, sizeof p->radiance);
+        memcpy(p->beta, l.beta, sizeof p->beta);
+        memcpy(p->wavelengths, l.wl, sizeof p->wavelengths);
+        p->depth = l.depth;
+        p->termination = l.termination;
+        p->depth_counters = (l.diffuse & 0xFFu) | ((l.glossy & 0xFFu) << 8) | ((l.specular & 0xFFu) << 16) |
+                            ((l.transmission & 0xFFu) << 24);
+        p->eta_scale = l.eta_scale;
+    }
+    rep.closest_hit_queries = queries[0];
+    rep.shadow_queries = queries[1];
+    if (report) *report = rep;
+    return 0;
 }
 
 int bfo_render(const bfo_scene* scene, const bf_frame_desc* frame, const bf_transport_options* options,

@@ -44,6 +44,12 @@ def load(abi):
                                   C.c_void_p]
     lib.bfo_scene_tlas_root.restype = C.c_uint32
     lib.bfo_scene_tlas_root.argtypes = [C.c_void_p]
+    lib.bfo_render_scalar.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
+                                      C.c_uint64, C.c_void_p, C.POINTER(abi.TransportReport), C.c_void_p]
+    lib.bfo_light_power_cdf.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p]
+    lib.bfo_scene_set_light_powers.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32]
+    lib.bfo_light_cdf_sample.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
+    lib.bfo_light_cdf_probability.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]
     lib.bfo_scene_light_count.restype = C.c_uint32
     lib.bfo_scene_light_count.argtypes = [C.c_void_p]
     lib.bfo_scene_light.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
@@ -211,12 +217,40 @@ class Oracle:
         self.abi = abi
         self.lib = load(abi)
 
+    def power_sampler(self, powers):
+        return PowerLightSampler(self, powers)
+
     def scene(self, scene):
         handle = C.c_void_p()
         status = self.lib.bfo_scene_build(C.byref(scene.desc), C.byref(handle))
         if status != 0:
             raise RuntimeError(f"oracle scene build failed with status {status}")
         return OracleScene(self, handle)
+
+
+class PowerLightSampler:
+    """The reference's power-weighted LightSampler restated (liboracle): boundaries from four-lane
+    spectral powers, sample(u) -> (index, probability), probability(i)."""
+
+    def __init__(self, oracle, powers):
+        self.lib = oracle.lib
+        self.powers = np.ascontiguousarray(powers, dtype=np.float32).reshape(-1, 4)
+        self.count = len(self.powers)
+        self.cdf = np.zeros(self.count + 1, dtype=np.float32)
+        self.status = self.lib.bfo_light_power_cdf(self.powers.ctypes.data, self.count, self.cdf.ctypes.data)
+
+    def sample(self, u):
+        index, probability = C.c_uint32(), C.c_float()
+        if not self.lib.bfo_light_cdf_sample(self.cdf.ctypes.data, self.count, float(np.float32(u)), C.byref(index),
+                                             C.byref(probability)):
+            return None
+        return index.value, probability.value
+
+    def probability(self, i):
+        probability = C.c_float()
+        if not self.lib.bfo_light_cdf_probability(self.cdf.ctypes.data, self.count, i, C.byref(probability)):
+            return None
+        return probability.value
 
 
 class OracleScene:
@@ -230,6 +264,35 @@ class OracleScene:
         if getattr(self, "handle", None):
             self.lib.bfo_scene_free(self.handle)
             self.handle = None
+
+    def render_scalar(self, frame, options, path_begin=0, path_count=None):
+        """The scalar per-path integrator (one path at a time, brute-force scene queries checked against
+        the BVH): returns (PathResult array, report, per-path count of near-coplanar ties)."""
+        cam = frame.camera
+        n = path_count if path_count is not None else cam.width * cam.height * frame.samples_per_pixel
+        paths = (self.abi.PathResult * n)()
+        report = self.abi.TransportReport()
+        ties = np.zeros(n, dtype=np.uint32)
+        status = self.lib.bfo_render_scalar(self.handle, C.byref(frame), C.byref(options), path_begin, n,
+                                            C.addressof(paths), C.byref(report), ties.ctypes.data)
+        if status != 0:
+            raise RuntimeError(f"oracle scalar render failed with status {status}")
+        return paths, report, ties
+
+    def light_count(self):
+        """Mesh-area lights of the scene (punctual lights are registered before them)."""
+        return self.lib.bfo_scene_light_count(self.handle)
+
+    def set_light_powers(self, powers):
+        """Power-weighted light selection from [lights][4] spectral powers (None: uniform)."""
+        if powers is None:
+            status = self.lib.bfo_scene_set_light_powers(self.handle, None, 0)
+        else:
+            p = np.ascontiguousarray(powers, dtype=np.float32).reshape(-1, 4)
+            status = self.lib.bfo_scene_set_light_powers(self.handle, p.ctypes.data, len(p))
+        if status != 0:
+            raise RuntimeError(f"oracle light powers rejected with status {status}")
+        return self
 
     def bvh(self):
         counts = (C.c_uint64 * 5)()

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_entry_point(hip):
 
 def test_record_sizes(abi):
     sizes = {abi.SampleStreamIndex: 24, abi.TransportRay: 48, abi.ClosestHitResult: 80, abi.OcclusionResult: 16,
-             abi.BvhNode: 48, abi.BvhBlas: 32, abi.BvhInstanceRef: 32, abi.ClosureRecord: 64}
+             abi.BvhNode: 48, abi.BvhBlas: 32, abi.BvhInstanceRef: 32, abi.ClosureRecord: 64, abi.StageAudit: 64}
     for record, size in sizes.items():
         assert C.sizeof(record) == size, record.__name__
 
