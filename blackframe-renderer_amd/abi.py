@@ -230,6 +230,14 @@ FAILURES_ABORT, FAILURES_COUNT = 0, 1
 LAUNCH_GRAPH = 0x80000000
 LAUNCH_GROUP_MASK = 0x0000FFFF
 LAUNCH_SHADE_ORDER = 0x00100000
+LAUNCH_TRACE_ORDER = 0x00200000
+LAUNCH_INFLIGHT_MASK = 0x01C00000
+
+
+def launch_in_flight(k):
+    """The launch-word field limiting the paths in flight per launch to capacity >> k."""
+    return (k & 7) << 22
+
 
 
 def launch_trace_block(lanes):

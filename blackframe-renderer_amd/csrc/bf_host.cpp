@@ -1590,7 +1590,7 @@ int validate_options(const bf_transport_options* o) {
     if (block_log2 != 0 && (block_log2 < 6 || block_log2 > 8))
         return set_error(BF_INVALID_ARGUMENT, "traversal workgroups of 2^%u lanes are not available (64, 128 or 256)", block_log2);
     if ((o->launch & ~(BF_LAUNCH_GRAPH | BF_LAUNCH_TRACE_BLOCK_MASK | BF_LAUNCH_GROUP_MASK | BF_LAUNCH_SHADE_ORDER |
-                       BF_LAUNCH_TRACE_ORDER)) != 0)
+                       BF_LAUNCH_TRACE_ORDER | BF_LAUNCH_INFLIGHT_MASK)) != 0)
         return set_error(BF_INVALID_ARGUMENT, "unknown launch bits 0x%08x", o->launch);
     return BF_OK;
 }
@@ -1639,6 +1639,8 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     const int ost = validate_options(o);
     if (ost != BF_OK) return ost;
     const bool adjoint = adjoints != nullptr;
+    /* paths in flight per launch: the workspace, or a power-of-two fraction of it (launch bits 22-24) */
+    const uint64_t in_flight = w->capacity >> ((o->launch & BF_LAUNCH_INFLIGHT_MASK) >> 22);
     /* tape bits are indexed by path vertex: at most diffuse + glossy + specular + 1 vertices */
     const uint64_t vertices = (uint64_t)o->diffuse_depth_limit + o->glossy_depth_limit + o->specular_depth_limit + 1u;
     int device = 0;
@@ -1657,9 +1659,9 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
                              "job %u: the scene's spectra are point-sampled; render in frame-packet mode at their wavelengths", j);
         const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
         if (pixels == 0 || J.frame.samples_per_pixel == 0) return set_error(BF_INVALID_ARGUMENT, "job %u: empty frame", j);
-        if (pixels > w->capacity)
-            return set_error(BF_RESOURCE_EXHAUSTED, "workspace capacity %llu is below one sample pass (%llu paths)",
-                             (unsigned long long)w->capacity, (unsigned long long)pixels);
+        if (pixels > in_flight)
+            return set_error(BF_RESOURCE_EXHAUSTED, "%llu paths in flight per launch are below one sample pass (%llu paths)",
+                             (unsigned long long)in_flight, (unsigned long long)pixels);
         if (!adjoint && !inputs && (!J.film || !J.film_compensated || !J.film_counts))
             return set_error(BF_INVALID_ARGUMENT, "job %u: film buffers are required", j);
         if ((J.tape_radiance == nullptr) != (J.tape_visibility == nullptr))
@@ -1783,7 +1785,7 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
             const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
             const uint32_t remaining = J.frame.samples_per_pixel - next_pass[j];
             if (remaining == 0) continue;
-            const uint64_t fit = (w->capacity - used) / pixels;
+            const uint64_t fit = (in_flight - used) / pixels;
             if (fit == 0) break;
             const uint32_t take = (uint32_t)std::min<uint64_t>(remaining, fit);
             chunk.push_back({j, next_pass[j], take});
@@ -2136,6 +2138,14 @@ int blackframe_hip_autotune_launch(bf_hip_workspace* w, const bf_frame_job* jobs
         const uint32_t shape = out.launch;
         const uint32_t orders[3] = {BF_LAUNCH_SHADE_ORDER, BF_LAUNCH_TRACE_ORDER, BF_LAUNCH_SHADE_ORDER | BF_LAUNCH_TRACE_ORDER};
         for (uint32_t k = 0; k < 3 && status == BF_OK; ++k) status = consider(shape | orders[k]);
+        /* stage 3: paths in flight per launch (the whole workspace, a half, a quarter), while every
+         * trial frame's sample pass still fits */
+        uint64_t largest_pass = 0;
+        for (const bf_frame_job& J : trial)
+            largest_pass = std::max<uint64_t>(largest_pass, (uint64_t)J.frame.camera.width * J.frame.camera.height);
+        const uint32_t best = out.launch;
+        for (uint32_t k = 1; k <= 2 && status == BF_OK; ++k)
+            if ((w->capacity >> k) >= largest_pass) status = consider(best | BF_LAUNCH_INFLIGHT(k));
     }
     if (start) (void)hipEventDestroy(start);
     if (stop) (void)hipEventDestroy(stop);

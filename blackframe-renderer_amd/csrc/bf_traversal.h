@@ -153,6 +153,9 @@ BF_D uint32_t slab_exact(float lx, float ly, float lz, float hx, float hy, float
 struct child_near {
     uint2 entry;
     float key_lo, key_hi;
+    uint32_t tie_axis; /* 0-2: that axis's near plane alone sets the binary64 near value; 3: t_min
+                        * does; 4: not known */
+    float tie_plane;   /* that plane's coordinate */
 };
 
 BF_D float round_up_float(double v) {
@@ -166,6 +169,8 @@ BF_D float round_up_float(double v) {
 BF_D void exact_child(uint32_t ref, double v, child_near& c) {
     c.entry = make_uint2(ref | BF_NEAR_EXACT, __float_as_uint(round_up_float(v)));
     c.key_lo = c.key_hi = (float)v;
+    c.tie_axis = 4u;
+    c.tie_plane = 0.0f;
 }
 
 /*
@@ -190,8 +195,9 @@ BF_D void bounded_child(uint32_t ref, float lo, float hi, child_near& c) {
  * interval end lies within e = 4u max|q~| + max |1/d| 2^-147 + 2^-126 of its binary32 estimate,
  * and when the estimated interval is empty (or non-empty) by more than 2e the binary64 test
  * reaches the same verdict. Only a box within that band takes binary64 arithmetic; a certain hit
- * returns its near value as the binary32 interval [nv - e, nv + e] (widened outward by one ulp
- * each side, and never below t_min, which bounds the binary64 value from below).
+ * returns its near value as the binary32 interval [nv - en, nv + en], en = 4u nv + the absolute
+ * terms (widened outward by one ulp each side, and never below t_min, which bounds the binary64
+ * value from below).
  */
 BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, const trav_space& s, float t_min,
                    float t_max, uint32_t ref, bool& hit, child_near& near) {
@@ -199,7 +205,9 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
     const float hi[3] = {hx, hy, hz};
     const float oo[3] = {s.o.x, s.o.y, s.o.z};
     const float dd[3] = {s.d.x, s.d.y, s.d.z};
-    float nv = t_min, fv = t_max, scale = 0.0f;
+    /* nv: the largest near estimate (t_min included), second: the next one, axis: nv's source */
+    float nv = t_min, second = -BF_INFINITY_F, fv = t_max, scale = 0.0f, plane = 0.0f;
+    uint32_t axis = 3u;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         if (dd[a] == 0.0f) {
@@ -212,7 +220,15 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
         const float inv = s.finv[a];
         const float q1 = (lo[a] - oo[a]) * inv;
         const float q2 = (hi[a] - oo[a]) * inv;
-        nv = fmaxf(nv, fminf(q1, q2));
+        const float qn = fminf(q1, q2);
+        if (qn > nv) {
+            second = nv;
+            nv = qn;
+            axis = (uint32_t)a;
+            plane = dd[a] > 0.0f ? lo[a] : hi[a];
+        } else {
+            second = fmaxf(second, qn);
+        }
         fv = fminf(fv, fmaxf(q1, q2));
         scale = fmaxf(scale, fmaxf(fabsf(q1), fabsf(q2)));
     }
@@ -222,13 +238,23 @@ BF_D uint32_t slab(float lx, float ly, float lz, float hx, float hy, float hz, c
             hit = false;
             return BF_HIT_MISS;
         }
-        const float up = nv + e; /* > 0: nv >= t_min >= 0, e > 0 */
+        /* The near value alone has a tighter bound than the interval test: an axis whose near
+         * estimate lies below t_min by more than its own error contributes t_min exactly, and every
+         * other axis's near quotient is at most nv (or tiny), so its error is under 4u nv plus the
+         * absolute terms. A ray nearly parallel to an axis (huge quotients there) keeps tight near
+         * bounds this way. */
+        const float en = 4.0f * 0x1p-24f * nv + s.efloor + 0x1p-126f;
+        const float up = nv + en; /* > 0: nv >= t_min >= 0, en > 0 */
         if (fv - nv > 2.0f * e && up < 0x1p127f) {
-            const float low = nv - e;
+            const float low = nv - en;
             hit = true;
             /* one ulp outward on each side (both positive where they bind) */
             bounded_child(ref, low > t_min ? __uint_as_float(__float_as_uint(low) - 1u) : t_min,
                           __uint_as_float(__float_as_uint(up) + 1u), near);
+            /* every estimate is within en of its binary64 term, so a lead of more than 2 en makes
+             * the source of nv the source of the binary64 near value too */
+            near.tie_axis = nv - second > 2.0f * en ? axis : 4u;
+            near.tie_plane = plane;
             return BF_HIT_MISS;
         }
     }
@@ -623,6 +649,8 @@ BF_D uint32_t traverse_step(const bf_dev_scene& S_, WorldRay world, trav_stack<S
         bool first_near;
         if (n0.key_hi <= n1.key_lo) first_near = true;
         else if (n0.key_lo > n1.key_hi) first_near = false;
+        else if (n0.tie_axis < 4u && n0.tie_axis == n1.tie_axis && (n0.tie_axis == 3u || n0.tie_plane == n1.tie_plane))
+            first_near = true; /* both near values are t_min, or the same plane's quotient: a tie */
         else {
             const float b0[6] = {N.lo_x[0], N.lo_y[0], N.lo_z[0], N.hi_x[0], N.hi_y[0], N.hi_z[0]};
             const float b1[6] = {N.lo_x[1], N.lo_y[1], N.lo_z[1], N.hi_x[1], N.hi_y[1], N.hi_z[1]};

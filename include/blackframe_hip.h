@@ -274,8 +274,10 @@ typedef struct bf_launch_tuning {
  * the call allocates; the jobs' own buffers are not touched) for every combination of traversal
  * workgroup size {64, 128, 256}, bounce group size {2, 4, 8} and graph replay {off, on} (graph
  * replay only when `options` allow it), then the fastest of those with the closure-kind shade
- * order, the direction-octant traversal order and both, with hipEvents on `stream`, and returns
- * the fastest launch word. options->launch is ignored; every other option is used as given. Synchronises.
+ * order, the direction-octant traversal order and both, then the fastest of all with half and a
+ * quarter of the workspace's paths in flight per launch (BF_LAUNCH_INFLIGHT, where a sample pass
+ * of every frame still fits), with hipEvents on `stream`, and returns the fastest launch word.
+ * options->launch is ignored; every other option is used as given. Synchronises.
  */
 int blackframe_hip_autotune_launch(bf_hip_workspace* workspace, const bf_frame_job* jobs, uint32_t job_count,
                                    const bf_transport_options* options, uint32_t trial_passes,

@@ -712,6 +712,7 @@ typedef struct bf_transport_options {
      *   bits 16-19 log2 of the traversal workgroup size: 6, 7 or 8 (0: 7, 128 lanes);
      *   bit 20     BF_LAUNCH_SHADE_ORDER: shade the hit queue grouped by closure kind;
      *   bit 21     BF_LAUNCH_TRACE_ORDER: traverse rays grouped by direction octant;
+     *   bits 22-24 k: at most capacity >> k paths in flight per launch (0: the whole workspace);
      *   bit 31     BF_LAUNCH_GRAPH: each bounce group is one replayed hipGraph (size rounded up
      *              to even). */
     uint32_t launch;
@@ -723,6 +724,8 @@ typedef struct bf_transport_options {
 #define BF_LAUNCH_TRACE_BLOCK(log2_lanes) ((uint32_t)(log2_lanes) << 16)
 #define BF_LAUNCH_SHADE_ORDER 0x00100000u
 #define BF_LAUNCH_TRACE_ORDER 0x00200000u
+#define BF_LAUNCH_INFLIGHT_MASK 0x01C00000u
+#define BF_LAUNCH_INFLIGHT(k) (((uint32_t)(k) & 7u) << 22)
 
 /* What a numerically failed path (termination BF_TERM_FAILED) does to the call. The reference
  * aborts the batch on the first failing work item (WavefrontTransport.cpp:880-925), which is

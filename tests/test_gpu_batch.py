@@ -88,19 +88,40 @@ def test_traversal_workgroup_sizes_bit_exact(bf, oracle, hip, lanes, order):
 
 
 def test_autotuner_choice_is_valid_and_bit_exact(bf, hip):
-    """The autotuner times 18 launch shapes, then three work orders on the best one, and returns
-    the fastest; rendering with it gives the same bits as the default launch."""
+    """The autotuner times 18 launch shapes, then three work orders on the best one, then half and
+    a quarter of the workspace in flight where a sample pass still fits, and returns the fastest;
+    rendering with it gives the same bits as the default launch."""
     items = frames(bf)
     options = bf.scenes.transport_options(max_depth=10, mis=bf.abi.MIS_POWER)
     xscenes = [hip.XpuScene(scene) for scene, _ in items]
     ws = hip.XpuWavefrontTransportWorkspace(1 << 16)
     tuning = hip.autotune_launch(ws, [(x, f) for x, (_, f) in zip(xscenes, items)], options, trial_passes=2)
-    assert tuning.candidates == 21
+    largest = max(f.camera.width * f.camera.height for _, f in items)
+    fractions = sum(1 for k in (1, 2) if (1 << 16) >> k >= largest)
+    assert tuning.candidates == 21 + fractions
     assert tuning.best_ms <= tuning.default_ms and tuning.best_ms > 0.0
-    assert tuning.launch == 0 or tuning.launch in list(tuning.tried)[:21]
+    assert tuning.launch == 0 or tuning.launch in list(tuning.tried)[:tuning.candidates]
+    for k in range(21, tuning.candidates):
+        assert tuning.tried[k] & bf.abi.LAUNCH_INFLIGHT_MASK == bf.abi.launch_in_flight(k - 20)
     reference = hip.trace_xpu_wavefront_batch(ws, [(x, f) for x, (_, f) in zip(xscenes, items)], options, want_paths=True)
     options.launch = tuning.launch
     tuned = hip.trace_xpu_wavefront_batch(ws, [(x, f) for x, (_, f) in zip(xscenes, items)], options, want_paths=True)
     for k in range(len(items)):
         assert np.array_equal(path_arrays(tuned["paths"][k]), path_arrays(reference["paths"][k]))
         assert np.array_equal(tuned["films"][k].host()["film"], reference["films"][k].host()["film"])
+
+
+def test_paths_in_flight_fraction_is_bit_exact(bf, hip):
+    """A launch word holding a quarter of the workspace in flight splits the batch into more
+    launches of whole sample passes; every frame's paths and film are those of the default."""
+    items = frames(bf)
+    options = bf.scenes.transport_options(max_depth=10, mis=bf.abi.MIS_POWER)
+    xscenes = [hip.XpuScene(scene) for scene, _ in items]
+    ws = hip.XpuWavefrontTransportWorkspace(8192)  # 7 456 paths: one launch, or several of <= 2 048
+    jobs = [(x, f) for x, (_, f) in zip(xscenes, items)]
+    reference = hip.trace_xpu_wavefront_batch(ws, jobs, options, want_paths=True)
+    options.launch = bf.abi.launch_in_flight(2)
+    split = hip.trace_xpu_wavefront_batch(ws, jobs, options, want_paths=True)
+    for k in range(len(items)):
+        assert np.array_equal(path_arrays(split["paths"][k]), path_arrays(reference["paths"][k]))
+        assert np.array_equal(split["films"][k].host()["film"], reference["films"][k].host()["film"])
