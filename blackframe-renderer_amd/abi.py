@@ -227,6 +227,8 @@ class FrameDesc(C.Structure):
 MAX_BOUNCES = 64
 INSTRUMENT_DISABLED, INSTRUMENT_STAGE_EVENTS = 0, 1
 FAILURES_ABORT, FAILURES_COUNT = 0, 1
+HIT_MISS, HIT_HIT, HIT_INVALID_RAY, HIT_INVALID_SCENE, HIT_INVALID_BVH = 0, 1, 2, 3, 4
+HIT_STACK_OVERFLOW, HIT_INVALID_TOPOLOGY, HIT_NUMERICAL_FAILURE = 5, 6, 7
 LAUNCH_GRAPH = 0x80000000
 LAUNCH_GROUP_MASK = 0x0000FFFF
 LAUNCH_SHADE_ORDER = 0x00100000

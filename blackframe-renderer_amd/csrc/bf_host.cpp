@@ -2242,6 +2242,32 @@ int blackframe_hip_launch_wavefront_queue_compaction(bf_queue_soa queues, const 
     return BF_OK;
 }
 
+namespace {
+int blob_traverse(bool any, const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes, size_t bvh_size,
+                  const bf_transport_ray* rays, uint32_t ray_count, void* results, void* stream_handle) {
+    if (!scene_bytes || scene_size < sizeof(bf_scene_soa_header) || !bvh_bytes || bvh_size < sizeof(bf_bvh_header) ||
+        (ray_count != 0u && (!rays || !results)))
+        return set_error(BF_INVALID_ARGUMENT,
+                         "scene and BVH blobs of at least their header sizes, and rays and results when rays are given");
+    if (ray_count == 0u) return BF_OK;
+    HIP_TRY(bfk_blob_traverse(any, scene_bytes, scene_size, bvh_bytes, bvh_size, rays, ray_count, results,
+                              (hipStream_t)stream_handle));
+    return BF_OK;
+}
+} // namespace
+
+int blackframe_hip_launch_scene_closest_hit_blob(const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes,
+                                                 size_t bvh_size, const bf_transport_ray* rays, uint32_t ray_count,
+                                                 bf_closest_hit_result* results, void* stream) {
+    return blob_traverse(false, scene_bytes, scene_size, bvh_bytes, bvh_size, rays, ray_count, results, stream);
+}
+
+int blackframe_hip_launch_scene_occlusion_blob(const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes,
+                                               size_t bvh_size, const bf_transport_ray* rays, uint32_t ray_count,
+                                               bf_occlusion_result* results, void* stream) {
+    return blob_traverse(true, scene_bytes, scene_size, bvh_bytes, bvh_size, rays, ray_count, results, stream);
+}
+
 int blackframe_hip_launch_wavefront_audit_stage(const bf_stage_outcome* outcomes, uint32_t work_count,
                                                 uint32_t allowed_route_mask, uint32_t path_capacity,
                                                 uint32_t stage_kind, bf_stage_audit* audit, void* stream_handle) {

@@ -129,6 +129,10 @@ uint32_t bfk_shade_resident_blocks(int device);
 hipError_t bfk_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, unsigned long long* out, hipStream_t s);
 hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
                                  bf_closest_hit_result* results, uint32_t* counters, hipStream_t s);
+/* traversal over the reference's device blobs (scene "BLKFSOA4", BVH "BLKFBVH1"); results are
+ * bf_occlusion_result when `any`, bf_closest_hit_result otherwise */
+hipError_t bfk_blob_traverse(bool any, const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes,
+                             size_t bvh_size, const bf_transport_ray* rays, uint32_t count, void* results, hipStream_t s);
 hipError_t bfk_scene_occlusion(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
                                bf_occlusion_result* results, hipStream_t s);
 hipError_t bfk_sample_dump(uint32_t kind, const uint64_t* directions, uint32_t dims, const bf_sample_stream_index* indices,

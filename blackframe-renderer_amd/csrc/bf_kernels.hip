@@ -1952,6 +1952,385 @@ __global__ __launch_bounds__(BF_API_BLOCK) void k_scene_occlusion(bf_dev_scene S
     results[i] = out;
 }
 
+/* ---- stand-alone traversal over the reference's device blobs ----------------------------------
+ * blackframe_hip_launch_scene_closest_hit_blob / _occlusion_blob take the reference launcher's
+ * arguments (SceneClosestHit.hpp, CudaSceneClosestHit.cu:188-1330): "BLKFSOA4" scene bytes and a
+ * "BLKFBVH1" BVH in device memory. The first lane of every workgroup validates both layouts
+ * (validate_scene_layout / validate_bvh_layout, :199-358) and a bad one answers every ray of the
+ * workgroup with invalid_scene / invalid_bvh. Each lane then traverses one ray over the canonical
+ * 48-byte nodes in the reference's order: pop, re-test the node's own box against the current
+ * t_max, push both intersecting children far then near (ties to the first), enter an instance leaf
+ * in object space after checking it against the scene's columns, and test a BLAS leaf's triangle
+ * from the scene's index and position columns. The box and triangle decisions are this library's
+ * binary64 slab and watertight test, so the results equal those of
+ * blackframe_hip_launch_scene_closest_hit on the scene the blobs describe. The TLAS and each BLAS
+ * get their own 64-entry LDS stack, as in the reference. */
+struct blob_view {
+    const uint8_t* scene;
+    const bf_scene_soa_header* sh;
+    const uint8_t* bvh;
+    const bf_bvh_header* bh;
+};
+
+template <typename T>
+BF_D const T* soa_col(const blob_view& b, uint32_t column) {
+    return (const T*)(b.scene + b.sh->columns[column].offset_bytes);
+}
+
+template <typename T>
+BF_D const T* bvh_arr(const blob_view& b, uint32_t array) {
+    return (const T*)(b.bvh + b.bh->arrays[array].offset_bytes);
+}
+
+BF_D uint64_t blob_align(uint64_t v, uint64_t a) { return (v + a - 1u) / a * a; }
+
+/* validate_scene_layout (:199-250) */
+BF_D uint32_t blob_scene_layout(const uint8_t* bytes, size_t n) {
+    if (n < sizeof(bf_scene_soa_header)) return BF_HIT_INVALID_SCENE;
+    const bf_scene_soa_header& h = *(const bf_scene_soa_header*)bytes;
+    if (h.magic != BF_SCENE_SOA_MAGIC || h.abi_major != BF_SCENE_SOA_ABI_MAJOR || h.abi_minor != BF_SCENE_SOA_ABI_MINOR ||
+        h.header_size != sizeof(bf_scene_soa_header) || h.column_count != BF_SCENE_SOA_COLUMN_COUNT ||
+        h.hash_algorithm != BF_SCENE_SOA_HASH_FNV1A64 || h.environment_count > 1u || h.total_size_bytes != n ||
+        h.total_size_bytes < sizeof(bf_scene_soa_header) || h.texture_count > 0xFFFFFFFFull ||
+        h.image_texture_count > 0xFFFFFFFFull || h.image_mip_count > 0xFFFFFFFFull)
+        return BF_HIT_INVALID_SCENE;
+    for (int k = 0; k < 5; ++k)
+        if (h.reserved[k] != 0u) return BF_HIT_INVALID_SCENE;
+    const uint64_t max = ~0ull;
+    uint64_t cursor = blob_align(sizeof(bf_scene_soa_header), BF_SCENE_SOA_COLUMN_ALIGNMENT);
+    for (uint32_t c = 0; c < BF_SCENE_SOA_COLUMN_COUNT; ++c) {
+        const bf_scene_soa_column& d = h.columns[c];
+        const uint64_t count = bf_soa::column_count(h, c);
+        const uint64_t size = bf_soa::column_element_size(c);
+        if (d.element_count != count || d.element_size != size || d.reserved != 0u) return BF_HIT_INVALID_SCENE;
+        if (count == 0u) {
+            if (d.offset_bytes != 0u) return BF_HIT_INVALID_SCENE;
+            continue;
+        }
+        if (cursor > max - (BF_SCENE_SOA_COLUMN_ALIGNMENT - 1u)) return BF_HIT_INVALID_SCENE;
+        cursor = blob_align(cursor, BF_SCENE_SOA_COLUMN_ALIGNMENT);
+        if (d.offset_bytes != cursor || count > max / size) return BF_HIT_INVALID_SCENE;
+        const uint64_t bytes_in_column = count * size;
+        if (cursor > max - bytes_in_column) return BF_HIT_INVALID_SCENE;
+        cursor += bytes_in_column;
+    }
+    return cursor == h.total_size_bytes ? BF_HIT_MISS : BF_HIT_INVALID_SCENE;
+}
+
+/* validate_bvh_layout (:291-358) */
+BF_D uint32_t blob_bvh_layout(const uint8_t* bytes, size_t n, const bf_scene_soa_header& scene) {
+    if (n < sizeof(bf_bvh_header)) return BF_HIT_INVALID_BVH;
+    const bf_bvh_header& h = *(const bf_bvh_header*)bytes;
+    const uint64_t invalid = BF_INVALID_INDEX;
+    if (h.magic != BF_BVH_MAGIC || h.abi_major != BF_BVH_ABI_MAJOR || h.abi_minor != BF_BVH_ABI_MINOR ||
+        h.header_size != sizeof(bf_bvh_header) || h.array_count != BF_BVH_ARRAY_COUNT ||
+        h.hash_algorithm != BF_BVH_HASH_FNV1A64 || h.leaf_capacity != BF_BVH_LEAF_CAPACITY || h.total_size_bytes != n ||
+        h.total_size_bytes < sizeof(bf_bvh_header) || h.source_scene_hash != scene.content_hash ||
+        h.blas_count != scene.geometry_count || h.primitive_reference_count != scene.triangle_count ||
+        h.instance_reference_count != scene.instance_count || h.blas_count > invalid || h.blas_node_count > invalid ||
+        h.primitive_reference_count > invalid || h.tlas_node_count > invalid || h.instance_reference_count > invalid)
+        return BF_HIT_INVALID_BVH;
+    if ((h.blas_count == 0u && (h.blas_node_count != 0u || h.primitive_reference_count != 0u)) ||
+        (h.blas_count != 0u && (h.primitive_reference_count < h.blas_count ||
+                                h.blas_node_count != h.primitive_reference_count * 2u - h.blas_count)) ||
+        (h.instance_reference_count == 0u && h.tlas_node_count != 0u) ||
+        (h.instance_reference_count != 0u && h.tlas_node_count != h.instance_reference_count * 2u - 1u) ||
+        (h.tlas_node_count == 0u && h.tlas_root_node != BF_INVALID_INDEX) ||
+        (h.tlas_node_count != 0u && h.tlas_root_node != 0u))
+        return BF_HIT_INVALID_BVH;
+    for (int k = 0; k < 5; ++k)
+        if (h.reserved[k] != 0u) return BF_HIT_INVALID_BVH;
+    const uint64_t counts[BF_BVH_ARRAY_COUNT] = {h.blas_count, h.blas_node_count, h.primitive_reference_count,
+                                                 h.tlas_node_count, h.instance_reference_count};
+    const uint64_t sizes[BF_BVH_ARRAY_COUNT] = {sizeof(bf_bvh_blas), sizeof(bf_bvh_node), sizeof(uint32_t),
+                                                sizeof(bf_bvh_node), sizeof(bf_bvh_instance_ref)};
+    const uint64_t max = ~0ull;
+    uint64_t cursor = blob_align(sizeof(bf_bvh_header), BF_BVH_ARRAY_ALIGNMENT);
+    for (uint32_t a = 0; a < BF_BVH_ARRAY_COUNT; ++a) {
+        const bf_bvh_array_descriptor& d = h.arrays[a];
+        if (d.element_count != counts[a] || d.element_size != sizes[a] || d.reserved != 0u) return BF_HIT_INVALID_BVH;
+        if (counts[a] == 0u) {
+            if (d.offset_bytes != 0u) return BF_HIT_INVALID_BVH;
+            continue;
+        }
+        if (cursor > max - (BF_BVH_ARRAY_ALIGNMENT - 1u)) return BF_HIT_INVALID_BVH;
+        cursor = blob_align(cursor, BF_BVH_ARRAY_ALIGNMENT);
+        if (d.offset_bytes != cursor || counts[a] > max / sizes[a]) return BF_HIT_INVALID_BVH;
+        const uint64_t bytes_in_array = counts[a] * sizes[a];
+        if (cursor > max - bytes_in_array) return BF_HIT_INVALID_BVH;
+        cursor += bytes_in_array;
+    }
+    return cursor == h.total_size_bytes ? BF_HIT_MISS : BF_HIT_INVALID_BVH;
+}
+
+/* intersect_bounds (:407-455): the node's box, validated, in the reference's binary64 slab */
+BF_D uint32_t blob_box(const bf_bvh_node& n, const trav_space& s, float t_min, float t_max, bool& hit, float& nearv) {
+    if (!bf_isfinite(n.min_x) || !bf_isfinite(n.min_y) || !bf_isfinite(n.min_z) || !bf_isfinite(n.max_x) ||
+        !bf_isfinite(n.max_y) || !bf_isfinite(n.max_z) || n.min_x > n.max_x || n.min_y > n.max_y || n.min_z > n.max_z)
+        return BF_HIT_INVALID_TOPOLOGY;
+    double v = 0.0;
+    const uint32_t st = slab_exact(n.min_x, n.min_y, n.min_z, n.max_x, n.max_y, n.max_z, s, t_min, t_max, hit, v);
+    nearv = (float)v;
+    return st;
+}
+
+/* push_intersecting_children (:466-512) */
+BF_D uint32_t blob_push_children(const bf_bvh_node* nodes, const bf_bvh_node& n, uint32_t begin, uint32_t count,
+                                 const trav_space& s, float t_min, float t_max, uint32_t* stack, uint32_t& size) {
+    const uint64_t end = (uint64_t)begin + count;
+    const uint32_t a = n.first_child, b = n.second_child;
+    if (a == b || a < begin || b < begin || a >= end || b >= end) return BF_HIT_INVALID_TOPOLOGY;
+    bool ha, hb;
+    float na = 0.0f, nb = 0.0f;
+    uint32_t st = blob_box(nodes[a], s, t_min, t_max, ha, na);
+    if (st != BF_HIT_MISS) return st;
+    st = blob_box(nodes[b], s, t_min, t_max, hb, nb);
+    if (st != BF_HIT_MISS) return st;
+    if (!ha && !hb) return BF_HIT_MISS;
+    if (ha != hb) {
+        if (size >= BF_STRUCTURE_STACK) return BF_HIT_STACK_OVERFLOW;
+        stack[(size++) * BF_API_BLOCK] = ha ? a : b;
+        return BF_HIT_MISS;
+    }
+    const bool a_near = na < nb || (na == nb && a < b);
+    if (size >= BF_STRUCTURE_STACK) return BF_HIT_STACK_OVERFLOW;
+    stack[(size++) * BF_API_BLOCK] = a_near ? b : a;
+    if (size >= BF_STRUCTURE_STACK) return BF_HIT_STACK_OVERFLOW;
+    stack[(size++) * BF_API_BLOCK] = a_near ? a : b;
+    return BF_HIT_MISS;
+}
+
+/* a 3x4 affine matrix from one of the scene's 16-element matrix column groups (:513-537) */
+BF_D uint32_t blob_matrix(const blob_view& b, uint32_t first_column, uint32_t instance, float* m) {
+    float full[16];
+    for (uint32_t e = 0; e < 16u; ++e) {
+        full[e] = soa_col<float>(b, first_column + e)[instance];
+        if (!bf_isfinite(full[e])) return BF_HIT_NUMERICAL_FAILURE;
+    }
+    if (full[12] != 0.0f || full[13] != 0.0f || full[14] != 0.0f || full[15] != 1.0f) return BF_HIT_INVALID_TOPOLOGY;
+    for (uint32_t e = 0; e < 12u; ++e) m[e] = full[e];
+    return BF_HIT_MISS;
+}
+
+struct blob_candidate {
+    bool present;
+    float t;
+    uint32_t instance_id, primitive;
+    bf_closest_hit hit;
+};
+
+/* traverse_blas (:895-1085) */
+template <bool ANY>
+BF_D uint32_t blob_blas(const blob_view& b, const bf_bvh_instance_ref& ref, const bf_transport_ray& ray, v3f lo, v3f ld,
+                        uint32_t* stack, blob_candidate& best, bool& occluded) {
+    const bf_bvh_blas* blases = bvh_arr<bf_bvh_blas>(b, 0);
+    const bf_bvh_node* nodes = bvh_arr<bf_bvh_node>(b, 1);
+    const uint32_t* prims = bvh_arr<uint32_t>(b, 2);
+    if (ref.blas_index >= b.bh->blas_count || ref.scene_geometry_index >= b.sh->geometry_count)
+        return BF_HIT_INVALID_TOPOLOGY;
+    const bf_bvh_blas B = blases[ref.blas_index];
+    const uint64_t node_end = (uint64_t)B.node_offset + B.node_count;
+    const uint64_t ref_end = (uint64_t)B.primitive_reference_offset + B.primitive_reference_count;
+    if (B.reserved[0] != 0u || B.reserved[1] != 0u || B.geometry_id != ref.geometry_id || B.root_node != B.node_offset ||
+        B.node_count == 0u || node_end > b.bh->blas_node_count ||
+        B.node_count != (uint64_t)B.primitive_reference_count * 2u - 1u || ref_end > b.bh->primitive_reference_count)
+        return BF_HIT_INVALID_TOPOLOGY;
+    const uint32_t g = ref.scene_geometry_index;
+    const uint64_t vertex_offset = soa_col<uint64_t>(b, bf_soa::C_GEOMETRY_VERTEX_OFFSET)[g];
+    const uint64_t vertex_count = soa_col<uint64_t>(b, bf_soa::C_GEOMETRY_VERTEX_COUNT)[g];
+    const uint64_t triangle_offset = soa_col<uint64_t>(b, bf_soa::C_GEOMETRY_TRIANGLE_OFFSET)[g];
+    const uint64_t triangle_count = soa_col<uint64_t>(b, bf_soa::C_GEOMETRY_TRIANGLE_COUNT)[g];
+    if (soa_col<uint32_t>(b, bf_soa::C_GEOMETRY_ID)[g] != ref.geometry_id || triangle_count != B.primitive_reference_count ||
+        vertex_count == 0u || vertex_offset > b.sh->vertex_count || vertex_count > b.sh->vertex_count - vertex_offset ||
+        triangle_offset > b.sh->triangle_count || triangle_count > b.sh->triangle_count - triangle_offset)
+        return BF_HIT_INVALID_TOPOLOGY;
+    trav_space s;
+    make_space(lo, ld, s);
+    trav_shear sh;
+    make_shear(ld, sh);
+    uint32_t size = 0, visited = 0;
+    stack[(size++) * BF_API_BLOCK] = B.root_node;
+    while (size != 0u) {
+        if (visited++ >= B.node_count) return BF_HIT_INVALID_TOPOLOGY;
+        const uint32_t index = stack[(--size) * BF_API_BLOCK];
+        if (index < B.node_offset || index >= node_end) return BF_HIT_INVALID_TOPOLOGY;
+        const bf_bvh_node n = nodes[index];
+        const float t_max = ANY ? ray.t_max : best.t;
+        bool h;
+        float nearv;
+        uint32_t st = blob_box(n, s, ray.t_min, t_max, h, nearv);
+        if (st != BF_HIT_MISS) return st;
+        if (!h) continue;
+        if (n.kind == 0u) {
+            if (n.split_axis > 2u || n.reference_offset != 0u || n.reference_count != 0u) return BF_HIT_INVALID_TOPOLOGY;
+            st = blob_push_children(nodes, n, B.node_offset, B.node_count, s, ray.t_min, t_max, stack, size);
+            if (st != BF_HIT_MISS) return st;
+            continue;
+        }
+        if (n.kind != 1u || n.split_axis != BF_INVALID_INDEX || n.first_child != BF_INVALID_INDEX ||
+            n.second_child != BF_INVALID_INDEX || n.reference_count != 1u || n.reference_offset < B.primitive_reference_offset ||
+            n.reference_offset >= ref_end)
+            return BF_HIT_INVALID_TOPOLOGY;
+        const uint32_t prim = prims[n.reference_offset];
+        if (prim >= triangle_count) return BF_HIT_INVALID_TOPOLOGY;
+        const uint64_t tri = triangle_offset + prim;
+        v3f v[3];
+        for (uint32_t c = 0; c < 3u; ++c) {
+            const uint32_t local = soa_col<uint32_t>(b, bf_soa::C_TRIANGLE_VERTEX_0 + c)[tri];
+            if (local >= vertex_count) return BF_HIT_INVALID_TOPOLOGY;
+            const uint64_t gv = vertex_offset + local;
+            v[c] = mk(soa_col<float>(b, bf_soa::C_POSITION_X)[gv], soa_col<float>(b, bf_soa::C_POSITION_X + 1)[gv],
+                      soa_col<float>(b, bf_soa::C_POSITION_X + 2)[gv]);
+            if (!fin3(v[c])) return BF_HIT_NUMERICAL_FAILURE;
+        }
+        tri_result tr;
+        tr.hit = false;
+        st = watertight<!ANY>(v[0], v[1], v[2], lo, ld, sh, ray.t_min, t_max, tr);
+        if (st != BF_HIT_MISS) return st;
+        if (!tr.hit) continue;
+        if (ANY) {
+            occluded = true;
+            return BF_HIT_MISS;
+        }
+        if (best.present && !(tr.t < best.t ||
+                              (tr.t == best.t && (ref.instance_id < best.instance_id ||
+                                                  (ref.instance_id == best.instance_id && prim < best.primitive)))))
+            continue;
+        float m[12];
+        st = blob_matrix(b, bf_soa::C_INSTANCE_LOCAL_TO_WORLD, ref.scene_instance_index, m);
+        if (st != BF_HIT_MISS) return st;
+        v3f nrm;
+        if (!world_normal(m, v[0], v[1], v[2], nrm)) return BF_HIT_NUMERICAL_FAILURE;
+        const uint32_t inst = ref.scene_instance_index;
+        if (soa_col<uint32_t>(b, bf_soa::C_INSTANCE_ID)[inst] != ref.instance_id ||
+            soa_col<uint32_t>(b, bf_soa::C_INSTANCE_GEOMETRY_ID)[inst] != ref.geometry_id)
+            return BF_HIT_INVALID_TOPOLOGY;
+        best.present = true;
+        best.t = tr.t;
+        best.instance_id = ref.instance_id;
+        best.primitive = prim;
+        bf_closest_hit& hr = best.hit;
+        memset(&hr, 0, sizeof hr);
+        hr.parameter = tr.t;
+        hr.barycentric0 = tr.b0;
+        hr.barycentric1 = tr.b1;
+        hr.barycentric2 = tr.b2;
+        hr.normal_x = nrm.x;
+        hr.normal_y = nrm.y;
+        hr.normal_z = nrm.z;
+        hr.ids.object = soa_col<uint32_t>(b, bf_soa::C_INSTANCE_OBJECT_ID)[inst];
+        hr.ids.instance = ref.instance_id;
+        hr.ids.geometry = ref.geometry_id;
+        hr.ids.primitive = prim;
+        hr.ids.material = soa_col<uint32_t>(b, bf_soa::C_INSTANCE_MATERIAL_ID)[inst];
+    }
+    return BF_HIT_MISS;
+}
+
+/* trace_scene (:1087-1186) */
+template <bool ANY>
+BF_D uint32_t blob_trace(const blob_view& b, const bf_transport_ray& ray, uint32_t* tlas_stack, uint32_t* blas_stack,
+                         blob_candidate& best, bool& occluded) {
+    if (b.bh->tlas_node_count == 0u) return BF_HIT_MISS;
+    const bf_bvh_node* nodes = bvh_arr<bf_bvh_node>(b, 3);
+    const bf_bvh_instance_ref* refs = bvh_arr<bf_bvh_instance_ref>(b, 4);
+    const uint32_t tlas_count = (uint32_t)b.bh->tlas_node_count;
+    trav_space world;
+    const v3f o = mk(ray.origin_x, ray.origin_y, ray.origin_z), d = mk(ray.direction_x, ray.direction_y, ray.direction_z);
+    make_space(o, d, world);
+    uint32_t size = 0, visited = 0;
+    tlas_stack[(size++) * BF_API_BLOCK] = b.bh->tlas_root_node;
+    while (size != 0u) {
+        if (visited++ >= tlas_count) return BF_HIT_INVALID_TOPOLOGY;
+        const uint32_t index = tlas_stack[(--size) * BF_API_BLOCK];
+        if (index >= tlas_count) return BF_HIT_INVALID_TOPOLOGY;
+        const bf_bvh_node n = nodes[index];
+        const float t_max = ANY ? ray.t_max : best.t;
+        bool h;
+        float nearv;
+        uint32_t st = blob_box(n, world, ray.t_min, t_max, h, nearv);
+        if (st != BF_HIT_MISS) return st;
+        if (!h) continue;
+        if (n.kind == 0u) {
+            if (n.split_axis > 2u || n.reference_offset != 0u || n.reference_count != 0u) return BF_HIT_INVALID_TOPOLOGY;
+            st = blob_push_children(nodes, n, 0u, tlas_count, world, ray.t_min, t_max, tlas_stack, size);
+            if (st != BF_HIT_MISS) return st;
+            continue;
+        }
+        if (n.kind != 1u || n.split_axis != BF_INVALID_INDEX || n.first_child != BF_INVALID_INDEX ||
+            n.second_child != BF_INVALID_INDEX || n.reference_count != 1u ||
+            n.reference_offset >= b.bh->instance_reference_count)
+            return BF_HIT_INVALID_TOPOLOGY;
+        const bf_bvh_instance_ref ref = refs[n.reference_offset];
+        if (ref.reserved[0] != 0u || ref.reserved[1] != 0u || ref.scene_instance_index >= b.sh->instance_count ||
+            ref.scene_geometry_index >= b.sh->geometry_count || ref.blas_index >= b.bh->blas_count)
+            return BF_HIT_INVALID_TOPOLOGY;
+        const uint32_t inst = ref.scene_instance_index;
+        if (soa_col<uint32_t>(b, bf_soa::C_INSTANCE_ID)[inst] != ref.instance_id ||
+            soa_col<uint32_t>(b, bf_soa::C_INSTANCE_GEOMETRY_ID)[inst] != ref.geometry_id ||
+            soa_col<uint32_t>(b, bf_soa::C_INSTANCE_VISIBILITY_MASK)[inst] != ref.visibility_mask)
+            return BF_HIT_INVALID_TOPOLOGY;
+        if ((ray.visibility_mask & ref.visibility_mask) == 0u) continue;
+        float m[12];
+        st = blob_matrix(b, bf_soa::C_INSTANCE_WORLD_TO_LOCAL, inst, m);
+        if (st != BF_HIT_MISS) return st;
+        const v3f lo = xform_point(m, o), ld = xform_vector(m, d);
+        if (!fin3(lo) || !fin3(ld) || (ld.x == 0.0f && ld.y == 0.0f && ld.z == 0.0f)) return BF_HIT_NUMERICAL_FAILURE;
+        st = blob_blas<ANY>(b, ref, ray, lo, ld, blas_stack, best, occluded);
+        if (st != BF_HIT_MISS) return st;
+        if (ANY && occluded) return BF_HIT_MISS;
+    }
+    return BF_HIT_MISS;
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(BF_API_BLOCK) void k_blob_traverse(const uint8_t* scene_bytes, size_t scene_size,
+                                                                const uint8_t* bvh_bytes, size_t bvh_size,
+                                                                const bf_transport_ray* rays, uint32_t count,
+                                                                void* results) {
+    __shared__ uint32_t s_layout;
+    __shared__ uint32_t s_tlas[BF_STRUCTURE_STACK * BF_API_BLOCK];
+    __shared__ uint32_t s_blas[BF_STRUCTURE_STACK * BF_API_BLOCK];
+    if (threadIdx.x == 0u) {
+        uint32_t st = blob_scene_layout(scene_bytes, scene_size);
+        if (st == BF_HIT_MISS) st = blob_bvh_layout(bvh_bytes, bvh_size, *(const bf_scene_soa_header*)scene_bytes);
+        s_layout = st;
+    }
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t status = s_layout;
+    blob_candidate best;
+    best.present = false;
+    bool occluded = false;
+    if (status == BF_HIT_MISS) {
+        const bf_transport_ray r = rays[i];
+        if (!valid_traversal_ray(r)) {
+            status = BF_HIT_INVALID_RAY;
+        } else {
+            const blob_view b{scene_bytes, (const bf_scene_soa_header*)scene_bytes, bvh_bytes,
+                              (const bf_bvh_header*)bvh_bytes};
+            best.t = r.t_max;
+            status = blob_trace<ANY>(b, r, s_tlas + threadIdx.x, s_blas + threadIdx.x, best, occluded);
+        }
+    }
+    if (ANY) {
+        bf_occlusion_result out;
+        memset(&out, 0, sizeof out);
+        out.status = status != BF_HIT_MISS ? status : (occluded ? BF_OCCLUDED : BF_VISIBLE);
+        ((bf_occlusion_result*)results)[i] = out;
+    } else {
+        bf_closest_hit_result out;
+        memset(&out, 0, sizeof out);
+        if (status != BF_HIT_MISS) out.status = status;
+        else if (best.present) {
+            out.status = BF_HIT_HIT;
+            out.hit = best.hit;
+        }
+        ((bf_closest_hit_result*)results)[i] = out;
+    }
+}
+
 /* ---- sampler probe (SampleStreamKernel.hpp:170-176 counterpart) ------------------------------ */
 __global__ void k_sample_dump(sampler_ctx sc, const bf_sample_stream_index* indices, const uint64_t* dims, uint32_t count,
                               float* out) {
@@ -2784,6 +3163,19 @@ hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* 
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_scene_closest_hit, dim3(blocks_for(count, BF_API_BLOCK)), dim3(BF_API_BLOCK), 0, s, *S, rays,
                        count, results, counters);
+    return hipGetLastError();
+}
+
+hipError_t bfk_blob_traverse(bool any, const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes,
+                             size_t bvh_size, const bf_transport_ray* rays, uint32_t count, void* results, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    const dim3 grid(blocks_for(count, BF_API_BLOCK)), block(BF_API_BLOCK);
+    if (any)
+        hipLaunchKernelGGL(k_blob_traverse<true>, grid, block, 0, s, scene_bytes, scene_size, bvh_bytes, bvh_size, rays,
+                           count, results);
+    else
+        hipLaunchKernelGGL(k_blob_traverse<false>, grid, block, 0, s, scene_bytes, scene_size, bvh_bytes, bvh_size, rays,
+                           count, results);
     return hipGetLastError();
 }
 

@@ -82,6 +82,8 @@ def library():
                                                              C.POINTER(abi.TransportOptions), vp, vp, vp, vp,
                                                              C.POINTER(abi.TransportReport), vp, vp]
     lib.blackframe_hip_launch_wavefront_audit_stage.argtypes = [vp, u32, u32, u32, u32, vp, vp]
+    lib.blackframe_hip_launch_scene_closest_hit_blob.argtypes = [vp, C.c_size_t, vp, C.c_size_t, vp, u32, vp, vp]
+    lib.blackframe_hip_launch_scene_occlusion_blob.argtypes = [vp, C.c_size_t, vp, C.c_size_t, vp, u32, vp, vp]
     lib.blackframe_hip_light_power_cdf.argtypes = [vp, u32, vp]
     lib.blackframe_hip_scene_set_light_powers.argtypes = [vp, vp, u32]
     lib.blackframe_hip_trace_wavefront_paths.argtypes = [vp, vp, C.POINTER(abi.PathInputs),
@@ -577,6 +579,27 @@ def queue_compaction_scratch_bytes(max_input_count):
     out = C.c_size_t()
     _check(library().blackframe_hip_query_wavefront_queue_compaction_scratch_bytes(max_input_count, C.byref(out)))
     return out.value
+
+
+def blob_closest_hit(scene_blob, bvh_blob, rays, occlusion=False, scene_size=None, bvh_size=None, stream=None):
+    """blackframe_hip_launch_scene_closest_hit_blob (or _occlusion_blob): the two blobs (bytes, copied
+    to the device) and host rays; scene_size / bvh_size override the byte counts passed (to test
+    mismatches). Returns host results."""
+    import torch
+
+    n = len(rays)
+    d_scene = torch.frombuffer(bytearray(scene_blob), dtype=torch.uint8).cuda()
+    d_bvh = torch.frombuffer(bytearray(bvh_blob), dtype=torch.uint8).cuda()
+    d_rays = torch.frombuffer(bytearray(memoryview(rays).cast("B")), dtype=torch.uint8).cuda()
+    record = abi.OcclusionResult if occlusion else abi.ClosestHitResult
+    d_out = torch.zeros(max(1, n) * C.sizeof(record), dtype=torch.uint8, device="cuda")
+    fn = (library().blackframe_hip_launch_scene_occlusion_blob if occlusion
+          else library().blackframe_hip_launch_scene_closest_hit_blob)
+    _check(fn(C.c_void_p(d_scene.data_ptr()), len(scene_blob) if scene_size is None else scene_size,
+              C.c_void_p(d_bvh.data_ptr()), len(bvh_blob) if bvh_size is None else bvh_size,
+              C.c_void_p(d_rays.data_ptr()), n, C.c_void_p(d_out.data_ptr()), _stream_handle(stream)))
+    torch.cuda.synchronize()
+    return (record * n).from_buffer_copy(d_out.cpu().numpy().tobytes()[: n * C.sizeof(record)])
 
 
 def audit_stage(outcomes, allowed_route_mask, path_capacity, stage_kind):

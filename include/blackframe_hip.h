@@ -116,6 +116,30 @@ int blackframe_hip_launch_scene_occlusion(const bf_hip_scene* scene, const bf_tr
                                           bf_occlusion_result* results, void* stream);
 
 /*
+ * The same traversal in the reference launcher's form: a "BLKFSOA4" scene blob and the "BLKFBVH1"
+ * BVH built from it (its source_scene_hash is the scene's content hash), both in device memory,
+ * `ray_count` device rays into device results. Every workgroup first checks both layouts as the
+ * reference does (validate_scene_layout / validate_bvh_layout) and answers each of its rays with
+ * BF_HIT_INVALID_SCENE or BF_HIT_INVALID_BVH when one fails; each ray then traverses the canonical
+ * 48-byte nodes in the reference's order (a 64-entry stack per structure), entering instances
+ * through the scene's world-to-local columns and testing triangles from its index and position
+ * columns. Results equal blackframe_hip_launch_scene_closest_hit's on the scene the blobs
+ * describe. Null pointers and blobs shorter than their headers are rejected before any launch;
+ * asynchronous on `stream`, nothing allocated.
+ * Replaces: blackframe_cuda_launch_scene_closest_hit (XPU/CUDA/Include/Blackframe/XPU/CUDA/
+ *           SceneClosestHit.hpp; Source/CudaSceneClosestHit.cu:188-1242,1287-1308).
+ */
+int blackframe_hip_launch_scene_closest_hit_blob(const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes,
+                                                 size_t bvh_size, const bf_transport_ray* rays, uint32_t ray_count,
+                                                 bf_closest_hit_result* results, void* stream);
+
+/* Any-hit form of the blob traversal.
+ * Replaces: blackframe_cuda_launch_scene_occlusion (SceneOcclusion.hpp; CudaSceneClosestHit.cu:1243-1286,1310-1330). */
+int blackframe_hip_launch_scene_occlusion_blob(const uint8_t* scene_bytes, size_t scene_size, const uint8_t* bvh_bytes,
+                                               size_t bvh_size, const bf_transport_ray* rays, uint32_t ray_count,
+                                               bf_occlusion_result* results, void* stream);
+
+/*
  * For every device request: the indexed-hash bits and binary32 sample of (sample_stream, dimension)
  * and the ten transport dimensions of bounce_index (status bounce_out_of_range with zero dimensions
  * past BF_SAMPLE_MAXIMUM_BOUNCE); thread 0 also writes the dimension manifest. A schema other than
