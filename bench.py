@@ -477,6 +477,28 @@ def fit_section(bf, hip, args, log):
     first, last = float(losses[0].sum()), float(losses[-1].sum())
     log(f"fit: {steps} steps in {elapsed:.1f} s ({elapsed / steps * 1e3:.1f} ms/step: forward {forward_ms:.1f}, "
         f"loss {loss_ms:.2f}, adjoint {adjoint_ms:.1f}, Adam {adam_ms:.2f} ms); loss {first:.5g} -> {last:.5g}")
+
+    # the same fit continued with each step replayed as one recorded graph
+    graph = None
+    if args.fit_graph_steps > 0:
+        fit.use_graph = True
+        t0 = time.perf_counter()
+        fit.step()
+        torch.cuda.synchronize()
+        record_s = time.perf_counter() - t0
+        g_steps = args.fit_graph_steps
+        torch.cuda.synchronize()
+        start = time.perf_counter()
+        for _ in range(g_steps):
+            g_losses = fit.step()
+        torch.cuda.synchronize()
+        g_elapsed = time.perf_counter() - start
+        forward_report, adjoint_report = fit.graph_reports()
+        log(f"fit: {g_steps} graph steps in {g_elapsed:.1f} s ({g_elapsed / g_steps * 1e3:.1f} ms/step; recording and "
+            f"first replay {record_s:.2f} s); loss {float(g_losses.sum()):.5g}")
+        graph = {"steps": g_steps, "ms_per_step": round(g_elapsed / g_steps * 1e3, 3),
+                 "record_and_first_replay_s": round(record_s, 3), "loss_last_step": float(g_losses.sum()),
+                 "failed_paths": int(forward_report.failed_paths + adjoint_report.failed_paths)}
     return {
         "workload": (f"16 seeded scenes ({triangles} tris in total, 5e4-2e5 each), 256^2-512^2 x 32 spp ({paths} "
                      "paths), per scene 1024^2 RGB albedo + 1024^2 (alpha_x, alpha_y) textures on a Lambert + "
@@ -493,6 +515,7 @@ def fit_section(bf, hip, args, log):
         "adjoint_over_forward": round(float(adjoint_ms / forward_ms), 4) if err <= 1e-2 else None,
         "loss_first_step": first,
         "loss_last_step": last,
+        "graph": graph,
     }
 
 
@@ -527,6 +550,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=5)
     ap.add_argument("--no-fit", action="store_true", help="skip the 16-scene texture fit")
     ap.add_argument("--fit-steps", type=int, default=100, help="Adam steps of the texture fit")
+    ap.add_argument("--fit-graph-steps", type=int, default=20,
+                    help="steps of the fit replayed as one recorded graph each, after the direct steps (0: none)")
     ap.add_argument("--fit-only", action="store_true",
                     help="run only the texture fit (prints its JSON object; for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

@@ -382,6 +382,14 @@ class AdamOptions(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+class FitStepDesc(C.Structure):
+    """bf_fit_step_desc: the buffers of one texture-fit step recorded as one graph."""
+    _fields_ = [("frames", C.POINTER(FrameJob)), ("adjoints", C.POINTER(AdjointJob)), ("losses", C.POINTER(FilmLossJob)),
+                ("loss_scales", C.POINTER(C.c_float)), ("loss_values", C.c_void_p), ("params", C.POINTER(AdamParam)),
+                ("frame_count", C.c_uint32), ("param_count", C.c_uint32), ("adam", AdamOptions),
+                ("max_steps", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class PathResult(C.Structure):
     _fields_ = [
         ("radiance", C.c_float * 4), ("beta", C.c_float * 4), ("wavelengths", C.c_float * 4),
@@ -395,7 +403,8 @@ for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitR
                        (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48), (LayoutManifest, 320),
                        (TransportOptions, 80), (FrameDesc, 112), (SampleDimensionManifest, 160), (SampleDumpRequest, 40),
                        (SampleDumpResult, 104), (PathStateLane, 128), (RayCone, 8), (LaneControl, 16),
-                       (PathInputs, 48), (PathOutputs, 48), (FrameJob, 184), (TransportReport, 1808), (StageMetric, 24)):
+                       (PathInputs, 48), (PathOutputs, 48), (FrameJob, 184), (TransportReport, 1808), (StageMetric, 24),
+                       (FitStepDesc, 96)):
     assert C.sizeof(_record) == _size, (_record.__name__, C.sizeof(_record))
 
 

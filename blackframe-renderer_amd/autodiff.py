@@ -101,7 +101,10 @@ class TextureFit:
          (blackframe_hip_film_l2_loss, image = film sums / sample counts);
       3. one batched path-replay adjoint of every frame (blackframe_hip_trace_wavefront_adjoint);
       4. one projected Adam step over every fitted texture (blackframe_hip_adam_step).
-    The host waits only where the batched calls read live-path counts.
+    Direct calls wait on the host where the batched calls read live-path counts. With graph=True
+    the first step() records the four phases once as one graph (hip.FitGraph) and every step()
+    replays it: no host work and no host read inside a step (forward, loss and adjoint of every
+    replay equal the direct calls', the adjoint's float atomics aside).
 
         fit = TextureFit(frames, options, [(albedo, 0.02, 0.0, 1.0), (widths, 0.01, 0.01, 1.0)])
         for _ in range(100):
@@ -114,8 +117,11 @@ class TextureFit:
     gets the sum of their gradients.
     """
 
-    def __init__(self, frames, options, parameters, workspace=None, betas=(0.9, 0.999), epsilon=1e-8):
+    def __init__(self, frames, options, parameters, workspace=None, betas=(0.9, 0.999), epsilon=1e-8, graph=False,
+                 max_graph_steps=4096):
         self.options = options
+        self.use_graph, self.max_graph_steps = graph, max_graph_steps
+        self.graph = None
         self.betas, self.epsilon = betas, epsilon
         self.parameters = []
         by_pointer = {}
@@ -180,8 +186,26 @@ class TextureFit:
     def step(self):
         """forward, loss, adjoint, update; returns the per-frame losses of this step's render
         (a CUDA float64 tensor, valid once the stream reaches it)."""
+        if self.use_graph:
+            if self.graph is None:
+                self.graph = hip.FitGraph(
+                    self.workspace,
+                    [(f["scene"], f["frame"], f["film"], f["tape"], f["target"], f["film_gradient"], f["gradients"],
+                      1.0 / (3.0 * f["pixels"])) for f in self.frames],
+                    [(p["value"], p["gradient"], p["first"], p["second"], p["lr"], p["lower"], p["upper"])
+                     for p in self.parameters],
+                    self.losses, self.options, step=self.step_count + 1, betas=self.betas, epsilon=self.epsilon,
+                    max_steps=self.max_graph_steps)
+            self.graph.launch()
+            self.step_count += 1
+            return self.losses.clone()
         self.forward()
         losses = self.loss()
         self.adjoint()
         self.update()
         return losses.clone()
+
+    def graph_reports(self):
+        """(forward report, adjoint report) of the last graph replay (waits for the stream)."""
+        forward, adjoint, _ = self.graph.report()
+        return forward, adjoint

@@ -1626,6 +1626,46 @@ int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transpo
     return BF_OK;
 }
 
+/* Whole sample passes of the frames packed into launches of at most `in_flight` paths (and
+ * BF_MAX_BATCH_JOBS frames), in job order: one contiguous slot range per frame and launch. */
+std::vector<std::vector<chunk_job>> pack_launches(const bf_frame_job* jobs, uint32_t job_count, uint64_t in_flight) {
+    std::vector<std::vector<chunk_job>> launches;
+    std::vector<uint32_t> next_pass(job_count, 0u);
+    uint32_t cursor = 0;
+    for (;;) {
+        std::vector<chunk_job> chunk;
+        uint64_t used = 0;
+        for (uint32_t j = cursor; j < job_count && chunk.size() < BF_MAX_BATCH_JOBS; ++j) {
+            const bf_frame_job& J = jobs[j];
+            const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
+            const uint32_t remaining = J.frame.samples_per_pixel - next_pass[j];
+            if (remaining == 0) continue;
+            const uint64_t fit = (in_flight - used) / pixels;
+            if (fit == 0) break;
+            const uint32_t take = (uint32_t)std::min<uint64_t>(remaining, fit);
+            chunk.push_back({j, next_pass[j], take});
+            used += take * pixels;
+            if (take < remaining) break;
+        }
+        if (chunk.empty()) break;
+        for (const chunk_job& c : chunk) next_pass[c.job] += c.passes;
+        while (cursor < job_count && next_pass[cursor] == jobs[cursor].frame.samples_per_pixel) ++cursor;
+        launches.push_back(std::move(chunk));
+    }
+    return launches;
+}
+
+/* Record mode of trace_jobs, for a caller that captures the whole call into one graph: every
+ * launch is issued as copies and kernels only, its job table from a pinned staging row of its own,
+ * all depth-bound + 1 bounces without a host read (an exhausted queue makes a bounce's kernels
+ * exit at once), and the report's lane counts and failure words stay on the device. */
+struct device_loop {
+    bf_dev_job* staging = nullptr;        /* pinned, BF_MAX_BATCH_JOBS job records per launch */
+    uint32_t launches = 0;                /* staging rows */
+    uint32_t used = 0;                    /* rows taken by the launches issued so far */
+    unsigned long long* totals = nullptr; /* device: bfk_history_totals accumulators */
+};
+
 /*
  * The transport driver shared by the single-frame and the batched entry points: frames are packed
  * into launches of whole sample passes (one contiguous slot range per frame and launch), and each
@@ -1633,7 +1673,8 @@ int issue_bounce(bf_hip_workspace* w, bf_transport_report& rep, const bf_transpo
  */
 int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_job* adjoints, uint32_t job_count,
                const bf_transport_options* o, bf_transport_report* report, uint64_t* traversal_counters,
-               hipStream_t stream, const bf_path_inputs* inputs = nullptr, const bf_path_outputs* outputs = nullptr) {
+               hipStream_t stream, const bf_path_inputs* inputs = nullptr, const bf_path_outputs* outputs = nullptr,
+               device_loop* loop = nullptr) {
     if (!w || !jobs || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, jobs or options");
     if (job_count == 0) return set_error(BF_INVALID_ARGUMENT, "empty batch");
     const int ost = validate_options(o);
@@ -1683,10 +1724,13 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         }
     }
 
+    if (loop && (inputs || traversal_counters || o->record_queues || o->instrumentation == BF_INSTRUMENT_STAGE_EVENTS))
+        return set_error(BF_INVALID_ARGUMENT, "a recorded step takes frames without counters, queue records or stage events");
     const bool instrument = o->instrumentation == BF_INSTRUMENT_STAGE_EVENTS;
     uint32_t group = o->launch & BF_LAUNCH_GROUP_MASK;
     if (group == 0) group = 4;
-    const bool use_graph = (o->launch & BF_LAUNCH_GRAPH) != 0 && !instrument && !o->record_queues && !traversal_counters;
+    const bool use_graph =
+        (o->launch & BF_LAUNCH_GRAPH) != 0 && !instrument && !o->record_queues && !traversal_counters && !loop;
     if (use_graph) group += group & 1u; /* even: the ray / continuation queue roles repeat */
 
     bf_transport_params P{};
@@ -1773,27 +1817,13 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         STAGE(BF_STAGE_AUXILIARY, 1,
               bfk_set_words((uint32_t*)w->replicas, BF_GRADIENT_REPLICAS * BF_GRADIENT_ROW, 0u, stream));
 
-    std::vector<uint32_t> next_pass(job_count, 0u);
-    uint32_t cursor = 0;
+    const std::vector<std::vector<chunk_job>> launches = pack_launches(jobs, job_count, in_flight);
+    if (loop && launches.size() > loop->launches - loop->used)
+        return set_error(BF_RESOURCE_EXHAUSTED, "the recorded step has job tables for %u more launches, the frames need %zu",
+                         loop->launches - loop->used, launches.size());
     bool first_launch = true;
-    for (;;) {
-        /* pack whole sample passes of the remaining frames, in job order */
-        std::vector<chunk_job> chunk;
-        uint64_t used = 0;
-        for (uint32_t j = cursor; j < job_count && chunk.size() < BF_MAX_BATCH_JOBS; ++j) {
-            const bf_frame_job& J = jobs[j];
-            const uint64_t pixels = (uint64_t)J.frame.camera.width * J.frame.camera.height;
-            const uint32_t remaining = J.frame.samples_per_pixel - next_pass[j];
-            if (remaining == 0) continue;
-            const uint64_t fit = (in_flight - used) / pixels;
-            if (fit == 0) break;
-            const uint32_t take = (uint32_t)std::min<uint64_t>(remaining, fit);
-            chunk.push_back({j, next_pass[j], take});
-            used += take * pixels;
-            if (take < remaining) break;
-        }
-        if (chunk.empty()) break;
-        const bool batch = chunk.size() > 1 || use_graph;
+    for (const std::vector<chunk_job>& chunk : launches) {
+        const bool batch = chunk.size() > 1 || use_graph || loop;
         uint32_t slot = 0;
         for (size_t c = 0; c < chunk.size(); ++c) {
             const bf_frame_job& J = jobs[chunk[c].job];
@@ -1842,15 +1872,18 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
                 }
             }
             slot += D.slot_count;
-            next_pass[chunk[c].job] += chunk[c].passes;
         }
-        while (cursor < job_count && next_pass[cursor] == jobs[cursor].frame.samples_per_pixel) ++cursor;
         const uint32_t n = slot;
         P.job_count = (uint32_t)chunk.size();
         bool textured = false;
         for (size_t c = 0; c < chunk.size(); ++c) textured = textured || jobs[chunk[c].job].scene->dev.has_textures != 0u;
         P.shade_mode = adjoint ? BF_SHADE_ADJOINT : (textured ? BF_SHADE_TEXTURED : BF_SHADE_PLAIN);
-        if (batch) {
+        if (loop) {
+            /* the graph's copy node reads this launch's own staging row at every replay */
+            bf_dev_job* row = loop->staging + (size_t)loop->used++ * BF_MAX_BATCH_JOBS;
+            std::memcpy(row, w->host_jobs, chunk.size() * sizeof(bf_dev_job));
+            HIP_TRY(hipMemcpyAsync(w->jobs, row, chunk.size() * sizeof(bf_dev_job), hipMemcpyHostToDevice, stream));
+        } else if (batch) {
             HIP_TRY(hipMemcpyAsync(w->jobs, w->host_jobs, chunk.size() * sizeof(bf_dev_job), hipMemcpyHostToDevice, stream));
         } else {
             P.job0 = w->host_jobs[0];
@@ -1875,6 +1908,15 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         const bool record = o->record_queues && first_launch;
         uint32_t bounce = 0;
         bool live = true;
+        if (loop) {
+            for (; bounce <= bound; ++bounce) {
+                const int ist = issue_bounce(w, rep, P, batch, false, n, ray_q, cont_q, nullptr, nullptr, stream);
+                if (ist != BF_OK) return ist;
+                std::swap(ray_q, cont_q);
+            }
+            HIP_TRY(bfk_history_totals(w->history, std::min<uint32_t>(bounce, BF_HISTORY_BOUNCES), loop->totals, stream));
+            live = false;
+        }
         while (live && bounce <= bound) {
             const uint32_t steps = (uint32_t)std::min<uint64_t>(group, bound + 1 - bounce);
             if (use_graph && steps == group) {
@@ -1937,12 +1979,15 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
             live = w->host_words[0] != 0u;
         }
         /* per-bounce sizes of this launch */
-        const uint32_t recorded = std::min<uint32_t>(bounce, BF_HISTORY_BOUNCES);
-        HIP_TRY(hipMemcpyAsync(w->host_history, w->history, (size_t)recorded * 16, hipMemcpyDeviceToHost, stream));
-        if (record)
-            HIP_TRY(hipMemcpyAsync(w->host_hashes, w->hashes, (size_t)std::min<uint32_t>(recorded, BF_REPORT_MAX_BOUNCES) * 16,
-                                   hipMemcpyDeviceToHost, stream));
-        HIP_TRY(hipStreamSynchronize(stream));
+        const uint32_t recorded = loop ? 0u : std::min<uint32_t>(bounce, BF_HISTORY_BOUNCES);
+        if (!loop) {
+            HIP_TRY(hipMemcpyAsync(w->host_history, w->history, (size_t)recorded * 16, hipMemcpyDeviceToHost, stream));
+            if (record)
+                HIP_TRY(hipMemcpyAsync(w->host_hashes, w->hashes,
+                                       (size_t)std::min<uint32_t>(recorded, BF_REPORT_MAX_BOUNCES) * 16,
+                                       hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        }
         uint32_t active_bounces = 0;
         for (uint32_t b = 0; b < recorded; ++b) {
             const uint32_t* h = w->host_history + 4 * b;
@@ -1980,6 +2025,11 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         STAGE(BF_STAGE_AUXILIARY, 1, bfk_reduce_replicas(w->replicas, r.offset, r.count, r.dst, stream));
     for (uint32_t j = 0; j < job_count; ++j)
         rep.path_count += (uint64_t)jobs[j].frame.camera.width * jobs[j].frame.camera.height * jobs[j].frame.samples_per_pixel;
+    if (loop) {
+        /* the failure words and sample counts stay on the device for the caller to copy */
+        if (report) *report = rep;
+        return BF_OK;
+    }
     HIP_TRY(hipMemcpyAsync(w->host_words, w->failure, 16, hipMemcpyDeviceToHost, stream));
     HIP_TRY(hipMemcpyAsync(w->host_words + 4, w->sample_counts, 8, hipMemcpyDeviceToHost, stream));
     unsigned long long trav[6] = {0, 0, 0, 0, 0, 0};
@@ -2297,6 +2347,49 @@ int blackframe_hip_launch_wavefront_queue_compaction_static(bf_queue_soa queues,
     return BF_OK;
 }
 
+} /* extern "C" */
+
+namespace {
+
+int validate_adam(const bf_adam_param* params, uint32_t param_count, const bf_adam_options* o) {
+    if (!params || !o) return set_error(BF_INVALID_ARGUMENT, "NULL parameters or options");
+    if (o->step == 0) return set_error(BF_INVALID_ARGUMENT, "Adam steps count from 1");
+    if (o->reserved != 0) return set_error(BF_INVALID_ARGUMENT, "Adam options: reserved word must be zero");
+    if (!(o->beta1 >= 0.0 && o->beta1 < 1.0) || !(o->beta2 >= 0.0 && o->beta2 < 1.0))
+        return set_error(BF_INVALID_ARGUMENT, "Adam betas must lie in [0, 1)");
+    if (!(o->epsilon > 0.0) || !std::isfinite(o->epsilon)) return set_error(BF_INVALID_ARGUMENT, "Adam epsilon must be positive");
+    for (uint32_t k = 0; k < param_count; ++k) {
+        const bf_adam_param& P = params[k];
+        if (P.reserved != 0) return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: reserved word must be zero", k);
+        if (P.count && (!P.value || !P.gradient || !P.first_moment || !P.second_moment))
+            return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: NULL buffer", k);
+        if (!std::isfinite(P.learning_rate) || !(P.lower <= P.upper))
+            return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: bad learning rate or bounds", k);
+    }
+    return BF_OK;
+}
+
+/* The float coefficients of one parameter at step number `step`, from binary64 bias corrections. */
+bool adam_coefficients(const bf_adam_options& o, const bf_adam_param& P, uint64_t step, bfk_adam_coefficients& c) {
+    const double bias1 = 1.0 - std::pow(o.beta1, (double)step);
+    const double bias2 = 1.0 - std::pow(o.beta2, (double)step);
+    if (!(bias1 > 0.0) || !(bias2 > 0.0)) return false;
+    c = bfk_adam_coefficients{(float)o.beta1,
+                              (float)(1.0 - o.beta1),
+                              (float)o.beta2,
+                              (float)(1.0 - o.beta2),
+                              (float)((double)P.learning_rate / bias1),
+                              (float)std::sqrt(bias2),
+                              (float)o.epsilon,
+                              P.lower,
+                              P.upper};
+    return true;
+}
+
+} /* namespace */
+
+extern "C" {
+
 int blackframe_hip_film_l2_loss(const bf_film_loss_job* jobs, uint32_t job_count, float scale, double* losses,
                                 void* stream_handle) {
     if (job_count == 0) return BF_OK;
@@ -2322,39 +2415,246 @@ int blackframe_hip_film_l2_loss(const bf_film_loss_job* jobs, uint32_t job_count
 int blackframe_hip_adam_step(const bf_adam_param* params, uint32_t param_count, const bf_adam_options* o,
                              void* stream_handle) {
     if (param_count == 0) return BF_OK;
-    if (!params || !o) return set_error(BF_INVALID_ARGUMENT, "NULL parameters or options");
-    if (o->step == 0) return set_error(BF_INVALID_ARGUMENT, "Adam steps count from 1");
-    if (o->reserved != 0) return set_error(BF_INVALID_ARGUMENT, "Adam options: reserved word must be zero");
-    if (!(o->beta1 >= 0.0 && o->beta1 < 1.0) || !(o->beta2 >= 0.0 && o->beta2 < 1.0))
-        return set_error(BF_INVALID_ARGUMENT, "Adam betas must lie in [0, 1)");
-    if (!(o->epsilon > 0.0) || !std::isfinite(o->epsilon)) return set_error(BF_INVALID_ARGUMENT, "Adam epsilon must be positive");
-    for (uint32_t k = 0; k < param_count; ++k) {
-        const bf_adam_param& P = params[k];
-        if (P.reserved != 0) return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: reserved word must be zero", k);
-        if (P.count && (!P.value || !P.gradient || !P.first_moment || !P.second_moment))
-            return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: NULL buffer", k);
-        if (!std::isfinite(P.learning_rate) || !(P.lower <= P.upper))
-            return set_error(BF_INVALID_ARGUMENT, "Adam parameter %u: bad learning rate or bounds", k);
-    }
-    const double bias1 = 1.0 - std::pow(o->beta1, (double)o->step);
-    const double bias2 = 1.0 - std::pow(o->beta2, (double)o->step);
-    if (!(bias1 > 0.0) || !(bias2 > 0.0)) return set_error(BF_INVALID_ARGUMENT, "Adam bias correction vanishes");
+    const int st = validate_adam(params, param_count, o);
+    if (st != BF_OK) return st;
     const hipStream_t s = (hipStream_t)stream_handle;
     for (uint32_t k = 0; k < param_count; ++k) {
         const bf_adam_param& P = params[k];
-        const bfk_adam_coefficients c{(float)o->beta1,
-                                      (float)(1.0 - o->beta1),
-                                      (float)o->beta2,
-                                      (float)(1.0 - o->beta2),
-                                      (float)((double)P.learning_rate / bias1),
-                                      (float)std::sqrt(bias2),
-                                      (float)o->epsilon,
-                                      P.lower,
-                                      P.upper};
-        HIP_TRY(bfk_adam(P.value, P.gradient, P.first_moment, P.second_moment, P.count, &c, s));
+        bfk_adam_coefficients c{};
+        if (!adam_coefficients(*o, P, o->step, c)) return set_error(BF_INVALID_ARGUMENT, "Adam bias correction vanishes");
+        HIP_TRY(bfk_adam(P.value, P.gradient, P.first_moment, P.second_moment, P.count, &c, nullptr, nullptr, s));
     }
     return BF_OK;
 }
+
+/* ---- one fit step as one graph ------------------------------------------------------------ */
+
+struct bf_hip_fit_graph {
+    bf_hip_workspace* w = nullptr;
+    int device = 0;
+    hipStream_t capture = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bf_dev_job* staging = nullptr;           /* pinned job tables, one row per launch of either render */
+    uint32_t* words = nullptr;               /* device: forward failure[4] samples[2], adjoint failure[4] samples[2], step */
+    unsigned long long* totals = nullptr;    /* device: forward totals[4], adjoint totals[4] */
+    float2* schedule = nullptr;              /* device: [param][max_steps] (step size, sqrt(bias2)) */
+    uint32_t max_steps = 0, steps = 0, frame_count = 0;
+    uint64_t paths = 0;
+    uint32_t failure_policy = 0;
+};
+
+namespace {
+
+int fit_graph_release(bf_hip_fit_graph* g) {
+    if (!g) return BF_OK;
+    hipError_t first = hipSuccess;
+    const auto keep = [&](hipError_t e) {
+        if (first == hipSuccess) first = e;
+    };
+    if (g->exec) keep(hipGraphExecDestroy(g->exec));
+    if (g->capture) keep(hipStreamDestroy(g->capture));
+    if (g->staging) keep(hipHostFree(g->staging));
+    if (g->words) keep(hipFree(g->words));
+    if (g->totals) keep(hipFree(g->totals));
+    if (g->schedule) keep(hipFree(g->schedule));
+    delete g;
+    return first == hipSuccess ? BF_OK : hip_error(first, "fit graph release");
+}
+
+/* Everything one step issues, on the capturing stream. */
+int record_fit_step(bf_hip_fit_graph* g, const bf_fit_step_desc& d, const bf_transport_options* o, uint32_t rows) {
+    bf_hip_workspace* w = g->w;
+    const hipStream_t s = g->capture;
+    HIP_TRY(bfk_set_words((uint32_t*)g->totals, 16, 0u, s));
+    for (uint32_t k = 0; k < d.frame_count; ++k) {
+        const bf_frame_job& J = d.frames[k];
+        const uint32_t pixels = J.frame.camera.width * J.frame.camera.height;
+        HIP_TRY(bfk_set_words((uint32_t*)J.film, 3u * pixels, 0u, s));
+        HIP_TRY(bfk_set_words((uint32_t*)J.film_compensated, 12u * pixels, 0u, s));
+        HIP_TRY(bfk_set_words(J.film_counts, pixels, 0u, s));
+    }
+    device_loop forward;
+    forward.staging = g->staging;
+    forward.launches = rows;
+    forward.totals = g->totals;
+    int st = trace_jobs(w, d.frames, nullptr, d.frame_count, o, nullptr, nullptr, s, nullptr, nullptr, &forward);
+    if (st != BF_OK) return st;
+    HIP_TRY(bfk_copy_words(w->failure, g->words, 4, s));
+    HIP_TRY(bfk_copy_words(w->sample_counts, g->words + 4, 2, s));
+    for (uint32_t k = 0; k < d.frame_count; ++k) {
+        st = blackframe_hip_film_l2_loss(&d.losses[k], 1, d.loss_scales[k], d.loss_values + k, s);
+        if (st != BF_OK) return st;
+    }
+    for (uint32_t k = 0; k < d.param_count; ++k) {
+        const bf_adam_param& P = d.params[k];
+        for (uint64_t done = 0; done < P.count;) {
+            const uint32_t n = (uint32_t)std::min<uint64_t>(P.count - done, 1u << 30);
+            HIP_TRY(bfk_set_words((uint32_t*)(P.gradient + done), n, 0u, s));
+            done += n;
+        }
+    }
+    device_loop adjoint;
+    adjoint.staging = g->staging + (size_t)rows * BF_MAX_BATCH_JOBS;
+    adjoint.launches = rows;
+    adjoint.totals = g->totals + 4;
+    st = trace_jobs(w, d.frames, d.adjoints, d.frame_count, o, nullptr, nullptr, s, nullptr, nullptr, &adjoint);
+    if (st != BF_OK) return st;
+    HIP_TRY(bfk_copy_words(w->failure, g->words + 6, 4, s));
+    HIP_TRY(bfk_copy_words(w->sample_counts, g->words + 10, 2, s));
+    for (uint32_t k = 0; k < d.param_count; ++k) {
+        const bf_adam_param& P = d.params[k];
+        bfk_adam_coefficients c{};
+        if (!adam_coefficients(d.adam, P, d.adam.step, c)) return set_error(BF_INVALID_ARGUMENT, "Adam bias correction vanishes");
+        HIP_TRY(bfk_adam(P.value, P.gradient, P.first_moment, P.second_moment, P.count, &c,
+                         g->schedule + (size_t)k * g->max_steps, g->words + 12, s));
+    }
+    HIP_TRY(bfk_tick(g->words + 12, s));
+    return BF_OK;
+}
+
+void report_from_words(const uint32_t* words, const unsigned long long* totals, uint64_t paths, bf_transport_report& r) {
+    r = bf_transport_report{};
+    r.path_count = paths;
+    r.closest_hit_queries = totals[0];
+    r.shade_lanes = totals[1];
+    r.miss_lanes = totals[0] - totals[1];
+    r.shadow_queries = totals[2];
+    r.bounces = (uint32_t)totals[3];
+    r.failed_paths = words[0];
+    r.first_failure_slot = BF_INVALID_INDEX;
+    if (r.failed_paths != 0) {
+        const uint64_t key = (uint64_t)words[2] | ((uint64_t)words[3] << 32);
+        r.first_failure_job = (uint32_t)(key >> 48);
+        r.first_failure_slot = (uint32_t)((key >> 8) & 0xFFFFFFFFFFull);
+        r.first_failure_code = (uint32_t)(key & 0xFFu);
+    }
+    r.closure_samples = words[4];
+    r.light_samples = words[5];
+}
+
+} /* namespace */
+
+int blackframe_hip_fit_graph_create(bf_hip_workspace* w, const bf_fit_step_desc* d, const bf_transport_options* o,
+                                    bf_hip_fit_graph** out) {
+    if (!out) return set_error(BF_INVALID_ARGUMENT, "NULL output");
+    *out = nullptr;
+    if (!w || !d || !o) return set_error(BF_INVALID_ARGUMENT, "NULL workspace, step descriptor or options");
+    if (d->reserved != 0) return set_error(BF_INVALID_ARGUMENT, "step descriptor: reserved word must be zero");
+    if (d->frame_count == 0 || !d->frames || !d->adjoints || !d->losses || !d->loss_scales || !d->loss_values)
+        return set_error(BF_INVALID_ARGUMENT, "a fit step needs frames, adjoint jobs, loss jobs, scales and loss values");
+    if (d->max_steps == 0) return set_error(BF_INVALID_ARGUMENT, "max_steps must be positive");
+    if (d->param_count) {
+        const int st = validate_adam(d->params, d->param_count, &d->adam);
+        if (st != BF_OK) return st;
+    }
+    for (uint32_t k = 0; k < d->frame_count; ++k) {
+        const bf_film_loss_job& L = d->losses[k];
+        const bf_frame_job& J = d->frames[k];
+        if (!std::isfinite(d->loss_scales[k])) return set_error(BF_INVALID_ARGUMENT, "frame %u: non-finite loss scale", k);
+        if (!J.film || !J.film_compensated || !J.film_counts)
+            return set_error(BF_INVALID_ARGUMENT, "frame %u: film buffers are required", k);
+        if (L.film != J.film || L.film_counts != J.film_counts || L.film_gradient != d->adjoints[k].film_gradient ||
+            L.pixel_count != J.frame.camera.width * J.frame.camera.height || !L.target || L.reserved != 0)
+            return set_error(BF_INVALID_ARGUMENT, "frame %u: the loss job must read the frame's film and write its film gradient", k);
+    }
+    const uint64_t in_flight = w->capacity >> ((o->launch & BF_LAUNCH_INFLIGHT_MASK) >> 22);
+    const size_t rows = pack_launches(d->frames, d->frame_count, in_flight).size();
+    if (rows == 0) return set_error(BF_RESOURCE_EXHAUSTED, "the frames do not fit the workspace's paths in flight");
+    const uint64_t* directions = nullptr;
+    int st = sobol_table_for_current_device(&directions); /* uploaded before the capture */
+    if (st != BF_OK) return st;
+
+    bf_hip_fit_graph* g = new bf_hip_fit_graph;
+    g->w = w;
+    g->device = w->device;
+    g->max_steps = d->max_steps;
+    g->frame_count = d->frame_count;
+    g->failure_policy = o->failure_policy;
+    for (uint32_t k = 0; k < d->frame_count; ++k)
+        g->paths += (uint64_t)d->frames[k].frame.camera.width * d->frames[k].frame.camera.height *
+                    d->frames[k].frame.samples_per_pixel;
+    const auto fail = [&](int status) {
+        (void)fit_graph_release(g);
+        return status;
+    };
+    const auto hip_fail = [&](hipError_t e, const char* what) { return fail(hip_error(e, what)); };
+    hipError_t e = hipHostMalloc((void**)&g->staging, 2 * rows * BF_MAX_BATCH_JOBS * sizeof(bf_dev_job), hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc (job tables)");
+    if ((e = hipMalloc((void**)&g->words, 16 * sizeof(uint32_t))) != hipSuccess) return hip_fail(e, "hipMalloc (step words)");
+    if ((e = hipMalloc((void**)&g->totals, 8 * sizeof(unsigned long long))) != hipSuccess)
+        return hip_fail(e, "hipMalloc (step totals)");
+    std::vector<float2> schedule((size_t)std::max<uint32_t>(d->param_count, 1u) * d->max_steps);
+    for (uint32_t k = 0; k < d->param_count; ++k)
+        for (uint32_t i = 0; i < d->max_steps; ++i) {
+            bfk_adam_coefficients c{};
+            if (!adam_coefficients(d->adam, d->params[k], (uint64_t)d->adam.step + i, c))
+                return fail(set_error(BF_INVALID_ARGUMENT, "Adam bias correction vanishes at step %u",
+                                      d->adam.step + i));
+            schedule[(size_t)k * d->max_steps + i] = make_float2(c.step_size, c.bias2_sqrt);
+        }
+    if ((e = hipMalloc((void**)&g->schedule, schedule.size() * sizeof(float2))) != hipSuccess)
+        return hip_fail(e, "hipMalloc (Adam schedule)");
+    if ((e = hipMemcpy(g->schedule, schedule.data(), schedule.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy (Adam schedule)");
+    const uint32_t zero_words[16] = {};
+    if ((e = hipMemcpy(g->words, zero_words, sizeof zero_words, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy (step words)");
+    if ((e = hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking)) != hipSuccess)
+        return hip_fail(e, "hipStreamCreateWithFlags");
+    if ((e = hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_fail(e, "hipStreamBeginCapture");
+    st = record_fit_step(g, *d, o, (uint32_t)rows);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(g->capture, &graph);
+    if (st != BF_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(st);
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g->exec = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    *out = g;
+    return BF_OK;
+}
+
+int blackframe_hip_fit_graph_launch(bf_hip_fit_graph* g, void* stream_handle) {
+    if (!g) return set_error(BF_INVALID_ARGUMENT, "NULL fit graph");
+    if (g->steps >= g->max_steps)
+        return set_error(BF_RESOURCE_EXHAUSTED, "the fit graph's Adam schedule covers %u steps", g->max_steps);
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    if (device != g->device) return set_error(BF_INCOMPATIBLE, "fit graph lives on device %d, current device is %d", g->device, device);
+    HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream_handle));
+    ++g->steps;
+    return BF_OK;
+}
+
+int blackframe_hip_fit_graph_report(bf_hip_fit_graph* g, bf_transport_report* forward, bf_transport_report* adjoint,
+                                    uint32_t* steps, void* stream_handle) {
+    if (!g) return set_error(BF_INVALID_ARGUMENT, "NULL fit graph");
+    const hipStream_t s = (hipStream_t)stream_handle;
+    uint32_t words[16];
+    unsigned long long totals[8];
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipMemcpy(words, g->words, sizeof words, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(totals, g->totals, sizeof totals, hipMemcpyDeviceToHost));
+    bf_transport_report f{}, a{};
+    report_from_words(words, totals, g->paths, f);
+    report_from_words(words + 6, totals + 4, g->paths, a);
+    if (forward) *forward = f;
+    if (adjoint) *adjoint = a;
+    if (steps) *steps = words[12];
+    if (g->failure_policy == BF_FAILURES_ABORT && (f.failed_paths != 0 || a.failed_paths != 0))
+        return set_error(BF_TRANSPORT_FAILURE, "%llu forward and %llu adjoint paths failed in the last step",
+                         (unsigned long long)f.failed_paths, (unsigned long long)a.failed_paths);
+    return BF_OK;
+}
+
+int blackframe_hip_fit_graph_destroy(bf_hip_fit_graph* g) { return fit_graph_release(g); }
 
 } /* extern "C" */
 

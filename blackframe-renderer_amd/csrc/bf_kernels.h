@@ -153,8 +153,13 @@ hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t 
 typedef struct bfk_adam_coefficients {
     float beta1, one_minus_beta1, beta2, one_minus_beta2, step_size, bias2_sqrt, epsilon, lower, upper;
 } bfk_adam_coefficients;
+/* schedule (optional): per step number, (step size, sqrt(bias2)) read at schedule[*step] on the device */
 hipError_t bfk_adam(float* value, const float* gradient, float* first, float* second, uint64_t count,
-                    const bfk_adam_coefficients* c, hipStream_t s);
+                    const bfk_adam_coefficients* c, const float2* schedule, const uint32_t* step, hipStream_t s);
+hipError_t bfk_copy_words(const uint32_t* src, uint32_t* dst, uint32_t count, hipStream_t s);
+hipError_t bfk_tick(uint32_t* counter, hipStream_t s);
+/* totals[0..2] += closest-hit, shade, shadow lanes of history rows; totals[3] = max(active bounces) */
+hipError_t bfk_history_totals(const uint32_t* history, uint32_t rows, unsigned long long* totals, hipStream_t s);
 /* loss[0] += scale * sum d^2 and gradient = (two_scale * d) / max(count, 1), d = film / max(count, 1) - target */
 hipError_t bfk_film_l2(const float* film, const uint32_t* counts, const float* target, float* gradient, uint32_t pixels,
                        float two_scale, double scale, double* loss, hipStream_t s);

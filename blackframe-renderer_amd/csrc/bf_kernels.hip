@@ -2411,7 +2411,15 @@ __device__ inline void adam_element(float& p, float g, float& m, float& v, const
  * elements is done by the first threads of the grid). */
 template <bool VEC>
 __global__ __launch_bounds__(BF_BLOCK) void k_adam(float* value, const float* gradient, float* first, float* second,
-                                                   uint64_t count, bfk_adam_coefficients c) {
+                                                   uint64_t count, bfk_adam_coefficients c, const float2* schedule,
+                                                   const uint32_t* step) {
+    if (schedule) {
+        /* a replayed step: the bias-corrected step size and sqrt(bias2) of the current step number,
+         * computed on the host in the same binary64 arithmetic as a direct call */
+        const float2 row = schedule[*step];
+        c.step_size = row.x;
+        c.bias2_sqrt = row.y;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t scalar_begin = 0;
@@ -2780,6 +2788,36 @@ __global__ void k_set_words(uint32_t* dst, uint32_t count, uint32_t value) {
 
 __global__ void k_copy_word(const uint32_t* src, uint32_t* dst) {
     if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = src[0];
+}
+
+__global__ void k_copy_words(const uint32_t* src, uint32_t* dst, uint32_t count) {
+    for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[i];
+}
+
+/* advances a replayed step counter (after the kernels that read it) */
+__global__ void k_tick(uint32_t* counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] += 1u;
+}
+
+/* Adds one launch's per-bounce queue sizes (rows of [ray, hit, shadow, continuation], ended by the
+ * first empty ray queue) to totals [closest-hit queries, shade lanes, shadow queries, largest bounce
+ * count]: the report of a launch whose host never reads its history. */
+__global__ void k_history_totals(const uint32_t* history, uint32_t rows, unsigned long long* totals) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long ray = 0, hit = 0, shadow = 0;
+    uint32_t active = 0;
+    for (uint32_t b = 0; b < rows; ++b) {
+        const uint32_t* h = history + 4 * b;
+        if (h[0] == 0u) break;
+        ray += h[0];
+        hit += h[1];
+        shadow += h[2];
+        active = b + 1u;
+    }
+    totals[0] += ray;
+    totals[1] += hit;
+    totals[2] += shadow;
+    if (active > totals[3]) totals[3] = active;
 }
 
 __global__ __launch_bounds__(BF_BLOCK) void k_export_paths(bf_dev_paths paths, bf_dev_job J) {
@@ -3213,16 +3251,18 @@ hipError_t bfk_reduce_replicas(const float* replicas, uint32_t offset, uint32_t 
 }
 
 hipError_t bfk_adam(float* value, const float* gradient, float* first, float* second, uint64_t count,
-                    const bfk_adam_coefficients* c, hipStream_t s) {
+                    const bfk_adam_coefficients* c, const float2* schedule, const uint32_t* step, hipStream_t s) {
     if (count == 0) return hipSuccess;
     const bool vec = ((reinterpret_cast<uintptr_t>(value) | reinterpret_cast<uintptr_t>(gradient) |
                        reinterpret_cast<uintptr_t>(first) | reinterpret_cast<uintptr_t>(second)) & 15u) == 0;
     const uint64_t work = vec ? (count + 3u) / 4u : count;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>((work + BF_BLOCK - 1u) / BF_BLOCK, 16384u);
     if (vec)
-        hipLaunchKernelGGL(k_adam<true>, dim3(blocks), dim3(BF_BLOCK), 0, s, value, gradient, first, second, count, *c);
+        hipLaunchKernelGGL(k_adam<true>, dim3(blocks), dim3(BF_BLOCK), 0, s, value, gradient, first, second, count, *c,
+                           schedule, step);
     else
-        hipLaunchKernelGGL(k_adam<false>, dim3(blocks), dim3(BF_BLOCK), 0, s, value, gradient, first, second, count, *c);
+        hipLaunchKernelGGL(k_adam<false>, dim3(blocks), dim3(BF_BLOCK), 0, s, value, gradient, first, second, count, *c,
+                           schedule, step);
     return hipGetLastError();
 }
 
@@ -3336,6 +3376,21 @@ static_assert(BF_STATIC_COMPACTION_MAX_INPUT % BF_BLOCK == 0u, "the static scrat
 hipError_t bfk_set_words(uint32_t* dst, uint32_t count, uint32_t value, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_set_words, dim3(blocks_for(count, BF_BLOCK)), dim3(BF_BLOCK), 0, s, dst, count, value);
+    return hipGetLastError();
+}
+
+hipError_t bfk_copy_words(const uint32_t* src, uint32_t* dst, uint32_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_words, dim3(1), dim3(64), 0, s, src, dst, count);
+    return hipGetLastError();
+}
+
+hipError_t bfk_tick(uint32_t* counter, hipStream_t s) {
+    hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, s, counter);
+    return hipGetLastError();
+}
+
+hipError_t bfk_history_totals(const uint32_t* history, uint32_t rows, unsigned long long* totals, hipStream_t s) {
+    hipLaunchKernelGGL(k_history_totals, dim3(1), dim3(64), 0, s, history, rows, totals);
     return hipGetLastError();
 }
 

@@ -76,6 +76,12 @@ def library():
     lib.blackframe_hip_launch_wavefront_queue_compaction.argtypes = [abi.QueueSoa, vp, u32, u32, vp, C.c_size_t, vp, vp]
     lib.blackframe_hip_launch_wavefront_queue_compaction_static.argtypes = [abi.QueueSoa, vp, u32, u32, vp, vp]
     lib.blackframe_hip_adam_step.argtypes = [C.POINTER(abi.AdamParam), u32, C.POINTER(abi.AdamOptions), vp]
+    lib.blackframe_hip_fit_graph_create.argtypes = [vp, C.POINTER(abi.FitStepDesc), C.POINTER(abi.TransportOptions),
+                                                    C.POINTER(vp)]
+    lib.blackframe_hip_fit_graph_launch.argtypes = [vp, vp]
+    lib.blackframe_hip_fit_graph_report.argtypes = [vp, C.POINTER(abi.TransportReport), C.POINTER(abi.TransportReport),
+                                                    C.POINTER(u32), vp]
+    lib.blackframe_hip_fit_graph_destroy.argtypes = [vp]
     lib.blackframe_query_extension.restype = C.c_uint32
     lib.blackframe_query_extension.argtypes = [C.POINTER(abi.ExtensionQuery), C.POINTER(abi.ExtensionApi)]
     lib.blackframe_hip_trace_wavefront_transport.argtypes = [vp, vp, C.POINTER(abi.FrameDesc),
@@ -694,6 +700,105 @@ def adam_step(params, step, beta1=0.9, beta2=0.999, epsilon=1e-8, stream=None):
         records[k].learning_rate, records[k].lower, records[k].upper = lr, lower, upper
     options = abi.AdamOptions(beta1, beta2, epsilon, step, 0)
     _check(library().blackframe_hip_adam_step(records, len(params), C.byref(options), _stream_handle(stream)))
+
+
+class FitGraph:
+    """One texture-fit step recorded once as one graph (blackframe_hip_fit_graph_*): clear films,
+    taped forward, per-frame L2 loss, clear gradients, adjoint, projected Adam; launch() replays it
+    with no host work.
+
+    frames: sequence of (XpuScene, FrameDesc, Film, Tape, target, film_gradient, texture_gradients,
+    loss_scale[, Survival]); params as adam_step's (value, gradient, first, second, lr, lower,
+    upper); loss_values: float64 CUDA tensor of len(frames); step: the Adam step number of the
+    first replay. Every tensor must outlive the graph."""
+
+    def __init__(self, workspace, frames, params, loss_values, options, step=1, betas=(0.9, 0.999), epsilon=1e-8,
+                 max_steps=4096):
+        import torch
+
+        n = len(frames)
+        jobs = (abi.FrameJob * n)()
+        adjoints = (abi.AdjointJob * n)()
+        losses = (abi.FilmLossJob * n)()
+        scales = (C.c_float * n)()
+        for k, item in enumerate(frames):
+            scene, frame, film, tape, target, film_gradient, texture_gradients, scale = item[:8]
+            cam = frame.camera
+            pixels = cam.width * cam.height
+            if tape.path_count != pixels * frame.samples_per_pixel:
+                raise XpuError(1, f"frame {k}: tape holds {tape.path_count} paths")
+            for t in (target, film_gradient):
+                if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.numel() != pixels * 3:
+                    raise XpuError(1, f"frame {k}: target and film gradient must be contiguous float32 CUDA tensors "
+                                      f"of {pixels * 3} elements")
+            jobs[k].scene = scene.handle.value
+            jobs[k].frame = frame
+            jobs[k].film = film.float_sum.data_ptr()
+            jobs[k].film_compensated = film.compensated.data_ptr()
+            jobs[k].film_counts = film.counts.data_ptr()
+            jobs[k].tape_radiance = tape.radiance.data_ptr()
+            jobs[k].tape_visibility = tape.visibility.data_ptr()
+            _bind_survival(jobs[k], item[8] if len(item) > 8 else None, tape.path_count, k)
+            adjoints[k].film_gradient = film_gradient.data_ptr()
+            gradients = list(texture_gradients or [])
+            if len(gradients) > len(scene.texture_sizes):
+                raise XpuError(1, f"frame {k}: {len(gradients)} gradient buffers for {len(scene.texture_sizes)} textures")
+            for slot, g in enumerate(gradients):
+                if g is not None:
+                    if g.dtype != torch.float32 or not g.is_contiguous() or g.numel() != scene.texture_sizes[slot]:
+                        raise XpuError(1, f"frame {k}: texture gradient {slot} must be contiguous float32 "
+                                          f"[{scene.texture_sizes[slot]}]")
+                    adjoints[k].texture_gradients[slot] = g.data_ptr()
+            losses[k].film = film.float_sum.data_ptr()
+            losses[k].film_counts = film.counts.data_ptr()
+            losses[k].target = target.data_ptr()
+            losses[k].film_gradient = film_gradient.data_ptr()
+            losses[k].pixel_count = pixels
+            scales[k] = scale
+        records = (abi.AdamParam * max(1, len(params)))()
+        for k, (value, gradient, m, v, lr, lower, upper) in enumerate(params):
+            for t in (value, gradient, m, v):
+                if t.dtype != torch.float32 or not t.is_cuda or not t.is_contiguous() or t.numel() != value.numel():
+                    raise XpuError(1, f"Adam parameter {k}: tensors must be contiguous float32 CUDA tensors of one size")
+            records[k].value, records[k].gradient = value.data_ptr(), gradient.data_ptr()
+            records[k].first_moment, records[k].second_moment = m.data_ptr(), v.data_ptr()
+            records[k].count = value.numel()
+            records[k].learning_rate, records[k].lower, records[k].upper = lr, lower, upper
+        if loss_values.dtype != torch.float64 or not loss_values.is_cuda or loss_values.numel() != n:
+            raise XpuError(1, f"loss values must be a float64 CUDA tensor of {n} elements")
+        desc = abi.FitStepDesc()
+        desc.frames, desc.adjoints, desc.losses, desc.loss_scales = jobs, adjoints, losses, scales
+        desc.loss_values = loss_values.data_ptr()
+        desc.params = records
+        desc.frame_count, desc.param_count = n, len(params)
+        desc.adam = abi.AdamOptions(betas[0], betas[1], epsilon, step, 0)
+        desc.max_steps = max_steps
+        self._lib = library()
+        self.handle = C.c_void_p()
+        torch.cuda.synchronize()
+        _check(self._lib.blackframe_hip_fit_graph_create(workspace.handle, C.byref(desc), C.byref(options),
+                                                         C.byref(self.handle)))
+
+    def launch(self, stream=None):
+        _check(self._lib.blackframe_hip_fit_graph_launch(self.handle, _stream_handle(stream)))
+
+    def report(self, stream=None):
+        """Waits for the stream; returns (forward report, adjoint report, replays so far)."""
+        forward, adjoint, steps = abi.TransportReport(), abi.TransportReport(), C.c_uint32()
+        _check(self._lib.blackframe_hip_fit_graph_report(self.handle, C.byref(forward), C.byref(adjoint), C.byref(steps),
+                                                         _stream_handle(stream)))
+        return forward, adjoint, steps.value
+
+    def close(self):
+        if self.handle:
+            _check(self._lib.blackframe_hip_fit_graph_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def autotune_launch(workspace, items, options, trial_passes=1, stream=None):

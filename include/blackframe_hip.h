@@ -489,6 +489,50 @@ typedef struct bf_adam_options {
 int blackframe_hip_adam_step(const bf_adam_param* params, uint32_t param_count, const bf_adam_options* options,
                              void* stream);
 
+/*
+ * One texture-fit step as one graph (no counterpart in the reference). A step is: clear the films,
+ * the taped forward render of every frame (as blackframe_hip_trace_wavefront_batch), each frame's
+ * L2 loss and film gradient (blackframe_hip_film_l2_loss with its own scale), clear the fitted
+ * gradients, the path-replay adjoint of every frame (blackframe_hip_trace_wavefront_adjoint) and one
+ * projected Adam step of every parameter (blackframe_hip_adam_step). _create records all of it,
+ * once, into one HIP graph; _launch replays it with no host work and no host read: every launch of
+ * a render runs all depth-bound + 1 bounces, an exhausted queue making a bounce's kernels exit at
+ * once, and the Adam step number is a device counter advanced by each replay, whose bias-corrected
+ * coefficients come from a table computed on the host in the arithmetic of a direct
+ * blackframe_hip_adam_step call. A replay's results therefore equal the four direct calls', up to
+ * the order of the adjoint's float atomics.
+ *
+ * The graph holds the descriptor's pointers: every buffer must stay allocated, and the workspace
+ * unused by other calls while a replay runs. adam.step is the step number of the first replay;
+ * at most max_steps replays. Frames, options and launch word are validated as the direct calls do;
+ * stage events, traversal counters and queue records are refused.
+ */
+typedef struct bf_hip_fit_graph bf_hip_fit_graph;
+typedef struct bf_fit_step_desc {
+    const bf_frame_job* frames;       /* [frame_count] taped forward jobs: films, tapes, survival */
+    const bf_adjoint_job* adjoints;   /* [frame_count] film gradient and texture gradients */
+    const bf_film_loss_job* losses;   /* [frame_count] film, counts, target, film gradient */
+    const float* loss_scales;         /* [frame_count] host: each frame's loss scale */
+    double* loss_values;              /* [frame_count] device: each frame's loss, written by every replay */
+    const bf_adam_param* params;      /* [param_count] fitted arrays; gradients are cleared by every replay */
+    uint32_t frame_count;
+    uint32_t param_count;
+    bf_adam_options adam;
+    uint32_t max_steps;
+    uint32_t reserved;                /* must be 0 */
+} bf_fit_step_desc;
+
+int blackframe_hip_fit_graph_create(bf_hip_workspace* workspace, const bf_fit_step_desc* desc,
+                                    const bf_transport_options* options, bf_hip_fit_graph** out);
+/* Replays one step on `stream` (asynchronous). BF_RESOURCE_EXHAUSTED after max_steps replays. */
+int blackframe_hip_fit_graph_launch(bf_hip_fit_graph* graph, void* stream);
+/* Waits for `stream`, then reports the last replay: its forward and adjoint lane counts, failures
+ * and sample counts (either report may be NULL) and the replays so far. Returns
+ * BF_TRANSPORT_FAILURE when a path failed under BF_FAILURES_ABORT. */
+int blackframe_hip_fit_graph_report(bf_hip_fit_graph* graph, bf_transport_report* forward, bf_transport_report* adjoint,
+                                    uint32_t* steps, void* stream);
+int blackframe_hip_fit_graph_destroy(bf_hip_fit_graph* graph);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
@@ -497,5 +541,6 @@ BF_STATIC_ASSERT(sizeof(bf_frame_job) == 184, "frame job is 184 bytes");
 BF_STATIC_ASSERT(sizeof(bf_film_loss_job) == 40, "film loss job is 40 bytes");
 BF_STATIC_ASSERT(sizeof(bf_adam_param) == 56, "Adam parameter record is 56 bytes");
 BF_STATIC_ASSERT(sizeof(bf_adam_options) == 32, "Adam options are 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_fit_step_desc) == 96, "fit step descriptor is 96 bytes");
 
 #endif /* BLACKFRAME_HIP_H */
