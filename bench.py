@@ -687,6 +687,16 @@ def main():
             "algorithmic_bytes_per_launch": round(closest_bytes / closest_launches),
             "avg_launch_ms": round(closest_avg_s * 1e3, 4),
             "frame_algorithmic_gbs": round(frame_bytes * args.steps / elapsed / 1e9, 2),
+            # the other two large kernels, priced the same way over their stage time per frame:
+            # any-hit = ray + result records, node / triangle / instance fetches and the 208-byte
+            # pending-shadow record per lane; shade = 680 bytes of lane state per shade lane
+            "others": {name: {"algorithmic_bytes_per_frame": int(b),
+                              "ms_per_frame": round(stage_ns[k] / args.steps * 1e-6, 3),
+                              "achieved": round(b / (stage_ns[k] / args.steps * 1e-9) / 1e9, 2),
+                              "frac": round(b / (stage_ns[k] / args.steps * 1e-9) / 1e9 / HBM_PEAK_GBS, 5)}
+                       for name, k, b in (("k_trace_shadow", 3, shadow_bytes + 208 * report.shadow_queries),
+                                          ("k_shade", 2, 680 * report.shade_lanes))
+                       if stage_ns[k] > 0},
         },
         "stages": stages,
         "cpu_baseline": None,

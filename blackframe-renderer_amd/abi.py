@@ -179,7 +179,29 @@ class SceneDesc(C.Structure):
         ("spectra", C.c_void_p), ("spectrum_values", C.c_void_p),
         ("environment_spectrum", C.c_uint32), ("texture_count", C.c_uint32), ("textures", C.c_void_p),
         ("punctual_count", C.c_uint32), ("reserved_punctual", C.c_uint32), ("punctuals", C.c_void_p),
+        ("image_texture_count", C.c_uint32), ("reserved_maps", C.c_uint32), ("image_textures", C.c_void_p),
+        ("surface_maps", C.c_void_p),
     ]
+
+
+TEXTURE_WRAP_REPEAT, TEXTURE_WRAP_CLAMP, TEXTURE_WRAP_MIRROR, TEXTURE_WRAP_BORDER = 0, 1, 2, 3
+MAX_EWA_ANISOTROPY = 64
+
+
+class ImageTextureDesc(C.Structure):
+    """bf_image_texture_desc: a mip chain of float texels for the surface maps' EWA filter."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("channel_count", C.c_uint32), ("mip_count", C.c_uint32),
+                ("texels", C.c_void_p), ("texel_count", C.c_uint64)]
+
+
+class SurfaceMapsDesc(C.Structure):
+    """bf_surface_maps_desc: a material's normal map and bump map (INVALID_INDEX textures: none)."""
+    _fields_ = [("normal_texture", C.c_uint32), ("normal_channels", C.c_uint32 * 3), ("normal_u_wrap", C.c_uint32),
+                ("normal_v_wrap", C.c_uint32), ("normal_max_anisotropy", C.c_uint32),
+                ("normal_max_texel_visits", C.c_uint32), ("normal_y_convention", C.c_uint32),
+                ("bump_texture", C.c_uint32), ("bump_channel", C.c_uint32), ("bump_scale", C.c_float),
+                ("bump_u_wrap", C.c_uint32), ("bump_v_wrap", C.c_uint32), ("bump_max_anisotropy", C.c_uint32),
+                ("bump_max_texel_visits", C.c_uint32)]
 
 
 PUNCTUAL_POINT, PUNCTUAL_DIRECTIONAL, PUNCTUAL_SPOT = 0, 1, 2
@@ -399,7 +421,7 @@ class PathResult(C.Structure):
 
 for _record, _size in ((SampleStreamIndex, 24), (TransportRay, 48), (ClosestHitResult, 80), (OcclusionResult, 16),
                        (BvhNode, 48), (BvhBlas, 32), (ClosureRecord, 64), (BvhInstanceRef, 32), (ClosureDesc, 80), (MaterialDesc, 112),
-                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 160), (PunctualDesc, 56), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
+                       (InstanceDesc, 128), (PathResult, 64), (SceneDesc, 184), (ImageTextureDesc, 32), (SurfaceMapsDesc, 64), (PunctualDesc, 56), (CameraDesc, 96), (TextureDesc, 32), (AdjointJob, 72), (LaunchTuning, 400),
                        (FilmLossJob, 40), (AdamParam, 56), (AdamOptions, 32), (QueueSoa, 24), (CompactionResult, 48), (LayoutManifest, 320),
                        (TransportOptions, 80), (FrameDesc, 112), (SampleDimensionManifest, 160), (SampleDumpRequest, 40),
                        (SampleDumpResult, 104), (PathStateLane, 128), (RayCone, 8), (LaneControl, 16),

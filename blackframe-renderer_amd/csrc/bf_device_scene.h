@@ -114,6 +114,13 @@ typedef struct bf_dev_texture {
     uint32_t width, height, channels, reserved;
 } bf_dev_texture;
 
+/* An image texture of the surface maps: level 0 extent, channels, levels, and the index of its
+ * first value in the scene's texel array (levels follow each other, halving). */
+typedef struct bf_dev_image {
+    uint32_t width, height, channels, mips;
+    uint64_t texel_base;
+} bf_dev_image;
+
 typedef struct bf_dev_scene {
     /* traversal */
     const bf_wide_node* tlas_nodes;
@@ -157,6 +164,11 @@ typedef struct bf_dev_scene {
     const bf_dev_texture* textures;
     uint32_t texture_count;
     uint32_t has_textures;             /* some closure binds a texture */
+    /* surface maps: images, their texels and per material its maps; maps is null when no material
+     * has one */
+    const bf_dev_image* images;
+    const float* image_texels;
+    const bf_surface_maps_desc* maps;
 } bf_dev_scene;
 
 /* Path-state columns (one entry per path slot). */

@@ -432,6 +432,7 @@ struct bf_hip_scene {
     uint32_t max_blas_depth = 0, tlas_depth = 0;
     uint32_t light_count = 0;
     std::vector<uint32_t> texture_floats; /* width * height * channels per texture slot */
+    bool has_maps = false;                /* some material has a normal or bump map */
     bool sampled = false;                 /* holds point-sampled spectra at sampled_wavelengths */
     float sampled_wavelengths[BF_SPECTRUM_LANES] = {};
     uint64_t source_hash = 0;             /* content hash of the scene blob it was made from, if any */
@@ -880,6 +881,55 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
             textured = true;
         }
     }
+    /* -- surface maps: image mip chains and each material's normal / bump map, checked as the
+     * reference's stages check them per lookup (load_image_texture, load_image_mip,
+     * apply_normal_map, apply_bump_map: CudaWavefrontStages.cu:2876-2990, 3395-3560) -- */
+    if (d->image_texture_count && !d->image_textures)
+        return set_error(BF_INVALID_ARGUMENT, "image textures are declared without an array");
+    if (d->reserved_maps != 0) return set_error(BF_INVALID_ARGUMENT, "reserved word after image_texture_count must be zero");
+    std::vector<bf_dev_image> images(d->image_texture_count);
+    uint64_t image_texels = 0;
+    for (uint32_t t = 0; t < d->image_texture_count; ++t) {
+        const bf_image_texture_desc& T = d->image_textures[t];
+        if (T.width == 0 || T.height == 0 || T.channel_count == 0 || T.mip_count == 0 || T.mip_count > BF_MAX_IMAGE_MIPS ||
+            T.width > (1u << 24) || T.height > (1u << 24) || !T.texels)
+            return set_error(BF_INVALID_ARGUMENT, "image texture %u has an invalid size, mip count or no texels", t);
+        uint64_t w = T.width, h = T.height, total = 0;
+        for (uint32_t k = 0; k < T.mip_count; ++k) {
+            total += w * h * T.channel_count;
+            w = w > 1 ? w / 2 : 1;
+            h = h > 1 ? h / 2 : 1;
+        }
+        if (total != T.texel_count)
+            return set_error(BF_INVALID_ARGUMENT, "image texture %u holds %llu values, its mip chain %llu", t,
+                             (unsigned long long)T.texel_count, (unsigned long long)total);
+        images[t] = bf_dev_image{T.width, T.height, T.channel_count, T.mip_count, image_texels};
+        image_texels += total;
+    }
+    s->has_maps = false;
+    const auto map_ok = [&](uint32_t texture, uint32_t u_wrap, uint32_t v_wrap, uint32_t anisotropy, uint32_t visits) {
+        return texture < d->image_texture_count && u_wrap <= BF_TEXTURE_WRAP_BORDER && v_wrap <= BF_TEXTURE_WRAP_BORDER &&
+               anisotropy >= 1 && anisotropy <= BF_MAX_EWA_ANISOTROPY && visits >= 1;
+    };
+    for (uint32_t m = 0; d->surface_maps && m < d->material_count; ++m) {
+        const bf_surface_maps_desc& M = d->surface_maps[m];
+        if (M.normal_texture != BF_INVALID_INDEX) {
+            const uint32_t* ch = M.normal_channels;
+            if (!map_ok(M.normal_texture, M.normal_u_wrap, M.normal_v_wrap, M.normal_max_anisotropy, M.normal_max_texel_visits) ||
+                M.normal_y_convention > 1u)
+                return set_error(BF_INVALID_ARGUMENT, "material %u: invalid normal map", m);
+            const uint32_t cc = d->image_textures[M.normal_texture].channel_count;
+            if (ch[0] >= cc || ch[1] >= cc || ch[2] >= cc || ch[0] == ch[1] || ch[0] == ch[2] || ch[1] == ch[2])
+                return set_error(BF_INVALID_ARGUMENT, "material %u: normal map channels must be distinct channels of its image", m);
+            s->has_maps = true;
+        }
+        if (M.bump_texture != BF_INVALID_INDEX) {
+            if (!map_ok(M.bump_texture, M.bump_u_wrap, M.bump_v_wrap, M.bump_max_anisotropy, M.bump_max_texel_visits) ||
+                M.bump_channel >= d->image_textures[M.bump_texture].channel_count || !std::isfinite(M.bump_scale))
+                return set_error(BF_INVALID_ARGUMENT, "material %u: invalid bump map", m);
+            s->has_maps = true;
+        }
+    }
     /* -- spectra: grid spectra in range; point-sampled ones share one wavelength packet -- */
     s->sampled = false;
     for (uint32_t i = 0; i < d->spectrum_count; ++i) {
@@ -930,6 +980,16 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     const size_t o_lno = A.add(normals4.data(), normals4.size() * sizeof(float));
     const size_t o_pun = A.add(punctuals.data(), punctuals.size() * sizeof(bf_dev_punctual));
     const size_t o_tex = A.add(nullptr, d->texture_count * sizeof(bf_dev_texture));
+    const size_t o_img = A.add(images.data(), s->has_maps ? images.size() * sizeof(bf_dev_image) : 0);
+    const size_t o_maps = A.add(d->surface_maps, s->has_maps ? d->material_count * sizeof(bf_surface_maps_desc) : 0);
+    std::vector<float> all_texels;
+    if (s->has_maps) {
+        all_texels.reserve(image_texels);
+        for (uint32_t t = 0; t < d->image_texture_count; ++t)
+            all_texels.insert(all_texels.end(), d->image_textures[t].texels,
+                              d->image_textures[t].texels + d->image_textures[t].texel_count);
+    }
+    const size_t o_itx = A.add(all_texels.data(), all_texels.size() * sizeof(float));
     size_t o_texels[BF_MAX_TEXTURES] = {};
     for (uint32_t t = 0; t < d->texture_count; ++t) {
         const bf_texture_desc& T = d->textures[t];
@@ -987,6 +1047,14 @@ int build_scene(const bf_scene_desc* d, bf_hip_scene* s, bool upload) {
     S.textures = (const bf_dev_texture*)(base + o_tex);
     S.texture_count = d->texture_count;
     S.has_textures = textured ? 1u : 0u;
+    S.images = nullptr;
+    S.image_texels = nullptr;
+    S.maps = nullptr;
+    if (s->has_maps) {
+        S.images = (const bf_dev_image*)(base + o_img);
+        S.image_texels = (const float*)(base + o_itx);
+        S.maps = (const bf_surface_maps_desc*)(base + o_maps);
+    }
     return BF_OK;
 }
 
@@ -1694,6 +1762,10 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         if (!J.scene->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
         if (J.scene->device != device)
             return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", J.scene->device, device);
+        if (J.scene->has_maps && !inputs)
+            return set_error(BF_UNAVAILABLE,
+                             "job %u: the scene's normal and bump maps need ray cones: render it through "
+                             "blackframe_hip_trace_wavefront_paths", j);
         if (J.scene->sampled && (o->wavelength_mode != BF_WAVELENGTHS_FRAME_PACKET ||
                                  std::memcmp(o->packet_wavelengths, J.scene->sampled_wavelengths, sizeof J.scene->sampled_wavelengths) != 0))
             return set_error(BF_INVALID_ARGUMENT,
@@ -1876,8 +1948,13 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         const uint32_t n = slot;
         P.job_count = (uint32_t)chunk.size();
         bool textured = false;
-        for (size_t c = 0; c < chunk.size(); ++c) textured = textured || jobs[chunk[c].job].scene->dev.has_textures != 0u;
-        P.shade_mode = adjoint ? BF_SHADE_ADJOINT : (textured ? BF_SHADE_TEXTURED : BF_SHADE_PLAIN);
+        bool mapped = false;
+        for (size_t c = 0; c < chunk.size(); ++c) {
+            textured = textured || jobs[chunk[c].job].scene->dev.has_textures != 0u;
+            mapped = mapped || jobs[chunk[c].job].scene->has_maps;
+        }
+        P.shade_mode = adjoint ? BF_SHADE_ADJOINT
+                               : (mapped ? BF_SHADE_MAPPED : (textured ? BF_SHADE_TEXTURED : BF_SHADE_PLAIN));
         if (loop) {
             /* the graph's copy node reads this launch's own staging row at every replay */
             bf_dev_job* row = loop->staging + (size_t)loop->used++ * BF_MAX_BATCH_JOBS;

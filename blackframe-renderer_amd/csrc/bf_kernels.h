@@ -66,6 +66,7 @@ typedef struct bf_dev_job {
 #define BF_SHADE_PLAIN 0u
 #define BF_SHADE_TEXTURED 1u
 #define BF_SHADE_ADJOINT 2u
+#define BF_SHADE_MAPPED 3u /* texture lookups and normal / bump maps compiled in */
 
 typedef struct bf_transport_params {
     bf_dev_job job0;            /* the job of a single-frame launch (kernel argument, no gather) */

@@ -15,6 +15,7 @@
 
 #include "../../include/blackframe_texture.h"
 #include "bf_device_adjoint.h"
+#include "bf_device_maps.h"
 #include "bf_kernels.h"
 #include "bf_scene_soa.h"
 #include "bf_shading.h"
@@ -630,7 +631,7 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
 
 /* ---- shade stage: CudaWavefrontStages.cu:4107-4727 -------------------------------------------- */
 struct surface_t {
-    v3f position, error, ng, ns, dpdu;
+    v3f position, error, ng, ns, dpdu, dpdv;
     float uv[2];
     uint32_t material;
 };
@@ -676,12 +677,12 @@ BF_D bool surface_data(const bf_dev_scene& S, uint32_t inst, uint32_t prim, floa
     const float du1 = tc[1][0] - tc[0][0], dv1 = tc[1][1] - tc[0][1];
     const float du2 = tc[2][0] - tc[0][0], dv2 = tc[2][1] - tc[0][1];
     const float det = fmaf(du1, dv2, -dv1 * du2);
-    v3f dpdu = mk(0.0f, 0.0f, 0.0f);
+    v3f dpdu = mk(0.0f, 0.0f, 0.0f), dpdv = mk(0.0f, 0.0f, 0.0f);
     if (!fin3(e1) || !fin3(e2) || !bf_isfinite(det)) return false;
     if (det != 0.0f) {
         const float rcp = 1.0f / det;
         dpdu = mul3(sub3(mul3(e1, dv2), mul3(e2, dv1)), rcp);
-        const v3f dpdv = mul3(sub3(mul3(e2, du1), mul3(e1, du2)), rcp);
+        dpdv = mul3(sub3(mul3(e2, du1), mul3(e1, du2)), rcp);
         if (!bf_isfinite(rcp) || !fin3(dpdu) || !fin3(dpdv)) return false;
     }
     /* texture coordinates at the hit, for textured closures */
@@ -695,6 +696,7 @@ BF_D bool surface_data(const bf_dev_scene& S, uint32_t inst, uint32_t prim, floa
     out.ng = ng;
     out.ns = ns;
     out.dpdu = dpdu;
+    out.dpdv = dpdv;
     out.material = I.material_index;
     return fin3(out.position) && fin3(out.error);
 }
@@ -1102,7 +1104,8 @@ BF_D void texel_gradients(const bf_transport_params& P, const bf_dev_job& J, con
 }
 
 /* route codes written per queue item: 0 terminated, 1 shadow, 2 continuation. MODE is
- * BF_SHADE_PLAIN (no texture lookups compiled in), BF_SHADE_TEXTURED, or BF_SHADE_ADJOINT: the
+ * BF_SHADE_PLAIN (no texture lookups compiled in), BF_SHADE_TEXTURED, BF_SHADE_MAPPED (textures and
+ * normal / bump maps), or BF_SHADE_ADJOINT: the
  * adjoint replay resolves the light sample from the tape, never routes to the shadow queue and
  * accumulates texel gradients. */
 template <bool BATCH, uint32_t MODE>
@@ -1136,6 +1139,16 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
 
     surface_t surf;
     if (!surface_data(S, ids.x, ids.y, hit, ro, rd, surf)) return fail_path(P, J, slot, 10);
+    if (MODE == BF_SHADE_MAPPED && S.maps &&
+        (S.maps[surf.material].normal_texture != BF_INVALID_INDEX || S.maps[surf.material].bump_texture != BF_INVALID_INDEX)) {
+        /* the shade stage's surface maps (:4170-4193): the cone advanced to the hit sets the
+         * footprint; 46 the cone, 47-49 the maps (invalid, numerical, texel budget) */
+        float2 at_hit;
+        if (!J.in_cones || !advance_cone(P.paths.cone[slot], rd, hit.x, at_hit)) return fail_path(P, J, slot, 46);
+        const int st = apply_surface_maps(S, surf.material, at_hit, rd, surf.uv[0], surf.uv[1], surf.ng, surf.dpdu,
+                                          surf.dpdv, surf.ns);
+        if (st != BF_MAP_OK) return fail_path(P, J, slot, 46u + (uint32_t)st);
+    }
     v3f wo;
     if (!normalize3(mk(-rd.x, -rd.y, -rd.z), wo)) return fail_path(P, J, slot, 11);
     bool front = false;
@@ -3109,10 +3122,11 @@ uint32_t bfk_trace_resident_lanes(int device) {
 }
 
 uint32_t bfk_shade_resident_blocks(int device) {
-    const void* kernels[6] = {(const void*)k_shade<false, BF_SHADE_PLAIN>,    (const void*)k_shade<true, BF_SHADE_PLAIN>,
+    const void* kernels[8] = {(const void*)k_shade<false, BF_SHADE_PLAIN>,    (const void*)k_shade<true, BF_SHADE_PLAIN>,
                               (const void*)k_shade<false, BF_SHADE_TEXTURED>, (const void*)k_shade<true, BF_SHADE_TEXTURED>,
-                              (const void*)k_shade<false, BF_SHADE_ADJOINT>,  (const void*)k_shade<true, BF_SHADE_ADJOINT>};
-    return (uint32_t)max_resident(kernels, 6, BF_BLOCK, device);
+                              (const void*)k_shade<false, BF_SHADE_ADJOINT>,  (const void*)k_shade<true, BF_SHADE_ADJOINT>,
+                              (const void*)k_shade<false, BF_SHADE_MAPPED>,   (const void*)k_shade<true, BF_SHADE_MAPPED>};
+    return (uint32_t)max_resident(kernels, 8, BF_BLOCK, device);
 }
 
 } /* extern "C" */
@@ -3126,6 +3140,9 @@ static void launch_shade(const bf_transport_params* P, dim3 grid, const uint32_t
                            sample_counts);
     else if (mode == BF_SHADE_TEXTURED)
         hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_TEXTURED>), grid, block, 0, s, *P, queue, count_ptr, order, route,
+                           sample_counts);
+    else if (mode == BF_SHADE_MAPPED)
+        hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_MAPPED>), grid, block, 0, s, *P, queue, count_ptr, order, route,
                            sample_counts);
     else
         hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_PLAIN>), grid, block, 0, s, *P, queue, count_ptr, order, route,

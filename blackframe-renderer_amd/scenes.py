@@ -76,6 +76,8 @@ class Scene:
         self.spectra, self.spectrum_values = [], []
         self.textures = []
         self.punctuals = []
+        self.image_textures = []  # (width, height, channels, mip count, float32 texels of every level)
+        self.surface_maps = {}    # material index -> abi.SurfaceMapsDesc
         self.vertex_count = 0
         self.triangle_count = 0
         self.environment_spectrum = abi.INVALID
@@ -186,6 +188,40 @@ class Scene:
         self.instances.append(rec)
         return len(self.instances) - 1
 
+    def add_image_texture(self, levels):
+        """An image texture for the surface maps from its mip chain: a list of (height, width,
+        channels) arrays, each level max(1, floor(previous / 2)) in each axis. Returns its index."""
+        levels = [np.asarray(l, dtype=np.float32).reshape(np.shape(l)[0], np.shape(l)[1], -1) for l in levels]
+        h, w, c = levels[0].shape
+        for k in range(1, len(levels)):
+            ph, pw, _ = levels[k - 1].shape
+            if levels[k].shape != (max(1, ph // 2), max(1, pw // 2), c):
+                raise ValueError(f"mip level {k} has shape {levels[k].shape}")
+        texels = np.ascontiguousarray(np.concatenate([l.reshape(-1) for l in levels]))
+        self.image_textures.append((w, h, c, len(levels), texels))
+        return len(self.image_textures) - 1
+
+    def set_surface_maps(self, material, normal=None, bump=None):
+        """A material's normal map, dict(texture, channels=(0, 1, 2), u_wrap, v_wrap, max_anisotropy,
+        max_texel_visits, y_convention), and / or bump map, dict(texture, channel, scale, u_wrap,
+        v_wrap, max_anisotropy, max_texel_visits)."""
+        m = abi.SurfaceMapsDesc()
+        m.normal_texture = m.bump_texture = abi.INVALID
+        if normal is not None:
+            m.normal_texture = normal["texture"]
+            for k, ch in enumerate(normal.get("channels", (0, 1, 2))):
+                m.normal_channels[k] = ch
+            m.normal_u_wrap, m.normal_v_wrap = normal.get("u_wrap", 0), normal.get("v_wrap", 0)
+            m.normal_max_anisotropy = normal.get("max_anisotropy", 8)
+            m.normal_max_texel_visits = normal.get("max_texel_visits", 4096)
+            m.normal_y_convention = normal.get("y_convention", 0)
+        if bump is not None:
+            m.bump_texture, m.bump_channel, m.bump_scale = bump["texture"], bump.get("channel", 0), bump["scale"]
+            m.bump_u_wrap, m.bump_v_wrap = bump.get("u_wrap", 0), bump.get("v_wrap", 0)
+            m.bump_max_anisotropy = bump.get("max_anisotropy", 8)
+            m.bump_max_texel_visits = bump.get("max_texel_visits", 4096)
+        self.surface_maps[material] = m
+
     def set_environment(self, radiance):
         self.environment_spectrum = self.spectrum(radiance)
 
@@ -258,8 +294,38 @@ class Scene:
         self._punctual_array = (abi.PunctualDesc * max(1, len(self.punctuals)))(*self.punctuals)
         d.punctual_count = len(self.punctuals)
         d.punctuals = C.addressof(self._punctual_array) if self.punctuals else None
+        self._image_array = (abi.ImageTextureDesc * max(1, len(self.image_textures)))()
+        for i, (w, h, c, mips, texels) in enumerate(self.image_textures):
+            t = self._image_array[i]
+            t.width, t.height, t.channel_count, t.mip_count = w, h, c, mips
+            t.texels, t.texel_count = texels.ctypes.data, texels.size
+        d.image_texture_count = len(self.image_textures)
+        d.image_textures = C.addressof(self._image_array) if self.image_textures else None
+        self._maps_array = None
+        if self.surface_maps:
+            self._maps_array = (abi.SurfaceMapsDesc * len(self.materials))()
+            for i in range(len(self.materials)):
+                m = self.surface_maps.get(i)
+                if m is None:
+                    m = abi.SurfaceMapsDesc()
+                    m.normal_texture = m.bump_texture = abi.INVALID
+                self._maps_array[i] = m
+            d.surface_maps = C.addressof(self._maps_array)
         self.desc = d
         return self
+
+
+def mip_chain(image):
+    """Levels of an (height, width, channels) image down to 1x1: each level is the 2x2 box average
+    of the previous (an odd last row or column folds into its neighbour)."""
+    levels = [np.asarray(image, dtype=np.float64)]
+    while levels[-1].shape[0] > 1 or levels[-1].shape[1] > 1:
+        prev = levels[-1]
+        h, w = max(1, prev.shape[0] // 2), max(1, prev.shape[1] // 2)
+        ys = [slice(2 * y, prev.shape[0] if y == h - 1 else 2 * y + 2) for y in range(h)]
+        xs = [slice(2 * x, prev.shape[1] if x == w - 1 else 2 * x + 2) for x in range(w)]
+        levels.append(np.array([[prev[yy, xx].mean(axis=(0, 1)) for xx in xs] for yy in ys]))
+    return [l.astype(np.float32) for l in levels]
 
 
 def project_probabilities(weights):

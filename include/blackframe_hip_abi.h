@@ -574,6 +574,42 @@ typedef struct bf_texture_desc {
     const float* data;   /* host texels, copied into the scene at creation ... */
     float* device_data;  /* ... or caller-owned device texels, read in place by every render */
 } bf_texture_desc;
+/* An image texture with a mip chain, read by the EWA filter of the surface maps (the scene blob's
+ * image_texture / image_mip / image_texel columns, SceneSoaAbi.hpp; sampled as
+ * CudaWavefrontStages.cu:2876-3330). Level k is max(1, floor(level k-1 / 2)) texels in each axis;
+ * texels holds every level in order, each as rows of pixels of channel_count values, row 0 at v = 0. */
+#define BF_TEXTURE_WRAP_REPEAT 0u
+#define BF_TEXTURE_WRAP_CLAMP 1u
+#define BF_TEXTURE_WRAP_MIRROR 2u
+#define BF_TEXTURE_WRAP_BORDER 3u /* reads outside the image are 0 */
+#define BF_MAX_EWA_ANISOTROPY 64u
+#define BF_MAX_IMAGE_MIPS 32u
+typedef struct bf_image_texture_desc {
+    uint32_t width, height;
+    uint32_t channel_count;
+    uint32_t mip_count;
+    const float* texels;
+    uint64_t texel_count; /* sum over the levels of width * height * channel_count */
+} bf_image_texture_desc;
+
+/* A material's normal and bump maps (the material_normal_map_* / material_bump_map_* columns of the
+ * scene blob; applied as CudaWavefrontStages.cu:3395-3673). A normal map's three channels encode a
+ * tangent-space normal in [0, 1] (2 c - 1); y_convention 1 flips its y. A bump map's channel is a
+ * height, scaled by bump_scale. Both are EWA-filtered over the hit's ray-cone footprint with at most
+ * max_anisotropy (1..64) and max_texel_visits texel reads. */
+typedef struct bf_surface_maps_desc {
+    uint32_t normal_texture; /* image texture index, BF_INVALID_INDEX: none */
+    uint32_t normal_channels[3];
+    uint32_t normal_u_wrap, normal_v_wrap;
+    uint32_t normal_max_anisotropy, normal_max_texel_visits;
+    uint32_t normal_y_convention;
+    uint32_t bump_texture; /* image texture index, BF_INVALID_INDEX: none */
+    uint32_t bump_channel;
+    float bump_scale;
+    uint32_t bump_u_wrap, bump_v_wrap;
+    uint32_t bump_max_anisotropy, bump_max_texel_visits;
+} bf_surface_maps_desc;
+
 /* A point-sampled spectrum: count == (BF_SPECTRUM_SAMPLED | BF_SPECTRUM_LANES) and its eight
  * values are four wavelengths (nm) then the value at each. It is defined only at those wavelengths,
  * so a scene holding one renders in frame-packet mode with packet_wavelengths equal to them (every
@@ -677,6 +713,12 @@ typedef struct bf_scene_desc {
     uint32_t punctual_count;
     uint32_t reserved_punctual;
     const bf_punctual_desc* punctuals;
+    /* surface maps: image textures and, per material, its maps (NULL: no material has one). A
+     * scene with maps renders through the path-input entry only, whose rays carry ray cones. */
+    uint32_t image_texture_count;
+    uint32_t reserved_maps;
+    const bf_image_texture_desc* image_textures;
+    const bf_surface_maps_desc* surface_maps; /* [material_count] or NULL */
 } bf_scene_desc;
 
 /* ---- camera / render description ------------------------------------------------------------ */
@@ -868,6 +910,9 @@ BF_STATIC_ASSERT(sizeof(bf_wide_node) == 64, "wide node is one 64-byte line");
 BF_STATIC_ASSERT(sizeof(bf_closure_record) == 64, "ClosureRecord is 64 bytes");
 BF_STATIC_ASSERT(sizeof(bf_closure_desc) == 80, "closure descriptor is 80 bytes");
 BF_STATIC_ASSERT(sizeof(bf_texture_desc) == 32, "texture descriptor is 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_image_texture_desc) == 32, "image texture descriptor is 32 bytes");
+BF_STATIC_ASSERT(sizeof(bf_surface_maps_desc) == 64, "surface maps descriptor is 64 bytes");
+BF_STATIC_ASSERT(sizeof(bf_scene_desc) == 184, "scene descriptor is 184 bytes");
 BF_STATIC_ASSERT(sizeof(bf_transport_options) == 80, "transport options are 80 bytes");
 BF_STATIC_ASSERT(sizeof(bf_stage_metric) == 24, "stage metric is 24 bytes");
 BF_STATIC_ASSERT(sizeof(bf_transport_report) == 1808, "transport report is 1808 bytes");

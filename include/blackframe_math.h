@@ -16,6 +16,10 @@
  *   atan2   : reduction to a ratio t in [0, 1] by octant, the half-angle identity
  *             atan(t) = 2 atan(t / (1 + sqrt(1 + t^2))) applied twice (|t| <= tan(pi/16)), then
  *             the Taylor series of atan through t^31 (truncation error < 1e-24).
+ *   exp/expm1: Cody–Waite reduction by ln 2 (two-term split, the high term exact in k*hi), then
+ *             the Taylor series of exp(r) - 1 through r^17 on |r| <= ln2/2, scaled by 2^k.
+ *   log     : m 2^e with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh((m - 1) / (m + 1)) by its
+ *             series through s^25, plus e ln 2 with the same split constant.
  * Exactly rounded primitives (nextafter, frexp, ldexp) are implemented on the bit patterns.
  */
 #ifndef BLACKFRAME_MATH_H
@@ -243,6 +247,87 @@ BF_HD float bf_atan2f(float y, float x) {
     if (bf_signbit(x)) angle = pi - angle;
     if (bf_signbit(y)) angle = -angle;
     return (float)angle;
+}
+
+/* exp(x) - 1 in binary64 parts: x = k ln2 + r with |r| <= ln2 / 2, *em1 = exp(r) - 1 by its
+ * Taylor series through r^17 / 17! (truncation below 2^-70 relative), *two_k = 2^k. */
+BF_HD void bf_exp_parts(double x, double* em1, double* two_k) {
+    const double ln2_hi = 6.93147180369123816490e-01; /* 32 significant bits: k * ln2_hi is exact */
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double k = rint(x * 1.44269504088896338700);
+    const double r = fma(-k, ln2_lo, fma(-k, ln2_hi, x));
+    double p = 1.0 / 355687428096000.0;     /* 1/17! */
+    p = fma(p, r, 1.0 / 20922789888000.0);  /* 1/16! */
+    p = fma(p, r, 1.0 / 1307674368000.0);   /* 1/15! */
+    p = fma(p, r, 1.0 / 87178291200.0);     /* 1/14! */
+    p = fma(p, r, 1.0 / 6227020800.0);      /* 1/13! */
+    p = fma(p, r, 1.0 / 479001600.0);       /* 1/12! */
+    p = fma(p, r, 1.0 / 39916800.0);        /* 1/11! */
+    p = fma(p, r, 1.0 / 3628800.0);         /* 1/10! */
+    p = fma(p, r, 1.0 / 362880.0);          /* 1/9! */
+    p = fma(p, r, 1.0 / 40320.0);           /* 1/8! */
+    p = fma(p, r, 1.0 / 5040.0);            /* 1/7! */
+    p = fma(p, r, 1.0 / 720.0);             /* 1/6! */
+    p = fma(p, r, 1.0 / 120.0);             /* 1/5! */
+    p = fma(p, r, 1.0 / 24.0);              /* 1/4! */
+    p = fma(p, r, 1.0 / 6.0);               /* 1/3! */
+    p = fma(p, r, 0.5);                     /* 1/2! */
+    *em1 = fma(p * r, r, r);
+    *two_k = bf_bits_double((uint64_t)(1023 + (long long)k) << 52);
+}
+
+/* exp(x) - 1 of a binary32 argument, rounded once from binary64. */
+BF_HD float bf_expm1f(float x) {
+    if (bf_isnan(x)) return x;
+    if (x > 89.0f) return bf_bits_float(0x7F800000u);
+    if (x < -20.0f) return -1.0f;
+    double em1, two_k;
+    bf_exp_parts((double)x, &em1, &two_k);
+    return (float)fma(two_k, em1, two_k - 1.0);
+}
+
+/* exp(x) of a binary32 argument, rounded once from binary64. */
+BF_HD float bf_expf(float x) {
+    if (bf_isnan(x)) return x;
+    if (x > 89.0f) return bf_bits_float(0x7F800000u);
+    if (x < -104.0f) return 0.0f;
+    double em1, two_k;
+    bf_exp_parts((double)x, &em1, &two_k);
+    return (float)fma(two_k, em1, two_k);
+}
+
+/* log(x) of a binary32 argument, rounded once from binary64: x = m 2^e with m in
+ * [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with s = (m - 1) / (m + 1), |s| <= 0.1716, summed
+ * through s^25 / 25 (truncation below 2^-64 relative). */
+BF_HD float bf_logf(float x) {
+    if (bf_isnan(x) || x < 0.0f) return bf_bits_float(0x7FC00000u);
+    if (x == 0.0f) return bf_bits_float(0xFF800000u);
+    if (!bf_isfinite(x)) return x;
+    const uint64_t bits = bf_double_bits((double)x);
+    int e = (int)((bits >> 52) & 0x7FFu) - 1023;
+    double m = bf_bits_double((bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > 1.41421356237309504880) {
+        m *= 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double u = s * s;
+    double p = 1.0 / 25.0;
+    p = fma(p, u, 1.0 / 23.0);
+    p = fma(p, u, 1.0 / 21.0);
+    p = fma(p, u, 1.0 / 19.0);
+    p = fma(p, u, 1.0 / 17.0);
+    p = fma(p, u, 1.0 / 15.0);
+    p = fma(p, u, 1.0 / 13.0);
+    p = fma(p, u, 1.0 / 11.0);
+    p = fma(p, u, 1.0 / 9.0);
+    p = fma(p, u, 1.0 / 7.0);
+    p = fma(p, u, 1.0 / 5.0);
+    p = fma(p, u, 1.0 / 3.0);
+    const double log_m = 2.0 * fma(p * u, s, s);
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    return (float)fma((double)e, ln2_hi, fma((double)e, ln2_lo, log_m));
 }
 
 #endif /* BLACKFRAME_MATH_H */

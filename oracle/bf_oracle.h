@@ -155,6 +155,20 @@ float bfo_hypotf(float x, float y);
 float bfo_atan2f(float y, float x);
 void bfo_sincosf(float angle, float* sine, float* cosine);
 
+/* Surface maps (CudaWavefrontStages.cu:2992-3673): the EWA filter of one channel of an image over
+ * (du/dx, dv/dx, du/dy, dv/dy), the ray cone's (u, v) differentials, and a material's maps applied
+ * to a shading normal (ns in and out). Statuses: 0 ok, 1 invalid, 2 numerical failure, 3 over the
+ * texel budget. */
+int bfo_map_filter(const bf_image_texture_desc* image, float u, float v, const float differentials[4], uint32_t channel,
+                   uint32_t u_wrap, uint32_t v_wrap, uint32_t max_anisotropy, uint32_t max_visits, float* out);
+int bfo_map_cone_differentials(const float cone[2], const float direction[3], const float ng[3], const float dpdu[3],
+                               const float dpdv[3], float out[4]);
+int bfo_map_apply(const bfo_scene* scene, uint32_t material, const float cone[2], const float direction[3],
+                  const float uv[2], const float ng[3], const float dpdu[3], const float dpdv[3], float ns[3]);
+float bfo_expf(float x);
+float bfo_expm1f(float x);
+float bfo_logf(float x);
+
 #ifdef __cplusplus
 }
 #endif

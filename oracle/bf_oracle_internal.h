@@ -79,6 +79,11 @@ struct bfo_scene {
     float* punctual_cosines;
     /* power-weighted light selection CDF (punctual_count + light_count + 1 boundaries), or NULL */
     float* light_cdf;
+    /* surface maps: image textures (owned texels) and per material its maps, or NULL when no
+     * material has one */
+    uint32_t image_texture_count;
+    bf_image_texture_desc* image_textures;
+    bf_surface_maps_desc* surface_maps;
 };
 
 /* vector helpers restated from Blackframe/XPU/CUDA/Source/CudaWavefrontStages.cu:298-349 */

@@ -62,6 +62,9 @@ int bfo_ggx_widths_representable(float ax, float ay) { return mf_ggx_widths(ax, 
 float bfo_hypotf(float x, float y) { return bf_hypotf(x, y); }
 float bfo_atan2f(float y, float x) { return bf_atan2f(y, x); }
 void bfo_sincosf(float a, float* s, float* c) { bf_sincosf(a, s, c); }
+float bfo_expf(float x) { return bf_expf(x); }
+float bfo_expm1f(float x) { return bf_expm1f(x); }
+float bfo_logf(float x) { return bf_logf(x); }
 
 int bfo_mis_weight(uint32_t heuristic, float sampled, float competing, float* weight) {
     return lobe_mis_weight(heuristic, sampled, competing, weight);

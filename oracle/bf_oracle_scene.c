@@ -562,6 +562,62 @@ int bfo_scene_build(const bf_scene_desc* desc, bfo_scene** out) {
                 status = BF_INVALID_ARGUMENT;
         }
     }
+    /* surface maps: mip chains whose sizes add up, and maps that name existing channels */
+    s->desc.image_textures = NULL;
+    s->desc.surface_maps = NULL;
+    if (desc->image_texture_count && !desc->image_textures) status = BF_INVALID_ARGUMENT;
+    if (status == BF_OK && desc->image_texture_count) {
+        s->image_texture_count = desc->image_texture_count;
+        s->image_textures = (bf_image_texture_desc*)calloc(desc->image_texture_count, sizeof(bf_image_texture_desc));
+        for (uint32_t t = 0; t < desc->image_texture_count && status == BF_OK; ++t) {
+            const bf_image_texture_desc* T = &desc->image_textures[t];
+            uint64_t w = T->width, h = T->height, total = 0;
+            for (uint32_t k = 0; k < T->mip_count; ++k) {
+                total += w * h * T->channel_count;
+                w = w > 1 ? w / 2 : 1;
+                h = h > 1 ? h / 2 : 1;
+            }
+            if (!T->width || !T->height || !T->channel_count || !T->mip_count || T->mip_count > BF_MAX_IMAGE_MIPS ||
+                !T->texels || total != T->texel_count) {
+                status = BF_INVALID_ARGUMENT;
+                break;
+            }
+            s->image_textures[t] = *T;
+            float* copy = NULL;
+            DUP(copy, T->texels, total, float);
+            s->image_textures[t].texels = copy;
+        }
+    }
+    int mapped = 0;
+    for (uint32_t m = 0; desc->surface_maps && m < desc->material_count && status == BF_OK; ++m) {
+        const bf_surface_maps_desc* M = &desc->surface_maps[m];
+        if (M->normal_texture != BF_INVALID_INDEX) {
+            if (M->normal_texture >= desc->image_texture_count) {
+                status = BF_INVALID_ARGUMENT;
+                break;
+            }
+            const uint32_t cc = desc->image_textures[M->normal_texture].channel_count;
+            const uint32_t* ch = M->normal_channels;
+            if (ch[0] >= cc || ch[1] >= cc || ch[2] >= cc || ch[0] == ch[1] || ch[0] == ch[2] || ch[1] == ch[2] ||
+                M->normal_u_wrap > 3 || M->normal_v_wrap > 3 || !M->normal_max_anisotropy ||
+                M->normal_max_anisotropy > BF_MAX_EWA_ANISOTROPY || !M->normal_max_texel_visits || M->normal_y_convention > 1)
+                status = BF_INVALID_ARGUMENT;
+            mapped = 1;
+        }
+        if (M->bump_texture != BF_INVALID_INDEX) {
+            if (M->bump_texture >= desc->image_texture_count ||
+                M->bump_channel >= desc->image_textures[M->bump_texture].channel_count || M->bump_u_wrap > 3 ||
+                M->bump_v_wrap > 3 || !M->bump_max_anisotropy || M->bump_max_anisotropy > BF_MAX_EWA_ANISOTROPY ||
+                !M->bump_max_texel_visits || !bf_isfinite(M->bump_scale))
+                status = BF_INVALID_ARGUMENT;
+            mapped = 1;
+        }
+    }
+    if (status == BF_OK && mapped) {
+        DUP(s->surface_maps, desc->surface_maps, desc->material_count, bf_surface_maps_desc);
+        s->desc.surface_maps = s->surface_maps;
+        s->desc.image_textures = s->image_textures;
+    }
     /* punctual lights: the exact_special_cosine of CudaWavefrontStages.cu:515-526 for spot cones */
     s->punctual_count = desc->punctual_count;
     s->desc.punctuals = NULL;
@@ -603,6 +659,9 @@ void bfo_scene_free(bfo_scene* s) {
     free(s->punctual_cosines);
     free(s->light_cdf);
     for (uint32_t t = 0; t < BF_MAX_TEXTURES; ++t) free(s->texels[t]);
+    for (uint32_t t = 0; s->image_textures && t < s->image_texture_count; ++t) free((void*)s->image_textures[t].texels);
+    free(s->image_textures);
+    free(s->surface_maps);
     free(s->blases);
     free(s->blas_nodes);
     free(s->primitive_refs);

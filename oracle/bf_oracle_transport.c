@@ -438,6 +438,8 @@ static int normalize3(v3 v, v3* out) {
     return v3_finite(*out);
 }
 
+#include "bf_oracle_maps.h"
+
 typedef struct {
     v3 scaled, direction;
     float scale, scaled_distance_cubed, length;
@@ -662,7 +664,7 @@ static int geometric_event_support(v3 ng, v3 wo, v3 wi, uint32_t lobes, int* sup
 
 /* ---- stages --------------------------------------------------------------------------------- */
 typedef struct {
-    v3 position, error, ng, ns, dpdu;
+    v3 position, error, ng, ns, dpdu, dpdv;
     float uv[2];
     uint32_t material;
 } surface_t;
@@ -709,14 +711,15 @@ static int surface_data(const bfo_scene* s, const lane* l, surface_t* out) {
     const float du1 = tc[1][0] - tc[0][0], dv1 = tc[1][1] - tc[0][1];
     const float du2 = tc[2][0] - tc[0][0], dv2 = tc[2][1] - tc[0][1];
     const float det = fmaf(du1, dv2, -dv1 * du2);
-    v3 dpdu = v3_make(0.0f, 0.0f, 0.0f);
+    v3 dpdu = v3_make(0.0f, 0.0f, 0.0f), dpdv = v3_make(0.0f, 0.0f, 0.0f);
     if (!v3_finite(e1) || !v3_finite(e2) || !bf_isfinite(det)) return 0;
     if (det != 0.0f) {
         const float rcp = 1.0f / det;
         dpdu = v3_mul(v3_sub(v3_mul(e1, dv2), v3_mul(e2, dv1)), rcp);
-        const v3 dpdv = v3_mul(v3_sub(v3_mul(e2, du1), v3_mul(e1, du2)), rcp);
+        dpdv = v3_mul(v3_sub(v3_mul(e2, du1), v3_mul(e1, du2)), rcp);
         if (!bf_isfinite(rcp) || !v3_finite(dpdu) || !v3_finite(dpdv)) return 0;
     }
+    out->dpdv = dpdv;
     positioned_t obj, world;
     const v3 traversal = v3_make(fmaf(l->hit.parameter, l->ray.direction_x, l->ray.origin_x),
                                  fmaf(l->hit.parameter, l->ray.direction_y, l->ray.origin_y),
@@ -1162,6 +1165,17 @@ static void shade_lane(const ctx* c, lane* l) {
     const bf_transport_options* o = c->options;
     surface_t surf;
     if (!surface_data(s, l, &surf)) return fail(l, 10);
+    if (s->surface_maps && (s->surface_maps[surf.material].normal_texture != BF_INVALID_INDEX ||
+                            s->surface_maps[surf.material].bump_texture != BF_INVALID_INDEX)) {
+        /* the shade stage's surface maps (CudaWavefrontStages.cu:4170-4193) at the cone advanced
+         * to the hit: 46 the cone, 47-49 the maps (invalid, numerical, texel budget) */
+        bf_ray_cone at_hit;
+        if (!c->in_cones || !advance_cone(l->cone, &l->ray, l->hit.parameter, &at_hit)) return fail(l, 46);
+        const int st = map_apply(s, surf.material, at_hit,
+                                 v3_make(l->ray.direction_x, l->ray.direction_y, l->ray.direction_z), surf.uv[0],
+                                 surf.uv[1], surf.ng, surf.dpdu, surf.dpdv, &surf.ns);
+        if (st != MAP_OK) return fail(l, 46u + (uint32_t)st);
+    }
     v3 wo = v3_make(-l->ray.direction_x, -l->ray.direction_y, -l->ray.direction_z);
     if (!normalize3(wo, &wo)) return fail(l, 11);
     int front = 0;
@@ -1919,4 +1933,39 @@ int bfo_texture_sample(uint32_t width, uint32_t height, uint32_t channels, const
 
 float bfo_texel_reflectance(const float* value, uint32_t channels, float wavelength) {
     return bf_texel_reflectance(rgb_basis, value, channels, wavelength);
+}
+
+/* ---- surface-map probes for the pin tests -------------------------------------------------- */
+int bfo_map_filter(const bf_image_texture_desc* image, float u, float v, const float differentials[4], uint32_t channel,
+                   uint32_t u_wrap, uint32_t v_wrap, uint32_t max_anisotropy, uint32_t max_visits, float* out) {
+    const map_diff g = {differentials[0], differentials[1], differentials[2], differentials[3]};
+    return map_filter(image, u, v, &g, channel, u_wrap, v_wrap, max_anisotropy, max_visits, out);
+}
+
+int bfo_map_cone_differentials(const float cone[2], const float direction[3], const float ng[3], const float dpdu[3],
+                               const float dpdv[3], float out[4]) {
+    const bf_ray_cone c = {cone[0], cone[1]};
+    map_diff g;
+    if (!map_cone_diff(c, v3_make(direction[0], direction[1], direction[2]), v3_make(ng[0], ng[1], ng[2]),
+                       v3_make(dpdu[0], dpdu[1], dpdu[2]), v3_make(dpdv[0], dpdv[1], dpdv[2]), &g))
+        return 0;
+    out[0] = g.dudx;
+    out[1] = g.dvdx;
+    out[2] = g.dudy;
+    out[3] = g.dvdy;
+    return 1;
+}
+
+int bfo_map_apply(const bfo_scene* s, uint32_t material, const float cone[2], const float direction[3], const float uv[2],
+                  const float ng[3], const float dpdu[3], const float dpdv[3], float ns[3]) {
+    if (!s->surface_maps || material >= s->desc.material_count) return MAP_INVALID;
+    const bf_ray_cone c = {cone[0], cone[1]};
+    v3 n = v3_make(ns[0], ns[1], ns[2]);
+    const int st = map_apply(s, material, c, v3_make(direction[0], direction[1], direction[2]), uv[0], uv[1],
+                             v3_make(ng[0], ng[1], ng[2]), v3_make(dpdu[0], dpdu[1], dpdu[2]),
+                             v3_make(dpdv[0], dpdv[1], dpdv[2]), &n);
+    ns[0] = n.x;
+    ns[1] = n.y;
+    ns[2] = n.z;
+    return st;
 }

@@ -39,6 +39,14 @@ def load(abi):
     lib.bfo_cie_xyz.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.POINTER(C.c_float)]
     lib.bfo_xyz_to_rgb.argtypes = [C.POINTER(C.c_float), C.POINTER(C.c_float)]
     lib.bfo_scene_build.argtypes = [C.POINTER(abi.SceneDesc), C.POINTER(C.c_void_p)]
+    fp = C.POINTER(C.c_float)
+    lib.bfo_map_filter.argtypes = [C.POINTER(abi.ImageTextureDesc), C.c_float, C.c_float, fp, C.c_uint32, C.c_uint32,
+                                   C.c_uint32, C.c_uint32, C.c_uint32, fp]
+    lib.bfo_map_cone_differentials.argtypes = [fp, fp, fp, fp, fp, fp]
+    lib.bfo_map_apply.argtypes = [C.c_void_p, C.c_uint32, fp, fp, fp, fp, fp, fp, fp]
+    for name in ("bfo_expf", "bfo_expm1f", "bfo_logf"):
+        getattr(lib, name).restype = C.c_float
+        getattr(lib, name).argtypes = [C.c_float]
     lib.bfo_scene_free.argtypes = [C.c_void_p]
     lib.bfo_scene_bvh.argtypes = [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]
@@ -217,6 +225,23 @@ class Oracle:
         self.abi = abi
         self.lib = load(abi)
 
+    def map_filter(self, levels, u, v, differentials, channel=0, u_wrap=0, v_wrap=0, anisotropy=8, visits=1 << 20):
+        """EWA filter of one channel of an image given as its mip levels ((h, w, c) arrays):
+        (status, value)."""
+        texels = np.ascontiguousarray(np.concatenate([np.asarray(l, np.float32).reshape(-1) for l in levels]))
+        h, w = np.shape(levels[0])[:2]
+        desc = self.abi.ImageTextureDesc(w, h, np.shape(levels[0])[2], len(levels), texels.ctypes.data, texels.size)
+        out = C.c_float()
+        st = self.lib.bfo_map_filter(C.byref(desc), float(u), float(v), (C.c_float * 4)(*differentials), channel, u_wrap,
+                                     v_wrap, anisotropy, visits, C.byref(out))
+        return st, out.value
+
+    def cone_differentials(self, cone, direction, ng, dpdu, dpdv):
+        f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+        out = (C.c_float * 4)()
+        ok = self.lib.bfo_map_cone_differentials((C.c_float * 2)(*cone), f3(direction), f3(ng), f3(dpdu), f3(dpdv), out)
+        return list(out) if ok else None
+
     def power_sampler(self, powers):
         return PowerLightSampler(self, powers)
 
@@ -264,6 +289,14 @@ class OracleScene:
         if getattr(self, "handle", None):
             self.lib.bfo_scene_free(self.handle)
             self.handle = None
+
+    def apply_maps(self, material, cone, direction, uv, ng, dpdu, dpdv, ns):
+        """A material's normal / bump maps applied to shading normal ns: (status, new ns)."""
+        f3 = lambda v: (C.c_float * 3)(*[float(x) for x in v])
+        n = f3(ns)
+        st = self.lib.bfo_map_apply(self.handle, material, (C.c_float * 2)(*cone), f3(direction),
+                                    (C.c_float * 2)(*uv), f3(ng), f3(dpdu), f3(dpdv), n)
+        return st, np.array(list(n), dtype=np.float32)
 
     def render_scalar(self, frame, options, path_begin=0, path_count=None):
         """The scalar per-path integrator (one path at a time, brute-force scene queries checked against

@@ -1,0 +1,70 @@
+"""Normal and bump maps on the GPU (EWA-filtered image mip chains over each hit's ray-cone
+footprint, CudaWavefrontStages.cu:2876-3673 and the shade stage's use of them at :4170-4193)
+against the oracle, through the path-input transport whose rays carry cones:
+
+  - a Cornell box whose white walls carry a normal map (repeat wrap) and whose right wall carries a
+    bump map (mirror wrap), 24x24x4 paths with cones of spread 1.5e-3, roulette on: every terminal
+    record bit-identical to the oracle's, and the maps change the radiance of most paths;
+  - the same with a y-flipped normal map, clamp / border wraps and an anisotropy of 2;
+  - a texel budget too small for the footprints fails those paths (code 49) as the oracle does;
+  - a mapped scene is refused by the frame entry point (its camera rays carry no cones).
+"""
+import numpy as np
+import pytest
+
+from path_inputs import as_u32, frame_inputs
+from test_gpu_paths import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def mapped_cornell(bf, abi, y_convention=0, wraps=(0, 2), anisotropy=8, visits=4096):
+    scene, cam = bf.scenes.cornell(24, 24)
+    n = 64
+    y, x = np.mgrid[0:n, 0:n]
+    u, v = (x + 0.5) / n, (y + 0.5) / n
+    tilt_x = 0.35 * np.sin(2 * np.pi * 3 * u) * np.cos(2 * np.pi * 2 * v)
+    tilt_y = 0.35 * np.cos(2 * np.pi * 4 * v)
+    normal = np.stack([tilt_x, tilt_y, np.sqrt(1 - tilt_x ** 2 - tilt_y ** 2)], axis=-1)
+    height = 0.5 + 0.5 * np.sin(2 * np.pi * 5 * u) * np.sin(2 * np.pi * 3 * v)
+    t_normal = scene.add_image_texture(bf.scenes.mip_chain(0.5 * (normal + 1.0)))
+    t_height = scene.add_image_texture(bf.scenes.mip_chain(height[..., None]))
+    scene.set_surface_maps(0, normal={"texture": t_normal, "u_wrap": wraps[0], "v_wrap": wraps[0],
+                                      "max_anisotropy": anisotropy, "max_texel_visits": visits,
+                                      "y_convention": y_convention})
+    scene.set_surface_maps(2, bump={"texture": t_height, "scale": 0.05, "u_wrap": wraps[1], "v_wrap": wraps[1],
+                                    "max_anisotropy": anisotropy, "max_texel_visits": visits})
+    return scene.finalize(), cam
+
+
+@pytest.mark.parametrize("y_convention,wraps,anisotropy", [(0, (0, 2), 8), (1, (1, 3), 2)])
+def test_mapped_cornell_matches_the_oracle(bf, abi, oracle, hip, y_convention, wraps, anisotropy):
+    scene, cam = mapped_cornell(bf, abi, y_convention, wraps, anisotropy)
+    frame = bf.scenes.frame_desc(cam, 4, seed=5)
+    options = bf.scenes.transport_options(max_depth=6, rr_first_depth=2)
+    inputs = frame_inputs(abi, oracle, frame, options, cone=(0.0, 1.5e-3))
+    got, report = compare(hip, oracle, scene, inputs, options)
+    assert report.failed_paths <= len(inputs[1]) // 100
+    plain, _ = bf.scenes.cornell(24, 24)
+    ws = hip.XpuWavefrontTransportWorkspace(len(inputs[1]))
+    flat = hip.trace_xpu_wavefront_paths(ws, hip.XpuScene(plain), inputs, options)
+    changed = (as_u32(got["states"])[:, 4:8] != as_u32(flat["states"])[:, 4:8]).any(axis=1)
+    assert changed.mean() > 0.5
+
+
+def test_texel_budget_fails_paths_like_the_oracle(bf, abi, oracle, hip):
+    scene, cam = mapped_cornell(bf, abi, visits=2)
+    frame = bf.scenes.frame_desc(cam, 2, seed=1)
+    options = bf.scenes.transport_options(max_depth=4)
+    options.failure_policy = abi.FAILURES_COUNT
+    inputs = frame_inputs(abi, oracle, frame, options, cone=(0.0, 4e-3))
+    got, report = compare(hip, oracle, scene, inputs, options)
+    assert report.failed_paths > 0 and report.first_failure_code == 49
+
+
+def test_frame_entry_refuses_mapped_scenes(bf, abi, hip):
+    scene, cam = mapped_cornell(bf, abi)
+    ws = hip.XpuWavefrontTransportWorkspace(24 * 24)
+    with pytest.raises(hip.XpuError):
+        hip.trace_xpu_wavefront_transport(ws, hip.XpuScene(scene), bf.scenes.frame_desc(cam, 1, seed=0),
+                                          bf.scenes.transport_options(max_depth=4))
