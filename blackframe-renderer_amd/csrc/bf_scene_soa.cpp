@@ -133,10 +133,6 @@ int index_of(const std::vector<uint32_t>& sorted_ids, uint32_t id, uint32_t& ind
 int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t flags, bf_hip_scene** out) {
     (void)flags;
     const Blob B{bytes, h};
-    for (uint8_t present : B.column<uint8_t>(C_NORMAL_MAP_PRESENT))
-        if (present) return bf_report_error(BF_UNAVAILABLE, "scene blob: normal maps are not rendered by this transport");
-    for (uint8_t present : B.column<uint8_t>(C_BUMP_MAP_PRESENT))
-        if (present) return bf_report_error(BF_UNAVAILABLE, "scene blob: bump maps are not rendered by this transport");
     if (h.geometry_count == 0 || h.instance_count == 0 || h.material_count == 0)
         return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: a renderable scene needs geometries, materials and instances");
     if (h.geometry_count > 0xFFFFFFFFull || h.material_count > 0xFFFFFFFFull || h.instance_count > 0xFFFFFFFFull ||
@@ -388,6 +384,96 @@ int build_from_blob(const uint8_t* bytes, const bf_scene_soa_header& h, uint32_t
     d.textures = nullptr;
     d.punctual_count = (uint32_t)P;
     d.punctuals = punctuals.empty() ? nullptr : punctuals.data();
+
+    /* -- normal and bump maps over the image textures (SceneSoA.cpp image / mip / texel columns;
+     * checked as load_image_texture, load_image_mip and the map loaders check them,
+     * CudaWavefrontStages.cu:2876-2990, 3395-3560, once here instead of per lookup) -- */
+    if (h.image_texture_count > 0xFFFFFFFFull || h.image_mip_count > 0xFFFFFFFFull)
+        return bf_report_error(BF_RESOURCE_EXHAUSTED, "scene blob: image counts exceed 32 bits");
+    const uint32_t IT = (uint32_t)h.image_texture_count;
+    const auto image_ids = B.column<uint32_t>(C_IMAGE_TEXTURE_ID);
+    const auto mip_offsets = B.column<uint64_t>(C_IMAGE_TEXTURE_MIP_OFFSET);
+    const auto mip_counts = B.column<uint64_t>(C_IMAGE_TEXTURE_MIP_COUNT);
+    const auto channel_counts = B.column<uint32_t>(C_IMAGE_TEXTURE_MIP_COUNT + 1);
+    const auto color_spaces = B.column<uint32_t>(C_IMAGE_TEXTURE_MIP_COUNT + 2);
+    const auto mip_widths = B.column<uint32_t>(C_IMAGE_MIP_WIDTH);
+    const auto mip_heights = B.column<uint32_t>(C_IMAGE_MIP_WIDTH + 1);
+    const auto texel_offsets = B.column<uint64_t>(C_IMAGE_MIP_TEXEL_OFFSET);
+    const auto texel_counts = B.column<uint64_t>(C_IMAGE_MIP_TEXEL_COUNT);
+    const auto texels = B.column<float>(C_IMAGE_TEXEL_VALUE);
+    if (!strictly_increasing(image_ids)) return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: image texture ids are not sorted");
+    std::vector<bf_image_texture_desc> images(IT);
+    uint64_t next_mip = 0, next_texel = 0;
+    for (uint32_t t = 0; t < IT; ++t) {
+        if (mip_counts[t] == 0 || mip_counts[t] > BF_MAX_IMAGE_MIPS || mip_offsets[t] != next_mip ||
+            mip_counts[t] > h.image_mip_count - mip_offsets[t] || channel_counts[t] == 0)
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: image texture %u has an invalid mip range", t);
+        const uint64_t first = mip_offsets[t];
+        for (uint64_t k = first; k < first + mip_counts[t]; ++k) {
+            const uint64_t w = mip_widths[k], hh = mip_heights[k];
+            const bool halves = k == first || (w == (mip_widths[k - 1] > 1 ? mip_widths[k - 1] / 2 : 1) &&
+                                               hh == (mip_heights[k - 1] > 1 ? mip_heights[k - 1] / 2 : 1));
+            if (w == 0 || hh == 0 || w > (1u << 24) || hh > (1u << 24) || !halves || texel_offsets[k] != next_texel ||
+                texel_counts[k] != w * hh * channel_counts[t] || texel_counts[k] > h.image_texel_count - next_texel)
+                return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: image texture %u level %llu is malformed", t,
+                                       (unsigned long long)(k - first));
+            next_texel += texel_counts[k];
+        }
+        images[t] = bf_image_texture_desc{mip_widths[first], mip_heights[first], channel_counts[t], (uint32_t)mip_counts[t],
+                                          texels.data() + texel_offsets[first], next_texel - texel_offsets[first]};
+        next_mip += mip_counts[t];
+    }
+    if (next_mip != h.image_mip_count || next_texel != h.image_texel_count)
+        return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: image mips or texels are not all referenced");
+    std::vector<bf_surface_maps_desc> maps(M);
+    bool mapped = false;
+    std::vector<std::vector<uint32_t>> ncol, bcol;
+    for (uint32_t c = 0; c < 9; ++c) ncol.push_back(B.column<uint32_t>(C_NORMAL_MAP_PRESENT + 1 + c));
+    for (uint32_t c = 0; c < 7; ++c) bcol.push_back(B.column<uint32_t>(C_BUMP_MAP_PRESENT + 1 + c));
+    const auto normal_present = B.column<uint8_t>(C_NORMAL_MAP_PRESENT);
+    const auto bump_present = B.column<uint8_t>(C_BUMP_MAP_PRESENT);
+    const auto image_index = [&](uint32_t id, uint32_t& index) {
+        return index_of(image_ids, id, index) && color_spaces[index] == 0u; /* data textures only */
+    };
+    for (uint32_t m = 0; m < M; ++m) {
+        bf_surface_maps_desc& S = maps[m];
+        S.normal_texture = S.bump_texture = BF_INVALID_INDEX;
+        if (normal_present[m] > 1 || bump_present[m] > 1)
+            return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u has an invalid map flag", m);
+        if (!normal_present[m]) {
+            for (uint32_t c = 0; c < 9; ++c)
+                if (ncol[c][m]) return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u: absent normal map is not zero", m);
+        } else {
+            if (!image_index(ncol[0][m], S.normal_texture))
+                return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u: normal map texture %u is not a data image", m,
+                                       ncol[0][m]);
+            for (uint32_t c = 0; c < 3; ++c) S.normal_channels[c] = ncol[1 + c][m];
+            S.normal_u_wrap = ncol[4][m];
+            S.normal_v_wrap = ncol[5][m];
+            S.normal_max_anisotropy = ncol[6][m];
+            S.normal_max_texel_visits = ncol[7][m];
+            S.normal_y_convention = ncol[8][m];
+            mapped = true;
+        }
+        if (!bump_present[m]) {
+            for (uint32_t c = 0; c < 7; ++c)
+                if (bcol[c][m]) return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u: absent bump map is not zero", m);
+        } else {
+            if (!image_index(bcol[0][m], S.bump_texture))
+                return bf_report_error(BF_INVALID_ARGUMENT, "scene blob: material %u: bump map texture %u is not a data image", m,
+                                       bcol[0][m]);
+            S.bump_channel = bcol[1][m];
+            std::memcpy(&S.bump_scale, &bcol[2][m], sizeof(float));
+            S.bump_u_wrap = bcol[3][m];
+            S.bump_v_wrap = bcol[4][m];
+            S.bump_max_anisotropy = bcol[5][m];
+            S.bump_max_texel_visits = bcol[6][m];
+            mapped = true;
+        }
+    }
+    d.image_texture_count = mapped ? IT : 0u;
+    d.image_textures = mapped && IT ? images.data() : nullptr;
+    d.surface_maps = mapped ? maps.data() : nullptr;
     const int st = blackframe_hip_scene_create(&d, out);
     if (st == BF_OK) bf_scene_set_source_hash(*out, h.content_hash);
     return st;

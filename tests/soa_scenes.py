@@ -160,4 +160,23 @@ def scene_from_canonical(bf, canon, constant_spectra=False):
         s.punctuals.append(rec)
     if canon["environment"] is not None:
         s.environment_spectrum = int(spectrum(packet, canon["environment"]["radiance"]))
+    # normal / bump maps over the image textures, which keep their canonical (id) order
+    image_index = {}
+    for t in canon["image_textures"]:
+        levels = [np.asarray(l["pixels"], np.float32).reshape(l["height"], l["width"], l["channels"]) for l in t["levels"]]
+        image_index[t["id"]] = s.add_image_texture(levels)
+    for m in canon["materials"]:
+        sp = m["spectral"]
+        nm, bm = sp.get("normal_map"), sp.get("bump_map")
+        if nm is None and bm is None:
+            continue
+        normal = None if nm is None else {
+            "texture": image_index[nm["texture"]], "channels": (nm["red_channel"], nm["green_channel"], nm["blue_channel"]),
+            "u_wrap": nm["u_wrap"], "v_wrap": nm["v_wrap"], "max_anisotropy": nm["maximum_anisotropy"],
+            "max_texel_visits": nm["maximum_texel_visits"], "y_convention": nm["y_convention"]}
+        bump = None if bm is None else {
+            "texture": image_index[bm["texture"]], "channel": bm["channel"], "scale": float(F32(bm["scale"])),
+            "u_wrap": bm["u_wrap"], "v_wrap": bm["v_wrap"], "max_anisotropy": bm["maximum_anisotropy"],
+            "max_texel_visits": bm["maximum_texel_visits"]}
+        s.set_surface_maps(mindex[m["id"]], normal=normal, bump=bump)
     return s.finalize()

@@ -7,6 +7,7 @@ against the oracle, through the path-input transport whose rays carry cones:
     record bit-identical to the oracle's, and the maps change the radiance of most paths;
   - the same with a y-flipped normal map, clamp / border wraps and an anisotropy of 2;
   - a texel budget too small for the footprints fails those paths (code 49) as the oracle does;
+  - the maps delivered in the reference's scene blob render like the oracle's Scene-builder twin;
   - a mapped scene is refused by the frame entry point (its camera rays carry no cones).
 """
 import numpy as np
@@ -60,6 +61,52 @@ def test_texel_budget_fails_paths_like_the_oracle(bf, abi, oracle, hip):
     inputs = frame_inputs(abi, oracle, frame, options, cone=(0.0, 4e-3))
     got, report = compare(hip, oracle, scene, inputs, options)
     assert report.failed_paths > 0 and report.first_failure_code == 49
+
+
+def test_blob_maps_match_the_oracle(bf, abi, oracle, hip):
+    """The same maps arriving in the reference's scene blob (image texture, mip and texel columns;
+    material normal- and bump-map columns): the blob scene's path-input render on the device equals
+    the oracle's render of its Scene-builder twin, record for record."""
+    from soa_scenes import cornell_description, scene_from_canonical, soa
+    from test_gpu_scene_soa import camera_frame, packet_options
+
+    d, wl = cornell_description(bf)
+    n = 32
+    y, x = np.mgrid[0:n, 0:n]
+    u, v = (x + 0.5) / n, (y + 0.5) / n
+    tilt = 0.3 * np.sin(2 * np.pi * 2 * u)
+    normal = np.stack([tilt, 0.2 * np.cos(2 * np.pi * 3 * v), np.ones_like(u)], axis=-1)
+    normal /= np.linalg.norm(normal, axis=-1, keepdims=True)
+    height = np.cos(2 * np.pi * 4 * u) * np.sin(2 * np.pi * 2 * v)
+
+    def image(tid, pixels):
+        return {"id": tid, "storage_color_space": 0,
+                "levels": [{"width": l.shape[1], "height": l.shape[0], "channels": l.shape[2],
+                            "pixels": [float(e) for e in l.reshape(-1)]} for l in bf.scenes.mip_chain(pixels)]}
+
+    d["image_textures"] = [image(700, 0.5 * (normal + 1.0)), image(701, height[..., None])]
+    by_id = {m["id"]: m for m in d["materials"]}
+    by_id[21]["spectral"]["normal_map"] = {"texture": 700, "red_channel": 0, "green_channel": 1, "blue_channel": 2,
+                                           "u_wrap": 0, "v_wrap": 2, "maximum_anisotropy": 8,
+                                           "maximum_texel_visits": 4096, "y_convention": 1}
+    by_id[23]["spectral"]["bump_map"] = {"texture": 701, "channel": 0, "scale": 0.04, "u_wrap": 1, "v_wrap": 0,
+                                         "maximum_anisotropy": 4, "maximum_texel_visits": 4096}
+    canon = soa.frame_scene(d)
+    blob = soa.serialize(canon)
+    twin = scene_from_canonical(bf, canon)
+    frame = camera_frame(bf, spp=2, seed=3)
+    options = packet_options(bf, wl, max_depth=5)
+    options.record_queues = 0
+    inputs = frame_inputs(abi, oracle, frame, options, cone=(0.0, 1.5e-3))
+    ws = hip.XpuWavefrontTransportWorkspace(len(inputs[1]))
+    got = hip.trace_xpu_wavefront_paths(ws, hip.XpuScene.from_soa(blob), inputs, options)
+    states, rays, cones, controls, report = oracle.scene(twin).render_paths(inputs, options, threads=8)
+    assert got["report"].failed_paths == report.failed_paths
+    for name, want in (("states", states), ("rays", rays), ("cones", cones), ("controls", controls)):
+        assert np.array_equal(as_u32(got[name]), as_u32(want)), name
+    flat = oracle.scene(scene_from_canonical(bf, soa.frame_scene(cornell_description(bf)[0]))).render_paths(
+        inputs, options, threads=8)
+    assert (as_u32(states)[:, 4:8] != as_u32(flat[0])[:, 4:8]).any(axis=1).mean() > 0.3
 
 
 def test_frame_entry_refuses_mapped_scenes(bf, abi, hip):

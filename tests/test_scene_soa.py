@@ -244,7 +244,11 @@ def test_blob_ingest_rejections_before_any_device_work(bf, hip, abi):
     assert ingest_status(hip, patched(blob, soa.MESH_AREA_LIGHT_INSTANCE_ID, 0, 31))[0] == abi.INVALID_ARGUMENT
     assert ingest_status(hip, patched(blob, soa.INSTANCE_MATERIAL_ID, 0, 99))[0] == abi.INVALID_ARGUMENT
     assert ingest_status(hip, patched(blob, soa.MATERIAL_ID, 0, 0xFFFFFFFF))[0] == abi.INVALID_ARGUMENT
-    assert ingest_status(hip, patched(blob, soa.NORMAL_MAP_PRESENT, 0, 1))[0] == abi.UNAVAILABLE
+    # a normal map naming an image the blob does not hold, and an absent bump map with a field set
+    status, message = ingest_status(hip, patched(blob, soa.NORMAL_MAP_PRESENT, 0, 1))
+    assert status == abi.INVALID_ARGUMENT and "not a data image" in message
+    status, message = ingest_status(hip, patched(blob, soa.BUMP_MAP_PRESENT + 2, 0, 3))
+    assert status == abi.INVALID_ARGUMENT and "absent bump map" in message
     assert ingest_status(hip, patched(blob, soa.INSTANCE_LOCAL_TO_WORLD + 12, 0, 1.0))[0] == abi.INVALID_ARGUMENT
     assert h["counts"]["mesh_area_light"] == 1
 
