@@ -48,7 +48,10 @@
 #define BF_SHADOW_WAVES_PER_SIMD 7 /* the any-hit traversal keeps less state than closest hit; 7 waves (72 VGPRs) measured fastest */
 #endif
 #ifndef BF_TRACE_REFILL
-#define BF_TRACE_REFILL 16u /* idle lanes of a traversal wave that trigger a fetch of new rays */
+#define BF_TRACE_REFILL 24u /* idle lanes of a closest-hit wave that trigger a fetch of new rays (16, 20, 28, 32, 40 measured slower) */
+#endif
+#ifndef BF_SHADOW_REFILL
+#define BF_SHADOW_REFILL 16u /* the same for an any-hit wave (8 and 24 measured slower) */
 #endif
 #ifndef BF_SHADE_WAVES_PER_SIMD
 #define BF_SHADE_WAVES_PER_SIMD 2 /* register budget of the shade kernels (waves per SIMD) */

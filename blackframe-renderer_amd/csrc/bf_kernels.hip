@@ -520,7 +520,7 @@ BF_D void closest_finish(const bf_transport_params& P, uint8_t* route, lane_ray&
 
 /*
  * The persistent lane loop shared by both traversal stages. Every lane holds one ray; a lane whose
- * ray is done stays idle until BF_TRACE_REFILL lanes of its wave are (or all are), and then the
+ * ray is done stays idle until REFILL lanes of its wave are (or all are), and then the
  * wave claims that many queue items with one atomic on the launch's fetch counter. A wave
  * therefore keeps working while any ray of the queue is unclaimed instead of waiting for its
  * slowest ray before it moves on. Within a wave, an iteration steps either the lanes whose next
@@ -529,14 +529,14 @@ BF_D void closest_finish(const bf_transport_params& P, uint8_t* route, lane_ray&
  * traversal is the same sequence of steps in any schedule, so results do not depend on it; the
  * loop ends when the counter has passed the item count and every lane of the wave is idle.
  */
-template <typename Start, typename Step, typename Finish>
+template <uint32_t REFILL, typename Start, typename Step, typename Finish>
 BF_D void lane_loop(uint32_t n, uint32_t* fetch, Start start, Step step, Finish finish) {
     const uint32_t lane = threadIdx.x & 63u;
     bool active = false, exhausted = false;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         const uint32_t want = (uint32_t)__popcll(idle);
-        if (!exhausted && (want >= BF_TRACE_REFILL || want == 64u)) {
+        if (!exhausted && (want >= REFILL || want == 64u)) {
             uint32_t base = 0u;
             if (lane == 0u) base = atomicAdd(fetch, want);
             base = __shfl(base, 0, 64);
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
 #else
     double* inv = nullptr;
 #endif
-    lane_loop(
+    lane_loop<BF_TRACE_REFILL>(
         n, P.fetch,
         [&](uint32_t i) {
             const uint32_t st = closest_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr);
@@ -1516,7 +1516,7 @@ __global__ __launch_bounds__(BLOCK, BF_SHADOW_WAVES_PER_SIMD) void k_trace_shado
         BF_D bool pending() const { return stack_depth(*stack) != 0u; }
         BF_D bool leaf_next() const { return ::leaf_next(*stack); }
     };
-    lane_loop(
+    lane_loop<BF_SHADOW_REFILL>(
         n, P.fetch + 1, [&](uint32_t i) { return shadow_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr); },
         stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { shadow_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters + 3);
