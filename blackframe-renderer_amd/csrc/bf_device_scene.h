@@ -41,6 +41,9 @@
  * every iteration, each under its own lane mask). 16 measured best of {1, 4, 8, 12, 16, 32, 64}. */
 #define BF_LEAF_PHASE 16u
 #endif
+#ifndef BF_SHADOW_LEAF_PHASE
+#define BF_SHADOW_LEAF_PHASE 16u /* the same for any-hit waves (8, 24 and 32 measured slower) */
+#endif
 #ifndef BF_WORLD_INV_CACHE
 #define BF_WORLD_INV_CACHE 1 /* closest-hit lanes keep their world-space reciprocals in LDS */
 #endif

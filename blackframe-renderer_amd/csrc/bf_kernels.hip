@@ -524,12 +524,12 @@ BF_D void closest_finish(const bf_transport_params& P, uint8_t* route, lane_ray&
  * wave claims that many queue items with one atomic on the launch's fetch counter. A wave
  * therefore keeps working while any ray of the queue is unclaimed instead of waiting for its
  * slowest ray before it moves on. Within a wave, an iteration steps either the lanes whose next
- * entry is an interior node or those whose next entry is a leaf (BF_LEAF_PHASE), so the wave issues
+ * entry is an interior node or those whose next entry is a leaf (LEAF lanes), so the wave issues
  * one kind of step code at a time instead of every kind under partial masks. Every ray's
  * traversal is the same sequence of steps in any schedule, so results do not depend on it; the
  * loop ends when the counter has passed the item count and every lane of the wave is idle.
  */
-template <uint32_t REFILL, typename Start, typename Step, typename Finish>
+template <uint32_t REFILL, uint32_t LEAF, typename Start, typename Step, typename Finish>
 BF_D void lane_loop(uint32_t n, uint32_t* fetch, Start start, Step step, Finish finish) {
     const uint32_t lane = threadIdx.x & 63u;
     bool active = false, exhausted = false;
@@ -555,12 +555,12 @@ BF_D void lane_loop(uint32_t n, uint32_t* fetch, Start start, Step step, Finish 
             continue;
         }
         bool go = active;
-        if (BF_LEAF_PHASE != 0u) {
-            /* one kind of step per iteration: interior nodes, unless BF_LEAF_PHASE lanes (or all
+        if (LEAF != 0u) {
+            /* one kind of step per iteration: interior nodes, unless LEAF lanes (or all
              * active ones) have a leaf next */
             const bool leaf = active && step.leaf_next();
             const uint64_t leaves = __ballot(leaf);
-            const bool leaf_phase = (uint32_t)__popcll(leaves) >= BF_LEAF_PHASE || leaves == __ballot(active);
+            const bool leaf_phase = (uint32_t)__popcll(leaves) >= LEAF || leaves == __ballot(active);
             go = active && leaf == leaf_phase;
         }
         if (go) {
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(BLOCK, BF_TRACE_WAVES_PER_SIMD) void k_trace_closes
 #else
     double* inv = nullptr;
 #endif
-    lane_loop<BF_TRACE_REFILL>(
+    lane_loop<BF_TRACE_REFILL, BF_LEAF_PHASE>(
         n, P.fetch,
         [&](uint32_t i) {
             const uint32_t st = closest_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr);
@@ -1516,7 +1516,7 @@ __global__ __launch_bounds__(BLOCK, BF_SHADOW_WAVES_PER_SIMD) void k_trace_shado
         BF_D bool pending() const { return stack_depth(*stack) != 0u; }
         BF_D bool leaf_next() const { return ::leaf_next(*stack); }
     };
-    lane_loop<BF_SHADOW_REFILL>(
+    lane_loop<BF_SHADOW_REFILL, BF_SHADOW_LEAF_PHASE>(
         n, P.fetch + 1, [&](uint32_t i) { return shadow_start<COUNT, BATCH>(P, queue, order ? order[i] : i, stack, L, ctr); },
         stepper{&P, &stack, &L, &ctr}, [&](uint32_t st) { shadow_finish<BATCH>(P, route, L, st); });
     if (COUNT) reduce_counters(ctr, counters + 3);
