@@ -57,7 +57,10 @@
 #define BF_SHADOW_REFILL 16u /* the same for an any-hit wave (8 and 24 measured slower) */
 #endif
 #ifndef BF_SHADE_WAVES_PER_SIMD
-#define BF_SHADE_WAVES_PER_SIMD 2 /* register budget of the shade kernels (waves per SIMD) */
+#define BF_SHADE_WAVES_PER_SIMD 3 /* register budget of the forward shade kernels (waves per SIMD; 1 and 2 measured slower) */
+#endif
+#ifndef BF_ADJOINT_SHADE_WAVES_PER_SIMD
+#define BF_ADJOINT_SHADE_WAVES_PER_SIMD 2 /* the adjoint shade kernel keeps more state per lane */
 #endif
 
 /* A punctual light with its spot-cone cosines formed once on the host. */

@@ -1089,7 +1089,7 @@ struct bf_hip_workspace {
     bf_dev_job* host_jobs = nullptr;         /* pinned staging of the job table */
     uint2* spill = nullptr;                  /* traversal stack spill, BF_STACK_DEPTH x spill_lanes */
     uint32_t spill_lanes = 0;
-    uint32_t shade_blocks = 0;
+    uint32_t shade_blocks = 0, adjoint_shade_blocks = 0;
     int device = 0;
     /* a captured group of bounces, replayed while paths remain (keyed by what its arguments hold) */
     hipGraphExec_t graph = nullptr;
@@ -1542,8 +1542,9 @@ int blackframe_hip_workspace_create(uint64_t capacity, bf_hip_workspace** out) {
     const uint32_t tiles = bfk_compact_tiles((uint32_t)n);
     hipError_t de = hipGetDevice(&w->device);
     const uint32_t resident = de == hipSuccess ? bfk_trace_resident_lanes(w->device) : 0u;
-    w->shade_blocks = de == hipSuccess ? bfk_shade_resident_blocks(w->device) : 0u;
-    if (resident == 0 || w->shade_blocks == 0) {
+    w->shade_blocks = de == hipSuccess ? bfk_shade_resident_blocks(w->device, false) : 0u;
+    w->adjoint_shade_blocks = de == hipSuccess ? bfk_shade_resident_blocks(w->device, true) : 0u;
+    if (resident == 0 || w->shade_blocks == 0 || w->adjoint_shade_blocks == 0) {
         delete w;
         return set_error(BF_DEVICE_ERROR, "cannot query the kernels' occupancy on the current device");
     }
@@ -1840,6 +1841,7 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
     }
     P.spill_stride = w->spill_lanes;
     P.shade_blocks = w->shade_blocks;
+    P.adjoint_shade_blocks = w->adjoint_shade_blocks;
     const uint32_t block_log2 = (o->launch & BF_LAUNCH_TRACE_BLOCK_MASK) >> 16;
     P.trace_block = block_log2 == 0 ? 128u : 1u << block_log2;
     P.jobs = w->jobs;

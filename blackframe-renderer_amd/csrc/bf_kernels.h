@@ -85,7 +85,8 @@ typedef struct bf_transport_params {
     uint32_t* failure;          /* [0] failed path count, [2..3] lowest (job << 48 | path << 8 | code) */
     uint2* spill;               /* traversal stack spill: BF_STACK_DEPTH entries per persistent lane */
     uint32_t spill_stride;      /* persistent lanes = resident traversal workgroups * BF_TRACE_BLOCK */
-    uint32_t shade_blocks;      /* resident shade workgroups (persistent grid) */
+    uint32_t shade_blocks;      /* resident forward shade workgroups (persistent grid) */
+    uint32_t adjoint_shade_blocks; /* resident adjoint shade workgroups */
     uint32_t* fetch;            /* traversal work counters [closest, shadow], cleared before each launch */
     uint32_t trace_block;       /* lanes per traversal workgroup: 64, 128 or 256 */
     float film_pdf;             /* wavelength pdf and sensor normalisation of the film (adjoint) */
@@ -126,7 +127,7 @@ hipError_t bfk_bounce_end(uint32_t* sizes, uint32_t* history, uint32_t history_b
 hipError_t bfk_film(const bf_dev_paths* paths, const bf_dev_job* job, float pdf, float normalization, hipStream_t s);
 /* Workgroups the device holds at once for the traversal / shade kernels, 0 on error. */
 uint32_t bfk_trace_resident_lanes(int device);
-uint32_t bfk_shade_resident_blocks(int device);
+uint32_t bfk_shade_resident_blocks(int device, bool adjoint);
 hipError_t bfk_queue_hash(const uint32_t* queue, const uint32_t* count_ptr, unsigned long long* out, hipStream_t s);
 hipError_t bfk_scene_closest_hit(const bf_dev_scene* S, const bf_transport_ray* rays, uint32_t count,
                                  bf_closest_hit_result* results, uint32_t* counters, hipStream_t s);

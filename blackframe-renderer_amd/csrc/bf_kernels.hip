@@ -1416,7 +1416,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
 }
 
 template <bool BATCH, uint32_t MODE>
-__global__ __launch_bounds__(BF_BLOCK, BF_SHADE_WAVES_PER_SIMD) void k_shade(bf_transport_params P, const uint32_t* queue,
+__global__ __launch_bounds__(BF_BLOCK, MODE == BF_SHADE_ADJOINT ? BF_ADJOINT_SHADE_WAVES_PER_SIMD : BF_SHADE_WAVES_PER_SIMD) void k_shade(bf_transport_params P, const uint32_t* queue,
                                                                              const uint32_t* count_ptr, const uint32_t* order,
                                                                              uint8_t* route, uint32_t* sample_counts) {
     const uint32_t n = *count_ptr;
@@ -3121,12 +3121,14 @@ uint32_t bfk_trace_resident_lanes(int device) {
     return std::max(a, std::max(b, c));
 }
 
-uint32_t bfk_shade_resident_blocks(int device) {
-    const void* kernels[8] = {(const void*)k_shade<false, BF_SHADE_PLAIN>,    (const void*)k_shade<true, BF_SHADE_PLAIN>,
+/* Resident workgroups of the forward shade variants, or of the adjoint ones (which hold fewer
+ * waves per SIMD): the persistent shade grid of each, so that no workgroup waits for a slot. */
+uint32_t bfk_shade_resident_blocks(int device, bool adjoint) {
+    const void* forward[6] = {(const void*)k_shade<false, BF_SHADE_PLAIN>,    (const void*)k_shade<true, BF_SHADE_PLAIN>,
                               (const void*)k_shade<false, BF_SHADE_TEXTURED>, (const void*)k_shade<true, BF_SHADE_TEXTURED>,
-                              (const void*)k_shade<false, BF_SHADE_ADJOINT>,  (const void*)k_shade<true, BF_SHADE_ADJOINT>,
                               (const void*)k_shade<false, BF_SHADE_MAPPED>,   (const void*)k_shade<true, BF_SHADE_MAPPED>};
-    return (uint32_t)max_resident(kernels, 8, BF_BLOCK, device);
+    const void* backward[2] = {(const void*)k_shade<false, BF_SHADE_ADJOINT>, (const void*)k_shade<true, BF_SHADE_ADJOINT>};
+    return adjoint ? (uint32_t)max_resident(backward, 2, BF_BLOCK, device) : (uint32_t)max_resident(forward, 6, BF_BLOCK, device);
 }
 
 } /* extern "C" */
@@ -3177,7 +3179,8 @@ hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const 
         order = P->shade_order;
     }
     const uint32_t needed = blocks_for(max_count, BF_BLOCK);
-    const dim3 grid(needed < P->shade_blocks ? needed : P->shade_blocks);
+    const uint32_t resident = mode == BF_SHADE_ADJOINT ? P->adjoint_shade_blocks : P->shade_blocks;
+    const dim3 grid(needed < resident ? needed : resident);
     if (batch) launch_shade<true>(P, grid, queue, count_ptr, order, route, sample_counts, mode, s);
     else launch_shade<false>(P, grid, queue, count_ptr, order, route, sample_counts, mode, s);
     return hipGetLastError();
