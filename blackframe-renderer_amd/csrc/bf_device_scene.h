@@ -60,7 +60,7 @@
 #define BF_SHADE_WAVES_PER_SIMD 3 /* register budget of the forward shade kernels (waves per SIMD; 1 and 2 measured slower) */
 #endif
 #ifndef BF_ADJOINT_SHADE_WAVES_PER_SIMD
-#define BF_ADJOINT_SHADE_WAVES_PER_SIMD 2 /* the adjoint shade kernel keeps more state per lane */
+#define BF_ADJOINT_SHADE_WAVES_PER_SIMD 3 /* the adjoint shade kernel (1 and 2 measured slower) */
 #endif
 
 /* A punctual light with its spot-cone cosines formed once on the host. */
