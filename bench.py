@@ -84,8 +84,8 @@ def algorithmic_bytes(report, tc, path_count):
 
 def host_cpu():
     """The host CPU the baseline ran on: model name, logical CPUs of the machine, and the CPUs this
-    process may use (the GPU box gives one GPU's job a share of 16; the worker count is that share,
-    the reference scheduler's hardware_concurrency() of the job)."""
+    process may use (a one-GPU job is given a share of 16 CPUs; the worker count is that share, the
+    reference scheduler's hardware_concurrency() of the job)."""
     model = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
