@@ -59,6 +59,9 @@
 #ifndef BF_SHADE_WAVES_PER_SIMD
 #define BF_SHADE_WAVES_PER_SIMD 3 /* register budget of the forward shade kernels (waves per SIMD; 1 and 2 measured slower) */
 #endif
+#ifndef BF_SHADE_BLOCK
+#define BF_SHADE_BLOCK 256u /* lanes per shade workgroup (64 measured slower, 128 equal) */
+#endif
 #ifndef BF_ADJOINT_SHADE_WAVES_PER_SIMD
 #define BF_ADJOINT_SHADE_WAVES_PER_SIMD 3 /* the adjoint shade kernel (1 and 2 measured slower) */
 #endif

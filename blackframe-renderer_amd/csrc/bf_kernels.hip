@@ -1416,7 +1416,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
 }
 
 template <bool BATCH, uint32_t MODE>
-__global__ __launch_bounds__(BF_BLOCK, MODE == BF_SHADE_ADJOINT ? BF_ADJOINT_SHADE_WAVES_PER_SIMD : BF_SHADE_WAVES_PER_SIMD) void k_shade(bf_transport_params P, const uint32_t* queue,
+__global__ __launch_bounds__(BF_SHADE_BLOCK, MODE == BF_SHADE_ADJOINT ? BF_ADJOINT_SHADE_WAVES_PER_SIMD : BF_SHADE_WAVES_PER_SIMD) void k_shade(bf_transport_params P, const uint32_t* queue,
                                                                              const uint32_t* count_ptr, const uint32_t* order,
                                                                              uint8_t* route, uint32_t* sample_counts) {
     const uint32_t n = *count_ptr;
@@ -3128,7 +3128,8 @@ uint32_t bfk_shade_resident_blocks(int device, bool adjoint) {
                               (const void*)k_shade<false, BF_SHADE_TEXTURED>, (const void*)k_shade<true, BF_SHADE_TEXTURED>,
                               (const void*)k_shade<false, BF_SHADE_MAPPED>,   (const void*)k_shade<true, BF_SHADE_MAPPED>};
     const void* backward[2] = {(const void*)k_shade<false, BF_SHADE_ADJOINT>, (const void*)k_shade<true, BF_SHADE_ADJOINT>};
-    return adjoint ? (uint32_t)max_resident(backward, 2, BF_BLOCK, device) : (uint32_t)max_resident(forward, 6, BF_BLOCK, device);
+    return adjoint ? (uint32_t)max_resident(backward, 2, BF_SHADE_BLOCK, device)
+                   : (uint32_t)max_resident(forward, 6, BF_SHADE_BLOCK, device);
 }
 
 } /* extern "C" */
@@ -3136,7 +3137,7 @@ uint32_t bfk_shade_resident_blocks(int device, bool adjoint) {
 template <bool BATCH>
 static void launch_shade(const bf_transport_params* P, dim3 grid, const uint32_t* queue, const uint32_t* count_ptr,
                          const uint32_t* order, uint8_t* route, uint32_t* sample_counts, uint32_t mode, hipStream_t s) {
-    const dim3 block(BF_BLOCK);
+    const dim3 block(BF_SHADE_BLOCK);
     if (mode == BF_SHADE_ADJOINT)
         hipLaunchKernelGGL((k_shade<BATCH, BF_SHADE_ADJOINT>), grid, block, 0, s, *P, queue, count_ptr, order, route,
                            sample_counts);
@@ -3178,7 +3179,7 @@ hipError_t bfk_shade(const bf_transport_params* P, const uint32_t* queue, const 
                                                     P->bucket_offsets, P->shade_order, s);
         order = P->shade_order;
     }
-    const uint32_t needed = blocks_for(max_count, BF_BLOCK);
+    const uint32_t needed = blocks_for(max_count, BF_SHADE_BLOCK);
     const uint32_t resident = mode == BF_SHADE_ADJOINT ? P->adjoint_shade_blocks : P->shade_blocks;
     const dim3 grid(needed < resident ? needed : resident);
     if (batch) launch_shade<true>(P, grid, queue, count_ptr, order, route, sample_counts, mode, s);
