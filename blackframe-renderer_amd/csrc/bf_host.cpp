@@ -1763,10 +1763,9 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
         if (!J.scene->device_block) return set_error(BF_INVALID_ARGUMENT, "scene was prepared without a device upload");
         if (J.scene->device != device)
             return set_error(BF_INCOMPATIBLE, "scene lives on device %d, current device is %d", J.scene->device, device);
-        if (J.scene->has_maps && !inputs)
+        if (J.scene->has_maps && adjoint)
             return set_error(BF_UNAVAILABLE,
-                             "job %u: the scene's normal and bump maps need ray cones: render it through "
-                             "blackframe_hip_trace_wavefront_paths", j);
+                             "job %u: the adjoint replay does not differentiate normal and bump maps", j);
         if (J.scene->sampled && (o->wavelength_mode != BF_WAVELENGTHS_FRAME_PACKET ||
                                  std::memcmp(o->packet_wavelengths, J.scene->sampled_wavelengths, sizeof J.scene->sampled_wavelengths) != 0))
             return set_error(BF_INVALID_ARGUMENT,
@@ -1928,6 +1927,8 @@ int trace_jobs(bf_hip_workspace* w, const bf_frame_job* jobs, const bf_adjoint_j
                 D.out_cones = outputs->terminal_cones;
                 D.out_controls = outputs->controls;
             }
+            /* camera paths of a mapped scene start with the camera's cone (path inputs bring theirs) */
+            D.frame_cones = !inputs && J.scene->has_maps ? 1u : 0u;
             if (J.survival_mode != BF_SURVIVAL_NONE) {
                 D.tape_survival = J.tape_survival;
                 D.survival_stride = (uint32_t)vertices;

@@ -40,6 +40,8 @@ typedef struct bf_dev_job {
     float* tape_survival;       /* roulette survival probabilities [path * survival_stride + vertex] */
     uint32_t survival_stride;
     uint32_t survival_mode;     /* bf_survival_mode */
+    uint32_t frame_cones;       /* camera frames of mapped scenes: the camera stage starts each path's
+                                 * cone, the shade stage carries it */
     /* path-input frames (blackframe_hip_trace_wavefront_paths): the inputs replace the camera, the
      * outputs receive the terminal records; all null for camera frames */
     const bf_sample_stream_index* in_samples;

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <mutex>
 
+#include "../../include/blackframe_camera_cone.h"
 #include "../../include/blackframe_texture.h"
 #include "bf_device_adjoint.h"
 #include "bf_device_maps.h"
@@ -101,7 +102,10 @@ BF_D bool robust_unit(v3f d, v3f& out) {
     return fin3(out);
 }
 
-BF_D bool camera_ray(const bf_camera_desc& cam, const sampler_ctx& sc, const sample_key& k, v3f& origin, v3f& dir) {
+/* `cone` (optional) receives the primary ray's cone (width, spread) from its one-pixel
+ * differentials, for frames whose materials carry normal or bump maps */
+BF_D bool camera_ray(const bf_camera_desc& cam, const sampler_ctx& sc, const sample_key& k, v3f& origin, v3f& dir,
+                     float2* cone = nullptr) {
     float ox = 0.5f, oy = 0.5f;
     if (cam.jitter_mode == BF_JITTER_UNIFORM) {
         ox = sample_1d(sc, k, BF_DIM_RASTER_X);
@@ -121,6 +125,12 @@ BF_D bool camera_ray(const bf_camera_desc& cam, const sampler_ctx& sc, const sam
                       cam.tangent[2] * unit.x + cam.bitangent[2] * unit.y + cam.normal[2] * unit.z);
     if (!robust_unit(wd, dir)) return false;
     origin = mk(cam.origin[0], cam.origin[1], cam.origin[2]);
+    if (cone) {
+        const float unit_dir[3] = {dir.x, dir.y, dir.z};
+        float width, spread;
+        if (!bf_camera_cone(&cam, (float)k.px + ox, (float)k.py + oy, unit_dir, &width, &spread)) return false;
+        *cone = make_float2(width, spread);
+    }
     return true;
 }
 
@@ -385,10 +395,12 @@ __global__ __launch_bounds__(BF_BLOCK) void k_camera(bf_transport_params P, uint
             J.tape_visibility[path] = 0ull;
         }
     }
-    if (!camera_ray(J.camera, P.sampler, k, o, d)) {
+    float2 cone;
+    if (!camera_ray(J.camera, P.sampler, k, o, d, J.frame_cones ? &cone : nullptr)) {
         fail_path(P, J, slot, 1);
         return;
     }
+    if (J.frame_cones) P.paths.cone[slot] = cone;
     P.paths.ray_origin[slot] = make_float4(o.x, o.y, o.z, J.camera.time);
     P.paths.ray_direction[slot] = make_float4(d.x, d.y, d.z, J.camera.t_min);
 }
@@ -1144,7 +1156,8 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
         /* the shade stage's surface maps (:4170-4193): the cone advanced to the hit sets the
          * footprint; 46 the cone, 47-49 the maps (invalid, numerical, texel budget) */
         float2 at_hit;
-        if (!J.in_cones || !advance_cone(P.paths.cone[slot], rd, hit.x, at_hit)) return fail_path(P, J, slot, 46);
+        if (!(J.in_cones || J.frame_cones) || !advance_cone(P.paths.cone[slot], rd, hit.x, at_hit))
+            return fail_path(P, J, slot, 46);
         const int st = apply_surface_maps(S, surf.material, at_hit, rd, surf.uv[0], surf.uv[1], surf.ng, surf.dpdu,
                                           surf.dpdv, surf.ns);
         if (st != BF_MAP_OK) return fail_path(P, J, slot, 46u + (uint32_t)st);
@@ -1330,7 +1343,7 @@ BF_D void shade_item(const bf_transport_params& P, const uint32_t* queue, uint32
         beta4 = make_float4(nb[0], nb[1], nb[2], nb[3]);
         P.paths.previous[slot] = delta_event ? make_float4(0.0f, 0.0f, 0.0f, 0.0f)
                                              : make_float4(surf.position.x, surf.position.y, surf.position.z, bs.pdf);
-        if (J.in_cones) {
+        if (J.in_cones || J.frame_cones) {
             /* the sampled closure's compact index, then the cone advanced to the hit and scattered */
             uint32_t compact = mix.active_count, matches = 0u;
             for (uint32_t k = 0; k < mix.active_count; ++k)

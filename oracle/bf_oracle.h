@@ -81,6 +81,8 @@ void bfo_closest_hit_brute_force(const bfo_scene* scene, const bf_transport_ray*
 /* ---- camera --------------------------------------------------------------------------------- */
 int bfo_camera_ray(const bf_camera_desc* camera, uint32_t sampler, const bf_sample_stream_index* index,
                    bf_transport_ray* ray);
+int bfo_camera_cone(const bf_camera_desc* camera, uint32_t sampler, const bf_sample_stream_index* index,
+                    const bf_transport_ray* ray, bf_ray_cone* cone);
 
 /* ---- transport ------------------------------------------------------------------------------
  * Renders paths [path_begin, path_begin + path_count) of the frame with the 7-queue CPU

@@ -10,6 +10,7 @@
 #include <pthread.h>
 #include <stdio.h>
 
+#include "../include/blackframe_camera_cone.h"
 #include "../include/blackframe_texture.h"
 #include "bf_oracle_adjoint.h"
 #include "bf_oracle_internal.h"
@@ -77,6 +78,8 @@ typedef struct {
     const bf_sample_stream_index* in_samples;
     const bf_transport_ray* in_rays;
     const bf_ray_cone* in_cones;
+    /* the lanes carry ray cones: path inputs, or camera frames of a scene with normal or bump maps */
+    int cones;
     const bf_path_state_lane* in_states;
     bf_path_state_lane* out_states;
     bf_transport_ray* out_rays;
@@ -203,6 +206,29 @@ int bfo_camera_ray(const bf_camera_desc* cam, uint32_t sampler, const bf_sample_
     return 1;
 }
 
+/* The cone of a camera ray made by bfo_camera_ray for the same sample (PinholeCamera.hpp:224-252,
+ * RayCone.hpp:194-220): one-pixel differentials at the sample's raster point, zero width. */
+int bfo_camera_cone(const bf_camera_desc* cam, uint32_t sampler, const bf_sample_stream_index* index,
+                    const bf_transport_ray* ray, bf_ray_cone* cone) {
+    float ox = 0.5f, oy = 0.5f;
+    if (cam->jitter_mode == BF_JITTER_UNIFORM) {
+        ox = bfo_sample_1d(sampler, index, BFO_DIM_RASTER_X);
+        oy = bfo_sample_1d(sampler, index, BFO_DIM_RASTER_Y);
+    }
+    const float dir[3] = {ray->direction_x, ray->direction_y, ray->direction_z};
+    return bf_camera_cone(cam, (float)index->pixel_x + ox, (float)index->pixel_y + oy, dir, &cone->width,
+                          &cone->spread);
+}
+
+/* some material carries a normal or a bump map */
+static int scene_has_maps(const bfo_scene* s) {
+    if (!s->surface_maps) return 0;
+    for (uint32_t m = 0; m < s->desc.material_count; ++m)
+        if (s->surface_maps[m].normal_texture != BF_INVALID_INDEX || s->surface_maps[m].bump_texture != BF_INVALID_INDEX)
+            return 1;
+    return 0;
+}
+
 /* WavelengthSampling.hpp:62-84 */
 static void sample_wavelengths(float u, float* wl) {
     for (int lane = 0; lane < 4; ++lane) {
@@ -245,7 +271,8 @@ static void camera_lane(const ctx* c, lane* l, uint64_t global_path) {
     l->sample.pixel_y = (uint32_t)(rem / f->camera.width);
     l->sample.sample_index = f->first_sample_index + s;
     l->sample.seed = f->seed;
-    if (!bfo_camera_ray(&f->camera, c->options->sampler, &l->sample, &l->ray)) {
+    if (!bfo_camera_ray(&f->camera, c->options->sampler, &l->sample, &l->ray) ||
+        (c->cones && !bfo_camera_cone(&f->camera, c->options->sampler, &l->sample, &l->ray, &l->cone))) {
         fail(l, 1);
         return;
     }
@@ -1170,7 +1197,7 @@ static void shade_lane(const ctx* c, lane* l) {
         /* the shade stage's surface maps (CudaWavefrontStages.cu:4170-4193) at the cone advanced
          * to the hit: 46 the cone, 47-49 the maps (invalid, numerical, texel budget) */
         bf_ray_cone at_hit;
-        if (!c->in_cones || !advance_cone(l->cone, &l->ray, l->hit.parameter, &at_hit)) return fail(l, 46);
+        if (!c->cones || !advance_cone(l->cone, &l->ray, l->hit.parameter, &at_hit)) return fail(l, 46);
         const int st = map_apply(s, surf.material, at_hit,
                                  v3_make(l->ray.direction_x, l->ray.direction_y, l->ray.direction_z), surf.uv[0],
                                  surf.uv[1], surf.ng, surf.dpdu, surf.dpdv, &surf.ns);
@@ -1352,7 +1379,7 @@ static void shade_lane(const ctx* c, lane* l) {
         l->prev_ctx[0] = l->prev_ctx[1] = l->prev_ctx[2] = 0.0f;
         l->prev_pdf = 0.0f;
     }
-    if (c->in_cones) {
+    if (c->cones) {
         /* the sampled closure's compact index, then the cone advanced to the hit and scattered */
         uint32_t compact = mix.active_count, matches = 0;
         for (uint32_t k = 0; k < mix.active_count; ++k)
@@ -1578,6 +1605,7 @@ static int render(ctx* c0, const bfo_scene* scene, const bf_frame_desc* frame, c
     c.frame = frame;
     c.options = options;
     c.lanes = lanes;
+    c.cones = c.in_cones != NULL || scene_has_maps(scene);
     bf_transport_report rep;
     memset(&rep, 0, sizeof rep);
     rep.path_count = n;
@@ -1785,6 +1813,7 @@ int bfo_render_scalar(const bfo_scene* scene, const bf_frame_desc* frame, const 
     c.scene = scene;
     c.frame = frame;
     c.options = options;
+    c.cones = scene_has_maps(scene);
     bf_transport_report rep;
     memset(&rep, 0, sizeof rep);
     rep.path_count = path_count;

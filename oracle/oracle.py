@@ -68,6 +68,8 @@ def load(abi):
     lib.bfo_closest_hit_brute_force.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
     lib.bfo_camera_ray.argtypes = [C.POINTER(abi.CameraDesc), C.c_uint32, C.POINTER(abi.SampleStreamIndex),
                                    C.POINTER(abi.TransportRay)]
+    lib.bfo_camera_cone.argtypes = [C.POINTER(abi.CameraDesc), C.c_uint32, C.POINTER(abi.SampleStreamIndex),
+                                    C.POINTER(abi.TransportRay), C.POINTER(abi.RayCone)]
     lib.bfo_render.argtypes = [C.c_void_p, C.POINTER(abi.FrameDesc), C.POINTER(abi.TransportOptions), C.c_uint64,
                                C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.POINTER(abi.TransportReport), C.c_void_p]

@@ -8,7 +8,8 @@ against the oracle, through the path-input transport whose rays carry cones:
   - the same with a y-flipped normal map, clamp / border wraps and an anisotropy of 2;
   - a texel budget too small for the footprints fails those paths (code 49) as the oracle does;
   - the maps delivered in the reference's scene blob render like the oracle's Scene-builder twin;
-  - a mapped scene is refused by the frame entry point (its camera rays carry no cones).
+  - the frame entry point, whose camera starts each path with its one-pixel cone, renders the
+    mapped Cornell box bit-identically to the oracle's frame render.
 """
 import numpy as np
 import pytest
@@ -109,9 +110,26 @@ def test_blob_maps_match_the_oracle(bf, abi, oracle, hip):
     assert (as_u32(states)[:, 4:8] != as_u32(flat[0])[:, 4:8]).any(axis=1).mean() > 0.3
 
 
-def test_frame_entry_refuses_mapped_scenes(bf, abi, hip):
-    scene, cam = mapped_cornell(bf, abi)
-    ws = hip.XpuWavefrontTransportWorkspace(24 * 24)
-    with pytest.raises(hip.XpuError):
-        hip.trace_xpu_wavefront_transport(ws, hip.XpuScene(scene), bf.scenes.frame_desc(cam, 1, seed=0),
-                                          bf.scenes.transport_options(max_depth=4))
+@pytest.mark.parametrize("y_convention,wraps,anisotropy", [(0, (0, 2), 8), (1, (1, 3), 2)])
+def test_frame_entry_renders_mapped_scenes_like_the_oracle(bf, abi, oracle, hip, y_convention, wraps, anisotropy):
+    """Camera frames start each path with the camera's one-pixel cone (PinholeCamera.hpp:224-252),
+    so the frame entry point renders mapped scenes: every path record and the float film
+    bit-identical to the oracle's frame render, and the maps change most paths."""
+    from test_gpu_transport import path_arrays
+
+    scene, cam = mapped_cornell(bf, abi, y_convention, wraps, anisotropy)
+    frame = bf.scenes.frame_desc(cam, 4, seed=7)
+    options = bf.scenes.transport_options(max_depth=6, rr_first_depth=2)
+    options.failure_policy = abi.FAILURES_COUNT
+    want = oracle.scene(scene).render(frame, options, threads=8)
+    ws = hip.XpuWavefrontTransportWorkspace(24 * 24 * 4)
+    got = hip.trace_xpu_wavefront_transport(ws, hip.XpuScene(scene), frame, options, want_paths=True)
+    er, gr = want["report"], got["report"]
+    assert gr.failed_paths == er.failed_paths <= 24 * 24 * 4 // 100
+    assert np.array_equal(path_arrays(got["paths"]), path_arrays(want["paths"]))
+    film = got["film"].host()
+    assert np.array_equal(film["film"].view(np.uint32), np.asarray(want["film"]).view(np.uint32))
+    plain, _ = bf.scenes.cornell(24, 24)
+    flat = oracle.scene(plain).render(frame, options, threads=8)
+    changed = (path_arrays(want["paths"])[:, :4] != path_arrays(flat["paths"])[:, :4]).any(axis=1)
+    assert changed.mean() > 0.5
